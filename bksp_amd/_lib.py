@@ -1,0 +1,119 @@
+"""ctypes loader for libbksp_amd.so (the HIP kernels, C-ABI and C host layer).
+
+The library is built in-tree by ``make -C bksp_amd/csrc``. Importing this module
+raises if the shared object is missing: there is no Python or CPU fallback for
+any solver or kernel.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbksp_amd.so")
+
+c_int = ctypes.c_int
+c_ll = ctypes.c_longlong
+c_dbl = ctypes.c_double
+c_vp = ctypes.c_void_p
+c_ip = ctypes.POINTER(ctypes.c_int)
+c_dp = ctypes.POINTER(ctypes.c_double)
+c_u64 = ctypes.c_uint64
+
+
+class DSpMat(ctypes.Structure):
+    """The reference's ``dspmat`` (include/bksp.h:12-21): CSR with int* size fields."""
+    _fields_ = [("row", c_ip), ("col", c_ip), ("nnz", c_ip), ("row_size", c_ip),
+                ("col_size", c_ip), ("value", c_dp), ("format", ctypes.c_char_p),
+                ("type", ctypes.c_char_p)]
+
+
+class DMat(ctypes.Structure):
+    """The reference's ``dmat`` (include/bksp.h:23-27): column-major dense block."""
+    _fields_ = [("row_size", c_int), ("col_size", c_int), ("value", c_dp)]
+
+
+class BkhOpts(ctypes.Structure):
+    _fields_ = [("precision", c_int), ("graph_iters", c_int), ("fix_blcg_p0", c_int),
+                ("max_inner", c_int), ("inner_tol", c_dbl)]
+
+
+class BkhInfo(ctypes.Structure):
+    _fields_ = [("ret", c_int), ("status", c_int), ("iters", c_int), ("inner_iters", c_int),
+                ("passes", c_int), ("rec_res", c_dbl), ("true_res", c_dbl), ("time_ms", c_dbl)]
+
+
+class CsrHost(ctypes.Structure):
+    _fields_ = [("n", c_int), ("nnz", c_int), ("row_ptr", c_ip), ("col", c_ip), ("val", c_dp)]
+
+
+_SIGS = {
+    # runtime / memory
+    "bkh_init": (c_int, []),
+    "bkh_last_error": (ctypes.c_char_p, []),
+    "bkh_device_name": (c_int, [ctypes.c_char_p, c_int]),
+    "bkh_sync": (c_int, []),
+    "bkh_malloc": (c_vp, [ctypes.c_size_t]),
+    "bkh_free": (None, [c_vp]),
+    "bkh_h2d": (c_int, [c_vp, c_vp, ctypes.c_size_t]),
+    "bkh_d2h": (c_int, [c_vp, c_vp, ctypes.c_size_t]),
+    "bkh_d2d": (c_int, [c_vp, c_vp, ctypes.c_size_t]),
+    "bkh_memset": (c_int, [c_vp, c_int, ctypes.c_size_t]),
+    # matrices and kernels
+    "bkh_matrix_create": (c_vp, [c_int, c_vp, c_vp, c_vp, c_int, c_int, c_int]),
+    "bkh_matrix_destroy": (None, [c_vp]),
+    "bkh_matrix_nnz": (c_ll, [c_vp]),
+    "bkh_matrix_rows": (c_int, [c_vp]),
+    "bkh_matrix_blocks": (c_int, [c_vp]),
+    "bkh_spmv": (c_int, [c_vp, ctypes.c_char, c_dbl, c_vp, c_dbl, c_vp]),
+    "bkh_spmv32": (c_int, [c_vp, c_vp, c_vp]),
+    "bkh_dot": (c_int, [c_ll, c_vp, c_vp, c_dp]),
+    "bkh_last_solve_ms": (c_dbl, []),
+    "bkh_bench_spmv": (c_int, [c_vp, c_int, c_vp, c_vp, c_int, c_dp]),
+    "bkh_bench_copy": (c_int, [c_vp, c_vp, ctypes.c_size_t, c_int, c_dp]),
+    # solvers (device vectors)
+    "bkh_cg": (c_int, [c_vp, c_vp, c_vp, c_dbl, c_int, ctypes.POINTER(BkhOpts),
+                       ctypes.POINTER(BkhInfo)]),
+    # bksp.h drop-in (host arrays)
+    "bksp_dcg": (c_int, [ctypes.POINTER(DSpMat), c_dp, c_dbl, c_int, c_dp]),
+    "dmat_create": (ctypes.POINTER(DMat), [c_int, c_int]),
+    "dmat_free": (c_int, [ctypes.POINTER(DMat)]),
+    "darray2dmat": (c_int, [c_dp, c_int, c_int, ctypes.POINTER(DMat)]),
+    "dvec2mat": (c_int, [c_dp, c_int, ctypes.POINTER(DMat)]),
+    # generators
+    "bkh_csr_host_free": (None, [ctypes.POINTER(CsrHost)]),
+    "bkh_gen_uniform": (None, [c_u64, c_ll, c_dbl, c_dbl, c_dp]),
+    "bkh_gen_normal": (None, [c_u64, c_ll, c_dp]),
+    "bkh_gen_convdiff2d": (c_int, [c_int, c_dbl, c_dbl, ctypes.POINTER(CsrHost)]),
+    "bkh_gen_poisson3d": (c_int, [c_int, c_int, ctypes.POINTER(CsrHost)]),
+    "bkh_gen_convdiff3d": (c_int, [c_int, c_dbl, c_dbl, c_dbl, ctypes.POINTER(CsrHost)]),
+    "bkh_gen_random_rows": (c_int, [c_u64, c_int, ctypes.POINTER(CsrHost)]),
+    "bkh_gen_batch_sizes": (None, [c_u64, c_int, c_dbl, c_dbl, c_ip]),
+    "bkh_gen_grf": (c_int, [c_u64, c_int, c_dbl, c_dbl, c_dp]),
+    "bkh_gen_fv_diffusion": (c_int, [c_int, c_dp, c_int, ctypes.POINTER(CsrHost)]),
+}
+
+_lib = None
+
+
+def load():
+    """Loads libbksp_amd.so and declares every exported prototype."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `make -C bksp_amd/csrc` "
+            "(bksp_amd has no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    """Names this module declares (used by the export test)."""
+    return list(_SIGS)

@@ -1,0 +1,58 @@
+"""Python mirror of the reference's public C API (include/bksp.h).
+
+Same names, argument meaning and return convention as the reference solvers:
+``bksp_dcg(A, b, tol, max_iter, x)`` returns the 0-based loop index at
+convergence or a negative code, and overwrites ``x`` (the initial guess) with
+the solution. ``A`` is a ``DSpMat`` built by ``dspmat``; a "symmetric" matrix
+holds one stored triangle. Every call goes through libbksp_amd.so to the GPU.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import DSpMat, load
+
+NOT_CONVERGED = -1
+DIVERGENCE = -2
+INVALID_PARAMETERS = -3
+INVALID_MATRIX_TYPE = -4
+BKH_DEVICE_ERROR = -5
+BKH_BAD_ARGUMENT = -6
+
+
+class SpMat:
+    """Owns the arrays behind a ``DSpMat`` (include/bksp.h:12-21)."""
+
+    def __init__(self, n, row_ptr, col, val, mtype="unsymmetric"):
+        self.rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+        self.ci = np.ascontiguousarray(col, dtype=np.int32)
+        self.val = np.ascontiguousarray(val, dtype=np.float64)
+        self._n = ctypes.c_int(int(n))
+        self._m = ctypes.c_int(int(n))
+        self._nnz = ctypes.c_int(int(self.rp[-1]))
+        self._fmt = b"csr"
+        self._type = mtype.encode()
+        self.s = DSpMat(self.rp.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                        self.ci.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                        ctypes.pointer(self._nnz), ctypes.pointer(self._n), ctypes.pointer(self._m),
+                        self.val.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                        self._fmt, self._type)
+        self.n = int(n)
+
+    @property
+    def ref(self):
+        return ctypes.byref(self.s)
+
+
+def dspmat(n, row_ptr, col, val, symmetric=False):
+    return SpMat(n, row_ptr, col, val, "symmetric" if symmetric else "unsymmetric")
+
+
+def _dp(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def bksp_dcg(A, b, tol, max_iter, x):
+    """CG (reference src/cg.c:6-71)."""
+    return load().bksp_dcg(A.ref, _dp(b), tol, max_iter, _dp(x))

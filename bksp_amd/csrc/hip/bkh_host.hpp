@@ -1,0 +1,125 @@
+// bkh_host.hpp -- host-side internals of the device layer (C++ inside the .hip
+// translation units; the public surface is the extern "C" API in bksp_hip.h).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <vector>
+#include "bkh_core.hpp"
+#include "../../../include/bksp_hip.h"
+
+// Records a message retrievable with bkh_last_error() (thread-local).
+void bkh_set_error(const char *fmt, ...);
+
+#define HIP_OK(expr)                                                              \
+    do {                                                                          \
+        hipError_t e_ = (expr);                                                   \
+        if (e_ != hipSuccess) {                                                   \
+            bkh_set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),  \
+                          __FILE__, __LINE__);                                    \
+            return BKH_DEVICE_ERROR;                                              \
+        }                                                                         \
+    } while (0)
+
+#define HIP_OK_NULL(expr)                                                         \
+    do {                                                                          \
+        hipError_t e_ = (expr);                                                   \
+        if (e_ != hipSuccess) {                                                   \
+            bkh_set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),  \
+                          __FILE__, __LINE__);                                    \
+            return nullptr;                                                       \
+        }                                                                         \
+    } while (0)
+
+#define LAUNCH_OK()                                                               \
+    do {                                                                          \
+        hipError_t e_ = hipGetLastError();                                        \
+        if (e_ != hipSuccess) {                                                   \
+            bkh_set_error("kernel launch failed: %s (%s:%d)",                     \
+                          hipGetErrorString(e_), __FILE__, __LINE__);             \
+            return BKH_DEVICE_ERROR;                                              \
+        }                                                                         \
+    } while (0)
+
+// CSR-stream row blocks: a block of <= SPMV_ROWS rows whose nonzeros fit in
+// SPMV_CAP LDS slots is streamed into LDS with coalesced loads; a single row with
+// more nonzeros than SPMV_CAP forms a block of its own (workgroup reduction).
+#define SPMV_BS 256
+#define SPMV_ROWS 256
+#define SPMV_CAP 2048
+
+// Device-resident CSR. Rows are stored in the reference's accumulation order
+// (see bkh_matrix_create), so a sequential per-row sum reproduces src/matvec.c.
+struct bkh_matrix {
+    int n = 0;
+    long long nnz = 0;                 // expanded nonzeros
+    int *rp = nullptr, *ci = nullptr;  // row_ptr[n+1], col[nnz]
+    double *val = nullptr;             // FP64 values
+    float *val32 = nullptr;            // FP32 copy (precision extension)
+    double *diag = nullptr;            // diagonal (for Jacobi)
+    int nblk = 0;                      // CSR-stream row blocks
+    int *blk = nullptr;                // blk[nblk+1] row boundaries
+    bkh_matrix *T = nullptr;           // A^T in the same format (BiCG)
+    int stored_triangle = 0;
+};
+
+// Per-thread device context: one stream and a grow-only workspace arena.
+struct ThreadCtx {
+    hipStream_t stream = nullptr;
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+    Ctl *ctl_host = nullptr;           // pinned mirror of the control block
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+ThreadCtx *bkh_tctx();                 // nullptr (with error set) if init failed
+// Returns a device workspace of at least `bytes` (valid until the next call).
+void *bkh_workspace(size_t bytes);
+
+// Carves aligned sub-buffers out of one workspace allocation.
+struct Carve {
+    char *base;
+    size_t off = 0;
+    explicit Carve(void *b) : base((char *)b) {}
+    template <class T> T *take(size_t count) {
+        off = (off + 255) & ~(size_t)255;
+        T *p = (T *)(base + off);
+        off += count * sizeof(T);
+        return p;
+    }
+};
+static inline size_t carve_size(size_t bytes) { return ((bytes + 255) & ~(size_t)255) + 256; }
+
+// Grid for streaming vector kernels.
+static inline int vec_grid(long long n, int bs, int per_thread = 4) {
+    long long g = (n + (long long)bs * per_thread - 1) / ((long long)bs * per_thread);
+    if (g < 1) g = 1;
+    if (g > 4096) g = 4096;
+    return (int)g;
+}
+
+// Runs `body(stream)` K times inside a stream capture and returns an executable
+// graph, or nullptr on failure.
+template <class Body>
+hipGraphExec_t capture_graph(hipStream_t s, int K, Body body) {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+    for (int k = 0; k < K; k++) body();
+    if (hipStreamEndCapture(s, &g) != hipSuccess || !g) return nullptr;
+    if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) ge = nullptr;
+    hipGraphDestroy(g);
+    return ge;
+}
+
+// Replays `ge` until the control block says done. Keeps one graph in flight
+// ahead of the host check. Returns 0 or BKH_DEVICE_ERROR.
+int run_graph_until_done(ThreadCtx *t, hipGraphExec_t ge, Ctl *ctl_dev, int max_launches);
+
+// Initialises a control block on the device (async, on the thread's stream).
+int ctl_init(ThreadCtx *t, Ctl *ctl_dev, double tol, int max_it);
+// Copies the control block to t->ctl_host and synchronises.
+int ctl_fetch(ThreadCtx *t, const Ctl *ctl_dev);
+
+// Maps a finished control block to the bksp return convention.
+int ctl_result(const Ctl *c);

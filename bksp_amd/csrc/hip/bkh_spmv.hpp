@@ -1,0 +1,128 @@
+// bkh_spmv.hpp -- CSR SpMV kernel templates (gfx950).
+//
+// CSR-stream layout: each workgroup owns a contiguous row range whose nonzeros
+// fit in LDS. Phase 1 streams val/col with coalesced non-temporal loads, gathers
+// x (L2/MALL-resident for banded matrices) and stores the products in LDS.
+// Phase 2 gives each thread one row and adds that row's products in stored
+// order, so y[i] = (((0 + a_0 x_0) + a_1 x_1) + ...) exactly as src/matvec.c:22-25
+// computes it. A row longer than the LDS capacity gets a workgroup to itself and
+// a tree reduction. An optional epilogue accumulates NV dot products between
+// y and up to 6 vectors read at the same rows (the Krylov scalars that follow
+// an SpMV), reduced deterministically with a solver-specific finalizer.
+#pragma once
+#include "bkh_core.hpp"
+#include "bkh_host.hpp"
+#include <initializer_list>
+#include <utility>
+
+#define EPI_Y 15   // epilogue operand index meaning "the freshly computed y"
+
+template <int NV, typename WT = double>
+struct Epi {
+    const WT *w[6];
+    unsigned char a[NV > 0 ? NV : 1], b[NV > 0 ? NV : 1];
+};
+
+struct NoFin {
+    __device__ void operator()(Ctl *, const double *) const {}
+};
+
+template <typename VT, typename YT>
+struct SpmvArgs {
+    const int *rp, *ci, *blk;
+    const VT *val;
+    const YT *x;
+    YT *y;
+    double alpha, beta;
+    int alpha_one, has_beta;
+};
+
+template <int NV, typename WT>
+__device__ __forceinline__ double epi_operand(const Epi<NV, WT> &e, int idx, int row, double y) {
+    return idx == EPI_Y ? y : (double)e.w[idx][row];
+}
+
+template <typename VT, typename YT, int NV, class Fin>
+__global__ __launch_bounds__(SPMV_BS) void k_spmv(SpmvArgs<VT, YT> a, Epi<NV, YT> e, Ctl *ctl,
+                                                  double *part, Fin fin) {
+    if (ctl && ctl_stopped(ctl)) return;
+    __shared__ YT prod[SPMV_CAP];
+    const int tid = threadIdx.x;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = a.blk[b], r1 = a.blk[b + 1];
+    const int k0 = a.rp[r0], k1 = a.rp[r1];
+    double dv[NV > 0 ? NV : 1];
+#pragma unroll
+    for (int i = 0; i < NV; i++) dv[i] = 0.0;
+
+    if (k1 - k0 <= SPMV_CAP) {
+        const YT al = (YT)a.alpha;
+        for (int k = k0 + tid; k < k1; k += SPMV_BS) {
+            YT v = (YT)__builtin_nontemporal_load(&a.val[k]);
+            int c = __builtin_nontemporal_load(&a.ci[k]);
+            YT xv = a.x[c];
+            prod[k - k0] = a.alpha_one ? mul_rn(v, xv) : mul_rn(mul_rn(al, v), xv);
+        }
+        __syncthreads();
+        const int row = r0 + tid;
+        if (row < r1) {
+            const int s0 = a.rp[row] - k0, s1 = a.rp[row + 1] - k0;
+            YT s = a.has_beta ? mul_rn((YT)a.beta, a.y[row]) : (YT)0;
+            for (int k = s0; k < s1; k++) s = add_rn(s, prod[k]);
+            a.y[row] = s;
+#pragma unroll
+            for (int i = 0; i < NV; i++)
+                dv[i] = epi_operand(e, e.a[i], row, (double)s) * epi_operand(e, e.b[i], row, (double)s);
+        }
+    } else {
+        // one long row: strided partial sums + deterministic tree reduction
+        double acc[1] = {0.0};
+        for (int k = k0 + tid; k < k1; k += SPMV_BS) {
+            double v = (double)a.val[k];
+            if (!a.alpha_one) v = a.alpha * v;
+            acc[0] += v * (double)a.x[a.ci[k]];
+        }
+        __shared__ double shl[SPMV_BS / BKH_WAVE];
+        block_sum<1, SPMV_BS>(acc, shl);
+        __shared__ double ybc;
+        if (tid == 0) {
+            double s = acc[0];
+            if (a.has_beta) s += a.beta * (double)a.y[r0];
+            a.y[r0] = (YT)s;
+            ybc = (double)(YT)s;
+        }
+        __syncthreads();
+        if (tid == 0) {
+#pragma unroll
+            for (int i = 0; i < NV; i++)
+                dv[i] = epi_operand(e, e.a[i], r0, ybc) * epi_operand(e, e.b[i], r0, ybc);
+        }
+    }
+    if constexpr (NV > 0) {
+        grid_sum<NV, SPMV_BS>(dv, part, &ctl->ticket, [&](double *tot) { fin(ctl, tot); });
+    }
+}
+
+// Host launcher. `ctl`/`part` may be null when NV == 0.
+template <typename VT, typename YT, int NV, class Fin>
+static inline void launch_spmv(hipStream_t s, const bkh_matrix *A, const VT *val, const YT *x,
+                               YT *y, double alpha, double beta, const Epi<NV, YT> &e, Ctl *ctl,
+                               double *part, Fin fin) {
+    SpmvArgs<VT, YT> a;
+    a.rp = A->rp; a.ci = A->ci; a.blk = A->blk; a.val = val; a.x = x; a.y = y;
+    a.alpha = alpha; a.beta = beta; a.alpha_one = (alpha == 1.0); a.has_beta = (beta != 0.0);
+    hipLaunchKernelGGL((k_spmv<VT, YT, NV, Fin>), dim3(A->nblk), dim3(SPMV_BS), 0, s, a, e, ctl,
+                       part, fin);
+}
+
+template <int NV, typename WT = double>
+static inline Epi<NV, WT> make_epi(std::initializer_list<const WT *> w,
+                                   std::initializer_list<std::pair<int, int>> pairs) {
+    Epi<NV, WT> e;
+    memset(&e, 0, sizeof(e));
+    int i = 0;
+    for (auto p : w) e.w[i++] = p;
+    i = 0;
+    for (auto pr : pairs) { e.a[i] = (unsigned char)pr.first; e.b[i] = (unsigned char)pr.second; i++; }
+    return e;
+}

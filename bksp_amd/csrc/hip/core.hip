@@ -1,0 +1,570 @@
+// core.hip -- runtime, memory, matrix upload and single-kernel entry points.
+#include <mutex>
+#include <new>
+#include <stdlib.h>
+#include "bkh_host.hpp"
+#include "bkh_spmv.hpp"
+#include "bkh_solver_common.hpp"
+
+static thread_local char g_err[512];
+static std::once_flag g_once;
+static int g_init_rc = 0;
+static thread_local ThreadCtx *g_tctx = nullptr;
+static thread_local double g_last_ms = 0.0;
+
+void bkh_set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+extern "C" const char *bkh_last_error(void) { return g_err; }
+
+static void global_init() {
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev < 1) {
+        bkh_set_error("no HIP device available (%s)", hipGetErrorString(e));
+        g_init_rc = BKH_DEVICE_ERROR;
+        return;
+    }
+    g_init_rc = 0;
+}
+
+extern "C" int bkh_init(void) {
+    std::call_once(g_once, global_init);
+    if (g_init_rc) bkh_set_error("no HIP device available");
+    return g_init_rc;
+}
+
+ThreadCtx *bkh_tctx() {
+    if (g_tctx) return g_tctx;
+    if (bkh_init() != 0) return nullptr;
+    ThreadCtx *t = new (std::nothrow) ThreadCtx();
+    if (!t) return nullptr;
+    if (hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipHostMalloc((void **)&t->ctl_host, sizeof(Ctl), hipHostMallocDefault) != hipSuccess ||
+        hipEventCreate(&t->ev0) != hipSuccess || hipEventCreate(&t->ev1) != hipSuccess) {
+        bkh_set_error("failed to create the thread's HIP stream/events");
+        delete t;
+        return nullptr;
+    }
+    g_tctx = t;
+    return t;
+}
+
+void *bkh_workspace(size_t bytes) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return nullptr;
+    if (bytes <= t->ws_bytes) return t->ws;
+    if (t->ws) {
+        hipStreamSynchronize(t->stream);
+        hipFree(t->ws);
+        t->ws = nullptr;
+        t->ws_bytes = 0;
+    }
+    size_t want = bytes + bytes / 8;
+    if (hipMalloc(&t->ws, want) != hipSuccess) {
+        bkh_set_error("workspace allocation of %zu bytes failed", want);
+        t->ws = nullptr;
+        return nullptr;
+    }
+    t->ws_bytes = want;
+    return t->ws;
+}
+
+extern "C" int bkh_device_name(char *buf, int len) {
+    if (bkh_init() != 0) return BKH_DEVICE_ERROR;
+    hipDeviceProp_t p;
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    HIP_OK(hipGetDeviceProperties(&p, dev));
+    snprintf(buf, (size_t)len, "%s %s CUs=%d", p.name, p.gcnArchName, p.multiProcessorCount);
+    return 0;
+}
+
+extern "C" int bkh_sync(void) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    HIP_OK(hipStreamSynchronize(t->stream));
+    return 0;
+}
+
+// ------------------------------------------------------------------ memory
+
+extern "C" void *bkh_malloc(size_t bytes) {
+    if (!bkh_tctx()) return nullptr;
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) {
+        bkh_set_error("hipMalloc(%zu) failed", bytes);
+        return nullptr;
+    }
+    return p;
+}
+
+extern "C" void bkh_free(void *p) {
+    if (!p) return;
+    ThreadCtx *t = bkh_tctx();
+    if (t) hipStreamSynchronize(t->stream);
+    hipFree(p);
+}
+
+extern "C" int bkh_h2d(void *d, const void *h, size_t bytes) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    HIP_OK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, t->stream));
+    HIP_OK(hipStreamSynchronize(t->stream));
+    return 0;
+}
+
+extern "C" int bkh_d2h(void *h, const void *d, size_t bytes) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    HIP_OK(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, t->stream));
+    HIP_OK(hipStreamSynchronize(t->stream));
+    return 0;
+}
+
+extern "C" int bkh_d2d(void *d, const void *s, size_t bytes) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    HIP_OK(hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToDevice, t->stream));
+    HIP_OK(hipStreamSynchronize(t->stream));
+    return 0;
+}
+
+__global__ void k_fill_bytes(unsigned char *p, unsigned char v, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t st = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += st) p[i] = v;
+}
+
+extern "C" int bkh_memset(void *d, int v, size_t bytes) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    // a fill kernel rather than hipMemsetAsync, so callers may capture it in graphs
+    hipLaunchKernelGGL(k_fill_bytes, dim3(vec_grid((long long)bytes, 256, 16)), dim3(256), 0,
+                       t->stream, (unsigned char *)d, (unsigned char)v, bytes);
+    LAUNCH_OK();
+    HIP_OK(hipStreamSynchronize(t->stream));
+    return 0;
+}
+
+// ------------------------------------------------------------ control block
+
+__global__ void k_ctl_init(Ctl *c, double tol, int max_it) {
+    if (threadIdx.x == 0) {
+        c->done = max_it <= 0 ? 1 : 0;
+        c->status = max_it <= 0 ? ST_MAXIT : ST_RUNNING;
+        c->it = 0; c->result_it = 0; c->max_it = max_it; c->inner_total = 0;
+        c->ticket = 0u; c->tol = tol; c->fnb = 0; c->err = 0;
+    }
+    for (int i = threadIdx.x; i < BKH_MAX_SCAL; i += blockDim.x) c->s[i] = 0.0;
+}
+
+int ctl_init(ThreadCtx *t, Ctl *ctl_dev, double tol, int max_it) {
+    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, t->stream, ctl_dev, tol, max_it);
+    LAUNCH_OK();
+    return 0;
+}
+
+int ctl_fetch(ThreadCtx *t, const Ctl *ctl_dev) {
+    HIP_OK(hipMemcpyAsync(t->ctl_host, ctl_dev, sizeof(Ctl), hipMemcpyDeviceToHost, t->stream));
+    HIP_OK(hipStreamSynchronize(t->stream));
+    return 0;
+}
+
+int ctl_result(const Ctl *c) {
+    if (c->status == ST_CONVERGED) return c->result_it;
+    if (c->status == ST_BREAKDOWN) return -2;   // DIVERGENCE
+    return -1;                                    // NOT_CONVERGED
+}
+
+int run_graph_until_done(ThreadCtx *t, hipGraphExec_t ge, Ctl *ctl_dev, int max_launches) {
+    // One graph stays queued ahead of the check; kernels of a stopped solve return
+    // at entry, so an extra replay costs only launch latency.
+    HIP_OK(hipGraphLaunch(ge, t->stream));
+    for (int l = 1; l <= max_launches; l++) {
+        HIP_OK(hipMemcpyAsync(t->ctl_host, ctl_dev, sizeof(Ctl), hipMemcpyDeviceToHost, t->stream));
+        hipEvent_t ev = t->ev1;
+        HIP_OK(hipEventRecord(ev, t->stream));
+        if (l < max_launches) HIP_OK(hipGraphLaunch(ge, t->stream));
+        HIP_OK(hipEventSynchronize(ev));
+        if (t->ctl_host->done) break;
+    }
+    HIP_OK(hipStreamSynchronize(t->stream));
+    return ctl_fetch(t, ctl_dev);
+}
+
+// --------------------------------------------------------------- matrices
+
+static int check_csr(int n, const int *rp, const int *ci) {
+    if (n < 1 || !rp || !ci) return BKH_BAD_ARGUMENT;
+    if (rp[0] != 0) return BKH_BAD_ARGUMENT;
+    for (int i = 0; i < n; i++)
+        if (rp[i + 1] < rp[i]) return BKH_BAD_ARGUMENT;
+    for (long long k = 0; k < rp[n]; k++)
+        if (ci[k] < 0 || ci[k] >= n) return BKH_BAD_ARGUMENT;
+    return 0;
+}
+
+// Builds the rows the device streams, in the order the reference accumulates:
+//  mode 'n': rows as stored (src/matvec.c:15-28 gathers in stored order);
+//  mode 'l': one stored triangle expanded; processing (j, k) in the order of
+//            src/matvec.c:108-118, the gather term goes to row j and the mirrored
+//            scatter term (col != j) to row col, each appended in processing order;
+//  mode 't': A^T, entry (j, k) appended to row col in processing order
+//            (src/matvec.c:38-45).
+static void build_ordered(int n, const int *rp, const int *ci, const double *v, char mode,
+                          std::vector<int> &orp, std::vector<int> &oci, std::vector<double> &ov) {
+    long long nnz = rp[n];
+    orp.assign((size_t)n + 1, 0);
+    if (mode == 'n') {
+        for (int i = 0; i <= n; i++) orp[i] = rp[i];
+        oci.assign(ci, ci + nnz);
+        ov.assign(v, v + nnz);
+        return;
+    }
+    std::vector<long long> cnt((size_t)n + 1, 0);
+    for (int j = 0; j < n; j++)
+        for (int k = rp[j]; k < rp[j + 1]; k++) {
+            if (mode == 'l') {
+                cnt[j]++;
+                if (ci[k] != j) cnt[ci[k]]++;
+            } else {
+                cnt[ci[k]]++;
+            }
+        }
+    long long tot = 0;
+    for (int i = 0; i < n; i++) { orp[i] = (int)tot; tot += cnt[i]; }
+    orp[n] = (int)tot;
+    oci.resize((size_t)tot);
+    ov.resize((size_t)tot);
+    std::vector<long long> cur(orp.begin(), orp.end() - 1);
+    for (int j = 0; j < n; j++)
+        for (int k = rp[j]; k < rp[j + 1]; k++) {
+            if (mode == 'l') {
+                if (ci[k] != j) { long long p = cur[ci[k]]++; oci[p] = j; ov[p] = v[k]; }
+                long long q = cur[j]++; oci[q] = ci[k]; ov[q] = v[k];
+            } else {
+                long long p = cur[ci[k]]++; oci[p] = j; ov[p] = v[k];
+            }
+        }
+}
+
+static void build_blocks(int n, const std::vector<int> &rp, std::vector<int> &blk) {
+    blk.clear();
+    blk.push_back(0);
+    int r = 0;
+    while (r < n) {
+        int len = rp[r + 1] - rp[r];
+        if (len > SPMV_CAP) { r++; blk.push_back(r); continue; }
+        int start = r, nz = 0;
+        while (r < n && r - start < SPMV_ROWS) {
+            int l = rp[r + 1] - rp[r];
+            if (l > SPMV_CAP || nz + l > SPMV_CAP) break;
+            nz += l;
+            r++;
+        }
+        blk.push_back(r);
+    }
+}
+
+static void free_matrix(bkh_matrix *A) {
+    if (!A) return;
+    hipFree(A->rp); hipFree(A->ci); hipFree(A->val); hipFree(A->val32);
+    hipFree(A->diag); hipFree(A->blk);
+    if (A->T) free_matrix(A->T);
+    delete A;
+}
+
+static bkh_matrix *upload(int n, const std::vector<int> &rp, const std::vector<int> &ci,
+                          const std::vector<double> &v, int want_fp32) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return nullptr;
+    bkh_matrix *A = new (std::nothrow) bkh_matrix();
+    if (!A) return nullptr;
+    A->n = n;
+    A->nnz = rp[n];
+    std::vector<int> blk;
+    build_blocks(n, rp, blk);
+    A->nblk = (int)blk.size() - 1;
+    std::vector<double> d((size_t)n, 0.0);
+    for (int i = 0; i < n; i++)
+        for (int k = rp[i]; k < rp[i + 1]; k++)
+            if (ci[k] == i) d[i] += v[k];
+    size_t nz = (size_t)(A->nnz > 0 ? A->nnz : 1);
+    bool ok = hipMalloc(&A->rp, sizeof(int) * (n + 1)) == hipSuccess &&
+              hipMalloc(&A->ci, sizeof(int) * nz) == hipSuccess &&
+              hipMalloc(&A->val, sizeof(double) * nz) == hipSuccess &&
+              hipMalloc(&A->diag, sizeof(double) * n) == hipSuccess &&
+              hipMalloc(&A->blk, sizeof(int) * blk.size()) == hipSuccess;
+    if (ok && want_fp32) ok = hipMalloc(&A->val32, sizeof(float) * nz) == hipSuccess;
+    if (!ok) {
+        bkh_set_error("device allocation for a %d-row matrix failed", n);
+        free_matrix(A);
+        return nullptr;
+    }
+    hipStream_t s = t->stream;
+    ok = hipMemcpyAsync(A->rp, rp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, s) == hipSuccess &&
+         hipMemcpyAsync(A->ci, ci.data(), sizeof(int) * A->nnz, hipMemcpyHostToDevice, s) == hipSuccess &&
+         hipMemcpyAsync(A->val, v.data(), sizeof(double) * A->nnz, hipMemcpyHostToDevice, s) == hipSuccess &&
+         hipMemcpyAsync(A->diag, d.data(), sizeof(double) * n, hipMemcpyHostToDevice, s) == hipSuccess &&
+         hipMemcpyAsync(A->blk, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice, s) == hipSuccess;
+    std::vector<float> v32;
+    if (ok && want_fp32) {
+        v32.resize((size_t)A->nnz);
+        for (long long k = 0; k < A->nnz; k++) v32[k] = (float)v[k];
+        ok = hipMemcpyAsync(A->val32, v32.data(), sizeof(float) * A->nnz, hipMemcpyHostToDevice, s) == hipSuccess;
+    }
+    if (!ok || hipStreamSynchronize(s) != hipSuccess) {
+        bkh_set_error("matrix upload failed");
+        free_matrix(A);
+        return nullptr;
+    }
+    return A;
+}
+
+extern "C" bkh_matrix *bkh_matrix_create(int n, const int *row_ptr, const int *col,
+                                         const double *val, int stored_triangle,
+                                         int want_transpose, int want_fp32) {
+    if (check_csr(n, row_ptr, col) != 0 || (!val && row_ptr[n] > 0)) {
+        bkh_set_error("invalid CSR input (n=%d)", n);
+        return nullptr;
+    }
+    std::vector<int> orp, oci;
+    std::vector<double> ov;
+    build_ordered(n, row_ptr, col, val, stored_triangle ? 'l' : 'n', orp, oci, ov);
+    bkh_matrix *A = upload(n, orp, oci, ov, want_fp32);
+    if (!A) return nullptr;
+    A->stored_triangle = stored_triangle;
+    if (want_transpose) {
+        if (stored_triangle) {
+            A->T = upload(n, orp, oci, ov, want_fp32);   // symmetric: A^T = A
+        } else {
+            std::vector<int> trp, tci;
+            std::vector<double> tv;
+            build_ordered(n, row_ptr, col, val, 't', trp, tci, tv);
+            A->T = upload(n, trp, tci, tv, want_fp32);
+        }
+        if (!A->T) { free_matrix(A); return nullptr; }
+    }
+    return A;
+}
+
+extern "C" void bkh_matrix_destroy(bkh_matrix *A) {
+    ThreadCtx *t = bkh_tctx();
+    if (t) hipStreamSynchronize(t->stream);
+    free_matrix(A);
+}
+
+extern "C" long long bkh_matrix_nnz(const bkh_matrix *A) { return A ? A->nnz : -1; }
+extern "C" int bkh_matrix_rows(const bkh_matrix *A) { return A ? A->n : -1; }
+extern "C" int bkh_matrix_blocks(const bkh_matrix *A) { return A ? A->nblk : -1; }
+
+// ------------------------------------------------------------- single kernels
+
+extern "C" int bkh_spmv(const bkh_matrix *A, char op, double alpha, const double *x, double beta,
+                        double *y) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (!A || !x || !y) return BKH_BAD_ARGUMENT;
+    const bkh_matrix *M = A;
+    if (op == 't') {
+        if (!A->T) { bkh_set_error("matrix was created without its transpose"); return BKH_BAD_ARGUMENT; }
+        M = A->T;
+    } else if (op != 'n') {
+        return BKH_BAD_ARGUMENT;
+    }
+    Epi<0> e;
+    memset(&e, 0, sizeof e);
+    launch_spmv<double, double, 0>(t->stream, M, M->val, x, y, alpha, beta, e, nullptr, nullptr, NoFin());
+    LAUNCH_OK();
+    HIP_OK(hipStreamSynchronize(t->stream));
+    return 0;
+}
+
+extern "C" int bkh_spmv32(const bkh_matrix *A, const float *x, float *y) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (!A || !A->val32 || !x || !y) {
+        bkh_set_error("bkh_spmv32 needs a matrix created with want_fp32");
+        return BKH_BAD_ARGUMENT;
+    }
+    Epi<0, float> e;
+    memset(&e, 0, sizeof e);
+    launch_spmv<float, float, 0>(t->stream, A, A->val32, x, y, 1.0, 0.0, e, nullptr, nullptr, NoFin());
+    LAUNCH_OK();
+    HIP_OK(hipStreamSynchronize(t->stream));
+    return 0;
+}
+
+// Deterministic dot: one partial per workgroup, fixed-order final sum.
+struct StoreFin {
+    __device__ void operator()(Ctl *c, const double *tot) const { c->s[0] = tot[0]; }
+};
+
+__global__ __launch_bounds__(256) void k_dot(long long n, const double *x, const double *y, Ctl *c,
+                                             double *part) {
+    double v[1] = {0.0};
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        v[0] += x[i] * y[i];
+    grid_sum<1, 256>(v, part, &c->ticket, [&](double *tot) { StoreFin()(c, tot); });
+}
+
+extern "C" int bkh_dot(long long n, const double *x, const double *y, double *out) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    int G = vec_grid(n, 256);
+    void *ws = bkh_workspace(carve_size(sizeof(Ctl)) + carve_size(sizeof(double) * G));
+    if (!ws) return BKH_DEVICE_ERROR;
+    Carve cv(ws);
+    Ctl *c = cv.take<Ctl>(1);
+    double *part = cv.take<double>(G);
+    if (ctl_init(t, c, 0.0, 1)) return BKH_DEVICE_ERROR;
+    hipLaunchKernelGGL(k_dot, dim3(G), dim3(256), 0, t->stream, n, x, y, c, part);
+    LAUNCH_OK();
+    if (ctl_fetch(t, c)) return BKH_DEVICE_ERROR;
+    *out = t->ctl_host->s[0];
+    return 0;
+}
+
+extern "C" double bkh_last_solve_ms(void) { return g_last_ms; }
+void bkh_note_solve_ms(double ms) { g_last_ms = ms; }
+
+// ------------------------------------------------- residuals and refinement
+
+struct ResidFin {
+    __device__ void operator()(Ctl *c, const double *tot) const {
+        double rn = sqrt(tot[0]), bn = sqrt(tot[1]);
+        c->s[0] = rn / bn;
+        c->s[1] = rn;
+        c->s[2] = bn;
+    }
+};
+
+// r = b - y - sigma x, where r holds y = A x on entry.
+__global__ __launch_bounds__(256) void k_resid(long long n, const double *b, const double *x,
+                                               double sigma, double *r, Ctl *c, double *part) {
+    double v[2] = {0.0, 0.0};
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        double ri = b[i] - r[i];
+        if (sigma != 0.0) ri -= sigma * x[i];
+        r[i] = ri;
+        v[0] += ri * ri;
+        v[1] += b[i] * b[i];
+    }
+    grid_sum<2, 256>(v, part, &c->ticket, [&](double *tot) { ResidFin()(c, tot); });
+}
+
+int resid_fp64(ThreadCtx *t, const bkh_matrix *A, double sigma, const double *b, const double *x,
+               double *r, Ctl *c, double *part) {
+    Epi<0> e;
+    memset(&e, 0, sizeof e);
+    launch_spmv<double, double, 0>(t->stream, A, A->val, x, r, 1.0, 0.0, e, nullptr, nullptr, NoFin());
+    hipLaunchKernelGGL(k_resid, dim3(vec_grid(A->n, 256)), dim3(256), 0, t->stream, (long long)A->n, b,
+                       x, sigma, r, c, part);
+    LAUNCH_OK();
+    return 0;
+}
+
+int true_residual(ThreadCtx *t, const bkh_matrix *A, double sigma, const double *b, const double *x,
+                  double *out) {
+    const int G = vec_grid(A->n, 256);
+    void *mem = nullptr;
+    size_t bytes = carve_size(sizeof(Ctl)) + carve_size(sizeof(double) * G * 2) +
+                   carve_size(sizeof(double) * A->n);
+    HIP_OK(hipMalloc(&mem, bytes));
+    Carve cv(mem);
+    Ctl *c = cv.take<Ctl>(1);
+    double *part = cv.take<double>((size_t)G * 2);
+    double *r = cv.take<double>(A->n);
+    int rc = ctl_init(t, c, 0.0, 1);
+    if (!rc) rc = resid_fp64(t, A, sigma, b, x, r, c, part);
+    if (!rc) rc = ctl_fetch(t, c);
+    if (!rc) *out = t->ctl_host->s[0];
+    hipFree(mem);
+    return rc;
+}
+
+__global__ __launch_bounds__(256) void k_ir_prep(long long n, const double *r64, const Ctl *c,
+                                                 float *r32, float *d32) {
+    const double inv = 1.0 / c->s[1];
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        r32[i] = (float)(r64[i] * inv);
+        d32[i] = 0.0f;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_ir_update(long long n, double *x, const float *d32,
+                                                   const Ctl *c) {
+    const double sc = c->s[1];
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        x[i] += sc * (double)d32[i];
+}
+
+int ir_prep(ThreadCtx *t, long long n, const double *r64, const Ctl *c, float *r32, float *d32) {
+    hipLaunchKernelGGL(k_ir_prep, dim3(vec_grid(n, 256)), dim3(256), 0, t->stream, n, r64, c, r32, d32);
+    LAUNCH_OK();
+    return 0;
+}
+
+int ir_update(ThreadCtx *t, long long n, double *x, const float *d32, const Ctl *c) {
+    hipLaunchKernelGGL(k_ir_update, dim3(vec_grid(n, 256)), dim3(256), 0, t->stream, n, x, d32, c);
+    LAUNCH_OK();
+    return 0;
+}
+
+// Times `reps` back-to-back SpMV launches on this thread's stream with HIP events
+// (the stream the kernels run on), returning the average milliseconds per launch.
+extern "C" int bkh_bench_spmv(const bkh_matrix *A, int fp32, const void *x, void *y, int reps,
+                              double *avg_ms) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (!A || !x || !y || reps < 1 || (fp32 && !A->val32)) return BKH_BAD_ARGUMENT;
+    HIP_OK(hipEventRecord(t->ev0, t->stream));
+    for (int r = 0; r < reps; r++) {
+        if (fp32) {
+            Epi<0, float> e;
+            memset(&e, 0, sizeof e);
+            launch_spmv<float, float, 0>(t->stream, A, A->val32, (const float *)x, (float *)y, 1.0, 0.0,
+                                         e, nullptr, nullptr, NoFin());
+        } else {
+            Epi<0> e;
+            memset(&e, 0, sizeof e);
+            launch_spmv<double, double, 0>(t->stream, A, A->val, (const double *)x, (double *)y, 1.0,
+                                           0.0, e, nullptr, nullptr, NoFin());
+        }
+    }
+    LAUNCH_OK();
+    HIP_OK(hipEventRecord(t->ev1, t->stream));
+    HIP_OK(hipEventSynchronize(t->ev1));
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, t->ev0, t->ev1));
+    *avg_ms = ms / reps;
+    return 0;
+}
+
+__global__ void k_copy(const double4 *__restrict__ a, double4 *__restrict__ b, long long n4) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+         i += (long long)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
+// Device-copy bandwidth probe: b = a over `bytes` (multiple of 32), averaged.
+extern "C" int bkh_bench_copy(const void *a, void *b, size_t bytes, int reps, double *avg_ms) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    long long n4 = (long long)(bytes / 32);
+    HIP_OK(hipEventRecord(t->ev0, t->stream));
+    for (int r = 0; r < reps; r++)
+        hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, t->stream, (const double4 *)a, (double4 *)b, n4);
+    LAUNCH_OK();
+    HIP_OK(hipEventRecord(t->ev1, t->stream));
+    HIP_OK(hipEventSynchronize(t->ev1));
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, t->ev0, t->ev1));
+    *avg_ms = ms / reps;
+    return 0;
+}

@@ -1,0 +1,111 @@
+/*
+ * bksp_api.c -- the bksp.h entry points (C host code).
+ *
+ * Each solver keeps the reference's contract (argument meaning, type checks,
+ * in/out x, return convention) and runs the iteration on the GPU through the
+ * C-ABI of bksp_hip.h: upload A, b and x0, solve on the device, download x.
+ * There is no CPU solve path: if the device layer fails, the call returns
+ * BKH_DEVICE_ERROR (-5) and bkh_last_error() says why.
+ *
+ * Setting BKSP_PRECISION=fp32ir in the environment makes these entry points use
+ * the FP32-inner / FP64-refinement path (default: plain FP64).
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "../../../include/bksp.h"
+#include "../../../include/bksp_hip.h"
+
+static bkh_opts default_opts(void) {
+    bkh_opts o;
+    memset(&o, 0, sizeof o);
+    const char *p = getenv("BKSP_PRECISION");
+    if (p && strcmp(p, "fp32ir") == 0) o.precision = BKH_FP32_IR;
+    return o;
+}
+
+static int is_symmetric(const dspmat *A) { return A->type && strcmp(A->type, "symmetric") == 0; }
+
+/* Device copies of A (expanded if one triangle is stored), b and x. */
+typedef struct {
+    bkh_matrix *M;
+    double *d_b, *d_x;
+    size_t n_b, n_x;
+} upload_t;
+
+static int upload_system(dspmat *A, const double *b, size_t nb, const double *x, size_t nx,
+                         int want_T, int want_fp32, upload_t *u) {
+    memset(u, 0, sizeof *u);
+    int n = *A->row_size;
+    u->M = bkh_matrix_create(n, A->row, A->col, A->value, is_symmetric(A), want_T, want_fp32);
+    if (!u->M) return BKH_DEVICE_ERROR;
+    u->d_b = (double *)bkh_malloc(sizeof(double) * nb);
+    u->d_x = (double *)bkh_malloc(sizeof(double) * nx);
+    if (!u->d_b || !u->d_x) return BKH_DEVICE_ERROR;
+    u->n_b = nb;
+    u->n_x = nx;
+    if (bkh_h2d(u->d_b, b, sizeof(double) * nb) || bkh_h2d(u->d_x, x, sizeof(double) * nx))
+        return BKH_DEVICE_ERROR;
+    return 0;
+}
+
+static int download_x(upload_t *u, double *x) {
+    return bkh_d2h(x, u->d_x, sizeof(double) * u->n_x) ? BKH_DEVICE_ERROR : 0;
+}
+
+static void release(upload_t *u) {
+    bkh_free(u->d_b);
+    bkh_free(u->d_x);
+    if (u->M) bkh_matrix_destroy(u->M);
+}
+
+/* reference src/cg.c:6-71 */
+int bksp_dcg(dspmat *A, double *b, double tol, int max_iter, double *x) {
+    if (strcmp(A->type, "symmetric") != 0) return INVALID_MATRIX_TYPE;
+    if (max_iter <= 0) return 0;                     /* cg.c:15, 67-70 */
+    bkh_opts o = default_opts();
+    int n = *A->row_size;
+    upload_t u;
+    int rc = upload_system(A, b, n, x, n, 0, o.precision == BKH_FP32_IR, &u);
+    if (rc == 0) {
+        rc = bkh_cg(u.M, u.d_b, u.d_x, tol, max_iter, &o, NULL);
+        if (rc > BKH_DEVICE_ERROR && download_x(&u, x)) rc = BKH_DEVICE_ERROR;
+    }
+    release(&u);
+    return rc;
+}
+
+/* ---------------------------------------------------- dense helpers (util.c) */
+
+/* reference src/util.c:5-18 */
+dmat *dmat_create(int m, int n) {
+    dmat *mat = (dmat *)malloc(sizeof(dmat));
+    mat->value = (double *)calloc((size_t)m * n, sizeof(double));
+    mat->row_size = m;
+    mat->col_size = n;
+    return mat;
+}
+
+/* reference src/util.c:20-25 */
+int dmat_free(dmat *mat) {
+    free(mat->value);
+    free(mat);
+    return 0;
+}
+
+/* reference src/util.c:27-33 */
+int darray2dmat(double *arr, int row_size, int col_size, dmat *mat) {
+    mat->row_size = row_size;
+    mat->col_size = col_size;
+    mat->value = arr;
+    return 0;
+}
+
+/* reference src/util.c:57-67: writes x into column m with stride col_size, as the
+ * reference does (row-major indexing of the column-major dmat is kept for ABI
+ * compatibility). */
+int dvec2mat(double *x, int m, dmat *mat) {
+    int l = mat->row_size, n = mat->col_size;
+    for (int i = 0; i < l; i++) mat->value[i * n + m] = x[i];
+    return 0;
+}

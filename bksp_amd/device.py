@@ -1,0 +1,154 @@
+"""Device-level Python handles over the bksp_hip.h C-ABI.
+
+``Matrix`` keeps a CSR resident in HBM, ``DeviceArray`` a device vector; the
+solver wrappers return a ``BkhInfo`` with the bksp.h return value, iteration
+index, recursive and FP64 true residuals and the device time of the solve.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import BkhInfo, BkhOpts, load
+
+BKH_FP64 = 0
+BKH_FP32_IR = 1
+
+
+class BkspDeviceError(RuntimeError):
+    pass
+
+
+def last_error():
+    msg = load().bkh_last_error()
+    return msg.decode() if msg else ""
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class DeviceArray:
+    """A device buffer holding a NumPy-typed array (float64 or float32)."""
+
+    def __init__(self, shape, dtype=np.float64):
+        self.shape = tuple(np.atleast_1d(shape)) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = load().bkh_malloc(max(self.nbytes, 1))
+        if not self.ptr:
+            raise BkspDeviceError("bkh_malloc failed: " + last_error())
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        if load().bkh_h2d(self.ptr, _ptr(a), self.nbytes):
+            raise BkspDeviceError("bkh_h2d failed: " + last_error())
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        if load().bkh_d2h(_ptr(out), self.ptr, self.nbytes):
+            raise BkspDeviceError("bkh_d2h failed: " + last_error())
+        return out
+
+    def free(self):
+        if self.ptr:
+            load().bkh_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Matrix:
+    """A device-resident CSR (bkh_matrix). ``symmetric=True`` means the arrays hold
+    one stored triangle, as a bksp "symmetric" dspmat does."""
+
+    def __init__(self, n, row_ptr, col, val, symmetric=False, transpose=False, fp32=False):
+        self._rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+        self._ci = np.ascontiguousarray(col, dtype=np.int32)
+        self._v = np.ascontiguousarray(val, dtype=np.float64)
+        self.n = int(n)
+        self.handle = load().bkh_matrix_create(self.n, _ptr(self._rp), _ptr(self._ci), _ptr(self._v),
+                                               int(symmetric), int(transpose), int(fp32))
+        if not self.handle:
+            raise BkspDeviceError("bkh_matrix_create failed: " + last_error())
+        self.nnz = load().bkh_matrix_nnz(self.handle)
+        self.nblocks = load().bkh_matrix_blocks(self.handle)
+        del self._rp, self._ci, self._v
+
+    def destroy(self):
+        if self.handle:
+            load().bkh_matrix_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def spmv(self, d_x, d_y, op="n", alpha=1.0, beta=0.0):
+        rc = load().bkh_spmv(self.handle, op.encode(), alpha, d_x.ptr, beta, d_y.ptr)
+        if rc:
+            raise BkspDeviceError(f"bkh_spmv -> {rc}: {last_error()}")
+
+    def spmv32(self, d_x, d_y):
+        rc = load().bkh_spmv32(self.handle, d_x.ptr, d_y.ptr)
+        if rc:
+            raise BkspDeviceError(f"bkh_spmv32 -> {rc}: {last_error()}")
+
+
+def make_opts(precision=BKH_FP64, graph_iters=0, fix_blcg_p0=0, max_inner=0, inner_tol=0.0):
+    return BkhOpts(precision, graph_iters, fix_blcg_p0, max_inner, inner_tol)
+
+
+def _check(rc, name):
+    if rc == -5 or rc == -6:
+        raise BkspDeviceError(f"{name} -> {rc}: {last_error()}")
+
+
+def cg(M, d_b, d_x, tol, max_iter, **kw):
+    opts = make_opts(**kw)
+    info = BkhInfo()
+    rc = load().bkh_cg(M.handle, d_b.ptr, d_x.ptr, tol, max_iter, ctypes.byref(opts), ctypes.byref(info))
+    _check(rc, "bkh_cg")
+    return info
+
+
+def dot(d_x, d_y):
+    out = ctypes.c_double()
+    rc = load().bkh_dot(int(np.prod(d_x.shape)), d_x.ptr, d_y.ptr, ctypes.byref(out))
+    _check(rc, "bkh_dot")
+    return out.value
+
+
+def device_name():
+    buf = ctypes.create_string_buffer(256)
+    if load().bkh_device_name(buf, 256):
+        raise BkspDeviceError(last_error())
+    return buf.value.decode()
+
+
+def bench_spmv(M, d_x, d_y, reps=50, fp32=False):
+    """Average ms per SpMV launch (HIP events on the kernels' stream)."""
+    out = ctypes.c_double()
+    rc = load().bkh_bench_spmv(M.handle, int(fp32), d_x.ptr, d_y.ptr, reps, ctypes.byref(out))
+    _check(rc, "bkh_bench_spmv")
+    return out.value
+
+
+def bench_copy(d_a, d_b, nbytes, reps=20):
+    out = ctypes.c_double()
+    rc = load().bkh_bench_copy(d_a.ptr, d_b.ptr, nbytes, reps, ctypes.byref(out))
+    _check(rc, "bkh_bench_copy")
+    return out.value

@@ -1,0 +1,118 @@
+/*
+ * bksp_hip.h -- extension API of the MI355X-native bksp drop-in.
+ *
+ * Nothing here exists in the reference: bksp.h's entry points take host arrays,
+ * upload them, solve on the GPU and download x. This header exposes the device
+ * layer underneath for callers that keep data resident in HBM (benchmarks, the
+ * precision/batched/preconditioned extensions) and for tests of single kernels.
+ * All pointers named d_* are device pointers obtained from bkh_malloc; all other
+ * pointers are host pointers. Every call runs on the calling thread's own HIP
+ * stream and is synchronous unless stated otherwise.
+ */
+#ifndef BKSP_HIP_H
+#define BKSP_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Extension status codes (bksp.h keeps -1..-4). */
+#define BKH_DEVICE_ERROR  (-5)
+#define BKH_BAD_ARGUMENT  (-6)
+
+/* Precision of the Krylov iteration. */
+#define BKH_FP64     0   /* plain FP64 */
+#define BKH_FP32_IR  1   /* FP32 inner solve + FP64 iterative refinement */
+
+/* Preconditioners of bkh_pcg. */
+#define BKH_PREC_NONE   0
+#define BKH_PREC_JACOBI 1
+#define BKH_PREC_FSAI   2
+
+typedef struct bkh_matrix bkh_matrix;   /* opaque device-resident CSR */
+
+typedef struct {
+    int    precision;    /* BKH_FP64 (default) or BKH_FP32_IR */
+    int    graph_iters;  /* iterations per captured hipGraph; 0 = library default */
+    int    fix_blcg_p0;  /* block CG: 1 = first direction P0 = V (the reference's
+                            uninitialised beta gives P0 = 2V, the default here) */
+    int    max_inner;    /* FP32-IR: inner iteration cap per refinement pass (0 = default) */
+    double inner_tol;    /* FP32-IR: inner relative tolerance (0 = default) */
+} bkh_opts;
+
+typedef struct {
+    int    ret;          /* value the bksp.h entry point returns */
+    int    status;       /* 1 converged, 2 max_iter reached, 3 breakdown/NaN */
+    int    iters;        /* loop index at the break (outer index for BiCGSTAB(l)) */
+    int    inner_iters;  /* FP32-IR: inner iterations summed over passes */
+    int    passes;       /* FP32-IR: refinement passes */
+    double rec_res;      /* final recursive residual ratio */
+    double true_res;     /* FP64 true residual ||b - A x|| / ||b|| (device) */
+    double time_ms;      /* device time of the solve, HIP events on the solve stream */
+} bkh_info;
+
+/* ---- runtime ---- */
+int         bkh_init(void);
+const char *bkh_last_error(void);
+int         bkh_device_name(char *buf, int len);
+int         bkh_sync(void);
+
+/* ---- device memory (the calling thread's stream) ---- */
+void *bkh_malloc(size_t bytes);
+void  bkh_free(void *d_ptr);
+int   bkh_h2d(void *d_dst, const void *src, size_t bytes);
+int   bkh_d2h(void *dst, const void *d_src, size_t bytes);
+int   bkh_d2d(void *d_dst, const void *d_src, size_t bytes);
+int   bkh_memset(void *d_ptr, int value, size_t bytes);
+
+/* ---- matrices ----
+ * stored_triangle = 1: row_ptr/col/val hold one triangle of a symmetric matrix
+ * (bksp.h "symmetric"); the device copy is the expanded full matrix whose rows
+ * list entries in the reference's accumulation order (src/matvec.c:100-127).
+ * want_transpose = 1 also builds A^T (src/matvec.c:29-46 order).
+ * want_fp32 = 1 also keeps FP32 values (BKH_FP32_IR). */
+bkh_matrix *bkh_matrix_create(int n, const int *row_ptr, const int *col,
+                              const double *val, int stored_triangle,
+                              int want_transpose, int want_fp32);
+void        bkh_matrix_destroy(bkh_matrix *A);
+long long   bkh_matrix_nnz(const bkh_matrix *A);
+int         bkh_matrix_rows(const bkh_matrix *A);
+int         bkh_matrix_blocks(const bkh_matrix *A);
+
+/* ---- kernels on device vectors ----
+ * y = alpha*op(A)*x + beta*y, op = 'n' (A) or 't' (A^T). beta == 0 overwrites y.
+ * Replaces src/matvec.c:11-53 (dcsrgemm0), :100-127 (dcsrsymm0), :130-172
+ * (dcsrgemm1) and :219-246 (dcsrsymm1) for one right-hand side. */
+int bkh_spmv(const bkh_matrix *A, char op, double alpha, const double *d_x,
+             double beta, double *d_y);
+/* FP32 values and vectors (precision extension). */
+int bkh_spmv32(const bkh_matrix *A, const float *d_x, float *d_y);
+/* Y = alpha*A*X + beta*Y for s column-major columns (ld = rows); replaces the
+ * mkl_dcsrmm dispatch of include/bksp_internal.h:63-79. */
+int bkh_spmm(const bkh_matrix *A, int s, double alpha, const double *d_X,
+             double beta, double *d_Y);
+/* Deterministic FP64 dot product (result to host). */
+int bkh_dot(long long n, const double *d_x, const double *d_y, double *out);
+/* G = X^T Y (s x s, column-major, to host) for m x s column-major X, Y, on FP64 MFMA. */
+int bkh_gram(int m, int s, const double *d_X, const double *d_Y, double *G);
+
+/* ---- solvers on device vectors (x: initial guess in, solution out) ----
+ * The return value is what the matching bksp.h entry point returns. */
+int bkh_cg(bkh_matrix *A, const double *d_b, double *d_x, double tol, int max_iter,
+           const bkh_opts *opts, bkh_info *info);
+
+/* Timing of the most recent solve on this thread (ms, HIP events). */
+double bkh_last_solve_ms(void);
+/* Average ms per launch of `reps` back-to-back SpMVs (FP64, or FP32 if fp32 = 1),
+ * HIP events on the stream the kernels run on. */
+int bkh_bench_spmv(const bkh_matrix *A, int fp32, const void *d_x, void *d_y, int reps,
+                   double *avg_ms);
+/* Average ms of a device copy of `bytes` (a bandwidth reference). */
+int bkh_bench_copy(const void *d_a, void *d_b, size_t bytes, int reps, double *avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BKSP_HIP_H */

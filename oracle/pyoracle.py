@@ -1,0 +1,237 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so) -- test infrastructure.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import
+this module; the product (bksp_amd) never does. ``ref()`` loads
+oracle/_ref/libref_matvec.so, the reference's own src/matvec.c and src/norm.c
+compiled unmodified, when it has been built in this container.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libref_matvec.so")
+
+_o = None
+_r = None
+c_int, c_dbl, c_char = ctypes.c_int, ctypes.c_double, ctypes.c_char
+IP = ctypes.POINTER(ctypes.c_int)
+DP = ctypes.POINTER(ctypes.c_double)
+
+
+def _ip(a):
+    return a.ctypes.data_as(IP)
+
+
+def _dp(a):
+    return a.ctypes.data_as(DP)
+
+
+def lib():
+    global _o
+    if _o is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle/liboracle.so missing: run `make -C oracle`")
+        L = ctypes.CDLL(ORACLE_SO)
+        for name in ("orc_ddot", "orc_dnrm2", "orc_sp_norm_f", "orc_wall_seconds"):
+            getattr(L, name).restype = c_dbl
+        _o = L
+    return _o
+
+
+def ref():
+    """The reference's compiled kernels, or None when they were not built."""
+    global _r
+    if _r is None and os.path.exists(REF_SO):
+        _r = ctypes.CDLL(REF_SO)
+        _r.sp_norm_f.restype = c_dbl
+    return _r
+
+
+def _csr(rp, ci, v):
+    return (np.ascontiguousarray(rp, np.int32), np.ascontiguousarray(ci, np.int32),
+            np.ascontiguousarray(v, np.float64))
+
+
+# ---------------------------------------------------------------- kernels
+
+def csrgemv(trans, n, rp, ci, v, x):
+    rp, ci, v = _csr(rp, ci, v)
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.empty(n)
+    lib().orc_csrgemv(c_char(trans.encode()), n, _dp(v), _ip(rp), _ip(ci), _dp(x), _dp(y))
+    return y
+
+
+def csrsymv(n, rp, ci, v, x):
+    rp, ci, v = _csr(rp, ci, v)
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.empty(n)
+    lib().orc_csrsymv(n, _dp(v), _ip(rp), _ip(ci), _dp(x), _dp(y))
+    return y
+
+
+def csrmm(kind, n, s, alpha, rp, ci, v, B, beta, C):
+    """mkl_dcsrmm dispatch semantics; B, C are n x s column-major (Fortran order)."""
+    rp, ci, v = _csr(rp, ci, v)
+    B = np.asfortranarray(B, np.float64)
+    C = np.array(C, dtype=np.float64, order="F", copy=True)
+    lib().orc_csrmm(c_char(kind.encode()), n, s, c_dbl(alpha), _dp(v), _ip(rp), _ip(ci),
+                    _dp(B), c_dbl(beta), _dp(C))
+    return C
+
+
+def sp_norm_f(n, rp, ci, v, symmetric):
+    rp, ci, v = _csr(rp, ci, v)
+    return lib().orc_sp_norm_f(n, _ip(rp), _ip(ci), _dp(v), int(symmetric))
+
+
+def dhorner(gamma, l, sigma):
+    g = np.ascontiguousarray(gamma, np.float64)
+    out = np.zeros(l + 1)
+    lib().orc_dhorner(_dp(g), l, c_dbl(sigma), _dp(out))
+    return out
+
+
+def comb(n, r):
+    return lib().orc_comb(n, r)
+
+
+def myqr(V):
+    V = np.array(V, dtype=np.float64, order="F", copy=True)
+    m, s = V.shape
+    R = np.zeros((s, s), order="F")
+    lib().orc_myqr(m, s, _dp(V), _dp(R))
+    return V, R
+
+
+def ddot(x, y):
+    x = np.ascontiguousarray(x, np.float64); y = np.ascontiguousarray(y, np.float64)
+    return lib().orc_ddot(len(x), _dp(x), _dp(y))
+
+
+def dnrm2(x):
+    x = np.ascontiguousarray(x, np.float64)
+    return lib().orc_dnrm2(len(x), _dp(x))
+
+
+# ---------------------------------------------------------------- solvers
+
+def _solve(fn, n, rp, ci, v, b, x0, *args):
+    rp, ci, v = _csr(rp, ci, v)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.array(x0 if x0 is not None else np.zeros(len(b)), dtype=np.float64, copy=True)
+    ret = fn(n, _ip(rp), _ip(ci), _dp(v), _dp(b), *args, _dp(x))
+    return ret, x
+
+
+def cg(n, rp, ci, v, b, tol, max_iter, x0=None):
+    return _solve(lib().orc_cg, n, rp, ci, v, b, x0, c_dbl(tol), max_iter)
+
+
+def cr(n, rp, ci, v, b, tol, max_iter, x0=None):
+    return _solve(lib().orc_cr, n, rp, ci, v, b, x0, c_dbl(tol), max_iter)
+
+
+def bicg(n, rp, ci, v, b, tol, max_iter, x0=None):
+    return _solve(lib().orc_bicg, n, rp, ci, v, b, x0, c_dbl(tol), max_iter)
+
+
+def bicgstabl(n, rp, ci, v, b, l, tol, max_iter, x0=None):
+    return _solve(lib().orc_bicgstabl, n, rp, ci, v, b, x0, l, c_dbl(tol), max_iter)
+
+
+def shbicgstabl(n, rp, ci, v, b, l, sigma, tol, max_iter):
+    rp, ci, v = _csr(rp, ci, v)
+    sig = np.ascontiguousarray(sigma, np.float64)
+    x = np.zeros(n * (len(sig) + 1))
+    ret = lib().orc_shbicgstabl(n, _ip(rp), _ip(ci), _dp(v), _dp(np.ascontiguousarray(b, np.float64)),
+                                l, _dp(sig), len(sig), c_dbl(tol), max_iter, _dp(x))
+    return ret, x.reshape(len(sig) + 1, n)
+
+
+def blcg(n, rp, ci, v, B, tol, max_iter, X0=None, p0_mode=0):
+    rp, ci, v = _csr(rp, ci, v)
+    B = np.asfortranarray(B, np.float64)
+    s = B.shape[1]
+    X = np.array(X0 if X0 is not None else np.zeros((n, s)), dtype=np.float64, order="F", copy=True)
+    ret = lib().orc_blcg(n, _ip(rp), _ip(ci), _dp(v), s, _dp(B), c_dbl(tol), max_iter, _dp(X), p0_mode)
+    return ret, X
+
+
+def blbicgstabl(n, rp, ci, v, B, l, tol, max_iter, X0=None):
+    rp, ci, v = _csr(rp, ci, v)
+    B = np.asfortranarray(B, np.float64)
+    s = B.shape[1]
+    X = np.array(X0 if X0 is not None else np.zeros((n, s)), dtype=np.float64, order="F", copy=True)
+    tr = ctypes.c_double()
+    ret = lib().orc_blbicgstabl(n, _ip(rp), _ip(ci), _dp(v), s, _dp(B), l, c_dbl(tol), max_iter,
+                                _dp(X), ctypes.byref(tr))
+    return ret, X, tr.value
+
+
+def wall():
+    return lib().orc_wall_seconds()
+
+
+# ------------------------------------------------------ reference kernels
+
+def ref_csrgemv(trans, n, rp, ci, v, x):
+    """The reference's dcsrgemm0 (src/matvec.c:11-53), for one column."""
+    R = ref()
+    rp, ci, v = _csr(rp, ci, v)
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.empty(n)
+    R.dcsrgemm0(c_char(trans.encode()), n, 1, _dp(v), _ip(rp), _ip(ci), int(rp[-1]), _dp(x), _dp(y))
+    return y
+
+
+def ref_csrsymv(n, rp, ci, v, x):
+    """The reference's dcsrsymm0 (src/matvec.c:100-127) on a zeroed y, as
+    mkl_cspblas_dcsrsymv prepares it (include/bksp_internal.h:39)."""
+    R = ref()
+    rp, ci, v = _csr(rp, ci, v)
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.zeros(n)
+    R.dcsrsymm0(c_char(b"l"), n, 1, _dp(v), _ip(rp), _ip(ci), int(rp[-1]), _dp(x), _dp(y))
+    return y
+
+
+def ref_csrgemm1(n, s, alpha, rp1, ci1, v, B, beta, C):
+    """The reference's dcsrgemm1 'n' (src/matvec.c:130-172) on 1-based arrays."""
+    R = ref()
+    rp1 = np.ascontiguousarray(rp1, np.int32); ci1 = np.ascontiguousarray(ci1, np.int32)
+    v = np.ascontiguousarray(v, np.float64)
+    B = np.asfortranarray(B, np.float64)
+    C = np.array(C, dtype=np.float64, order="F", copy=True)
+    R.dcsrgemm1(c_char(b"n"), c_dbl(alpha), n, s, _dp(v), _ip(rp1), _ip(ci1), int(rp1[-1] - 1),
+                _dp(B), c_dbl(beta), _dp(C))
+    return C
+
+
+def ref_csrsymm1(n, s, alpha, rp1, ci1, v, B, C):
+    """The reference's dcsrsymm1 'n' (src/matvec.c:219-246) accumulating into C."""
+    R = ref()
+    rp1 = np.ascontiguousarray(rp1, np.int32); ci1 = np.ascontiguousarray(ci1, np.int32)
+    v = np.ascontiguousarray(v, np.float64)
+    B = np.asfortranarray(B, np.float64)
+    C = np.array(C, dtype=np.float64, order="F", copy=True)
+    R.dcsrsymm1(c_char(b"n"), c_dbl(alpha), n, s, _dp(v), _ip(rp1), _ip(ci1), int(rp1[-1] - 1),
+                _dp(B), c_dbl(0.0), _dp(C))
+    return C
+
+
+def ref_sp_norm_f(n, rp, ci, v, symmetric):
+    """The reference's sp_norm_f (src/norm.c:37-47)."""
+    R = ref()
+
+    class _Mat(ctypes.Structure):
+        _fields_ = [("row", IP), ("col", IP), ("nnz", IP), ("row_size", IP), ("col_size", IP),
+                    ("value", DP), ("format", ctypes.c_char_p), ("type", ctypes.c_char_p)]
+    rp, ci, v = _csr(rp, ci, v)
+    nn, nz = ctypes.c_int(n), ctypes.c_int(int(rp[-1]))
+    m = _Mat(_ip(rp), _ip(ci), ctypes.pointer(nz), ctypes.pointer(nn), ctypes.pointer(nn), _dp(v),
+             b"csr", b"symmetric" if symmetric else b"unsymmetric")
+    return R.sp_norm_f(ctypes.byref(m))

@@ -1,0 +1,60 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libbksp_amd.so on the device)")
+
+
+def read_mtx(path):
+    """Matrix Market coordinate file -> 0-based CSR with the reference's in-row
+    order (src/read_matrix.c:122-173: a stable counting sort by row, so each row
+    keeps file order). Returns (n, row_ptr, col, val, symmetric)."""
+    with open(path) as f:
+        banner = f.readline().split()
+        line = f.readline()
+        while line.startswith("%"):
+            line = f.readline()
+        M, N, nz = (int(t) for t in line.split())
+        data = np.loadtxt(f, ndmin=2)
+    r = data[:, 0].astype(np.int64) - 1
+    c = data[:, 1].astype(np.int32) - 1
+    v = data[:, 2].astype(np.float64)
+    order = np.argsort(r, kind="stable")
+    rp = np.zeros(M + 1, np.int32)
+    np.add.at(rp, r + 1, 1)
+    rp = np.cumsum(rp).astype(np.int32)
+    sym = banner[4].lower() == "symmetric"
+    return M, rp, c[order], v[order], sym
+
+
+@pytest.fixture(scope="session")
+def bfwb398():
+    return read_mtx(os.path.join(GOLDEN, "testmat", "bfwb398.mtx"))
+
+
+@pytest.fixture(scope="session")
+def pde2961():
+    return read_mtx(os.path.join(GOLDEN, "testmat", "pde2961.mtx"))
+
+
+def expand_lower(n, rp, ci, v):
+    """Full matrix of a stored triangle as scipy CSR (independent of the oracle)."""
+    import scipy.sparse as sp
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    A = sp.coo_matrix((v, (rows, ci)), shape=(n, n)).tocsr()
+    off = sp.coo_matrix((v[rows != ci], (ci[rows != ci], rows[rows != ci])), shape=(n, n)).tocsr()
+    return (A + off).tocsr()
+
+
+def to_scipy(n, rp, ci, v):
+    import scipy.sparse as sp
+    return sp.csr_matrix((v, ci, rp), shape=(n, n))

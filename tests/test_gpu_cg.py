@@ -1,0 +1,92 @@
+"""GPU CG (bksp_dcg / bkh_cg) against the oracle's restatement of src/cg.c.
+
+Parity bar: the loop index at the break within +-5% (at least +-1) of the CPU
+solver's, x within 1e-8 relative of the CPU x when both stop at the same index,
+and a true FP64 residual ||b - A x||/||b|| below the tolerance (with the small
+slack the recursive/true residual gap allows).
+"""
+import numpy as np
+import pytest
+
+from bksp_amd import bksp
+from bksp_amd import device as dv
+from bksp_amd import gen
+from conftest import expand_lower
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def close_index(a, b):
+    return abs(a - b) <= max(1, int(0.05 * max(a, b)))
+
+
+def test_dropin_cg_reference_scenario(bfwb398):
+    """tests/test.c:93-108: bfwb398, b = e1, tol 1e-14, max_iter 500."""
+    n, rp, ci, v, _ = bfwb398
+    b = np.zeros(n); b[0] = 1.0
+    A = bksp.dspmat(n, rp, ci, v, symmetric=True)
+    x = np.zeros(n)
+    ret = bksp.bksp_dcg(A, b, 1e-14, 500, x)
+    cret, cx = po.cg(n, rp, ci, v, b, 1e-14, 500)
+    assert (ret >= 0) == (cret >= 0)
+    if ret >= 0:
+        assert close_index(ret, cret)
+    Af = expand_lower(n, rp, ci, v)
+    assert np.linalg.norm(b - Af @ x) / np.linalg.norm(b) < 1e-10
+    assert rel(x, cx) < 1e-8
+
+
+def test_dropin_cg_errors():
+    n, rp, ci, v = gen.poisson3d(4, lower=True)
+    b = np.ones(n)
+    A = bksp.dspmat(n, rp, ci, v, symmetric=False)
+    assert bksp.bksp_dcg(A, b, 1e-8, 10, np.zeros(n)) == bksp.INVALID_MATRIX_TYPE
+    A = bksp.dspmat(n, rp, ci, v, symmetric=True)
+    assert bksp.bksp_dcg(A, b, 1e-8, 0, np.zeros(n)) == 0            # cg.c:15,67-70
+    assert bksp.bksp_dcg(A, b, 1e-30, 1, np.zeros(n)) == bksp.NOT_CONVERGED
+
+
+@pytest.mark.parametrize("m", [16, 40])
+def test_cg_poisson_parity(m):
+    n, rp, ci, v = gen.poisson3d(m, lower=True)
+    b = gen.normal(202, n)
+    tol = 1e-8
+    cret, cx = po.cg(n, rp, ci, v, b, tol, 5000)
+    M = dv.Matrix(n, rp, ci, v, symmetric=True)
+    db, dx = dv.DeviceArray.from_numpy(b), dv.DeviceArray.from_numpy(np.zeros(n))
+    info = dv.cg(M, db, dx, tol, 5000)
+    x = dx.numpy()
+    assert info.ret >= 0 and cret >= 0
+    assert close_index(info.ret, cret), (info.ret, cret)
+    assert info.true_res < 1.5 * tol
+    if info.ret == cret:
+        assert rel(x, cx) < 1e-8
+
+
+def test_cg_cfg2_full_size_fp64_and_fp32ir():
+    """cfg2: 128^3 Poisson, RHS column 0 of N(0,1) seed 202, tol 1e-8."""
+    n, rp, ci, v = gen.poisson3d(128, lower=True)
+    b = gen.normal(202, n)
+    M = dv.Matrix(n, rp, ci, v, symmetric=True, fp32=True)
+    db = dv.DeviceArray.from_numpy(b)
+    dx = dv.DeviceArray.from_numpy(np.zeros(n))
+    i64 = dv.cg(M, db, dx, 1e-8, 5000)
+    x64 = dx.numpy()
+    assert i64.ret >= 0 and i64.true_res < 1.5e-8
+    # deterministic: a second solve is bitwise identical
+    dx.upload(np.zeros(n))
+    i64b = dv.cg(M, db, dx, 1e-8, 5000)
+    assert i64b.ret == i64.ret and np.array_equal(dx.numpy(), x64)
+    dx.upload(np.zeros(n))
+    i32 = dv.cg(M, db, dx, 1e-8, 5000, precision=dv.BKH_FP32_IR)
+    assert i32.ret >= 0 and i32.true_res < 1e-8
+    # error against the FP64 result, reported: both solves meet tol, so they agree
+    # to about cond(A) * tol
+    assert rel(dx.numpy(), x64) < 1e-4
+    print(f"cfg2 fp64 iters={i64.ret} {i64.time_ms:.2f} ms; fp32ir inner={i32.inner_iters} "
+          f"passes={i32.passes} {i32.time_ms:.2f} ms; |x32-x64|/|x64|={rel(dx.numpy(), x64):.2e}")
