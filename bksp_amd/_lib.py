@@ -63,6 +63,7 @@ _SIGS = {
     "bkh_matrix_nnz": (c_ll, [c_vp]),
     "bkh_matrix_rows": (c_int, [c_vp]),
     "bkh_matrix_blocks": (c_int, [c_vp]),
+    "bkh_matrix_set_spmv": (c_int, [c_vp, c_int]),
     "bkh_spmv": (c_int, [c_vp, ctypes.c_char, c_dbl, c_vp, c_dbl, c_vp]),
     "bkh_spmv32": (c_int, [c_vp, c_vp, c_vp]),
     "bkh_dot": (c_int, [c_ll, c_vp, c_vp, c_dp]),
@@ -72,8 +73,11 @@ _SIGS = {
     # solvers (device vectors)
     "bkh_cg": (c_int, [c_vp, c_vp, c_vp, c_dbl, c_int, ctypes.POINTER(BkhOpts),
                        ctypes.POINTER(BkhInfo)]),
+    "bkh_bicgstabl": (c_int, [c_vp, c_vp, c_int, c_vp, c_dbl, c_int, ctypes.POINTER(BkhOpts),
+                              ctypes.POINTER(BkhInfo)]),
     # bksp.h drop-in (host arrays)
     "bksp_dcg": (c_int, [ctypes.POINTER(DSpMat), c_dp, c_dbl, c_int, c_dp]),
+    "dbicgstabl": (c_int, [ctypes.POINTER(DSpMat), c_dp, c_int, c_dbl, c_int, c_dp]),
     "dmat_create": (ctypes.POINTER(DMat), [c_int, c_int]),
     "dmat_free": (c_int, [ctypes.POINTER(DMat)]),
     "darray2dmat": (c_int, [c_dp, c_int, c_int, ctypes.POINTER(DMat)]),

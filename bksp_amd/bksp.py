@@ -56,3 +56,8 @@ def _dp(a):
 def bksp_dcg(A, b, tol, max_iter, x):
     """CG (reference src/cg.c:6-71)."""
     return load().bksp_dcg(A.ref, _dp(b), tol, max_iter, _dp(x))
+
+
+def dbicgstabl(A, b, l, tol, max_iter, x):
+    """BiCGSTAB(l) (reference src/bicgstabl.c:13-138); returns the outer index."""
+    return load().dbicgstabl(A.ref, _dp(b), l, tol, max_iter, _dp(x))

@@ -6,9 +6,9 @@
 //     K iterations can be replayed without a host round trip per iteration and
 //     the exact break index is still recorded on the device.
 //   * deterministic reductions: wave64 shuffles -> LDS -> one partial per
-//     workgroup -> the last workgroup to finish (atomic ticket) sums the partials
-//     in a fixed order and runs the solver's scalar update. No float atomics, so
-//     repeated runs are bitwise identical.
+//     workgroup -> a one-workgroup finalize kernel sums the partials in a fixed
+//     order and runs the solver's scalar update. No atomics, so repeated runs
+//     are bitwise identical.
 //   * xcd_block(): maps the hardware's round-robin workgroup-to-XCD dispatch
 //     onto contiguous row ranges per XCD, so each XCD's private L2 sees
 //     neighbouring rows (and their x entries) together.
@@ -31,7 +31,7 @@ struct alignas(64) Ctl {
     int result_it;     // loop index at the break
     int max_it;
     int inner_total;   // iterations summed over refinement passes
-    unsigned ticket;   // grid-reduction ticket (reset by the last workgroup)
+    unsigned ticket;   // unused (kept for layout)
     int pad0;
     double tol;
     double fnb;        // norm of the right-hand side
@@ -41,6 +41,11 @@ struct alignas(64) Ctl {
 };
 
 // ---------------------------------------------------------------- reductions
+
+__device__ __forceinline__ bool ctl_stopped(const Ctl *c) {
+    return __builtin_nontemporal_load(&c->done) != 0;
+}
+
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -74,39 +79,41 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *sh) {
     }
 }
 
-// Grid reduction with a last-workgroup finalizer. Every workgroup of the launch
-// must call this exactly once (with its partial values in v). The finalizer runs
-// once, in thread 0 of the last workgroup, with the grid totals in `tot`.
-// `part` holds gridDim.x * NV doubles.
-template <int NV, int BS, class Fin>
-__device__ __forceinline__ void grid_sum(double (&v)[NV], double *part, unsigned *ticket,
-                                         Fin fin) {
+// Writes this workgroup's NV partial sums to part[blockIdx.x * NV + i]. The
+// grid totals are formed by k_finalize, launched right after the producing
+// kernel: the kernel boundary orders and publishes the partials, so no
+// device-scope fence or atomic is needed inside the producer (on a multi-XCD
+// part each such fence would write back the XCD's L2).
+template <int NV, int BS>
+__device__ __forceinline__ void block_partial(double (&v)[NV], double *part) {
     __shared__ double sh[NV * (BS / BKH_WAVE)];
-    __shared__ int is_last;
     block_sum<NV, BS>(v, sh);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int i = 0; i < NV; i++) part[(size_t)blockIdx.x * NV + i] = v[i];
-        __threadfence();
-        unsigned t = atomicAdd(ticket, 1u);
-        is_last = (t == gridDim.x - 1);
     }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
+}
+
+// One workgroup: sums the nblocks partials of each of the NV values in a fixed
+// order (deterministic) and runs the solver's scalar update `fin` in thread 0.
+template <int NV, class Fin>
+__global__ __launch_bounds__(256) void k_finalize(const double *part, int nblocks, Ctl *c, Fin fin) {
+    if (c && ctl_stopped(c)) return;
     double acc[NV];
 #pragma unroll
     for (int i = 0; i < NV; i++) acc[i] = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) {
+    for (int b = threadIdx.x; b < nblocks; b += 256) {
 #pragma unroll
-        for (int i = 0; i < NV; i++) acc[i] += __builtin_nontemporal_load(&part[(size_t)b * NV + i]);
+        for (int i = 0; i < NV; i++) acc[i] += part[(size_t)b * NV + i];
     }
-    block_sum<NV, BS>(acc, sh);
-    if (threadIdx.x == 0) {
-        fin(acc);
-        *ticket = 0u;
-        __threadfence();
-    }
+    __shared__ double sh[NV * 4];
+    block_sum<NV, 256>(acc, sh);
+    if (threadIdx.x == 0) fin(c, acc);
+}
+
+template <int NV, class Fin>
+static inline void launch_finalize(hipStream_t s, const double *part, int nblocks, Ctl *c, Fin fin) {
+    hipLaunchKernelGGL((k_finalize<NV, Fin>), dim3(1), dim3(256), 0, s, part, nblocks, c, fin);
 }
 
 // Logical workgroup index: XCD x (hardware dispatch blockIdx % 8) receives the
@@ -115,10 +122,6 @@ __device__ __forceinline__ int xcd_block(int b, int G) {
     const int x = b % BKH_NXCD, i = b / BKH_NXCD;
     const int per = G / BKH_NXCD, rem = G % BKH_NXCD;
     return (x < rem) ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
-}
-
-__device__ __forceinline__ bool ctl_stopped(const Ctl *c) {
-    return __builtin_nontemporal_load(&c->done) != 0;
 }
 
 // Records the end of one iteration: err = sqrt(rr)/fnb; converged if err < tol

@@ -48,6 +48,11 @@ void bkh_set_error(const char *fmt, ...);
 #define SPMV_BS 256
 #define SPMV_ROWS 256
 #define SPMV_CAP 2048
+// Wave-stream variant: each 64-lane wave owns <= 64 rows / <= SPMV_WCAP nonzeros
+// and needs no workgroup barrier; 4 waves per workgroup.
+#define SPMV_WROWS 64
+#define SPMV_WCAP 512
+enum { SPMV_BLOCK = 0, SPMV_WAVE = 1 };
 
 // Device-resident CSR. Rows are stored in the reference's accumulation order
 // (see bkh_matrix_create), so a sequential per-row sum reproduces src/matvec.c.
@@ -58,8 +63,12 @@ struct bkh_matrix {
     double *val = nullptr;             // FP64 values
     float *val32 = nullptr;            // FP32 copy (precision extension)
     double *diag = nullptr;            // diagonal (for Jacobi)
-    int nblk = 0;                      // CSR-stream row blocks
+    int nblk = 0;                      // CSR-stream row blocks (workgroup variant)
     int *blk = nullptr;                // blk[nblk+1] row boundaries
+    int nwblk = 0;                     // row blocks of the wave variant
+    int *wblk = nullptr;               // wblk[nwblk+1]
+    int variant = SPMV_WAVE;           // kernel used by every SpMV on this matrix
+    int max_row = 0;                   // longest row (nonzeros)
     bkh_matrix *T = nullptr;           // A^T in the same format (BiCG)
     int stored_triangle = 0;
 };

@@ -40,7 +40,7 @@ static inline void launch_spmv_dot_w(hipStream_t s, const bkh_matrix *A, const V
 
 // FP64 residual r = b - (A + sigma I) x, asynchronously on t's stream;
 // c->s[0] = ||r||/||b||, c->s[1] = ||r||, c->s[2] = ||b||. `r` doubles as the
-// SpMV output. `part` needs max(A->nblk, 4096) * 2 doubles.
+// SpMV output. `part` needs max(spmv_grid(A), 4096) * 2 doubles.
 int resid_fp64(ThreadCtx *t, const bkh_matrix *A, double sigma, const double *b, const double *x,
                double *r, Ctl *c, double *part);
 // Allocating, synchronous FP64 true residual ||b - (A + sigma I) x|| / ||b||.

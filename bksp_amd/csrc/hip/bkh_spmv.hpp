@@ -7,7 +7,7 @@
 // order, so y[i] = (((0 + a_0 x_0) + a_1 x_1) + ...) exactly as src/matvec.c:22-25
 // computes it. A row longer than the LDS capacity gets a workgroup to itself and
 // a tree reduction. An optional epilogue accumulates NV dot products between
-// y and up to 6 vectors read at the same rows (the Krylov scalars that follow
+// y and up to 8 vectors read at the same rows (the Krylov scalars that follow
 // an SpMV), reduced deterministically with a solver-specific finalizer.
 #pragma once
 #include "bkh_core.hpp"
@@ -19,7 +19,7 @@
 
 template <int NV, typename WT = double>
 struct Epi {
-    const WT *w[6];
+    const WT *w[8];
     unsigned char a[NV > 0 ? NV : 1], b[NV > 0 ? NV : 1];
 };
 
@@ -56,12 +56,36 @@ __global__ __launch_bounds__(SPMV_BS) void k_spmv(SpmvArgs<VT, YT> a, Epi<NV, YT
     for (int i = 0; i < NV; i++) dv[i] = 0.0;
 
     if (k1 - k0 <= SPMV_CAP) {
+        // 16-byte loads of values (VEC per thread per step) and matching column
+        // indices, from an aligned base; every step is predicated so the whole
+        // stage unrolls and all loads are in flight before the x gathers.
+        constexpr int VEC = 16 / sizeof(VT);
+        constexpr int STEPS = SPMV_CAP / (VEC * SPMV_BS) + 1;
+        typedef VT vvec __attribute__((ext_vector_type(VEC)));
+        typedef int ivec __attribute__((ext_vector_type(VEC)));
         const YT al = (YT)a.alpha;
-        for (int k = k0 + tid; k < k1; k += SPMV_BS) {
-            YT v = (YT)__builtin_nontemporal_load(&a.val[k]);
-            int c = __builtin_nontemporal_load(&a.ci[k]);
-            YT xv = a.x[c];
-            prod[k - k0] = a.alpha_one ? mul_rn(v, xv) : mul_rn(mul_rn(al, v), xv);
+        const int kb = k0 & ~(VEC - 1);
+        vvec vv[STEPS];
+        ivec cc[STEPS];
+#pragma unroll
+        for (int j = 0; j < STEPS; j++) {
+            const int k = kb + VEC * (tid + j * SPMV_BS);
+            if (k < k1) {
+                vv[j] = __builtin_nontemporal_load((const vvec *)(a.val + k));
+                cc[j] = __builtin_nontemporal_load((const ivec *)(a.ci + k));
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < STEPS; j++) {
+            const int k = kb + VEC * (tid + j * SPMV_BS);
+#pragma unroll
+            for (int q = 0; q < VEC; q++) {
+                if (k + q >= k0 && k + q < k1) {
+                    YT v = (YT)vv[j][q];
+                    YT xv = a.x[cc[j][q]];
+                    prod[k + q - k0] = a.alpha_one ? mul_rn(v, xv) : mul_rn(mul_rn(al, v), xv);
+                }
+            }
         }
         __syncthreads();
         const int row = r0 + tid;
@@ -98,9 +122,93 @@ __global__ __launch_bounds__(SPMV_BS) void k_spmv(SpmvArgs<VT, YT> a, Epi<NV, YT
                 dv[i] = epi_operand(e, e.a[i], r0, ybc) * epi_operand(e, e.b[i], r0, ybc);
         }
     }
-    if constexpr (NV > 0) {
-        grid_sum<NV, SPMV_BS>(dv, part, &ctl->ticket, [&](double *tot) { fin(ctl, tot); });
+    if constexpr (NV > 0) block_partial<NV, SPMV_BS>(dv, part);
+}
+
+// Wave-stream SpMV: as k_spmv, but every wave streams its own row block into
+// its own LDS slice, so the four waves of a workgroup never wait on each other.
+template <typename VT, typename YT, int NV, class Fin>
+__global__ __launch_bounds__(256) void k_spmv_wave(SpmvArgs<VT, YT> a, int nwblk, Epi<NV, YT> e,
+                                                   Ctl *ctl, double *part, Fin fin) {
+    if (ctl && ctl_stopped(ctl)) return;
+    __shared__ YT prod_all[4][SPMV_WCAP];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    YT *prod = prod_all[w];
+    const int b = xcd_block(blockIdx.x, gridDim.x) * 4 + w;
+    double dv[NV > 0 ? NV : 1];
+#pragma unroll
+    for (int i = 0; i < NV; i++) dv[i] = 0.0;
+    if (b < nwblk) {
+        const int r0 = a.blk[b], r1 = a.blk[b + 1];
+        const int k0 = a.rp[r0], k1 = a.rp[r1];
+        if (k1 - k0 <= SPMV_WCAP) {
+            constexpr int VEC = 16 / sizeof(VT);
+            constexpr int STEPS = SPMV_WCAP / (VEC * 64) + 1;
+            typedef VT vvec __attribute__((ext_vector_type(VEC)));
+            typedef int ivec __attribute__((ext_vector_type(VEC)));
+            const YT al = (YT)a.alpha;
+            const int kb = k0 & ~(VEC - 1);
+            vvec vv[STEPS];
+            ivec cc[STEPS];
+#pragma unroll
+            for (int j = 0; j < STEPS; j++) {
+                const int k = kb + VEC * (lane + j * 64);
+                if (k < k1) {
+                    vv[j] = __builtin_nontemporal_load((const vvec *)(a.val + k));
+                    cc[j] = __builtin_nontemporal_load((const ivec *)(a.ci + k));
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < STEPS; j++) {
+                const int k = kb + VEC * (lane + j * 64);
+#pragma unroll
+                for (int q = 0; q < VEC; q++) {
+                    if (k + q >= k0 && k + q < k1) {
+                        YT v = (YT)vv[j][q];
+                        YT xv = a.x[cc[j][q]];
+                        prod[k + q - k0] = a.alpha_one ? mul_rn(v, xv) : mul_rn(mul_rn(al, v), xv);
+                    }
+                }
+            }
+            // LDS writes of this wave complete before its reads (wave-local handoff)
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int row = r0 + lane;
+            if (row < r1) {
+                const int s0 = a.rp[row] - k0, s1 = a.rp[row + 1] - k0;
+                YT s = a.has_beta ? mul_rn((YT)a.beta, a.y[row]) : (YT)0;
+                for (int k = s0; k < s1; k++) s = add_rn(s, prod[k]);
+                a.y[row] = s;
+#pragma unroll
+                for (int i = 0; i < NV; i++)
+                    dv[i] = epi_operand(e, e.a[i], row, (double)s) * epi_operand(e, e.b[i], row, (double)s);
+            }
+        } else {
+            // one long row per wave: strided sums, butterfly reduction
+            double acc = 0.0;
+            for (int k = k0 + lane; k < k1; k += 64) {
+                double v = (double)a.val[k];
+                if (!a.alpha_one) v = a.alpha * v;
+                acc += v * (double)a.x[a.ci[k]];
+            }
+            acc = wave_sum(acc);
+            if (a.has_beta) acc += a.beta * (double)a.y[r0];
+            const double ys = (double)(YT)acc;
+            if (lane == 0) {
+                a.y[r0] = (YT)acc;
+#pragma unroll
+                for (int i = 0; i < NV; i++)
+                    dv[i] = epi_operand(e, e.a[i], r0, ys) * epi_operand(e, e.b[i], r0, ys);
+            }
+        }
     }
+    if constexpr (NV > 0) block_partial<NV, 256>(dv, part);
+}
+
+// Number of workgroups a launch uses (also the number of partials it writes).
+static inline int spmv_grid(const bkh_matrix *A) {
+    return A->variant == SPMV_WAVE ? (A->nwblk + 3) / 4 : A->nblk;
 }
 
 // Host launcher. `ctl`/`part` may be null when NV == 0.
@@ -111,8 +219,16 @@ static inline void launch_spmv(hipStream_t s, const bkh_matrix *A, const VT *val
     SpmvArgs<VT, YT> a;
     a.rp = A->rp; a.ci = A->ci; a.blk = A->blk; a.val = val; a.x = x; a.y = y;
     a.alpha = alpha; a.beta = beta; a.alpha_one = (alpha == 1.0); a.has_beta = (beta != 0.0);
-    hipLaunchKernelGGL((k_spmv<VT, YT, NV, Fin>), dim3(A->nblk), dim3(SPMV_BS), 0, s, a, e, ctl,
-                       part, fin);
+    const int G = spmv_grid(A);
+    if (A->variant == SPMV_WAVE) {
+        a.blk = A->wblk;
+        hipLaunchKernelGGL((k_spmv_wave<VT, YT, NV, Fin>), dim3(G), dim3(256), 0, s, a, A->nwblk, e,
+                           ctl, part, fin);
+    } else {
+        hipLaunchKernelGGL((k_spmv<VT, YT, NV, Fin>), dim3(G), dim3(SPMV_BS), 0, s, a, e, ctl,
+                           part, fin);
+    }
+    if constexpr (NV > 0) launch_finalize<NV>(s, part, G, ctl, fin);
 }
 
 template <int NV, typename WT = double>

@@ -54,7 +54,7 @@ __global__ __launch_bounds__(256) void k_cg_init(long long n, const T *b, const 
         v[0] += (double)ri * (double)ri;
         v[1] += (double)b[i] * (double)b[i];
     }
-    grid_sum<2, 256>(v, part, &c->ticket, [&](double *tot) { CgInitFin()(c, tot); });
+    block_partial<2, 256>(v, part);
 }
 
 template <typename T>
@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void k_cg_xr(long long n, T *x, T *r, const T 
         r[i] = ri;
         v[0] += (double)ri * (double)ri;
     }
-    grid_sum<1, 256>(v, part, &c->ticket, [&](double *tot) { CgBetaFin()(c, tot); });
+    block_partial<1, 256>(v, part);
 }
 
 template <typename T>
@@ -92,10 +92,12 @@ static int cg_core(ThreadCtx *t, const bkh_matrix *A, const VT *val, const T *b,
     memset(&e0, 0, sizeof e0);
     launch_spmv<VT, T, 0>(s, A, val, x, q, 1.0, 0.0, e0, nullptr, nullptr, NoFin());
     hipLaunchKernelGGL(k_cg_init<T>, dim3(G), dim3(256), 0, s, n, b, q, r, p, c, part);
+    launch_finalize<2>(s, part, G, c, CgInitFin());
     LAUNCH_OK();
     hipGraphExec_t ge = capture_graph(s, K, [&]() {
         launch_spmv_dot_self<VT, T>(s, A, val, p, q, c, part, CgAlphaFin());
         hipLaunchKernelGGL(k_cg_xr<T>, dim3(G), dim3(256), 0, s, n, x, r, p, q, c, part);
+        launch_finalize<1>(s, part, G, c, CgBetaFin());
         hipLaunchKernelGGL(k_cg_p<T>, dim3(G), dim3(256), 0, s, n, p, r, c);
     });
     if (!ge) { bkh_set_error("hipGraph capture of the CG iteration failed"); return BKH_DEVICE_ERROR; }
@@ -114,7 +116,7 @@ extern "C" int bkh_cg(bkh_matrix *A, const double *b, double *x, double tol, int
     memset(&inf, 0, sizeof inf);
     const long long n = A->n;
     const int K = o.graph_iters > 0 ? o.graph_iters : default_graph_iters(n);
-    const int nparts = std::max(A->nblk, vec_grid(n, 256)) * 4;
+    const int nparts = std::max(spmv_grid(A), vec_grid(n, 256)) * 4;
     if (max_iter <= 0) {                     // cg.c:15,67-70: error = 0 -> returns 0
         inf.ret = 0; inf.status = ST_CONVERGED;
         if (info) *info = inf;

@@ -253,17 +253,20 @@ static void build_ordered(int n, const int *rp, const int *ci, const double *v, 
         }
 }
 
-static void build_blocks(int n, const std::vector<int> &rp, std::vector<int> &blk) {
+// Greedy row blocks of <= max_rows rows and <= cap nonzeros; a row longer than
+// cap forms a block of its own.
+static void build_blocks(int n, const std::vector<int> &rp, int max_rows, int cap,
+                         std::vector<int> &blk) {
     blk.clear();
     blk.push_back(0);
     int r = 0;
     while (r < n) {
         int len = rp[r + 1] - rp[r];
-        if (len > SPMV_CAP) { r++; blk.push_back(r); continue; }
+        if (len > cap) { r++; blk.push_back(r); continue; }
         int start = r, nz = 0;
-        while (r < n && r - start < SPMV_ROWS) {
+        while (r < n && r - start < max_rows) {
             int l = rp[r + 1] - rp[r];
-            if (l > SPMV_CAP || nz + l > SPMV_CAP) break;
+            if (l > cap || nz + l > cap) break;
             nz += l;
             r++;
         }
@@ -274,7 +277,7 @@ static void build_blocks(int n, const std::vector<int> &rp, std::vector<int> &bl
 static void free_matrix(bkh_matrix *A) {
     if (!A) return;
     hipFree(A->rp); hipFree(A->ci); hipFree(A->val); hipFree(A->val32);
-    hipFree(A->diag); hipFree(A->blk);
+    hipFree(A->diag); hipFree(A->blk); hipFree(A->wblk);
     if (A->T) free_matrix(A->T);
     delete A;
 }
@@ -287,19 +290,28 @@ static bkh_matrix *upload(int n, const std::vector<int> &rp, const std::vector<i
     if (!A) return nullptr;
     A->n = n;
     A->nnz = rp[n];
-    std::vector<int> blk;
-    build_blocks(n, rp, blk);
+    std::vector<int> blk, wblk;
+    build_blocks(n, rp, SPMV_ROWS, SPMV_CAP, blk);
+    build_blocks(n, rp, SPMV_WROWS, SPMV_WCAP, wblk);
     A->nblk = (int)blk.size() - 1;
+    A->nwblk = (int)wblk.size() - 1;
+    for (int i = 0; i < n; i++) A->max_row = std::max(A->max_row, rp[i + 1] - rp[i]);
+    // the wave variant serves a row longer than SPMV_WCAP with a single wave; keep
+    // the workgroup variant for matrices with such rows
+    A->variant = (A->max_row <= SPMV_WCAP) ? SPMV_WAVE : SPMV_BLOCK;
     std::vector<double> d((size_t)n, 0.0);
     for (int i = 0; i < n; i++)
         for (int k = rp[i]; k < rp[i + 1]; k++)
             if (ci[k] == i) d[i] += v[k];
-    size_t nz = (size_t)(A->nnz > 0 ? A->nnz : 1);
+    // +4 elements: the SpMV's 16-byte vector loads start at an aligned index and
+    // may cover up to 3 entries past the last nonzero (masked, never used).
+    size_t nz = (size_t)(A->nnz > 0 ? A->nnz : 1) + 4;
     bool ok = hipMalloc(&A->rp, sizeof(int) * (n + 1)) == hipSuccess &&
               hipMalloc(&A->ci, sizeof(int) * nz) == hipSuccess &&
               hipMalloc(&A->val, sizeof(double) * nz) == hipSuccess &&
               hipMalloc(&A->diag, sizeof(double) * n) == hipSuccess &&
-              hipMalloc(&A->blk, sizeof(int) * blk.size()) == hipSuccess;
+              hipMalloc(&A->blk, sizeof(int) * blk.size()) == hipSuccess &&
+              hipMalloc(&A->wblk, sizeof(int) * wblk.size()) == hipSuccess;
     if (ok && want_fp32) ok = hipMalloc(&A->val32, sizeof(float) * nz) == hipSuccess;
     if (!ok) {
         bkh_set_error("device allocation for a %d-row matrix failed", n);
@@ -311,7 +323,8 @@ static bkh_matrix *upload(int n, const std::vector<int> &rp, const std::vector<i
          hipMemcpyAsync(A->ci, ci.data(), sizeof(int) * A->nnz, hipMemcpyHostToDevice, s) == hipSuccess &&
          hipMemcpyAsync(A->val, v.data(), sizeof(double) * A->nnz, hipMemcpyHostToDevice, s) == hipSuccess &&
          hipMemcpyAsync(A->diag, d.data(), sizeof(double) * n, hipMemcpyHostToDevice, s) == hipSuccess &&
-         hipMemcpyAsync(A->blk, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice, s) == hipSuccess;
+         hipMemcpyAsync(A->blk, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
+         hipMemcpyAsync(A->wblk, wblk.data(), sizeof(int) * wblk.size(), hipMemcpyHostToDevice, s) == hipSuccess;
     std::vector<float> v32;
     if (ok && want_fp32) {
         v32.resize((size_t)A->nnz);
@@ -361,7 +374,14 @@ extern "C" void bkh_matrix_destroy(bkh_matrix *A) {
 
 extern "C" long long bkh_matrix_nnz(const bkh_matrix *A) { return A ? A->nnz : -1; }
 extern "C" int bkh_matrix_rows(const bkh_matrix *A) { return A ? A->n : -1; }
-extern "C" int bkh_matrix_blocks(const bkh_matrix *A) { return A ? A->nblk : -1; }
+extern "C" int bkh_matrix_blocks(const bkh_matrix *A) { return A ? spmv_grid(A) : -1; }
+
+extern "C" int bkh_matrix_set_spmv(bkh_matrix *A, int variant) {
+    if (!A || (variant != SPMV_BLOCK && variant != SPMV_WAVE)) return BKH_BAD_ARGUMENT;
+    A->variant = variant;
+    if (A->T) A->T->variant = variant;
+    return 0;
+}
 
 // ------------------------------------------------------------- single kernels
 
@@ -410,7 +430,7 @@ __global__ __launch_bounds__(256) void k_dot(long long n, const double *x, const
     double v[1] = {0.0};
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
         v[0] += x[i] * y[i];
-    grid_sum<1, 256>(v, part, &c->ticket, [&](double *tot) { StoreFin()(c, tot); });
+    block_partial<1, 256>(v, part);
 }
 
 extern "C" int bkh_dot(long long n, const double *x, const double *y, double *out) {
@@ -424,6 +444,7 @@ extern "C" int bkh_dot(long long n, const double *x, const double *y, double *ou
     double *part = cv.take<double>(G);
     if (ctl_init(t, c, 0.0, 1)) return BKH_DEVICE_ERROR;
     hipLaunchKernelGGL(k_dot, dim3(G), dim3(256), 0, t->stream, n, x, y, c, part);
+    launch_finalize<1>(t->stream, part, G, c, StoreFin());
     LAUNCH_OK();
     if (ctl_fetch(t, c)) return BKH_DEVICE_ERROR;
     *out = t->ctl_host->s[0];
@@ -455,7 +476,7 @@ __global__ __launch_bounds__(256) void k_resid(long long n, const double *b, con
         v[0] += ri * ri;
         v[1] += b[i] * b[i];
     }
-    grid_sum<2, 256>(v, part, &c->ticket, [&](double *tot) { ResidFin()(c, tot); });
+    block_partial<2, 256>(v, part);
 }
 
 int resid_fp64(ThreadCtx *t, const bkh_matrix *A, double sigma, const double *b, const double *x,
@@ -463,8 +484,9 @@ int resid_fp64(ThreadCtx *t, const bkh_matrix *A, double sigma, const double *b,
     Epi<0> e;
     memset(&e, 0, sizeof e);
     launch_spmv<double, double, 0>(t->stream, A, A->val, x, r, 1.0, 0.0, e, nullptr, nullptr, NoFin());
-    hipLaunchKernelGGL(k_resid, dim3(vec_grid(A->n, 256)), dim3(256), 0, t->stream, (long long)A->n, b,
-                       x, sigma, r, c, part);
+    const int G = vec_grid(A->n, 256);
+    hipLaunchKernelGGL(k_resid, dim3(G), dim3(256), 0, t->stream, (long long)A->n, b, x, sigma, r, c, part);
+    launch_finalize<2>(t->stream, part, G, c, ResidFin());
     LAUNCH_OK();
     return 0;
 }

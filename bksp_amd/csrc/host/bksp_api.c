@@ -33,11 +33,11 @@ typedef struct {
     size_t n_b, n_x;
 } upload_t;
 
-static int upload_system(dspmat *A, const double *b, size_t nb, const double *x, size_t nx,
-                         int want_T, int want_fp32, upload_t *u) {
+static int upload_general(dspmat *A, const double *b, size_t nb, const double *x, size_t nx,
+                          int triangle, int want_T, int want_fp32, upload_t *u) {
     memset(u, 0, sizeof *u);
     int n = *A->row_size;
-    u->M = bkh_matrix_create(n, A->row, A->col, A->value, is_symmetric(A), want_T, want_fp32);
+    u->M = bkh_matrix_create(n, A->row, A->col, A->value, triangle, want_T, want_fp32);
     if (!u->M) return BKH_DEVICE_ERROR;
     u->d_b = (double *)bkh_malloc(sizeof(double) * nb);
     u->d_x = (double *)bkh_malloc(sizeof(double) * nx);
@@ -47,6 +47,18 @@ static int upload_system(dspmat *A, const double *b, size_t nb, const double *x,
     if (bkh_h2d(u->d_b, b, sizeof(double) * nb) || bkh_h2d(u->d_x, x, sizeof(double) * nx))
         return BKH_DEVICE_ERROR;
     return 0;
+}
+
+/* Symmetric solvers: a "symmetric" A is one stored triangle, expanded on upload. */
+static int upload_system_sym(dspmat *A, const double *b, size_t nb, const double *x, size_t nx,
+                             int want_fp32, upload_t *u) {
+    return upload_general(A, b, nb, x, nx, is_symmetric(A), 0, want_fp32, u);
+}
+
+/* Nonsymmetric solvers: the stored entries as they are (src/matvec.c:15-28). */
+static int upload_system(dspmat *A, const double *b, size_t nb, const double *x, size_t nx,
+                         int want_T, int want_fp32, upload_t *u) {
+    return upload_general(A, b, nb, x, nx, 0, want_T, want_fp32, u);
 }
 
 static int download_x(upload_t *u, double *x) {
@@ -66,9 +78,25 @@ int bksp_dcg(dspmat *A, double *b, double tol, int max_iter, double *x) {
     bkh_opts o = default_opts();
     int n = *A->row_size;
     upload_t u;
-    int rc = upload_system(A, b, n, x, n, 0, o.precision == BKH_FP32_IR, &u);
+    int rc = upload_system_sym(A, b, n, x, n, o.precision == BKH_FP32_IR, &u);
     if (rc == 0) {
         rc = bkh_cg(u.M, u.d_b, u.d_x, tol, max_iter, &o, NULL);
+        if (rc > BKH_DEVICE_ERROR && download_x(&u, x)) rc = BKH_DEVICE_ERROR;
+    }
+    release(&u);
+    return rc;
+}
+
+/* reference src/bicgstabl.c:13-138. Like the reference, the nonsymmetric solvers
+ * use the stored entries as a general CSR whatever A->type says. */
+int dbicgstabl(dspmat *A, double *b, int l, double tol, int max_iter, double *x) {
+    if (max_iter <= 0) return 0;                     /* bicgstabl.c:18, 134-137 */
+    bkh_opts o = default_opts();
+    int n = *A->row_size;
+    upload_t u;
+    int rc = upload_system(A, b, n, x, n, 0, o.precision == BKH_FP32_IR, &u);
+    if (rc == 0) {
+        rc = bkh_bicgstabl(u.M, u.d_b, l, u.d_x, tol, max_iter, &o, NULL);
         if (rc > BKH_DEVICE_ERROR && download_x(&u, x)) rc = BKH_DEVICE_ERROR;
     }
     release(&u);

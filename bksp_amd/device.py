@@ -97,6 +97,12 @@ class Matrix:
         except Exception:
             pass
 
+    def set_spmv(self, variant):
+        """0: workgroup CSR-stream kernel, 1: wave CSR-stream kernel."""
+        if load().bkh_matrix_set_spmv(self.handle, int(variant)):
+            raise BkspDeviceError("bkh_matrix_set_spmv failed")
+        self.nblocks = load().bkh_matrix_blocks(self.handle)
+
     def spmv(self, d_x, d_y, op="n", alpha=1.0, beta=0.0):
         rc = load().bkh_spmv(self.handle, op.encode(), alpha, d_x.ptr, beta, d_y.ptr)
         if rc:
@@ -122,6 +128,15 @@ def cg(M, d_b, d_x, tol, max_iter, **kw):
     info = BkhInfo()
     rc = load().bkh_cg(M.handle, d_b.ptr, d_x.ptr, tol, max_iter, ctypes.byref(opts), ctypes.byref(info))
     _check(rc, "bkh_cg")
+    return info
+
+
+def bicgstabl(M, d_b, l, d_x, tol, max_iter, **kw):
+    opts = make_opts(**kw)
+    info = BkhInfo()
+    rc = load().bkh_bicgstabl(M.handle, d_b.ptr, l, d_x.ptr, tol, max_iter, ctypes.byref(opts),
+                              ctypes.byref(info))
+    _check(rc, "bkh_bicgstabl")
     return info
 
 

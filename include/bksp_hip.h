@@ -80,6 +80,10 @@ void        bkh_matrix_destroy(bkh_matrix *A);
 long long   bkh_matrix_nnz(const bkh_matrix *A);
 int         bkh_matrix_rows(const bkh_matrix *A);
 int         bkh_matrix_blocks(const bkh_matrix *A);
+/* SpMV kernel variant: 0 = workgroup CSR-stream (256 rows / 2048 nonzeros per
+ * workgroup), 1 = wave CSR-stream (64 rows / 512 nonzeros per wave). Chosen at
+ * creation (1 unless a row exceeds 512 nonzeros); the autotuner may override. */
+int         bkh_matrix_set_spmv(bkh_matrix *A, int variant);
 
 /* ---- kernels on device vectors ----
  * y = alpha*op(A)*x + beta*y, op = 'n' (A) or 't' (A^T). beta == 0 overwrites y.
@@ -102,6 +106,9 @@ int bkh_gram(int m, int s, const double *d_X, const double *d_Y, double *G);
  * The return value is what the matching bksp.h entry point returns. */
 int bkh_cg(bkh_matrix *A, const double *d_b, double *d_x, double tol, int max_iter,
            const bkh_opts *opts, bkh_info *info);
+/* BiCGSTAB(l), 1 <= l <= 8 (replaces src/bicgstabl.c:13-138). */
+int bkh_bicgstabl(bkh_matrix *A, const double *d_b, int l, double *d_x, double tol,
+                  int max_iter, const bkh_opts *opts, bkh_info *info);
 
 /* Timing of the most recent solve on this thread (ms, HIP events). */
 double bkh_last_solve_ms(void);

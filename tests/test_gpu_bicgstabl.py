@@ -1,0 +1,119 @@
+"""GPU BiCGSTAB(l) (dbicgstabl / bkh_bicgstabl) against the oracle's restatement of
+src/bicgstabl.c. The device forms the MR step from a fused Gram matrix (same
+algebra as the reference's modified Gram-Schmidt), so parity is: outer index
+within +-5% (at least +-1), x within 1e-8 relative when the indices agree, and a
+true FP64 residual at the tolerance."""
+import numpy as np
+import pytest
+
+from bksp_amd import bksp
+from bksp_amd import device as dv
+from bksp_amd import gen
+from conftest import to_scipy
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def close_index(a, b):
+    return abs(a - b) <= max(1, int(0.05 * max(a, b)))
+
+
+def cfg1():
+    """64x64 convection-diffusion, velocity (10, 20), h = 1/65; x_true ~ U[-1,1] seed 101."""
+    n, rp, ci, v = gen.convdiff2d(64, 10.0, 20.0)
+    assert n == 4096 and rp[-1] == 20224
+    xt = gen.uniform(101, n, -1.0, 1.0)
+    b = po.csrgemv("n", n, rp, ci, v, xt)
+    return n, rp, ci, v, xt, b
+
+
+def test_cfg1_dropin_parity():
+    n, rp, ci, v, xt, b = cfg1()
+    A = bksp.dspmat(n, rp, ci, v)
+    x = np.zeros(n)
+    ret = bksp.dbicgstabl(A, b, 2, 1e-10, 1000, x)
+    cret, cx = po.bicgstabl(n, rp, ci, v, b, 2, 1e-10, 1000)
+    assert ret >= 0 and cret >= 0
+    assert close_index(ret, cret), (ret, cret)
+    S = to_scipy(n, rp, ci, v)
+    assert np.linalg.norm(b - S @ x) / np.linalg.norm(b) < 2e-10
+    if ret == cret:
+        assert rel(x, cx) < 1e-8
+    assert rel(x, xt) < 1e-6
+
+
+def test_reference_scenario_l4(pde2961):
+    """tests/test.c:193-209: pde2961, b = e1, l = 4, tol 1e-14, max_iter 500."""
+    n, rp, ci, v, _ = pde2961
+    b = np.zeros(n); b[0] = 1.0
+    A = bksp.dspmat(n, rp, ci, v)
+    x = np.zeros(n)
+    ret = bksp.dbicgstabl(A, b, 4, 1e-14, 500, x)
+    cret, cx = po.bicgstabl(n, rp, ci, v, b, 4, 1e-14, 500)
+    S = to_scipy(n, rp, ci, v)
+    assert (ret >= 0) == (cret >= 0)
+    if ret >= 0:
+        assert close_index(ret, cret), (ret, cret)
+    assert np.linalg.norm(b - S @ x) < 1e-9
+    assert rel(x, cx) < 1e-8
+
+
+@pytest.mark.parametrize("l", [1, 2, 3, 4, 6, 8])
+def test_degrees(pde2961, l):
+    n, rp, ci, v, _ = pde2961
+    b = gen.normal(7, n)
+    M = dv.Matrix(n, rp, ci, v)
+    db, dx = dv.DeviceArray.from_numpy(b), dv.DeviceArray.from_numpy(np.zeros(n))
+    info = dv.bicgstabl(M, db, l, dx, 1e-10, 2000)
+    cret, cx = po.bicgstabl(n, rp, ci, v, b, l, 1e-10, 2000)
+    assert info.ret >= 0 and cret >= 0
+    assert abs(info.ret - cret) <= max(2, int(0.1 * cret)), (info.ret, cret)
+    assert info.true_res < 5e-10
+    # both stop below 1e-10 relative residual, so their x agree to ~cond * 1e-10
+    assert rel(dx.numpy(), cx) < 1e-6
+
+
+def test_first_outer_iteration_matches_oracle(pde2961):
+    """With max_iter = k the device and oracle iterates agree to rounding."""
+    n, rp, ci, v, _ = pde2961
+    b = gen.normal(8, n)
+    M = dv.Matrix(n, rp, ci, v)
+    for l in (2, 4):
+        for k in (1, 2, 3):
+            db, dx = dv.DeviceArray.from_numpy(b), dv.DeviceArray.from_numpy(np.zeros(n))
+            dv.bicgstabl(M, db, l, dx, 1e-300, k)
+            _, cx = po.bicgstabl(n, rp, ci, v, b, l, 1e-300, k)
+            assert rel(dx.numpy(), cx) < 1e-10, (l, k)
+
+
+def test_identity_exact_and_edge_cases():
+    n = 1000
+    rp = np.arange(n + 1, dtype=np.int32)
+    ci = np.arange(n, dtype=np.int32)
+    A = bksp.dspmat(n, rp, ci, np.ones(n))
+    b = gen.normal(9, n)
+    x = np.zeros(n)
+    # r becomes exactly 0 in the first BiCG step: the device stops cleanly at index 0
+    # (the reference divides 0/0 in the MR step and returns max_iter with a NaN x)
+    assert bksp.dbicgstabl(A, b, 1, 1e-10, 50, x) == 0
+    assert np.array_equal(x, b)
+    assert bksp.dbicgstabl(A, b, 2, 1e-10, 0, np.zeros(n)) == 0
+    n2, rp2, ci2, v2 = gen.convdiff2d(16, 10.0, 20.0)
+    A2 = bksp.dspmat(n2, rp2, ci2, v2)
+    assert bksp.dbicgstabl(A2, np.ones(n2), 2, 1e-30, 1, np.zeros(n2)) == bksp.NOT_CONVERGED
+    bad = np.ones(n2); bad[3] = np.nan
+    assert bksp.dbicgstabl(A2, bad, 2, 1e-10, 20, np.zeros(n2)) == bksp.DIVERGENCE
+
+
+def test_cfg1_fp32_ir():
+    n, rp, ci, v, xt, b = cfg1()
+    M = dv.Matrix(n, rp, ci, v, fp32=True)
+    db, dx = dv.DeviceArray.from_numpy(b), dv.DeviceArray.from_numpy(np.zeros(n))
+    info = dv.bicgstabl(M, db, 2, dx, 1e-10, 2000, precision=dv.BKH_FP32_IR)
+    assert info.ret >= 0 and info.true_res < 1e-10
+    assert rel(dx.numpy(), xt) < 1e-6
