@@ -44,6 +44,20 @@ class SpMat:
         return ctypes.byref(self.s)
 
 
+class Mat:
+    """Owns the column-major array behind a ``dmat`` (include/bksp.h:23-27)."""
+
+    def __init__(self, a):
+        self.a = np.asfortranarray(a, dtype=np.float64)
+        from ._lib import DMat
+        self.s = DMat(self.a.shape[0], self.a.shape[1],
+                      self.a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+
+    @property
+    def ref(self):
+        return ctypes.byref(self.s)
+
+
 def dspmat(n, row_ptr, col, val, symmetric=False):
     return SpMat(n, row_ptr, col, val, "symmetric" if symmetric else "unsymmetric")
 
@@ -58,6 +72,21 @@ def bksp_dcg(A, b, tol, max_iter, x):
     return load().bksp_dcg(A.ref, _dp(b), tol, max_iter, _dp(x))
 
 
+def dcr(A, b, tol, max_iter, x):
+    """Conjugate residual (reference src/cr.c:6-80); the loop index starts at 1."""
+    return load().dcr(A.ref, _dp(b), tol, max_iter, _dp(x))
+
+
+def dbicg(A, b, tol, max_iter, x):
+    """BiCG (reference src/bicg.c:7-88)."""
+    return load().dbicg(A.ref, _dp(b), tol, max_iter, _dp(x))
+
+
 def dbicgstabl(A, b, l, tol, max_iter, x):
     """BiCGSTAB(l) (reference src/bicgstabl.c:13-138); returns the outer index."""
     return load().dbicgstabl(A.ref, _dp(b), l, tol, max_iter, _dp(x))
+
+
+def dblcg(A, B, tol, max_iter, X):
+    """Block CG (reference src/blcg.c:15-119). B, X are ``Mat`` (X overwritten)."""
+    return load().dblcg(A.ref, B.ref, tol, max_iter, X.ref)

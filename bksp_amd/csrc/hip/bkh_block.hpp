@@ -1,0 +1,191 @@
+// bkh_block.hpp -- kernels on m x s blocks of vectors (s <= 16), stored ROW-major
+// inside the device layer (the s values of one row are contiguous, so an SpMM
+// gather of X[c, :] is one contiguous read). The bksp.h dmat boundary is
+// column-major; k_cm2rm / k_rm2cm convert.
+//
+//   k_spmm        Y = alpha A X + beta Y: CSR-stream of A through LDS, one thread
+//                 per row, columns summed in stored order (src/matvec.c:130-172,
+//                 :219-246 order, bit for bit with alpha = 1).
+//   k_gram        G = X^T Y on FP64 MFMA (v_mfma_f64_16x16x4f64), one 16x16
+//                 accumulator per wave, per-workgroup partials.
+//   panel tiles   D(16 rows x 16) = X_tile(16 x s) * M(s x s) on MFMA: the skinny
+//                 panel-times-small-matrix products of the block methods, fused
+//                 with their axpy and, where the next step needs it, the Gram
+//                 matrix of the result (whose D layout is already the MFMA A/B
+//                 operand layout, so no data moves between the two products).
+//
+// MFMA f64 16x16x4 lane layout (checked on the device): A(16x4): lane L holds
+// A[L%16][L/16]; B(4x16): lane L holds B[L/16][L%16]; D(16x16): lane L register r
+// holds D[4*(L/16)+r][L%16].
+#pragma once
+#include "bkh_core.hpp"
+#include "bkh_host.hpp"
+#include "bkh_small.hpp"
+
+typedef double mf4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ mf4 mfma16x16x4(double a, double b, mf4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// ------------------------------------------------------------ layout changes
+
+// rm[i*s + j] = cm[i + j*m] and back (one thread per element).
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void k_cm2rm(long long m, int s, const TI *cm, TO *rm) {
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < m * s; e += (long long)gridDim.x * 256) {
+        const long long i = e / s, j = e % s;
+        rm[e] = (TO)cm[i + j * m];
+    }
+}
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void k_rm2cm(long long m, int s, const TI *rm, TO *cm) {
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < m * s; e += (long long)gridDim.x * 256) {
+        const long long i = e % m, j = e / m;
+        cm[e] = (TO)rm[i * s + j];
+    }
+}
+
+// ------------------------------------------------------------------- SpMM
+
+template <int S, typename VT, typename T>
+__global__ __launch_bounds__(SPMV_BS) void k_spmm(const int *__restrict__ rp, const int *__restrict__ ci,
+                                                  const VT *__restrict__ val,
+                                                  const int *__restrict__ blk, const T *X, T *Y,
+                                                  double alpha, double beta, const Ctl *ctl) {
+    if (ctl && ctl_stopped(ctl)) return;
+    __shared__ T sv[SPMV_CAP];
+    __shared__ int sc[SPMV_CAP];
+    const int tid = threadIdx.x;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = blk[b], r1 = blk[b + 1];
+    const int k0 = rp[r0], k1 = rp[r1];
+    const bool a1 = (alpha == 1.0);
+    if (k1 - k0 <= SPMV_CAP) {
+        for (int k = k0 + tid; k < k1; k += SPMV_BS) {
+            sv[k - k0] = a1 ? (T)val[k] : (T)(alpha * (double)val[k]);
+            sc[k - k0] = ci[k];
+        }
+        __syncthreads();
+        const int row = r0 + tid;
+        if (row < r1) {
+            T acc[S];
+#pragma unroll
+            for (int j = 0; j < S; j++) acc[j] = (beta != 0.0) ? mul_rn((T)beta, Y[(size_t)row * S + j]) : (T)0;
+            const int s0 = rp[row] - k0, s1 = rp[row + 1] - k0;
+            for (int k = s0; k < s1; k++) {
+                const T a = sv[k];
+                const T *xr = X + (size_t)sc[k] * S;
+#pragma unroll
+                for (int j = 0; j < S; j++) acc[j] = add_rn(acc[j], mul_rn(a, xr[j]));
+            }
+#pragma unroll
+            for (int j = 0; j < S; j++) Y[(size_t)row * S + j] = acc[j];
+        }
+    } else {
+        // a single long row: column j accumulated by the workgroup, tree-reduced
+        __shared__ double red[SPMV_BS / 64];
+        for (int j = 0; j < S; j++) {
+            double acc[1] = {0.0};
+            for (int k = k0 + tid; k < k1; k += SPMV_BS)
+                acc[0] += (a1 ? (double)val[k] : alpha * (double)val[k]) * (double)X[(size_t)ci[k] * S + j];
+            block_sum<1, SPMV_BS>(acc, red);
+            if (tid == 0)
+                Y[(size_t)r0 * S + j] = (T)(acc[0] + ((beta != 0.0) ? beta * (double)Y[(size_t)r0 * S + j] : 0.0));
+        }
+    }
+}
+
+// ------------------------------------------------------------ MFMA helpers
+
+// One 16-row tile product D = X_tile * M: X row-major (ld S), rows row0..row0+15
+// (rows >= m read as 0); M s x s column-major in `Ms` (shared or global), padded
+// with zeros beyond s. Lane L's A operand for K-step q is X[row0 + L%16][4q + L/16].
+template <int S, typename T>
+__device__ __forceinline__ mf4 tile_mul(const T *X, long long m, long long row0, const double *Ms) {
+    const int L = threadIdx.x & 63;
+    const long long row = row0 + (L & 15);
+    mf4 d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < (S + 3) / 4; q++) {
+        const int k = 4 * q + (L >> 4);
+        const double a = (row < m && k < S) ? (double)X[row * S + k] : 0.0;
+        const int j = L & 15;
+        const double bm = (k < S && j < S) ? Ms[k + j * S] : 0.0;
+        d = mfma16x16x4(a, bm, d);
+    }
+    return d;
+}
+
+// Loads/stores the tile elements held in the D layout (rows 4*(L/16)+r, col L%16).
+template <int S, typename T>
+__device__ __forceinline__ mf4 tile_load(const T *X, long long m, long long row0) {
+    const int L = threadIdx.x & 63, j = L & 15;
+    mf4 v;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const long long row = row0 + 4 * (L >> 4) + r;
+        v[r] = (row < m && j < S) ? (double)X[row * S + j] : 0.0;
+    }
+    return v;
+}
+
+template <int S, typename T>
+__device__ __forceinline__ void tile_store(T *X, long long m, long long row0, mf4 v) {
+    const int L = threadIdx.x & 63, j = L & 15;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const long long row = row0 + 4 * (L >> 4) + r;
+        if (row < m && j < S) X[row * S + j] = (T)v[r];
+    }
+}
+
+// Gram accumulation of a tile held in the D layout: register r of every lane is
+// row 4*(L/16)+r, column L%16, which is exactly the A operand (A[i][kk] with
+// kk <-> row 4kk + r) and the B operand of an MFMA with K slot kk.
+__device__ __forceinline__ mf4 tile_gram(mf4 v, mf4 w, mf4 acc) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) acc = mfma16x16x4(v[r], w[r], acc);
+    return acc;
+}
+
+// Sums the 16x16 wave accumulators of a workgroup and writes its s x s partial
+// (column-major) to part[blockIdx.x * s * s ...]. `sh` holds 4 * 256 doubles.
+template <int S>
+__device__ __forceinline__ void gram_block_partial(mf4 acc, double *sh, double *part) {
+    const int L = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int r = 0; r < 4; r++) sh[w * 256 + (4 * (L >> 4) + r) * 16 + (L & 15)] = acc[r];
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < S * S) {
+        const int i = t % S, j = t / S;
+        double v = 0.0;
+        for (int q = 0; q < 4; q++) v += sh[q * 256 + i * 16 + j];
+        part[(size_t)blockIdx.x * S * S + t] = v;
+    }
+}
+
+// G = X^T Y over m rows (row-major, ld S); grid-stride over 64-row chunks (16-row
+// tiles per wave).
+template <int S, typename T>
+__global__ __launch_bounds__(256) void k_gram(long long m, const T *X, const T *Y,
+                                              const Ctl *ctl, double *part) {
+    if (ctl && ctl_stopped(ctl)) return;
+    __shared__ double sh[4 * 256];
+    const int w = threadIdx.x >> 6;
+    mf4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (long long row0 = ((long long)blockIdx.x * 4 + w) * 16; row0 < m; row0 += (long long)gridDim.x * 64) {
+        mf4 a = tile_load<S>(X, m, row0);
+        mf4 b = (X == Y) ? a : tile_load<S>(Y, m, row0);
+        acc = tile_gram(a, b, acc);
+    }
+    gram_block_partial<S>(acc, sh, part);
+}
+
+static inline int block_grid(long long m) {
+    long long g = (m + 63) / 64;
+    if (g > 2048) g = 2048;
+    if (g < 1) g = 1;
+    return (int)g;
+}

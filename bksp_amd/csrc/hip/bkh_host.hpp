@@ -125,8 +125,9 @@ hipGraphExec_t capture_graph(hipStream_t s, int K, Body body) {
 // ahead of the host check. Returns 0 or BKH_DEVICE_ERROR.
 int run_graph_until_done(ThreadCtx *t, hipGraphExec_t ge, Ctl *ctl_dev, int max_launches);
 
-// Initialises a control block on the device (async, on the thread's stream).
-int ctl_init(ThreadCtx *t, Ctl *ctl_dev, double tol, int max_it);
+// Initialises a control block on the device (async, on the thread's stream);
+// it0 is the loop index of the first iteration (CR's loop starts at 1).
+int ctl_init(ThreadCtx *t, Ctl *ctl_dev, double tol, int max_it, int it0 = 0);
 // Copies the control block to t->ctl_host and synchronises.
 int ctl_fetch(ThreadCtx *t, const Ctl *ctl_dev);
 

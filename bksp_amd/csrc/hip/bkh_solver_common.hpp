@@ -51,6 +51,13 @@ int ir_prep(ThreadCtx *t, long long n, const double *r64, const Ctl *c, float *r
 // x += ||r64|| * d32.
 int ir_update(ThreadCtx *t, long long n, double *x, const float *d32, const Ctl *c);
 
+// Ends a solve whose start was recorded on t->ev0: fills `inf` from the fetched
+// control block (t->ctl_host) and the event time, adds the FP64 true residual
+// ||b - (A + sigma I) x|| / ||b|| when `info` is requested, and returns the
+// bksp.h return value.
+int finish_solve(ThreadCtx *t, const bkh_matrix *A, double sigma, const double *b,
+                 const double *x, bkh_info *inf, bkh_info *info);
+
 struct IrResult {
     int ret = -1, status = 0, passes = 0, inner_total = 0;
     double res = 0.0;
@@ -68,6 +75,7 @@ int ir_refine(ThreadCtx *t, const bkh_matrix *A, const double *b, double *x, dou
               Inner inner, float *r32, float *d32) {
     int total = 0;
     const int max_inner = o.max_inner > 0 ? o.max_inner : max_iter;
+    if (ctl_init(t, co, 0.0, 1)) return BKH_DEVICE_ERROR;   // the residual finalizers run on co
     for (int pass = 0;; pass++) {
         if (resid_fp64(t, A, 0.0, b, x, r64, co, part)) return BKH_DEVICE_ERROR;
         if (ctl_fetch(t, co)) return BKH_DEVICE_ERROR;

@@ -1,0 +1,326 @@
+// blcg.hip -- block CG with residual QR on the device (replaces src/blcg.c:15-119).
+//
+// Blocks are m x s (s <= 16), row-major inside the device layer. One iteration:
+//   SpMM      AP = A P                                                (blcg.c:74-75)
+//   k_gram    P^T AP on MFMA -> finalize: alpha = (P^T AP)^-1, W = alpha C  (:78-86)
+//   k_xv      X += P W, V -= AP alpha, Gram V^T V (MFMA)              (:87-92)
+//   k_vr      CholQR pass 1: V = V R1^-1, Gram V^T V                  (:95, myqr.c)
+//   finalize  R2 = chol, beta = R2 R1, C = beta C, ||C||_F/||B||_F     (:98-105)
+//   k_vp      CholQR pass 2: V = V R2^-1, and the next P = V + P beta^T  (:68-71)
+// The QR of the residual block is CholeskyQR2 (two Gram + triangular-solve passes,
+// the first with a Fukaya-style diagonal shift if the Gram matrix is numerically
+// singular). Its R has a positive diagonal where LAPACK's Householder R may have
+// negative entries; X is invariant under that sign choice (Q -> QD, R -> DR gives
+// P -> PD, alpha -> D alpha D, C -> DC, so P alpha C is unchanged).
+// First direction: blcg.c leaves beta's off-diagonal uninitialised (malloc) and sets
+// its diagonal to 1, so P0 = V + P beta^T; this build uses the calloc reading,
+// P0 = 2V, unless bkh_opts.fix_blcg_p0 asks for P0 = V.
+#include <math.h>
+#include "bkh_block.hpp"
+#include "bkh_host.hpp"
+#include "bkh_small.hpp"
+#include "bkh_solver_common.hpp"
+
+struct BlkMats {
+    double C[SM_N], alpha[SM_N], W[SM_N], R1[SM_N], R1i[SM_N], R2i[SM_N], betaT[SM_N];
+    double work[2 * SM_N];
+    double fnb2;
+    int m, s;
+};
+
+enum { FIN_QR1 = 0, FIN_QR2_INIT = 1, FIN_QR2_ITER = 2, FIN_ALPHA = 3, FIN_FNB = 4 };
+
+// One workgroup: sums the s x s Gram partials of `nblocks` workgroups in a fixed
+// order, then runs the block solver's small dense step `op`.
+template <int S, int OP>
+__global__ __launch_bounds__(256) void k_blk_fin(const double *part, int nblocks, BlkMats *mt,
+                                                 Ctl *c) {
+    if (ctl_stopped(c)) return;
+    __shared__ double G[SM_N], R[SM_N], Ri[SM_N], T1[SM_N], T2[SM_N], work[2 * SM_N];
+    __shared__ int flag[2];
+    __shared__ double red;
+    const int t = threadIdx.x;
+    if (OP == FIN_FNB) {
+        if (t == 0) {
+            double acc = 0.0;
+            for (int b = 0; b < nblocks; b++) acc += part[b];
+            mt->fnb2 = acc;
+            c->fnb = sqrt(acc);
+            if (!(c->fnb > 0.0)) ctl_breakdown(c);
+        }
+        return;
+    }
+    if (t < S * S) {
+        double acc = 0.0;
+        for (int b = 0; b < nblocks; b++) acc += part[(size_t)b * S * S + t];
+        G[t] = acc;
+    }
+    __syncthreads();
+    if (OP == FIN_ALPHA) {
+        // alpha = (P^T A P)^-1 (dgetrf/dgetri, blcg.c:81-82), W = alpha C (:85-86)
+        if (!sm_inverse(S, G, Ri, work, flag)) { if (t == 0) ctl_breakdown(c); return; }
+        sm_copy(S, Ri, mt->alpha);
+        sm_copy(S, mt->C, T1);
+        sm_mul(S, Ri, 0, T1, 0, T2);
+        sm_copy(S, T2, mt->W);
+        return;
+    }
+    // CholQR pass: G = R^T R. Pass 1 retries with a diagonal shift when G is
+    // numerically singular (shifted CholeskyQR, Fukaya et al. 2020).
+    bool ok = sm_chol_upper(S, G, 0.0, R, flag);
+    if (!ok && OP == FIN_QR1) {
+        double tr = 0.0;
+        if (t == 0) {
+            for (int k = 0; k < S; k++) tr += G[k + k * S];
+            red = 11.0 * ((double)mt->m * S + S * (S + 1)) * 1.1102230246251565e-16 * tr;
+        }
+        __syncthreads();
+        ok = sm_chol_upper(S, G, red, R, flag);
+    }
+    if (!ok) { if (t == 0) ctl_breakdown(c); return; }
+    sm_triu_inv(S, R, Ri);
+    if (OP == FIN_QR1) {
+        sm_copy(S, R, mt->R1);
+        sm_copy(S, Ri, mt->R1i);
+        return;
+    }
+    // pass 2: full R of the QR = R2 R1
+    sm_copy(S, mt->R1, T1);
+    sm_mul(S, R, 0, T1, 0, T2);                  // T2 = beta = R2 R1
+    sm_copy(S, Ri, mt->R2i);
+    if (OP == FIN_QR2_INIT) {                     // R0 = V C  (blcg.c:58)
+        sm_copy(S, T2, mt->C);
+        return;
+    }
+    // C = beta C; error = ||C||_F / ||B||_F (blcg.c:98-105); next P uses beta^T
+    sm_copy(S, mt->C, T1);
+    sm_mul(S, T2, 0, T1, 0, R);                  // R <- beta C
+    sm_copy(S, R, mt->C);
+    if (t < S * S) mt->betaT[(t / S) + (t % S) * S] = T2[t];
+    const double f2 = sm_fro2(S, R, &red);
+    if (t == 0) ctl_check(c, f2);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_blk_resid(long long ms, const T *B, const T *AX, T *V,
+                                                   double *part) {
+    double v[1] = {0.0};
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < ms; e += (long long)gridDim.x * 256) {
+        V[e] = B[e] - AX[e];
+        v[0] += (double)B[e] * (double)B[e];
+    }
+    block_partial<1, 256>(v, part);
+}
+
+// Loads an s x s matrix (column-major) into LDS.
+__device__ __forceinline__ void load_small(double *dst, const double *src, int n) {
+    for (int t = threadIdx.x; t < n; t += blockDim.x) dst[t] = src[t];
+}
+
+// X += P W; V -= AP alpha; Gram V^T V of the new V.
+template <int S, typename T>
+__global__ __launch_bounds__(256) void k_blk_xv(long long m, T *X, const T *P, T *V, const T *AP,
+                                                const BlkMats *mt, const Ctl *c, double *part) {
+    if (ctl_stopped(c)) return;
+    __shared__ double Wm[SM_N], Am[SM_N], sh[4 * 256];
+    load_small(Wm, mt->W, S * S);
+    load_small(Am, mt->alpha, S * S);
+    __syncthreads();
+    const int w = threadIdx.x >> 6;
+    mf4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (long long row0 = ((long long)blockIdx.x * 4 + w) * 16; row0 < m; row0 += (long long)gridDim.x * 64) {
+        mf4 dx = tile_mul<S>(P, m, row0, Wm);
+        mf4 xt = tile_load<S>(X, m, row0);
+        tile_store<S>(X, m, row0, xt + dx);
+        mf4 dv = tile_mul<S>(AP, m, row0, Am);
+        mf4 vt = tile_load<S>(V, m, row0) - dv;
+        tile_store<S>(V, m, row0, vt);
+        if constexpr (sizeof(T) != sizeof(double)) vt = tile_load<S>(V, m, row0);   // stored values
+        acc = tile_gram(vt, vt, acc);
+    }
+    gram_block_partial<S>(acc, sh, part);
+}
+
+// V = V R1^-1; Gram V^T V.
+template <int S, typename T>
+__global__ __launch_bounds__(256) void k_blk_vr(long long m, T *V, const BlkMats *mt, const Ctl *c,
+                                                double *part) {
+    if (ctl_stopped(c)) return;
+    __shared__ double Rm[SM_N], sh[4 * 256];
+    load_small(Rm, mt->R1i, S * S);
+    __syncthreads();
+    const int w = threadIdx.x >> 6;
+    mf4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (long long row0 = ((long long)blockIdx.x * 4 + w) * 16; row0 < m; row0 += (long long)gridDim.x * 64) {
+        mf4 vt = tile_mul<S>(V, m, row0, Rm);
+        tile_store<S>(V, m, row0, vt);
+        if constexpr (sizeof(T) != sizeof(double)) vt = tile_load<S>(V, m, row0);
+        acc = tile_gram(vt, vt, acc);
+    }
+    gram_block_partial<S>(acc, sh, part);
+}
+
+// V = V R2^-1; P = V + P beta^T (or P = pscale V at the start).
+template <int S, typename T>
+__global__ __launch_bounds__(256) void k_blk_vp(long long m, T *V, T *P, const BlkMats *mt,
+                                                double pscale, const Ctl *c) {
+    if (ctl_stopped(c)) return;
+    __shared__ double Rm[SM_N], Bm[SM_N];
+    load_small(Rm, mt->R2i, S * S);
+    load_small(Bm, mt->betaT, S * S);
+    __syncthreads();
+    const int w = threadIdx.x >> 6;
+    for (long long row0 = ((long long)blockIdx.x * 4 + w) * 16; row0 < m; row0 += (long long)gridDim.x * 64) {
+        mf4 vt = tile_mul<S>(V, m, row0, Rm);
+        tile_store<S>(V, m, row0, vt);
+        if constexpr (sizeof(T) != sizeof(double)) vt = tile_load<S>(V, m, row0);
+        mf4 pt = (pscale != 0.0) ? vt * pscale : vt + tile_mul<S>(P, m, row0, Bm);
+        tile_store<S>(P, m, row0, pt);
+    }
+}
+
+template <int S, typename VT, typename T>
+static void launch_spmm(hipStream_t s, const bkh_matrix *A, const VT *val, const T *X, T *Y,
+                        double alpha, double beta, const Ctl *c) {
+    hipLaunchKernelGGL((k_spmm<S, VT, T>), dim3(A->nblk), dim3(SPMV_BS), 0, s, A->rp, A->ci, val, A->blk,
+                       X, Y, alpha, beta, c);
+}
+
+// Block CG on row-major device blocks: Bt (m x s), Xt (in/out).
+template <int S, typename VT, typename T>
+static int blcg_core(ThreadCtx *t, const bkh_matrix *A, const VT *val, const T *Bt, T *Xt, T *V,
+                     T *P, T *AP, BlkMats *mt, Ctl *c, double *part, double tol, int max_it,
+                     int p0_mode, int K) {
+    hipStream_t s = t->stream;
+    const long long m = A->n;
+    const int G = block_grid(m), Gv = vec_grid(m * S, 256);
+    if (ctl_init(t, c, tol, max_it)) return BKH_DEVICE_ERROR;
+    BlkMats h;
+    memset(&h, 0, sizeof h);
+    h.m = (int)m;
+    h.s = S;
+    HIP_OK(hipMemcpyAsync(mt, &h, sizeof h, hipMemcpyHostToDevice, s));
+    // R0 = B - A X0 (blcg.c:54-57), V, C = qr(R0) (:58), fnb = ||B||_F (:60)
+    launch_spmm<S>(s, A, val, (const T *)Xt, AP, 1.0, 0.0, c);
+    hipLaunchKernelGGL(k_blk_resid<T>, dim3(Gv), dim3(256), 0, s, m * S, Bt, (const T *)AP, V, part);
+    hipLaunchKernelGGL((k_blk_fin<S, FIN_FNB>), dim3(1), dim3(256), 0, s, part, Gv, mt, c);
+    hipLaunchKernelGGL((k_gram<S, T>), dim3(G), dim3(256), 0, s, m, (const T *)V, (const T *)V, c, part);
+    hipLaunchKernelGGL((k_blk_fin<S, FIN_QR1>), dim3(1), dim3(256), 0, s, part, G, mt, c);
+    hipLaunchKernelGGL((k_blk_vr<S, T>), dim3(G), dim3(256), 0, s, m, V, mt, c, part);
+    hipLaunchKernelGGL((k_blk_fin<S, FIN_QR2_INIT>), dim3(1), dim3(256), 0, s, part, G, mt, c);
+    // P = V (then P = V + P beta^T with beta = I: P0 = 2V, blcg.c:59-71)
+    hipLaunchKernelGGL((k_blk_vp<S, T>), dim3(G), dim3(256), 0, s, m, V, P, mt,
+                       p0_mode ? 1.0 : 2.0, c);
+    LAUNCH_OK();
+    hipGraphExec_t ge = capture_graph(s, K, [&]() {
+        launch_spmm<S>(s, A, val, (const T *)P, AP, 1.0, 0.0, c);
+        hipLaunchKernelGGL((k_gram<S, T>), dim3(G), dim3(256), 0, s, m, (const T *)P, (const T *)AP, c, part);
+        hipLaunchKernelGGL((k_blk_fin<S, FIN_ALPHA>), dim3(1), dim3(256), 0, s, part, G, mt, c);
+        hipLaunchKernelGGL((k_blk_xv<S, T>), dim3(G), dim3(256), 0, s, m, Xt, (const T *)P, V,
+                           (const T *)AP, mt, c, part);
+        hipLaunchKernelGGL((k_blk_fin<S, FIN_QR1>), dim3(1), dim3(256), 0, s, part, G, mt, c);
+        hipLaunchKernelGGL((k_blk_vr<S, T>), dim3(G), dim3(256), 0, s, m, V, mt, c, part);
+        hipLaunchKernelGGL((k_blk_fin<S, FIN_QR2_ITER>), dim3(1), dim3(256), 0, s, part, G, mt, c);
+        hipLaunchKernelGGL((k_blk_vp<S, T>), dim3(G), dim3(256), 0, s, m, V, P, mt, 0.0, c);
+    });
+    if (!ge) { bkh_set_error("hipGraph capture of the block CG iteration failed"); return BKH_DEVICE_ERROR; }
+    int rc = run_graph_until_done(t, ge, c, max_it / K + 2);
+    hipGraphExecDestroy(ge);
+    return rc;
+}
+
+// Frobenius true residual ||B - A X||_F / ||B||_F of row-major blocks.
+template <int S>
+static int block_true_res(ThreadCtx *t, const bkh_matrix *A, const double *Bt, const double *Xt,
+                          double *tmp, double *part, Ctl *c, double *out) {
+    hipStream_t s = t->stream;
+    const long long m = A->n;
+    const int Gv = vec_grid(m * S, 256);
+    if (ctl_init(t, c, 0.0, 1)) return BKH_DEVICE_ERROR;
+    launch_spmm<S>(s, A, A->val, Xt, tmp, 1.0, 0.0, (const Ctl *)nullptr);
+    hipLaunchKernelGGL(k_blk_resid<double>, dim3(Gv), dim3(256), 0, s, m * S, Bt, (const double *)tmp, tmp, part);
+    // ||R||^2 from a second pass (tmp now holds R)
+    hipLaunchKernelGGL(k_blk_resid<double>, dim3(Gv), dim3(256), 0, s, m * S, (const double *)tmp,
+                       (const double *)tmp, tmp + 0, part + Gv);
+    LAUNCH_OK();
+    std::vector<double> h((size_t)Gv * 2);
+    HIP_OK(hipMemcpyAsync(h.data(), part, sizeof(double) * Gv * 2, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    double bb = 0.0, rr = 0.0;
+    for (int i = 0; i < Gv; i++) { bb += h[i]; rr += h[Gv + i]; }
+    *out = sqrt(rr) / sqrt(bb);
+    return 0;
+}
+
+template <int S>
+static int blcg_dispatch(ThreadCtx *t, bkh_matrix *A, const double *B, double *X, double tol,
+                         int max_iter, const bkh_opts &o, bkh_info *inf, bool want_true) {
+    const long long m = A->n;
+    const int G = block_grid(m);
+    const int Gv = vec_grid(m * S, 256);
+    const int K = o.graph_iters > 0 ? o.graph_iters : std::max(2, default_graph_iters(m) / 4);
+    const int nparts = std::max(G * S * S, 2 * Gv);
+    size_t bytes = carve_size(sizeof(Ctl)) + carve_size(sizeof(BlkMats)) + carve_size(sizeof(double) * nparts) +
+                   5 * carve_size(sizeof(double) * m * S);
+    void *ws = bkh_workspace(bytes);
+    if (!ws) return BKH_DEVICE_ERROR;
+    Carve cv(ws);
+    Ctl *c = cv.take<Ctl>(1);
+    BlkMats *mt = cv.take<BlkMats>(1);
+    double *part = cv.take<double>(nparts);
+    double *Bt = cv.take<double>(m * S), *Xt = cv.take<double>(m * S), *V = cv.take<double>(m * S),
+           *P = cv.take<double>(m * S), *AP = cv.take<double>(m * S);
+    hipStream_t s = t->stream;
+    const int Gt = vec_grid(m * S, 256);
+    HIP_OK(hipEventRecord(t->ev0, s));
+    hipLaunchKernelGGL((k_cm2rm<double, double>), dim3(Gt), dim3(256), 0, s, m, S, B, Bt);
+    hipLaunchKernelGGL((k_cm2rm<double, double>), dim3(Gt), dim3(256), 0, s, m, S, (const double *)X, Xt);
+    LAUNCH_OK();
+    int rc = blcg_core<S, double, double>(t, A, A->val, Bt, Xt, V, P, AP, mt, c, part, tol, max_iter,
+                                          o.fix_blcg_p0, K);
+    if (rc) return rc;
+    Ctl hc = *t->ctl_host;
+    hipLaunchKernelGGL((k_rm2cm<double, double>), dim3(Gt), dim3(256), 0, s, m, S, (const double *)Xt, X);
+    LAUNCH_OK();
+    HIP_OK(hipEventRecord(t->ev1, s));
+    HIP_OK(hipEventSynchronize(t->ev1));
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, t->ev0, t->ev1));
+    inf->time_ms = ms;
+    bkh_note_solve_ms(ms);
+    inf->ret = ctl_result(&hc);
+    inf->status = hc.status;
+    inf->iters = hc.result_it;
+    inf->rec_res = hc.err;
+    if (want_true) {
+        double tr = 0.0;
+        if (block_true_res<S>(t, A, Bt, Xt, V, part, c, &tr) == 0) inf->true_res = tr;
+    }
+    return inf->ret;
+}
+
+extern "C" int bkh_blcg(bkh_matrix *A, int s, const double *B, double *X, double tol, int max_iter,
+                        const bkh_opts *opts, bkh_info *info) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (!A || !B || !X) return BKH_BAD_ARGUMENT;
+    if (s < 1 || s > SM_MAX) { bkh_set_error("block CG supports 1 <= s <= %d", SM_MAX); return -3; }
+    bkh_opts o = opts ? *opts : bkh_opts{};
+    bkh_info inf;
+    memset(&inf, 0, sizeof inf);
+    if (max_iter <= 0) {                 // blcg.c:31,115-118: error = 0 -> returns 0
+        inf.status = ST_CONVERGED;
+        if (info) *info = inf;
+        return 0;
+    }
+    int ret;
+    switch (s) {
+#define CASE(S_) case S_: ret = blcg_dispatch<S_>(t, A, B, X, tol, max_iter, o, &inf, info != nullptr); break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+        CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
+#undef CASE
+    default: return -3;
+    }
+    if (info) *info = inf;
+    return ret;
+}

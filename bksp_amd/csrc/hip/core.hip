@@ -153,18 +153,18 @@ extern "C" int bkh_memset(void *d, int v, size_t bytes) {
 
 // ------------------------------------------------------------ control block
 
-__global__ void k_ctl_init(Ctl *c, double tol, int max_it) {
+__global__ void k_ctl_init(Ctl *c, double tol, int max_it, int it0) {
     if (threadIdx.x == 0) {
-        c->done = max_it <= 0 ? 1 : 0;
-        c->status = max_it <= 0 ? ST_MAXIT : ST_RUNNING;
-        c->it = 0; c->result_it = 0; c->max_it = max_it; c->inner_total = 0;
+        c->done = max_it <= it0 ? 1 : 0;
+        c->status = max_it <= it0 ? ST_MAXIT : ST_RUNNING;
+        c->it = it0; c->result_it = it0; c->max_it = max_it; c->inner_total = 0;
         c->ticket = 0u; c->tol = tol; c->fnb = 0; c->err = 0;
     }
     for (int i = threadIdx.x; i < BKH_MAX_SCAL; i += blockDim.x) c->s[i] = 0.0;
 }
 
-int ctl_init(ThreadCtx *t, Ctl *ctl_dev, double tol, int max_it) {
-    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, t->stream, ctl_dev, tol, max_it);
+int ctl_init(ThreadCtx *t, Ctl *ctl_dev, double tol, int max_it, int it0) {
+    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, t->stream, ctl_dev, tol, max_it, it0);
     LAUNCH_OK();
     return 0;
 }
@@ -589,4 +589,25 @@ extern "C" int bkh_bench_copy(const void *a, void *b, size_t bytes, int reps, do
     HIP_OK(hipEventElapsedTime(&ms, t->ev0, t->ev1));
     *avg_ms = ms / reps;
     return 0;
+}
+
+int finish_solve(ThreadCtx *t, const bkh_matrix *A, double sigma, const double *b, const double *x,
+                 bkh_info *inf, bkh_info *info) {
+    HIP_OK(hipEventRecord(t->ev1, t->stream));
+    HIP_OK(hipEventSynchronize(t->ev1));
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, t->ev0, t->ev1));
+    inf->time_ms = ms;
+    bkh_note_solve_ms(ms);
+    const Ctl *c = t->ctl_host;
+    inf->ret = ctl_result(c);
+    inf->status = c->status;
+    inf->iters = c->result_it;
+    inf->rec_res = c->err;
+    if (info) {
+        double tr = 0.0;
+        if (true_residual(t, A, sigma, b, x, &tr) == 0) inf->true_res = tr;
+        *info = *inf;
+    }
+    return inf->ret;
 }

@@ -87,6 +87,37 @@ int bksp_dcg(dspmat *A, double *b, double tol, int max_iter, double *x) {
     return rc;
 }
 
+/* reference src/cr.c:6-80 */
+int dcr(dspmat *A, double *b, double tol, int max_iter, double *x) {
+    if (strcmp(A->type, "symmetric") != 0) return INVALID_MATRIX_TYPE;
+    if (max_iter <= 1) return 1;                     /* cr.c:37, 76-79 */
+    bkh_opts o = default_opts();
+    int n = *A->row_size;
+    upload_t u;
+    int rc = upload_system_sym(A, b, n, x, n, 0, &u);
+    if (rc == 0) {
+        rc = bkh_cr(u.M, u.d_b, u.d_x, tol, max_iter, &o, NULL);
+        if (rc > BKH_DEVICE_ERROR && download_x(&u, x)) rc = BKH_DEVICE_ERROR;
+    }
+    release(&u);
+    return rc;
+}
+
+/* reference src/bicg.c:7-88 */
+int dbicg(dspmat *A, double *b, double tol, int max_iter, double *x) {
+    if (max_iter <= 0) return 0;                     /* bicg.c:11, 84-87 */
+    bkh_opts o = default_opts();
+    int n = *A->row_size;
+    upload_t u;
+    int rc = upload_system(A, b, n, x, n, 1, 0, &u);
+    if (rc == 0) {
+        rc = bkh_bicg(u.M, u.d_b, u.d_x, tol, max_iter, &o, NULL);
+        if (rc > BKH_DEVICE_ERROR && download_x(&u, x)) rc = BKH_DEVICE_ERROR;
+    }
+    release(&u);
+    return rc;
+}
+
 /* reference src/bicgstabl.c:13-138. Like the reference, the nonsymmetric solvers
  * use the stored entries as a general CSR whatever A->type says. */
 int dbicgstabl(dspmat *A, double *b, int l, double tol, int max_iter, double *x) {
@@ -98,6 +129,24 @@ int dbicgstabl(dspmat *A, double *b, int l, double tol, int max_iter, double *x)
     if (rc == 0) {
         rc = bkh_bicgstabl(u.M, u.d_b, l, u.d_x, tol, max_iter, &o, NULL);
         if (rc > BKH_DEVICE_ERROR && download_x(&u, x)) rc = BKH_DEVICE_ERROR;
+    }
+    release(&u);
+    return rc;
+}
+
+/* reference src/blcg.c:15-119 */
+int dblcg(dspmat *A, dmat *B, double tol, int max_iter, dmat *X) {
+    if (strcmp(A->type, "symmetric") != 0) return INVALID_MATRIX_TYPE;
+    if (max_iter <= 0) return 0;                     /* blcg.c:31, 115-118 */
+    int m = *A->row_size, s = B->col_size;
+    if (s < 1 || s > 16) return INVALID_PARAMETERS;
+    bkh_opts o = default_opts();
+    upload_t u;
+    size_t ms = (size_t)m * s;
+    int rc = upload_system_sym(A, B->value, ms, X->value, ms, 0, &u);
+    if (rc == 0) {
+        rc = bkh_blcg(u.M, s, u.d_b, u.d_x, tol, max_iter, &o, NULL);
+        if (rc > BKH_DEVICE_ERROR && download_x(&u, X->value)) rc = BKH_DEVICE_ERROR;
     }
     release(&u);
     return rc;

@@ -131,12 +131,38 @@ def cg(M, d_b, d_x, tol, max_iter, **kw):
     return info
 
 
+def _solve1(fn, name, M, d_b, d_x, tol, max_iter, **kw):
+    opts = make_opts(**kw)
+    info = BkhInfo()
+    rc = fn(M.handle, d_b.ptr, d_x.ptr, tol, max_iter, ctypes.byref(opts), ctypes.byref(info))
+    _check(rc, name)
+    return info
+
+
+def cr(M, d_b, d_x, tol, max_iter, **kw):
+    return _solve1(load().bkh_cr, "bkh_cr", M, d_b, d_x, tol, max_iter, **kw)
+
+
+def bicg(M, d_b, d_x, tol, max_iter, **kw):
+    return _solve1(load().bkh_bicg, "bkh_bicg", M, d_b, d_x, tol, max_iter, **kw)
+
+
 def bicgstabl(M, d_b, l, d_x, tol, max_iter, **kw):
     opts = make_opts(**kw)
     info = BkhInfo()
     rc = load().bkh_bicgstabl(M.handle, d_b.ptr, l, d_x.ptr, tol, max_iter, ctypes.byref(opts),
                               ctypes.byref(info))
     _check(rc, "bkh_bicgstabl")
+    return info
+
+
+def blcg(M, s, d_B, d_X, tol, max_iter, **kw):
+    """Block CG; d_B, d_X hold m x s column-major blocks (Fortran order)."""
+    opts = make_opts(**kw)
+    info = BkhInfo()
+    rc = load().bkh_blcg(M.handle, s, d_B.ptr, d_X.ptr, tol, max_iter, ctypes.byref(opts),
+                         ctypes.byref(info))
+    _check(rc, "bkh_blcg")
     return info
 
 

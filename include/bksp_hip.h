@@ -106,9 +106,21 @@ int bkh_gram(int m, int s, const double *d_X, const double *d_Y, double *G);
  * The return value is what the matching bksp.h entry point returns. */
 int bkh_cg(bkh_matrix *A, const double *d_b, double *d_x, double tol, int max_iter,
            const bkh_opts *opts, bkh_info *info);
+/* Conjugate residual (replaces src/cr.c:6-80); the loop index starts at 1. */
+int bkh_cr(bkh_matrix *A, const double *d_b, double *d_x, double tol, int max_iter,
+           const bkh_opts *opts, bkh_info *info);
+/* BiCG (replaces src/bicg.c:7-88); A must be created with want_transpose = 1. */
+int bkh_bicg(bkh_matrix *A, const double *d_b, double *d_x, double tol, int max_iter,
+             const bkh_opts *opts, bkh_info *info);
 /* BiCGSTAB(l), 1 <= l <= 8 (replaces src/bicgstabl.c:13-138). */
 int bkh_bicgstabl(bkh_matrix *A, const double *d_b, int l, double *d_x, double tol,
                   int max_iter, const bkh_opts *opts, bkh_info *info);
+
+/* Block CG with residual QR (replaces src/blcg.c:15-119). B, X: m x s column-major
+ * device blocks (dmat layout), 1 <= s <= 16. info->true_res is the Frobenius
+ * ||B - A X||_F / ||B||_F. */
+int bkh_blcg(bkh_matrix *A, int s, const double *d_B, double *d_X, double tol, int max_iter,
+             const bkh_opts *opts, bkh_info *info);
 
 /* Timing of the most recent solve on this thread (ms, HIP events). */
 double bkh_last_solve_ms(void);

@@ -1,0 +1,101 @@
+"""GPU block CG (dblcg / bkh_blcg) against the oracle's restatement of src/blcg.c.
+
+The device QR is CholeskyQR2 (positive-diagonal R) where the reference uses
+Householder; X is invariant under the sign choice, so trajectories agree to
+rounding. Both sides use the calloc reading of blcg.c's first direction (P0 = 2V)."""
+import numpy as np
+import pytest
+
+from bksp_amd import bksp
+from bksp_amd import device as dv
+from bksp_amd import gen
+from conftest import expand_lower
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def close_index(a, b):
+    return abs(a - b) <= max(1, int(0.05 * max(a, b)))
+
+
+def gpu_blcg(n, rp, ci, v, B, tol, max_iter, fix_p0=0):
+    M = dv.Matrix(n, rp, ci, v, symmetric=True)
+    s = B.shape[1]
+    dB = dv.DeviceArray.from_numpy(np.ascontiguousarray(B.T))      # column-major m x s
+    dX = dv.DeviceArray.from_numpy(np.zeros((s, n)))
+    info = dv.blcg(M, s, dB, dX, tol, max_iter, fix_blcg_p0=fix_p0)
+    return info, dX.numpy().T
+
+
+def test_reference_scenario_16_unit_rhs(bfwb398):
+    """tests/test.c:132-153: bfwb398, B = [e1..e16], tol 1e-14, max_iter 500."""
+    n, rp, ci, v, _ = bfwb398
+    B = np.zeros((n, 16)); B[np.arange(16), np.arange(16)] = 1.0
+    A = bksp.dspmat(n, rp, ci, v, symmetric=True)
+    Bm, Xm = bksp.Mat(B), bksp.Mat(np.zeros((n, 16)))
+    ret = bksp.dblcg(A, Bm, 1e-14, 500, Xm)
+    cret, cX = po.blcg(n, rp, ci, v, B, 1e-14, 500)
+    Af = expand_lower(n, rp, ci, v)
+    res = np.linalg.norm(B - Af @ Xm.a) / np.linalg.norm(B)
+    assert res < 1e-9, (ret, cret, res)
+    if ret >= 0 and cret >= 0:
+        assert abs(ret - cret) <= max(2, int(0.1 * cret)), (ret, cret)
+
+
+@pytest.mark.parametrize("fix_p0", [0, 1])
+def test_first_steps_match_oracle(bfwb398, fix_p0):
+    n, rp, ci, v, _ = bfwb398
+    B = gen.normal(31, n * 4).reshape(4, n).T.copy()
+    for k in (1, 2, 5):
+        _, X = gpu_blcg(n, rp, ci, v, B, 1e-300, k, fix_p0)
+        _, cX = po.blcg(n, rp, ci, v, B, 1e-300, k, p0_mode=fix_p0)
+        assert rel(X, cX) < 1e-9, (k, rel(X, cX))
+    # one step with P0 = 2V is half the block Galerkin step (closed form)
+    A = expand_lower(n, rp, ci, v).toarray()
+    V = np.linalg.qr(B)[0]
+    full = V @ np.linalg.solve(V.T @ A @ V, V.T @ B)
+    _, X1 = gpu_blcg(n, rp, ci, v, B, 1e-300, 1, fix_p0)
+    assert rel(X1, full * (1.0 if fix_p0 else 0.5)) < 1e-9
+
+
+@pytest.mark.parametrize("m,s", [(24, 8), (40, 8), (20, 3), (20, 16)])
+def test_poisson_parity(m, s):
+    n, rp, ci, v = gen.poisson3d(m, lower=True)
+    B = gen.normal(202, n * s).reshape(s, n).T.copy()
+    info, X = gpu_blcg(n, rp, ci, v, B, 1e-8, 3000)
+    cret, cX = po.blcg(n, rp, ci, v, B, 1e-8, 3000)
+    assert info.ret >= 0 and cret >= 0
+    assert close_index(info.ret, cret), (info.ret, cret)
+    assert info.true_res < 2e-8
+    if info.ret == cret:
+        assert rel(X, cX) < 1e-8
+
+
+def test_cfg2_full_size_blcg_s8():
+    """cfg2: 128^3 Poisson, s = 8 RHS N(0,1) seed 202, tol 1e-8, reference P0 = 2V and
+    the corrected P0 = V (reported separately, never mixed into the parity claim)."""
+    n, rp, ci, v = gen.poisson3d(128, lower=True)
+    B = gen.normal(202, n * 8).reshape(8, n).T.copy()
+    info, X = gpu_blcg(n, rp, ci, v, B, 1e-8, 5000)
+    assert info.ret >= 0 and info.true_res < 2e-8
+    info1, _ = gpu_blcg(n, rp, ci, v, B, 1e-8, 5000, fix_p0=1)
+    assert info1.ret >= 0 and info1.true_res < 2e-8
+    assert info1.ret < info.ret
+    print(f"cfg2 block CG s=8: P0=2V {info.ret} iters {info.time_ms:.1f} ms; "
+          f"P0=V {info1.ret} iters {info1.time_ms:.1f} ms")
+
+
+def test_codes():
+    n, rp, ci, v = gen.poisson3d(6, lower=True)
+    B = np.ones((n, 2))
+    A = bksp.dspmat(n, rp, ci, v, symmetric=True)
+    assert bksp.dblcg(bksp.dspmat(n, rp, ci, v), bksp.Mat(B), 1e-8, 10, bksp.Mat(np.zeros((n, 2)))) \
+        == bksp.INVALID_MATRIX_TYPE
+    assert bksp.dblcg(A, bksp.Mat(B), 1e-8, 0, bksp.Mat(np.zeros((n, 2)))) == 0
+    assert bksp.dblcg(A, bksp.Mat(np.ones((n, 17))), 1e-8, 5, bksp.Mat(np.zeros((n, 17)))) \
+        == bksp.INVALID_PARAMETERS
