@@ -81,12 +81,15 @@ _SIGS = {
                               ctypes.POINTER(BkhInfo)]),
     "bkh_blcg": (c_int, [c_vp, c_int, c_vp, c_vp, c_dbl, c_int, ctypes.POINTER(BkhOpts),
                          ctypes.POINTER(BkhInfo)]),
+    "bkh_shbicgstabl": (c_int, [c_vp, c_vp, c_int, c_dp, c_int, c_vp, c_dbl, c_int,
+                                ctypes.POINTER(BkhOpts), ctypes.POINTER(BkhInfo), c_dp]),
     # bksp.h drop-in (host arrays)
     "bksp_dcg": (c_int, [ctypes.POINTER(DSpMat), c_dp, c_dbl, c_int, c_dp]),
     "dcr": (c_int, [ctypes.POINTER(DSpMat), c_dp, c_dbl, c_int, c_dp]),
     "dbicg": (c_int, [ctypes.POINTER(DSpMat), c_dp, c_dbl, c_int, c_dp]),
     "dbicgstabl": (c_int, [ctypes.POINTER(DSpMat), c_dp, c_int, c_dbl, c_int, c_dp]),
     "dblcg": (c_int, [ctypes.POINTER(DSpMat), ctypes.POINTER(DMat), c_dbl, c_int, ctypes.POINTER(DMat)]),
+    "dshbicgstabl": (c_int, [ctypes.POINTER(DSpMat), c_dp, c_int, c_dp, c_int, c_dbl, c_int, c_dp]),
     "dmat_create": (ctypes.POINTER(DMat), [c_int, c_int]),
     "dmat_free": (c_int, [ctypes.POINTER(DMat)]),
     "darray2dmat": (c_int, [c_dp, c_int, c_int, ctypes.POINTER(DMat)]),

@@ -90,3 +90,10 @@ def dbicgstabl(A, b, l, tol, max_iter, x):
 def dblcg(A, B, tol, max_iter, X):
     """Block CG (reference src/blcg.c:15-119). B, X are ``Mat`` (X overwritten)."""
     return load().dblcg(A.ref, B.ref, tol, max_iter, X.ref)
+
+
+def dshbicgstabl(A, b, l, sigma, tol, max_iter, x):
+    """Shifted BiCGSTAB(l) (reference src/shbicgstabl.c:7-285). x has n*(len(sigma)+1)
+    entries: seed first, then shift k at x[(k+1)*n:]."""
+    sig = np.ascontiguousarray(sigma, dtype=np.float64)
+    return load().dshbicgstabl(A.ref, _dp(b), l, _dp(sig), len(sig), tol, max_iter, _dp(x))

@@ -16,7 +16,7 @@
 //
 // MFMA f64 16x16x4 lane layout (checked on the device): A(16x4): lane L holds
 // A[L%16][L/16]; B(4x16): lane L holds B[L/16][L%16]; D(16x16): lane L register r
-// holds D[4*(L/16)+r][L%16].
+// holds D[L/16 + 4r][L%16].
 #pragma once
 #include "bkh_core.hpp"
 #include "bkh_host.hpp"
@@ -117,14 +117,14 @@ __device__ __forceinline__ mf4 tile_mul(const T *X, long long m, long long row0,
     return d;
 }
 
-// Loads/stores the tile elements held in the D layout (rows 4*(L/16)+r, col L%16).
+// Loads/stores the tile elements held in the D layout (row L/16 + 4r, col L%16).
 template <int S, typename T>
 __device__ __forceinline__ mf4 tile_load(const T *X, long long m, long long row0) {
     const int L = threadIdx.x & 63, j = L & 15;
     mf4 v;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-        const long long row = row0 + 4 * (L >> 4) + r;
+        const long long row = row0 + (L >> 4) + 4 * r;
         v[r] = (row < m && j < S) ? (double)X[row * S + j] : 0.0;
     }
     return v;
@@ -135,14 +135,14 @@ __device__ __forceinline__ void tile_store(T *X, long long m, long long row0, mf
     const int L = threadIdx.x & 63, j = L & 15;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-        const long long row = row0 + 4 * (L >> 4) + r;
+        const long long row = row0 + (L >> 4) + 4 * r;
         if (row < m && j < S) X[row * S + j] = (T)v[r];
     }
 }
 
 // Gram accumulation of a tile held in the D layout: register r of every lane is
-// row 4*(L/16)+r, column L%16, which is exactly the A operand (A[i][kk] with
-// kk <-> row 4kk + r) and the B operand of an MFMA with K slot kk.
+// row L/16 + 4r, column L%16, which is exactly the A operand (A[i][kk] with
+// kk <-> row kk + 4r) and the B operand of an MFMA with K slot kk.
 __device__ __forceinline__ mf4 tile_gram(mf4 v, mf4 w, mf4 acc) {
 #pragma unroll
     for (int r = 0; r < 4; r++) acc = mfma16x16x4(v[r], w[r], acc);
@@ -155,7 +155,7 @@ template <int S>
 __device__ __forceinline__ void gram_block_partial(mf4 acc, double *sh, double *part) {
     const int L = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
-    for (int r = 0; r < 4; r++) sh[w * 256 + (4 * (L >> 4) + r) * 16 + (L & 15)] = acc[r];
+    for (int r = 0; r < 4; r++) sh[w * 256 + ((L >> 4) + 4 * r) * 16 + (L & 15)] = acc[r];
     __syncthreads();
     const int t = threadIdx.x;
     if (t < S * S) {

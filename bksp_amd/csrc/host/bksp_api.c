@@ -134,6 +134,23 @@ int dbicgstabl(dspmat *A, double *b, int l, double tol, int max_iter, double *x)
     return rc;
 }
 
+/* reference src/shbicgstabl.c:7-285 */
+int dshbicgstabl(dspmat *A, double *b, int l, double *sigma, int sigma_size, double tol,
+                 int max_iter, double *x) {
+    if (max_iter <= 0) return 0;                     /* shbicgstabl.c:12, 280-284 */
+    bkh_opts o = default_opts();
+    int n = *A->row_size;
+    upload_t u;
+    size_t nx = (size_t)n * (sigma_size + 1);
+    int rc = upload_system(A, b, n, x, nx, 0, 0, &u);
+    if (rc == 0) {
+        rc = bkh_shbicgstabl(u.M, u.d_b, l, sigma, sigma_size, u.d_x, tol, max_iter, &o, NULL, NULL);
+        if (rc > BKH_DEVICE_ERROR && download_x(&u, x)) rc = BKH_DEVICE_ERROR;
+    }
+    release(&u);
+    return rc;
+}
+
 /* reference src/blcg.c:15-119 */
 int dblcg(dspmat *A, dmat *B, double tol, int max_iter, dmat *X) {
     if (strcmp(A->type, "symmetric") != 0) return INVALID_MATRIX_TYPE;

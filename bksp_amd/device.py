@@ -166,6 +166,20 @@ def blcg(M, s, d_B, d_X, tol, max_iter, **kw):
     return info
 
 
+def shbicgstabl(M, d_b, l, sigma, d_x, tol, max_iter, **kw):
+    """Shifted BiCGSTAB(l); d_x holds n*(len(sigma)+1) entries (seed first).
+    Returns (info, per-system FP64 true residuals)."""
+    opts = make_opts(**kw)
+    info = BkhInfo()
+    sig = np.ascontiguousarray(sigma, dtype=np.float64)
+    tr = np.zeros(len(sig) + 1)
+    rc = load().bkh_shbicgstabl(M.handle, d_b.ptr, l, sig.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                len(sig), d_x.ptr, tol, max_iter, ctypes.byref(opts), ctypes.byref(info),
+                                tr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    _check(rc, "bkh_shbicgstabl")
+    return info, tr
+
+
 def dot(d_x, d_y):
     out = ctypes.c_double()
     rc = load().bkh_dot(int(np.prod(d_x.shape)), d_x.ptr, d_y.ptr, ctypes.byref(out))

@@ -122,6 +122,17 @@ int bkh_bicgstabl(bkh_matrix *A, const double *d_b, int l, double *d_x, double t
 int bkh_blcg(bkh_matrix *A, int s, const double *d_B, double *d_X, double tol, int max_iter,
              const bkh_opts *opts, bkh_info *info);
 
+/* Shifted BiCGSTAB(l) (replaces src/shbicgstabl.c:7-285): solves A x_0 = b and
+ * (A + sigma_k I) x_k = b for ns <= 256 shifts with one Krylov space. d_x holds
+ * n*(ns+1) entries: the seed solution, then shift k at d_x + (k+1)*n; the shifted
+ * parts should enter as 0. Convergence is tested on the seed only, as in the
+ * reference. opts->precision = BKH_FP32_IR stores the per-shift direction vectors
+ * in FP32 (seed, x and all recurrences stay FP64). shift_true_res (host, ns+1
+ * entries, may be NULL) receives every system's FP64 true residual. */
+int bkh_shbicgstabl(bkh_matrix *A, const double *d_b, int l, const double *sigma, int ns,
+                    double *d_x, double tol, int max_iter, const bkh_opts *opts,
+                    bkh_info *info, double *shift_true_res);
+
 /* Timing of the most recent solve on this thread (ms, HIP events). */
 double bkh_last_solve_ms(void);
 /* Average ms per launch of `reps` back-to-back SpMVs (FP64, or FP32 if fp32 = 1),
