@@ -33,7 +33,7 @@ class DMat(ctypes.Structure):
 
 class BkhOpts(ctypes.Structure):
     _fields_ = [("precision", c_int), ("graph_iters", c_int), ("fix_blcg_p0", c_int),
-                ("max_inner", c_int), ("inner_tol", c_dbl)]
+                ("max_inner", c_int), ("inner_tol", c_dbl), ("path", c_int)]
 
 
 class BkhInfo(ctypes.Structure):
@@ -67,6 +67,8 @@ _SIGS = {
     "bkh_spmv": (c_int, [c_vp, ctypes.c_char, c_dbl, c_vp, c_dbl, c_vp]),
     "bkh_spmv32": (c_int, [c_vp, c_vp, c_vp]),
     "bkh_dot": (c_int, [c_ll, c_vp, c_vp, c_dp]),
+    "bkh_qr": (c_int, [c_int, c_int, c_vp, c_dp]),
+    "bkh_sp_norm_f": (c_int, [c_int, c_vp, c_vp, c_vp, c_int, c_dp]),
     "bkh_last_solve_ms": (c_dbl, []),
     "bkh_bench_spmv": (c_int, [c_vp, c_int, c_vp, c_vp, c_int, c_dp]),
     "bkh_bench_copy": (c_int, [c_vp, c_vp, ctypes.c_size_t, c_int, c_dp]),
@@ -90,6 +92,15 @@ _SIGS = {
     "dbicgstabl": (c_int, [ctypes.POINTER(DSpMat), c_dp, c_int, c_dbl, c_int, c_dp]),
     "dblcg": (c_int, [ctypes.POINTER(DSpMat), ctypes.POINTER(DMat), c_dbl, c_int, ctypes.POINTER(DMat)]),
     "dshbicgstabl": (c_int, [ctypes.POINTER(DSpMat), c_dp, c_int, c_dp, c_int, c_dbl, c_int, c_dp]),
+    "read_matrix_market": (ctypes.POINTER(DSpMat), [ctypes.c_char_p]),
+    "myqr": (c_int, [c_int, c_int, c_dp, c_dp]),
+    "sp_norm_f": (c_dbl, [ctypes.POINTER(DSpMat)]),
+    "dcsrgemm0": (c_int, [ctypes.c_char, c_int, c_int, c_dp, c_ip, c_ip, c_int, c_dp, c_dp]),
+    "dcsrsymm0": (c_int, [ctypes.c_char, c_int, c_int, c_dp, c_ip, c_ip, c_int, c_dp, c_dp]),
+    "dcsrgemm1": (c_int, [ctypes.c_char, c_dbl, c_int, c_int, c_dp, c_ip, c_ip, c_int, c_dp, c_dbl, c_dp]),
+    "dcsrsymm1": (c_int, [ctypes.c_char, c_dbl, c_int, c_int, c_dp, c_ip, c_ip, c_int, c_dp, c_dbl, c_dp]),
+    "dhorner": (c_int, [c_dp, c_int, c_dbl, c_dp]),
+    "comb": (c_int, [c_int, c_int]),
     "dmat_create": (ctypes.POINTER(DMat), [c_int, c_int]),
     "dmat_free": (c_int, [ctypes.POINTER(DMat)]),
     "darray2dmat": (c_int, [c_dp, c_int, c_int, ctypes.POINTER(DMat)]),

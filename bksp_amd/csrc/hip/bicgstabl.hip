@@ -219,6 +219,9 @@ static int bl_core(ThreadCtx *t, const bkh_matrix *A, const VT *val, int l, cons
     return rc;
 }
 
+bool bicgstabl_small(ThreadCtx *t, const bkh_matrix *A, int l, const double *b, double *x, double tol,
+                     int max_iter, int *rc);
+
 static int bl_parts(const bkh_matrix *A, int l) {
     int nv = (l + 1) * (l + 2) / 2;
     return std::max(spmv_grid(A), vec_grid(A->n, 256)) * std::max(nv, 2);
@@ -272,6 +275,11 @@ extern "C" int bkh_bicgstabl(bkh_matrix *A, const double *b, int l, double *x, d
         inf.iters = ir.ret;
         inf.rec_res = ir.res;
     } else {
+        int src = 0;
+        if (o.path != 1 && (o.path == 2 || n <= 16384) && bicgstabl_small(t, A, l, b, x, tol, max_iter, &src)) {
+            if (src) return src;
+            return finish_solve(t, A, 0.0, b, x, &inf, info);
+        }
         size_t bytes = carve_size(sizeof(Ctl)) + carve_size(sizeof(double) * nparts) +
                        (2 * (l + 1) + 1) * carve_size(sizeof(double) * n);
         void *ws = bkh_workspace(bytes);

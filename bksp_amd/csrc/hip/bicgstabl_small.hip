@@ -1,0 +1,210 @@
+// bicgstabl_small.hip -- BiCGSTAB(l) for small systems in ONE persistent workgroup.
+//
+// At n of a few thousand (the 64x64 convection-diffusion case) an outer iteration
+// of the multi-kernel path is ~14 dependent launches whose latency dwarfs the
+// arithmetic. Here a single 1024-thread workgroup runs the whole solve with
+// workgroup barriers between phases: the same steps as bicgstabl.hip
+// (src/bicgstabl.c:54-126), the SpMV as a sequential row sum per thread (the
+// reference's order, src/matvec.c:22-25), dot products as workgroup tree
+// reductions, the MR step from the fused Gram matrix (bkh_mr.hpp) in thread 0.
+// Vectors live in global memory (L2-resident at these sizes). The loop is bounded
+// by max_iter, so the kernel always terminates.
+#include <math.h>
+#include "bkh_host.hpp"
+#include "bkh_mr.hpp"
+#include "bkh_solver_common.hpp"
+
+#define SMALL_BS 1024
+#define SMALL_MAX_N 16384
+#define SMALL_MAX_NNZ 400000
+
+template <int NV>
+__device__ __forceinline__ void wg_sum(double (&v)[NV], double *sh) {
+    // sh: NV * 16 doubles; result broadcast to every thread
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; i++) v[i] = wave_sum(v[i]);
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; i++) sh[i * 16 + w] = v[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+        double t = sh[i * 16];
+        for (int k = 1; k < 16; k++) t += sh[i * 16 + k];
+        v[i] = t;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void wg_spmv(int n, const int *rp, const int *ci, const double *val,
+                                        const double *x, double *y) {
+    for (int i = threadIdx.x; i < n; i += SMALL_BS) {
+        double s = 0.0;
+        for (int k = rp[i]; k < rp[i + 1]; k++) s = add_rn(s, mul_rn(val[k], x[ci[k]]));
+        y[i] = s;
+    }
+}
+
+template <int L>
+__global__ __launch_bounds__(SMALL_BS) void k_bicgstabl_small(int n, const int *rp, const int *ci,
+                                                              const double *val, const double *b,
+                                                              double *x, double *R, double *P,
+                                                              double *rs, Ctl *c) {
+    constexpr int NG = (L + 1) * (L + 2) / 2;
+    __shared__ double sh[NG * 16 + 32];
+    __shared__ double coef[3 * MR_D + 4];
+    __shared__ int flags[2];   // [0]: stop, [1]: status
+    const int tid = threadIdx.x;
+    int max_it = c->max_it;
+    const double tol = c->tol;
+    // r0 = b - A x0; p0 = rs = r0 (bicgstabl.c:44-51)
+    wg_spmv(n, rp, ci, val, x, R);
+    __syncthreads();
+    double v2[2] = {0.0, 0.0};
+    for (int i = tid; i < n; i += SMALL_BS) {
+        double ri = b[i] - R[i];
+        R[i] = ri; P[i] = ri; rs[i] = ri;
+        v2[0] += ri * ri;
+        v2[1] += b[i] * b[i];
+    }
+    wg_sum<2>(v2, sh);
+    const double fnb = sqrt(v2[1]);
+    double alpha = 0.0, beta = 0.0, g0 = v2[0], oml = 1.0, err = 0.0;
+    int status = ST_MAXIT, it = 0, result = 0;
+    if (!(fnb > 0.0)) { status = ST_BREAKDOWN; max_it = 0; }
+    else if (v2[0] == 0.0) { status = ST_CONVERGED; result = 0; }
+    for (it = 0; it < max_it && status != ST_CONVERGED; it++) {
+        bool stop = false;
+        for (int j = 0; j < L && !stop; j++) {
+            for (int i = tid; i < n; i += SMALL_BS)                // (:64-67)
+                for (int q = 0; q <= j; q++) P[q * n + i] = add_rn(mul_rn(beta, P[q * n + i]), R[q * n + i]);
+            __syncthreads();
+            wg_spmv(n, rp, ci, val, P + j * n, P + (j + 1) * n);   // (:68-69)
+            __syncthreads();
+            double d1[1] = {0.0};
+            for (int i = tid; i < n; i += SMALL_BS) d1[0] += rs[i] * P[(j + 1) * n + i];
+            wg_sum<1>(d1, sh);
+            if (bad(d1[0])) { status = ST_BREAKDOWN; stop = true; break; }
+            alpha = g0 / d1[0];                                    // (:71)
+            for (int i = tid; i < n; i += SMALL_BS) {              // (:73-75, :79)
+                for (int q = 0; q <= j; q++) R[q * n + i] = fma(-alpha, P[(q + 1) * n + i], R[q * n + i]);
+                x[i] = fma(alpha, P[i], x[i]);
+            }
+            __syncthreads();
+            wg_spmv(n, rp, ci, val, R + j * n, R + (j + 1) * n);   // (:76-77)
+            __syncthreads();
+            if (j < L - 1) {
+                double d2[1] = {0.0};
+                for (int i = tid; i < n; i += SMALL_BS) d2[0] += rs[i] * R[(j + 1) * n + i];
+                wg_sum<1>(d2, sh);
+                if (bad(g0)) { status = ST_BREAKDOWN; stop = true; break; }
+                beta = -alpha * d2[0] / g0;                        // (:60-62)
+                g0 = d2[0];
+            }
+        }
+        if (stop) break;
+        // MR step from the Gram matrix of r_0..r_L (:84-109)
+        double gv[NG];
+#pragma unroll
+        for (int q = 0; q < NG; q++) gv[q] = 0.0;
+        for (int i = tid; i < n; i += SMALL_BS) {
+            double rv[L + 1];
+#pragma unroll
+            for (int q = 0; q <= L; q++) rv[q] = R[q * n + i];
+            int idx = 0;
+#pragma unroll
+            for (int a = 0; a <= L; a++)
+#pragma unroll
+                for (int bb = a; bb <= L; bb++) gv[idx++] += rv[a] * rv[bb];
+        }
+        wg_sum<NG>(gv, sh);
+        if (tid == 0) {
+            double G[MR_D][MR_D];
+            gram_unpack(L, gv, G);
+            MrOut o;
+            flags[0] = 0;
+            if (G[0][0] == 0.0) {
+                flags[0] = 1; flags[1] = ST_CONVERGED;
+            } else if (!mr_from_gram(L, G, o)) {
+                flags[0] = 1; flags[1] = ST_BREAKDOWN;
+            } else {
+                coef[0] = o.om[1];
+                for (int k = 1; k < L; k++) {
+                    double t = 0.0;
+                    for (int i = k; i < L; i++) t += o.odd[i] * o.C[i][k];
+                    coef[k] = t;
+                }
+                for (int k = 1; k <= L; k++) {
+                    double t = 0.0;
+                    for (int i = k; i <= L; i++) t += o.od[i] * o.C[i][k];
+                    coef[MR_D + k] = t;
+                    coef[2 * MR_D + k] = o.om[k];
+                }
+                coef[3 * MR_D] = o.om[L];
+            }
+        }
+        __syncthreads();
+        if (flags[0]) {
+            status = flags[1];
+            result = it;
+            break;
+        }
+        double v3[2] = {0.0, 0.0};
+        for (int i = tid; i < n; i += SMALL_BS) {                  // (:113-120)
+            double xi = x[i], r0 = R[i], p0 = P[i];
+            xi += coef[0] * r0;
+            for (int k = 1; k <= L; k++) {
+                const double rk = R[k * n + i];
+                if (k < L) xi += coef[k] * rk;
+                r0 -= coef[MR_D + k] * rk;
+                p0 -= coef[2 * MR_D + k] * P[k * n + i];
+            }
+            x[i] = xi; R[i] = r0; P[i] = p0;
+            v3[0] += r0 * r0;
+            v3[1] += rs[i] * r0;
+        }
+        wg_sum<2>(v3, sh);
+        oml = coef[3 * MR_D];
+        err = sqrt(v3[0]) / fnb;                                   // (:122-125)
+        if (err < tol) { status = ST_CONVERGED; result = it; break; }
+        if (!(err == err) || err > 1e300) { status = ST_BREAKDOWN; result = it; break; }
+        const double gg = -oml * g0;                               // (:57-62)
+        if (bad(gg)) { status = ST_BREAKDOWN; result = it; break; }
+        beta = -alpha * v3[1] / gg;
+        g0 = v3[1];
+    }
+    if (tid == 0) {
+        c->fnb = fnb;
+        c->err = err;
+        c->status = status;
+        c->result_it = (status == ST_MAXIT) ? (it > 0 ? it - 1 : 0) : result;
+        c->it = it;
+        c->done = 1;
+    }
+}
+
+// Returns true (and the solve's result in t->ctl_host) if the small path ran.
+bool bicgstabl_small(ThreadCtx *t, const bkh_matrix *A, int l, const double *b, double *x, double tol,
+                     int max_iter, int *rc) {
+    if (A->n > SMALL_MAX_N || A->nnz > SMALL_MAX_NNZ || l < 1 || l > 8) return false;
+    const long long n = A->n;
+    void *ws = bkh_workspace(carve_size(sizeof(Ctl)) + (2 * (l + 1) + 1) * carve_size(sizeof(double) * n));
+    if (!ws) { *rc = BKH_DEVICE_ERROR; return true; }
+    Carve cv(ws);
+    Ctl *c = cv.take<Ctl>(1);
+    double *R = cv.take<double>((l + 1) * n), *P = cv.take<double>((l + 1) * n), *rs = cv.take<double>(n);
+    hipStream_t s = t->stream;
+    *rc = ctl_init(t, c, tol, max_iter);
+    if (*rc) return true;
+    switch (l) {
+#define CASE(L_) case L_: hipLaunchKernelGGL((k_bicgstabl_small<L_>), dim3(1), dim3(SMALL_BS), 0, s, (int)n, \
+                     (const int *)A->rp, (const int *)A->ci, (const double *)A->val, b, x, R, P, rs, c); break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+    }
+    if (hipGetLastError() != hipSuccess) { bkh_set_error("small BiCGSTAB(l) launch failed"); *rc = BKH_DEVICE_ERROR; return true; }
+    *rc = ctl_fetch(t, c);
+    return true;
+}

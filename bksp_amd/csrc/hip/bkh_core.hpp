@@ -94,26 +94,52 @@ __device__ __forceinline__ void block_partial(double (&v)[NV], double *part) {
     }
 }
 
-// One workgroup: sums the nblocks partials of each of the NV values in a fixed
-// order (deterministic) and runs the solver's scalar update `fin` in thread 0.
-template <int NV, class Fin>
-__global__ __launch_bounds__(256) void k_finalize(const double *part, int nblocks, Ctl *c, Fin fin) {
-    if (c && ctl_stopped(c)) return;
-    double acc[NV];
+// First level of a two-level partial reduction: workgroup r sums partial rows
+// [r*RED_CHUNK, (r+1)*RED_CHUNK) of an nblocks x E array (E <= 256 values per
+// row) into out[r*E + e], in a fixed order.
+#define RED_CHUNK 128
+__global__ __launch_bounds__(256) void k_part_reduce(const double *part, int nblocks, int E, double *out);
+
+// Sums the nblocks rows of NV partials in a fixed order; valid in thread 0 of the
+// calling workgroup (256 threads).
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double *part, int nblocks, double (&acc)[NV]) {
+    __shared__ double sh[NV * 4];
 #pragma unroll
     for (int i = 0; i < NV; i++) acc[i] = 0.0;
     for (int b = threadIdx.x; b < nblocks; b += 256) {
 #pragma unroll
         for (int i = 0; i < NV; i++) acc[i] += part[(size_t)b * NV + i];
     }
-    __shared__ double sh[NV * 4];
     block_sum<NV, 256>(acc, sh);
+}
+
+// One workgroup: sums the partials in a fixed order (deterministic) and runs the
+// solver's scalar update `fin` in thread 0.
+template <int NV, class Fin>
+__global__ __launch_bounds__(256) void k_finalize(const double *part, int nblocks, Ctl *c, Fin fin) {
+    if (c && ctl_stopped(c)) return;
+    double acc[NV];
+    sum_partials<NV>(part, nblocks, acc);
     if (threadIdx.x == 0) fin(c, acc);
 }
 
+// Second-level buffer of the thread's stream (see k_part_reduce); nullptr if the
+// runtime is not initialised.
+double *bkh_red2();
+
+// Launches the finalize of nblocks x NV partials; wide ones go through
+// k_part_reduce first so no thread sums more than a few hundred values serially.
 template <int NV, class Fin>
 static inline void launch_finalize(hipStream_t s, const double *part, int nblocks, Ctl *c, Fin fin) {
-    hipLaunchKernelGGL((k_finalize<NV, Fin>), dim3(1), dim3(256), 0, s, part, nblocks, c, fin);
+    if ((long long)nblocks * NV > 8192) {
+        double *r2 = bkh_red2();
+        const int nb2 = (nblocks + RED_CHUNK - 1) / RED_CHUNK;
+        hipLaunchKernelGGL(k_part_reduce, dim3(nb2), dim3(256), 0, s, part, nblocks, NV, r2);
+        hipLaunchKernelGGL((k_finalize<NV, Fin>), dim3(1), dim3(256), 0, s, (const double *)r2, nb2, c, fin);
+    } else {
+        hipLaunchKernelGGL((k_finalize<NV, Fin>), dim3(1), dim3(256), 0, s, part, nblocks, c, fin);
+    }
 }
 
 // Logical workgroup index: XCD x (hardware dispatch blockIdx % 8) receives the

@@ -79,9 +79,11 @@ struct ThreadCtx {
     void *ws = nullptr;
     size_t ws_bytes = 0;
     Ctl *ctl_host = nullptr;           // pinned mirror of the control block
+    double *red2 = nullptr;            // second-level partial sums (RED2_CAP doubles)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 ThreadCtx *bkh_tctx();                 // nullptr (with error set) if init failed
+#define RED2_CAP (1 << 17)
 // Returns a device workspace of at least `bytes` (valid until the next call).
 void *bkh_workspace(size_t bytes);
 

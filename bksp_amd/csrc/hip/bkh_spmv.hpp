@@ -212,11 +212,12 @@ static inline int spmv_grid(const bkh_matrix *A) {
     return A->variant == SPMV_WAVE ? (A->nwblk + 3) / 4 : A->nblk;
 }
 
-// Host launcher. `ctl`/`part` may be null when NV == 0.
+// Launches the SpMV kernel only (partials written, no finalize); returns the
+// number of workgroups, i.e. of partial rows.
 template <typename VT, typename YT, int NV, class Fin>
-static inline void launch_spmv(hipStream_t s, const bkh_matrix *A, const VT *val, const YT *x,
-                               YT *y, double alpha, double beta, const Epi<NV, YT> &e, Ctl *ctl,
-                               double *part, Fin fin) {
+static inline int launch_spmv_kernel(hipStream_t s, const bkh_matrix *A, const VT *val, const YT *x,
+                                     YT *y, double alpha, double beta, const Epi<NV, YT> &e, Ctl *ctl,
+                                     double *part, Fin fin) {
     SpmvArgs<VT, YT> a;
     a.rp = A->rp; a.ci = A->ci; a.blk = A->blk; a.val = val; a.x = x; a.y = y;
     a.alpha = alpha; a.beta = beta; a.alpha_one = (alpha == 1.0); a.has_beta = (beta != 0.0);
@@ -229,6 +230,16 @@ static inline void launch_spmv(hipStream_t s, const bkh_matrix *A, const VT *val
         hipLaunchKernelGGL((k_spmv<VT, YT, NV, Fin>), dim3(G), dim3(SPMV_BS), 0, s, a, e, ctl,
                            part, fin);
     }
+    return G;
+}
+
+// Host launcher: SpMV plus, when NV > 0, the finalize that reduces its epilogue
+// dots and runs `fin`. `ctl`/`part` may be null when NV == 0.
+template <typename VT, typename YT, int NV, class Fin>
+static inline void launch_spmv(hipStream_t s, const bkh_matrix *A, const VT *val, const YT *x,
+                               YT *y, double alpha, double beta, const Epi<NV, YT> &e, Ctl *ctl,
+                               double *part, Fin fin) {
+    const int G = launch_spmv_kernel<VT, YT, NV>(s, A, val, x, y, alpha, beta, e, ctl, part, fin);
     if constexpr (NV > 0) launch_finalize<NV>(s, part, G, ctl, fin);
 }
 

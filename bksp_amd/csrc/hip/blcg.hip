@@ -101,6 +101,15 @@ __global__ __launch_bounds__(256) void k_blk_fin(const double *part, int nblocks
     if (t == 0) ctl_check(c, f2);
 }
 
+template <int S, int OP>
+static void launch_blk_fin(hipStream_t s, const double *part, int nblocks, BlkMats *mt, Ctl *c) {
+    const int E = (OP == FIN_FNB) ? 1 : S * S;
+    double *r2 = bkh_red2();
+    const int nb2 = (nblocks + RED_CHUNK - 1) / RED_CHUNK;
+    hipLaunchKernelGGL(k_part_reduce, dim3(nb2), dim3(256), 0, s, part, nblocks, E, r2);
+    hipLaunchKernelGGL((k_blk_fin<S, OP>), dim3(1), dim3(256), 0, s, (const double *)r2, nb2, mt, c);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_blk_resid(long long ms, const T *B, const T *AX, T *V,
                                                    double *part) {
@@ -203,11 +212,11 @@ static int blcg_core(ThreadCtx *t, const bkh_matrix *A, const VT *val, const T *
     // R0 = B - A X0 (blcg.c:54-57), V, C = qr(R0) (:58), fnb = ||B||_F (:60)
     launch_spmm<S>(s, A, val, (const T *)Xt, AP, 1.0, 0.0, c);
     hipLaunchKernelGGL(k_blk_resid<T>, dim3(Gv), dim3(256), 0, s, m * S, Bt, (const T *)AP, V, part);
-    hipLaunchKernelGGL((k_blk_fin<S, FIN_FNB>), dim3(1), dim3(256), 0, s, part, Gv, mt, c);
+    launch_blk_fin<S, FIN_FNB>(s, part, Gv, mt, c);
     hipLaunchKernelGGL((k_gram<S, T>), dim3(G), dim3(256), 0, s, m, (const T *)V, (const T *)V, c, part);
-    hipLaunchKernelGGL((k_blk_fin<S, FIN_QR1>), dim3(1), dim3(256), 0, s, part, G, mt, c);
+    launch_blk_fin<S, FIN_QR1>(s, part, G, mt, c);
     hipLaunchKernelGGL((k_blk_vr<S, T>), dim3(G), dim3(256), 0, s, m, V, mt, c, part);
-    hipLaunchKernelGGL((k_blk_fin<S, FIN_QR2_INIT>), dim3(1), dim3(256), 0, s, part, G, mt, c);
+    launch_blk_fin<S, FIN_QR2_INIT>(s, part, G, mt, c);
     // P = V (then P = V + P beta^T with beta = I: P0 = 2V, blcg.c:59-71)
     hipLaunchKernelGGL((k_blk_vp<S, T>), dim3(G), dim3(256), 0, s, m, V, P, mt,
                        p0_mode ? 1.0 : 2.0, c);
@@ -215,12 +224,12 @@ static int blcg_core(ThreadCtx *t, const bkh_matrix *A, const VT *val, const T *
     hipGraphExec_t ge = capture_graph(s, K, [&]() {
         launch_spmm<S>(s, A, val, (const T *)P, AP, 1.0, 0.0, c);
         hipLaunchKernelGGL((k_gram<S, T>), dim3(G), dim3(256), 0, s, m, (const T *)P, (const T *)AP, c, part);
-        hipLaunchKernelGGL((k_blk_fin<S, FIN_ALPHA>), dim3(1), dim3(256), 0, s, part, G, mt, c);
+        launch_blk_fin<S, FIN_ALPHA>(s, part, G, mt, c);
         hipLaunchKernelGGL((k_blk_xv<S, T>), dim3(G), dim3(256), 0, s, m, Xt, (const T *)P, V,
                            (const T *)AP, mt, c, part);
-        hipLaunchKernelGGL((k_blk_fin<S, FIN_QR1>), dim3(1), dim3(256), 0, s, part, G, mt, c);
+        launch_blk_fin<S, FIN_QR1>(s, part, G, mt, c);
         hipLaunchKernelGGL((k_blk_vr<S, T>), dim3(G), dim3(256), 0, s, m, V, mt, c, part);
-        hipLaunchKernelGGL((k_blk_fin<S, FIN_QR2_ITER>), dim3(1), dim3(256), 0, s, part, G, mt, c);
+        launch_blk_fin<S, FIN_QR2_ITER>(s, part, G, mt, c);
         hipLaunchKernelGGL((k_blk_vp<S, T>), dim3(G), dim3(256), 0, s, m, V, P, mt, 0.0, c);
     });
     if (!ge) { bkh_set_error("hipGraph capture of the block CG iteration failed"); return BKH_DEVICE_ERROR; }
@@ -323,4 +332,54 @@ extern "C" int bkh_blcg(bkh_matrix *A, int s, const double *B, double *X, double
     }
     if (info) *info = inf;
     return ret;
+}
+
+// Thin QR of an m x s column-major device panel (CholeskyQR2): V <- Q, R (host,
+// s x s column-major, upper, positive diagonal). Replaces src/myqr.c:4-39, whose
+// LAPACK Householder R may differ from this R by the signs of its rows.
+extern "C" int bkh_qr(int m, int s, double *V, double *R) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (!V || !R || m < 1) return BKH_BAD_ARGUMENT;
+    if (s < 1 || s > SM_MAX) { bkh_set_error("QR supports 1 <= s <= %d", SM_MAX); return -3; }
+    const int G = block_grid(m);
+    size_t bytes = carve_size(sizeof(Ctl)) + carve_size(sizeof(BlkMats)) +
+                   carve_size(sizeof(double) * G * SM_N) + 2 * carve_size(sizeof(double) * (size_t)m * s);
+    void *ws = bkh_workspace(bytes);
+    if (!ws) return BKH_DEVICE_ERROR;
+    Carve cv(ws);
+    Ctl *c = cv.take<Ctl>(1);
+    BlkMats *mt = cv.take<BlkMats>(1);
+    double *part = cv.take<double>((size_t)G * SM_N);
+    double *Vt = cv.take<double>((size_t)m * s), *Pt = cv.take<double>((size_t)m * s);
+    hipStream_t st = t->stream;
+    if (ctl_init(t, c, 0.0, 1)) return BKH_DEVICE_ERROR;
+    BlkMats h;
+    memset(&h, 0, sizeof h);
+    h.m = m;
+    h.s = s;
+    HIP_OK(hipMemcpyAsync(mt, &h, sizeof h, hipMemcpyHostToDevice, st));
+    const int Gt = vec_grid((long long)m * s, 256);
+    hipLaunchKernelGGL((k_cm2rm<double, double>), dim3(Gt), dim3(256), 0, st, (long long)m, s, (const double *)V, Vt);
+    switch (s) {
+#define CASE(S_)                                                                                            \
+    case S_:                                                                                                \
+        hipLaunchKernelGGL((k_gram<S_, double>), dim3(G), dim3(256), 0, st, (long long)m, (const double *)Vt, \
+                           (const double *)Vt, c, part);                                                     \
+        launch_blk_fin<S_, FIN_QR1>(st, part, G, mt, c);             \
+        hipLaunchKernelGGL((k_blk_vr<S_, double>), dim3(G), dim3(256), 0, st, (long long)m, Vt, mt, c, part); \
+        launch_blk_fin<S_, FIN_QR2_INIT>(st, part, G, mt, c);        \
+        hipLaunchKernelGGL((k_blk_vp<S_, double>), dim3(G), dim3(256), 0, st, (long long)m, Vt, Pt, mt, 1.0, c); \
+        break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+        CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
+#undef CASE
+    }
+    hipLaunchKernelGGL((k_rm2cm<double, double>), dim3(Gt), dim3(256), 0, st, (long long)m, s, (const double *)Vt, V);
+    LAUNCH_OK();
+    if (ctl_fetch(t, c)) return BKH_DEVICE_ERROR;
+    if (t->ctl_host->status == ST_BREAKDOWN) { bkh_set_error("QR: rank-deficient panel"); return -2; }
+    HIP_OK(hipMemcpyAsync(R, mt->C, sizeof(double) * s * s, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    return 0;
 }

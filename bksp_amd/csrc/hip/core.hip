@@ -45,13 +45,40 @@ ThreadCtx *bkh_tctx() {
     if (!t) return nullptr;
     if (hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking) != hipSuccess ||
         hipHostMalloc((void **)&t->ctl_host, sizeof(Ctl), hipHostMallocDefault) != hipSuccess ||
-        hipEventCreate(&t->ev0) != hipSuccess || hipEventCreate(&t->ev1) != hipSuccess) {
+        hipEventCreate(&t->ev0) != hipSuccess || hipEventCreate(&t->ev1) != hipSuccess ||
+        hipMalloc((void **)&t->red2, sizeof(double) * RED2_CAP) != hipSuccess) {
         bkh_set_error("failed to create the thread's HIP stream/events");
         delete t;
         return nullptr;
     }
     g_tctx = t;
     return t;
+}
+
+double *bkh_red2() {
+    ThreadCtx *t = bkh_tctx();
+    return t ? t->red2 : nullptr;
+}
+
+__global__ __launch_bounds__(256) void k_part_reduce(const double *part, int nblocks, int E, double *out) {
+    __shared__ double sh[256];
+    const int t = threadIdx.x;
+    const int b0 = blockIdx.x * RED_CHUNK, b1 = min(nblocks, b0 + RED_CHUNK);
+    const int groups = E >= 256 ? 1 : 256 / E;
+    for (int e0 = 0; e0 < E; e0 += 256) {
+        const int e = e0 + (E >= 256 ? t : t % E), g = E >= 256 ? 0 : t / E;
+        double acc = 0.0;
+        if (e < E && g < groups)
+            for (int b = b0 + g; b < b1; b += groups) acc += part[(size_t)b * E + e];
+        sh[t] = acc;
+        __syncthreads();
+        if (t < 256 && (E >= 256 ? true : t < E) && e < E) {
+            double v = 0.0;
+            for (int q = 0; q < groups; q++) v += sh[(E >= 256 ? t : t % E) + q * (E >= 256 ? 0 : E)];
+            out[(size_t)blockIdx.x * E + e] = v;
+        }
+        __syncthreads();
+    }
 }
 
 void *bkh_workspace(size_t bytes) {
@@ -610,4 +637,50 @@ int finish_solve(ThreadCtx *t, const bkh_matrix *A, double sigma, const double *
         *info = *inf;
     }
     return inf->ret;
+}
+
+// ||A||_F with a stored triangle's off-diagonals counted twice (src/norm.c:5-47).
+__global__ __launch_bounds__(256) void k_spnorm(int n, const int *rp, const int *ci, const double *v,
+                                                int sym, double *part) {
+    double acc[1] = {0.0};
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        for (int k = rp[i]; k < rp[i + 1]; k++) {
+            double t = v[k] * v[k];
+            acc[0] += (sym && ci[k] != i) ? 2.0 * t : t;
+        }
+    block_partial<1, 256>(acc, part);
+}
+
+extern "C" int bkh_sp_norm_f(int n, const int *row_ptr, const int *col, const double *val,
+                             int symmetric, double *out) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (check_csr(n, row_ptr, col) != 0) return BKH_BAD_ARGUMENT;
+    const long long nnz = row_ptr[n];
+    const int G = vec_grid(n, 256);
+    void *mem = nullptr;
+    size_t bytes = carve_size(sizeof(Ctl)) + carve_size(sizeof(double) * G) + carve_size(sizeof(int) * (n + 1)) +
+                   carve_size(sizeof(int) * (nnz + 1)) + carve_size(sizeof(double) * (nnz + 1));
+    HIP_OK(hipMalloc(&mem, bytes));
+    Carve cv(mem);
+    Ctl *c = cv.take<Ctl>(1);
+    double *part = cv.take<double>(G);
+    int *drp = cv.take<int>(n + 1), *dci = cv.take<int>(nnz + 1);
+    double *dv = cv.take<double>(nnz + 1);
+    int rc = ctl_init(t, c, 0.0, 1);
+    hipStream_t s = t->stream;
+    if (!rc && (hipMemcpyAsync(drp, row_ptr, sizeof(int) * (n + 1), hipMemcpyHostToDevice, s) != hipSuccess ||
+                hipMemcpyAsync(dci, col, sizeof(int) * nnz, hipMemcpyHostToDevice, s) != hipSuccess ||
+                hipMemcpyAsync(dv, val, sizeof(double) * nnz, hipMemcpyHostToDevice, s) != hipSuccess))
+        rc = BKH_DEVICE_ERROR;
+    if (!rc) {
+        hipLaunchKernelGGL(k_spnorm, dim3(G), dim3(256), 0, s, n, (const int *)drp, (const int *)dci,
+                           (const double *)dv, symmetric, part);
+        launch_finalize<1>(s, part, G, c, StoreFin());
+        if (hipGetLastError() != hipSuccess) rc = BKH_DEVICE_ERROR;
+    }
+    if (!rc) rc = ctl_fetch(t, c);
+    if (!rc) *out = sqrt(t->ctl_host->s[0]);
+    hipFree(mem);
+    return rc;
 }

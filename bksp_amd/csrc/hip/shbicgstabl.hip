@@ -1,17 +1,18 @@
 // shbicgstabl.hip -- shifted BiCGSTAB(l) on the device (replaces src/shbicgstabl.c:7-285
 // with src/util_shift.c:5-27 dhorner and :55-63 comb).
 //
-// The seed system (sigma = 0) runs the BiCGSTAB(l) kernels of bicgstabl.hip's
-// design; every per-shift update of a BiCG step runs in ONE fused kernel over all
-// shifts (k_sh_step): each thread reads the seed vectors r_0..r_j, p_{1..j+1} of
-// its row once, updates the seed's r_i and x, then loops over the shifts updating
-// p_shift[k][0..j+1] and x_k (shbicgstabl.c:111-134, :137-139, :146-153). The MR
-// step's per-shift updates (:202-237) and the seed update (:194-199, :240-242)
-// likewise share one kernel (k_sh_upd). The pi/theta/psi recurrences and the
-// Horner re-expansion are FP64 scalar work in the finalizers (one thread loops
-// over the shifts). x holds the seed solution first, then shift k at x + (k+1) n
-// (shbicgstabl.c:62). PT is the storage type of p_shift (double, or float for the
-// reduced-traffic variant).
+// The seed system (sigma = 0) runs BiCGSTAB(l) as in bicgstabl.hip. The per-shift
+// work of each BiCG step runs in one kernel over all shifts (k_sh_shift_step): a
+// 2-D grid of (rows x groups of SH_SPT shifts) in which every thread reads the
+// seed vectors of its row and issues the loads of its shifts' p_shift vectors
+// together (shbicgstabl.c:111-134, :146-153); the seed's own r/x update
+// (:107-108, :137-139) follows in k_sh_seed_r, so every shift thread reads the
+// pre-update r. The MR step's per-shift updates (:202-237, k_sh_shift_upd) and
+// the seed update (:194-199, :240-242, k_sh_seed_upd) are ordered the same way.
+// The pi/theta/psi recurrences and the Horner re-expansion are FP64 scalar work
+// in the finalize kernels, one shift per thread. x holds the seed solution first,
+// then shift k at x + (k+1) n (shbicgstabl.c:62). PT is the storage type of
+// p_shift (double, or float for the reduced-traffic variant).
 #include <math.h>
 #include "bkh_host.hpp"
 #include "bkh_mr.hpp"
@@ -47,34 +48,46 @@ struct ShInitFin {
 };
 
 // alpha = rho / (rs . p_{j+1}) (:105); per-shift pi_new, beta_k, alpha_k (:113-121)
-// and the coefficients of this step's fused update; then pi_old <- pi, pi <- pi_new
-// and alpha_old <- alpha (:156-161).
-struct ShAlphaFin {
-    ShState *st;
-    __device__ void operator()(Ctl *c, const double *tot) const {
-        if (bad(tot[0])) { ctl_breakdown(c); return; }
-        const double al = c->s[SB_G0] / tot[0];
-        const double be = -c->s[SB_BETA];            // shbicgstabl.c's beta (p = -beta p + r)
-        const double aold = c->s[SB_AOLD];
-        c->s[SB_ALPHA] = al;
-        for (int k = 0; k < st->ns; k++) {
-            const double pi = st->pi[k], po = st->pi_old[k], sg = st->sigma[k], th = st->theta[k];
-            const double pn = (1.0 + al * sg) * pi + al * be / aold * (po - pi);
-            const double bk = (po / pi) * (po / pi) * be;
-            const double ak = pi / pn * al;
-            st->cA[k] = -bk;
-            st->cB[k] = 1.0 / (pi * th);
-            st->cC[k] = 1.0 / pi;
-            st->cD[k] = -1.0 / pn;
-            st->cE[k] = 1.0 / (th * ak);
-            st->ak[k] = ak;
-            st->pi_old[k] = pi;
-            st->pi[k] = pn;
-            if (!(pn == pn) || pn == 0.0) { ctl_breakdown(c); return; }
+// and the coefficients of this step's fused update, one shift per thread; then
+// pi_old <- pi, pi <- pi_new and alpha_old <- alpha (:156-161).
+__global__ __launch_bounds__(256) void k_sh_alpha_fin(const double *part, int nblocks, Ctl *c,
+                                                      ShState *st) {
+    if (ctl_stopped(c)) return;
+    double acc[1];
+    sum_partials<1>(part, nblocks, acc);
+    __shared__ double al_s, be_s, ao_s;
+    __shared__ int ok;
+    if (threadIdx.x == 0) {
+        ok = !bad(acc[0]);
+        if (!ok) {
+            ctl_breakdown(c);
+        } else {
+            al_s = c->s[SB_G0] / acc[0];
+            be_s = -c->s[SB_BETA];                  // shbicgstabl.c's beta (p = -beta p + r)
+            ao_s = c->s[SB_AOLD];
+            c->s[SB_ALPHA] = al_s;
+            c->s[SB_AOLD] = al_s;
         }
-        c->s[SB_AOLD] = al;
     }
-};
+    __syncthreads();
+    if (!ok) return;
+    const double al = al_s, be = be_s, aold = ao_s;
+    for (int k = threadIdx.x; k < st->ns; k += blockDim.x) {
+        const double pi = st->pi[k], po = st->pi_old[k], sg = st->sigma[k], th = st->theta[k];
+        const double pn = (1.0 + al * sg) * pi + al * be / aold * (po - pi);
+        const double bk = (po / pi) * (po / pi) * be;
+        const double ak = pi / pn * al;
+        st->cA[k] = -bk;
+        st->cB[k] = 1.0 / (pi * th);
+        st->cC[k] = 1.0 / pi;
+        st->cD[k] = -1.0 / pn;
+        st->cE[k] = 1.0 / (th * ak);
+        st->ak[k] = ak;
+        st->pi_old[k] = pi;
+        st->pi[k] = pn;
+        if (!(pn == pn) || pn == 0.0) ctl_breakdown(c);
+    }
+}
 
 struct ShGammaFin {   // rho = r_{j+1} . rs; beta = alpha rho / rho_old (:91-92)
     __device__ void operator()(Ctl *c, const double *tot) const {
@@ -91,71 +104,83 @@ __device__ __forceinline__ double binom(int n, int r) {
     return c;
 }
 
-// Seed MR step (Gram -> coefficients, :166-199) and, per shift, Horner re-expansion
-// of the MR polynomial (util_shift.c:5-27), theta update and the x_k / p_shift[k][0]
-// coefficients (:202-237), re-expressed on the unmodified r_0..r_{l-1}.
-struct ShMrFin {
-    int l;
-    ShState *st;
-    __device__ void operator()(Ctl *c, const double *tot) const {
+// Seed MR step (Gram -> coefficients, :166-199) in thread 0, then, one shift per
+// thread, the Horner re-expansion of the MR polynomial (util_shift.c:5-27), the
+// theta update and the x_k / p_shift[k][0] coefficients (:202-237), re-expressed
+// on the unmodified r_0..r_{l-1}.
+template <int L>
+__global__ __launch_bounds__(256) void k_sh_mr_fin(const double *part, int nblocks, Ctl *c, ShState *st) {
+    if (ctl_stopped(c)) return;
+    constexpr int NV = (L + 1) * (L + 2) / 2;
+    double acc[NV];
+    sum_partials<NV>(part, nblocks, acc);
+    __shared__ MrOut o;
+    __shared__ int ok;
+    const int l = L;
+    if (threadIdx.x == 0) {
         double G[MR_D][MR_D];
-        gram_unpack(l, tot, G);
-        if (G[0][0] == 0.0) { c->done = 1; c->status = ST_CONVERGED; c->result_it = c->it; return; }
-        MrOut o;
-        if (!mr_from_gram(l, G, o)) { ctl_breakdown(c); return; }
-        c->s[SB_CX] = o.om[1];
-        for (int k = 1; k < l; k++) {
-            double t = 0.0;
-            for (int i = k; i < l; i++) t += o.odd[i] * o.C[i][k];
-            c->s[SB_CX + k] = t;
-        }
-        for (int k = 1; k <= l; k++) {
-            double t = 0.0;
-            for (int i = k; i <= l; i++) t += o.od[i] * o.C[i][k];
-            c->s[SB_CR + k] = t;
-            c->s[SB_CP + k] = o.om[k];
-            st->om[k] = o.om[k];
-        }
-        c->s[SB_OML] = o.om[l];
-        // shbicgstabl.c indexes omega[0..l-1] = om[1..l] and tau_s[i][j] = tau[i+1][j+1]
-        double out[MR_D], oms[MR_LMAX], osd[MR_LMAX], osdd[MR_LMAX];
-        for (int k = 0; k < st->ns; k++) {
-            const double sg = st->sigma[k];
-            out[l] = -o.om[l];                                   // dhorner
-            for (int i = l - 1; i > 0; i--) out[i] = -sg * out[i + 1] - o.om[i];
-            out[0] = -sg * out[1] + 1.0;
-            for (int i = 1; i < l; i++)
-                for (int j = l - 1; j >= i; j--) out[j] += -sg * out[j + 1];
-            for (int i = 1; i <= l; i++) out[i] = -out[i] / out[0];
-            const double psi = out[0];
-            for (int i = 0; i < l; i++) oms[i] = out[i + 1];
-            const double e = st->theta[k] * st->pi[k];
-            st->theta[k] *= psi;
-            for (int j = 0; j < l; j++) {                        // omega_shift_d (:212-218)
+        gram_unpack(l, acc, G);
+        ok = 1;
+        if (G[0][0] == 0.0) {
+            c->done = 1; c->status = ST_CONVERGED; c->result_it = c->it; ok = 0;
+        } else if (!mr_from_gram(l, G, o)) {
+            ctl_breakdown(c); ok = 0;
+        } else {
+            c->s[SB_CX] = o.om[1];
+            for (int k = 1; k < l; k++) {
                 double t = 0.0;
-                for (int i = j; i < l; i++) {
-                    double mij = (i == j) ? 1.0 : pow(sg, (double)(i - j)) * binom(i, j);
-                    t += mij * oms[i];
-                }
-                osd[j] = t;
+                for (int i = k; i < l; i++) t += o.odd[i] * o.C[i][k];
+                c->s[SB_CX + k] = t;
             }
-            for (int j = 0; j < l - 1; j++) {                    // omega_shift_dd (:220-226)
+            for (int k = 1; k <= l; k++) {
                 double t = 0.0;
-                for (int i = j + 1; i < l - 1; i++) t += o.tau[j + 1][i + 1] * osd[i + 1];
-                osdd[j] = osd[j + 1] + t;
+                for (int i = k; i <= l; i++) t += o.od[i] * o.C[i][k];
+                c->s[SB_CR + k] = t;
+                c->s[SB_CP + k] = o.om[k];
+                st->om[k] = o.om[k];
             }
-            // x_k += osd[0]/e r_0 + sum_{i=1}^{l-1} osdd[i-1]/e r'_i   (:229-234)
-            st->xc[k][0] = osd[0] / e;
-            for (int kk = 1; kk < l; kk++) {
-                double t = 0.0;
-                for (int i = kk; i < l; i++) t += osdd[i - 1] * o.C[i][kk];
-                st->xc[k][kk] = t / e;
-            }
-            st->psi_inv[k] = 1.0 / psi;
-            if (!(psi == psi) || psi == 0.0 || !(e == e) || e == 0.0) { ctl_breakdown(c); return; }
+            c->s[SB_OML] = o.om[l];
         }
     }
-};
+    __syncthreads();
+    if (!ok) return;
+    // shbicgstabl.c indexes omega[0..l-1] = om[1..l] and tau_s[i][j] = tau[i+1][j+1]
+    for (int k = threadIdx.x; k < st->ns; k += blockDim.x) {
+        double out[MR_D], oms[MR_LMAX], osd[MR_LMAX], osdd[MR_LMAX];
+        const double sg = st->sigma[k];
+        out[l] = -o.om[l];                                       // dhorner
+        for (int i = l - 1; i > 0; i--) out[i] = -sg * out[i + 1] - o.om[i];
+        out[0] = -sg * out[1] + 1.0;
+        for (int i = 1; i < l; i++)
+            for (int j = l - 1; j >= i; j--) out[j] += -sg * out[j + 1];
+        for (int i = 1; i <= l; i++) out[i] = -out[i] / out[0];
+        const double psi = out[0];
+        for (int i = 0; i < l; i++) oms[i] = out[i + 1];
+        const double e = st->theta[k] * st->pi[k];
+        st->theta[k] *= psi;
+        for (int j = 0; j < l; j++) {                            // omega_shift_d (:212-218)
+            double t = 0.0;
+            for (int i = j; i < l; i++) {
+                double mij = (i == j) ? 1.0 : pow(sg, (double)(i - j)) * binom(i, j);
+                t += mij * oms[i];
+            }
+            osd[j] = t;
+        }
+        for (int j = 0; j < l - 1; j++) {                        // omega_shift_dd (:220-226)
+            double t = 0.0;
+            for (int i = j + 1; i < l - 1; i++) t += o.tau[j + 1][i + 1] * osd[i + 1];
+            osdd[j] = osd[j + 1] + t;
+        }
+        st->xc[k][0] = osd[0] / e;                               // (:229-234)
+        for (int kk = 1; kk < l; kk++) {
+            double t = 0.0;
+            for (int i = kk; i < l; i++) t += osdd[i - 1] * o.C[i][kk];
+            st->xc[k][kk] = t / e;
+        }
+        st->psi_inv[k] = 1.0 / psi;
+        if (!(psi == psi) || psi == 0.0 || !(e == e) || e == 0.0) ctl_breakdown(c);
+    }
+}
 
 struct ShUpdFin {     // seed error check; next outer iteration's beta (:244-247, :88-92)
     __device__ void operator()(Ctl *c, const double *tot) const {
@@ -191,82 +216,137 @@ __global__ __launch_bounds__(256) void k_sh_p(long long n, int j, double *P, con
     }
 }
 
-// Fused BiCG-step update of the seed and of all shifts (step j).
-template <typename PT>
-__global__ __launch_bounds__(256) void k_sh_step(long long n, int j, int l, const double *P, double *R,
-                                                 double *x, PT *PS, const ShState *st, const Ctl *c) {
+// Seed part of BiCG step j: r_q -= alpha p_{q+1} (q <= j), x += alpha p_0
+// (:107-108, :137-139). Runs after the shift kernel, which reads the old r.
+__global__ __launch_bounds__(256) void k_sh_seed_r(long long n, int j, const double *P, double *R,
+                                                   double *x, const Ctl *c) {
     if (ctl_stopped(c)) return;
     const double al = c->s[SB_ALPHA];
-    const int ns = st->ns, l1 = l + 1;
     ROWLOOP(i, n) {
-        double rold[MR_D];
-        for (int q = 0; q <= j; q++) rold[q] = R[q * n + i];
-        // seed: r_q -= alpha p_{q+1} (q <= j), x += alpha p_0    (:107-108, :137-139)
-        for (int q = 0; q <= j; q++) R[q * n + i] = fma(-al, P[(q + 1) * n + i], rold[q]);
-        const double rjn = R[j * n + i];
+        for (int q = 0; q <= j; q++) R[q * n + i] = fma(-al, P[(q + 1) * n + i], R[q * n + i]);
         x[i] = fma(al, P[i], x[i]);
-        for (int k = 0; k < ns; k++) {
-            PT *ps = PS + (size_t)k * l1 * n;
-            const double cA = st->cA[k], cB = st->cB[k];
-            double pj = 0.0, p0 = 0.0;
-            for (int q = 0; q <= j; q++) {                        // (:124-128)
-                double v = fma(cB, rold[q], mul_rn(cA, (double)ps[q * n + i]));
-                ps[q * n + i] = (PT)v;
-                if (q == 0) p0 = (double)(PT)v;
-                if (q == j) pj = (double)(PT)v;
+    }
+}
+
+#define SH_SPT 8   // shifts per thread
+
+// Shift part of BiCG step J for shifts [blockIdx.y*SH_SPT, +SH_SPT): p_shift[k][0..J+1]
+// and x_k (:124-133, :146-153). r_J after the seed update is recomputed here with
+// the seed kernel's exact operation, so every thread reads the pre-update r.
+template <int J, typename PT>
+__global__ __launch_bounds__(256) void k_sh_shift_step(long long n, int l, const double *P, const double *R,
+                                                       double *x, PT *PS, const ShState *st, const Ctl *c) {
+    if (ctl_stopped(c)) return;
+    const double al = c->s[SB_ALPHA];
+    const int ns = st->ns, l1 = l + 1, k0 = blockIdx.y * SH_SPT;
+    ROWLOOP(i, n) {
+        double rold[J + 1];
+#pragma unroll
+        for (int q = 0; q <= J; q++) rold[q] = R[q * n + i];
+        const double rjn = fma(-al, P[(J + 1) * n + i], rold[J]);
+        double pv[SH_SPT][J + 1], xv[SH_SPT];
+#pragma unroll
+        for (int u = 0; u < SH_SPT; u++) {
+            const int k = k0 + u;
+            if (k < ns) {
+                const PT *ps = PS + (size_t)k * l1 * n;
+#pragma unroll
+                for (int q = 0; q <= J; q++) pv[u][q] = (double)ps[q * n + i];
+                xv[u] = x[(size_t)(k + 1) * n + i];
             }
-            double t = mul_rn(rold[j], st->cC[k]);               // (:129-130)
-            t = fma(st->cD[k], rjn, t);                          // (:147-148)
-            t = mul_rn(t, st->cE[k]);                            // (:149-150)
-            t = fma(-st->sigma[k], pj, t);                       // (:151-152)
-            ps[(j + 1) * n + i] = (PT)t;
-            double *xk = x + (size_t)(k + 1) * n;
-            xk[i] = fma(st->ak[k], p0, xk[i]);                    // (:133)
+        }
+#pragma unroll
+        for (int u = 0; u < SH_SPT; u++) {
+            const int k = k0 + u;
+            if (k < ns) {
+                PT *ps = PS + (size_t)k * l1 * n;
+                const double cA = st->cA[k], cB = st->cB[k];
+                double p0 = 0.0, pj = 0.0;
+#pragma unroll
+                for (int q = 0; q <= J; q++) {
+                    const PT v = (PT)fma(cB, rold[q], mul_rn(cA, pv[u][q]));
+                    ps[q * n + i] = v;
+                    if (q == 0) p0 = (double)v;
+                    if (q == J) pj = (double)v;
+                }
+                double t = mul_rn(rold[J], st->cC[k]);
+                t = fma(st->cD[k], rjn, t);
+                t = mul_rn(t, st->cE[k]);
+                t = fma(-st->sigma[k], pj, t);
+                ps[(J + 1) * n + i] = (PT)t;
+                x[(size_t)(k + 1) * n + i] = fma(st->ak[k], p0, xv[u]);
+            }
         }
     }
 }
 
-// MR-step update of the seed (x, r_0, p_0, with r_0.r_0 and rs.r_0) and of all shifts.
-template <typename PT>
-__global__ __launch_bounds__(256) void k_sh_upd(long long n, int l, double *P, double *R, double *x,
-                                                const double *rs, PT *PS, const ShState *st,
-                                                const Ctl *c, double *part) {
+// Seed MR update (x, r_0, p_0; r_0.r_0 and rs.r_0), after the shift update.
+__global__ __launch_bounds__(256) void k_sh_seed_upd(long long n, int l, double *P, double *R, double *x,
+                                                     const double *rs, const Ctl *c, double *part) {
     if (ctl_stopped(c)) return;
     double cx[MR_D], cr[MR_D], cp[MR_D];
     for (int k = 0; k <= l; k++) {
         cx[k] = c->s[SB_CX + k]; cr[k] = c->s[SB_CR + k]; cp[k] = c->s[SB_CP + k];
     }
-    const int ns = st->ns, l1 = l + 1;
     double v[2] = {0.0, 0.0};
     ROWLOOP(i, n) {
-        double r[MR_D];
-        for (int k = 0; k <= l; k++) r[k] = R[k * n + i];
-        double xi = x[i] + cx[0] * r[0], r0 = r[0], p0 = P[i];
+        double r0 = R[i], xi = x[i] + cx[0] * r0, p0 = P[i];
         for (int k = 1; k <= l; k++) {
-            if (k < l) xi += cx[k] * r[k];
-            r0 -= cr[k] * r[k];
+            const double rk = R[k * n + i];
+            if (k < l) xi += cx[k] * rk;
+            r0 -= cr[k] * rk;
             p0 -= cp[k] * P[k * n + i];
         }
         x[i] = xi; R[i] = r0; P[i] = p0;
         v[0] += r0 * r0;
         v[1] += rs[i] * r0;
-        for (int k = 0; k < ns; k++) {
-            PT *ps = PS + (size_t)k * l1 * n;
-            double *xk = x + (size_t)(k + 1) * n;
-            double xv = xk[i];
-            for (int q = 0; q < l; q++) xv += st->xc[k][q] * r[q];
-            xk[i] = xv;
-            double pv = (double)ps[i];
-            for (int q = 1; q <= l; q++) pv -= st->om[q] * (double)ps[q * n + i];
-            ps[i] = (PT)(pv * st->psi_inv[k]);
-        }
     }
     block_partial<2, 256>(v, part);
+}
+
+// Shift part of the MR step (:202-237): x_k += sum_q xc[k][q] r_q (original r),
+// p_shift[k][0] = (p_shift[k][0] - sum_i omega_i p_shift[k][i]) / psi_k.
+template <int L, typename PT>
+__global__ __launch_bounds__(256) void k_sh_shift_upd(long long n, const double *R, double *x, PT *PS,
+                                                      const ShState *st, const Ctl *c) {
+    if (ctl_stopped(c)) return;
+    const int ns = st->ns, k0 = blockIdx.y * SH_SPT;
+    ROWLOOP(i, n) {
+        double r[L];
+#pragma unroll
+        for (int q = 0; q < L; q++) r[q] = R[q * n + i];
+        double pv[SH_SPT][L + 1], xv[SH_SPT];
+#pragma unroll
+        for (int u = 0; u < SH_SPT; u++) {
+            const int k = k0 + u;
+            if (k < ns) {
+                const PT *ps = PS + (size_t)k * (L + 1) * n;
+#pragma unroll
+                for (int q = 0; q <= L; q++) pv[u][q] = (double)ps[q * n + i];
+                xv[u] = x[(size_t)(k + 1) * n + i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < SH_SPT; u++) {
+            const int k = k0 + u;
+            if (k < ns) {
+                double xk = xv[u];
+#pragma unroll
+                for (int q = 0; q < L; q++) xk += st->xc[k][q] * r[q];
+                x[(size_t)(k + 1) * n + i] = xk;
+                double p = pv[u][0];
+#pragma unroll
+                for (int q = 1; q <= L; q++) p -= st->om[q] * pv[u][q];
+                PS[(size_t)k * (L + 1) * n + i] = (PT)(p * st->psi_inv[k]);
+            }
+        }
+    }
 }
 
 __global__ void k_zero_pt(float *p, long long n) { ROWLOOP(i, n) p[i] = 0.0f; }
 __global__ void k_zero_pd(double *p, long long n) { ROWLOOP(i, n) p[i] = 0.0; }
 
+// SpMV r_L = A r_{L-1} with the Gram epilogue of [r_0..r_L], then the MR finalize.
 template <int L>
 static void sh_gram_spmv(hipStream_t s, const bkh_matrix *A, double *R, long long n, Ctl *c,
                          double *part, ShState *st) {
@@ -281,8 +361,46 @@ static void sh_gram_spmv(hipStream_t s, const bkh_matrix *A, double *R, long lon
             e.b[idx] = (unsigned char)(b == L ? EPI_Y : b);
             idx++;
         }
-    launch_spmv<double, double, NV>(s, A, A->val, R + (L - 1) * n, R + L * n, 1.0, 0.0, e, c, part,
-                                    ShMrFin{L, st});
+    const int G = launch_spmv_kernel<double, double, NV>(s, A, A->val, R + (L - 1) * n, R + L * n, 1.0, 0.0,
+                                                         e, c, part, NoFin());
+    double *r2 = bkh_red2();
+    const int nb2 = (G + RED_CHUNK - 1) / RED_CHUNK;
+    hipLaunchKernelGGL(k_part_reduce, dim3(nb2), dim3(256), 0, s, (const double *)part, G, NV, r2);
+    hipLaunchKernelGGL((k_sh_mr_fin<L>), dim3(1), dim3(256), 0, s, (const double *)r2, nb2, c, st);
+}
+
+// p_{j+1} = A p_j with rs.p_{j+1}, then the per-shift alpha finalize.
+static void sh_alpha_spmv(hipStream_t s, const bkh_matrix *A, const double *x, double *y, const double *rs,
+                          Ctl *c, double *part, ShState *st) {
+    Epi<1, double> e;
+    memset(&e, 0, sizeof e);
+    e.w[0] = rs;
+    e.a[0] = 0;
+    e.b[0] = EPI_Y;
+    const int G = launch_spmv_kernel<double, double, 1>(s, A, A->val, x, y, 1.0, 0.0, e, c, part, NoFin());
+    hipLaunchKernelGGL(k_sh_alpha_fin, dim3(1), dim3(256), 0, s, (const double *)part, G, c, st);
+}
+
+template <typename PT>
+static void sh_shift_step(int j, dim3 grid, hipStream_t s, long long n, int l, const double *P,
+                          const double *R, double *x, PT *PS, const ShState *st, const Ctl *c) {
+    switch (j) {
+#define CASE(J_) case J_: hipLaunchKernelGGL((k_sh_shift_step<J_, PT>), grid, dim3(256), 0, s, n, l, P, R, x, PS, st, c); break;
+        CASE(0) CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6)
+        default: hipLaunchKernelGGL((k_sh_shift_step<7, PT>), grid, dim3(256), 0, s, n, l, P, R, x, PS, st, c); break;
+#undef CASE
+    }
+}
+
+template <typename PT>
+static void sh_shift_upd(int l, dim3 grid, hipStream_t s, long long n, const double *R, double *x, PT *PS,
+                         const ShState *st, const Ctl *c) {
+    switch (l) {
+#define CASE(L_) case L_: hipLaunchKernelGGL((k_sh_shift_upd<L_, PT>), grid, dim3(256), 0, s, n, R, x, PS, st, c); break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
+        default: hipLaunchKernelGGL((k_sh_shift_upd<8, PT>), grid, dim3(256), 0, s, n, R, x, PS, st, c); break;
+#undef CASE
+    }
 }
 
 static void sh_gram_dispatch(int l, hipStream_t s, const bkh_matrix *A, double *R, long long n, Ctl *c,
@@ -306,6 +424,7 @@ static int sh_core(ThreadCtx *t, const bkh_matrix *A, int l, const double *b, do
     hipStream_t s = t->stream;
     const long long n = A->n;
     const int G = vec_grid(n, 256);
+    const dim3 g2(G, (ns + SH_SPT - 1) / SH_SPT > 0 ? (ns + SH_SPT - 1) / SH_SPT : 1);
     if (ctl_init(t, c, tol, max_it)) return BKH_DEVICE_ERROR;
     {
         ShState *h = new ShState();
@@ -332,18 +451,17 @@ static int sh_core(ThreadCtx *t, const bkh_matrix *A, int l, const double *b, do
     hipGraphExec_t ge = capture_graph(s, K, [&]() {
         for (int j = 0; j < l; j++) {
             hipLaunchKernelGGL(k_sh_p, dim3(G), dim3(256), 0, s, n, j, P, (const double *)R, c);
-            launch_spmv_dot_w<double, double>(s, A, A->val, P + j * n, P + (j + 1) * n, rs, c, part,
-                                              ShAlphaFin{st});
-            hipLaunchKernelGGL((k_sh_step<PT>), dim3(G), dim3(256), 0, s, n, j, l, (const double *)P, R, x,
-                               PS, (const ShState *)st, c);
+            sh_alpha_spmv(s, A, P + j * n, P + (j + 1) * n, rs, c, part, st);
+            if (ns > 0) sh_shift_step<PT>(j, g2, s, n, l, (const double *)P, (const double *)R, x, PS, st, c);
+            hipLaunchKernelGGL(k_sh_seed_r, dim3(G), dim3(256), 0, s, n, j, (const double *)P, R, x, c);
             if (j < l - 1)
                 launch_spmv_dot_w<double, double>(s, A, A->val, R + j * n, R + (j + 1) * n, rs, c, part,
                                                   ShGammaFin());
             else
                 sh_gram_dispatch(l, s, A, R, n, c, part, st);
         }
-        hipLaunchKernelGGL((k_sh_upd<PT>), dim3(G), dim3(256), 0, s, n, l, P, R, x, (const double *)rs, PS,
-                           (const ShState *)st, c, part);
+        if (ns > 0) sh_shift_upd<PT>(l, g2, s, n, (const double *)R, x, PS, st, c);
+        hipLaunchKernelGGL(k_sh_seed_upd, dim3(G), dim3(256), 0, s, n, l, P, R, x, (const double *)rs, c, part);
         launch_finalize<2>(s, part, G, c, ShUpdFin());
     });
     if (!ge) { bkh_set_error("hipGraph capture of the shifted BiCGSTAB(l) iteration failed"); return BKH_DEVICE_ERROR; }

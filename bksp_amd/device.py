@@ -114,8 +114,8 @@ class Matrix:
             raise BkspDeviceError(f"bkh_spmv32 -> {rc}: {last_error()}")
 
 
-def make_opts(precision=BKH_FP64, graph_iters=0, fix_blcg_p0=0, max_inner=0, inner_tol=0.0):
-    return BkhOpts(precision, graph_iters, fix_blcg_p0, max_inner, inner_tol)
+def make_opts(precision=BKH_FP64, graph_iters=0, fix_blcg_p0=0, max_inner=0, inner_tol=0.0, path=0):
+    return BkhOpts(precision, graph_iters, fix_blcg_p0, max_inner, inner_tol, path)
 
 
 def _check(rc, name):

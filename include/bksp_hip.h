@@ -40,6 +40,8 @@ typedef struct {
                             uninitialised beta gives P0 = 2V, the default here) */
     int    max_inner;    /* FP32-IR: inner iteration cap per refinement pass (0 = default) */
     double inner_tol;    /* FP32-IR: inner relative tolerance (0 = default) */
+    int    path;         /* BiCGSTAB(l): 0 auto, 1 multi-kernel graph, 2 one persistent
+                            workgroup (systems up to 16384 rows / 400000 nonzeros) */
 } bkh_opts;
 
 typedef struct {
@@ -97,6 +99,13 @@ int bkh_spmv32(const bkh_matrix *A, const float *d_x, float *d_y);
  * mkl_dcsrmm dispatch of include/bksp_internal.h:63-79. */
 int bkh_spmm(const bkh_matrix *A, int s, double alpha, const double *d_X,
              double beta, double *d_Y);
+/* Thin QR of an m x s column-major device panel, s <= 16 (CholeskyQR2): d_V <- Q,
+ * R (host, s x s column-major, positive diagonal). Replaces src/myqr.c:4-39. */
+int bkh_qr(int m, int s, double *d_V, double *R);
+/* ||A||_F of host CSR arrays, stored-triangle off-diagonals counted twice when
+ * symmetric = 1 (replaces src/norm.c:5-47). */
+int bkh_sp_norm_f(int n, const int *row_ptr, const int *col, const double *val,
+                  int symmetric, double *out);
 /* Deterministic FP64 dot product (result to host). */
 int bkh_dot(long long n, const double *d_x, const double *d_y, double *out);
 /* G = X^T Y (s x s, column-major, to host) for m x s column-major X, Y, on FP64 MFMA. */

@@ -1,0 +1,38 @@
+/*
+ * bksp_kernels.h -- the reference library's exported kernel symbols, for callers
+ * that use them directly (the reference's tests/test.c calls sp_norm_f and, through
+ * include/bksp_internal.h's inline wrappers, the CSR kernels).
+ *
+ *   dcsrgemm0 / dcsrsymm0 / dcsrgemm1 / dcsrsymm1   reference include/bksp_internal.h:19-26
+ *                                                    (defined in src/matvec.c:11-246)
+ *   sp_norm_f                                        reference src/norm.c:37-47 (undeclared there)
+ *   dhorner / comb                                   reference include/bksp_internal.h:132-135
+ *
+ * The CSR kernels take host arrays, run on the GPU (upload, CSR-stream SpMV per
+ * column, download) and reproduce the reference's per-entry summation order.
+ * B and C are column-major with leading dimension m. 0-based (…0) or 1-based (…1)
+ * indices as in the reference. They return 0, or -1 with "invalid parameter 0."
+ * printed for an unknown `type`, like the reference; -5 on a device error.
+ */
+#ifndef BKSP_KERNELS_H
+#define BKSP_KERNELS_H
+#include "bksp.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int dcsrgemm0(char type, int m, int n, double *A, int *Ai, int *Aj, int nnz, double *B, double *C);
+int dcsrsymm0(char type, int m, int n, double *A, int *Ai, int *Aj, int nnz, double *B, double *C);
+int dcsrgemm1(char type, double alpha, int m, int n, double *A, int *Ai, int *Aj, int nnz,
+              double *B, double beta, double *C);
+int dcsrsymm1(char type, double alpha, int m, int n, double *A, int *Ai, int *Aj, int nnz,
+              double *B, double beta, double *C);
+double sp_norm_f(dspmat *A);
+int dhorner(double *gamma, int l, double sigma, double *out);
+int comb(int n, int r);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -125,3 +125,22 @@ def test_cfg1_fp32_ir():
     info = dv.bicgstabl(M, db, 2, dx, 1e-10, 2000, precision=dv.BKH_FP32_IR)
     assert info.ret >= 0 and info.true_res < 1e-10
     assert rel(dx.numpy(), xt) < 1e-6
+
+
+@pytest.mark.parametrize("path", [1, 2])
+def test_cfg1_both_paths(path):
+    """cfg1 through the multi-kernel graph path (1) and the single persistent
+    workgroup path (2): both meet the oracle's index and the true residual."""
+    n, rp, ci, v, xt, b = cfg1()
+    M = dv.Matrix(n, rp, ci, v)
+    db, dx = dv.DeviceArray.from_numpy(b), dv.DeviceArray.from_numpy(np.zeros(n))
+    info = dv.bicgstabl(M, db, 2, dx, 1e-10, 1000, path=path)
+    cret, cx = po.bicgstabl(n, rp, ci, v, b, 2, 1e-10, 1000)
+    assert info.ret >= 0 and close_index(info.ret, cret), (path, info.ret, cret)
+    assert info.true_res < 2e-10
+    assert rel(dx.numpy(), xt) < 1e-6
+    for k in (1, 4):
+        dx.upload(np.zeros(n))
+        dv.bicgstabl(M, db, 2, dx, 1e-300, k, path=path)
+        _, ck = po.bicgstabl(n, rp, ci, v, b, 2, 1e-300, k)
+        assert rel(dx.numpy(), ck) < 1e-10, (path, k)
