@@ -1,0 +1,188 @@
+// bkh_cgfused.hpp -- CG in two kernels per iteration, with no finalize kernels.
+//
+// Iteration k of cg.c:32-60 becomes
+//   A(k): p_k = r_k + beta_k p_{k-1} folded into the SpMV's gather (cg.c:53-54),
+//         q = A p_k (cg.c:35-36), partial sums of p_k.q (cg.c:39);
+//   B(k): x += alpha_k p_k, r -= alpha_k q (cg.c:42-48), partial sums of r.r.
+// Each kernel needs one scalar of the kernel before it: instead of a one-workgroup
+// finalize kernel between them (a launch plus a serial reduction on the critical
+// path), every wave of the consumer sums the producer's partials itself, all in the
+// same fixed order, so every wave holds the same bits (the butterfly below adds the
+// same two operands in both lanes of each pair). A(k) also runs the convergence test
+// of iteration k-1 (cg.c:56-59) and stops the solve; a one-wave tail kernel at the
+// end of a graph replay runs the test of the replay's last iteration and advances
+// Ctl::base, the loop index of the replay's first iteration.
+//
+// p alternates between two buffers by the parity of k; r.r of iteration k lives in
+// Ctl::s[CGF_RR + (k & 1)], written by block 0 of A(k) and read by B(k) and A(k+1)
+// (which writes the other slot), so no kernel reads a slot that the same kernel
+// writes. Partial counts are capped at CGF_NPART so the redundant sums stay short.
+//
+// Column indices are int32, or 16-bit deltas from the wave block's first row
+// (6 bytes per FP32 nonzero instead of 8) when the matrix has them.
+#pragma once
+#include "bkh_core.hpp"
+#include "bkh_host.hpp"
+
+#define CGF_NPART 1024
+#define CGF_WPB 8                     // waves per workgroup of A
+enum { CGF_RR = 8, CGF_FRESH = 10 };  // Ctl::s slots (8, 9: r.r by parity)
+
+// Sum of np partials (np <= CGF_NPART) in a fixed order, valid in every lane.
+__device__ __forceinline__ double wave_sum_parts(const double *part, int np) {
+    const int lane = threadIdx.x & 63;
+    double v[CGF_NPART / 64];
+#pragma unroll
+    for (int m = 0; m < CGF_NPART / 64; m++) {
+        const int i = lane + 64 * m;
+        v[m] = i < np ? part[i] : 0.0;
+    }
+    double a = 0.0;
+#pragma unroll
+    for (int m = 0; m < CGF_NPART / 64; m++) a += v[m];
+    return wave_sum(a);
+}
+
+// Convergence test of iteration k-1 given r_k.r_k (the reference's test, cg.c:56-59;
+// NaN/Inf stops with ST_BREAKDOWN). Returns the status; workgroup 0 records it.
+__device__ __forceinline__ int cgf_check(Ctl *c, int k, double rr, bool writer) {
+    const double err = sqrt(rr) / c->fnb;
+    int st = ST_RUNNING;
+    if (err < c->tol) st = ST_CONVERGED;
+    else if (!(err == err) || err > 1.0e300) st = ST_BREAKDOWN;
+    else if (k >= c->max_it) st = ST_MAXIT;
+    if (writer) {
+        c->err = err;
+        c->it = k;
+        if (st != ST_RUNNING) {
+            c->status = st;
+            c->result_it = k - 1;
+            c->done = 1;
+        }
+    }
+    return st;
+}
+
+// A(k). Workgroup g streams the contiguous range of `chunk` wave blocks starting
+// at g*chunk (g from the XCD-aware remap), its waves taking every CGF_WPB-th.
+template <typename VT, typename T, bool D16>
+__global__ __launch_bounds__(64 * CGF_WPB) void k_cgf_a(int nwblk, int chunk, const int *__restrict__ rp,
+                                                       const int *__restrict__ wblk, const int *__restrict__ ci,
+                                                       const short *__restrict__ cd, const VT *__restrict__ val,
+                                                       const T *r, T *P0, T *P1, T *q, Ctl *c, int j,
+                                                       const double *partB, int npB, double *partA) {
+    if (ctl_stopped(c)) return;
+    const int k = c->base + j;
+    const bool writer = blockIdx.x == 0 && threadIdx.x == 0;
+    double rr;
+    if (k == c->first) {
+        rr = c->s[CGF_RR + (k & 1)];
+    } else {
+        rr = wave_sum_parts(partB, npB);
+        if (cgf_check(c, k, rr, writer) != ST_RUNNING) return;
+        if (writer) c->s[CGF_RR + (k & 1)] = rr;
+    }
+    const bool fresh = k == c->first && c->s[CGF_FRESH] != 0.0;
+    const T be = fresh ? (T)0 : (T)(rr / c->s[CGF_RR + ((k + 1) & 1)]);
+    const T *pold = (k & 1) ? P1 : P0;
+    T *pnew = (k & 1) ? P0 : P1;
+
+    __shared__ T prod_all[CGF_WPB][SPMV_WCAP];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    T *prod = prod_all[w];
+    const int g = xcd_block(blockIdx.x, gridDim.x);
+    const int b_end = min(nwblk, (g + 1) * chunk);
+    double dv[1] = {0.0};
+    constexpr int VEC = 16 / sizeof(VT);
+    constexpr int STEPS = SPMV_WCAP / (VEC * 64) + 1;
+    typedef VT vvec __attribute__((ext_vector_type(VEC)));
+    typedef int ivec __attribute__((ext_vector_type(VEC)));
+    typedef short svec __attribute__((ext_vector_type(VEC)));
+    for (int b = g * chunk + w; b < b_end; b += CGF_WPB) {
+        const int r0 = wblk[b], r1 = wblk[b + 1];
+        const int k0 = rp[r0], k1 = rp[r1];
+        const int kb = k0 & ~(VEC - 1);
+        vvec vv[STEPS];
+        ivec cc[STEPS];
+#pragma unroll
+        for (int s = 0; s < STEPS; s++) {
+            const int kk = kb + VEC * (lane + s * 64);
+            if (kk < k1) {
+                vv[s] = *(const vvec *)(val + kk);
+                if (D16) {
+                    const svec s16 = *(const svec *)(cd + kk);
+#pragma unroll
+                    for (int u = 0; u < VEC; u++) cc[s][u] = r0 + (int)s16[u];
+                } else {
+                    cc[s] = *(const ivec *)(ci + kk);
+                }
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < STEPS; s++) {
+            const int kk = kb + VEC * (lane + s * 64);
+#pragma unroll
+            for (int u = 0; u < VEC; u++) {
+                if (kk + u >= k0 && kk + u < k1) {
+                    const int col = cc[s][u];
+                    const T pc = add_rn(mul_rn(be, pold[col]), r[col]);
+                    prod[kk + u - k0] = mul_rn((T)vv[s][u], pc);
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int row = r0 + lane;
+        if (row < r1) {
+            const int s0 = rp[row] - k0, s1 = rp[row + 1] - k0;
+            T sum = (T)0;
+            for (int kk = s0; kk < s1; kk++) sum = add_rn(sum, prod[kk]);
+            q[row] = sum;
+            const T pn = add_rn(mul_rn(be, pold[row]), r[row]);
+            pnew[row] = pn;
+            dv[0] += (double)pn * (double)sum;
+        }
+        // the next block's products overwrite prod: every lane has read its rows
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    block_partial<1, 64 * CGF_WPB>(dv, partA);
+}
+
+// B(k): alpha_k = (r_k.r_k) / (p_k.q); x += alpha p_k, r -= alpha q; r.r partials.
+template <typename T>
+__global__ __launch_bounds__(512) void k_cgf_b(long long n, T *x, T *r, const T *P0, const T *P1,
+                                               const T *q, Ctl *c, int j, const double *partA, int npA,
+                                               double *partB) {
+    if (ctl_stopped(c)) return;
+    const int k = c->base + j;
+    const double pq = wave_sum_parts(partA, npA);
+    if (bad(pq)) {                                    // breakdown (zero or non-finite p.q)
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            c->status = ST_BREAKDOWN; c->result_it = k; c->done = 1;
+        }
+        return;
+    }
+    const T al = (T)(c->s[CGF_RR + (k & 1)] / pq);
+    const T *p = (k & 1) ? P0 : P1;
+    double v[1] = {0.0};
+    for (long long i = (long long)blockIdx.x * 512 + threadIdx.x; i < n; i += (long long)gridDim.x * 512) {
+        x[i] = fma(al, p[i], x[i]);
+        const T ri = fma(-al, q[i], r[i]);
+        r[i] = ri;
+        v[0] += (double)ri * (double)ri;
+    }
+    block_partial<1, 512>(v, partB);
+}
+
+// End of a graph replay of K iterations: test of iteration base+K-1, base += K.
+__global__ __launch_bounds__(64) void k_cgf_tail(Ctl *c, int K, const double *partB, int npB) {
+    if (ctl_stopped(c)) return;
+    const int k = c->base + K;
+    const double rr = wave_sum_parts(partB, npB);
+    const bool writer = threadIdx.x == 0;
+    if (cgf_check(c, k, rr, writer) != ST_RUNNING) return;
+    if (writer) c->base = k;
+}

@@ -31,8 +31,8 @@ struct alignas(64) Ctl {
     int result_it;     // loop index at the break
     int max_it;
     int inner_total;   // iterations summed over refinement passes
-    unsigned ticket;   // unused (kept for layout)
-    int pad0;
+    int base;          // loop index of the first iteration of a graph replay
+    int first;         // loop index at which the current pass started
     double tol;
     double fnb;        // norm of the right-hand side
     double err;        // last recursive residual ratio

@@ -62,6 +62,8 @@ struct bkh_matrix {
     int *rp = nullptr, *ci = nullptr;  // row_ptr[n+1], col[nnz]
     double *val = nullptr;             // FP64 values
     float *val32 = nullptr;            // FP32 copy (precision extension)
+    short *cd16 = nullptr;             // column - first row of the entry's wave block,
+                                       // when every such delta fits in 16 bits
     double *diag = nullptr;            // diagonal (for Jacobi)
     int nblk = 0;                      // CSR-stream row blocks (workgroup variant)
     int *blk = nullptr;                // blk[nblk+1] row boundaries

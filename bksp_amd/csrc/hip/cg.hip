@@ -1,19 +1,28 @@
 // cg.hip -- conjugate gradients on the device (replaces src/cg.c:6-71).
 //
-// One iteration is three kernels, captured K at a time into a hipGraph:
-//   1. q = A p (CSR-stream SpMV) with the p.q dot in its epilogue; the last
-//      workgroup forms alpha = (r.r)/(p.q)                           (cg.c:35-39)
-//   2. x += alpha p, r -= alpha q, r.r; the last workgroup forms beta, records
-//      the recursive residual ||r||/||b|| and the break index      (cg.c:42-59)
-//   3. p = beta p + r                                              (cg.c:53-54)
-// Scalars never leave the device; the host reads the control block once per
-// graph replay. BKH_FP32_IR runs the same kernels on FP32 values and vectors
-// (FP64 scalars and dot accumulation) inside an FP64 refinement loop whose
-// residual b - A x and update x += d are FP64.
+// Plain FP64 path: two kernels per iteration (bkh_cgfused.hpp), K iterations and a
+// one-wave tail captured in a hipGraph:
+//   A: p = r + beta p folded into the SpMV's gather, q = A p, p.q partials
+//      (cg.c:35-39, :53-54), after the convergence test of the previous iteration;
+//   B: alpha = (r.r)/(p.q) from A's partials, x += alpha p, r -= alpha q, r.r
+//      partials (cg.c:42-48).
+// Matrices with rows longer than a wave block use an unfused sequence (SpMV with
+// p.q, finalize, x/r update, finalize, p update). Scalars never leave the device;
+// the host reads the control block once per graph replay.
+//
+// BKH_FP32_IR: mixed precision with reliable updates (Clark et al., "Solving
+// lattice QCD systems of equations using mixed precision solvers on GPUs", 2010).
+// The Krylov iteration runs in FP32 (values, 16-bit column deltas where they fit,
+// vectors; FP64 scalars and dot accumulation) on a correction x32; each time its
+// recursive residual has fallen by a factor RU_DELTA the host replaces it by the
+// FP64 true residual: x += x32 and r = b - A x in FP64, r32 = r, x32 = 0, and the
+// FP32 iteration continues with its search direction, so no Krylov information is
+// discarded (a restarted refinement would lose it at every pass).
 #include <math.h>
+#include "bkh_cgfused.hpp"
 #include "bkh_host.hpp"
-#include "bkh_spmv.hpp"
 #include "bkh_solver_common.hpp"
+#include "bkh_spmv.hpp"
 
 enum { CG_RR = 0, CG_ALPHA = 1, CG_BETA = 2 };
 
@@ -38,19 +47,24 @@ struct CgBetaFin {
 struct CgInitFin {
     __device__ void operator()(Ctl *c, const double *tot) const {
         c->s[CG_RR] = tot[0];
+        c->s[CG_BETA] = 0.0;
         c->fnb = sqrt(tot[1]);
         if (!(c->fnb > 0.0)) ctl_breakdown(c);
     }
 };
 
+#define ROWLOOP(i, n) \
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < (n); i += (long long)gridDim.x * 256)
+
+// r = b - q; p_a = r (unfused path) or p_a = 0 (fused: beta = 0 gives p = r).
 template <typename T>
 __global__ __launch_bounds__(256) void k_cg_init(long long n, const T *b, const T *q, T *r, T *p,
-                                                 Ctl *c, double *part) {
+                                                 int fused, double *part) {
     double v[2] = {0.0, 0.0};
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    ROWLOOP(i, n) {
         T ri = b[i] - q[i];
         r[i] = ri;
-        p[i] = ri;
+        p[i] = fused ? (T)0 : ri;
         v[0] += (double)ri * (double)ri;
         v[1] += (double)b[i] * (double)b[i];
     }
@@ -63,7 +77,7 @@ __global__ __launch_bounds__(256) void k_cg_xr(long long n, T *x, T *r, const T 
     if (ctl_stopped(c)) return;
     const T al = (T)c->s[CG_ALPHA];
     double v[1] = {0.0};
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    ROWLOOP(i, n) {
         x[i] = fma(al, p[i], x[i]);
         T ri = fma(-al, q[i], r[i]);
         r[i] = ri;
@@ -76,34 +90,213 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_cg_p(long long n, T *p, const T *r, const Ctl *c) {
     if (ctl_stopped(c)) return;
     const T be = (T)c->s[CG_BETA];
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
-        p[i] = add_rn(mul_rn(be, p[i]), r[i]);
+    ROWLOOP(i, n) p[i] = add_rn(mul_rn(be, p[i]), r[i]);
 }
 
-// One CG solve on device vectors of type T (values `val`), x updated in place.
-template <typename T, typename VT>
-static int cg_core(ThreadCtx *t, const bkh_matrix *A, const VT *val, const T *b, T *x, T *r, T *p,
-                   T *q, Ctl *c, double *part, double tol, int max_it, int K) {
+static inline bool cg_can_fuse(const bkh_matrix *A) { return A->max_row <= SPMV_WCAP; }
+
+struct CgfGrid { int GA, chunk, GB; };
+
+static CgfGrid cgf_grid(const bkh_matrix *A) {
+    CgfGrid g;
+    g.GA = std::max(1, std::min((A->nwblk + CGF_WPB - 1) / CGF_WPB, CGF_NPART));
+    g.chunk = (A->nwblk + g.GA - 1) / g.GA;
+    g.GB = (int)std::max(1LL, std::min((long long)((A->n + 2047) / 2048), (long long)CGF_NPART));
+    return g;
+}
+
+// Graph of K fused iterations plus the tail, on vectors T with values VT
+// (cd: 16-bit column deltas or null).
+template <typename VT, typename T>
+static hipGraphExec_t cgf_capture(hipStream_t s, const bkh_matrix *A, const VT *val, const short *cd, T *x,
+                                  T *r, T *P0, T *P1, T *q, Ctl *c, double *partA, double *partB, int K) {
+    const CgfGrid g = cgf_grid(A);
+    const long long n = A->n;
+    return capture_graph(s, 1, [&]() {
+        for (int j = 0; j < K; j++) {
+            if (cd)
+                hipLaunchKernelGGL((k_cgf_a<VT, T, true>), dim3(g.GA), dim3(64 * CGF_WPB), 0, s, A->nwblk, g.chunk,
+                                   A->rp, A->wblk, A->ci, cd, val, (const T *)r, P0, P1, q, c, j,
+                                   (const double *)partB, g.GB, partA);
+            else
+                hipLaunchKernelGGL((k_cgf_a<VT, T, false>), dim3(g.GA), dim3(64 * CGF_WPB), 0, s, A->nwblk, g.chunk,
+                                   A->rp, A->wblk, A->ci, cd, val, (const T *)r, P0, P1, q, c, j,
+                                   (const double *)partB, g.GB, partA);
+            hipLaunchKernelGGL(k_cgf_b<T>, dim3(g.GB), dim3(512), 0, s, n, x, r, (const T *)P0, (const T *)P1,
+                               (const T *)q, c, j, (const double *)partA, g.GA, partB);
+        }
+        hipLaunchKernelGGL(k_cgf_tail, dim3(1), dim3(64), 0, s, c, K, (const double *)partB, g.GB);
+    });
+}
+
+// Start of the fused path: r.r of iteration 0 in its parity slot, beta_0 = 0.
+struct CgfInitFin {
+    __device__ void operator()(Ctl *c, const double *tot) const {
+        c->s[CGF_RR] = tot[0];
+        c->s[CGF_FRESH] = 1.0;
+        c->fnb = sqrt(tot[1]);
+        if (!(c->fnb > 0.0)) ctl_breakdown(c);
+    }
+};
+
+// Plain FP64 CG; x updated in place. P holds 2n entries (the two p buffers),
+// part the init partials, partA/partB CGF_NPART each.
+static int cg_fp64(ThreadCtx *t, const bkh_matrix *A, const double *b, double *x, double *r, double *P,
+                   double *q, Ctl *c, double *part, double *partA, double *partB, double tol, int max_it,
+                   int K) {
     hipStream_t s = t->stream;
     const long long n = A->n;
     const int G = vec_grid(n, 256);
+    const bool fused = cg_can_fuse(A);
     if (ctl_init(t, c, tol, max_it)) return BKH_DEVICE_ERROR;
-    Epi<0, T> e0;
+    Epi<0> e0;
     memset(&e0, 0, sizeof e0);
-    launch_spmv<VT, T, 0>(s, A, val, x, q, 1.0, 0.0, e0, nullptr, nullptr, NoFin());
-    hipLaunchKernelGGL(k_cg_init<T>, dim3(G), dim3(256), 0, s, n, b, q, r, p, c, part);
-    launch_finalize<2>(s, part, G, c, CgInitFin());
+    launch_spmv<double, double, 0>(s, A, A->val, x, q, 1.0, 0.0, e0, nullptr, nullptr, NoFin());
+    hipLaunchKernelGGL(k_cg_init<double>, dim3(G), dim3(256), 0, s, n, b, q, r, P, (int)fused, part);
+    if (fused) launch_finalize<2>(s, part, G, c, CgfInitFin());
+    else launch_finalize<2>(s, part, G, c, CgInitFin());
     LAUNCH_OK();
-    hipGraphExec_t ge = capture_graph(s, K, [&]() {
-        launch_spmv_dot_self<VT, T>(s, A, val, p, q, c, part, CgAlphaFin());
-        hipLaunchKernelGGL(k_cg_xr<T>, dim3(G), dim3(256), 0, s, n, x, r, p, q, c, part);
-        launch_finalize<1>(s, part, G, c, CgBetaFin());
-        hipLaunchKernelGGL(k_cg_p<T>, dim3(G), dim3(256), 0, s, n, p, r, c);
-    });
+    hipGraphExec_t ge;
+    if (fused) {
+        ge = cgf_capture<double, double>(s, A, A->val, nullptr, x, r, P, P + n, q, c, partA, partB, K);
+    } else {
+        ge = capture_graph(s, K, [&]() {
+            launch_spmv_dot_self<double, double>(s, A, A->val, P, q, c, part, CgAlphaFin());
+            hipLaunchKernelGGL(k_cg_xr<double>, dim3(G), dim3(256), 0, s, n, x, r, P, q, c, part);
+            launch_finalize<1>(s, part, G, c, CgBetaFin());
+            hipLaunchKernelGGL(k_cg_p<double>, dim3(G), dim3(256), 0, s, n, P, r, c);
+        });
+    }
     if (!ge) { bkh_set_error("hipGraph capture of the CG iteration failed"); return BKH_DEVICE_ERROR; }
     int rc = run_graph_until_done(t, ge, c, max_it / K + 2);
     hipGraphExecDestroy(ge);
     return rc;
+}
+
+// ------------------------------------------------ FP32 with reliable updates
+
+#define RU_DELTA 0.1
+
+// r32 = (float) r64, x32 = 0, and (first time) the p buffer p0 = 0 or = r32.
+__global__ __launch_bounds__(256) void k_ru_load(long long n, const double *r64, float *r32, float *x32,
+                                                 float *p0, int p_is_r, double *part) {
+    double v[1] = {0.0};
+    ROWLOOP(i, n) {
+        const float ri = (float)r64[i];
+        r32[i] = ri;
+        x32[i] = 0.0f;
+        if (p0) p0[i] = p_is_r ? ri : 0.0f;
+        v[0] += (double)ri * (double)ri;
+    }
+    block_partial<1, 256>(v, part);
+}
+
+// x += x32 (FP64), the reliable update's solution step.
+__global__ __launch_bounds__(256) void k_ru_addx(long long n, double *x, const float *x32) {
+    ROWLOOP(i, n) x[i] += (double)x32[i];
+}
+
+// Inner control after a reliable update: r.r of the new r32, a new target, the
+// iteration index continues (the fused path's pass starts at it, with beta from
+// the new r.r over the last iterated one); ||b|| comes from the FP64 residual
+// (co->s[2]).
+struct RuResumeFin {
+    double target;
+    int first;
+    const Ctl *co;
+    __device__ void operator()(Ctl *c, const double *tot) const {
+        c->s[CG_RR] = tot[0];
+        if (first) c->s[CG_BETA] = 0.0;
+        c->s[CGF_RR + (c->it & 1)] = tot[0];
+        c->s[CGF_FRESH] = first ? 1.0 : 0.0;
+        c->base = c->it;
+        c->first = c->it;
+        c->fnb = co->s[2];
+        c->tol = target;
+        c->done = 0;
+        c->status = ST_RUNNING;
+        if (!(c->fnb > 0.0) || !(tot[0] == tot[0])) ctl_breakdown(c);
+    }
+};
+
+__global__ void k_ctl_resume_gate(Ctl *c) {
+    // finalize kernels return early on a stopped control block: clear it first
+    if (threadIdx.x == 0) c->done = 0;
+}
+
+static int cg_reliable(ThreadCtx *t, const bkh_matrix *A, const double *b, double *x, double tol,
+                       int max_iter, int K, bkh_info *inf) {
+    hipStream_t s = t->stream;
+    const long long n = A->n;
+    const int G = vec_grid(n, 256);
+    const bool fused = cg_can_fuse(A);
+    const int nparts = std::max(spmv_grid(A), G) * 2;
+    size_t bytes = 2 * carve_size(sizeof(Ctl)) + carve_size(sizeof(double) * nparts) +
+                   2 * carve_size(sizeof(double) * CGF_NPART) + carve_size(sizeof(double) * n) +
+                   5 * carve_size(sizeof(float) * n);
+    void *ws = bkh_workspace(bytes);
+    if (!ws) return BKH_DEVICE_ERROR;
+    Carve cv(ws);
+    Ctl *co = cv.take<Ctl>(1), *ci = cv.take<Ctl>(1);
+    double *part = cv.take<double>(nparts);
+    double *partA = cv.take<double>(CGF_NPART), *partB = cv.take<double>(CGF_NPART);
+    double *r64 = cv.take<double>(n);
+    float *x32 = cv.take<float>(n), *r32 = cv.take<float>(n), *P = cv.take<float>(2 * n), *q = cv.take<float>(n);
+    const short *cd = A->cd16;
+    if (ctl_init(t, co, 0.0, 1) || ctl_init(t, ci, 1.0, max_iter)) return BKH_DEVICE_ERROR;
+    hipGraphExec_t ge;
+    if (fused) {
+        ge = cgf_capture<float, float>(s, A, A->val32, cd, x32, r32, P, P + n, q, ci, partA, partB, K);
+    } else {
+        ge = capture_graph(s, K, [&]() {
+            launch_spmv_dot_self<float, float>(s, A, A->val32, P, q, ci, part, CgAlphaFin());
+            hipLaunchKernelGGL(k_cg_xr<float>, dim3(G), dim3(256), 0, s, n, x32, r32, P, q, ci, part);
+            launch_finalize<1>(s, part, G, ci, CgBetaFin());
+            hipLaunchKernelGGL(k_cg_p<float>, dim3(G), dim3(256), 0, s, n, P, r32, ci);
+        });
+    }
+    if (!ge) { bkh_set_error("hipGraph capture of the FP32 CG iteration failed"); return BKH_DEVICE_ERROR; }
+    int rc = 0, updates = 0;
+    double rel = 0.0;
+    for (;;) {
+        // reliable update: x += x32; r = b - A x in FP64; r32 = r, x32 = 0
+        if (updates > 0) hipLaunchKernelGGL(k_ru_addx, dim3(G), dim3(256), 0, s, n, x, (const float *)x32);
+        if ((rc = resid_fp64(t, A, 0.0, b, x, r64, co, part))) break;
+        if ((rc = ctl_fetch(t, co))) break;
+        rel = t->ctl_host->s[0];
+        if (rel < tol || !(rel == rel)) break;
+        if ((rc = ctl_fetch(t, ci))) break;
+        const int used = t->ctl_host->it;
+        if (updates > 0 && (t->ctl_host->status == ST_BREAKDOWN || used >= max_iter)) break;
+        const double target = std::max(0.5 * tol, RU_DELTA * rel);
+        // first pass: the p buffer read by the first iteration is p0 = 0 (fused,
+        // beta = 0 gives p = r) or p = r (unfused)
+        hipLaunchKernelGGL(k_ru_load, dim3(G), dim3(256), 0, s, n, (const double *)r64, r32, x32,
+                           updates == 0 ? P : nullptr, (int)!fused, part);
+        hipLaunchKernelGGL(k_ctl_resume_gate, dim3(1), dim3(64), 0, s, ci);
+        launch_finalize<1>(s, part, G, ci, RuResumeFin{target, updates == 0, co});
+        // unfused path: the stopped iteration skipped its direction update p = r + beta p;
+        // apply it with the new r32 (the fused path forms p inside its next SpMV)
+        if (!fused && updates > 0)
+            hipLaunchKernelGGL(k_cg_p<float>, dim3(G), dim3(256), 0, s, n, P, (const float *)r32, ci);
+        LAUNCH_OK();
+        if ((rc = run_graph_until_done(t, ge, ci, (max_iter - used) / K + 2))) break;
+        updates++;
+    }
+    hipGraphExecDestroy(ge);
+    if (rc) return rc;
+    if (ctl_fetch(t, ci)) return BKH_DEVICE_ERROR;
+    const int iters = t->ctl_host->it;
+    inf->inner_iters = iters;
+    inf->passes = updates;
+    inf->rec_res = rel;
+    if (rel < tol) {
+        inf->status = ST_CONVERGED;
+        inf->iters = iters > 0 ? iters - 1 : 0;
+        return inf->iters;
+    }
+    inf->status = (rel == rel && t->ctl_host->status != ST_BREAKDOWN) ? ST_MAXIT : ST_BREAKDOWN;
+    return inf->status == ST_MAXIT ? -1 : -2;
 }
 
 extern "C" int bkh_cg(bkh_matrix *A, const double *b, double *x, double tol, int max_iter,
@@ -115,8 +308,7 @@ extern "C" int bkh_cg(bkh_matrix *A, const double *b, double *x, double tol, int
     bkh_info inf;
     memset(&inf, 0, sizeof inf);
     const long long n = A->n;
-    const int K = o.graph_iters > 0 ? o.graph_iters : default_graph_iters(n);
-    const int nparts = std::max(spmv_grid(A), vec_grid(n, 256)) * 4;
+    int K = o.graph_iters > 0 ? o.graph_iters : default_graph_iters(n);
     if (max_iter <= 0) {                     // cg.c:15,67-70: error = 0 -> returns 0
         inf.ret = 0; inf.status = ST_CONVERGED;
         if (info) *info = inf;
@@ -126,42 +318,20 @@ extern "C" int bkh_cg(bkh_matrix *A, const double *b, double *x, double tol, int
     HIP_OK(hipEventRecord(t->ev0, t->stream));
     if (o.precision == BKH_FP32_IR) {
         if (!A->val32) { bkh_set_error("FP32_IR needs a matrix created with want_fp32"); return BKH_BAD_ARGUMENT; }
-        size_t bytes = carve_size(sizeof(Ctl)) * 2 + carve_size(sizeof(double) * nparts) +
-                       carve_size(sizeof(double) * n) + 5 * carve_size(sizeof(float) * n);
-        void *ws = bkh_workspace(bytes);
-        if (!ws) return BKH_DEVICE_ERROR;
-        Carve cv(ws);
-        Ctl *co = cv.take<Ctl>(1), *ci = cv.take<Ctl>(1);
-        double *part = cv.take<double>(nparts);
-        double *r64 = cv.take<double>(n);
-        float *r32 = cv.take<float>(n), *d32 = cv.take<float>(n), *rr = cv.take<float>(n),
-              *pp = cv.take<float>(n), *qq = cv.take<float>(n);
-        IrResult ir;
-        ret = ir_refine(t, A, b, x, r64, co, part, tol, max_iter, o, &ir,
-                        [&](double inner_tol, int inner_max, int *inner_it) -> int {
-                            // d32 = 0, solve A d = r32 to inner_tol in FP32
-                            int rc2 = cg_core<float, float>(t, A, A->val32, r32, d32, rr, pp, qq, ci,
-                                                            part, inner_tol, inner_max, K);
-                            if (rc2) return rc2;
-                            *inner_it = t->ctl_host->it;
-                            return t->ctl_host->status == ST_BREAKDOWN ? -2 : 0;
-                        },
-                        r32, d32);
-        inf.inner_iters = ir.inner_total;
-        inf.passes = ir.passes;
-        inf.status = ir.status;
-        inf.iters = ir.ret;
-        inf.rec_res = ir.res;
+        ret = cg_reliable(t, A, b, x, tol, max_iter, K, &inf);
+        if (ret < -2) return ret;
     } else {
+        const int nparts = std::max(spmv_grid(A), vec_grid(n, 256)) * 2;
         size_t bytes = carve_size(sizeof(Ctl)) + carve_size(sizeof(double) * nparts) +
-                       3 * carve_size(sizeof(double) * n);
+                       2 * carve_size(sizeof(double) * CGF_NPART) + 4 * carve_size(sizeof(double) * n);
         void *ws = bkh_workspace(bytes);
         if (!ws) return BKH_DEVICE_ERROR;
         Carve cv(ws);
         Ctl *c = cv.take<Ctl>(1);
         double *part = cv.take<double>(nparts);
-        double *r = cv.take<double>(n), *p = cv.take<double>(n), *q = cv.take<double>(n);
-        int rc = cg_core<double, double>(t, A, A->val, b, x, r, p, q, c, part, tol, max_iter, K);
+        double *partA = cv.take<double>(CGF_NPART), *partB = cv.take<double>(CGF_NPART);
+        double *r = cv.take<double>(n), *P = cv.take<double>(2 * n), *q = cv.take<double>(n);
+        int rc = cg_fp64(t, A, b, x, r, P, q, c, part, partA, partB, tol, max_iter, K);
         if (rc) return rc;
         ret = ctl_result(t->ctl_host);
         inf.status = t->ctl_host->status;

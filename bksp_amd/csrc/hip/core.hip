@@ -185,7 +185,7 @@ __global__ void k_ctl_init(Ctl *c, double tol, int max_it, int it0) {
         c->done = max_it <= it0 ? 1 : 0;
         c->status = max_it <= it0 ? ST_MAXIT : ST_RUNNING;
         c->it = it0; c->result_it = it0; c->max_it = max_it; c->inner_total = 0;
-        c->ticket = 0u; c->tol = tol; c->fnb = 0; c->err = 0;
+        c->base = it0; c->first = it0; c->tol = tol; c->fnb = 0; c->err = 0;
     }
     for (int i = threadIdx.x; i < BKH_MAX_SCAL; i += blockDim.x) c->s[i] = 0.0;
 }
@@ -304,7 +304,7 @@ static void build_blocks(int n, const std::vector<int> &rp, int max_rows, int ca
 static void free_matrix(bkh_matrix *A) {
     if (!A) return;
     hipFree(A->rp); hipFree(A->ci); hipFree(A->val); hipFree(A->val32);
-    hipFree(A->diag); hipFree(A->blk); hipFree(A->wblk);
+    hipFree(A->diag); hipFree(A->blk); hipFree(A->wblk); hipFree(A->cd16);
     if (A->T) free_matrix(A->T);
     delete A;
 }
@@ -353,6 +353,25 @@ static bkh_matrix *upload(int n, const std::vector<int> &rp, const std::vector<i
          hipMemcpyAsync(A->blk, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
          hipMemcpyAsync(A->wblk, wblk.data(), sizeof(int) * wblk.size(), hipMemcpyHostToDevice, s) == hipSuccess;
     std::vector<float> v32;
+    if (ok && want_fp32) {
+        // 16-bit column deltas relative to the first row of the entry's wave block
+        std::vector<short> cd((size_t)A->nnz);
+        bool fits = true;
+        for (size_t b = 0; b + 1 < wblk.size() && fits; b++) {
+            const int r0 = wblk[b];
+            for (int r = wblk[b]; r < wblk[b + 1] && fits; r++)
+                for (int k = rp[r]; k < rp[r + 1]; k++) {
+                    const int d = ci[k] - r0;
+                    if (d < -32768 || d > 32767) { fits = false; break; }
+                    cd[k] = (short)d;
+                }
+        }
+        if (fits && A->max_row <= SPMV_WCAP) {
+            ok = hipMalloc(&A->cd16, sizeof(short) * nz) == hipSuccess &&
+                 hipMemcpyAsync(A->cd16, cd.data(), sizeof(short) * A->nnz, hipMemcpyHostToDevice, s) == hipSuccess &&
+                 hipStreamSynchronize(s) == hipSuccess;
+        }
+    }
     if (ok && want_fp32) {
         v32.resize((size_t)A->nnz);
         for (long long k = 0; k < A->nnz; k++) v32[k] = (float)v[k];

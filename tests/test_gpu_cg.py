@@ -98,3 +98,52 @@ def test_cg_cfg2_full_size_fp64_and_fp32ir():
     assert rel(dx.numpy(), x64) < 1e-4
     print(f"cfg2 fp64 iters={i64.ret} {i64.time_ms:.2f} ms; fp32ir inner={i32.inner_iters} "
           f"passes={i32.passes} {i32.time_ms:.2f} ms; |x32-x64|/|x64|={rel(dx.numpy(), x64):.2e}")
+
+
+def _arrow(n, seed=7):
+    """SPD arrowhead matrix, lower triangle: dense first column, so the expanded
+    first row (n entries) is longer than a wave block and CG takes the unfused path."""
+    rng = np.random.default_rng(seed)
+    rp, ci, v = [0], [], []
+    for i in range(n):
+        if i == 0:
+            ci += [0]; v += [float(n)]
+        else:
+            ci += [0, i]; v += [rng.uniform(-0.5, 0.5), 2.0 + rng.uniform(0, 1)]
+        rp.append(len(ci))
+    return n, np.array(rp, np.int32), np.array(ci, np.int32), np.array(v)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_cg_long_row_unfused(prec):
+    n, rp, ci, v = _arrow(3000)
+    b = gen.normal(5, n)
+    tol = 1e-10
+    cret, cx = po.cg(n, rp, ci, v, b, tol, 500)
+    M = dv.Matrix(n, rp, ci, v, symmetric=True, fp32=True)
+    db, dx = dv.DeviceArray.from_numpy(b), dv.DeviceArray.from_numpy(np.zeros(n))
+    info = dv.cg(M, db, dx, tol, 500, precision=prec)
+    assert cret >= 0 and info.ret >= 0
+    assert info.true_res < 1.5 * tol
+    if prec == 0:
+        assert close_index(info.ret, cret), (info.ret, cret)
+        if info.ret == cret:
+            assert rel(dx.numpy(), cx) < 1e-8
+    else:
+        assert rel(dx.numpy(), cx) < 1e-6
+
+
+@pytest.mark.parametrize("m", [16, 40])
+def test_cg_fp32_reliable_small(m):
+    """FP32 inner iteration with reliable updates (16-bit column deltas on these
+    stencils) reaches the FP64 tolerance."""
+    n, rp, ci, v = gen.poisson3d(m, lower=True)
+    b = gen.normal(202, n)
+    tol = 1e-10
+    cret, cx = po.cg(n, rp, ci, v, b, tol, 5000)
+    M = dv.Matrix(n, rp, ci, v, symmetric=True, fp32=True)
+    db, dx = dv.DeviceArray.from_numpy(b), dv.DeviceArray.from_numpy(np.zeros(n))
+    info = dv.cg(M, db, dx, tol, 5000, precision=dv.BKH_FP32_IR)
+    assert info.ret >= 0 and info.true_res < tol
+    assert rel(dx.numpy(), cx) < 1e-7
+    assert info.passes >= 1
