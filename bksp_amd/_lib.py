@@ -83,6 +83,8 @@ _SIGS = {
                               ctypes.POINTER(BkhInfo)]),
     "bkh_blcg": (c_int, [c_vp, c_int, c_vp, c_vp, c_dbl, c_int, ctypes.POINTER(BkhOpts),
                          ctypes.POINTER(BkhInfo)]),
+    "bkh_blbicgstabl": (c_int, [c_vp, c_int, c_vp, c_int, c_vp, c_dbl, c_int, ctypes.POINTER(BkhOpts),
+                                ctypes.POINTER(BkhInfo)]),
     "bkh_shbicgstabl": (c_int, [c_vp, c_vp, c_int, c_dp, c_int, c_vp, c_dbl, c_int,
                                 ctypes.POINTER(BkhOpts), ctypes.POINTER(BkhInfo), c_dp]),
     # bksp.h drop-in (host arrays)
@@ -91,6 +93,11 @@ _SIGS = {
     "dbicg": (c_int, [ctypes.POINTER(DSpMat), c_dp, c_dbl, c_int, c_dp]),
     "dbicgstabl": (c_int, [ctypes.POINTER(DSpMat), c_dp, c_int, c_dbl, c_int, c_dp]),
     "dblcg": (c_int, [ctypes.POINTER(DSpMat), ctypes.POINTER(DMat), c_dbl, c_int, ctypes.POINTER(DMat)]),
+    "dblbicgstabl": (c_int, [ctypes.POINTER(DSpMat), ctypes.POINTER(DMat), c_int, c_dbl, c_int,
+                             ctypes.POINTER(DMat)]),
+    "dbicgstab": (c_int, [ctypes.POINTER(DSpMat), c_dp, c_dbl, c_int, c_dp]),
+    "dblbicgstab": (c_int, [ctypes.POINTER(DSpMat), ctypes.POINTER(DMat), c_dbl, c_int, ctypes.POINTER(DMat)]),
+    "dshbicg": (c_int, [ctypes.POINTER(DSpMat), c_dp, c_dp, c_int, c_dbl, c_int, c_dp]),
     "dshbicgstabl": (c_int, [ctypes.POINTER(DSpMat), c_dp, c_int, c_dp, c_int, c_dbl, c_int, c_dp]),
     "read_matrix_market": (ctypes.POINTER(DSpMat), [ctypes.c_char_p]),
     "myqr": (c_int, [c_int, c_int, c_dp, c_dp]),

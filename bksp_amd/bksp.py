@@ -92,6 +92,28 @@ def dblcg(A, B, tol, max_iter, X):
     return load().dblcg(A.ref, B.ref, tol, max_iter, X.ref)
 
 
+def dblbicgstabl(A, B, l, tol, max_iter, X):
+    """Block BiCGSTAB(l) (reference src/blbicgstabl.c:16-252); prints the reference's
+    summary. B, X are ``Mat`` (X overwritten)."""
+    return load().dblbicgstabl(A.ref, B.ref, l, tol, max_iter, X.ref)
+
+
+def dbicgstab(A, b, tol, max_iter, x):
+    """BiCGSTAB (reference src/bicgstab.c:5-73), run as BiCGSTAB(1)."""
+    return load().dbicgstab(A.ref, _dp(b), tol, max_iter, _dp(x))
+
+
+def dblbicgstab(A, B, tol, max_iter, X):
+    """Block BiCGSTAB (reference src/blbicgstab.c:11-124), run as block BiCGSTAB(1)."""
+    return load().dblbicgstab(A.ref, B.ref, tol, max_iter, X.ref)
+
+
+def dshbicg(A, b, sigma, tol, max_iter, x):
+    """Shifted BiCG (reference src/shbicg.c:4-130); x has n*(len(sigma)+1) entries."""
+    sig = np.ascontiguousarray(sigma, dtype=np.float64)
+    return load().dshbicg(A.ref, _dp(b), _dp(sig), len(sig), tol, max_iter, _dp(x))
+
+
 def dshbicgstabl(A, b, l, sigma, tol, max_iter, x):
     """Shifted BiCGSTAB(l) (reference src/shbicgstabl.c:7-285). x has n*(len(sigma)+1)
     entries: seed first, then shift k at x[(k+1)*n:]."""

@@ -19,6 +19,7 @@
 #include "bkh_block.hpp"
 #include "bkh_host.hpp"
 #include "bkh_small.hpp"
+#include "bkh_spmv.hpp"
 #include "bkh_solver_common.hpp"
 
 struct BlkMats {
@@ -381,5 +382,81 @@ extern "C" int bkh_qr(int m, int s, double *V, double *R) {
     if (t->ctl_host->status == ST_BREAKDOWN) { bkh_set_error("QR: rank-deficient panel"); return -2; }
     HIP_OK(hipMemcpyAsync(R, mt->C, sizeof(double) * s * s, hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
+    return 0;
+}
+
+// Y = alpha A X + beta Y for s column-major columns (the mkl_dcsrmm dispatch of
+// include/bksp_internal.h:63-79): s <= 16 through the row-major SpMM kernel, wider
+// blocks column by column through the SpMV.
+extern "C" int bkh_spmm(const bkh_matrix *A, int s, double alpha, const double *X, double beta, double *Y) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (!A || !X || !Y || s < 1) return BKH_BAD_ARGUMENT;
+    hipStream_t st = t->stream;
+    const long long m = A->n;
+    if (s > SM_MAX) {
+        Epi<0> e;
+        memset(&e, 0, sizeof e);
+        for (int j = 0; j < s; j++)
+            launch_spmv<double, double, 0>(st, A, A->val, X + (size_t)j * m, Y + (size_t)j * m, alpha, beta, e,
+                                           nullptr, nullptr, NoFin());
+        LAUNCH_OK();
+        HIP_OK(hipStreamSynchronize(st));
+        return 0;
+    }
+    void *ws = bkh_workspace(2 * carve_size(sizeof(double) * m * s));
+    if (!ws) return BKH_DEVICE_ERROR;
+    Carve cv(ws);
+    double *Xt = cv.take<double>(m * s), *Yt = cv.take<double>(m * s);
+    const int Gt = vec_grid(m * s, 256);
+    hipLaunchKernelGGL((k_cm2rm<double, double>), dim3(Gt), dim3(256), 0, st, m, s, X, Xt);
+    if (beta != 0.0) hipLaunchKernelGGL((k_cm2rm<double, double>), dim3(Gt), dim3(256), 0, st, m, s, (const double *)Y, Yt);
+    switch (s) {
+#define CASE(S_) case S_: launch_spmm<S_>(st, A, A->val, (const double *)Xt, Yt, alpha, beta, (const Ctl *)nullptr); break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+        CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
+#undef CASE
+    }
+    hipLaunchKernelGGL((k_rm2cm<double, double>), dim3(Gt), dim3(256), 0, st, m, s, (const double *)Yt, Y);
+    LAUNCH_OK();
+    HIP_OK(hipStreamSynchronize(st));
+    return 0;
+}
+
+// G = X^T Y (s x s, column-major, to host) for m x s column-major X, Y, on FP64 MFMA;
+// partials summed in a fixed order.
+extern "C" int bkh_gram(int m, int s, const double *X, const double *Y, double *G) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (!X || !Y || !G || m < 1) return BKH_BAD_ARGUMENT;
+    if (s < 1 || s > SM_MAX) { bkh_set_error("Gram supports 1 <= s <= %d", SM_MAX); return -3; }
+    hipStream_t st = t->stream;
+    const int Gb = block_grid(m), Gt = vec_grid((long long)m * s, 256);
+    void *ws = bkh_workspace(2 * carve_size(sizeof(double) * (size_t)m * s) + carve_size(sizeof(double) * Gb * SM_N));
+    if (!ws) return BKH_DEVICE_ERROR;
+    Carve cv(ws);
+    double *Xt = cv.take<double>((size_t)m * s), *Yt = cv.take<double>((size_t)m * s);
+    double *part = cv.take<double>((size_t)Gb * SM_N);
+    hipLaunchKernelGGL((k_cm2rm<double, double>), dim3(Gt), dim3(256), 0, st, (long long)m, s, X, Xt);
+    hipLaunchKernelGGL((k_cm2rm<double, double>), dim3(Gt), dim3(256), 0, st, (long long)m, s, Y, Yt);
+    switch (s) {
+#define CASE(S_) case S_: hipLaunchKernelGGL((k_gram<S_, double>), dim3(Gb), dim3(256), 0, st, (long long)m, \
+                                             (const double *)Xt, (const double *)Yt, (const Ctl *)nullptr, part); break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+        CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
+#undef CASE
+    }
+    double *r2 = bkh_red2();
+    const int nb2 = (Gb + RED_CHUNK - 1) / RED_CHUNK;
+    hipLaunchKernelGGL(k_part_reduce, dim3(nb2), dim3(256), 0, st, (const double *)part, Gb, s * s, r2);
+    LAUNCH_OK();
+    std::vector<double> h((size_t)nb2 * s * s);
+    HIP_OK(hipMemcpyAsync(h.data(), r2, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    for (int e = 0; e < s * s; e++) {
+        double acc = 0.0;
+        for (int b = 0; b < nb2; b++) acc += h[(size_t)b * s * s + e];
+        G[e] = acc;
+    }
     return 0;
 }

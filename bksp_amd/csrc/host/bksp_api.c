@@ -169,6 +169,72 @@ int dblcg(dspmat *A, dmat *B, double tol, int max_iter, dmat *X) {
     return rc;
 }
 
+/* reference src/blbicgstabl.c:16-252 (prints the same summary, :220-224) */
+int dblbicgstabl(dspmat *A, dmat *B, int l, double tol, int max_iter, dmat *X) {
+    int m = *A->row_size, s = B->col_size;
+    if (s < 1 || s > 16 || l < 1 || l > 8) return INVALID_PARAMETERS;
+    bkh_opts o = default_opts();
+    o.precision = BKH_FP64;
+    upload_t u;
+    size_t ms = (size_t)m * s;
+    bkh_info info;
+    memset(&info, 0, sizeof info);
+    int rc = upload_system(A, B->value, ms, X->value, ms, 0, 0, &u);
+    if (rc == 0) {
+        rc = bkh_blbicgstabl(u.M, s, u.d_b, l, u.d_x, tol, max_iter, &o, &info);
+        if (rc > BKH_DEVICE_ERROR && download_x(&u, X->value)) rc = BKH_DEVICE_ERROR;
+    }
+    release(&u);
+    if (rc > BKH_DEVICE_ERROR)
+        printf("Block BiCGSTAB(%d) has converged at:\n"
+               "        The number of iteration ... %d\n"
+               "        Frobenius norm of the relative residual ... %e\n"
+               "        Frobenius norm of the true residual ... %e\n",
+               l, max_iter <= 0 ? 0 : info.iters, info.rec_res, info.true_res);
+    fflush(stdout);
+    return rc;
+}
+
+/* reference src/bicgstab.c:5-73: BiCGSTAB is BiCGSTAB(1) (the same iterates in
+ * exact arithmetic; both return the loop index at convergence). */
+int dbicgstab(dspmat *A, double *b, double tol, int max_iter, double *x) {
+    return dbicgstabl(A, b, 1, tol, max_iter, x);
+}
+
+/* reference src/blbicgstab.c:11-124: block BiCGSTAB, here as block BiCGSTAB(1) with
+ * residual QR (the same block Krylov iterates in exact arithmetic; no summary). */
+int dblbicgstab(dspmat *A, dmat *B, double tol, int max_iter, dmat *X) {
+    int m = *A->row_size, s = B->col_size;
+    if (s < 1 || s > 16) return INVALID_PARAMETERS;
+    if (max_iter <= 0) return 0;
+    bkh_opts o = default_opts();
+    o.precision = BKH_FP64;
+    upload_t u;
+    size_t ms = (size_t)m * s;
+    int rc = upload_system(A, B->value, ms, X->value, ms, 0, 0, &u);
+    if (rc == 0) {
+        rc = bkh_blbicgstabl(u.M, s, u.d_b, 1, u.d_x, tol, max_iter, &o, NULL);
+        if (rc > BKH_DEVICE_ERROR && download_x(&u, X->value)) rc = BKH_DEVICE_ERROR;
+    }
+    release(&u);
+    return rc;
+}
+
+/* reference src/shbicg.c:4-130 */
+int dshbicg(dspmat *A, double *b, double *sigma, int sigma_size, double tol, int max_iter, double *x) {
+    if (max_iter <= 0) return 0;
+    int n = *A->row_size;
+    upload_t u;
+    size_t nx = (size_t)n * (sigma_size + 1);
+    int rc = upload_system(A, b, n, x, nx, 1, 0, &u);
+    if (rc == 0) {
+        rc = bkh_shbicg(u.M, u.d_b, sigma, sigma_size, u.d_x, tol, max_iter);
+        if (rc > BKH_DEVICE_ERROR && download_x(&u, x)) rc = BKH_DEVICE_ERROR;
+    }
+    release(&u);
+    return rc;
+}
+
 /* ---------------------------------------------------- dense helpers (util.c) */
 
 /* reference src/util.c:5-18 */

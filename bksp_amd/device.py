@@ -166,6 +166,16 @@ def blcg(M, s, d_B, d_X, tol, max_iter, **kw):
     return info
 
 
+def blbicgstabl(M, s, d_B, l, d_X, tol, max_iter, **kw):
+    """Block BiCGSTAB(l); d_B, d_X hold m x s column-major blocks (Fortran order)."""
+    opts = make_opts(**kw)
+    info = BkhInfo()
+    rc = load().bkh_blbicgstabl(M.handle, s, d_B.ptr, l, d_X.ptr, tol, max_iter, ctypes.byref(opts),
+                                ctypes.byref(info))
+    _check(rc, "bkh_blbicgstabl")
+    return info
+
+
 def shbicgstabl(M, d_b, l, sigma, d_x, tol, max_iter, **kw):
     """Shifted BiCGSTAB(l); d_x holds n*(len(sigma)+1) entries (seed first).
     Returns (info, per-system FP64 true residuals)."""

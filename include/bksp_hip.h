@@ -131,6 +131,18 @@ int bkh_bicgstabl(bkh_matrix *A, const double *d_b, int l, double *d_x, double t
 int bkh_blcg(bkh_matrix *A, int s, const double *d_B, double *d_X, double tol, int max_iter,
              const bkh_opts *opts, bkh_info *info);
 
+/* Block BiCGSTAB(l) with residual QR (replaces src/blbicgstabl.c:16-252). B, X: m x s
+ * column-major device blocks, 1 <= s <= 16, 1 <= l <= 8. Returns the outer iteration
+ * index at convergence (||xi||_F / ||B||_F < tol). info->true_res is the Frobenius
+ * ||B - A X||_F / ||B||_F. The initial residual is B - A X0. */
+int bkh_blbicgstabl(bkh_matrix *A, int s, const double *d_B, int l, double *d_X, double tol,
+                    int max_iter, const bkh_opts *opts, bkh_info *info);
+
+/* Shifted BiCG (replaces src/shbicg.c:4-130); host-driven, for link parity. A must be
+ * created with want_transpose = 1; d_x holds n*(ns+1) entries (seed first). */
+int bkh_shbicg(bkh_matrix *A, const double *d_b, const double *sigma, int ns, double *d_x,
+               double tol, int max_iter);
+
 /* Shifted BiCGSTAB(l) (replaces src/shbicgstabl.c:7-285): solves A x_0 = b and
  * (A + sigma_k I) x_k = b for ns <= 256 shifts with one Krylov space. d_x holds
  * n*(ns+1) entries: the seed solution, then shift k at d_x + (k+1)*n; the shifted

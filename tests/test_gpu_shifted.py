@@ -85,3 +85,22 @@ def test_cfg3_full_size(prec):
     assert tr[0] < 2e-8
     if prec == 0:
         assert np.all(tr < 2e-8), tr.max()
+
+
+def test_dropin_shifted_bicg_reference_scenario(pde2961):
+    """tests/test.c:243-262: dshbicg on pde2961 with sigma = {0, 0.2, 0.4, 0.8},
+    tol 1e-14; every system (A + sigma_k I) x_k = b is judged by its true residual."""
+    n, rp, ci, v, _ = pde2961
+    b = np.zeros(n); b[0] = 1.0
+    sig = np.array([0.0, 0.2, 0.4, 0.8])
+    A = bksp.dspmat(n, rp, ci, v, symmetric=False)
+    x = np.zeros(n * 5)
+    ret = bksp.dshbicg(A, b, sig, 1e-14, 500, x)
+    assert ret >= 0
+    import scipy.sparse as sp
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    As = sp.csr_matrix((v, (rows, ci)), shape=(n, n))
+    assert np.linalg.norm(b - As @ x[:n]) < 1e-13
+    for k, s in enumerate(sig):
+        xk = x[(k + 1) * n:(k + 2) * n]
+        assert np.linalg.norm(b - As @ xk - s * xk) < 1e-12, (s, np.linalg.norm(b - As @ xk - s * xk))
