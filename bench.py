@@ -35,6 +35,138 @@ def load_traffic(name):
     return None
 
 
+def _timed(fn, reps=1):
+    from bksp_amd import load
+    load().bkh_sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        r = fn()
+    load().bkh_sync()
+    return r, (time.perf_counter() - t0) * 1e3 / reps
+
+
+def bench_cfg1(dv, gen, po):
+    """BiCGSTAB(2), 64x64 convection-diffusion (n = 4096), tol 1e-10: latency case."""
+    n, rp, ci, v = gen.convdiff2d(64, 10.0, 20.0)
+    xt = gen.uniform(101, n, -1.0, 1.0)
+    import scipy.sparse as sp
+    b = sp.csr_matrix((v, ci, rp), shape=(n, n)) @ xt
+    M = dv.Matrix(n, rp, ci, v)
+    db, dx = dv.DeviceArray.from_numpy(b), dv.DeviceArray.from_numpy(np.zeros(n))
+    zeros = np.zeros(n)
+
+    def run(**kw):
+        dx.upload(zeros)
+        return dv.bicgstabl(M, db, 2, dx, 1e-10, 1000, **kw)
+
+    run()
+    info, ms = _timed(run, 10)
+    info_g, ms_g = _timed(lambda: run(path=1), 5)
+    cpu = None
+    if po is not None:                       # CPU baseline leg: the oracle's BiCGSTAB(2)
+        t0 = po.wall()
+        cret, _ = po.bicgstabl(n, rp, ci, v, b, 2, 1e-10, 1000)
+        cpu = {"value": round((po.wall() - t0) * 1e3, 3), "unit": "ms", "cores": 1, "kind": "port",
+               "outer_iters": cret}
+    return {"workload": "BiCGSTAB(2) 64x64 convection-diffusion, n=4096, tol 1e-10",
+            "gpu_ms": round(ms, 3), "outer_iters": info.ret, "true_res": info.true_res,
+            "path": "one persistent workgroup", "multi_kernel_graph_ms": round(ms_g, 3),
+            "per_outer_iter_us": round(ms * 1e3 / max(1, info.ret + 1), 1), "cpu_baseline": cpu}
+
+
+def bench_cfg2_block(dv, gen, m=128, s=8):
+    """Block CG (s = 8) on the 128^3 Poisson matrix, tol 1e-8, FP64 (P0 as the reference)."""
+    n, rp, ci, v = gen.poisson3d(m, lower=True)
+    B = gen.normal(202, n * s).reshape(s, n)            # column-major m x s: column j = B[j]
+    M = dv.Matrix(n, rp, ci, v, symmetric=True)
+    dB, dX = dv.DeviceArray.from_numpy(B), dv.DeviceArray.from_numpy(np.zeros((s, n)))
+    zeros = np.zeros((s, n))
+
+    def run(fix):
+        dX.upload(zeros)
+        return dv.blcg(M, s, dB, dX, 1e-8, 5000, fix_blcg_p0=fix)
+
+    run(0)
+    info, ms = _timed(lambda: run(0), 2)
+    info_f, ms_f = _timed(lambda: run(1), 2)
+    return {"workload": f"block CG s={s}, Poisson {m}^3, tol 1e-8, FP64",
+            "ms": round(ms, 2), "iters": info.ret, "true_res": info.true_res,
+            "ms_per_iter": round(ms / max(1, info.ret + 1), 3),
+            "fixed_p0": {"ms": round(ms_f, 2), "iters": info_f.ret, "true_res": info_f.true_res}}
+
+
+def bench_cfg3(dv, gen, m=96):
+    """Shifted BiCGSTAB(4), 96^3 convection-diffusion, 64 shifts, tol 1e-8 (seed)."""
+    px, py, pz = gen.cfg3_peclet()
+    n, rp, ci, v = gen.convdiff3d(m, px, py, pz)
+    b = gen.normal(303, n)
+    sig = 10.0 ** (-3 + 3 * np.arange(64) / 63.0)
+    M = dv.Matrix(n, rp, ci, v)
+    db, dx = dv.DeviceArray.from_numpy(b), dv.DeviceArray((n * 65,))
+    zeros = np.zeros(n * 65)
+
+    def run(**kw):
+        dx.upload(zeros)
+        return dv.shbicgstabl(M, db, 4, sig, dx, 1e-8, 1000, **kw)
+
+    run()
+    (info, tr), ms = _timed(run, 1)
+    (info32, tr32), ms32 = _timed(lambda: run(precision=dv.BKH_FP32_IR), 1)
+    return {"workload": f"shifted BiCGSTAB(4), convection-diffusion {m}^3, n={n}, 64 shifts, tol 1e-8",
+            "fp64_ms": round(ms, 2), "outer_iters": info.ret, "max_true_res": float(tr.max()),
+            "fp32_ir": {"ms": round(ms32, 2), "outer_iters": info32.ret, "max_true_res": float(tr32.max()),
+                        "speedup_vs_fp64": round(ms / ms32, 3)}}
+
+
+def bench_cfg4(dv, gen):
+    """BiCGSTAB(2) on 512 independent uneven systems, tol 1e-8: batched (one launch per
+    phase, autotuned layout) FP32-IR and FP64 against 512 sequential plain FP64 solves."""
+    systems = gen.cfg4_batch()
+    sizes = [sy[0] for sy in systems]
+    Bt = dv.Batch(systems, fp32=True)
+    tune_ms = Bt.tune(fp32=True)
+    b = gen.normal(405, Bt.N)
+    db, dx = dv.DeviceArray.from_numpy(b), dv.DeviceArray((Bt.N,))
+    zeros = np.zeros(Bt.N)
+
+    def run(**kw):
+        dx.upload(zeros)
+        return Bt.bicgstabl(db, 2, dx, 1e-8, 1000, **kw)
+
+    run(precision=dv.BKH_FP32_IR)
+    (i32, r32, t32), ms32 = _timed(lambda: run(precision=dv.BKH_FP32_IR), 2)
+    (i64, r64, t64), ms64 = _timed(lambda: run(), 2)
+    # the plain FP64 path: one bkh_bicgstabl per system, in sequence
+    mats, bs, xs = [], [], []
+    off = Bt.offsets
+    for q, sy in enumerate(systems):
+        mats.append(dv.Matrix(*sy))
+        bs.append(dv.DeviceArray.from_numpy(b[off[q]:off[q + 1]]))
+        xs.append(dv.DeviceArray((sy[0],)))
+    del systems
+    zq = [np.zeros(n) for n in sizes]
+
+    def seq():
+        worst = 0.0
+        for q in range(len(mats)):
+            xs[q].upload(zq[q])
+            inf = dv.bicgstabl(mats[q], bs[q], 2, xs[q], 1e-8, 1000)
+            worst = max(worst, inf.true_res)
+        return worst
+
+    worst_seq, ms_seq = _timed(seq, 1)
+    layout = Bt.layout()
+    return {"workload": f"512 uneven systems n in [2^10, 2^17], N={Bt.N}, nnz={Bt.nnz}, BiCGSTAB(2), tol 1e-8",
+            "sequential_fp64_ms": round(ms_seq, 1), "sequential_max_true_res": worst_seq,
+            "batched_fp64_ms": round(ms64, 1), "batched_fp64_max_true_res": float(t64.max()),
+            "batched_fp32_ir_ms": round(ms32, 1), "batched_fp32_ir_max_true_res": float(t32.max()),
+            "fp32_ir_passes": i32.passes, "converged": int((r32 >= 0).sum()),
+            "speedup_vs_sequential_fp64": round(ms_seq / ms32, 2),
+            "speedup_vs_batched_fp64": round(ms64 / ms32, 3),
+            "autotuned_layout": {"rows_per_wave": layout[0], "nnz_per_wave": layout[1],
+                                 "tile_rows": layout[2], "fp32_spmv_ms": round(tune_ms, 3)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -42,6 +174,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--m", type=int, default=128)
+    ap.add_argument("--configs", default="cfg1,cfg2b,cfg3,cfg4",
+                    help="extra workloads reported beside the main line ('' for none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -125,6 +259,23 @@ def main():
                          f"1 thread) on the same system, {per_it * 1e3:.1f} ms/iteration, "
                          f"scaled to the {iters} iterations the solve takes"}
 
+    extra = {}
+    want = [c for c in args.configs.split(",") if c]
+    if want:
+        po = None
+        if not args.no_cpu:
+            from oracle import pyoracle as po    # cpu_baseline legs only
+        del M, db, dx, dy, db32, dy32
+        for name, fn in (("cfg1", lambda: bench_cfg1(dv, gen, po)), ("cfg2_block", lambda: bench_cfg2_block(dv, gen)),
+                         ("cfg3", lambda: bench_cfg3(dv, gen)), ("cfg4", lambda: bench_cfg4(dv, gen))):
+            key = {"cfg2_block": "cfg2b"}.get(name, name)
+            if key in want:
+                try:
+                    extra[name] = fn()
+                except Exception as e:          # reported, never silently dropped
+                    extra[name] = {"error": repr(e)}
+                print(f"[bench] {name}: {json.dumps(extra[name])}", file=sys.stderr, flush=True)
+
     out = {
         "metric": METRIC, "value": round(wall_ms, 3), "unit": "ms", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall_ms, 3),
@@ -137,6 +288,7 @@ def main():
         "fp32_ir": {"ms": round(ir_wall, 3), "inner_iters": ir[-1].inner_iters, "passes": ir[-1].passes,
                     "true_res": ir[-1].true_res, "speedup_vs_fp64": round(wall_ms / ir_wall, 3)},
         "roofline": roofline, "cpu_baseline": cpu,
+        "configs": extra,
     }
     print(json.dumps(out))
 

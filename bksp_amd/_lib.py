@@ -87,6 +87,18 @@ _SIGS = {
                                 ctypes.POINTER(BkhInfo)]),
     "bkh_shbicgstabl": (c_int, [c_vp, c_vp, c_int, c_dp, c_int, c_vp, c_dbl, c_int,
                                 ctypes.POINTER(BkhOpts), ctypes.POINTER(BkhInfo), c_dp]),
+    # batches of independent systems
+    "bkh_batch_create": (c_vp, [c_int, c_ip, ctypes.POINTER(c_ip), ctypes.POINTER(c_ip),
+                                ctypes.POINTER(c_dp), c_int]),
+    "bkh_batch_destroy": (None, [c_vp]),
+    "bkh_batch_rows": (c_ll, [c_vp]),
+    "bkh_batch_nnz": (c_ll, [c_vp]),
+    "bkh_batch_offset": (c_ll, [c_vp, c_int]),
+    "bkh_batch_layout": (None, [c_vp, c_ip, c_ip, c_ip]),
+    "bkh_batch_tune": (c_int, [c_vp, c_int, c_dp]),
+    "bkh_batch_spmv": (c_int, [c_vp, c_int, c_vp, c_vp]),
+    "bkh_batch_bicgstabl": (c_int, [c_vp, c_vp, c_int, c_vp, c_dbl, c_int, ctypes.POINTER(BkhOpts), c_ip,
+                                    c_dp, ctypes.POINTER(BkhInfo)]),
     # bksp.h drop-in (host arrays)
     "bksp_dcg": (c_int, [ctypes.POINTER(DSpMat), c_dp, c_dbl, c_int, c_dp]),
     "dcr": (c_int, [ctypes.POINTER(DSpMat), c_dp, c_dbl, c_int, c_dp]),

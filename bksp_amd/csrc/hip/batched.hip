@@ -1,0 +1,814 @@
+// batched.hip -- many independent systems in one launch per phase, with an autotuner.
+//
+// The reference solves one system per call (src/bicgstabl.c:13-138); a batch of
+// uneven systems would be that many sequential solves, each a long chain of small
+// launches. Here the systems are stacked block-diagonally in one CSR (global row
+// offsets, column indices shifted by the system's offset) and every phase of
+// BiCGSTAB(l) is ONE launch over the whole batch:
+//   * SpMV units: groups of four wave blocks (<= rows_cap rows, <= nnz_cap nonzeros
+//     each) that never straddle two systems (each system's list is padded with empty
+//     blocks), so a unit's epilogue partial sums belong to one system;
+//   * vector tiles: row ranges of <= tile_rows rows inside one system;
+//   * per-system finalize: one workgroup per system sums that system's partials in
+//     a fixed order and runs its own scalar recurrence (alpha, beta, the MR step,
+//     the convergence test) in its BtSys record;
+//   * every unit and tile reads its system's `done` flag at entry, so converged
+//     systems cost nothing while the rest continue; a one-workgroup kernel at the end
+//     of each outer iteration stops the graph when no system is active.
+// With BKH_FP32_IR the iteration runs in FP32 on a correction, inside per-system
+// FP64 iterative refinement (true residual b - A x in FP64 for every system at once).
+// bkh_batch_tune() times candidate SpMV blockings (rows per wavefront, nonzeros per
+// wavefront) and vector tile sizes on the batch itself and keeps the fastest.
+#include <math.h>
+#include <algorithm>
+#include <vector>
+#include "bkh_host.hpp"
+#include "bkh_mr.hpp"
+#include "bkh_solver_common.hpp"
+#include "bkh_spmv.hpp"
+
+struct BtSys {
+    double g0, alpha, beta, oml, fnb, err, tol, rel;
+    double cx[MR_D], cr[MR_D], cp[MR_D];
+    int done, status, it, result, max_it, pad[3];
+};
+
+struct bkh_batch {
+    int ns = 0;
+    long long N = 0, nnz = 0;
+    std::vector<long long> off;     // row offsets, ns + 1
+    std::vector<int> rp_h;          // global row pointers (host), N + 1
+    int *rp = nullptr, *ci = nullptr;
+    double *val = nullptr;
+    float *val32 = nullptr;
+    int max_row = 0;
+    // SpMV units
+    int *wblk = nullptr, *usys = nullptr, *ub = nullptr;
+    int nunit = 0;
+    // vector tiles
+    int *trow = nullptr, *tsys = nullptr, *tb = nullptr;
+    int ntile = 0;
+    int rows_cap = SPMV_WROWS, nnz_cap = SPMV_WCAP, tile_rows = 1024;
+};
+
+// ------------------------------------------------------------------ kernels
+
+template <int NV>
+__device__ __forceinline__ void unit_partial(double (&v)[NV], double *part, int u) {
+    __shared__ double sh[NV * 4 > 0 ? NV * 4 : 1];
+    block_sum<NV, 256>(v, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; i++) part[(size_t)u * NV + i] = v[i];
+    }
+}
+
+// y = A x over the batch; epilogue dot products of y with the vectors in e, per unit.
+template <typename VT, typename T, int NV>
+__global__ __launch_bounds__(256) void k_bt_spmv(const int *__restrict__ rp, const int *__restrict__ ci,
+                                                 const VT *__restrict__ val, const int *__restrict__ wblk,
+                                                 const int *__restrict__ usys, const BtSys *sy, const T *x, T *y,
+                                                 Epi<NV, T> e, double *part) {
+    const int u = xcd_block(blockIdx.x, gridDim.x);
+    if (sy && sy[usys[u]].done) return;
+    __shared__ T prod_all[4][SPMV_WCAP];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    T *prod = prod_all[w];
+    const int b = u * 4 + w;
+    double dv[NV > 0 ? NV : 1];
+#pragma unroll
+    for (int i = 0; i < NV; i++) dv[i] = 0.0;
+    const int r0 = wblk[b], r1 = wblk[b + 1];
+    if (r1 > r0) {
+        const int k0 = rp[r0], k1 = rp[r1];
+        constexpr int VEC = 16 / sizeof(VT);
+        constexpr int STEPS = SPMV_WCAP / (VEC * 64) + 1;
+        typedef VT vvec __attribute__((ext_vector_type(VEC)));
+        typedef int ivec __attribute__((ext_vector_type(VEC)));
+        const int kb = k0 & ~(VEC - 1);
+        vvec vv[STEPS];
+        ivec cc[STEPS];
+#pragma unroll
+        for (int j = 0; j < STEPS; j++) {
+            const int k = kb + VEC * (lane + j * 64);
+            if (k < k1) {
+                vv[j] = *(const vvec *)(val + k);
+                cc[j] = *(const ivec *)(ci + k);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < STEPS; j++) {
+            const int k = kb + VEC * (lane + j * 64);
+#pragma unroll
+            for (int q = 0; q < VEC; q++)
+                if (k + q >= k0 && k + q < k1) prod[k + q - k0] = mul_rn((T)vv[j][q], x[cc[j][q]]);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int row = r0 + lane;
+        if (row < r1) {
+            const int s0 = rp[row] - k0, s1 = rp[row + 1] - k0;
+            T s = (T)0;
+            for (int k = s0; k < s1; k++) s = add_rn(s, prod[k]);
+            y[row] = s;
+#pragma unroll
+            for (int i = 0; i < NV; i++)
+                dv[i] = epi_operand(e, e.a[i], row, (double)s) * epi_operand(e, e.b[i], row, (double)s);
+        }
+    }
+    if constexpr (NV > 0) unit_partial<NV>(dv, part, u);
+}
+
+#define TILE_ROWS(i)                                                                  \
+    const int t_ = blockIdx.x;                                                        \
+    const int sys = tsys[t_];                                                         \
+    for (long long i = (long long)trow[t_] + threadIdx.x; i < trow[t_ + 1]; i += 256)
+
+// r0 = b - A x (A x in r0 on entry); p0 = rs = r0; partials {r0.r0, b.b}.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bt_init(const int *trow, const int *tsys, long long N, const T *b,
+                                                 T *R, T *P, T *rs, const BtSys *sy, double *part) {
+    double v[2] = {0.0, 0.0};
+    bool live = !sy[tsys[blockIdx.x]].done;
+    if (live) {
+        TILE_ROWS(i) {
+            const T ri = b[i] - R[i];
+            R[i] = ri; P[i] = ri; rs[i] = ri;
+            v[0] += (double)ri * (double)ri;
+            v[1] += (double)b[i] * (double)b[i];
+        }
+        (void)sys;
+    }
+    unit_partial<2>(v, part, blockIdx.x);
+}
+
+// p_q = beta p_q + r_q, q <= j (bicgstabl.c:64-67)
+template <typename T>
+__global__ __launch_bounds__(256) void k_bt_p(const int *trow, const int *tsys, long long N, int j, T *P,
+                                              const T *R, const BtSys *sy) {
+    if (sy[tsys[blockIdx.x]].done) return;
+    TILE_ROWS(i) {
+        const T be = (T)sy[sys].beta;
+        for (int q = 0; q <= j; q++) P[q * N + i] = add_rn(mul_rn(be, P[q * N + i]), R[q * N + i]);
+    }
+}
+
+// r_q -= alpha p_{q+1}, q <= j; x += alpha p_0 (:73-75, :79)
+template <typename T>
+__global__ __launch_bounds__(256) void k_bt_r(const int *trow, const int *tsys, long long N, int j, const T *P,
+                                              T *R, T *x, const BtSys *sy) {
+    if (sy[tsys[blockIdx.x]].done) return;
+    TILE_ROWS(i) {
+        const T al = (T)sy[sys].alpha;
+        for (int q = 0; q <= j; q++) R[q * N + i] = fma(-al, P[(q + 1) * N + i], R[q * N + i]);
+        x[i] = fma(al, P[i], x[i]);
+    }
+}
+
+// MR update of x, r_0, p_0 with partials {r0.r0, rs.r0} (:113-125)
+template <typename T>
+__global__ __launch_bounds__(256) void k_bt_upd(const int *trow, const int *tsys, long long N, int l, T *P, T *R,
+                                                T *x, const T *rs, const BtSys *sy, double *part) {
+    double v[2] = {0.0, 0.0};
+    const int sy_i = tsys[blockIdx.x];
+    if (!sy[sy_i].done) {
+        double cx[MR_D], cr[MR_D], cp[MR_D];
+        for (int k = 0; k <= l; k++) { cx[k] = sy[sy_i].cx[k]; cr[k] = sy[sy_i].cr[k]; cp[k] = sy[sy_i].cp[k]; }
+        TILE_ROWS(i) {
+            double xi = (double)x[i], r0 = (double)R[i], p0 = (double)P[i];
+            xi += cx[0] * r0;
+            for (int k = 1; k <= l; k++) {
+                const double rk = (double)R[k * N + i];
+                if (k < l) xi += cx[k] * rk;
+                r0 -= cr[k] * rk;
+                p0 -= cp[k] * (double)P[k * N + i];
+            }
+            x[i] = (T)xi; R[i] = (T)r0; P[i] = (T)p0;
+            const double r0t = (double)(T)r0;
+            v[0] += r0t * r0t;
+            v[1] += (double)rs[i] * r0t;
+        }
+        (void)sys;
+    }
+    unit_partial<2>(v, part, blockIdx.x);
+}
+
+enum { BT_INIT = 0, BT_ALPHA = 1, BT_GAMMA = 2, BT_MR = 3, BT_UPD = 4, BT_RESID = 5 };
+
+__device__ __forceinline__ void bt_stop(BtSys &s, int status) {
+    s.done = 1; s.status = status; s.result = s.it;
+}
+
+// One workgroup per system: the system's partial rows [rng[s], rng[s+1]) summed in a
+// fixed order, then its scalar step (the same recurrences as the single-system path).
+template <int OP, int NV>
+__global__ __launch_bounds__(256) void k_bt_fin(const double *part, const int *rng, BtSys *sy, int l) {
+    BtSys &S = sy[blockIdx.x];
+    if (OP != BT_RESID && S.done) return;
+    __shared__ double sh[NV * 4];
+    double acc[NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++) acc[i] = 0.0;
+    for (int r = rng[blockIdx.x] + threadIdx.x; r < rng[blockIdx.x + 1]; r += 256)
+#pragma unroll
+        for (int i = 0; i < NV; i++) acc[i] += part[(size_t)r * NV + i];
+    block_sum<NV, 256>(acc, sh);
+    if (threadIdx.x != 0) return;
+    if (OP == BT_RESID) {                         // FP64 true residual: ||r||, ||b||, ratio
+        S.err = sqrt(acc[0]);
+        S.fnb = sqrt(acc[1]);
+        S.rel = acc[1] > 0.0 ? sqrt(acc[0] / acc[1]) : 0.0;
+        return;
+    }
+    if (OP == BT_INIT) {
+        S.fnb = sqrt(acc[1]);
+        if (!(S.fnb > 0.0)) { bt_stop(S, ST_BREAKDOWN); return; }
+        if (acc[0] == 0.0) { S.done = 1; S.status = ST_CONVERGED; S.result = 0; return; }
+        S.alpha = 0.0; S.beta = 0.0; S.g0 = acc[0];
+        return;
+    }
+    if (OP == BT_ALPHA) {
+        if (bad(acc[0])) { bt_stop(S, ST_BREAKDOWN); return; }
+        S.alpha = S.g0 / acc[0];
+        return;
+    }
+    if (OP == BT_GAMMA) {
+        if (bad(S.g0)) { bt_stop(S, ST_BREAKDOWN); return; }
+        S.beta = -S.alpha * acc[0] / S.g0;
+        S.g0 = acc[0];
+        return;
+    }
+    if (OP == BT_MR) {
+        double G[MR_D][MR_D];
+        gram_unpack(l, acc, G);
+        if (G[0][0] == 0.0) { S.done = 1; S.status = ST_CONVERGED; S.result = S.it; return; }
+        MrOut o;
+        if (!mr_from_gram(l, G, o)) { bt_stop(S, ST_BREAKDOWN); return; }
+        S.cx[0] = o.om[1];
+        for (int k = 1; k < l; k++) {
+            double t = 0.0;
+            for (int i = k; i < l; i++) t += o.odd[i] * o.C[i][k];
+            S.cx[k] = t;
+        }
+        for (int k = 1; k <= l; k++) {
+            double t = 0.0;
+            for (int i = k; i <= l; i++) t += o.od[i] * o.C[i][k];
+            S.cr[k] = t;
+            S.cp[k] = o.om[k];
+        }
+        S.oml = o.om[l];
+        return;
+    }
+    // BT_UPD: error = ||r0|| / ||b|| (:122-125), then the next first beta (:57-62)
+    const double err = sqrt(acc[0]) / S.fnb;
+    S.err = err;
+    const int it = S.it;
+    S.it = it + 1;
+    if (err < S.tol) { S.done = 1; S.status = ST_CONVERGED; S.result = it; return; }
+    if (!(err == err) || err > 1.0e300) { S.done = 1; S.status = ST_BREAKDOWN; S.result = it; return; }
+    if (it + 1 >= S.max_it) { S.done = 1; S.status = ST_MAXIT; S.result = it; return; }
+    const double g0 = -S.oml * S.g0;
+    if (bad(g0)) { bt_stop(S, ST_BREAKDOWN); return; }
+    S.beta = -S.alpha * acc[1] / g0;
+    S.g0 = acc[1];
+}
+
+// Stops the graph once no system is active.
+__global__ __launch_bounds__(256) void k_bt_alldone(const BtSys *sy, int ns, Ctl *c) {
+    __shared__ int cnt[256];
+    int a = 0;
+    for (int s = threadIdx.x; s < ns; s += 256) a += sy[s].done ? 0 : 1;
+    cnt[threadIdx.x] = a;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) cnt[threadIdx.x] += cnt[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        c->it += 1;
+        if (cnt[0] == 0) { c->done = 1; c->status = ST_CONVERGED; }
+        else if (c->it >= c->max_it) { c->done = 1; c->status = ST_MAXIT; }
+    }
+}
+
+// Per-system control at the start of a (pass of a) solve.
+__global__ void k_bt_sys_init(BtSys *sy, int ns, const double *tol_each, double tol, const int *frozen,
+                              int max_it) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    BtSys &S = sy[s];
+    S.tol = tol_each ? tol_each[s] : tol;
+    S.max_it = max_it;
+    S.it = 0; S.result = 0; S.err = 0.0;
+    S.alpha = 0.0; S.beta = 0.0; S.g0 = 0.0; S.oml = 1.0;
+    const bool fz = frozen && frozen[s];
+    S.done = (fz || max_it <= 0) ? 1 : 0;
+    S.status = fz ? ST_CONVERGED : (max_it <= 0 ? ST_MAXIT : ST_RUNNING);
+}
+
+// r = b - r (A x in r), partials {r.r, b.b} (FP64 true residual of every system).
+__global__ __launch_bounds__(256) void k_bt_resid(const int *trow, const int *tsys, const double *b, double *r,
+                                                  double *part) {
+    double v[2] = {0.0, 0.0};
+    TILE_ROWS(i) {
+        const double ri = b[i] - r[i];
+        r[i] = ri;
+        v[0] += ri * ri;
+        v[1] += b[i] * b[i];
+    }
+    (void)sys;
+    unit_partial<2>(v, part, blockIdx.x);
+}
+
+// r32 = r64 (scaled by 1/||r_s|| so every inner system starts at unit norm), d = 0.
+__global__ __launch_bounds__(256) void k_bt_ir_prep(const int *trow, const int *tsys, const double *r64,
+                                                    float *r32, float *d, const double *scale) {
+    TILE_ROWS(i) {
+        r32[i] = (float)(r64[i] * scale[sys]);
+        d[i] = 0.0f;
+    }
+}
+
+// x += d / scale_s (FP64)
+__global__ __launch_bounds__(256) void k_bt_ir_update(const int *trow, const int *tsys, double *x, const float *d,
+                                                      const double *scale) {
+    TILE_ROWS(i) x[i] += (double)d[i] / scale[sys];
+}
+
+// --------------------------------------------------------------- building
+
+static void bt_free(bkh_batch *B) {
+    hipFree(B->rp); hipFree(B->ci); hipFree(B->val); hipFree(B->val32);
+    hipFree(B->wblk); hipFree(B->usys); hipFree(B->ub);
+    hipFree(B->trow); hipFree(B->tsys); hipFree(B->tb);
+}
+
+static void bt_free_layout(bkh_batch *B) {
+    hipFree(B->wblk); hipFree(B->usys); hipFree(B->ub);
+    hipFree(B->trow); hipFree(B->tsys); hipFree(B->tb);
+    B->wblk = B->usys = B->ub = B->trow = B->tsys = B->tb = nullptr;
+}
+
+// SpMV units and vector tiles for the given caps.
+static int bt_layout(bkh_batch *B, int rows_cap, int nnz_cap, int tile_rows) {
+    std::vector<int> wb, us, ub, tr, ts, tb;
+    wb.push_back(0);
+    ub.push_back(0);
+    tb.push_back(0);
+    tr.push_back(0);
+    for (int s = 0; s < B->ns; s++) {
+        const int a = (int)B->off[s], e = (int)B->off[s + 1];
+        int nb = 0, r = a;
+        while (r < e) {
+            int start = r, nz = 0;
+            while (r < e && r - start < rows_cap) {
+                const int l = B->rp_h[r + 1] - B->rp_h[r];
+                if (nz + l > nnz_cap && r > start) break;
+                nz += l;
+                r++;
+            }
+            wb.push_back(r);
+            nb++;
+        }
+        while (nb % 4) { wb.push_back(e); nb++; }
+        for (int q = 0; q < nb / 4; q++) us.push_back(s);
+        ub.push_back((int)us.size());
+        for (int r0 = a; r0 < e; r0 += tile_rows) {
+            tr.push_back(std::min(e, r0 + tile_rows));
+            ts.push_back(s);
+        }
+        tb.push_back((int)ts.size());
+    }
+    bt_free_layout(B);
+    B->nunit = (int)us.size();
+    B->ntile = (int)ts.size();
+    bool ok = hipMalloc(&B->wblk, sizeof(int) * wb.size()) == hipSuccess &&
+              hipMalloc(&B->usys, sizeof(int) * std::max<size_t>(us.size(), 1)) == hipSuccess &&
+              hipMalloc(&B->ub, sizeof(int) * ub.size()) == hipSuccess &&
+              hipMalloc(&B->trow, sizeof(int) * tr.size()) == hipSuccess &&
+              hipMalloc(&B->tsys, sizeof(int) * std::max<size_t>(ts.size(), 1)) == hipSuccess &&
+              hipMalloc(&B->tb, sizeof(int) * tb.size()) == hipSuccess;
+    ok = ok && hipMemcpy(B->wblk, wb.data(), sizeof(int) * wb.size(), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(B->usys, us.data(), sizeof(int) * us.size(), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(B->ub, ub.data(), sizeof(int) * ub.size(), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(B->trow, tr.data(), sizeof(int) * tr.size(), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(B->tsys, ts.data(), sizeof(int) * ts.size(), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(B->tb, tb.data(), sizeof(int) * tb.size(), hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) { bkh_set_error("batch layout upload failed"); return BKH_DEVICE_ERROR; }
+    B->rows_cap = rows_cap;
+    B->nnz_cap = nnz_cap;
+    B->tile_rows = tile_rows;
+    return 0;
+}
+
+extern "C" bkh_batch *bkh_batch_create(int ns, const int *n_each, const int *const *row_ptr,
+                                       const int *const *col, const double *const *val, int want_fp32) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return nullptr;
+    if (ns < 1 || !n_each || !row_ptr || !col || !val) { bkh_set_error("bad batch arguments"); return nullptr; }
+    bkh_batch *B = new bkh_batch;
+    B->ns = ns;
+    B->off.assign(ns + 1, 0);
+    for (int s = 0; s < ns; s++) {
+        if (n_each[s] < 1 || !row_ptr[s] || row_ptr[s][0] != 0) { bkh_set_error("system %d: bad CSR", s); delete B; return nullptr; }
+        B->off[s + 1] = B->off[s] + n_each[s];
+        B->nnz += row_ptr[s][n_each[s]];
+    }
+    B->N = B->off[ns];
+    if (B->N >= (1LL << 31) || B->nnz >= (1LL << 31) - 8) { bkh_set_error("batch exceeds int32 indexing"); delete B; return nullptr; }
+    B->rp_h.assign(B->N + 1, 0);
+    std::vector<int> ci((size_t)B->nnz + 4, 0);
+    std::vector<double> v((size_t)B->nnz + 4, 0.0);
+    long long k = 0;
+    for (int s = 0; s < ns; s++) {
+        const long long o = B->off[s];
+        const int n = n_each[s];
+        for (int i = 0; i < n; i++) {
+            const int a = row_ptr[s][i], e = row_ptr[s][i + 1];
+            if (e < a) { bkh_set_error("system %d: row pointers decrease", s); delete B; return nullptr; }
+            B->max_row = std::max(B->max_row, e - a);
+            for (int q = a; q < e; q++) {
+                const int c = col[s][q];
+                if (c < 0 || c >= n) { bkh_set_error("system %d: column out of range", s); delete B; return nullptr; }
+                ci[k] = (int)(c + o);
+                v[k] = val[s][q];
+                k++;
+            }
+            B->rp_h[o + i + 1] = (int)k;
+        }
+    }
+    if (B->max_row > SPMV_WCAP) { bkh_set_error("batched SpMV supports rows of <= %d nonzeros", SPMV_WCAP); delete B; return nullptr; }
+    const size_t nz = (size_t)B->nnz + 4;
+    bool ok = hipMalloc(&B->rp, sizeof(int) * (B->N + 1)) == hipSuccess &&
+              hipMalloc(&B->ci, sizeof(int) * nz) == hipSuccess &&
+              hipMalloc(&B->val, sizeof(double) * nz) == hipSuccess;
+    if (ok && want_fp32) ok = hipMalloc(&B->val32, sizeof(float) * nz) == hipSuccess;
+    ok = ok && hipMemcpy(B->rp, B->rp_h.data(), sizeof(int) * (B->N + 1), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(B->ci, ci.data(), sizeof(int) * nz, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(B->val, v.data(), sizeof(double) * nz, hipMemcpyHostToDevice) == hipSuccess;
+    if (ok && want_fp32) {
+        std::vector<float> v32(nz);
+        for (size_t q = 0; q < nz; q++) v32[q] = (float)v[q];
+        ok = hipMemcpy(B->val32, v32.data(), sizeof(float) * nz, hipMemcpyHostToDevice) == hipSuccess;
+    }
+    if (!ok || bt_layout(B, SPMV_WROWS, SPMV_WCAP, 1024)) {
+        if (!ok) bkh_set_error("batch upload failed");
+        bt_free(B);
+        delete B;
+        return nullptr;
+    }
+    return B;
+}
+
+extern "C" void bkh_batch_destroy(bkh_batch *B) {
+    if (!B) return;
+    bt_free(B);
+    delete B;
+}
+
+extern "C" long long bkh_batch_rows(const bkh_batch *B) { return B ? B->N : 0; }
+extern "C" long long bkh_batch_nnz(const bkh_batch *B) { return B ? B->nnz : 0; }
+extern "C" long long bkh_batch_offset(const bkh_batch *B, int s) {
+    return (B && s >= 0 && s <= B->ns) ? B->off[s] : -1;
+}
+extern "C" void bkh_batch_layout(const bkh_batch *B, int *rows_cap, int *nnz_cap, int *tile_rows) {
+    if (!B) return;
+    if (rows_cap) *rows_cap = B->rows_cap;
+    if (nnz_cap) *nnz_cap = B->nnz_cap;
+    if (tile_rows) *tile_rows = B->tile_rows;
+}
+
+// ----------------------------------------------------------------- launches
+
+template <typename VT, typename T, int NV>
+static void bt_spmv(hipStream_t s, const bkh_batch *B, const VT *val, const BtSys *sy, const T *x, T *y,
+                    const Epi<NV, T> &e, double *part) {
+    hipLaunchKernelGGL((k_bt_spmv<VT, T, NV>), dim3(B->nunit), dim3(256), 0, s, B->rp, B->ci, val, B->wblk,
+                       B->usys, sy, x, y, e, part);
+}
+
+template <int OP, int NV>
+static void bt_fin(hipStream_t s, const bkh_batch *B, const double *part, bool units, BtSys *sy, int l) {
+    hipLaunchKernelGGL((k_bt_fin<OP, NV>), dim3(B->ns), dim3(256), 0, s, part, units ? B->ub : B->tb, sy, l);
+}
+
+template <typename VT, typename T>
+static void bt_dot_spmv(hipStream_t s, const bkh_batch *B, const VT *val, BtSys *sy, const T *x, T *y,
+                        const T *w, double *part) {
+    Epi<1, T> e;
+    memset(&e, 0, sizeof e);
+    e.w[0] = w;
+    e.a[0] = 0;
+    e.b[0] = EPI_Y;
+    bt_spmv<VT, T, 1>(s, B, val, sy, x, y, e, part);
+}
+
+template <int L, typename VT, typename T>
+static void bt_gram_spmv(hipStream_t s, const bkh_batch *B, const VT *val, BtSys *sy, T *R, double *part) {
+    constexpr int NV = (L + 1) * (L + 2) / 2;
+    const long long N = B->N;
+    Epi<NV, T> e;
+    memset(&e, 0, sizeof e);
+    for (int k = 0; k < L; k++) e.w[k] = R + k * N;
+    int idx = 0;
+    for (int a = 0; a <= L; a++)
+        for (int b = a; b <= L; b++) {
+            e.a[idx] = (unsigned char)(a == L ? EPI_Y : a);
+            e.b[idx] = (unsigned char)(b == L ? EPI_Y : b);
+            idx++;
+        }
+    bt_spmv<VT, T, NV>(s, B, val, sy, R + (L - 1) * N, R + L * N, e, part);
+    bt_fin<BT_MR, NV>(s, B, part, true, sy, L);
+}
+
+// One batched BiCGSTAB(L) solve (all systems) on vectors T; sy initialised by caller.
+template <int L, typename VT, typename T>
+static int bt_core(ThreadCtx *t, const bkh_batch *B, const VT *val, const T *b, T *x, T *R, T *P, T *rs,
+                   BtSys *sy, Ctl *c, double *part, int max_it, int K) {
+    hipStream_t s = t->stream;
+    const long long N = B->N;
+    if (ctl_init(t, c, 0.0, max_it)) return BKH_DEVICE_ERROR;
+    Epi<0, T> e0;
+    memset(&e0, 0, sizeof e0);
+    bt_spmv<VT, T, 0>(s, B, val, sy, x, R, e0, nullptr);
+    hipLaunchKernelGGL(k_bt_init<T>, dim3(B->ntile), dim3(256), 0, s, B->trow, B->tsys, N, b, R, P, rs,
+                       (const BtSys *)sy, part);
+    bt_fin<BT_INIT, 2>(s, B, part, false, sy, L);
+    LAUNCH_OK();
+    hipGraphExec_t ge = capture_graph(s, K, [&]() {
+        for (int j = 0; j < L; j++) {
+            hipLaunchKernelGGL(k_bt_p<T>, dim3(B->ntile), dim3(256), 0, s, B->trow, B->tsys, N, j, P, (const T *)R,
+                               (const BtSys *)sy);
+            bt_dot_spmv<VT, T>(s, B, val, sy, P + j * N, P + (j + 1) * N, rs, part);
+            bt_fin<BT_ALPHA, 1>(s, B, part, true, sy, L);
+            hipLaunchKernelGGL(k_bt_r<T>, dim3(B->ntile), dim3(256), 0, s, B->trow, B->tsys, N, j, (const T *)P, R,
+                               x, (const BtSys *)sy);
+            if (j < L - 1) {
+                bt_dot_spmv<VT, T>(s, B, val, sy, R + j * N, R + (j + 1) * N, rs, part);
+                bt_fin<BT_GAMMA, 1>(s, B, part, true, sy, L);
+            } else {
+                bt_gram_spmv<L, VT, T>(s, B, val, sy, R, part);
+            }
+        }
+        hipLaunchKernelGGL(k_bt_upd<T>, dim3(B->ntile), dim3(256), 0, s, B->trow, B->tsys, N, L, P, R, x,
+                           (const T *)rs, (const BtSys *)sy, part);
+        bt_fin<BT_UPD, 2>(s, B, part, false, sy, L);
+        hipLaunchKernelGGL(k_bt_alldone, dim3(1), dim3(256), 0, s, (const BtSys *)sy, B->ns, c);
+    });
+    if (!ge) { bkh_set_error("hipGraph capture of the batched iteration failed"); return BKH_DEVICE_ERROR; }
+    int rc = run_graph_until_done(t, ge, c, max_it / K + 2);
+    hipGraphExecDestroy(ge);
+    return rc;
+}
+
+// FP64 true residual of every system into sy[s].rel (r64 = b - A x).
+static int bt_true_res(ThreadCtx *t, const bkh_batch *B, const double *b, const double *x, double *r64,
+                       BtSys *sy, double *part) {
+    hipStream_t s = t->stream;
+    Epi<0, double> e0;
+    memset(&e0, 0, sizeof e0);
+    bt_spmv<double, double, 0>(s, B, B->val, nullptr, x, r64, e0, nullptr);
+    hipLaunchKernelGGL(k_bt_resid, dim3(B->ntile), dim3(256), 0, s, B->trow, B->tsys, b, r64, part);
+    bt_fin<BT_RESID, 2>(s, B, part, false, sy, 0);
+    LAUNCH_OK();
+    return 0;
+}
+
+template <int L>
+static int bt_solve(ThreadCtx *t, const bkh_batch *B, const double *b, double *x, double tol, int max_iter,
+                    const bkh_opts &o, int *ret, double *true_res, bkh_info *inf) {
+    hipStream_t s = t->stream;
+    const long long N = B->N;
+    const int ns = B->ns;
+    constexpr int NG = (L + 1) * (L + 2) / 2;
+    const size_t nparts = (size_t)std::max((long long)B->nunit * NG, (long long)B->ntile * 2);
+    const int K = o.graph_iters > 0 ? o.graph_iters : 4;
+    const bool ir = o.precision == BKH_FP32_IR;
+    if (ir && !B->val32) { bkh_set_error("FP32_IR needs a batch created with want_fp32"); return BKH_BAD_ARGUMENT; }
+    const size_t vbytes = ir ? sizeof(float) : sizeof(double);
+    size_t bytes = carve_size(sizeof(Ctl)) + carve_size(sizeof(BtSys) * ns) + carve_size(sizeof(double) * nparts) +
+                   carve_size(sizeof(double) * ns) + carve_size(sizeof(int) * ns) + carve_size(sizeof(double) * ns) +
+                   carve_size(sizeof(double) * N) + (2 * (L + 1) + 3) * carve_size(vbytes * N);
+    void *ws = bkh_workspace(bytes);
+    if (!ws) return BKH_DEVICE_ERROR;
+    Carve cv(ws);
+    Ctl *c = cv.take<Ctl>(1);
+    BtSys *sy = cv.take<BtSys>(ns);
+    double *part = cv.take<double>(nparts);
+    double *tol_d = cv.take<double>(ns);
+    int *frozen_d = cv.take<int>(ns);
+    double *scale_d = cv.take<double>(ns);
+    double *r64 = cv.take<double>(N);
+    std::vector<BtSys> hs(ns);
+    std::vector<int> total_it(ns, 0);
+    HIP_OK(hipEventRecord(t->ev0, s));
+    int rc = 0, passes = 0;
+    if (!ir) {
+        double *R = cv.take<double>((L + 1) * N), *P = cv.take<double>((L + 1) * N), *rs = cv.take<double>(N);
+        hipLaunchKernelGGL(k_bt_sys_init, dim3((ns + 255) / 256), dim3(256), 0, s, sy, ns, (const double *)nullptr,
+                           tol, (const int *)nullptr, max_iter);
+        rc = bt_core<L, double, double>(t, B, B->val, b, x, R, P, rs, sy, c, part, max_iter, K);
+        if (rc) return rc;
+        HIP_OK(hipMemcpyAsync(hs.data(), sy, sizeof(BtSys) * ns, hipMemcpyDeviceToHost, s)); HIP_OK(hipStreamSynchronize(s));
+        for (int q = 0; q < ns; q++) {
+            ret[q] = hs[q].status == ST_CONVERGED ? hs[q].result : (hs[q].status == ST_BREAKDOWN ? -2 : -1);
+            total_it[q] = hs[q].it;
+        }
+        passes = 1;
+    } else {
+        float *R = cv.take<float>((L + 1) * N), *P = cv.take<float>((L + 1) * N), *rs = cv.take<float>(N);
+        float *r32 = cv.take<float>(N), *d32 = cv.take<float>(N);
+        const double itol_min = o.inner_tol > 0.0 ? o.inner_tol : 1e-5;
+        const int max_pass = o.max_inner > 0 ? o.max_inner : 12;
+        std::vector<double> tol_h(ns), scale_h(ns);
+        std::vector<int> frozen_h(ns), done_h(ns, 0);
+        for (;;) {
+            // FP64 true residual of every system
+            hipLaunchKernelGGL(k_bt_sys_init, dim3((ns + 255) / 256), dim3(256), 0, s, sy, ns,
+                               (const double *)nullptr, tol, (const int *)nullptr, 1);
+            if ((rc = bt_true_res(t, B, b, x, r64, sy, part))) return rc;
+            HIP_OK(hipMemcpyAsync(hs.data(), sy, sizeof(BtSys) * ns, hipMemcpyDeviceToHost, s)); HIP_OK(hipStreamSynchronize(s));
+            int active = 0;
+            for (int q = 0; q < ns; q++) {
+                const double rel = hs[q].rel;
+                const bool conv = rel < tol;
+                const bool spent = total_it[q] >= max_iter || done_h[q] < 0;
+                frozen_h[q] = (conv || spent) ? 1 : 0;
+                if (conv) ret[q] = total_it[q] > 0 ? total_it[q] - 1 : 0;
+                else if (spent) ret[q] = (done_h[q] == -2) ? -2 : -1;
+                if (true_res) true_res[q] = rel;
+                // inner system scaled to unit norm; relative inner target
+                scale_h[q] = hs[q].err > 0.0 ? 1.0 / hs[q].err : 1.0;
+                tol_h[q] = std::max(itol_min, 0.5 * tol / std::max(rel, 1e-300));
+                active += frozen_h[q] ? 0 : 1;
+            }
+            if (active == 0 || passes >= max_pass) {
+                for (int q = 0; q < ns; q++)
+                    if (!frozen_h[q]) ret[q] = -1;
+                break;
+            }
+            HIP_OK(hipMemcpyAsync(tol_d, tol_h.data(), sizeof(double) * ns, hipMemcpyHostToDevice, s)); HIP_OK(hipStreamSynchronize(s));
+            HIP_OK(hipMemcpyAsync(frozen_d, frozen_h.data(), sizeof(int) * ns, hipMemcpyHostToDevice, s)); HIP_OK(hipStreamSynchronize(s));
+            HIP_OK(hipMemcpyAsync(scale_d, scale_h.data(), sizeof(double) * ns, hipMemcpyHostToDevice, s)); HIP_OK(hipStreamSynchronize(s));
+            hipLaunchKernelGGL(k_bt_ir_prep, dim3(B->ntile), dim3(256), 0, s, B->trow, B->tsys, (const double *)r64,
+                               r32, d32, (const double *)scale_d);
+            hipLaunchKernelGGL(k_bt_sys_init, dim3((ns + 255) / 256), dim3(256), 0, s, sy, ns, (const double *)tol_d,
+                               0.0, (const int *)frozen_d, max_iter);
+            rc = bt_core<L, float, float>(t, B, B->val32, r32, d32, R, P, rs, sy, c, part, max_iter, K);
+            if (rc) return rc;
+            HIP_OK(hipMemcpyAsync(hs.data(), sy, sizeof(BtSys) * ns, hipMemcpyDeviceToHost, s)); HIP_OK(hipStreamSynchronize(s));
+            for (int q = 0; q < ns; q++) {
+                if (frozen_h[q]) continue;
+                total_it[q] += hs[q].it;
+                if (hs[q].status == ST_BREAKDOWN && hs[q].it == 0) done_h[q] = -2;
+            }
+            hipLaunchKernelGGL(k_bt_ir_update, dim3(B->ntile), dim3(256), 0, s, B->trow, B->tsys, x,
+                               (const float *)d32, (const double *)scale_d);
+            LAUNCH_OK();
+            passes++;
+        }
+    }
+    HIP_OK(hipEventRecord(t->ev1, s));
+    HIP_OK(hipEventSynchronize(t->ev1));
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, t->ev0, t->ev1));
+    bkh_note_solve_ms(ms);
+    if (!ir && true_res) {
+        if ((rc = bt_true_res(t, B, b, x, r64, sy, part))) return rc;
+        HIP_OK(hipMemcpyAsync(hs.data(), sy, sizeof(BtSys) * ns, hipMemcpyDeviceToHost, s)); HIP_OK(hipStreamSynchronize(s));
+        for (int q = 0; q < ns; q++) true_res[q] = hs[q].rel;
+    }
+    int worst = 0, conv = 0, maxit_ = 0;
+    for (int q = 0; q < ns; q++) {
+        if (ret[q] >= 0) { conv++; maxit_ = std::max(maxit_, ret[q]); }
+        else worst = std::min(worst, ret[q]);
+    }
+    if (inf) {
+        memset(inf, 0, sizeof *inf);
+        inf->time_ms = ms;
+        inf->iters = maxit_;
+        inf->passes = passes;
+        int mx = 0;
+        for (int q = 0; q < ns; q++) mx = std::max(mx, total_it[q]);
+        inf->inner_iters = mx;
+        inf->status = conv == ns ? ST_CONVERGED : (worst == -2 ? ST_BREAKDOWN : ST_MAXIT);
+        inf->ret = conv == ns ? maxit_ : worst;
+        if (true_res) {
+            double mxr = 0.0;
+            for (int q = 0; q < ns; q++) mxr = std::max(mxr, true_res[q]);
+            inf->true_res = mxr;
+        }
+    }
+    return conv == ns ? maxit_ : worst;
+}
+
+extern "C" int bkh_batch_bicgstabl(bkh_batch *B, const double *d_b, int l, double *d_x, double tol, int max_iter,
+                                   const bkh_opts *opts, int *ret, double *true_res, bkh_info *info) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (!B || !d_b || !d_x || !ret) return BKH_BAD_ARGUMENT;
+    if (l < 1 || l > 4) { bkh_set_error("batched BiCGSTAB(l) supports 1 <= l <= 4"); return -3; }
+    bkh_opts o = opts ? *opts : bkh_opts{};
+    switch (l) {
+    case 1: return bt_solve<1>(t, B, d_b, d_x, tol, max_iter, o, ret, true_res, info);
+    case 2: return bt_solve<2>(t, B, d_b, d_x, tol, max_iter, o, ret, true_res, info);
+    case 3: return bt_solve<3>(t, B, d_b, d_x, tol, max_iter, o, ret, true_res, info);
+    default: return bt_solve<4>(t, B, d_b, d_x, tol, max_iter, o, ret, true_res, info);
+    }
+}
+
+// y = A x over the whole batch (FP64, or FP32 values and vectors).
+extern "C" int bkh_batch_spmv(const bkh_batch *B, int fp32, const void *x, void *y) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (!B || !x || !y || (fp32 && !B->val32)) return BKH_BAD_ARGUMENT;
+    hipStream_t s = t->stream;
+    if (fp32) {
+        Epi<0, float> e;
+        memset(&e, 0, sizeof e);
+        bt_spmv<float, float, 0>(s, B, B->val32, nullptr, (const float *)x, (float *)y, e, nullptr);
+    } else {
+        Epi<0, double> e;
+        memset(&e, 0, sizeof e);
+        bt_spmv<double, double, 0>(s, B, B->val, nullptr, (const double *)x, (double *)y, e, nullptr);
+    }
+    LAUNCH_OK();
+    HIP_OK(hipStreamSynchronize(s));
+    return 0;
+}
+
+// Autotuner: times the batched SpMV for candidate blockings (rows per wavefront,
+// nonzeros per wavefront) and a vector kernel for candidate tile sizes, on this
+// batch, and keeps the fastest layout. fp32 selects the FP32 SpMV (the inner
+// iteration of BKH_FP32_IR). best_ms receives the chosen SpMV's time.
+extern "C" int bkh_batch_tune(bkh_batch *B, int fp32, double *best_ms) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (!B || (fp32 && !B->val32)) return BKH_BAD_ARGUMENT;
+    hipStream_t s = t->stream;
+    const long long N = B->N;
+    const size_t vb = fp32 ? sizeof(float) : sizeof(double);
+    void *ws = bkh_workspace(4 * carve_size(vb * N) + carve_size(sizeof(BtSys) * B->ns));
+    if (!ws) return BKH_DEVICE_ERROR;
+    Carve cv(ws);
+    char *x = cv.take<char>(vb * N), *y = cv.take<char>(vb * N), *z = cv.take<char>(2 * vb * N);
+    BtSys *sy = cv.take<BtSys>(B->ns);
+    HIP_OK(hipMemsetAsync(x, 0, vb * N, s));
+    HIP_OK(hipMemsetAsync(z, 0, 2 * vb * N, s));
+    HIP_OK(hipMemsetAsync(sy, 0, sizeof(BtSys) * B->ns, s));
+    const int cand[][2] = {{64, 512}, {64, 256}, {32, 512}, {32, 256}, {16, 256}, {64, 384}};
+    const int tiles[] = {512, 1024, 2048, 4096};
+    const int reps = 5;
+    double best = 1e300;
+    int bi = 0;
+    auto time_spmv = [&]() -> double {
+        for (int r = 0; r < reps + 1; r++) {
+            if (r == 1) hipEventRecord(t->ev0, s);
+            if (fp32) {
+                Epi<0, float> e;
+                memset(&e, 0, sizeof e);
+                bt_spmv<float, float, 0>(s, B, B->val32, nullptr, (const float *)x, (float *)y, e, nullptr);
+            } else {
+                Epi<0, double> e;
+                memset(&e, 0, sizeof e);
+                bt_spmv<double, double, 0>(s, B, B->val, nullptr, (const double *)x, (double *)y, e, nullptr);
+            }
+        }
+        hipEventRecord(t->ev1, s);
+        hipEventSynchronize(t->ev1);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, t->ev0, t->ev1);
+        return ms / reps;
+    };
+    for (int q = 0; q < (int)(sizeof cand / sizeof cand[0]); q++) {
+        if (bt_layout(B, cand[q][0], cand[q][1], B->tile_rows)) return BKH_DEVICE_ERROR;
+        const double ms = time_spmv();
+        if (ms < best) { best = ms; bi = q; }
+    }
+    if (bt_layout(B, cand[bi][0], cand[bi][1], B->tile_rows)) return BKH_DEVICE_ERROR;
+    if (best_ms) *best_ms = best;
+    // vector tiles: time the BiCG update kernel (r -= alpha p, x += alpha p) per size
+    double bt = 1e300;
+    int ti = 1;
+    for (int q = 0; q < (int)(sizeof tiles / sizeof tiles[0]); q++) {
+        if (bt_layout(B, cand[bi][0], cand[bi][1], tiles[q])) return BKH_DEVICE_ERROR;
+        for (int r = 0; r < reps + 1; r++) {
+            if (r == 1) hipEventRecord(t->ev0, s);
+            if (fp32)
+                hipLaunchKernelGGL(k_bt_r<float>, dim3(B->ntile), dim3(256), 0, s, B->trow, B->tsys, N, 0,
+                                   (const float *)z, (float *)y, (float *)x, (const BtSys *)sy);
+            else
+                hipLaunchKernelGGL(k_bt_r<double>, dim3(B->ntile), dim3(256), 0, s, B->trow, B->tsys, N, 0,
+                                   (const double *)z, (double *)y, (double *)x, (const BtSys *)sy);
+        }
+        hipEventRecord(t->ev1, s);
+        hipEventSynchronize(t->ev1);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, t->ev0, t->ev1);
+        if (ms < bt) { bt = ms; ti = q; }
+    }
+    if (bt_layout(B, cand[bi][0], cand[bi][1], tiles[ti])) return BKH_DEVICE_ERROR;
+    LAUNCH_OK();
+    return 0;
+}

@@ -217,3 +217,66 @@ def bench_copy(d_a, d_b, nbytes, reps=20):
     rc = load().bkh_bench_copy(d_a.ptr, d_b.ptr, nbytes, reps, ctypes.byref(out))
     _check(rc, "bkh_bench_copy")
     return out.value
+
+
+class Batch:
+    """Independent CSR systems stacked block-diagonally on the device (bkh_batch).
+    ``systems`` is a list of (n, row_ptr, col, val). Vectors of the batch are
+    concatenated: system s occupies rows offset(s) .. offset(s + 1)."""
+
+    def __init__(self, systems, fp32=False):
+        ns = len(systems)
+        self.ns = ns
+        self._keep = [(np.ascontiguousarray(rp, np.int32), np.ascontiguousarray(ci, np.int32),
+                       np.ascontiguousarray(v, np.float64)) for (_, rp, ci, v) in systems]
+        n_each = np.array([int(sy[0]) for sy in systems], np.int32)
+        IP = ctypes.POINTER(ctypes.c_int)
+        DP = ctypes.POINTER(ctypes.c_double)
+        rps = (IP * ns)(*[k[0].ctypes.data_as(IP) for k in self._keep])
+        cis = (IP * ns)(*[k[1].ctypes.data_as(IP) for k in self._keep])
+        vs = (DP * ns)(*[k[2].ctypes.data_as(DP) for k in self._keep])
+        self.handle = load().bkh_batch_create(ns, n_each.ctypes.data_as(IP), rps, cis, vs, int(fp32))
+        del self._keep
+        if not self.handle:
+            raise BkspDeviceError("bkh_batch_create failed: " + last_error())
+        self.N = load().bkh_batch_rows(self.handle)
+        self.nnz = load().bkh_batch_nnz(self.handle)
+        self.offsets = np.array([load().bkh_batch_offset(self.handle, q) for q in range(ns + 1)], np.int64)
+
+    def destroy(self):
+        if self.handle:
+            load().bkh_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def layout(self):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        load().bkh_batch_layout(self.handle, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return a.value, b.value, c.value
+
+    def tune(self, fp32=False):
+        ms = ctypes.c_double()
+        rc = load().bkh_batch_tune(self.handle, int(fp32), ctypes.byref(ms))
+        _check(rc, "bkh_batch_tune")
+        return ms.value
+
+    def spmv(self, d_x, d_y, fp32=False):
+        _check(load().bkh_batch_spmv(self.handle, int(fp32), d_x.ptr, d_y.ptr), "bkh_batch_spmv")
+
+    def bicgstabl(self, d_b, l, d_x, tol, max_iter, **kw):
+        """Returns (info, per-system return values, per-system FP64 true residuals)."""
+        opts = make_opts(**kw)
+        info = BkhInfo()
+        ret = np.zeros(self.ns, np.int32)
+        tr = np.zeros(self.ns)
+        rc = load().bkh_batch_bicgstabl(self.handle, d_b.ptr, l, d_x.ptr, tol, max_iter, ctypes.byref(opts),
+                                        ret.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                        tr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(info))
+        if rc < -4:
+            raise BkspDeviceError(f"bkh_batch_bicgstabl -> {rc}: {last_error()}")
+        return info, ret, tr

@@ -82,3 +82,15 @@ def cfg3_peclet(pmax=0.4, v=(1.0, 2.0, 3.0)):
     """Cell Peclet numbers p_i = |v_i| h / (2 nu) scaled so that max p_i = pmax."""
     s = pmax / max(v)
     return tuple(s * vi for vi in v)
+
+
+def cfg4_system_seed(s, base=SEEDS["cfg4_struct"]):
+    """Seed of system s of the uneven batch (structure and values)."""
+    return base * 65536 + s
+
+
+def cfg4_batch(count=512, seed=SEEDS["cfg4_struct"], lo2=10.0, hi2=17.0):
+    """The uneven batch: sizes n_s = round(2^U[lo2, hi2]) from `seed`, system s from
+    random_rows(cfg4_system_seed(s), n_s). Returns a list of (n, row_ptr, col, val)."""
+    sizes = batch_sizes(seed, count, lo2, hi2)
+    return [random_rows(cfg4_system_seed(q, seed), int(n)) for q, n in enumerate(sizes)]

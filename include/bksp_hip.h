@@ -154,6 +154,33 @@ int bkh_shbicgstabl(bkh_matrix *A, const double *d_b, int l, const double *sigma
                     double *d_x, double tol, int max_iter, const bkh_opts *opts,
                     bkh_info *info, double *shift_true_res);
 
+/* ---- batches of independent systems (no reference counterpart: the reference
+ * solves one system per call, so a batch is that many sequential dbicgstabl calls) */
+typedef struct bkh_batch bkh_batch;
+/* Stacks ns CSR systems (0-based, square, n_each[s] rows, rows of <= 512 nonzeros)
+ * block-diagonally on the device. want_fp32 keeps an FP32 copy for BKH_FP32_IR. */
+bkh_batch *bkh_batch_create(int ns, const int *n_each, const int *const *row_ptr,
+                            const int *const *col, const double *const *val, int want_fp32);
+void bkh_batch_destroy(bkh_batch *B);
+long long bkh_batch_rows(const bkh_batch *B);           /* total rows N */
+long long bkh_batch_nnz(const bkh_batch *B);
+long long bkh_batch_offset(const bkh_batch *B, int s);  /* first row of system s */
+/* The SpMV blocking and vector tile size in use (set by bkh_batch_tune). */
+void bkh_batch_layout(const bkh_batch *B, int *rows_cap, int *nnz_cap, int *tile_rows);
+/* Autotuner: times candidate SpMV blockings (rows and nonzeros per wavefront) and
+ * vector tile sizes on this batch (FP32 SpMV if fp32) and keeps the fastest. */
+int bkh_batch_tune(bkh_batch *B, int fp32, double *best_ms);
+/* y = A x over the batch; x, y device vectors of N doubles (floats if fp32). */
+int bkh_batch_spmv(const bkh_batch *B, int fp32, const void *d_x, void *d_y);
+/* BiCGSTAB(l), 1 <= l <= 4, on every system at once (one launch per phase over the
+ * whole batch). d_b, d_x: device vectors of N entries (system s at its offset); ret
+ * (host, ns) receives each system's result as dbicgstabl returns it; true_res (host,
+ * ns, may be NULL) each system's FP64 true residual. opts->precision = BKH_FP32_IR runs
+ * FP32 inner solves inside per-system FP64 refinement (ret = inner iterations used). */
+int bkh_batch_bicgstabl(bkh_batch *B, const double *d_b, int l, double *d_x, double tol,
+                        int max_iter, const bkh_opts *opts, int *ret, double *true_res,
+                        bkh_info *info);
+
 /* Timing of the most recent solve on this thread (ms, HIP events). */
 double bkh_last_solve_ms(void);
 /* Average ms per launch of `reps` back-to-back SpMVs (FP64, or FP32 if fp32 = 1),

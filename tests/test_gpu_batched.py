@@ -1,0 +1,104 @@
+"""Batched BiCGSTAB(l) over independent uneven systems (bkh_batch_*), against the
+oracle's single-system restatement of src/bicgstabl.c run on each system."""
+import numpy as np
+import pytest
+
+from bksp_amd import device as dv
+from bksp_amd import gen
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [40, 700, 64, 2500, 1031, 300, 97]
+
+
+def small_batch(seed=4040):
+    return [gen.random_rows(seed + q, n) for q, n in enumerate(SIZES)]
+
+
+def true_res(sy, b, x):
+    n, rp, ci, v = sy
+    y = po.csrgemv("n", n, rp, ci, v, x)
+    return np.linalg.norm(b - y) / np.linalg.norm(b)
+
+
+def test_batched_spmv_bitwise_and_tuner():
+    systems = small_batch()
+    B = dv.Batch(systems, fp32=True)
+    x = gen.normal(7, B.N)
+    dx, dy = dv.DeviceArray.from_numpy(x), dv.DeviceArray.from_numpy(np.zeros(B.N))
+    B.spmv(dx, dy)
+    y = dy.numpy()
+    for q, (n, rp, ci, v) in enumerate(systems):
+        o = B.offsets[q]
+        ref = po.csrgemv("n", n, rp, ci, v, x[o:o + n])
+        assert np.array_equal(y[o:o + n], ref), q
+    ms = B.tune(fp32=False)
+    assert ms > 0
+    layout = B.layout()
+    dy.upload(np.zeros(B.N))
+    B.spmv(dx, dy)
+    assert np.array_equal(dy.numpy(), y), layout   # per-row order does not depend on blocking
+    B.tune(fp32=True)
+    print("tuned layout", B.layout())
+
+
+def test_batched_fp64_matches_oracle_per_system():
+    systems = small_batch()
+    B = dv.Batch(systems)
+    b = gen.normal(405, B.N)
+    db, dx = dv.DeviceArray.from_numpy(b), dv.DeviceArray.from_numpy(np.zeros(B.N))
+    tol = 1e-10
+    info, ret, tr = B.bicgstabl(db, 2, dx, tol, 500)
+    x = dx.numpy()
+    for q, sy in enumerate(systems):
+        n = sy[0]
+        o = B.offsets[q]
+        cret, cx = po.bicgstabl(n, sy[1], sy[2], sy[3], b[o:o + n], 2, tol, 500)
+        assert ret[q] >= 0 and cret >= 0, (q, ret[q], cret)
+        assert abs(ret[q] - cret) <= max(1, int(0.05 * cret)), (q, ret[q], cret)
+        assert tr[q] < 1.5 * tol
+        assert abs(true_res(sy, b[o:o + n], x[o:o + n]) - tr[q]) <= 1e-3 * tr[q] + 1e-18
+        if ret[q] == cret:
+            assert np.linalg.norm(x[o:o + n] - cx) / np.linalg.norm(cx) < 1e-8
+    assert info.ret == ret.max()
+
+
+def test_batched_fp32_refined_reaches_fp64_tolerance():
+    systems = small_batch()
+    B = dv.Batch(systems, fp32=True)
+    b = gen.normal(405, B.N)
+    db = dv.DeviceArray.from_numpy(b)
+    dx64 = dv.DeviceArray.from_numpy(np.zeros(B.N))
+    B.bicgstabl(db, 2, dx64, 1e-8, 500)
+    dx = dv.DeviceArray.from_numpy(np.zeros(B.N))
+    info, ret, tr = B.bicgstabl(db, 2, dx, 1e-8, 500, precision=dv.BKH_FP32_IR)
+    assert (ret >= 0).all(), ret
+    assert tr.max() < 1e-8
+    x, x64 = dx.numpy(), dx64.numpy()
+    for q, sy in enumerate(systems):
+        o, n = B.offsets[q], sy[0]
+        assert true_res(sy, b[o:o + n], x[o:o + n]) < 1e-8
+        # both solves meet 1e-8; they agree to about cond * tol (reported)
+        assert np.linalg.norm(x[o:o + n] - x64[o:o + n]) / np.linalg.norm(x64[o:o + n]) < 1e-5
+    print(f"fp32ir passes={info.passes} inner max={info.inner_iters} {info.time_ms:.2f} ms")
+
+
+def test_cfg4_full_batch():
+    """512 uneven systems, n log-uniform in [2^10, 2^17], BiCGSTAB(2), tol 1e-8."""
+    systems = gen.cfg4_batch()
+    B = dv.Batch(systems, fp32=True)
+    del systems
+    B.tune(fp32=True)
+    b = gen.normal(405, B.N)
+    db = dv.DeviceArray.from_numpy(b)
+    dx = dv.DeviceArray.from_numpy(np.zeros(B.N))
+    info, ret, tr = B.bicgstabl(db, 2, dx, 1e-8, 1000, precision=dv.BKH_FP32_IR)
+    print(f"cfg4 N={B.N} nnz={B.nnz} fp32ir {info.time_ms:.1f} ms passes={info.passes} "
+          f"max inner={info.inner_iters} max true res={tr.max():.2e} layout={B.layout()}")
+    assert (ret >= 0).all()
+    assert tr.max() < 1e-8
+    dx.upload(np.zeros(B.N))
+    info64, ret64, tr64 = B.bicgstabl(db, 2, dx, 1e-8, 1000)
+    print(f"cfg4 fp64 batched {info64.time_ms:.1f} ms max outer={info64.iters} max true res={tr64.max():.2e}")
+    assert (ret64 >= 0).all() and tr64.max() < 1.5e-8
