@@ -99,6 +99,21 @@ _SIGS = {
     "bkh_batch_spmv": (c_int, [c_vp, c_int, c_vp, c_vp]),
     "bkh_batch_bicgstabl": (c_int, [c_vp, c_vp, c_int, c_vp, c_dbl, c_int, ctypes.POINTER(BkhOpts), c_ip,
                                     c_dp, ctypes.POINTER(BkhInfo)]),
+    # learned FSAI and PCG
+    "bkh_fsai_nparam": (c_int, []),
+    "bkh_fsai_model_create": (c_vp, [ctypes.POINTER(ctypes.c_float)]),
+    "bkh_fsai_model_destroy": (None, [c_vp]),
+    "bkh_fsai_model_get": (c_int, [c_vp, ctypes.POINTER(ctypes.c_float)]),
+    "bkh_fsai_model_set": (c_int, [c_vp, ctypes.POINTER(ctypes.c_float)]),
+    "bkh_fsai_field_create": (c_vp, [c_int, c_vp, c_vp, c_vp, c_int, c_vp]),
+    "bkh_fsai_field_destroy": (None, [c_vp]),
+    "bkh_fsai_field_matrix": (c_vp, [c_vp]),
+    "bkh_fsai_factor": (c_int, [c_vp, c_vp, c_vp]),
+    "bkh_fsai_loss_grad": (c_int, [c_vp, c_vp, c_dp, c_vp]),
+    "bkh_fsai_train_step": (c_int, [c_vp, c_vp, c_dbl, c_dp]),
+    "bkh_fsai_bench_forward": (c_int, [c_vp, c_vp, c_int, c_dp]),
+    "bkh_fsai_pcg": (c_int, [c_vp, c_vp, c_vp, c_dbl, c_int, ctypes.POINTER(BkhOpts), ctypes.POINTER(BkhInfo)]),
+    "bkh_pcg": (c_int, [c_vp, c_int, c_vp, c_vp, c_dbl, c_int, ctypes.POINTER(BkhOpts), ctypes.POINTER(BkhInfo)]),
     # bksp.h drop-in (host arrays)
     "bksp_dcg": (c_int, [ctypes.POINTER(DSpMat), c_dp, c_dbl, c_int, c_dp]),
     "dcr": (c_int, [ctypes.POINTER(DSpMat), c_dp, c_dbl, c_int, c_dp]),

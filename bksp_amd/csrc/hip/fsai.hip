@@ -1,0 +1,827 @@
+// fsai.hip -- learned factorised sparse approximate inverse (FSAI) preconditioner and
+// preconditioned CG. No reference counterpart (the reference has no preconditioners).
+//
+// Model. For every edge (i, j), j <= i, of tril(A) an MLP 8 -> 64 -> 64 -> 1 (tanh)
+// maps eight local features to h, and the factor G (pattern of tril(A)) is
+//   g_ii = softplus(h + c0) / sqrt(a_ii),  g_ij = h / sqrt(a_jj),  c0 = log(e - 1),
+// so M^-1 = G^T G is SPD for any weights and h = 0 gives Jacobi. Weights live in
+// FP32 (Adam master copy) and enter the matrix cores as BF16 with FP32 accumulation.
+//
+// MLP on v_mfma_f32_16x16x16bf16_1k, one 16-edge tile per wave step, computed
+// transposed (hidden x edges): the 16x16 result layout of one layer (lane L, register
+// r: row 4*(L/16) + r, column L%16) is exactly the B-operand layout of the next, so
+// activations never leave registers in the forward pass. The backward pass
+// recomputes the forward, takes dL/dh per edge, runs dH1 = W2 dZ2 on the matrix cores
+// the same way, and accumulates the weight gradients (sums over edges) with MFMAs
+// whose operands are staged through a per-wave LDS transpose.
+//
+// Loss (Kaporin condition number of G A G^T, the classical FSAI objective, with the
+// trace estimated on probe vectors z_p -- Rademacher vectors and residuals of the
+// solver itself): L = log(T/n) - (2/n) sum_i log g_ii, T = (1/P) sum_p w_p.(A w_p),
+// w_p = G^T z_p; dL/dg_ij = (2/(P T)) sum_p z_p[i] (A w_p)_j - [i == j] 2/(n g_ii).
+#include <math.h>
+#include <vector>
+#include "bkh_host.hpp"
+#include "bkh_solver_common.hpp"
+#include "bkh_spmv.hpp"
+
+#define FS_H 64
+#define FS_NF 8
+#define FS_W1 0
+#define FS_B1 (FS_W1 + FS_NF * FS_H)
+#define FS_W2 (FS_B1 + FS_H)
+#define FS_B2 (FS_W2 + FS_H * FS_H)
+#define FS_W3 (FS_B2 + FS_H)
+#define FS_B3 (FS_W3 + FS_H)
+#define FS_NP (FS_B3 + 1)
+#define FS_BLOCKS 256
+#define FS_WAVES (FS_BLOCKS * 4)
+
+typedef short s4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+static const double FS_C0 = 0.54132485461291810;   // log(e - 1)
+
+struct bkh_fsai_model {
+    float *params = nullptr, *m = nullptr, *v = nullptr;
+    double *grad = nullptr;
+    float *part = nullptr;        // per-wave gradient partials
+    int step = 0;
+};
+
+struct bkh_fsai_field {
+    int n = 0;
+    long long nnzl = 0;
+    bkh_matrix *A = nullptr;      // expanded symmetric matrix
+    bkh_matrix *G = nullptr;      // factor (pattern tril(A)); G->T = G^T
+    int *rpl = nullptr, *cil = nullptr, *erow = nullptr, *perm = nullptr;
+    double *vl = nullptr;
+    double4 *rows = nullptr;      // sqrt(a_ii), coupling sum, lower count, diagonal excess
+    double *Z = nullptr, *W = nullptr, *U = nullptr, *gE = nullptr;
+    int P = 0;
+};
+
+__device__ __forceinline__ short f2bf(float x) {
+    unsigned u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+
+__device__ __forceinline__ float bf_round(float x) { return __uint_as_float(((unsigned)(unsigned short)f2bf(x)) << 16); }
+
+__device__ __forceinline__ f4v mfma_bf16(s4v a, s4v b, f4v c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// Row data from the expanded matrix (the oracle's orc_fsai_rows).
+__global__ __launch_bounds__(256) void k_fsai_rows(int n, const int *rp, const int *ci, const double *v,
+                                                   const double *diag, double4 *rows) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        double acc = 0.0, asum = 0.0;
+        int nl = 0;
+        const double di = diag[i];
+        for (int k = rp[i]; k < rp[i + 1]; k++) {
+            const int j = ci[k];
+            if (j == i) continue;
+            acc += fabs(v[k]) / sqrt(di * diag[j]);
+            asum += fabs(v[k]);
+            if (j < i) nl++;
+        }
+        rows[i] = make_double4(sqrt(di), acc, (double)nl, (di - asum) / di);
+    }
+}
+
+// Weights staged in LDS as BF16: W2T[o][i] and W2[i][o] (A operands of the forward
+// layer 2 and of dH1 = W2 dZ2).
+struct alignas(16) FsSmem {
+    short w2t[FS_H * FS_H];
+    short w2[FS_H * FS_H];
+};
+
+// The four features of lane group gq (0: f0..f3, 1: f4..f7, 2-3: zero padding).
+__device__ __forceinline__ void fs_features(int n, int i, int j, double aij, const double4 *rows, int gq,
+                                            float (&f)[4]) {
+    const double4 ri = rows[i], rj = rows[j];
+    if (gq == 0) {
+        f[0] = (float)(i == j ? 1.0 : 0.0);
+        f[1] = (float)(aij / (ri.x * rj.x));
+        f[2] = (float)(0.25 * log((ri.x * ri.x) / (rj.x * rj.x)));
+        f[3] = (float)ri.y;
+    } else if (gq == 1) {
+        f[0] = (float)rj.y;
+        f[1] = (float)(log2(1.0 + (double)(i - j)) / log2(1.0 + (double)n));
+        f[2] = (float)(0.25 * ri.z);
+        f[3] = (float)ri.w;
+    } else {
+        f[0] = f[1] = f[2] = f[3] = 0.0f;
+    }
+}
+
+// Forward (BWD = false: writes the factor values) or forward + backward (BWD = true:
+// per-wave gradient partials). Persistent waves stride over 16-edge tiles.
+template <bool BWD>
+__global__ __launch_bounds__(256) void k_fsai_mlp(int n, long long nnzl, const int *erow, const int *cil,
+                                                  const double *vl, const double4 *rows, const float *params,
+                                                  const int *perm, double *gval, float *gval32, double *gtval,
+                                                  float *gtval32, const double *gE, float *part) {
+    __shared__ FsSmem sm;
+    __shared__ __attribute__((aligned(16))) short lds_all[4][3 * FS_H * 16 + 16 * 16];
+    const int L = threadIdx.x & 63, w = threadIdx.x >> 6, gq = L >> 4, el = L & 15;
+    for (int e = threadIdx.x; e < FS_H * FS_H; e += 256) {
+        const int i = e / FS_H, o = e % FS_H;
+        const short b = f2bf(params[FS_W2 + e]);
+        sm.w2[e] = b;                         // [i][o]
+        sm.w2t[o * FS_H + i] = b;             // [o][i]
+    }
+    __syncthreads();
+    // layer-1 A operand: W1^T[16m + L%16][4gq + j] = W1[4gq + j][16m + L%16]
+    s4v A1[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int f = 4 * gq + j;
+            A1[m][j] = f < FS_NF ? f2bf(params[FS_W1 + f * FS_H + 16 * m + el]) : (short)0;
+        }
+    float b1[4][4], b2[4][4], w3b[4][4];
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int hh = 16 * m + 4 * gq + r;
+            b1[m][r] = params[FS_B1 + hh];
+            b2[m][r] = params[FS_B2 + hh];
+            w3b[m][r] = bf_round(params[FS_W3 + hh]);
+        }
+    const float b3 = params[FS_B3];
+    short *lds = lds_all[w];
+    short *Sdz2 = lds, *Sa1 = lds + FS_H * 16, *Sdz1 = lds + 2 * FS_H * 16, *Sx = lds + 3 * FS_H * 16;
+    f4v gW2[4][4], gW1[4];
+    float gw3[4][4], gb2[4][4], gb1[4][4], gb3 = 0.0f;
+    if (BWD) {
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            gW1[a] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int b = 0; b < 4; b++) { gW2[a][b] = f4v{0.f, 0.f, 0.f, 0.f}; gw3[a][b] = gb2[a][b] = gb1[a][b] = 0.f; }
+        }
+    }
+    const long long ntile = (nnzl + 15) / 16;
+    const int gw = blockIdx.x * 4 + w, nw = gridDim.x * 4;
+    for (long long tile = gw; tile < ntile; tile += nw) {
+        const long long e = tile * 16 + el;
+        const bool valid = e < nnzl;
+        const int i = valid ? erow[e] : 0, j = valid ? cil[e] : 0;
+        const double aij = valid ? vl[e] : 0.0;
+        float f[4];
+        if (valid) fs_features(n, i, j, aij, rows, gq, f);
+        else f[0] = f[1] = f[2] = f[3] = 0.0f;
+        s4v xb;
+#pragma unroll
+        for (int q = 0; q < 4; q++) xb[q] = f2bf(f[q]);
+        // layer 1: H1^T = W1^T X^T (4 row blocks of 16 hidden units)
+        float a1[4][4];
+        s4v B2[4];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const f4v d = mfma_bf16(A1[m], xb, f4v{0.f, 0.f, 0.f, 0.f});
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                a1[m][r] = tanhf(d[r] + b1[m][r]);
+                B2[m][r] = f2bf(a1[m][r]);
+            }
+        }
+        // layer 2: H2^T = W2^T H1^T
+        float a2[4][4];
+#pragma unroll
+        for (int mo = 0; mo < 4; mo++) {
+            f4v acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const s4v A2 = *(const s4v *)&sm.w2t[(16 * mo + el) * FS_H + 16 * q + 4 * gq];
+                acc = mfma_bf16(A2, B2[q], acc);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) a2[mo][r] = tanhf(acc[r] + b2[mo][r]);
+        }
+        // layer 3: h = w3 . H2 + b3 (reduce the four lane groups of each edge)
+        float ps = 0.0f;
+#pragma unroll
+        for (int mo = 0; mo < 4; mo++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) ps += a2[mo][r] * w3b[mo][r];
+        ps += __shfl_xor(ps, 16, 64);
+        ps += __shfl_xor(ps, 32, 64);
+        const float h = ps + b3;
+        if (!BWD) {
+            if (valid && gq == 0) {
+                const double hd = (double)h;
+                const double g = (i == j) ? (hd + FS_C0 > 30.0 ? hd + FS_C0 : log1p(exp(hd + FS_C0))) / rows[i].x
+                                          : hd / rows[j].x;
+                gval[e] = g;
+                gval32[e] = (float)g;
+                const int pe = perm[e];
+                gtval[pe] = g;
+                gtval32[pe] = (float)g;
+            }
+            continue;
+        }
+        // ---- backward: dL/dh from dL/dg through the parametrisation
+        float dh = 0.0f;
+        if (valid) {
+            const double hd = (double)h, dg = gE[e];
+            dh = (float)((i == j) ? dg / (1.0 + exp(-(hd + FS_C0))) / rows[i].x : dg / rows[j].x);
+        }
+        if (gq == 0) gb3 += dh;
+        float dz2[4][4];
+        s4v Bdz[4];
+#pragma unroll
+        for (int mo = 0; mo < 4; mo++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                gw3[mo][r] += a2[mo][r] * dh;
+                dz2[mo][r] = w3b[mo][r] * dh * (1.0f - a2[mo][r] * a2[mo][r]);
+                gb2[mo][r] += dz2[mo][r];
+                Bdz[mo][r] = f2bf(dz2[mo][r]);
+            }
+        // dH1^T = W2 dZ2^T
+        float dz1[4][4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            f4v acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int mo = 0; mo < 4; mo++) {
+                const s4v A3 = *(const s4v *)&sm.w2[(16 * q + el) * FS_H + 16 * mo + 4 * gq];
+                acc = mfma_bf16(A3, Bdz[mo], acc);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                dz1[q][r] = acc[r] * (1.0f - a1[q][r] * a1[q][r]);
+                gb1[q][r] += dz1[q][r];
+            }
+        }
+        // weight gradients (sums over the tile's edges) via an LDS transpose
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = 16 * m + 4 * gq + r;
+                Sdz2[row * 16 + el] = Bdz[m][r];
+                Sa1[row * 16 + el] = B2[m][r];
+                Sdz1[row * 16 + el] = f2bf(dz1[m][r]);
+            }
+#pragma unroll
+        for (int q = 0; q < 4; q++) Sx[(4 * gq + q) * 16 + el] = xb[q];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int mo = 0; mo < 4; mo++) {
+            const s4v Ad = *(const s4v *)&Sdz2[(16 * mo + el) * 16 + 4 * gq];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const s4v Bh = *(const s4v *)&Sa1[(16 * q + el) * 16 + 4 * gq];
+                gW2[mo][q] = mfma_bf16(Ad, Bh, gW2[mo][q]);        // dW2^T[o][i]
+            }
+        }
+        const s4v Bx = *(const s4v *)&Sx[el * 16 + 4 * gq];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const s4v Ad1 = *(const s4v *)&Sdz1[(16 * q + el) * 16 + 4 * gq];
+            gW1[q] = mfma_bf16(Ad1, Bx, gW1[q]);                    // dW1^T[i][f]
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    if (!BWD) return;
+    // ---- per-wave partial gradient
+    float *pw = part + (size_t)gw * FS_NP;
+#pragma unroll
+    for (int mo = 0; mo < 4; mo++)
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int o = 16 * mo + 4 * gq + r, ii = 16 * q + el;
+                pw[FS_W2 + ii * FS_H + o] = gW2[mo][q][r];
+            }
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int ii = 16 * q + 4 * gq + r;
+            if (el < FS_NF) pw[FS_W1 + el * FS_H + ii] = gW1[q][r];
+        }
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            float a = gw3[m][r], b = gb2[m][r], c = gb1[m][r];
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                a += __shfl_xor(a, o, 64);
+                b += __shfl_xor(b, o, 64);
+                c += __shfl_xor(c, o, 64);
+            }
+            if (el == 0) {
+                const int hh = 16 * m + 4 * gq + r;
+                pw[FS_W3 + hh] = a;
+                pw[FS_B2 + hh] = b;
+                pw[FS_B1 + hh] = c;
+            }
+        }
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) gb3 += __shfl_xor(gb3, o, 64);
+    if (L == 0) pw[FS_B3] = gb3;
+}
+
+// grad[k] = sum over waves of the partials, in wave order (double).
+__global__ __launch_bounds__(256) void k_fsai_reduce(const float *part, int nw, double *grad) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= FS_NP) return;
+    double acc = 0.0;
+    for (int q = 0; q < nw; q++) acc += (double)part[(size_t)q * FS_NP + k];
+    grad[k] = acc;
+}
+
+// dL/dg per edge from the probes: (2/(P T)) sum_p z_p[i] u_p[j] - [i == j] 2/(n g_ii).
+__global__ __launch_bounds__(256) void k_fsai_edge_grad(int n, long long nnzl, const int *erow, const int *cil,
+                                                        const double *g, int P, const double *Z, const double *U,
+                                                        const Ctl *c, double *gE) {
+    const double T = c->s[0] / P;
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < nnzl; e += (long long)gridDim.x * 256) {
+        const int i = erow[e], j = cil[e];
+        double acc = 0.0;
+        for (int p = 0; p < P; p++) acc += Z[(size_t)p * n + i] * U[(size_t)p * n + j];
+        gE[e] = 2.0 * acc / (P * T) - ((i == j) ? 2.0 / (n * g[e]) : 0.0);
+    }
+}
+
+// partial sums of log g_ii over the diagonal edges
+__global__ __launch_bounds__(256) void k_fsai_logdiag(long long nnzl, const int *erow, const int *cil,
+                                                      const double *g, double *part) {
+    double v[1] = {0.0};
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < nnzl; e += (long long)gridDim.x * 256)
+        if (erow[e] == cil[e]) v[0] += log(g[e]);
+    block_partial<1, 256>(v, part);
+}
+
+struct FsAddFin {
+    int slot;
+    __device__ void operator()(Ctl *c, const double *tot) const { c->s[slot] += tot[0]; }
+};
+
+// Adam on the FP32 master weights (bias-corrected).
+__global__ __launch_bounds__(256) void k_fsai_adam(float *p, float *m, float *v, const double *g, float lr,
+                                                   float b1t, float b2t) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= FS_NP) return;
+    const float gk = (float)g[k];
+    const float mk = 0.9f * m[k] + 0.1f * gk, vk = 0.999f * v[k] + 0.001f * gk * gk;
+    m[k] = mk;
+    v[k] = vk;
+    p[k] -= lr * (mk / (1.0f - b1t)) / (sqrtf(vk / (1.0f - b2t)) + 1e-8f);
+}
+
+// ---------------------------------------------------------------- host side
+
+static int fs_grid(long long nnzl) {
+    long long tiles = (nnzl + 15) / 16;
+    long long b = (tiles + 3) / 4;
+    return (int)std::max(1LL, std::min(b, (long long)FS_BLOCKS));
+}
+
+extern "C" bkh_fsai_model *bkh_fsai_model_create(const float *init) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return nullptr;
+    bkh_fsai_model *M = new bkh_fsai_model;
+    bool ok = hipMalloc(&M->params, sizeof(float) * FS_NP) == hipSuccess &&
+              hipMalloc(&M->m, sizeof(float) * FS_NP) == hipSuccess &&
+              hipMalloc(&M->v, sizeof(float) * FS_NP) == hipSuccess &&
+              hipMalloc(&M->grad, sizeof(double) * FS_NP) == hipSuccess &&
+              hipMalloc(&M->part, sizeof(float) * FS_NP * FS_WAVES) == hipSuccess;
+    std::vector<float> h(FS_NP, 0.0f);
+    if (init) for (int k = 0; k < FS_NP; k++) h[k] = init[k];
+    ok = ok && hipMemcpy(M->params, h.data(), sizeof(float) * FS_NP, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemset(M->m, 0, sizeof(float) * FS_NP) == hipSuccess && hipMemset(M->v, 0, sizeof(float) * FS_NP) == hipSuccess;
+    if (!ok) {
+        bkh_set_error("FSAI model allocation failed");
+        hipFree(M->params); hipFree(M->m); hipFree(M->v); hipFree(M->grad); hipFree(M->part);
+        delete M;
+        return nullptr;
+    }
+    return M;
+}
+
+extern "C" void bkh_fsai_model_destroy(bkh_fsai_model *M) {
+    if (!M) return;
+    hipFree(M->params); hipFree(M->m); hipFree(M->v); hipFree(M->grad); hipFree(M->part);
+    delete M;
+}
+
+extern "C" int bkh_fsai_model_get(const bkh_fsai_model *M, float *params) {
+    if (!M || !params) return BKH_BAD_ARGUMENT;
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    HIP_OK(hipMemcpyAsync(params, M->params, sizeof(float) * FS_NP, hipMemcpyDeviceToHost, t->stream));
+    HIP_OK(hipStreamSynchronize(t->stream));
+    return 0;
+}
+
+extern "C" int bkh_fsai_model_set(bkh_fsai_model *M, const float *params) {
+    if (!M || !params) return BKH_BAD_ARGUMENT;
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    HIP_OK(hipMemcpyAsync(M->params, params, sizeof(float) * FS_NP, hipMemcpyHostToDevice, t->stream));
+    HIP_OK(hipMemsetAsync(M->m, 0, sizeof(float) * FS_NP, t->stream));
+    HIP_OK(hipMemsetAsync(M->v, 0, sizeof(float) * FS_NP, t->stream));
+    HIP_OK(hipStreamSynchronize(t->stream));
+    M->step = 0;
+    return 0;
+}
+
+extern "C" int bkh_fsai_nparam(void) { return FS_NP; }
+
+static void fs_field_free(bkh_fsai_field *F) {
+    if (F->A) bkh_matrix_destroy(F->A);
+    if (F->G) bkh_matrix_destroy(F->G);
+    hipFree(F->rpl); hipFree(F->cil); hipFree(F->erow); hipFree(F->perm); hipFree(F->vl); hipFree(F->rows);
+    hipFree(F->Z); hipFree(F->W); hipFree(F->U); hipFree(F->gE);
+    delete F;
+}
+
+extern "C" void bkh_fsai_field_destroy(bkh_fsai_field *F) {
+    if (F) fs_field_free(F);
+}
+
+// A field: the lower triangle (with diagonal) of an SPD matrix and P probe vectors
+// (row-major P x n, host; may be NULL with P = 0 for evaluation-only fields).
+extern "C" bkh_fsai_field *bkh_fsai_field_create(int n, const int *rpl, const int *cil, const double *vl, int P,
+                                                 const double *Z) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return nullptr;
+    if (n < 1 || !rpl || !cil || !vl || P < 0 || (P > 0 && !Z)) { bkh_set_error("bad FSAI field"); return nullptr; }
+    const long long nnzl = rpl[n];
+    for (int i = 0; i < n; i++) {
+        bool diag = false;
+        for (int k = rpl[i]; k < rpl[i + 1]; k++) {
+            if (cil[k] > i || cil[k] < 0) { bkh_set_error("FSAI field: row %d is not lower triangular", i); return nullptr; }
+            if (cil[k] == i) diag = true;
+        }
+        if (!diag) { bkh_set_error("FSAI field: row %d has no diagonal", i); return nullptr; }
+    }
+    bkh_fsai_field *F = new bkh_fsai_field;
+    F->n = n;
+    F->nnzl = nnzl;
+    F->P = P;
+    F->A = bkh_matrix_create(n, rpl, cil, vl, 1, 0, 0);
+    F->G = bkh_matrix_create(n, rpl, cil, vl, 0, 1, 1);
+    if (!F->A || !F->G) { fs_field_free(F); return nullptr; }
+    std::vector<int> erow((size_t)nnzl), perm((size_t)nnzl), cnt((size_t)n + 1, 0);
+    for (int i = 0; i < n; i++)
+        for (int k = rpl[i]; k < rpl[i + 1]; k++) { erow[k] = i; cnt[cil[k] + 1]++; }
+    for (int j = 0; j < n; j++) cnt[j + 1] += cnt[j];
+    for (int i = 0; i < n; i++)                            // G^T rows in the 't' order of the upload
+        for (int k = rpl[i]; k < rpl[i + 1]; k++) perm[k] = cnt[cil[k]]++;
+    const size_t np = (size_t)std::max(P, 1) * n;
+    bool ok = hipMalloc(&F->rpl, sizeof(int) * (n + 1)) == hipSuccess &&
+              hipMalloc(&F->cil, sizeof(int) * nnzl) == hipSuccess &&
+              hipMalloc(&F->erow, sizeof(int) * nnzl) == hipSuccess &&
+              hipMalloc(&F->perm, sizeof(int) * nnzl) == hipSuccess &&
+              hipMalloc(&F->vl, sizeof(double) * nnzl) == hipSuccess &&
+              hipMalloc(&F->rows, sizeof(double4) * n) == hipSuccess &&
+              hipMalloc(&F->Z, sizeof(double) * np) == hipSuccess &&
+              hipMalloc(&F->W, sizeof(double) * np) == hipSuccess &&
+              hipMalloc(&F->U, sizeof(double) * np) == hipSuccess &&
+              hipMalloc(&F->gE, sizeof(double) * nnzl) == hipSuccess;
+    hipStream_t s = t->stream;
+    ok = ok && hipMemcpyAsync(F->rpl, rpl, sizeof(int) * (n + 1), hipMemcpyHostToDevice, s) == hipSuccess &&
+         hipMemcpyAsync(F->cil, cil, sizeof(int) * nnzl, hipMemcpyHostToDevice, s) == hipSuccess &&
+         hipMemcpyAsync(F->erow, erow.data(), sizeof(int) * nnzl, hipMemcpyHostToDevice, s) == hipSuccess &&
+         hipMemcpyAsync(F->perm, perm.data(), sizeof(int) * nnzl, hipMemcpyHostToDevice, s) == hipSuccess &&
+         hipMemcpyAsync(F->vl, vl, sizeof(double) * nnzl, hipMemcpyHostToDevice, s) == hipSuccess;
+    if (ok && P > 0) ok = hipMemcpyAsync(F->Z, Z, sizeof(double) * np, hipMemcpyHostToDevice, s) == hipSuccess;
+    if (ok) {
+        const bkh_matrix *A = F->A;
+        hipLaunchKernelGGL(k_fsai_rows, dim3(vec_grid(n, 256, 1)), dim3(256), 0, s, n, A->rp, A->ci, A->val, A->diag,
+                           F->rows);
+        ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(s) == hipSuccess;
+    }
+    if (!ok) { bkh_set_error("FSAI field upload failed"); fs_field_free(F); return nullptr; }
+    return F;
+}
+
+// G's values from the model (G and G^T, FP64 and FP32 copies); g_out (host) optional.
+static int fs_factor(ThreadCtx *t, const bkh_fsai_model *M, bkh_fsai_field *F) {
+    hipLaunchKernelGGL((k_fsai_mlp<false>), dim3(fs_grid(F->nnzl)), dim3(256), 0, t->stream, F->n, F->nnzl, F->erow,
+                       F->cil, F->vl, F->rows, M->params, F->perm, F->G->val, F->G->val32, F->G->T->val,
+                       F->G->T->val32, (const double *)nullptr, (float *)nullptr);
+    LAUNCH_OK();
+    return 0;
+}
+
+extern "C" int bkh_fsai_factor(const bkh_fsai_model *M, bkh_fsai_field *F, double *g_out) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (!M || !F) return BKH_BAD_ARGUMENT;
+    int rc = fs_factor(t, M, F);
+    if (rc) return rc;
+    if (g_out) HIP_OK(hipMemcpyAsync(g_out, F->G->val, sizeof(double) * F->nnzl, hipMemcpyDeviceToHost, t->stream));
+    HIP_OK(hipStreamSynchronize(t->stream));
+    return 0;
+}
+
+// Loss and gradient (device, into M->grad); loss to *loss.
+static int fs_loss_grad(ThreadCtx *t, bkh_fsai_model *M, bkh_fsai_field *F, double *loss) {
+    if (F->P < 1) { bkh_set_error("FSAI field has no probe vectors"); return BKH_BAD_ARGUMENT; }
+    hipStream_t s = t->stream;
+    const int n = F->n;
+    const int G = vec_grid(n, 256);
+    const int Ge = vec_grid(F->nnzl, 256);
+    const int nparts = std::max(std::max(spmv_grid(F->A), spmv_grid(F->G->T)), std::max(G, Ge)) * 2;
+    void *ws = bkh_workspace(carve_size(sizeof(Ctl)) + carve_size(sizeof(double) * nparts));
+    if (!ws) return BKH_DEVICE_ERROR;
+    Carve cv(ws);
+    Ctl *c = cv.take<Ctl>(1);
+    double *part = cv.take<double>(nparts);
+    int rc = fs_factor(t, M, F);
+    if (rc) return rc;
+    if (ctl_init(t, c, 0.0, 1)) return BKH_DEVICE_ERROR;
+    Epi<0> e0;
+    memset(&e0, 0, sizeof e0);
+    for (int p = 0; p < F->P; p++) {
+        double *z = F->Z + (size_t)p * n, *w = F->W + (size_t)p * n, *u = F->U + (size_t)p * n;
+        const bkh_matrix *Gt = F->G->T;
+        launch_spmv<double, double, 0>(s, Gt, Gt->val, z, w, 1.0, 0.0, e0, nullptr, nullptr, NoFin());
+        Epi<1> e1;
+        memset(&e1, 0, sizeof e1);
+        e1.w[0] = w;
+        e1.a[0] = 0;
+        e1.b[0] = EPI_Y;
+        launch_spmv<double, double, 1>(s, F->A, F->A->val, w, u, 1.0, 0.0, e1, c, part, FsAddFin{0});
+    }
+    hipLaunchKernelGGL(k_fsai_logdiag, dim3(Ge), dim3(256), 0, s, F->nnzl, F->erow, F->cil, (const double *)F->G->val,
+                       part);
+    launch_finalize<1>(s, part, Ge, c, FsAddFin{1});
+    hipLaunchKernelGGL(k_fsai_edge_grad, dim3(Ge), dim3(256), 0, s, n, F->nnzl, F->erow, F->cil,
+                       (const double *)F->G->val, F->P, (const double *)F->Z, (const double *)F->U, (const Ctl *)c,
+                       F->gE);
+    hipLaunchKernelGGL((k_fsai_mlp<true>), dim3(FS_BLOCKS), dim3(256), 0, s, n, F->nnzl, F->erow, F->cil, F->vl,
+                       F->rows, M->params, F->perm, (double *)nullptr, (float *)nullptr, (double *)nullptr,
+                       (float *)nullptr, (const double *)F->gE, M->part);
+    hipLaunchKernelGGL(k_fsai_reduce, dim3((FS_NP + 255) / 256), dim3(256), 0, s, (const float *)M->part, FS_WAVES,
+                       M->grad);
+    LAUNCH_OK();
+    if (loss) {
+        if (ctl_fetch(t, c)) return BKH_DEVICE_ERROR;
+        const double T = t->ctl_host->s[0] / F->P;
+        *loss = log(T / n) - 2.0 * t->ctl_host->s[1] / n;
+    }
+    return 0;
+}
+
+extern "C" int bkh_fsai_loss_grad(bkh_fsai_model *M, bkh_fsai_field *F, double *loss, double *grad) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (!M || !F) return BKH_BAD_ARGUMENT;
+    int rc = fs_loss_grad(t, M, F, loss);
+    if (rc) return rc;
+    if (grad) HIP_OK(hipMemcpyAsync(grad, M->grad, sizeof(double) * FS_NP, hipMemcpyDeviceToHost, t->stream));
+    HIP_OK(hipStreamSynchronize(t->stream));
+    return 0;
+}
+
+// One Adam step on the field's loss; the loss before the step to *loss (may be NULL,
+// which saves the host round trip).
+extern "C" int bkh_fsai_train_step(bkh_fsai_model *M, bkh_fsai_field *F, double lr, double *loss) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (!M || !F) return BKH_BAD_ARGUMENT;
+    int rc = fs_loss_grad(t, M, F, loss);
+    if (rc) return rc;
+    M->step++;
+    const float b1t = powf(0.9f, (float)M->step), b2t = powf(0.999f, (float)M->step);
+    hipLaunchKernelGGL(k_fsai_adam, dim3((FS_NP + 255) / 256), dim3(256), 0, t->stream, M->params, M->m, M->v,
+                       (const double *)M->grad, (float)lr, b1t, b2t);
+    LAUNCH_OK();
+    return 0;
+}
+
+// Average ms of one forward pass (factor build) over reps launches, HIP events on the
+// kernel's stream.
+extern "C" int bkh_fsai_bench_forward(const bkh_fsai_model *M, bkh_fsai_field *F, int reps, double *avg_ms) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (!M || !F || reps < 1) return BKH_BAD_ARGUMENT;
+    fs_factor(t, M, F);
+    HIP_OK(hipEventRecord(t->ev0, t->stream));
+    for (int r = 0; r < reps; r++) fs_factor(t, M, F);
+    HIP_OK(hipEventRecord(t->ev1, t->stream));
+    HIP_OK(hipEventSynchronize(t->ev1));
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, t->ev0, t->ev1));
+    if (avg_ms) *avg_ms = ms / reps;
+    return 0;
+}
+
+// ------------------------------------------------------ preconditioned CG
+
+enum { PC_RZ = 0, PC_ALPHA = 1, PC_BETA = 2 };
+
+enum { PCK_NONE = 0, PCK_JACOBI = 1, PCK_FSAI = 2 };
+
+// r = b - A x (A x in r on entry); z = D^-1 r (Jacobi) or z = r (none; FSAI forms z
+// later). Partials {r.z, b.b}.
+__global__ __launch_bounds__(256) void k_pcg_init(long long n, const double *b, double *r, double *z,
+                                                  const double *diag, int kind, double *part) {
+    double v[2] = {0.0, 0.0};
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const double ri = b[i] - r[i];
+        r[i] = ri;
+        const double zi = kind == PCK_JACOBI ? ri / diag[i] : ri;
+        if (kind != PCK_FSAI) z[i] = zi;
+        v[0] += ri * zi;
+        v[1] += b[i] * b[i];
+    }
+    block_partial<2, 256>(v, part);
+}
+
+struct PcInitFin {
+    int fsai;
+    __device__ void operator()(Ctl *c, const double *tot) const {
+        c->fnb = sqrt(tot[1]);
+        if (!(c->fnb > 0.0)) { ctl_breakdown(c); return; }
+        if (!fsai) c->s[PC_RZ] = tot[0];
+    }
+};
+
+struct PcRzFin {            // after z = G^T G r: r.z (at the start), or the next beta
+    int init;
+    __device__ void operator()(Ctl *c, const double *tot) const {
+        if (init) { c->s[PC_RZ] = tot[0]; return; }
+        if (bad(c->s[PC_RZ])) { ctl_breakdown(c); return; }
+        c->s[PC_BETA] = tot[0] / c->s[PC_RZ];
+        c->s[PC_RZ] = tot[0];
+    }
+};
+
+struct PcAlphaFin {
+    __device__ void operator()(Ctl *c, const double *tot) const {
+        if (bad(tot[0])) { ctl_breakdown(c); return; }
+        c->s[PC_ALPHA] = c->s[PC_RZ] / tot[0];
+    }
+};
+
+// x += alpha p, r -= alpha q; z = D^-1 r (Jacobi) or z = r (none). Partials {r.r, r.z}.
+__global__ __launch_bounds__(256) void k_pcg_xr(long long n, double *x, double *r, double *z, const double *p,
+                                                const double *q, const double *diag, int kind, const Ctl *c,
+                                                double *part) {
+    if (ctl_stopped(c)) return;
+    const double al = c->s[PC_ALPHA];
+    double v[2] = {0.0, 0.0};
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        x[i] = fma(al, p[i], x[i]);
+        const double ri = fma(-al, q[i], r[i]);
+        r[i] = ri;
+        v[0] += ri * ri;
+        if (kind != PCK_FSAI) {
+            const double zi = kind == PCK_JACOBI ? ri / diag[i] : ri;
+            z[i] = zi;
+            v[1] += ri * zi;
+        }
+    }
+    block_partial<2, 256>(v, part);
+}
+
+// ||r|| / ||b|| test (the iteration's index); Jacobi/none also form beta from r.z.
+struct PcCheckFin {
+    int fsai;
+    __device__ void operator()(Ctl *c, const double *tot) const {
+        ctl_check(c, tot[0]);
+        if (c->done || fsai) return;
+        if (bad(c->s[PC_RZ])) { ctl_breakdown(c); return; }
+        c->s[PC_BETA] = tot[1] / c->s[PC_RZ];
+        c->s[PC_RZ] = tot[1];
+    }
+};
+
+// p = z + beta p (beta = 0 on the first call)
+__global__ __launch_bounds__(256) void k_pcg_p(long long n, double *p, const double *z, const Ctl *c, int first) {
+    if (ctl_stopped(c)) return;
+    const double be = first ? 0.0 : c->s[PC_BETA];
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        p[i] = first ? z[i] : fma(be, p[i], z[i]);
+}
+
+// z = G^T (G r) with the epilogue r.z -> fin (FP32 factor values, FP64 vectors).
+template <class Fin>
+static void fs_apply(hipStream_t s, const bkh_matrix *G, const double *r, double *y, double *z, Ctl *c, double *part,
+                     Fin fin) {
+    Epi<0> e0;
+    memset(&e0, 0, sizeof e0);
+    launch_spmv<float, double, 0>(s, G, G->val32, r, y, 1.0, 0.0, e0, c, nullptr, NoFin());
+    Epi<1> e1;
+    memset(&e1, 0, sizeof e1);
+    e1.w[0] = r;
+    e1.a[0] = 0;
+    e1.b[0] = EPI_Y;
+    launch_spmv<float, double, 1>(s, G->T, G->T->val32, y, z, 1.0, 0.0, e1, c, part, fin);
+}
+
+// Preconditioned CG: q = A p, alpha = r.z / p.q, x += alpha p, r -= alpha q, stop on
+// ||r|| / ||b|| < tol (the index convention of cg.c:56-59), z = M^-1 r, beta = r.z'/r.z,
+// p = z + beta p.
+static int pcg_run(ThreadCtx *t, const bkh_matrix *A, const bkh_matrix *G, int kind, const double *b, double *x,
+                   double tol, int max_iter, const bkh_opts &o, bkh_info *inf) {
+    hipStream_t s = t->stream;
+    const long long n = A->n;
+    const int Gv = vec_grid(n, 256);
+    int nparts = std::max(spmv_grid(A), Gv);
+    if (G) nparts = std::max(nparts, std::max(spmv_grid(G), spmv_grid(G->T)));
+    nparts *= 2;
+    void *ws = bkh_workspace(carve_size(sizeof(Ctl)) + carve_size(sizeof(double) * nparts) +
+                             5 * carve_size(sizeof(double) * n));
+    if (!ws) return BKH_DEVICE_ERROR;
+    Carve cv(ws);
+    Ctl *c = cv.take<Ctl>(1);
+    double *part = cv.take<double>(nparts);
+    double *r = cv.take<double>(n), *z = cv.take<double>(n), *p = cv.take<double>(n), *q = cv.take<double>(n),
+           *y = cv.take<double>(n);
+    const int K = o.graph_iters > 0 ? o.graph_iters : default_graph_iters(n);
+    const bool fsai = kind == PCK_FSAI;
+    HIP_OK(hipEventRecord(t->ev0, s));
+    if (ctl_init(t, c, tol, max_iter)) return BKH_DEVICE_ERROR;
+    Epi<0> e0;
+    memset(&e0, 0, sizeof e0);
+    launch_spmv<double, double, 0>(s, A, A->val, x, r, 1.0, 0.0, e0, nullptr, nullptr, NoFin());
+    hipLaunchKernelGGL(k_pcg_init, dim3(Gv), dim3(256), 0, s, n, b, r, z, (const double *)A->diag, kind, part);
+    launch_finalize<2>(s, part, Gv, c, PcInitFin{(int)fsai});
+    if (fsai) fs_apply(s, G, r, y, z, c, part, PcRzFin{1});
+    hipLaunchKernelGGL(k_pcg_p, dim3(Gv), dim3(256), 0, s, n, p, (const double *)z, (const Ctl *)c, 1);
+    LAUNCH_OK();
+    hipGraphExec_t ge = capture_graph(s, K, [&]() {
+        Epi<1> e1;
+        memset(&e1, 0, sizeof e1);
+        e1.w[0] = p;
+        e1.a[0] = 0;
+        e1.b[0] = EPI_Y;
+        launch_spmv<double, double, 1>(s, A, A->val, p, q, 1.0, 0.0, e1, c, part, PcAlphaFin());
+        hipLaunchKernelGGL(k_pcg_xr, dim3(Gv), dim3(256), 0, s, n, x, r, z, (const double *)p, (const double *)q,
+                           (const double *)A->diag, kind, (const Ctl *)c, part);
+        launch_finalize<2>(s, part, Gv, c, PcCheckFin{(int)fsai});
+        if (fsai) fs_apply(s, G, r, y, z, c, part, PcRzFin{0});
+        hipLaunchKernelGGL(k_pcg_p, dim3(Gv), dim3(256), 0, s, n, p, (const double *)z, (const Ctl *)c, 0);
+    });
+    if (!ge) { bkh_set_error("hipGraph capture of the PCG iteration failed"); return BKH_DEVICE_ERROR; }
+    int rc = run_graph_until_done(t, ge, c, max_iter / K + 2);
+    hipGraphExecDestroy(ge);
+    if (rc) return rc;
+    Ctl hc = *t->ctl_host;
+    HIP_OK(hipEventRecord(t->ev1, s));
+    HIP_OK(hipEventSynchronize(t->ev1));
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, t->ev0, t->ev1));
+    memset(inf, 0, sizeof *inf);
+    inf->time_ms = ms;
+    bkh_note_solve_ms(ms);
+    inf->ret = ctl_result(&hc);
+    inf->status = hc.status;
+    inf->iters = hc.result_it;
+    inf->rec_res = hc.err;
+    double tr = 0.0;
+    if (true_residual(t, A, 0.0, b, x, &tr) == 0) inf->true_res = tr;
+    return inf->ret;
+}
+
+// PCG with Jacobi (kind 1) or no (kind 0) preconditioning on any SPD matrix.
+extern "C" int bkh_pcg(bkh_matrix *A, int kind, const double *b, double *x, double tol, int max_iter,
+                       const bkh_opts *opts, bkh_info *info) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (!A || !b || !x || (kind != PCK_NONE && kind != PCK_JACOBI)) return BKH_BAD_ARGUMENT;
+    bkh_opts o = opts ? *opts : bkh_opts{};
+    bkh_info inf;
+    if (max_iter <= 0) { if (info) { memset(info, 0, sizeof *info); info->status = ST_CONVERGED; } return 0; }
+    int ret = pcg_run(t, A, nullptr, kind, b, x, tol, max_iter, o, &inf);
+    if (ret < -4) return ret;
+    if (info) *info = inf;
+    return ret;
+}
+
+// PCG on the field's matrix with the learned factor (call bkh_fsai_factor first).
+extern "C" int bkh_fsai_pcg(bkh_fsai_field *F, const double *b, double *x, double tol, int max_iter,
+                            const bkh_opts *opts, bkh_info *info) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (!F || !b || !x) return BKH_BAD_ARGUMENT;
+    bkh_opts o = opts ? *opts : bkh_opts{};
+    bkh_info inf;
+    if (max_iter <= 0) { if (info) { memset(info, 0, sizeof *info); info->status = ST_CONVERGED; } return 0; }
+    int ret = pcg_run(t, F->A, F->G, PCK_FSAI, b, x, tol, max_iter, o, &inf);
+    if (ret < -4) return ret;
+    if (info) *info = inf;
+    return ret;
+}
+
+extern "C" bkh_matrix *bkh_fsai_field_matrix(bkh_fsai_field *F) { return F ? F->A : nullptr; }

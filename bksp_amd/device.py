@@ -280,3 +280,111 @@ class Batch:
         if rc < -4:
             raise BkspDeviceError(f"bkh_batch_bicgstabl -> {rc}: {last_error()}")
         return info, ret, tr
+
+
+def pcg(M, kind, d_b, d_x, tol, max_iter, **kw):
+    """Preconditioned CG: kind 0 = none, 1 = Jacobi."""
+    opts = make_opts(**kw)
+    info = BkhInfo()
+    _check(load().bkh_pcg(M.handle, int(kind), d_b.ptr, d_x.ptr, tol, max_iter, ctypes.byref(opts),
+                          ctypes.byref(info)), "bkh_pcg")
+    return info
+
+
+class FsaiModel:
+    """The learned-FSAI edge MLP (FP32 master weights on the device, Adam state)."""
+
+    def __init__(self, params=None):
+        self.nparam = load().bkh_fsai_nparam()
+        p = None
+        if params is not None:
+            self._p = np.ascontiguousarray(params, np.float32)
+            assert self._p.size == self.nparam
+            p = self._p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        self.handle = load().bkh_fsai_model_create(p)
+        if not self.handle:
+            raise BkspDeviceError("bkh_fsai_model_create failed: " + last_error())
+
+    def get(self):
+        out = np.empty(self.nparam, np.float32)
+        _check(load().bkh_fsai_model_get(self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))), "get")
+        return out
+
+    def set(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        _check(load().bkh_fsai_model_set(self.handle, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))), "set")
+
+    def destroy(self):
+        if self.handle:
+            load().bkh_fsai_model_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class FsaiField:
+    """tril(A) of an SPD matrix (with probes Z, P x n, for training) on the device."""
+
+    def __init__(self, n, rpl, cil, vl, Z=None):
+        rpl = np.ascontiguousarray(rpl, np.int32)
+        cil = np.ascontiguousarray(cil, np.int32)
+        vl = np.ascontiguousarray(vl, np.float64)
+        P = 0 if Z is None else int(Z.shape[0])
+        Zc = np.ascontiguousarray(Z if Z is not None else np.zeros(1), np.float64)
+        self.n, self.nnzl = int(n), int(rpl[-1])
+        self.handle = load().bkh_fsai_field_create(self.n, _ptr(rpl), _ptr(cil), _ptr(vl), P, _ptr(Zc))
+        if not self.handle:
+            raise BkspDeviceError("bkh_fsai_field_create failed: " + last_error())
+
+    def factor(self, model, want=False):
+        g = np.empty(self.nnzl) if want else None
+        _check(load().bkh_fsai_factor(model.handle, self.handle, _ptr(g) if want else None), "bkh_fsai_factor")
+        return g
+
+    def loss_grad(self, model):
+        L = ctypes.c_double()
+        g = np.empty(load().bkh_fsai_nparam())
+        _check(load().bkh_fsai_loss_grad(model.handle, self.handle, ctypes.byref(L), _ptr(g)), "loss_grad")
+        return L.value, g
+
+    def train_step(self, model, lr, want_loss=False):
+        L = ctypes.c_double()
+        _check(load().bkh_fsai_train_step(model.handle, self.handle, lr, ctypes.byref(L) if want_loss else None),
+               "train_step")
+        return L.value if want_loss else None
+
+    def bench_forward(self, model, reps=10):
+        ms = ctypes.c_double()
+        _check(load().bkh_fsai_bench_forward(model.handle, self.handle, reps, ctypes.byref(ms)), "bench_forward")
+        return ms.value
+
+    def pcg(self, d_b, d_x, tol, max_iter, **kw):
+        opts = make_opts(**kw)
+        info = BkhInfo()
+        _check(load().bkh_fsai_pcg(self.handle, d_b.ptr, d_x.ptr, tol, max_iter, ctypes.byref(opts),
+                                   ctypes.byref(info)), "bkh_fsai_pcg")
+        return info
+
+    def pcg_plain(self, kind, d_b, d_x, tol, max_iter, **kw):
+        """PCG on this field's matrix with kind 0 (none) or 1 (Jacobi)."""
+        opts = make_opts(**kw)
+        info = BkhInfo()
+        A = load().bkh_fsai_field_matrix(self.handle)
+        _check(load().bkh_pcg(A, int(kind), d_b.ptr, d_x.ptr, tol, max_iter, ctypes.byref(opts),
+                              ctypes.byref(info)), "bkh_pcg")
+        return info
+
+    def destroy(self):
+        if self.handle:
+            load().bkh_fsai_field_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

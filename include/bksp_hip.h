@@ -181,6 +181,43 @@ int bkh_batch_bicgstabl(bkh_batch *B, const double *d_b, int l, double *d_x, dou
                         int max_iter, const bkh_opts *opts, int *ret, double *true_res,
                         bkh_info *info);
 
+/* ---- learned FSAI preconditioner and preconditioned CG (no reference counterpart:
+ * the reference has no preconditioners) ----
+ * An edge MLP (8 -> 64 -> 64 -> 1, tanh; FP32 master weights used as BF16 on the
+ * matrix cores) maps local features of every edge of tril(A) to the factor G:
+ * g_ii = softplus(h + log(e-1)) / sqrt(a_ii), g_ij = h / sqrt(a_jj), so M^-1 = G^T G
+ * is SPD for any weights. Parameters are a flat float array of bkh_fsai_nparam()
+ * entries: W1[8][64], b1[64], W2[64][64], b2[64], w3[64], b3. */
+typedef struct bkh_fsai_model bkh_fsai_model;
+typedef struct bkh_fsai_field bkh_fsai_field;
+int bkh_fsai_nparam(void);
+bkh_fsai_model *bkh_fsai_model_create(const float *params);     /* NULL: all zero (Jacobi) */
+void bkh_fsai_model_destroy(bkh_fsai_model *M);
+int bkh_fsai_model_get(const bkh_fsai_model *M, float *params);
+int bkh_fsai_model_set(bkh_fsai_model *M, const float *params); /* also resets Adam */
+/* A field: tril(A) with diagonal (host CSR) of an SPD matrix, and P probe vectors
+ * (host, P x n row-major; P = 0 for a field that is only solved on). */
+bkh_fsai_field *bkh_fsai_field_create(int n, const int *rp_lower, const int *col_lower,
+                                      const double *val_lower, int P, const double *Z);
+void bkh_fsai_field_destroy(bkh_fsai_field *F);
+bkh_matrix *bkh_fsai_field_matrix(bkh_fsai_field *F);           /* the expanded A */
+/* Factor values from the model (g_out: host, nnz(tril A), may be NULL). */
+int bkh_fsai_factor(const bkh_fsai_model *M, bkh_fsai_field *F, double *g_out);
+/* Loss L = log(T/n) - (2/n) sum log g_ii, T = mean_p (G^T z_p)^T A (G^T z_p), and its
+ * parameter gradient (host, may be NULL), from the device forward/backward kernels. */
+int bkh_fsai_loss_grad(bkh_fsai_model *M, bkh_fsai_field *F, double *loss, double *grad);
+/* One Adam step (lr) on the field's loss; loss before the step to *loss (may be NULL). */
+int bkh_fsai_train_step(bkh_fsai_model *M, bkh_fsai_field *F, double lr, double *loss);
+/* Average ms of the factor build (MLP forward over all edges), HIP events. */
+int bkh_fsai_bench_forward(const bkh_fsai_model *M, bkh_fsai_field *F, int reps, double *avg_ms);
+/* PCG on the field's matrix with the factor last built by bkh_fsai_factor. */
+int bkh_fsai_pcg(bkh_fsai_field *F, const double *d_b, double *d_x, double tol, int max_iter,
+                 const bkh_opts *opts, bkh_info *info);
+/* PCG with kind 0 (none: CG) or 1 (Jacobi) on an SPD matrix created with its
+ * stored triangle. */
+int bkh_pcg(bkh_matrix *A, int kind, const double *d_b, double *d_x, double tol, int max_iter,
+            const bkh_opts *opts, bkh_info *info);
+
 /* Timing of the most recent solve on this thread (ms, HIP events). */
 double bkh_last_solve_ms(void);
 /* Average ms per launch of `reps` back-to-back SpMVs (FP64, or FP32 if fp32 = 1),

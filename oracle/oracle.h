@@ -92,6 +92,24 @@ int orc_blbicgstabl(int n, const int *rp, const int *ci, const double *v,
                     int s, const double *B, int l, double tol, int max_iter,
                     double *X, double *true_res);
 
+/* ---- learned FSAI preconditioner (orc_fsai.c); no reference counterpart ---- */
+int    orc_fsai_nparam(void);
+void   orc_fsai_rows(int n, const int *rp, const int *ci, const double *v, double *sqd, double *s,
+                     double *nlow, double *excess);
+void   orc_fsai_edge_features(int n, int i, int j, double aij, const double *sqd, const double *s,
+                              const double *nlow, const double *excess, float *f);
+void   orc_fsai_forward(int n, const int *rpl, const int *cil, const double *vl, const double *sqd,
+                        const double *s, const double *nlow, const double *excess,
+                        const float *params, int bf16, double *h_out);
+void   orc_fsai_gvals(int n, const int *rpl, const int *cil, const double *sqd, const double *h,
+                      double *g);
+double orc_fsai_loss_dg(int n, const int *rp, const int *ci, const double *v, const int *rpl,
+                        const int *cil, const double *g, int P, const double *Z, double *dg);
+void   orc_fsai_backward(int n, const int *rpl, const int *cil, const double *vl,
+                         const double *sqd, const double *s, const double *nlow,
+                         const double *excess, const float *params, int bf16, const double *h,
+                         const double *dg, double *grad);
+
 /* Time one call of a solver in seconds (CLOCK_MONOTONIC), for the CPU baseline. */
 double orc_wall_seconds(void);
 

@@ -235,3 +235,89 @@ def ref_sp_norm_f(n, rp, ci, v, symmetric):
     m = _Mat(_ip(rp), _ip(ci), ctypes.pointer(nz), ctypes.pointer(nn), ctypes.pointer(nn), _dp(v),
              b"csr", b"symmetric" if symmetric else b"unsymmetric")
     return R.sp_norm_f(ctypes.byref(m))
+
+
+# ------------------------------------------------- learned FSAI (orc_fsai.c)
+
+FP = ctypes.POINTER(ctypes.c_float)
+
+
+def _fp(a):
+    return a.ctypes.data_as(FP)
+
+
+def fsai_nparam():
+    return lib().orc_fsai_nparam()
+
+
+def fsai_rows(n, rp, ci, v):
+    """sqrt(a_ii), normalised coupling sums, lower-entry counts, diagonal excess of
+    the expanded symmetric matrix."""
+    rp, ci, v = _csr(rp, ci, v)
+    out = [np.empty(n) for _ in range(4)]
+    lib().orc_fsai_rows(n, _ip(rp), _ip(ci), _dp(v), *[_dp(o) for o in out])
+    return tuple(out)
+
+
+def fsai_features(n, rpl, cil, vl, rows):
+    rpl, cil, vl = _csr(rpl, cil, vl)
+    sqd, s, nlow, ex = rows
+    F = np.empty((int(rpl[-1]), 8), np.float32)
+    f = np.empty(8, np.float32)
+    for i in range(n):
+        for k in range(rpl[i], rpl[i + 1]):
+            lib().orc_fsai_edge_features(n, i, int(cil[k]), c_dbl(vl[k]), _dp(sqd), _dp(s), _dp(nlow),
+                                         _dp(ex), _fp(f))
+            F[k] = f
+    return F
+
+
+def fsai_forward(n, rpl, cil, vl, rows, params, bf16):
+    rpl, cil, vl = _csr(rpl, cil, vl)
+    sqd, s, nlow, ex = rows
+    p = np.ascontiguousarray(params, np.float32)
+    h = np.empty(int(rpl[-1]))
+    lib().orc_fsai_forward(n, _ip(rpl), _ip(cil), _dp(vl), _dp(sqd), _dp(s), _dp(nlow), _dp(ex), _fp(p),
+                           int(bf16), _dp(h))
+    return h
+
+
+def fsai_gvals(n, rpl, cil, sqd, h):
+    rpl, cil = np.ascontiguousarray(rpl, np.int32), np.ascontiguousarray(cil, np.int32)
+    g = np.empty(int(rpl[-1]))
+    lib().orc_fsai_gvals(n, _ip(rpl), _ip(cil), _dp(np.ascontiguousarray(sqd)), _dp(np.ascontiguousarray(h)),
+                         _dp(g))
+    return g
+
+
+def fsai_loss_dg(n, rp, ci, v, rpl, cil, g, Z):
+    rp, ci, v = _csr(rp, ci, v)
+    rpl, cil = np.ascontiguousarray(rpl, np.int32), np.ascontiguousarray(cil, np.int32)
+    Z = np.ascontiguousarray(Z, np.float64)
+    g = np.ascontiguousarray(g, np.float64)
+    dg = np.empty(int(rpl[-1]))
+    lib().orc_fsai_loss_dg.restype = c_dbl
+    L = lib().orc_fsai_loss_dg(n, _ip(rp), _ip(ci), _dp(v), _ip(rpl), _ip(cil), _dp(g), Z.shape[0], _dp(Z),
+                               _dp(dg))
+    return L, dg
+
+
+def fsai_backward(n, rpl, cil, vl, rows, params, bf16, h, dg):
+    rpl, cil, vl = _csr(rpl, cil, vl)
+    sqd, s, nlow, ex = rows
+    p = np.ascontiguousarray(params, np.float32)
+    grad = np.empty(fsai_nparam())
+    lib().orc_fsai_backward(n, _ip(rpl), _ip(cil), _dp(vl), _dp(sqd), _dp(s), _dp(nlow), _dp(ex), _fp(p),
+                            int(bf16), _dp(np.ascontiguousarray(h)), _dp(np.ascontiguousarray(dg)),
+                            _dp(grad))
+    return grad
+
+
+def fsai_loss_grad(n, rp, ci, v, rpl, cil, vl, params, Z, bf16=False):
+    """Loss and parameter gradient of the model on one field (A expanded rp/ci/v,
+    its lower triangle rpl/cil/vl) with probes Z (P x n)."""
+    rows = fsai_rows(n, rp, ci, v)
+    h = fsai_forward(n, rpl, cil, vl, rows, params, bf16)
+    g = fsai_gvals(n, rpl, cil, rows[0], h)
+    L, dg = fsai_loss_dg(n, rp, ci, v, rpl, cil, g, Z)
+    return L, fsai_backward(n, rpl, cil, vl, rows, params, bf16, h, dg), g
