@@ -167,6 +167,59 @@ def bench_cfg4(dv, gen):
                                  "tile_rows": layout[2], "fp32_spmv_ms": round(tune_ms, 3)}}
 
 
+def bench_cfg5(dv, gen, n_train=200, m_train=64, n_test=8, m_test=128, n_plain=2, epochs=4):
+    """Learned-FSAI PCG on variable-coefficient 3D diffusion (kappa = exp(GRF), sigma 2,
+    correlation 8 cells): train the edge MLP on the device over n_train seeded fields
+    (m_train^3 grids, same statistics), then solve held-out m_test^3 fields with
+    Jacobi PCG and learned-FSAI PCG (and unpreconditioned CG on n_plain of them)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from bksp_amd import fsai
+    t0 = time.perf_counter()
+    seeds = list(range(5000, 5000 + n_train))
+    with ThreadPoolExecutor(16) as ex:                 # host GRF generation (C, GIL released)
+        systems = list(ex.map(lambda s: fsai.field_system(s, m_train), seeds))
+    fields = [dv.FsaiField(n, *low, Z=fsai.probes(n, low, 10 * s + 1)) for s, (n, low) in zip(seeds, systems)]
+    del systems
+    prep_s = time.perf_counter() - t0
+    model = dv.FsaiModel(fsai.init_params())
+    t0 = time.perf_counter()
+    hist = fsai.train(model, fields, epochs=epochs, lr=3e-3, lr_final=3e-4)
+    from bksp_amd import load
+    load().bkh_sync()
+    train_s = time.perf_counter() - t0
+    fwd_ms = fields[0].bench_forward(model, reps=10)
+    edges = fields[0].nnzl
+    for f in fields:
+        f.destroy()
+    rows = []
+    for q in range(n_test):
+        rows.append(fsai.solve_compare(model, 6000 + q, m_test, plain=q < n_plain))
+    jac = np.array([r["jacobi"]["ms"] for r in rows]); lea = np.array([r["learned"]["ms"] for r in rows])
+    ji = np.array([r["jacobi"]["iters"] for r in rows]); li = np.array([r["learned"]["iters"] for r in rows])
+    plain = [r["plain_cg"] for r in rows if "plain_cg" in r]
+    pl_ms = float(np.mean([p["ms"] for p in plain])) if plain else None
+    lea_p = float(np.mean(lea[:len(plain)])) if plain else None
+    flops = 2 * (8 * 64 + 64 * 64 + 64) * edges
+    return {"workload": f"learned-FSAI PCG, FV diffusion kappa=exp(GRF sigma 2, corr 8), "
+                        f"train {n_train} fields {m_train}^3, test {n_test} held-out fields {m_test}^3, tol 1e-8",
+            "train": {"prep_s": round(prep_s, 1), "train_s": round(train_s, 2), "steps": epochs * n_train,
+                      "loss_first_epoch": round(hist[0], 5), "loss_last_epoch": round(hist[-1], 5)},
+            "mlp_forward": {"edges": edges, "ms": round(fwd_ms, 4),
+                            "TFLOPs": round(flops / (fwd_ms * 1e-3) / 1e12, 2), "bound": "mfma bf16",
+                            "frac_of_2500": round(flops / (fwd_ms * 1e-3) / 1e12 / 2500.0, 4)},
+            "jacobi": {"mean_iters": float(ji.mean()), "mean_ms": round(float(jac.mean()), 2),
+                       "max_true_res": max(r["jacobi"]["true_res"] for r in rows)},
+            "learned": {"mean_iters": float(li.mean()), "mean_ms": round(float(lea.mean()), 2),
+                        "max_true_res": max(r["learned"]["true_res"] for r in rows),
+                        "mean_factor_build_ms": round(float(np.mean([r["learned"]["factor_build_ms"] for r in rows])), 3)},
+            "plain_cg": {"fields": len(plain), "mean_ms": round(pl_ms, 1) if pl_ms else None,
+                         "mean_iters": float(np.mean([p["iters"] for p in plain])) if plain else None,
+                         "max_true_res": max(p["true_res"] for p in plain) if plain else None},
+            "learned_iters_vs_jacobi": round(float(li.mean() / ji.mean()), 3),
+            "speedup_learned_vs_jacobi": round(float(jac.mean() / lea.mean()), 3),
+            "speedup_learned_vs_plain_fp64_cg": round(pl_ms / lea_p, 2) if plain else None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,7 +227,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--m", type=int, default=128)
-    ap.add_argument("--configs", default="cfg1,cfg2b,cfg3,cfg4",
+    ap.add_argument("--configs", default="cfg1,cfg2b,cfg3,cfg4,cfg5",
                     help="extra workloads reported beside the main line ('' for none)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -267,7 +320,8 @@ def main():
             from oracle import pyoracle as po    # cpu_baseline legs only
         del M, db, dx, dy, db32, dy32
         for name, fn in (("cfg1", lambda: bench_cfg1(dv, gen, po)), ("cfg2_block", lambda: bench_cfg2_block(dv, gen)),
-                         ("cfg3", lambda: bench_cfg3(dv, gen)), ("cfg4", lambda: bench_cfg4(dv, gen))):
+                         ("cfg3", lambda: bench_cfg3(dv, gen)), ("cfg4", lambda: bench_cfg4(dv, gen)),
+                         ("cfg5", lambda: bench_cfg5(dv, gen))):
             key = {"cfg2_block": "cfg2b"}.get(name, name)
             if key in want:
                 try:
