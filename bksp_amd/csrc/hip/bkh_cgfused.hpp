@@ -24,7 +24,7 @@
 #include "bkh_core.hpp"
 #include "bkh_host.hpp"
 
-#define CGF_NPART 1024
+#define CGF_NPART 2048
 #define CGF_WPB 8                     // waves per workgroup of A
 enum { CGF_RR = 8, CGF_FRESH = 10 };  // Ctl::s slots (8, 9: r.r by parity)
 
@@ -149,6 +149,54 @@ __global__ __launch_bounds__(64 * CGF_WPB) void k_cgf_a(int nwblk, int chunk, co
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     block_partial<1, 64 * CGF_WPB>(dv, partA);
+}
+
+// A(k) on the SELL-64 layout: a lane owns a row and forms p_k at each column of it
+// in the gather; workgroup g owns `chunk` consecutive slices, its waves taking every
+// fourth. IT: int columns or short (column - row).
+template <typename VT, typename T, typename IT>
+__global__ __launch_bounds__(256) void k_cgf_a_sell(int n, int nsl, int chunk, const int *__restrict__ rp,
+                                                    const int *__restrict__ so, const VT *__restrict__ val,
+                                                    const IT *__restrict__ col, const T *r, T *P0, T *P1, T *q,
+                                                    Ctl *c, int j, const double *partB, int npB, double *partA) {
+    if (ctl_stopped(c)) return;
+    const int k = c->base + j;
+    const bool writer = blockIdx.x == 0 && threadIdx.x == 0;
+    double rr;
+    if (k == c->first) {
+        rr = c->s[CGF_RR + (k & 1)];
+    } else {
+        rr = wave_sum_parts(partB, npB);
+        if (cgf_check(c, k, rr, writer) != ST_RUNNING) return;
+        if (writer) c->s[CGF_RR + (k & 1)] = rr;
+    }
+    const bool fresh = k == c->first && c->s[CGF_FRESH] != 0.0;
+    const T be = fresh ? (T)0 : (T)(rr / c->s[CGF_RR + ((k + 1) & 1)]);
+    const T *pold = (k & 1) ? P1 : P0;
+    T *pnew = (k & 1) ? P0 : P1;
+    const int g = xcd_block(blockIdx.x, gridDim.x);
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int send = min(nsl, (g + 1) * chunk);
+    double dv[1] = {0.0};
+    for (int sl = g * chunk + w; sl < send; sl += 4) {
+        const int row = sl * 64 + l;
+        if (row >= n) continue;
+        const int len = rp[row + 1] - rp[row];
+        const int base = so[sl] + l;
+        T acc = (T)0;
+#pragma unroll 4
+        for (int kk = 0; kk < len; kk++) {
+            const int e = base + 64 * kk;
+            const int cc = sizeof(IT) == 2 ? row + (int)col[e] : (int)col[e];
+            const T pc = add_rn(mul_rn(be, pold[cc]), r[cc]);
+            acc = add_rn(acc, mul_rn((T)val[e], pc));
+        }
+        q[row] = acc;
+        const T pn = add_rn(mul_rn(be, pold[row]), r[row]);
+        pnew[row] = pn;
+        dv[0] += (double)pn * (double)acc;
+    }
+    block_partial<1, 256>(dv, partA);
 }
 
 // B(k): alpha_k = (r_k.r_k) / (p_k.q); x += alpha p_k, r -= alpha q; r.r partials.

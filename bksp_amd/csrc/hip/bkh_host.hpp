@@ -52,7 +52,12 @@ void bkh_set_error(const char *fmt, ...);
 // and needs no workgroup barrier; 4 waves per workgroup.
 #define SPMV_WROWS 64
 #define SPMV_WCAP 512
-enum { SPMV_BLOCK = 0, SPMV_WAVE = 1 };
+// SELL-64: rows in slices of 64 (one per wavefront); entry k of the slice's row l
+// sits at slice_offset + 64 k + l, padded to the slice's longest row, so a lane owns
+// a row and every load of a step is one contiguous 64-entry run. Used when padding
+// stays below SELL_MAX_PAD of the nonzeros.
+#define SELL_MAX_PAD 1.2
+enum { SPMV_BLOCK = 0, SPMV_WAVE = 1, SPMV_SELL = 2, SPMV_SELL32 = 3 };
 
 // Device-resident CSR. Rows are stored in the reference's accumulation order
 // (see bkh_matrix_create), so a sequential per-row sum reproduces src/matvec.c.
@@ -69,6 +74,12 @@ struct bkh_matrix {
     int *blk = nullptr;                // blk[nblk+1] row boundaries
     int nwblk = 0;                     // row blocks of the wave variant
     int *wblk = nullptr;               // wblk[nwblk+1]
+    int nsl = 0;                       // SELL-64 slices (0: no SELL copy)
+    int *sso = nullptr;                // slice offsets in entries, nsl + 1
+    double *sval = nullptr;            // SELL values (padding 0)
+    float *sval32 = nullptr;           // FP32 copy
+    int *scol = nullptr;               // SELL columns (padding: the row itself)
+    short *scd = nullptr;              // column - row, when every delta fits in 16 bits
     int variant = SPMV_WAVE;           // kernel used by every SpMV on this matrix
     int max_row = 0;                   // longest row (nonzeros)
     bkh_matrix *T = nullptr;           // A^T in the same format (BiCG)

@@ -207,8 +207,45 @@ __global__ __launch_bounds__(256) void k_spmv_wave(SpmvArgs<VT, YT> a, int nwblk
     if constexpr (NV > 0) block_partial<NV, 256>(dv, part);
 }
 
+// SELL-64 SpMV: one lane per row, four slices (256 rows) per workgroup. Step k of
+// a wave loads 64 consecutive values and columns (entry k of each of its rows) and
+// gathers x at those columns; each lane adds its row's products in stored order, so
+// the result is the sequential row sum of src/matvec.c:22-25 bit for bit. IT is int
+// (columns) or short (column - row).
+template <typename VT, typename YT, typename IT, int NV, class Fin>
+__global__ __launch_bounds__(256) void k_spmv_sell(const int *__restrict__ rp, const int *__restrict__ so,
+                                                   const VT *__restrict__ val, const IT *__restrict__ col, int n,
+                                                   const YT *x, YT *y, double alpha, double beta, int alpha_one,
+                                                   int has_beta, Epi<NV, YT> e, Ctl *ctl, double *part, Fin fin) {
+    if (ctl && ctl_stopped(ctl)) return;
+    const int row = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    double dv[NV > 0 ? NV : 1];
+#pragma unroll
+    for (int i = 0; i < NV; i++) dv[i] = 0.0;
+    if (row < n) {
+        const int sl = row >> 6;
+        const int len = rp[row + 1] - rp[row];
+        const int base = so[sl] + (row & 63);
+        const YT al = (YT)alpha;
+        YT s = has_beta ? mul_rn((YT)beta, y[row]) : (YT)0;
+#pragma unroll 4
+        for (int k = 0; k < len; k++) {
+            const int ei = base + 64 * k;
+            const int c = sizeof(IT) == 2 ? row + (int)col[ei] : (int)col[ei];
+            const YT v = (YT)val[ei];
+            s = add_rn(s, alpha_one ? mul_rn(v, x[c]) : mul_rn(mul_rn(al, v), x[c]));
+        }
+        y[row] = s;
+#pragma unroll
+        for (int i = 0; i < NV; i++)
+            dv[i] = epi_operand(e, e.a[i], row, (double)s) * epi_operand(e, e.b[i], row, (double)s);
+    }
+    if constexpr (NV > 0) block_partial<NV, 256>(dv, part);
+}
+
 // Number of workgroups a launch uses (also the number of partials it writes).
 static inline int spmv_grid(const bkh_matrix *A) {
+    if (A->variant == SPMV_SELL || A->variant == SPMV_SELL32) return (A->n + 255) / 256;
     return A->variant == SPMV_WAVE ? (A->nwblk + 3) / 4 : A->nblk;
 }
 
@@ -222,6 +259,20 @@ static inline int launch_spmv_kernel(hipStream_t s, const bkh_matrix *A, const V
     a.rp = A->rp; a.ci = A->ci; a.blk = A->blk; a.val = val; a.x = x; a.y = y;
     a.alpha = alpha; a.beta = beta; a.alpha_one = (alpha == 1.0); a.has_beta = (beta != 0.0);
     const int G = spmv_grid(A);
+    if (A->variant == SPMV_SELL || A->variant == SPMV_SELL32) {
+        const VT *sv;
+        if constexpr (sizeof(VT) == sizeof(float)) sv = (const VT *)A->sval32;
+        else sv = (const VT *)A->sval;
+        if (A->variant == SPMV_SELL && A->scd)
+            hipLaunchKernelGGL((k_spmv_sell<VT, YT, short, NV, Fin>), dim3(G), dim3(256), 0, s, A->rp, A->sso, sv,
+                               (const short *)A->scd, A->n, x, y, alpha, beta, (int)(alpha == 1.0),
+                               (int)(beta != 0.0), e, ctl, part, fin);
+        else
+            hipLaunchKernelGGL((k_spmv_sell<VT, YT, int, NV, Fin>), dim3(G), dim3(256), 0, s, A->rp, A->sso, sv,
+                               (const int *)A->scol, A->n, x, y, alpha, beta, (int)(alpha == 1.0),
+                               (int)(beta != 0.0), e, ctl, part, fin);
+        return G;
+    }
     if (A->variant == SPMV_WAVE) {
         a.blk = A->wblk;
         hipLaunchKernelGGL((k_spmv_wave<VT, YT, NV, Fin>), dim3(G), dim3(256), 0, s, a, A->nwblk, e,

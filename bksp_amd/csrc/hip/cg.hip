@@ -93,14 +93,24 @@ __global__ __launch_bounds__(256) void k_cg_p(long long n, T *p, const T *r, con
     ROWLOOP(i, n) p[i] = add_rn(mul_rn(be, p[i]), r[i]);
 }
 
-static inline bool cg_can_fuse(const bkh_matrix *A) { return A->max_row <= SPMV_WCAP; }
+static inline bool cgf_sell(const bkh_matrix *A) {
+    return A->nsl > 0 && (A->variant == SPMV_SELL || A->variant == SPMV_SELL32);
+}
 
-struct CgfGrid { int GA, chunk, GB; };
+static inline bool cg_can_fuse(const bkh_matrix *A) { return cgf_sell(A) || A->max_row <= SPMV_WCAP; }
+
+struct CgfGrid { int GA, chunk, GB; bool sell; };
 
 static CgfGrid cgf_grid(const bkh_matrix *A) {
     CgfGrid g;
-    g.GA = std::max(1, std::min((A->nwblk + CGF_WPB - 1) / CGF_WPB, CGF_NPART));
-    g.chunk = (A->nwblk + g.GA - 1) / g.GA;
+    g.sell = cgf_sell(A);
+    if (g.sell) {
+        g.GA = std::max(1, std::min((A->nsl + 3) / 4, CGF_NPART));
+        g.chunk = (A->nsl + g.GA - 1) / g.GA;
+    } else {
+        g.GA = std::max(1, std::min((A->nwblk + CGF_WPB - 1) / CGF_WPB, CGF_NPART));
+        g.chunk = (A->nwblk + g.GA - 1) / g.GA;
+    }
     g.GB = (int)std::max(1LL, std::min((long long)((A->n + 2047) / 2048), (long long)CGF_NPART));
     return g;
 }
@@ -114,7 +124,19 @@ static hipGraphExec_t cgf_capture(hipStream_t s, const bkh_matrix *A, const VT *
     const long long n = A->n;
     return capture_graph(s, 1, [&]() {
         for (int j = 0; j < K; j++) {
-            if (cd)
+            if (g.sell) {
+                const VT *sv;
+                if constexpr (sizeof(VT) == sizeof(float)) sv = (const VT *)A->sval32;
+                else sv = (const VT *)A->sval;
+                if (A->scd && A->variant == SPMV_SELL)
+                    hipLaunchKernelGGL((k_cgf_a_sell<VT, T, short>), dim3(g.GA), dim3(256), 0, s, (int)n, A->nsl,
+                                       g.chunk, A->rp, A->sso, sv, (const short *)A->scd, (const T *)r, P0, P1, q,
+                                       c, j, (const double *)partB, g.GB, partA);
+                else
+                    hipLaunchKernelGGL((k_cgf_a_sell<VT, T, int>), dim3(g.GA), dim3(256), 0, s, (int)n, A->nsl,
+                                       g.chunk, A->rp, A->sso, sv, (const int *)A->scol, (const T *)r, P0, P1, q,
+                                       c, j, (const double *)partB, g.GB, partA);
+            } else if (cd)
                 hipLaunchKernelGGL((k_cgf_a<VT, T, true>), dim3(g.GA), dim3(64 * CGF_WPB), 0, s, A->nwblk, g.chunk,
                                    A->rp, A->wblk, A->ci, cd, val, (const T *)r, P0, P1, q, c, j,
                                    (const double *)partB, g.GB, partA);

@@ -305,8 +305,69 @@ static void free_matrix(bkh_matrix *A) {
     if (!A) return;
     hipFree(A->rp); hipFree(A->ci); hipFree(A->val); hipFree(A->val32);
     hipFree(A->diag); hipFree(A->blk); hipFree(A->wblk); hipFree(A->cd16);
+    hipFree(A->sso); hipFree(A->sval); hipFree(A->sval32); hipFree(A->scol); hipFree(A->scd);
     if (A->T) free_matrix(A->T);
     delete A;
+}
+
+// SELL-64 copy of the matrix (see bkh_host.hpp) when its padding is small; the
+// default SpMV kernel for such matrices. Returns false only on a device error.
+static bool upload_sell(bkh_matrix *A, int n, const std::vector<int> &rp, const std::vector<int> &ci,
+                        const std::vector<double> &v, int want_fp32, hipStream_t s) {
+    const int nsl = (n + 63) / 64;
+    std::vector<int> so((size_t)nsl + 1, 0);
+    long long tot = 0;
+    for (int q = 0; q < nsl; q++) {
+        int w = 0;
+        for (int r = q * 64; r < std::min(n, q * 64 + 64); r++) w = std::max(w, rp[r + 1] - rp[r]);
+        tot += 64LL * w;
+        if (tot >= (1LL << 31) - 64) return true;             // keep CSR only
+        so[q + 1] = (int)tot;
+    }
+    if ((double)tot > SELL_MAX_PAD * (double)std::max(1LL, A->nnz) || A->nnz == 0) return true;
+    std::vector<double> sv((size_t)tot + 4, 0.0);
+    std::vector<int> sc((size_t)tot + 4, 0);
+    std::vector<short> sd((size_t)tot + 4, 0);
+    bool fits = true;
+    for (int q = 0; q < nsl; q++)
+        for (int r = q * 64; r < std::min(n, q * 64 + 64); r++) {
+            const int l = r & 63, w = (so[q + 1] - so[q]) / 64;
+            for (int k = 0; k < w; k++) {
+                const size_t e = (size_t)so[q] + 64 * (size_t)k + l;
+                if (k < rp[r + 1] - rp[r]) {
+                    const int c = ci[rp[r] + k];
+                    sv[e] = v[rp[r] + k];
+                    sc[e] = c;
+                    const int d = c - r;
+                    if (d < -32768 || d > 32767) fits = false;
+                    sd[e] = (short)d;
+                } else {
+                    sc[e] = r;                                   // padding: a valid index, never read
+                }
+            }
+        }
+    const size_t ne = (size_t)tot + 4;
+    bool ok = hipMalloc(&A->sso, sizeof(int) * (nsl + 1)) == hipSuccess &&
+              hipMalloc(&A->sval, sizeof(double) * ne) == hipSuccess &&
+              hipMalloc(&A->scol, sizeof(int) * ne) == hipSuccess;
+    if (ok && fits) ok = hipMalloc(&A->scd, sizeof(short) * ne) == hipSuccess;
+    if (ok && want_fp32) ok = hipMalloc(&A->sval32, sizeof(float) * ne) == hipSuccess;
+    if (!ok) return false;
+    ok = hipMemcpyAsync(A->sso, so.data(), sizeof(int) * (nsl + 1), hipMemcpyHostToDevice, s) == hipSuccess &&
+         hipMemcpyAsync(A->sval, sv.data(), sizeof(double) * ne, hipMemcpyHostToDevice, s) == hipSuccess &&
+         hipMemcpyAsync(A->scol, sc.data(), sizeof(int) * ne, hipMemcpyHostToDevice, s) == hipSuccess;
+    if (ok && fits) ok = hipMemcpyAsync(A->scd, sd.data(), sizeof(short) * ne, hipMemcpyHostToDevice, s) == hipSuccess;
+    if (ok && want_fp32) {
+        std::vector<float> s32(ne);
+        for (size_t e = 0; e < ne; e++) s32[e] = (float)sv[e];
+        ok = hipMemcpyAsync(A->sval32, s32.data(), sizeof(float) * ne, hipMemcpyHostToDevice, s) == hipSuccess &&
+             hipStreamSynchronize(s) == hipSuccess;
+    }
+    ok = ok && hipStreamSynchronize(s) == hipSuccess;
+    if (!ok) return false;
+    A->nsl = nsl;
+    A->variant = SPMV_SELL;
+    return true;
 }
 
 static bkh_matrix *upload(int n, const std::vector<int> &rp, const std::vector<int> &ci,
@@ -377,6 +438,7 @@ static bkh_matrix *upload(int n, const std::vector<int> &rp, const std::vector<i
         for (long long k = 0; k < A->nnz; k++) v32[k] = (float)v[k];
         ok = hipMemcpyAsync(A->val32, v32.data(), sizeof(float) * A->nnz, hipMemcpyHostToDevice, s) == hipSuccess;
     }
+    if (ok) ok = upload_sell(A, n, rp, ci, v, want_fp32, s);
     if (!ok || hipStreamSynchronize(s) != hipSuccess) {
         bkh_set_error("matrix upload failed");
         free_matrix(A);
@@ -422,10 +484,22 @@ extern "C" long long bkh_matrix_nnz(const bkh_matrix *A) { return A ? A->nnz : -
 extern "C" int bkh_matrix_rows(const bkh_matrix *A) { return A ? A->n : -1; }
 extern "C" int bkh_matrix_blocks(const bkh_matrix *A) { return A ? spmv_grid(A) : -1; }
 
-extern "C" int bkh_matrix_set_spmv(bkh_matrix *A, int variant) {
-    if (!A || (variant != SPMV_BLOCK && variant != SPMV_WAVE)) return BKH_BAD_ARGUMENT;
+// 0: workgroup CSR-stream, 1: wave CSR-stream, 2: SELL-64 (16-bit column deltas
+// when they fit), 3: SELL-64 with 32-bit columns. SELL needs the matrix to have a
+// SELL copy (small padding); the wave variant needs rows of <= SPMV_WCAP nonzeros.
+static int set_variant(bkh_matrix *A, int variant) {
+    if (variant == SPMV_WAVE && A->max_row > SPMV_WCAP) return BKH_BAD_ARGUMENT;
+    if ((variant == SPMV_SELL || variant == SPMV_SELL32) && A->nsl == 0) return BKH_BAD_ARGUMENT;
+    if (variant < SPMV_BLOCK || variant > SPMV_SELL32) return BKH_BAD_ARGUMENT;
     A->variant = variant;
-    if (A->T) A->T->variant = variant;
+    return 0;
+}
+
+extern "C" int bkh_matrix_set_spmv(bkh_matrix *A, int variant) {
+    if (!A) return BKH_BAD_ARGUMENT;
+    int rc = set_variant(A, variant);
+    if (rc) { bkh_set_error("SpMV variant %d is not available for this matrix", variant); return rc; }
+    if (A->T) set_variant(A->T, variant);
     return 0;
 }
 

@@ -478,6 +478,10 @@ extern "C" bkh_fsai_field *bkh_fsai_field_create(int n, const int *rpl, const in
     F->A = bkh_matrix_create(n, rpl, cil, vl, 1, 0, 0);
     F->G = bkh_matrix_create(n, rpl, cil, vl, 0, 1, 1);
     if (!F->A || !F->G) { fs_field_free(F); return nullptr; }
+    // the factor's values are rewritten in CSR order by the MLP: keep G and G^T on the
+    // CSR-stream kernel (no SELL copy to keep in step)
+    F->G->variant = SPMV_WAVE;
+    F->G->T->variant = SPMV_WAVE;
     std::vector<int> erow((size_t)nnzl), perm((size_t)nnzl), cnt((size_t)n + 1, 0);
     for (int i = 0; i < n; i++)
         for (int k = rpl[i]; k < rpl[i + 1]; k++) { erow[k] = i; cnt[cil[k] + 1]++; }

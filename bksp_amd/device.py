@@ -98,7 +98,8 @@ class Matrix:
             pass
 
     def set_spmv(self, variant):
-        """0: workgroup CSR-stream kernel, 1: wave CSR-stream kernel."""
+        """0: workgroup CSR-stream, 1: wave CSR-stream, 2: SELL-64 (16-bit column
+        deltas when they fit), 3: SELL-64 with 32-bit columns."""
         if load().bkh_matrix_set_spmv(self.handle, int(variant)):
             raise BkspDeviceError("bkh_matrix_set_spmv failed")
         self.nblocks = load().bkh_matrix_blocks(self.handle)
