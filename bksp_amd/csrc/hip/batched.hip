@@ -345,6 +345,7 @@ static void bt_free(bkh_batch *B) {
 }
 
 static void bt_free_layout(bkh_batch *B) {
+    bkh_graph_epoch_bump();
     hipFree(B->wblk); hipFree(B->usys); hipFree(B->ub);
     hipFree(B->trow); hipFree(B->tsys); hipFree(B->tb);
     B->wblk = B->usys = B->ub = B->trow = B->tsys = B->tb = nullptr;
@@ -463,6 +464,7 @@ extern "C" bkh_batch *bkh_batch_create(int ns, const int *n_each, const int *con
 
 extern "C" void bkh_batch_destroy(bkh_batch *B) {
     if (!B) return;
+    bkh_graph_epoch_bump();
     bt_free(B);
     delete B;
 }
@@ -536,7 +538,9 @@ static int bt_core(ThreadCtx *t, const bkh_batch *B, const VT *val, const T *b, 
                        (const BtSys *)sy, part);
     bt_fin<BT_INIT, 2>(s, B, part, false, sy, L);
     LAUNCH_OK();
-    hipGraphExec_t ge = capture_graph(s, K, [&]() {
+    hipGraphExec_t ge = capture_cached(t, {0xBA00, (uintptr_t)B, (uintptr_t)val, (uintptr_t)b, (uintptr_t)x, (uintptr_t)R,
+                                           (uintptr_t)P, (uintptr_t)rs, (uintptr_t)sy, (uintptr_t)c, (uintptr_t)part,
+                                           (uintptr_t)L, (uintptr_t)sizeof(T)}, s, K, [&]() {
         for (int j = 0; j < L; j++) {
             hipLaunchKernelGGL(k_bt_p<T>, dim3(B->ntile), dim3(256), 0, s, B->trow, B->tsys, N, j, P, (const T *)R,
                                (const BtSys *)sy);
@@ -558,7 +562,6 @@ static int bt_core(ThreadCtx *t, const bkh_batch *B, const VT *val, const T *b, 
     });
     if (!ge) { bkh_set_error("hipGraph capture of the batched iteration failed"); return BKH_DEVICE_ERROR; }
     int rc = run_graph_until_done(t, ge, c, max_it / K + 2);
-    hipGraphExecDestroy(ge);
     return rc;
 }
 

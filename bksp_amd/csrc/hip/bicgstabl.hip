@@ -200,7 +200,9 @@ static int bl_core(ThreadCtx *t, const bkh_matrix *A, const VT *val, int l, cons
     hipLaunchKernelGGL(k_bl_init<T>, dim3(G), dim3(256), 0, s, n, b, R, P, rs, part);
     launch_finalize<2>(s, part, G, c, BlInitFin());
     LAUNCH_OK();
-    hipGraphExec_t ge = capture_graph(s, K, [&]() {
+    hipGraphExec_t ge = capture_cached(t, {0xB100, (uintptr_t)A, (uintptr_t)val, (uintptr_t)b, (uintptr_t)x, (uintptr_t)R,
+                                           (uintptr_t)P, (uintptr_t)rs, (uintptr_t)c, (uintptr_t)part, (uintptr_t)l,
+                                           (uintptr_t)sizeof(T)}, s, K, [&]() {
         for (int j = 0; j < l; j++) {
             hipLaunchKernelGGL(k_bl_p<T>, dim3(G), dim3(256), 0, s, n, j, P, (const T *)R, c);
             launch_spmv_dot_w<VT, T>(s, A, val, P + j * n, P + (j + 1) * n, rs, c, part, BlAlphaFin());
@@ -215,7 +217,6 @@ static int bl_core(ThreadCtx *t, const bkh_matrix *A, const VT *val, int l, cons
     });
     if (!ge) { bkh_set_error("hipGraph capture of the BiCGSTAB(l) iteration failed"); return BKH_DEVICE_ERROR; }
     int rc = run_graph_until_done(t, ge, c, max_it / K + 2);
-    hipGraphExecDestroy(ge);
     return rc;
 }
 

@@ -43,6 +43,29 @@ __device__ __forceinline__ double wave_sum_parts(const double *part, int np) {
     return wave_sum(a);
 }
 
+// The same sum formed by a whole workgroup of BS threads (each reads np/BS partials,
+// then a fixed-order workgroup reduction), broadcast to every thread: every workgroup
+// of one kernel computes identical bits, and the partial array is read once per
+// workgroup instead of once per wave.
+template <int BS>
+__device__ __forceinline__ double block_sum_parts(const double *part, int np) {
+    __shared__ double sh[BS / 64 + 1];
+    double a = 0.0;
+    for (int i = threadIdx.x; i < np; i += BS) a += part[i];
+    a = wave_sum(a);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) sh[w] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = sh[0];
+#pragma unroll
+        for (int q = 1; q < BS / 64; q++) t += sh[q];
+        sh[BS / 64] = t;
+    }
+    __syncthreads();
+    return sh[BS / 64];
+}
+
 // Convergence test of iteration k-1 given r_k.r_k (the reference's test, cg.c:56-59;
 // NaN/Inf stops with ST_BREAKDOWN). Returns the status; workgroup 0 records it.
 __device__ __forceinline__ int cgf_check(Ctl *c, int k, double rr, bool writer) {
@@ -78,7 +101,7 @@ __global__ __launch_bounds__(64 * CGF_WPB) void k_cgf_a(int nwblk, int chunk, co
     if (k == c->first) {
         rr = c->s[CGF_RR + (k & 1)];
     } else {
-        rr = wave_sum_parts(partB, npB);
+        rr = block_sum_parts<64 * CGF_WPB>(partB, npB);
         if (cgf_check(c, k, rr, writer) != ST_RUNNING) return;
         if (writer) c->s[CGF_RR + (k & 1)] = rr;
     }
@@ -166,7 +189,7 @@ __global__ __launch_bounds__(256) void k_cgf_a_sell(int n, int nsl, int chunk, c
     if (k == c->first) {
         rr = c->s[CGF_RR + (k & 1)];
     } else {
-        rr = wave_sum_parts(partB, npB);
+        rr = block_sum_parts<256>(partB, npB);
         if (cgf_check(c, k, rr, writer) != ST_RUNNING) return;
         if (writer) c->s[CGF_RR + (k & 1)] = rr;
     }
@@ -206,7 +229,7 @@ __global__ __launch_bounds__(512) void k_cgf_b(long long n, T *x, T *r, const T 
                                                double *partB) {
     if (ctl_stopped(c)) return;
     const int k = c->base + j;
-    const double pq = wave_sum_parts(partA, npA);
+    const double pq = block_sum_parts<512>(partA, npA);
     if (bad(pq)) {                                    // breakdown (zero or non-finite p.q)
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             c->status = ST_BREAKDOWN; c->result_it = k; c->done = 1;

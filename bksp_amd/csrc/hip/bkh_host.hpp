@@ -6,6 +6,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <vector>
+#include <utility>
 #include "bkh_core.hpp"
 #include "../../../include/bksp_hip.h"
 
@@ -94,6 +95,7 @@ struct ThreadCtx {
     Ctl *ctl_host = nullptr;           // pinned mirror of the control block
     double *red2 = nullptr;            // second-level partial sums (RED2_CAP doubles)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<std::pair<std::vector<uintptr_t>, hipGraphExec_t>> graphs;   // graph cache
 };
 ThreadCtx *bkh_tctx();                 // nullptr (with error set) if init failed
 #define RED2_CAP (1 << 17)
@@ -133,6 +135,29 @@ hipGraphExec_t capture_graph(hipStream_t s, int K, Body body) {
     if (hipStreamEndCapture(s, &g) != hipSuccess || !g) return nullptr;
     if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) ge = nullptr;
     hipGraphDestroy(g);
+    return ge;
+}
+
+// Cache of instantiated iteration graphs, per thread. A solver's graph depends only
+// on the matrix, the vectors and workspace it was captured with and a few launch
+// parameters; the key holds all of them plus a global epoch that is advanced
+// whenever a matrix is destroyed or re-laid out (so an address reused by another
+// matrix never matches). Cached graphs are owned by the cache.
+typedef std::vector<uintptr_t> GraphKey;
+hipGraphExec_t graph_cache_find(ThreadCtx *t, const GraphKey &key);
+void graph_cache_put(ThreadCtx *t, const GraphKey &key, hipGraphExec_t ge);
+uintptr_t bkh_graph_epoch();
+void bkh_graph_epoch_bump();
+
+template <class Body>
+hipGraphExec_t capture_cached(ThreadCtx *t, GraphKey key, hipStream_t s, int K, Body body) {
+    key.push_back(bkh_graph_epoch());
+    key.push_back((uintptr_t)t->ws);
+    key.push_back((uintptr_t)t->ws_bytes);
+    key.push_back((uintptr_t)K);
+    if (hipGraphExec_t ge = graph_cache_find(t, key)) return ge;
+    hipGraphExec_t ge = capture_graph(s, K, body);
+    if (ge) graph_cache_put(t, key, ge);
     return ge;
 }
 

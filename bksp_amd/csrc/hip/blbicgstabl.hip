@@ -372,7 +372,9 @@ static int blbicgstabl_dispatch(ThreadCtx *t, const bkh_matrix *A, int l, const 
                        (const Ctl *)c);
     hipLaunchKernelGGL(k_bb_copy2, dim3(Gv), dim3(256), 0, s, ms, (const double *)V, P, Rs);
     LAUNCH_OK();
-    hipGraphExec_t ge = capture_graph(s, K, [&]() {
+    hipGraphExec_t ge = capture_cached(t, {0xBB00, (uintptr_t)A, (uintptr_t)Xt, (uintptr_t)Rs, (uintptr_t)tmp, (uintptr_t)P,
+                                           (uintptr_t)V, (uintptr_t)R, (uintptr_t)mt, (uintptr_t)c, (uintptr_t)part,
+                                           (uintptr_t)l, (uintptr_t)S}, s, K, [&]() {
         for (int j = 0; j < l; j++) {
             double *Vj = V + j * ms, *Pj = P + j * ms;
             hipLaunchKernelGGL((k_gram<S, double>), dim3(G), dim3(256), 0, s, m, (const double *)Rs,
@@ -415,7 +417,6 @@ static int blbicgstabl_dispatch(ThreadCtx *t, const bkh_matrix *A, int l, const 
     });
     if (!ge) { bkh_set_error("hipGraph capture of the block BiCGSTAB(l) iteration failed"); return BKH_DEVICE_ERROR; }
     int rc = run_graph_until_done(t, ge, c, max_iter / K + 2);
-    hipGraphExecDestroy(ge);
     if (rc) return rc;
     Ctl hc = *t->ctl_host;
     hipLaunchKernelGGL((k_rm2cm<double, double>), dim3(Gv), dim3(256), 0, s, m, S, (const double *)Xt, X);

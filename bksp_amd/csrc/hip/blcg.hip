@@ -222,7 +222,9 @@ static int blcg_core(ThreadCtx *t, const bkh_matrix *A, const VT *val, const T *
     hipLaunchKernelGGL((k_blk_vp<S, T>), dim3(G), dim3(256), 0, s, m, V, P, mt,
                        p0_mode ? 1.0 : 2.0, c);
     LAUNCH_OK();
-    hipGraphExec_t ge = capture_graph(s, K, [&]() {
+    hipGraphExec_t ge = capture_cached(t, {0xB1C6, (uintptr_t)A, (uintptr_t)val, (uintptr_t)Bt, (uintptr_t)Xt, (uintptr_t)V,
+                                           (uintptr_t)P, (uintptr_t)AP, (uintptr_t)mt, (uintptr_t)c, (uintptr_t)part,
+                                           (uintptr_t)S, (uintptr_t)sizeof(T)}, s, K, [&]() {
         launch_spmm<S>(s, A, val, (const T *)P, AP, 1.0, 0.0, c);
         hipLaunchKernelGGL((k_gram<S, T>), dim3(G), dim3(256), 0, s, m, (const T *)P, (const T *)AP, c, part);
         launch_blk_fin<S, FIN_ALPHA>(s, part, G, mt, c);
@@ -235,7 +237,6 @@ static int blcg_core(ThreadCtx *t, const bkh_matrix *A, const VT *val, const T *
     });
     if (!ge) { bkh_set_error("hipGraph capture of the block CG iteration failed"); return BKH_DEVICE_ERROR; }
     int rc = run_graph_until_done(t, ge, c, max_it / K + 2);
-    hipGraphExecDestroy(ge);
     return rc;
 }
 

@@ -118,11 +118,14 @@ static CgfGrid cgf_grid(const bkh_matrix *A) {
 // Graph of K fused iterations plus the tail, on vectors T with values VT
 // (cd: 16-bit column deltas or null).
 template <typename VT, typename T>
-static hipGraphExec_t cgf_capture(hipStream_t s, const bkh_matrix *A, const VT *val, const short *cd, T *x,
-                                  T *r, T *P0, T *P1, T *q, Ctl *c, double *partA, double *partB, int K) {
+static hipGraphExec_t cgf_capture(ThreadCtx *t, hipStream_t s, const bkh_matrix *A, const VT *val, const short *cd,
+                                  T *x, T *r, T *P0, T *P1, T *q, Ctl *c, double *partA, double *partB, int K) {
     const CgfGrid g = cgf_grid(A);
     const long long n = A->n;
-    return capture_graph(s, 1, [&]() {
+    GraphKey key = {0xC6F0, (uintptr_t)A, (uintptr_t)val, (uintptr_t)cd, (uintptr_t)x, (uintptr_t)r, (uintptr_t)P0,
+                    (uintptr_t)P1, (uintptr_t)q, (uintptr_t)c, (uintptr_t)partA, (uintptr_t)partB,
+                    (uintptr_t)sizeof(T), (uintptr_t)K};
+    return capture_cached(t, key, s, 1, [&]() {
         for (int j = 0; j < K; j++) {
             if (g.sell) {
                 const VT *sv;
@@ -180,9 +183,10 @@ static int cg_fp64(ThreadCtx *t, const bkh_matrix *A, const double *b, double *x
     LAUNCH_OK();
     hipGraphExec_t ge;
     if (fused) {
-        ge = cgf_capture<double, double>(s, A, A->val, nullptr, x, r, P, P + n, q, c, partA, partB, K);
+        ge = cgf_capture<double, double>(t, s, A, A->val, nullptr, x, r, P, P + n, q, c, partA, partB, K);
     } else {
-        ge = capture_graph(s, K, [&]() {
+        ge = capture_cached(t, {0xC6F1, (uintptr_t)A, (uintptr_t)x, (uintptr_t)r, (uintptr_t)P, (uintptr_t)q,
+                                (uintptr_t)c, (uintptr_t)part}, s, K, [&]() {
             launch_spmv_dot_self<double, double>(s, A, A->val, P, q, c, part, CgAlphaFin());
             hipLaunchKernelGGL(k_cg_xr<double>, dim3(G), dim3(256), 0, s, n, x, r, P, q, c, part);
             launch_finalize<1>(s, part, G, c, CgBetaFin());
@@ -191,7 +195,6 @@ static int cg_fp64(ThreadCtx *t, const bkh_matrix *A, const double *b, double *x
     }
     if (!ge) { bkh_set_error("hipGraph capture of the CG iteration failed"); return BKH_DEVICE_ERROR; }
     int rc = run_graph_until_done(t, ge, c, max_it / K + 2);
-    hipGraphExecDestroy(ge);
     return rc;
 }
 
@@ -268,9 +271,10 @@ static int cg_reliable(ThreadCtx *t, const bkh_matrix *A, const double *b, doubl
     if (ctl_init(t, co, 0.0, 1) || ctl_init(t, ci, 1.0, max_iter)) return BKH_DEVICE_ERROR;
     hipGraphExec_t ge;
     if (fused) {
-        ge = cgf_capture<float, float>(s, A, A->val32, cd, x32, r32, P, P + n, q, ci, partA, partB, K);
+        ge = cgf_capture<float, float>(t, s, A, A->val32, cd, x32, r32, P, P + n, q, ci, partA, partB, K);
     } else {
-        ge = capture_graph(s, K, [&]() {
+        ge = capture_cached(t, {0xC6F2, (uintptr_t)A, (uintptr_t)x32, (uintptr_t)r32, (uintptr_t)P, (uintptr_t)q,
+                                (uintptr_t)ci, (uintptr_t)part}, s, K, [&]() {
             launch_spmv_dot_self<float, float>(s, A, A->val32, P, q, ci, part, CgAlphaFin());
             hipLaunchKernelGGL(k_cg_xr<float>, dim3(G), dim3(256), 0, s, n, x32, r32, P, q, ci, part);
             launch_finalize<1>(s, part, G, ci, CgBetaFin());
@@ -305,7 +309,6 @@ static int cg_reliable(ThreadCtx *t, const bkh_matrix *A, const double *b, doubl
         if ((rc = run_graph_until_done(t, ge, ci, (max_iter - used) / K + 2))) break;
         updates++;
     }
-    hipGraphExecDestroy(ge);
     if (rc) return rc;
     if (ctl_fetch(t, ci)) return BKH_DEVICE_ERROR;
     const int iters = t->ctl_host->it;

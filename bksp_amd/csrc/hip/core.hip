@@ -1,4 +1,5 @@
 // core.hip -- runtime, memory, matrix upload and single-kernel entry points.
+#include <atomic>
 #include <mutex>
 #include <new>
 #include <stdlib.h>
@@ -87,6 +88,7 @@ void *bkh_workspace(size_t bytes) {
     if (bytes <= t->ws_bytes) return t->ws;
     if (t->ws) {
         hipStreamSynchronize(t->stream);
+        bkh_graph_epoch_bump();
         hipFree(t->ws);
         t->ws = nullptr;
         t->ws_bytes = 0;
@@ -99,6 +101,37 @@ void *bkh_workspace(size_t bytes) {
     }
     t->ws_bytes = want;
     return t->ws;
+}
+
+// ----------------------------------------------------------- graph cache
+
+static std::atomic<uintptr_t> g_graph_epoch{1};
+uintptr_t bkh_graph_epoch() { return g_graph_epoch.load(); }
+void bkh_graph_epoch_bump() { g_graph_epoch.fetch_add(1); }
+
+#define GRAPH_CACHE_MAX 24
+
+hipGraphExec_t graph_cache_find(ThreadCtx *t, const GraphKey &key) {
+    for (auto &e : t->graphs)
+        if (e.first == key) return e.second;
+    return nullptr;
+}
+
+void graph_cache_put(ThreadCtx *t, const GraphKey &key, hipGraphExec_t ge) {
+    const uintptr_t ep = bkh_graph_epoch();
+    // drop graphs of past epochs, then the oldest if still full (all work on the
+    // thread's stream has completed before a new capture begins)
+    std::vector<std::pair<std::vector<uintptr_t>, hipGraphExec_t>> keep;
+    for (auto &e : t->graphs) {
+        if (e.first.size() >= 4 && e.first[e.first.size() - 4] == ep) keep.push_back(e);
+        else hipGraphExecDestroy(e.second);
+    }
+    if (keep.size() >= GRAPH_CACHE_MAX) {
+        hipGraphExecDestroy(keep.front().second);
+        keep.erase(keep.begin());
+    }
+    keep.emplace_back(key, ge);
+    t->graphs.swap(keep);
 }
 
 extern "C" int bkh_device_name(char *buf, int len) {
@@ -477,6 +510,7 @@ extern "C" bkh_matrix *bkh_matrix_create(int n, const int *row_ptr, const int *c
 extern "C" void bkh_matrix_destroy(bkh_matrix *A) {
     ThreadCtx *t = bkh_tctx();
     if (t) hipStreamSynchronize(t->stream);
+    bkh_graph_epoch_bump();
     free_matrix(A);
 }
 
@@ -492,6 +526,7 @@ static int set_variant(bkh_matrix *A, int variant) {
     if ((variant == SPMV_SELL || variant == SPMV_SELL32) && A->nsl == 0) return BKH_BAD_ARGUMENT;
     if (variant < SPMV_BLOCK || variant > SPMV_SELL32) return BKH_BAD_ARGUMENT;
     A->variant = variant;
+    bkh_graph_epoch_bump();
     return 0;
 }
 

@@ -140,7 +140,8 @@ extern "C" int bkh_cr(bkh_matrix *A, const double *b, double *x, double tol, int
     launch_spmv<double, double, 2>(s, A, A->val, r, tv, 1.0, 0.0, e2, c, part, CrInitFin());
     hipLaunchKernelGGL(k_copy_vec, dim3(G), dim3(256), 0, s, n, tv, ap, c);
     LAUNCH_OK();
-    hipGraphExec_t ge = capture_graph(s, K, [&]() {
+    hipGraphExec_t ge = capture_cached(t, {0xC200, (uintptr_t)A, (uintptr_t)b, (uintptr_t)x, (uintptr_t)r, (uintptr_t)p,
+                                           (uintptr_t)ap, (uintptr_t)tv, (uintptr_t)c, (uintptr_t)part}, s, K, [&]() {
         hipLaunchKernelGGL(k_cr_xr, dim3(G), dim3(256), 0, s, n, x, r, p, ap, c, part);
         launch_finalize<1>(s, part, G, c, CrRRFin());
         launch_spmv_dot_self<double, double>(s, A, A->val, r, tv, c, part, CrBetaFin());
@@ -149,7 +150,6 @@ extern "C" int bkh_cr(bkh_matrix *A, const double *b, double *x, double tol, int
     });
     if (!ge) { bkh_set_error("hipGraph capture of the CR iteration failed"); return BKH_DEVICE_ERROR; }
     int rc = run_graph_until_done(t, ge, c, max_iter / K + 2);
-    hipGraphExecDestroy(ge);
     if (rc) return rc;
     return finish_solve(t, A, 0.0, b, x, &inf, info);
 }
@@ -257,7 +257,9 @@ extern "C" int bkh_bicg(bkh_matrix *A, const double *b, double *x, double tol, i
     hipLaunchKernelGGL(k_bg_init, dim3(G), dim3(256), 0, s, n, b, ap, r, sr, p, sp, part);
     launch_finalize<2>(s, part, G, c, BgInitFin());
     LAUNCH_OK();
-    hipGraphExec_t ge = capture_graph(s, K, [&]() {
+    hipGraphExec_t ge = capture_cached(t, {0xB1C0, (uintptr_t)A, (uintptr_t)b, (uintptr_t)x, (uintptr_t)r, (uintptr_t)sr,
+                                           (uintptr_t)p, (uintptr_t)sp, (uintptr_t)ap, (uintptr_t)atp, (uintptr_t)c,
+                                           (uintptr_t)part}, s, K, [&]() {
         launch_spmv_dot_w<double, double>(s, A, A->val, p, ap, sp, c, part, BgAlphaFin());
         Epi<0> ez;
         memset(&ez, 0, sizeof ez);
@@ -268,7 +270,6 @@ extern "C" int bkh_bicg(bkh_matrix *A, const double *b, double *x, double tol, i
     });
     if (!ge) { bkh_set_error("hipGraph capture of the BiCG iteration failed"); return BKH_DEVICE_ERROR; }
     int rc = run_graph_until_done(t, ge, c, max_iter / K + 2);
-    hipGraphExecDestroy(ge);
     if (rc) return rc;
     return finish_solve(t, A, 0.0, b, x, &inf, info);
 }

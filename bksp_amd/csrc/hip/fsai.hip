@@ -764,7 +764,9 @@ static int pcg_run(ThreadCtx *t, const bkh_matrix *A, const bkh_matrix *G, int k
     if (fsai) fs_apply(s, G, r, y, z, c, part, PcRzFin{1});
     hipLaunchKernelGGL(k_pcg_p, dim3(Gv), dim3(256), 0, s, n, p, (const double *)z, (const Ctl *)c, 1);
     LAUNCH_OK();
-    hipGraphExec_t ge = capture_graph(s, K, [&]() {
+    hipGraphExec_t ge = capture_cached(t, {0xF5A1, (uintptr_t)A, (uintptr_t)G, (uintptr_t)kind, (uintptr_t)b, (uintptr_t)x,
+                                           (uintptr_t)r, (uintptr_t)z, (uintptr_t)p, (uintptr_t)q, (uintptr_t)y,
+                                           (uintptr_t)c, (uintptr_t)part}, s, K, [&]() {
         Epi<1> e1;
         memset(&e1, 0, sizeof e1);
         e1.w[0] = p;
@@ -779,7 +781,6 @@ static int pcg_run(ThreadCtx *t, const bkh_matrix *A, const bkh_matrix *G, int k
     });
     if (!ge) { bkh_set_error("hipGraph capture of the PCG iteration failed"); return BKH_DEVICE_ERROR; }
     int rc = run_graph_until_done(t, ge, c, max_iter / K + 2);
-    hipGraphExecDestroy(ge);
     if (rc) return rc;
     Ctl hc = *t->ctl_host;
     HIP_OK(hipEventRecord(t->ev1, s));

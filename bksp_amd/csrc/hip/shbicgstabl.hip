@@ -448,7 +448,9 @@ static int sh_core(ThreadCtx *t, const bkh_matrix *A, int l, const double *b, do
     hipLaunchKernelGGL(k_sh_init, dim3(G), dim3(256), 0, s, n, b, R, P, rs, part);
     launch_finalize<2>(s, part, G, c, ShInitFin{st});
     LAUNCH_OK();
-    hipGraphExec_t ge = capture_graph(s, K, [&]() {
+    hipGraphExec_t ge = capture_cached(t, {0x5B00, (uintptr_t)A, (uintptr_t)b, (uintptr_t)x, (uintptr_t)R, (uintptr_t)P,
+                                           (uintptr_t)rs, (uintptr_t)PS, (uintptr_t)st, (uintptr_t)c, (uintptr_t)part,
+                                           (uintptr_t)ns, (uintptr_t)l, (uintptr_t)sizeof(PT)}, s, K, [&]() {
         for (int j = 0; j < l; j++) {
             hipLaunchKernelGGL(k_sh_p, dim3(G), dim3(256), 0, s, n, j, P, (const double *)R, c);
             sh_alpha_spmv(s, A, P + j * n, P + (j + 1) * n, rs, c, part, st);
@@ -466,7 +468,6 @@ static int sh_core(ThreadCtx *t, const bkh_matrix *A, int l, const double *b, do
     });
     if (!ge) { bkh_set_error("hipGraph capture of the shifted BiCGSTAB(l) iteration failed"); return BKH_DEVICE_ERROR; }
     int rc = run_graph_until_done(t, ge, c, max_it / K + 2);
-    hipGraphExecDestroy(ge);
     return rc;
 }
 
