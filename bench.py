@@ -272,13 +272,21 @@ def main():
     ir = [solve(precision=dv.BKH_FP32_IR) for _ in range(max(1, args.steps // 2))]
     ir_wall = (time.perf_counter() - t0) * 1e3 / len(ir)
 
-    # dominant kernel: the FP64 CSR SpMV, timed with HIP events on its stream
+    # dominant kernel of the timed solve: CG's fused kernel A (SELL SpMV with the
+    # direction update p = r + beta p in its gather), timed with HIP events on its
+    # stream; the plain SpMV beside it.
+    nnz = int(M.nnz)
+    M.bench_cg_kernel(reps=5)
+    cga_ms = M.bench_cg_kernel(reps=50)
+    cga_alg = 12 * nnz + 4 * (n + 1) + 32 * n           # A, row pointers, p_old and r read, q and p_new written
+    cga_fmt = M.spmv_bytes() - 16 * n + 32 * n            # as stored (SELL entries, 16-bit column deltas)
+    cga_gbs = cga_alg / (cga_ms * 1e-3) / 1e9
     dy = dv.DeviceArray((n,))
     dv.bench_spmv(M, db, dy, reps=5)
     spmv_ms = dv.bench_spmv(M, db, dy, reps=50)
-    nnz = int(M.nnz)
     bytes64 = spmv_bytes(n, nnz)
     gbs = bytes64 / (spmv_ms * 1e-3) / 1e9
+    fmt64 = M.spmv_bytes()
     db32 = dv.DeviceArray.from_numpy(b.astype(np.float32))
     dy32 = dv.DeviceArray((n,), np.float32)
     spmv32_ms = dv.bench_spmv(M, db32, dy32, reps=50, fp32=True)
@@ -289,15 +297,25 @@ def main():
     copy_gbs = 2 * big / (dv.bench_copy(ca, cb, big, reps=20) * 1e-3) / 1e9
     ca.free(); cb.free()
 
-    traffic = load_traffic("spmv_fp64_poisson128_traffic.json")
-    roofline = {"bound": "hbm", "kernel": "k_spmv<double> (CSR-stream FP64)",
-                "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs / HBM_PEAK_GBS, 4),
-                "traffic": traffic.get("bytes_per_launch") if traffic else None,
-                "algorithmic_bytes": bytes64, "avg_launch_us": round(spmv_ms * 1e3, 2),
+    t_cga = load_traffic("cg_fused_poisson128_traffic.json")
+    t_spmv = load_traffic("spmv_fp64_poisson128_traffic.json")
+    roofline = {"bound": "hbm", "kernel": "k_cgf_a_sell<double,double,short> (CG: SELL-64 SpMV with p = r + beta p "
+                                          "in the gather)",
+                "achieved": round(cga_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(cga_gbs / HBM_PEAK_GBS, 4),
+                "traffic": t_cga.get("bytes_per_launch") if t_cga else None,
+                "algorithmic_bytes": cga_alg, "avg_launch_us": round(cga_ms * 1e3, 2),
+                "format_bytes": cga_fmt, "format_GBs": round(cga_fmt / (cga_ms * 1e-3) / 1e9, 1),
                 "measured_copy_GBs": round(copy_gbs, 1),
+                "spmv": {"kernel": "k_spmv_sell<double,double,short> (SELL-64, 16-bit column deltas)",
+                         "achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                         "algorithmic_bytes": bytes64, "avg_launch_us": round(spmv_ms * 1e3, 2),
+                         "format_bytes": fmt64, "format_GBs": round(fmt64 / (spmv_ms * 1e-3) / 1e9, 1),
+                         "format_frac": round(fmt64 / (spmv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "traffic": t_spmv.get("bytes_per_launch") if t_spmv else None},
                 "fp32_spmv": {"achieved": round(gbs32, 1), "avg_launch_us": round(spmv32_ms * 1e3, 2),
-                              "algorithmic_bytes": spmv_bytes(n, nnz, 4, 4)}}
+                              "algorithmic_bytes": spmv_bytes(n, nnz, 4, 4),
+                              "format_bytes": M.spmv_bytes(fp32=True)}}
 
     cpu = None
     if not args.no_cpu:

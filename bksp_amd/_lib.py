@@ -99,6 +99,8 @@ _SIGS = {
     "bkh_batch_spmv": (c_int, [c_vp, c_int, c_vp, c_vp]),
     "bkh_batch_bicgstabl": (c_int, [c_vp, c_vp, c_int, c_vp, c_dbl, c_int, ctypes.POINTER(BkhOpts), c_ip,
                                     c_dp, ctypes.POINTER(BkhInfo)]),
+    "bkh_matrix_spmv_bytes": (c_ll, [c_vp, c_int]),
+    "bkh_bench_cg_kernel": (c_int, [c_vp, c_int, c_int, c_dp]),
     # learned FSAI and PCG
     "bkh_fsai_nparam": (c_int, []),
     "bkh_fsai_model_create": (c_vp, [ctypes.POINTER(ctypes.c_float)]),

@@ -76,6 +76,7 @@ struct bkh_matrix {
     int nwblk = 0;                     // row blocks of the wave variant
     int *wblk = nullptr;               // wblk[nwblk+1]
     int nsl = 0;                       // SELL-64 slices (0: no SELL copy)
+    long long sell_entries = 0;        // stored SELL entries (nonzeros + padding)
     int *sso = nullptr;                // slice offsets in entries, nsl + 1
     double *sval = nullptr;            // SELL values (padding 0)
     float *sval32 = nullptr;           // FP32 copy

@@ -198,6 +198,15 @@ static int cg_fp64(ThreadCtx *t, const bkh_matrix *A, const double *b, double *x
     return rc;
 }
 
+// Control block of a running fused solve at its first iteration (base = first = 0,
+// r.r slots 1, not fresh), for timing kernel A alone.
+__global__ void k_cgf_bench_ctl(Ctl *c) {
+    if (threadIdx.x == 0) {
+        c->base = 0; c->first = 0; c->fnb = 1.0;
+        c->s[CGF_RR] = 1.0; c->s[CGF_RR + 1] = 1.0; c->s[CGF_FRESH] = 0.0;
+    }
+}
+
 // ------------------------------------------------ FP32 with reliable updates
 
 #define RU_DELTA 0.1
@@ -376,4 +385,66 @@ extern "C" int bkh_cg(bkh_matrix *A, const double *b, double *x, double tol, int
         *info = inf;
     }
     return ret;
+}
+
+// Average ms of the fused CG kernel A (SpMV with the direction update in its gather,
+// the dominant kernel of a CG iteration) over reps launches, HIP events on the
+// launching stream. Vectors are zero; the control block holds a running solve.
+extern "C" int bkh_bench_cg_kernel(bkh_matrix *A, int fp32, int reps, double *avg_ms) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (!A || reps < 1 || !cgf_sell(A) || (fp32 && !A->sval32)) {
+        bkh_set_error("the fused CG kernel needs a matrix on the SELL layout");
+        return BKH_BAD_ARGUMENT;
+    }
+    const long long n = A->n;
+    const size_t vb = fp32 ? sizeof(float) : sizeof(double);
+    void *ws = bkh_workspace(carve_size(sizeof(Ctl)) + 2 * carve_size(sizeof(double) * CGF_NPART) +
+                             4 * carve_size(vb * n));
+    if (!ws) return BKH_DEVICE_ERROR;
+    Carve cv(ws);
+    Ctl *c = cv.take<Ctl>(1);
+    double *partA = cv.take<double>(CGF_NPART), *partB = cv.take<double>(CGF_NPART);
+    char *r = cv.take<char>(vb * n), *P = cv.take<char>(2 * vb * n), *q = cv.take<char>(vb * n);
+    hipStream_t s = t->stream;
+    HIP_OK(hipMemsetAsync(r, 0, vb * n, s));
+    HIP_OK(hipMemsetAsync(P, 0, 2 * vb * n, s));
+    if (ctl_init(t, c, 0.0, 1 << 30)) return BKH_DEVICE_ERROR;
+    hipLaunchKernelGGL(k_cgf_bench_ctl, dim3(1), dim3(64), 0, s, c);
+    const CgfGrid g = cgf_grid(A);
+    auto launch = [&]() {
+        if (fp32) {
+            if (A->scd && A->variant == SPMV_SELL)
+                hipLaunchKernelGGL((k_cgf_a_sell<float, float, short>), dim3(g.GA), dim3(256), 0, s, (int)n, A->nsl,
+                                   g.chunk, A->rp, A->sso, (const float *)A->sval32, (const short *)A->scd,
+                                   (const float *)r, (float *)P, (float *)P + n, (float *)q, c, 0,
+                                   (const double *)partB, g.GB, partA);
+            else
+                hipLaunchKernelGGL((k_cgf_a_sell<float, float, int>), dim3(g.GA), dim3(256), 0, s, (int)n, A->nsl,
+                                   g.chunk, A->rp, A->sso, (const float *)A->sval32, (const int *)A->scol,
+                                   (const float *)r, (float *)P, (float *)P + n, (float *)q, c, 0,
+                                   (const double *)partB, g.GB, partA);
+        } else {
+            if (A->scd && A->variant == SPMV_SELL)
+                hipLaunchKernelGGL((k_cgf_a_sell<double, double, short>), dim3(g.GA), dim3(256), 0, s, (int)n,
+                                   A->nsl, g.chunk, A->rp, A->sso, (const double *)A->sval, (const short *)A->scd,
+                                   (const double *)r, (double *)P, (double *)P + n, (double *)q, c, 0,
+                                   (const double *)partB, g.GB, partA);
+            else
+                hipLaunchKernelGGL((k_cgf_a_sell<double, double, int>), dim3(g.GA), dim3(256), 0, s, (int)n,
+                                   A->nsl, g.chunk, A->rp, A->sso, (const double *)A->sval, (const int *)A->scol,
+                                   (const double *)r, (double *)P, (double *)P + n, (double *)q, c, 0,
+                                   (const double *)partB, g.GB, partA);
+        }
+    };
+    launch();
+    HIP_OK(hipEventRecord(t->ev0, s));
+    for (int k = 0; k < reps; k++) launch();
+    HIP_OK(hipEventRecord(t->ev1, s));
+    LAUNCH_OK();
+    HIP_OK(hipEventSynchronize(t->ev1));
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, t->ev0, t->ev1));
+    *avg_ms = ms / reps;
+    return 0;
 }

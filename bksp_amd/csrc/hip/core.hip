@@ -399,6 +399,7 @@ static bool upload_sell(bkh_matrix *A, int n, const std::vector<int> &rp, const 
     ok = ok && hipStreamSynchronize(s) == hipSuccess;
     if (!ok) return false;
     A->nsl = nsl;
+    A->sell_entries = tot;
     A->variant = SPMV_SELL;
     return true;
 }
@@ -695,6 +696,18 @@ int ir_update(ThreadCtx *t, long long n, double *x, const float *d32, const Ctl 
 
 // Times `reps` back-to-back SpMV launches on this thread's stream with HIP events
 // (the stream the kernels run on), returning the average milliseconds per launch.
+// Bytes one SpMV with the matrix's current kernel must move: stored entries x
+// (value + index bytes), row pointers, x read once and y written once.
+extern "C" long long bkh_matrix_spmv_bytes(const bkh_matrix *A, int fp32) {
+    if (!A) return -1;
+    const long long vb = fp32 ? 4 : 8, n = A->n;
+    if ((A->variant == SPMV_SELL || A->variant == SPMV_SELL32) && A->nsl > 0) {
+        const long long ib = (A->variant == SPMV_SELL && A->scd) ? 2 : 4;
+        return A->sell_entries * (vb + ib) + 4 * (n + 1) + 2 * vb * n;
+    }
+    return A->nnz * (vb + 4) + 4 * (n + 1) + 2 * vb * n;
+}
+
 extern "C" int bkh_bench_spmv(const bkh_matrix *A, int fp32, const void *x, void *y, int reps,
                               double *avg_ms) {
     ThreadCtx *t = bkh_tctx();

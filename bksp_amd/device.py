@@ -109,6 +109,16 @@ class Matrix:
         if rc:
             raise BkspDeviceError(f"bkh_spmv -> {rc}: {last_error()}")
 
+    def spmv_bytes(self, fp32=False):
+        """Bytes one SpMV with the current kernel moves (format bytes)."""
+        return load().bkh_matrix_spmv_bytes(self.handle, int(fp32))
+
+    def bench_cg_kernel(self, reps=20, fp32=False):
+        """Average ms of CG's fused SpMV + direction-update kernel."""
+        ms = ctypes.c_double()
+        _check(load().bkh_bench_cg_kernel(self.handle, int(fp32), reps, ctypes.byref(ms)), "bkh_bench_cg_kernel")
+        return ms.value
+
     def spmv32(self, d_x, d_y):
         rc = load().bkh_spmv32(self.handle, d_x.ptr, d_y.ptr)
         if rc:

@@ -220,6 +220,12 @@ int bkh_pcg(bkh_matrix *A, int kind, const double *d_b, double *d_x, double tol,
 
 /* Timing of the most recent solve on this thread (ms, HIP events). */
 double bkh_last_solve_ms(void);
+/* Bytes one SpMV with the matrix's current kernel moves (stored entries x value and
+ * index bytes, row pointers, x read once, y written once). */
+long long bkh_matrix_spmv_bytes(const bkh_matrix *A, int fp32);
+/* Average ms per launch of CG's fused kernel (SpMV with the direction update in its
+ * gather) on a SELL-layout matrix, HIP events on its stream. */
+int bkh_bench_cg_kernel(bkh_matrix *A, int fp32, int reps, double *avg_ms);
 /* Average ms per launch of `reps` back-to-back SpMVs (FP64, or FP32 if fp32 = 1),
  * HIP events on the stream the kernels run on. */
 int bkh_bench_spmv(const bkh_matrix *A, int fp32, const void *d_x, void *d_y, int reps,
