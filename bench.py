@@ -53,10 +53,8 @@ def bench_cfg1(dv, gen, po):
     b = sp.csr_matrix((v, ci, rp), shape=(n, n)) @ xt
     M = dv.Matrix(n, rp, ci, v)
     db, dx = dv.DeviceArray.from_numpy(b), dv.DeviceArray.from_numpy(np.zeros(n))
-    zeros = np.zeros(n)
-
     def run(**kw):
-        dx.upload(zeros)
+        dx.zero()
         return dv.bicgstabl(M, db, 2, dx, 1e-10, 1000, **kw)
 
     run()
@@ -80,10 +78,8 @@ def bench_cfg2_block(dv, gen, m=128, s=8):
     B = gen.normal(202, n * s).reshape(s, n)            # column-major m x s: column j = B[j]
     M = dv.Matrix(n, rp, ci, v, symmetric=True)
     dB, dX = dv.DeviceArray.from_numpy(B), dv.DeviceArray.from_numpy(np.zeros((s, n)))
-    zeros = np.zeros((s, n))
-
     def run(fix):
-        dX.upload(zeros)
+        dX.zero()
         return dv.blcg(M, s, dB, dX, 1e-8, 5000, fix_blcg_p0=fix)
 
     run(0)
@@ -103,10 +99,8 @@ def bench_cfg3(dv, gen, m=96):
     sig = 10.0 ** (-3 + 3 * np.arange(64) / 63.0)
     M = dv.Matrix(n, rp, ci, v)
     db, dx = dv.DeviceArray.from_numpy(b), dv.DeviceArray((n * 65,))
-    zeros = np.zeros(n * 65)
-
     def run(**kw):
-        dx.upload(zeros)
+        dx.zero()
         return dv.shbicgstabl(M, db, 4, sig, dx, 1e-8, 1000, **kw)
 
     run()
@@ -127,10 +121,8 @@ def bench_cfg4(dv, gen):
     tune_ms = Bt.tune(fp32=True)
     b = gen.normal(405, Bt.N)
     db, dx = dv.DeviceArray.from_numpy(b), dv.DeviceArray((Bt.N,))
-    zeros = np.zeros(Bt.N)
-
     def run(**kw):
-        dx.upload(zeros)
+        dx.zero()
         return Bt.bicgstabl(db, 2, dx, 1e-8, 1000, **kw)
 
     run(precision=dv.BKH_FP32_IR)
@@ -144,12 +136,11 @@ def bench_cfg4(dv, gen):
         bs.append(dv.DeviceArray.from_numpy(b[off[q]:off[q + 1]]))
         xs.append(dv.DeviceArray((sy[0],)))
     del systems
-    zq = [np.zeros(n) for n in sizes]
 
     def seq():
         worst = 0.0
         for q in range(len(mats)):
-            xs[q].upload(zq[q])
+            xs[q].zero()
             inf = dv.bicgstabl(mats[q], bs[q], 2, xs[q], 1e-8, 1000)
             worst = max(worst, inf.true_res)
         return worst
@@ -247,13 +238,8 @@ def main():
     tol = 1e-8
 
     def solve(**kw):
-        load_zero(dx)
+        dx.zero()                       # x0 = 0 on the device (no host transfer in the timed loop)
         return dv.cg(M, db, dx, tol, 20000, **kw)
-
-    zeros = np.zeros(n)
-
-    def load_zero(d):
-        d.upload(zeros)
 
     for _ in range(args.warmup):
         solve()

@@ -257,3 +257,147 @@ __global__ __launch_bounds__(64) void k_cgf_tail(Ctl *c, int K, const double *pa
     if (cgf_check(c, k, rr, writer) != ST_RUNNING) return;
     if (writer) c->base = k;
 }
+
+// ------------------------------------------- one kernel per iteration (FP32 inner)
+//
+// The FP32 inner iteration of the reliable-update solve runs as ONE kernel per
+// iteration. Kernel k reads r_{k-1}, q_{k-1}, p_{k-1} (buffers of parity (k-1)&1)
+// and the partial sums (p.q, r.q, q.q) of kernel k-1, and
+//   alpha_{k-1} = rho_{k-1} / (p_{k-1}.q_{k-1})
+//   rho_k       = rho_{k-1} - 2 alpha r_{k-1}.q_{k-1} + alpha^2 q_{k-1}.q_{k-1}
+//                 (= |r_{k-1} - alpha q_{k-1}|^2, FP64 sums of the stored FP32 values)
+//   beta_{k-1}  = rho_k / rho_{k-1}
+//   r_k = r_{k-1} - alpha q_{k-1},  p_k = r_k + beta p_{k-1}   (formed in the gather)
+//   q_k = A p_k,  x += alpha p_{k-1}  (the deferred update of iteration k-1)
+// and writes r_k, p_k, q_k (parity k&1) and its partials (p_k.q_k, r_k.q_k,
+// q_k.q_k). Same recurrence as the two-kernel form in exact arithmetic; one global
+// synchronisation per iteration instead of two. rho_k is the norm of the residual
+// that the next kernel materialises; the periodic FP64 residual replacement of the
+// reliable-update driver bounds any drift between it and the true residual.
+// At k == Ctl::first (a pass start) alpha = 0 (x, r already hold the replaced
+// values) and beta = rho_k / rho_{k-1} from the control block (0 when fresh).
+enum { CG1_ALPHA = 12 };              // Ctl::s slot: alpha_{k-1} at a stop
+
+// Fixed-order sum of np rows of 3 partials by a workgroup of BS threads,
+// broadcast to every thread.
+template <int BS>
+__device__ __forceinline__ void block_sum_parts3(const double *part, int np, double (&o)[3]) {
+    __shared__ double sh[3 * (BS / 64) + 3];
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < np; i += BS) {
+        a[0] += part[3 * i];
+        a[1] += part[3 * i + 1];
+        a[2] += part[3 * i + 2];
+    }
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int v = 0; v < 3; v++) a[v] = wave_sum(a[v]);
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int v = 0; v < 3; v++) sh[v * (BS / 64) + w] = a[v];
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double t = sh[threadIdx.x * (BS / 64)];
+#pragma unroll
+        for (int q = 1; q < BS / 64; q++) t += sh[threadIdx.x * (BS / 64) + q];
+        sh[3 * (BS / 64) + threadIdx.x] = t;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < 3; v++) o[v] = sh[3 * (BS / 64) + v];
+}
+
+// Scalars of kernel k (or of the tail check of iteration k). Returns false when the
+// solve stops at k (converged, max_it, breakdown); workgroup 0's thread 0 records.
+template <int BS>
+__device__ __forceinline__ bool cg1_scalars(Ctl *c, int k, const double *partPrev, int np, bool writer,
+                                            double &alpha, double &beta) {
+    if (k == c->first) {
+        alpha = 0.0;
+        const double rho = c->s[CGF_RR + (k & 1)];
+        beta = c->s[CGF_FRESH] != 0.0 ? 0.0 : rho / c->s[CGF_RR + ((k + 1) & 1)];
+        return true;
+    }
+    double d[3];
+    block_sum_parts3<BS>(partPrev, np, d);
+    const double rho_prev = c->s[CGF_RR + ((k + 1) & 1)];
+    if (bad(d[0])) {
+        if (writer) {
+            c->status = ST_BREAKDOWN; c->result_it = k - 1; c->it = k; c->done = 1;
+            c->s[CG1_ALPHA] = 0.0;
+        }
+        return false;
+    }
+    // the recurrence must use the alpha the FP32 update applies: with the unrounded
+    // FP64 alpha, rho drifts from |r|^2 by ~1e-4 relative within a pass and beta's
+    // error costs ~10 % more iterations (reproduced in a CPU model of the iteration)
+    alpha = (double)(float)(rho_prev / d[0]);
+    double rho = rho_prev - 2.0 * alpha * d[1] + alpha * alpha * d[2];
+    if (rho < 0.0) rho = 0.0;
+    if (cgf_check(c, k, rho, writer) != ST_RUNNING) {
+        if (writer) c->s[CG1_ALPHA] = alpha;
+        return false;
+    }
+    if (writer) c->s[CGF_RR + (k & 1)] = rho;
+    beta = rho / rho_prev;
+    return true;
+}
+
+// Kernel k on the SELL-64 layout (lane per row, workgroup g owns `chunk` slices).
+// R, Q, P hold two buffers each (parity), n entries apart; x32 is updated in place.
+template <typename VT, typename IT>
+__global__ __launch_bounds__(256) void k_cg1_sell(int n, int nsl, int chunk, const int *__restrict__ rp,
+                                                  const int *__restrict__ so, const VT *__restrict__ val,
+                                                  const IT *__restrict__ col, float *x, float *R, float *Q,
+                                                  float *P, Ctl *c, int j, const double *partPrev,
+                                                  int np, double *partOut) {
+    if (ctl_stopped(c)) return;
+    const int k = c->base + j;
+    const bool writer = blockIdx.x == 0 && threadIdx.x == 0;
+    double ad, bd;
+    if (!cg1_scalars<256>(c, k, partPrev, np, writer, ad, bd)) return;
+    const float al = (float)ad, be = (float)bd;
+    const size_t po = (size_t)((k + 1) & 1) * n, pn = (size_t)(k & 1) * n;
+    const float *rp_ = R + po, *qp_ = Q + po, *pp_ = P + po;
+    float *rn_ = R + pn, *qn_ = Q + pn, *pn_ = P + pn;
+    const int g = xcd_block(blockIdx.x, gridDim.x);
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int send = min(nsl, (g + 1) * chunk);
+    double dv[3] = {0.0, 0.0, 0.0};
+    for (int sl = g * chunk + w; sl < send; sl += 4) {
+        const int row = sl * 64 + l;
+        if (row >= n) continue;
+        const int len = rp[row + 1] - rp[row];
+        const int base = so[sl] + l;
+        float acc = 0.0f;
+#pragma unroll 4
+        for (int kk = 0; kk < len; kk++) {
+            const int e = base + 64 * kk;
+            const int cc = sizeof(IT) == 2 ? row + (int)col[e] : (int)col[e];
+            const float rc = fmaf(-al, qp_[cc], rp_[cc]);
+            const float pc = fmaf(be, pp_[cc], rc);
+            acc = add_rn(acc, mul_rn((float)val[e], pc));
+        }
+        const float pold = pp_[row];
+        const float rr = fmaf(-al, qp_[row], rp_[row]);
+        const float pnew = fmaf(be, pold, rr);
+        x[row] = fmaf(al, pold, x[row]);
+        rn_[row] = rr;
+        pn_[row] = pnew;
+        qn_[row] = acc;
+        dv[0] += (double)pnew * (double)acc;
+        dv[1] += (double)rr * (double)acc;
+        dv[2] += (double)acc * (double)acc;
+    }
+    block_partial<3, 256>(dv, partOut);
+}
+
+// End of a graph replay of K single-kernel iterations: test of iteration base+K.
+__global__ __launch_bounds__(256) void k_cg1_tail(Ctl *c, int K, const double *partPrev, int np) {
+    if (ctl_stopped(c)) return;
+    const int k = c->base + K;
+    double ad, bd;
+    if (!cg1_scalars<256>(c, k, partPrev, np, threadIdx.x == 0, ad, bd)) return;
+    if (threadIdx.x == 0) c->base = k;
+}

@@ -14,10 +14,13 @@
 // lattice QCD systems of equations using mixed precision solvers on GPUs", 2010).
 // The Krylov iteration runs in FP32 (values, 16-bit column deltas where they fit,
 // vectors; FP64 scalars and dot accumulation) on a correction x32; each time its
-// recursive residual has fallen by a factor RU_DELTA the host replaces it by the
-// FP64 true residual: x += x32 and r = b - A x in FP64, r32 = r, x32 = 0, and the
-// FP32 iteration continues with its search direction, so no Krylov information is
-// discarded (a restarted refinement would lose it at every pass).
+// recursive residual has fallen by a factor delta (RU_DELTA unless bkh_opts.inner_tol
+// sets it) the host replaces it by the FP64 true residual: x += x32 and r = b - A x in
+// FP64, r32 = r, x32 = 0, and the FP32 iteration continues with its search
+// direction, so no Krylov information is discarded (a restarted refinement would
+// lose it at every pass). On SELL matrices the FP32 iteration is one kernel per
+// iteration (k_cg1_sell: residual norm from the alpha recurrence, x update deferred
+// into the next kernel), elsewhere the two-kernel form of the FP64 path.
 #include <math.h>
 #include "bkh_cgfused.hpp"
 #include "bkh_host.hpp"
@@ -209,7 +212,7 @@ __global__ void k_cgf_bench_ctl(Ctl *c) {
 
 // ------------------------------------------------ FP32 with reliable updates
 
-#define RU_DELTA 0.1
+#define RU_DELTA 0.01
 
 // r32 = (float) r64, x32 = 0, and (first time) the p buffer p0 = 0 or = r32.
 __global__ __launch_bounds__(256) void k_ru_load(long long n, const double *r64, float *r32, float *x32,
@@ -258,8 +261,143 @@ __global__ void k_ctl_resume_gate(Ctl *c) {
     if (threadIdx.x == 0) c->done = 0;
 }
 
+// ---- single-kernel FP32 inner iteration (SELL matrices)
+
+// x += x32 + alpha_{k-1} p_{k-1} (the update the stopped kernel k had deferred), x32 = 0.
+__global__ __launch_bounds__(256) void k_cg1_fold(long long n, double *x, float *x32, const float *P,
+                                                  const Ctl *c) {
+    const int k = c->it;
+    const double al = (c->status == ST_RUNNING) ? 0.0 : c->s[CG1_ALPHA];
+    const float *p = P + (size_t)((k + 1) & 1) * n;
+    ROWLOOP(i, n) {
+        x[i] += (double)x32[i] + al * (double)p[i];
+        x32[i] = 0.0f;
+    }
+}
+
+// Pass start: the replaced residual r32 = r64 goes where kernel k reads r_{k-1}
+// (alpha = 0 there); the first pass also clears x32 and the q, p buffers it reads.
+__global__ __launch_bounds__(256) void k_cg1_load(long long n, const double *r64, float *x32, float *R,
+                                                  float *Q, float *P, int first_pass, const Ctl *c,
+                                                  double *part) {
+    const size_t po = first_pass ? (size_t)n : (size_t)((c->it + 1) & 1) * n;
+    double v[1] = {0.0};
+    ROWLOOP(i, n) {
+        const float ri = (float)r64[i];
+        R[po + i] = ri;
+        if (first_pass) { x32[i] = 0.0f; Q[po + i] = 0.0f; P[po + i] = 0.0f; }
+        v[0] += (double)ri * (double)ri;
+    }
+    block_partial<1, 256>(v, part);
+}
+
+// Inner control at a pass start: rho_k of the replaced residual in slot k&1, the
+// previous rho stays in the other slot (beta = rho_k / rho_{k-1}, 0 when fresh).
+struct Cg1ResumeFin {
+    double target;
+    int first;
+    const Ctl *co;
+    __device__ void operator()(Ctl *c, const double *tot) const {
+        if (first) { c->it = 0; c->s[CGF_RR + 1] = tot[0]; }
+        const int k = c->it;
+        c->s[CGF_RR + (k & 1)] = tot[0];
+        c->s[CGF_FRESH] = first ? 1.0 : 0.0;
+        c->s[CG1_ALPHA] = 0.0;            // nothing deferred until the pass's first kernel
+        c->base = k;
+        c->first = k;
+        c->fnb = co->s[2];
+        c->tol = target;
+        c->done = 0;
+        c->status = ST_RUNNING;
+        if (!(c->fnb > 0.0) || !(tot[0] == tot[0])) ctl_breakdown(c);
+    }
+};
+
+static hipGraphExec_t cg1_capture(ThreadCtx *t, hipStream_t s, const bkh_matrix *A, float *x32, float *R,
+                                  float *Q, float *P, Ctl *c, double *part2, int K) {
+    const CgfGrid g = cgf_grid(A);
+    const long long n = A->n;
+    const int np = g.GA;
+    GraphKey key = {0xC6F3, (uintptr_t)A, (uintptr_t)x32, (uintptr_t)R, (uintptr_t)Q, (uintptr_t)P, (uintptr_t)c,
+                    (uintptr_t)part2, (uintptr_t)K};
+    return capture_cached(t, key, s, 1, [&]() {
+        for (int j = 0; j < K; j++) {
+            // kernel base+j reads the partials of base+j-1: parity of j (K even)
+            const double *pin = part2 + (size_t)((j + 1) & 1) * 3 * CGF_NPART;
+            double *pout = part2 + (size_t)(j & 1) * 3 * CGF_NPART;
+            if (A->scd)
+                hipLaunchKernelGGL((k_cg1_sell<float, short>), dim3(g.GA), dim3(256), 0, s, (int)n, A->nsl, g.chunk,
+                                   A->rp, A->sso, (const float *)A->sval32, (const short *)A->scd, x32, R, Q, P, c,
+                                   j, pin, np, pout);
+            else
+                hipLaunchKernelGGL((k_cg1_sell<float, int>), dim3(g.GA), dim3(256), 0, s, (int)n, A->nsl, g.chunk,
+                                   A->rp, A->sso, (const float *)A->sval32, (const int *)A->scol, x32, R, Q, P, c,
+                                   j, pin, np, pout);
+        }
+        hipLaunchKernelGGL(k_cg1_tail, dim3(1), dim3(256), 0, s, c, K,
+                           (const double *)(part2 + (size_t)((K + 1) & 1) * 3 * CGF_NPART), np);
+    });
+}
+
+static int cg_reliable1(ThreadCtx *t, const bkh_matrix *A, const double *b, double *x, double tol,
+                        int max_iter, int K, double delta, bkh_info *inf) {
+    hipStream_t s = t->stream;
+    const long long n = A->n;
+    const int G = vec_grid(n, 256);
+    if (K & 1) K++;                       // partial buffers alternate by the parity of j
+    const int nparts = std::max(spmv_grid(A), G) * 2;
+    size_t bytes = 2 * carve_size(sizeof(Ctl)) + carve_size(sizeof(double) * nparts) +
+                   carve_size(sizeof(double) * 6 * CGF_NPART) + carve_size(sizeof(double) * n) +
+                   7 * carve_size(sizeof(float) * n);
+    void *ws = bkh_workspace(bytes);
+    if (!ws) return BKH_DEVICE_ERROR;
+    Carve cv(ws);
+    Ctl *co = cv.take<Ctl>(1), *ci = cv.take<Ctl>(1);
+    double *part = cv.take<double>(nparts);
+    double *part2 = cv.take<double>(6 * CGF_NPART);
+    double *r64 = cv.take<double>(n);
+    float *x32 = cv.take<float>(n), *R = cv.take<float>(2 * n), *Q = cv.take<float>(2 * n), *P = cv.take<float>(2 * n);
+    if (ctl_init(t, co, 0.0, 1) || ctl_init(t, ci, 1.0, max_iter)) return BKH_DEVICE_ERROR;
+    hipGraphExec_t ge = cg1_capture(t, s, A, x32, R, Q, P, ci, part2, K);
+    if (!ge) { bkh_set_error("hipGraph capture of the FP32 CG iteration failed"); return BKH_DEVICE_ERROR; }
+    int rc = 0, updates = 0;
+    double rel = 0.0;
+    for (;;) {
+        if (updates > 0)
+            hipLaunchKernelGGL(k_cg1_fold, dim3(G), dim3(256), 0, s, n, x, x32, (const float *)P, (const Ctl *)ci);
+        if ((rc = resid_fp64(t, A, 0.0, b, x, r64, co, part))) break;
+        if ((rc = ctl_fetch(t, co))) break;
+        rel = t->ctl_host->s[0];
+        if (rel < tol || !(rel == rel)) break;
+        if ((rc = ctl_fetch(t, ci))) break;
+        const int used = t->ctl_host->it;
+        if (updates > 0 && (t->ctl_host->status == ST_BREAKDOWN || used >= max_iter)) break;
+        const double target = std::max(0.5 * tol, delta * rel);
+        hipLaunchKernelGGL(k_cg1_load, dim3(G), dim3(256), 0, s, n, (const double *)r64, x32, R, Q, P,
+                           (int)(updates == 0), (const Ctl *)ci, part);
+        hipLaunchKernelGGL(k_ctl_resume_gate, dim3(1), dim3(64), 0, s, ci);
+        launch_finalize<1>(s, part, G, ci, Cg1ResumeFin{target, updates == 0, co});
+        LAUNCH_OK();
+        if ((rc = run_graph_until_done(t, ge, ci, (max_iter - used) / K + 2))) break;
+        updates++;
+    }
+    if (rc) return rc;
+    if (ctl_fetch(t, ci)) return BKH_DEVICE_ERROR;
+    const int iters = t->ctl_host->it;
+    inf->inner_iters = iters;
+    inf->passes = updates;
+    inf->rec_res = rel;
+    if (rel < tol) {
+        inf->status = ST_CONVERGED;
+        inf->iters = iters > 0 ? iters - 1 : 0;
+        return inf->iters;
+    }
+    inf->status = (rel == rel && t->ctl_host->status != ST_BREAKDOWN) ? ST_MAXIT : ST_BREAKDOWN;
+    return inf->status == ST_MAXIT ? -1 : -2;
+}
+
 static int cg_reliable(ThreadCtx *t, const bkh_matrix *A, const double *b, double *x, double tol,
-                       int max_iter, int K, bkh_info *inf) {
+                       int max_iter, int K, double delta, bkh_info *inf) {
     hipStream_t s = t->stream;
     const long long n = A->n;
     const int G = vec_grid(n, 256);
@@ -303,7 +441,7 @@ static int cg_reliable(ThreadCtx *t, const bkh_matrix *A, const double *b, doubl
         if ((rc = ctl_fetch(t, ci))) break;
         const int used = t->ctl_host->it;
         if (updates > 0 && (t->ctl_host->status == ST_BREAKDOWN || used >= max_iter)) break;
-        const double target = std::max(0.5 * tol, RU_DELTA * rel);
+        const double target = std::max(0.5 * tol, delta * rel);
         // first pass: the p buffer read by the first iteration is p0 = 0 (fused,
         // beta = 0 gives p = r) or p = r (unfused)
         hipLaunchKernelGGL(k_ru_load, dim3(G), dim3(256), 0, s, n, (const double *)r64, r32, x32,
@@ -352,7 +490,9 @@ extern "C" int bkh_cg(bkh_matrix *A, const double *b, double *x, double tol, int
     HIP_OK(hipEventRecord(t->ev0, t->stream));
     if (o.precision == BKH_FP32_IR) {
         if (!A->val32) { bkh_set_error("FP32_IR needs a matrix created with want_fp32"); return BKH_BAD_ARGUMENT; }
-        ret = cg_reliable(t, A, b, x, tol, max_iter, K, &inf);
+        const double delta = o.inner_tol > 0 ? o.inner_tol : RU_DELTA;
+        ret = (cgf_sell(A) && A->sval32 && o.path != 1) ? cg_reliable1(t, A, b, x, tol, max_iter, K, delta, &inf)
+                                                           : cg_reliable(t, A, b, x, tol, max_iter, K, delta, &inf);
         if (ret < -2) return ret;
     } else {
         const int nparts = std::max(spmv_grid(A), vec_grid(n, 256)) * 2;

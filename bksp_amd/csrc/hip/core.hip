@@ -93,7 +93,7 @@ void *bkh_workspace(size_t bytes) {
         t->ws = nullptr;
         t->ws_bytes = 0;
     }
-    size_t want = bytes + bytes / 8;
+    size_t want = bytes + bytes / 4;     // headroom: a regrowth invalidates cached graphs
     if (hipMalloc(&t->ws, want) != hipSuccess) {
         bkh_set_error("workspace allocation of %zu bytes failed", want);
         t->ws = nullptr;

@@ -51,6 +51,11 @@ class DeviceArray:
         if load().bkh_h2d(self.ptr, _ptr(a), self.nbytes):
             raise BkspDeviceError("bkh_h2d failed: " + last_error())
 
+    def zero(self):
+        """Sets every byte to 0 on the device (a fill kernel; no host transfer)."""
+        if load().bkh_memset(self.ptr, 0, self.nbytes):
+            raise BkspDeviceError("bkh_memset failed: " + last_error())
+
     def numpy(self):
         out = np.empty(self.shape, self.dtype)
         if load().bkh_d2h(_ptr(out), self.ptr, self.nbytes):

@@ -39,9 +39,12 @@ typedef struct {
     int    fix_blcg_p0;  /* block CG: 1 = first direction P0 = V (the reference's
                             uninitialised beta gives P0 = 2V, the default here) */
     int    max_inner;    /* FP32-IR: inner iteration cap per refinement pass (0 = default) */
-    double inner_tol;    /* FP32-IR: inner relative tolerance (0 = default) */
+    double inner_tol;    /* FP32-IR: inner relative tolerance (0 = default); for CG the
+                            residual reduction between reliable updates (default 0.01) */
     int    path;         /* BiCGSTAB(l): 0 auto, 1 multi-kernel graph, 2 one persistent
-                            workgroup (systems up to 16384 rows / 400000 nonzeros) */
+                            workgroup (systems up to 16384 rows / 400000 nonzeros).
+                            CG FP32-IR: 0 auto (one kernel per iteration on SELL
+                            matrices), 1 two kernels per iteration */
 } bkh_opts;
 
 typedef struct {
