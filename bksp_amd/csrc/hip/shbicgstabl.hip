@@ -2,17 +2,26 @@
 // with src/util_shift.c:5-27 dhorner and :55-63 comb).
 //
 // The seed system (sigma = 0) runs BiCGSTAB(l) as in bicgstabl.hip. The per-shift
-// work of each BiCG step runs in one kernel over all shifts (k_sh_shift_step): a
-// 2-D grid of (rows x groups of SH_SPT shifts) in which every thread reads the
-// seed vectors of its row and issues the loads of its shifts' p_shift vectors
-// together (shbicgstabl.c:111-134, :146-153); the seed's own r/x update
-// (:107-108, :137-139) follows in k_sh_seed_r, so every shift thread reads the
-// pre-update r. The MR step's per-shift updates (:202-237, k_sh_shift_upd) and
-// the seed update (:194-199, :240-242, k_sh_seed_upd) are ordered the same way.
-// The pi/theta/psi recurrences and the Horner re-expansion are FP64 scalar work
-// in the finalize kernels, one shift per thread. x holds the seed solution first,
-// then shift k at x + (k+1) n (shbicgstabl.c:62). PT is the storage type of
-// p_shift (double, or float for the reduced-traffic variant).
+// vectors are stored SHIFT-INTERLEAVED: p_shift[q] and x_k as n x SP row-major blocks
+// (SP = the shift count rounded up to 4; padded shifts carry zero coefficients and
+// stay zero), so a thread owns one row and V = 16/sizeof(PT) consecutive shifts and
+// every access is one contiguous 16-byte vector per thread -- a wave streams
+// 64 x 16 contiguous bytes per vector instead of touching 8 shifts x (j+2) separate
+// n-strided arrays. The per-shift work of BiCG step j is one kernel over all shifts
+// (k_sh_step, shbicgstabl.c:111-134, :146-153); the seed's own r/x update (:107-108,
+// :137-139) follows in k_sh_seed_r, so every shift thread reads the pre-update r.
+// The MR step's per-shift updates (:202-237, k_sh_upd) and the seed update
+// (:194-199, :240-242, k_sh_seed_upd) are ordered the same way. The pi/theta/psi
+// recurrences and the Horner re-expansion are FP64 scalar work in the finalize
+// kernels, one shift per thread. Each element follows the reference's operation
+// order, so the FP64 path matches the oracle's iterates to rounding.
+//
+// BKH_FP32_IR: p_shift and x_k are FP32 (the seed, its scalars and the pi/theta/psi
+// recurrences stay FP64). FP32 storage limits the shifted residuals to ~kappa*eps32,
+// so every shift is then refined in FP64: r_k = b - (A + sigma_k I) x_k (FP64, all
+// shifts in one kernel), and (A + sigma_k I) d_k = r_k is solved for all shifts at
+// once by a multi-shift BiCGSTAB in FP32 (per-shift scalars, shared SpMV sweeps over
+// the interleaved block), x_k += d_k in FP64, until every shift meets tol.
 #include <math.h>
 #include "bkh_host.hpp"
 #include "bkh_mr.hpp"
@@ -228,55 +237,84 @@ __global__ __launch_bounds__(256) void k_sh_seed_r(long long n, int j, const dou
     }
 }
 
-#define SH_SPT 8   // shifts per thread
+// ------------------------------------------------ shift-interleaved per-shift work
+//
+// Launch geometry of the interleaved kernels: a thread owns (row i, shift group gq)
+// with gq = tid % G (G = SP / V groups of V shifts); the grid-stride loop advances
+// by whole rows because the thread count is a multiple of G, so each thread loads
+// its shifts' coefficients once.
+struct ShGrid {
+    int SP, V, G, blocks;
+};
 
-// Shift part of BiCG step J for shifts [blockIdx.y*SH_SPT, +SH_SPT): p_shift[k][0..J+1]
-// and x_k (:124-133, :146-153). r_J after the seed update is recomputed here with
-// the seed kernel's exact operation, so every thread reads the pre-update r.
+static int gcd_i(int a, int b) { while (b) { int t = a % b; a = b; b = t; } return a; }
+
+static ShGrid sh_grid(long long n, int SP, int V, int cap = 8192) {
+    ShGrid g;
+    g.SP = SP;
+    g.V = V;
+    g.G = SP / V;
+    const int m = g.G / gcd_i(g.G, 256);                      // blocks * 256 % G == 0
+    long long want = (n * g.G + 255) / 256;
+    if (want > cap) want = cap;
+    want = (want + m - 1) / m * m;
+    g.blocks = (int)std::max<long long>(m, want);
+    return g;
+}
+
+// Shift part of BiCG step J (:124-133, :146-153) for the thread's V shifts:
+// p_shift[k][0..J] = -beta_k p_shift[k][q] + r_q / (pi_k theta_k), p_shift[k][J+1]
+// from r_J before and after the seed update, x_k += alpha_k p_shift[k][0]. r_J after
+// the seed update is recomputed here with the seed kernel's exact operation, so
+// every thread reads the pre-update r. PS: (l+1) interleaved n x SP blocks; XS: n x SP.
 template <int J, typename PT>
-__global__ __launch_bounds__(256) void k_sh_shift_step(long long n, int l, const double *P, const double *R,
-                                                       double *x, PT *PS, const ShState *st, const Ctl *c) {
+__global__ __launch_bounds__(256) void k_sh_step(long long n, int G, const double *P, const double *R,
+                                                 PT *XS, PT *PS, const ShState *st, const Ctl *c) {
     if (ctl_stopped(c)) return;
+    constexpr int V = 16 / sizeof(PT);
+    typedef PT pvec __attribute__((ext_vector_type(V)));
     const double al = c->s[SB_ALPHA];
-    const int ns = st->ns, l1 = l + 1, k0 = blockIdx.y * SH_SPT;
-    ROWLOOP(i, n) {
+    const long long tid = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long rstride = (long long)gridDim.x * 256 / G;
+    const int k0 = (int)(tid % G) * V;
+    const size_t SP = (size_t)G * V;
+    double cA[V], cB[V], cC[V], cD[V], cE[V], ak[V], sg[V];
+#pragma unroll
+    for (int u = 0; u < V; u++) {
+        const int k = k0 + u;
+        cA[u] = st->cA[k]; cB[u] = st->cB[k]; cC[u] = st->cC[k]; cD[u] = st->cD[k];
+        cE[u] = st->cE[k]; ak[u] = st->ak[k]; sg[u] = st->sigma[k];
+    }
+    for (long long i = tid / G; i < n; i += rstride) {
         double rold[J + 1];
 #pragma unroll
         for (int q = 0; q <= J; q++) rold[q] = R[q * n + i];
         const double rjn = fma(-al, P[(J + 1) * n + i], rold[J]);
-        double pv[SH_SPT][J + 1], xv[SH_SPT];
+        pvec pv[J + 1];
 #pragma unroll
-        for (int u = 0; u < SH_SPT; u++) {
-            const int k = k0 + u;
-            if (k < ns) {
-                const PT *ps = PS + (size_t)k * l1 * n;
+        for (int q = 0; q <= J; q++) pv[q] = *(const pvec *)(PS + ((size_t)q * n + i) * SP + k0);
+        const pvec xv = *(const pvec *)(XS + (size_t)i * SP + k0);
+        pvec out[J + 2], xo;
 #pragma unroll
-                for (int q = 0; q <= J; q++) pv[u][q] = (double)ps[q * n + i];
-                xv[u] = x[(size_t)(k + 1) * n + i];
+        for (int u = 0; u < V; u++) {
+            double p0 = 0.0, pj = 0.0;
+#pragma unroll
+            for (int q = 0; q <= J; q++) {
+                const PT v = (PT)fma(cB[u], rold[q], mul_rn(cA[u], (double)pv[q][u]));
+                out[q][u] = v;
+                if (q == 0) p0 = (double)v;
+                if (q == J) pj = (double)v;
             }
+            double t = mul_rn(rold[J], cC[u]);
+            t = fma(cD[u], rjn, t);
+            t = mul_rn(t, cE[u]);
+            t = fma(-sg[u], pj, t);
+            out[J + 1][u] = (PT)t;
+            xo[u] = (PT)fma(ak[u], p0, (double)xv[u]);
         }
 #pragma unroll
-        for (int u = 0; u < SH_SPT; u++) {
-            const int k = k0 + u;
-            if (k < ns) {
-                PT *ps = PS + (size_t)k * l1 * n;
-                const double cA = st->cA[k], cB = st->cB[k];
-                double p0 = 0.0, pj = 0.0;
-#pragma unroll
-                for (int q = 0; q <= J; q++) {
-                    const PT v = (PT)fma(cB, rold[q], mul_rn(cA, pv[u][q]));
-                    ps[q * n + i] = v;
-                    if (q == 0) p0 = (double)v;
-                    if (q == J) pj = (double)v;
-                }
-                double t = mul_rn(rold[J], st->cC[k]);
-                t = fma(st->cD[k], rjn, t);
-                t = mul_rn(t, st->cE[k]);
-                t = fma(-st->sigma[k], pj, t);
-                ps[(J + 1) * n + i] = (PT)t;
-                x[(size_t)(k + 1) * n + i] = fma(st->ak[k], p0, xv[u]);
-            }
-        }
+        for (int q = 0; q <= J + 1; q++) *(pvec *)(PS + ((size_t)q * n + i) * SP + k0) = out[q];
+        *(pvec *)(XS + (size_t)i * SP + k0) = xo;
     }
 }
 
@@ -304,47 +342,458 @@ __global__ __launch_bounds__(256) void k_sh_seed_upd(long long n, int l, double 
     block_partial<2, 256>(v, part);
 }
 
-// Shift part of the MR step (:202-237): x_k += sum_q xc[k][q] r_q (original r),
-// p_shift[k][0] = (p_shift[k][0] - sum_i omega_i p_shift[k][i]) / psi_k.
+// Shift part of the MR step (:202-237) for the thread's V shifts: x_k += sum_q
+// xc[k][q] r_q (original r), p_shift[k][0] = (p_shift[k][0] - sum_i omega_i
+// p_shift[k][i]) / psi_k.
 template <int L, typename PT>
-__global__ __launch_bounds__(256) void k_sh_shift_upd(long long n, const double *R, double *x, PT *PS,
-                                                      const ShState *st, const Ctl *c) {
+__global__ __launch_bounds__(256) void k_sh_upd(long long n, int G, const double *R, PT *XS, PT *PS,
+                                                const ShState *st, const Ctl *c) {
     if (ctl_stopped(c)) return;
-    const int ns = st->ns, k0 = blockIdx.y * SH_SPT;
-    ROWLOOP(i, n) {
+    constexpr int V = 16 / sizeof(PT);
+    typedef PT pvec __attribute__((ext_vector_type(V)));
+    const long long tid = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long rstride = (long long)gridDim.x * 256 / G;
+    const int k0 = (int)(tid % G) * V;
+    const size_t SP = (size_t)G * V;
+    double xc[V][L], pinv[V], om[L + 1];
+#pragma unroll
+    for (int u = 0; u < V; u++) {
+#pragma unroll
+        for (int q = 0; q < L; q++) xc[u][q] = st->xc[k0 + u][q];
+        pinv[u] = st->psi_inv[k0 + u];
+    }
+#pragma unroll
+    for (int q = 1; q <= L; q++) om[q] = st->om[q];
+    for (long long i = tid / G; i < n; i += rstride) {
         double r[L];
 #pragma unroll
         for (int q = 0; q < L; q++) r[q] = R[q * n + i];
-        double pv[SH_SPT][L + 1], xv[SH_SPT];
+        pvec pv[L + 1];
 #pragma unroll
-        for (int u = 0; u < SH_SPT; u++) {
-            const int k = k0 + u;
-            if (k < ns) {
-                const PT *ps = PS + (size_t)k * (L + 1) * n;
+        for (int q = 0; q <= L; q++) pv[q] = *(const pvec *)(PS + ((size_t)q * n + i) * SP + k0);
+        const pvec xv = *(const pvec *)(XS + (size_t)i * SP + k0);
+        pvec xo, po;
 #pragma unroll
-                for (int q = 0; q <= L; q++) pv[u][q] = (double)ps[q * n + i];
-                xv[u] = x[(size_t)(k + 1) * n + i];
-            }
+        for (int u = 0; u < V; u++) {
+            double xk = (double)xv[u];
+#pragma unroll
+            for (int q = 0; q < L; q++) xk += xc[u][q] * r[q];
+            xo[u] = (PT)xk;
+            double p = (double)pv[0][u];
+#pragma unroll
+            for (int q = 1; q <= L; q++) p -= om[q] * (double)pv[q][u];
+            po[u] = (PT)(p * pinv[u]);
         }
-#pragma unroll
-        for (int u = 0; u < SH_SPT; u++) {
-            const int k = k0 + u;
-            if (k < ns) {
-                double xk = xv[u];
-#pragma unroll
-                for (int q = 0; q < L; q++) xk += st->xc[k][q] * r[q];
-                x[(size_t)(k + 1) * n + i] = xk;
-                double p = pv[u][0];
-#pragma unroll
-                for (int q = 1; q <= L; q++) p -= st->om[q] * pv[u][q];
-                PS[(size_t)k * (L + 1) * n + i] = (PT)(p * st->psi_inv[k]);
-            }
-        }
+        *(pvec *)(XS + (size_t)i * SP + k0) = xo;
+        *(pvec *)(PS + (size_t)i * SP + k0) = po;
     }
 }
 
-__global__ void k_zero_pt(float *p, long long n) { ROWLOOP(i, n) p[i] = 0.0f; }
-__global__ void k_zero_pd(double *p, long long n) { ROWLOOP(i, n) p[i] = 0.0; }
+template <typename PT>
+__global__ __launch_bounds__(256) void k_sh_zero(PT *p, long long n) { ROWLOOP(i, n) p[i] = (PT)0; }
+
+// Tiled transposes between the reference's x layout (shift k at x + (k+1) n) and
+// the interleaved n x SP block (32 x 32 tiles through LDS, both sides coalesced).
+template <typename PT>
+__global__ __launch_bounds__(256) void k_xs_in(long long n, int ns, int SP, const double *x, PT *XS) {
+    __shared__ double tile[32][33];
+    const long long i0 = (long long)blockIdx.x * 32;
+    const int k0 = blockIdx.y * 32, tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) {
+        const int k = k0 + ty + 8 * jj;
+        const long long i = i0 + tx;
+        tile[ty + 8 * jj][tx] = (k < ns && i < n) ? x[(size_t)(k + 1) * n + i] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) {
+        const long long i = i0 + ty + 8 * jj;
+        const int k = k0 + tx;
+        if (i < n && k < SP) XS[(size_t)i * SP + k] = (PT)tile[tx][ty + 8 * jj];
+    }
+}
+
+template <typename PT>
+__global__ __launch_bounds__(256) void k_xs_out(long long n, int ns, int SP, const PT *XS, double *x) {
+    __shared__ double tile[32][33];
+    const long long i0 = (long long)blockIdx.x * 32;
+    const int k0 = blockIdx.y * 32, tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) {
+        const long long i = i0 + ty + 8 * jj;
+        const int k = k0 + tx;
+        tile[ty + 8 * jj][tx] = (i < n && k < SP) ? (double)XS[(size_t)i * SP + k] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) {
+        const int k = k0 + ty + 8 * jj;
+        const long long i = i0 + tx;
+        if (k < ns && i < n) x[(size_t)(k + 1) * n + i] = tile[tx][ty + 8 * jj];
+    }
+}
+
+// ---------------------------------------- multi-shift FP64 residual and refinement
+//
+// Per-shift state of the multi-shift BiCGSTAB that refines the FP32 shifted
+// solutions: (A + sigma_k I) d_k = r_k for every shift k at once, FP32 vectors in the
+// interleaved layout, FP64 scalars and dot accumulation, one shared sweep over A per
+// SpMV. A shift that has met its target (or broken down) is frozen: alpha = omega =
+// beta = 0 leave its d and r unchanged while the others continue.
+struct MsCol {
+    double rho, alpha, omega, beta, r0n2, tgt2, rn2, rel;
+    int act, it, pad[2];
+};
+#define MS_NB 2048          // workgroup cap of the multi-shift kernels (partials per shift)
+
+// Multi-shift kernels: thread t of the grid serves shift group t % G (G divides 256)
+// of rows t / G, t / G + stride, ... (the grid is a multiple of G threads), so all
+// workgroups sweep one window of rows together and the neighbour rows an SpMV
+// gathers are shared through L2/MALL while they are hot.
+#define MS_ROWS_SETUP                                                              \
+    const long long tid_ = (long long)blockIdx.x * 256 + threadIdx.x;              \
+    const long long rstride_ = (long long)gridDim.x * 256 / G;                     \
+    const int k0 = (int)(threadIdx.x % G) * 4;                                     \
+    const size_t SP = (size_t)G * 4
+#define MS_ROWS(i) for (long long i = tid_ / G; i < n; i += rstride_)
+
+// Sums NV x 4 per-shift values over the threads of one shift group (gq = lane % G,
+// G a power of two <= 64) and writes part[(blockIdx.x * NV + v) * SP + k], fixed order.
+template <int NV>
+__device__ __forceinline__ void ms_block_partial(double (&v)[NV][4], int G, int SP, double *part) {
+    __shared__ double sh[4][64 * 4 * NV];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int off = G; off < 64; off <<= 1) {
+#pragma unroll
+        for (int a = 0; a < NV; a++)
+#pragma unroll
+            for (int u = 0; u < 4; u++) v[a][u] += __shfl_xor(v[a][u], off, 64);
+    }
+    if (lane < G) {
+#pragma unroll
+        for (int a = 0; a < NV; a++)
+#pragma unroll
+            for (int u = 0; u < 4; u++) sh[w][(lane * 4 + u) * NV + a] = v[a][u];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < G * 4 * NV; e += 256) {
+        const double t = ((sh[0][e] + sh[1][e]) + sh[2][e]) + sh[3][e];
+        part[((size_t)blockIdx.x * NV + (e % NV)) * SP + e / NV] = t;
+    }
+}
+
+// r_k = b - (A + sigma_k I) x_k in FP64 for all shifts (XS: n x SP, FP64); the FP32
+// copy goes to R32 and RH when given; partials of |r_k|^2.
+template <typename VT>
+__global__ __launch_bounds__(256) void k_ms_resid(long long n, int G, const int *__restrict__ rp,
+                                                  const int *__restrict__ ci, const VT *__restrict__ val,
+                                                  const double *b, const double *XS, const ShState *st,
+                                                  float *R32, float *RH, double *part) {
+    typedef double dvec __attribute__((ext_vector_type(4)));
+    typedef float fvec __attribute__((ext_vector_type(4)));
+    MS_ROWS_SETUP;
+    double sg[4], dv[1][4] = {{0.0, 0.0, 0.0, 0.0}};
+#pragma unroll
+    for (int u = 0; u < 4; u++) sg[u] = st->sigma[k0 + u];
+    MS_ROWS(i) {
+        dvec acc = {0.0, 0.0, 0.0, 0.0};
+        for (int e = rp[i]; e < rp[i + 1]; e++) {
+            const double a = (double)val[e];
+            const dvec xv = *(const dvec *)(XS + (size_t)ci[e] * SP + k0);
+            acc += a * xv;
+        }
+        const dvec xi = *(const dvec *)(XS + (size_t)i * SP + k0);
+        const double bi = b[i];
+        fvec r32;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const double r = bi - acc[u] - sg[u] * xi[u];
+            r32[u] = (float)r;
+            dv[0][u] += r * r;
+        }
+        if (R32) {
+            *(fvec *)(R32 + (size_t)i * SP + k0) = r32;
+            *(fvec *)(RH + (size_t)i * SP + k0) = r32;
+        }
+    }
+    ms_block_partial<1>(dv, G, (int)SP, part);
+}
+
+// K0: p = r + beta (p - omega v) (own rows; v of the previous iteration).
+__global__ __launch_bounds__(256) void k_ms_k0(long long n, int G, const float *R, float *Pv, const float *Vv,
+                                               const MsCol *mc, const Ctl *c) {
+    if (ctl_stopped(c)) return;
+    typedef float fvec __attribute__((ext_vector_type(4)));
+    MS_ROWS_SETUP;
+    fvec be, om;
+#pragma unroll
+    for (int u = 0; u < 4; u++) { be[u] = (float)mc[k0 + u].beta; om[u] = (float)mc[k0 + u].omega; }
+    MS_ROWS(i) {
+        const size_t o = (size_t)i * SP + k0;
+        *(fvec *)(Pv + o) = *(const fvec *)(R + o) + be * (*(const fvec *)(Pv + o) - om * *(const fvec *)(Vv + o));
+    }
+}
+
+// Y = (A + sigma_k I) X for every shift (one gathered interleaved vector), with the
+// per-shift partials W.Y (NV = 1) or Y.X, Y.Y (NV = 2). K1: v = (A + sigma) p, r^.v;
+// K2: t = (A + sigma) s, t.s and t.t.
+template <typename VT, int NV>
+__global__ __launch_bounds__(256) void k_ms_spmv(long long n, int G, const int *__restrict__ rp,
+                                                 const int *__restrict__ ci, const VT *__restrict__ val,
+                                                 const float *X, float *Y, const float *W, const ShState *st,
+                                                 const Ctl *c, double *part) {
+    if (ctl_stopped(c)) return;
+    typedef float fvec __attribute__((ext_vector_type(4)));
+    MS_ROWS_SETUP;
+    fvec sg;
+#pragma unroll
+    for (int u = 0; u < 4; u++) sg[u] = (float)st->sigma[k0 + u];
+    double dv[NV][4];
+#pragma unroll
+    for (int a = 0; a < NV; a++)
+#pragma unroll
+        for (int u = 0; u < 4; u++) dv[a][u] = 0.0;
+    MS_ROWS(i) {
+        fvec acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int e = rp[i]; e < rp[i + 1]; e++)
+            acc += (float)val[e] * *(const fvec *)(X + (size_t)ci[e] * SP + k0);
+        const size_t o = (size_t)i * SP + k0;
+        const fvec xi = *(const fvec *)(X + o);
+        const fvec y = acc + sg * xi;
+        *(fvec *)(Y + o) = y;
+        if (NV == 1) {
+            const fvec w = *(const fvec *)(W + o);
+#pragma unroll
+            for (int u = 0; u < 4; u++) dv[0][u] += (double)w[u] * (double)y[u];
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                dv[0][u] += (double)y[u] * (double)xi[u];
+                dv[NV - 1][u] += (double)y[u] * (double)y[u];
+            }
+        }
+    }
+    ms_block_partial<NV>(dv, G, (int)SP, part);
+}
+
+// s = r - alpha v.
+__global__ __launch_bounds__(256) void k_ms_s(long long n, int G, const float *R, const float *Vv, float *S,
+                                              const MsCol *mc, const Ctl *c) {
+    if (ctl_stopped(c)) return;
+    typedef float fvec __attribute__((ext_vector_type(4)));
+    MS_ROWS_SETUP;
+    fvec al;
+#pragma unroll
+    for (int u = 0; u < 4; u++) al[u] = (float)mc[k0 + u].alpha;
+    MS_ROWS(i) {
+        const size_t o = (size_t)i * SP + k0;
+        *(fvec *)(S + o) = *(const fvec *)(R + o) - al * *(const fvec *)(Vv + o);
+    }
+}
+
+// K3: d += alpha p + omega s; r = s - omega t; r^.r and r.r partials.
+__global__ __launch_bounds__(256) void k_ms_k3(long long n, int G, float *D, float *R, const float *Pv,
+                                               const float *S, const float *T, const float *RH, const MsCol *mc,
+                                               const Ctl *c, double *part) {
+    if (ctl_stopped(c)) return;
+    typedef float fvec __attribute__((ext_vector_type(4)));
+    MS_ROWS_SETUP;
+    fvec al, om;
+#pragma unroll
+    for (int u = 0; u < 4; u++) { al[u] = (float)mc[k0 + u].alpha; om[u] = (float)mc[k0 + u].omega; }
+    double dv[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+    MS_ROWS(i) {
+        const size_t o = (size_t)i * SP + k0;
+        const fvec sv = *(const fvec *)(S + o);
+        *(fvec *)(D + o) = *(const fvec *)(D + o) + al * *(const fvec *)(Pv + o) + om * sv;
+        const fvec rn = sv - om * *(const fvec *)(T + o);
+        *(fvec *)(R + o) = rn;
+        const fvec rh = *(const fvec *)(RH + o);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            dv[0][u] += (double)rh[u] * (double)rn[u];
+            dv[1][u] += (double)rn[u] * (double)rn[u];
+        }
+    }
+    ms_block_partial<2>(dv, G, (int)SP, part);
+}
+
+// One workgroup per shift: fixed-order sum of the shift's partials, then `op`.
+template <int NV, class Op>
+__global__ __launch_bounds__(256) void k_ms_fin(const double *part, int nb, int SP, MsCol *mc, const Ctl *c, Op op) {
+    if (c && ctl_stopped(c)) return;
+    __shared__ double sh[NV * 4];
+    const int k = blockIdx.x;
+    double acc[NV];
+#pragma unroll
+    for (int a = 0; a < NV; a++) acc[a] = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) {
+#pragma unroll
+        for (int a = 0; a < NV; a++) acc[a] += part[((size_t)b * NV + a) * SP + k];
+    }
+    block_sum<NV, 256>(acc, sh);
+    if (threadIdx.x == 0) op(mc[k], acc, k);
+}
+
+struct MsStartOp {      // a pass: r0 = b - (A + sigma I) x, targets, r^ = r0
+    int ns;
+    double tol, fnb, itol_min;
+    __device__ void operator()(MsCol &m, const double *a, int k) const {
+        const double rel = sqrt(a[0]) / fnb;
+        m.rel = rel;
+        m.rn2 = a[0];
+        m.r0n2 = a[0];
+        m.rho = a[0];
+        m.alpha = 0.0; m.omega = 0.0; m.beta = 0.0; m.it = 0;
+        m.act = (k < ns && rel == rel && !(rel < tol) && a[0] > 0.0) ? 1 : 0;
+        double itol = 0.25 * tol / rel;
+        if (itol < itol_min) itol = itol_min;
+        if (itol > 0.5) itol = 0.5;
+        m.tgt2 = itol * itol * a[0];
+    }
+};
+struct MsAlphaOp {
+    __device__ void operator()(MsCol &m, const double *a, int) const {
+        if (!m.act) { m.alpha = 0.0; return; }
+        if (bad(a[0])) { m.act = 0; m.alpha = 0.0; m.beta = 0.0; return; }
+        m.alpha = m.rho / a[0];
+    }
+};
+struct MsOmegaOp {
+    __device__ void operator()(MsCol &m, const double *a, int) const {
+        if (!m.act) { m.omega = 0.0; return; }
+        m.omega = (a[1] > 0.0 && a[1] == a[1]) ? a[0] / a[1] : 0.0;
+    }
+};
+struct MsRhoOp {
+    int max_it;
+    __device__ void operator()(MsCol &m, const double *a, int) const {
+        if (!m.act) { m.alpha = 0.0; m.omega = 0.0; m.beta = 0.0; return; }
+        m.rn2 = a[1];
+        m.it++;
+        if (a[1] <= m.tgt2 || m.it >= max_it || m.omega == 0.0 || bad(a[0]) || !(a[1] == a[1])) {
+            m.act = 0; m.beta = 0.0;
+            return;
+        }
+        m.beta = (a[0] / m.rho) * (m.alpha / m.omega);
+        m.rho = a[0];
+    }
+};
+
+// Stops the pass (Ctl::done) when no shift is active; counts iterations in Ctl::it.
+__global__ __launch_bounds__(64) void k_ms_check(const MsCol *mc, int SP, Ctl *c) {
+    if (ctl_stopped(c)) return;
+    int any = 0;
+    for (int k = threadIdx.x; k < SP; k += 64) any |= mc[k].act;
+    for (int o = 32; o >= 1; o >>= 1) any |= __shfl_xor(any, o, 64);
+    if (threadIdx.x == 0) {
+        c->it++;
+        if (!any) { c->done = 1; c->status = ST_CONVERGED; }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_ms_apply(long long m, double *X, const float *D) {
+    ROWLOOP(i, m) X[i] += (double)D[i];
+}
+
+template <typename PT>
+__global__ __launch_bounds__(256) void k_to_f64(long long m, const PT *a, double *b) {
+    ROWLOOP(i, m) b[i] = (double)a[i];
+}
+
+struct MsBufs {
+    float *R, *RH, *D, *P, *V, *S, *T;
+};
+
+// Per-shift FP64 true residuals |b - (A + sigma_k I) x_k| / |b| of the interleaved
+// FP64 solutions; `rel` (host, SP entries).
+static int ms_true_res(ThreadCtx *t, const bkh_matrix *A, const double *b, const double *XS64, int SP,
+                       const ShState *st, MsCol *mc, double *part, double fnb, int ns, double *rel) {
+    hipStream_t s = t->stream;
+    const ShGrid g = sh_grid(A->n, SP, 4, MS_NB);
+    hipLaunchKernelGGL((k_ms_resid<double>), dim3(g.blocks), dim3(256), 0, s, (long long)A->n, g.G, A->rp, A->ci,
+                       (const double *)A->val, b, XS64, st, (float *)nullptr, (float *)nullptr, part);
+    hipLaunchKernelGGL((k_ms_fin<1, MsStartOp>), dim3(SP), dim3(256), 0, s, (const double *)part, g.blocks, SP, mc,
+                       (const Ctl *)nullptr, MsStartOp{ns, 0.0, fnb, 1e-6});
+    LAUNCH_OK();
+    std::vector<MsCol> h(SP);
+    HIP_OK(hipMemcpyAsync(h.data(), mc, sizeof(MsCol) * SP, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    for (int k = 0; k < SP; k++) rel[k] = h[k].rel;
+    return 0;
+}
+
+// FP64 refinement of every shift's solution XS64 (n x SP interleaved, FP64) with the
+// multi-shift FP32 BiCGSTAB; at most max_pass passes. Returns 0 or a device error.
+static int ms_refine(ThreadCtx *t, const bkh_matrix *A, const double *b, double *XS64, int SP, int ns,
+                     const ShState *st, MsCol *mc, Ctl *c, double *part, const MsBufs &B, double fnb,
+                     double tol, int max_inner, int max_pass, int *passes, int *inner_total) {
+    hipStream_t s = t->stream;
+    const long long n = A->n, m = n * SP;
+    const ShGrid g = sh_grid(n, SP, 4, MS_NB);
+    const float *v32 = A->val32;
+    const int Kg = 8;
+    *passes = 0;
+    *inner_total = 0;
+    hipGraphExec_t ge = capture_cached(t, {0x5B5F, (uintptr_t)A, (uintptr_t)XS64, (uintptr_t)SP, (uintptr_t)st,
+                                           (uintptr_t)mc, (uintptr_t)c, (uintptr_t)part, (uintptr_t)B.R,
+                                           (uintptr_t)B.P, (uintptr_t)max_inner}, s, 1, [&]() {
+        for (int j = 0; j < Kg; j++) {
+            hipLaunchKernelGGL(k_ms_k0, dim3(g.blocks), dim3(256), 0, s, n, g.G, (const float *)B.R, B.P,
+                               (const float *)B.V, (const MsCol *)mc, (const Ctl *)c);
+            if (v32)
+                hipLaunchKernelGGL((k_ms_spmv<float, 1>), dim3(g.blocks), dim3(256), 0, s, n, g.G, A->rp, A->ci, v32,
+                                   (const float *)B.P, B.V, (const float *)B.RH, st, (const Ctl *)c, part);
+            else
+                hipLaunchKernelGGL((k_ms_spmv<double, 1>), dim3(g.blocks), dim3(256), 0, s, n, g.G, A->rp, A->ci,
+                                   (const double *)A->val, (const float *)B.P, B.V, (const float *)B.RH, st,
+                                   (const Ctl *)c, part);
+            hipLaunchKernelGGL((k_ms_fin<1, MsAlphaOp>), dim3(SP), dim3(256), 0, s, (const double *)part, g.blocks,
+                               SP, mc, (const Ctl *)c, MsAlphaOp());
+            hipLaunchKernelGGL(k_ms_s, dim3(g.blocks), dim3(256), 0, s, n, g.G, (const float *)B.R,
+                               (const float *)B.V, B.S, (const MsCol *)mc, (const Ctl *)c);
+            if (v32)
+                hipLaunchKernelGGL((k_ms_spmv<float, 2>), dim3(g.blocks), dim3(256), 0, s, n, g.G, A->rp, A->ci, v32,
+                                   (const float *)B.S, B.T, (const float *)nullptr, st, (const Ctl *)c, part);
+            else
+                hipLaunchKernelGGL((k_ms_spmv<double, 2>), dim3(g.blocks), dim3(256), 0, s, n, g.G, A->rp, A->ci,
+                                   (const double *)A->val, (const float *)B.S, B.T, (const float *)nullptr, st,
+                                   (const Ctl *)c, part);
+            hipLaunchKernelGGL((k_ms_fin<2, MsOmegaOp>), dim3(SP), dim3(256), 0, s, (const double *)part, g.blocks,
+                               SP, mc, (const Ctl *)c, MsOmegaOp());
+            hipLaunchKernelGGL(k_ms_k3, dim3(g.blocks), dim3(256), 0, s, n, g.G, B.D, B.R, (const float *)B.P,
+                               (const float *)B.S, (const float *)B.T, (const float *)B.RH, (const MsCol *)mc,
+                               (const Ctl *)c, part);
+            hipLaunchKernelGGL((k_ms_fin<2, MsRhoOp>), dim3(SP), dim3(256), 0, s, (const double *)part, g.blocks,
+                               SP, mc, (const Ctl *)c, MsRhoOp{max_inner});
+            hipLaunchKernelGGL(k_ms_check, dim3(1), dim3(64), 0, s, (const MsCol *)mc, SP, c);
+        }
+    });
+    if (!ge) { bkh_set_error("hipGraph capture of the multi-shift refinement failed"); return BKH_DEVICE_ERROR; }
+    for (int pass = 0; pass < max_pass; pass++) {
+        if (ctl_init(t, c, 0.0, 1 << 30)) return BKH_DEVICE_ERROR;
+        hipLaunchKernelGGL((k_ms_resid<double>), dim3(g.blocks), dim3(256), 0, s, n, g.G, A->rp, A->ci,
+                           (const double *)A->val, b, (const double *)XS64, st, B.R, B.RH, part);
+        hipLaunchKernelGGL((k_ms_fin<1, MsStartOp>), dim3(SP), dim3(256), 0, s, (const double *)part, g.blocks, SP,
+                           mc, (const Ctl *)nullptr, MsStartOp{ns, tol, fnb, 1e-6});
+        // D, P, V = 0 (the first K0 then forms p = r)
+        hipLaunchKernelGGL(k_sh_zero<float>, dim3(vec_grid(m, 256)), dim3(256), 0, s, B.D, m);
+        hipLaunchKernelGGL(k_sh_zero<float>, dim3(vec_grid(m, 256)), dim3(256), 0, s, B.P, m);
+        hipLaunchKernelGGL(k_sh_zero<float>, dim3(vec_grid(m, 256)), dim3(256), 0, s, B.V, m);
+        hipLaunchKernelGGL(k_ms_check, dim3(1), dim3(64), 0, s, (const MsCol *)mc, SP, c);
+        LAUNCH_OK();
+        if (ctl_fetch(t, c)) return BKH_DEVICE_ERROR;
+        if (t->ctl_host->done) break;                      // every shift meets tol
+        if (ctl_init(t, c, 0.0, 1 << 30)) return BKH_DEVICE_ERROR;
+        if (run_graph_until_done(t, ge, c, max_inner / Kg + 2)) return BKH_DEVICE_ERROR;
+        *inner_total += t->ctl_host->it;
+        hipLaunchKernelGGL(k_ms_apply, dim3(vec_grid(m, 256)), dim3(256), 0, s, m, XS64, (const float *)B.D);
+        LAUNCH_OK();
+        *passes = pass + 1;
+    }
+    return 0;
+}
 
 // SpMV r_L = A r_{L-1} with the Gram epilogue of [r_0..r_L], then the MR finalize.
 template <int L>
@@ -382,23 +831,23 @@ static void sh_alpha_spmv(hipStream_t s, const bkh_matrix *A, const double *x, d
 }
 
 template <typename PT>
-static void sh_shift_step(int j, dim3 grid, hipStream_t s, long long n, int l, const double *P,
-                          const double *R, double *x, PT *PS, const ShState *st, const Ctl *c) {
+static void sh_shift_step(int j, const ShGrid &g, hipStream_t s, long long n, const double *P, const double *R,
+                          PT *XS, PT *PS, const ShState *st, const Ctl *c) {
     switch (j) {
-#define CASE(J_) case J_: hipLaunchKernelGGL((k_sh_shift_step<J_, PT>), grid, dim3(256), 0, s, n, l, P, R, x, PS, st, c); break;
+#define CASE(J_) case J_: hipLaunchKernelGGL((k_sh_step<J_, PT>), dim3(g.blocks), dim3(256), 0, s, n, g.G, P, R, XS, PS, st, c); break;
         CASE(0) CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6)
-        default: hipLaunchKernelGGL((k_sh_shift_step<7, PT>), grid, dim3(256), 0, s, n, l, P, R, x, PS, st, c); break;
+        default: hipLaunchKernelGGL((k_sh_step<7, PT>), dim3(g.blocks), dim3(256), 0, s, n, g.G, P, R, XS, PS, st, c); break;
 #undef CASE
     }
 }
 
 template <typename PT>
-static void sh_shift_upd(int l, dim3 grid, hipStream_t s, long long n, const double *R, double *x, PT *PS,
+static void sh_shift_upd(int l, const ShGrid &g, hipStream_t s, long long n, const double *R, PT *XS, PT *PS,
                          const ShState *st, const Ctl *c) {
     switch (l) {
-#define CASE(L_) case L_: hipLaunchKernelGGL((k_sh_shift_upd<L_, PT>), grid, dim3(256), 0, s, n, R, x, PS, st, c); break;
+#define CASE(L_) case L_: hipLaunchKernelGGL((k_sh_upd<L_, PT>), dim3(g.blocks), dim3(256), 0, s, n, g.G, R, XS, PS, st, c); break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
-        default: hipLaunchKernelGGL((k_sh_shift_upd<8, PT>), grid, dim3(256), 0, s, n, R, x, PS, st, c); break;
+        default: hipLaunchKernelGGL((k_sh_upd<8, PT>), dim3(g.blocks), dim3(256), 0, s, n, g.G, R, XS, PS, st, c); break;
 #undef CASE
     }
 }
@@ -417,14 +866,17 @@ static void sh_gram_dispatch(int l, hipStream_t s, const bkh_matrix *A, double *
     }
 }
 
+// Shifted BiCGSTAB(l) with the shifts' p_shift and x_k in interleaved PT blocks:
+// the seed in x[0, n), shift k's solution in XS (n x SP), starting from x's shifted
+// parts (the reference's x + (k+1) n layout).
 template <typename PT>
 static int sh_core(ThreadCtx *t, const bkh_matrix *A, int l, const double *b, double *x, double *R,
-                   double *P, double *rs, PT *PS, ShState *st, const double *sigma, int ns, Ctl *c,
-                   double *part, double tol, int max_it, int K) {
+                   double *P, double *rs, PT *PS, PT *XS, int SP, ShState *st, const double *sigma, int ns,
+                   Ctl *c, double *part, double tol, int max_it, int K) {
     hipStream_t s = t->stream;
     const long long n = A->n;
     const int G = vec_grid(n, 256);
-    const dim3 g2(G, (ns + SH_SPT - 1) / SH_SPT > 0 ? (ns + SH_SPT - 1) / SH_SPT : 1);
+    const ShGrid g = sh_grid(n, SP, 16 / (int)sizeof(PT));
     if (ctl_init(t, c, tol, max_it)) return BKH_DEVICE_ERROR;
     {
         ShState *h = new ShState();
@@ -437,24 +889,26 @@ static int sh_core(ThreadCtx *t, const bkh_matrix *A, int l, const double *b, do
         delete h;
         if (e != hipSuccess || e2 != hipSuccess) { bkh_set_error("ShState upload failed"); return BKH_DEVICE_ERROR; }
     }
-    const long long nps = (long long)ns * (l + 1) * n;      // p_shift starts at 0 (:53 calloc)
-    if (sizeof(PT) == 4)
-        hipLaunchKernelGGL(k_zero_pt, dim3(vec_grid(nps, 256)), dim3(256), 0, s, (float *)PS, nps);
-    else
-        hipLaunchKernelGGL(k_zero_pd, dim3(vec_grid(nps, 256)), dim3(256), 0, s, (double *)PS, nps);
+    if (ns > 0) {
+        const long long nps = (long long)SP * (l + 1) * n;     // p_shift starts at 0 (:53 calloc)
+        hipLaunchKernelGGL(k_sh_zero<PT>, dim3(vec_grid(nps, 256)), dim3(256), 0, s, PS, nps);
+        hipLaunchKernelGGL(k_xs_in<PT>, dim3((unsigned)((n + 31) / 32), (unsigned)((SP + 31) / 32)), dim3(256), 0, s,
+                           n, ns, SP, (const double *)x, XS);
+    }
     Epi<0> e0;
     memset(&e0, 0, sizeof e0);
     launch_spmv<double, double, 0>(s, A, A->val, x, R, 1.0, 0.0, e0, nullptr, nullptr, NoFin());
     hipLaunchKernelGGL(k_sh_init, dim3(G), dim3(256), 0, s, n, b, R, P, rs, part);
     launch_finalize<2>(s, part, G, c, ShInitFin{st});
     LAUNCH_OK();
-    hipGraphExec_t ge = capture_cached(t, {0x5B00, (uintptr_t)A, (uintptr_t)b, (uintptr_t)x, (uintptr_t)R, (uintptr_t)P,
-                                           (uintptr_t)rs, (uintptr_t)PS, (uintptr_t)st, (uintptr_t)c, (uintptr_t)part,
-                                           (uintptr_t)ns, (uintptr_t)l, (uintptr_t)sizeof(PT)}, s, K, [&]() {
+    hipGraphExec_t ge = capture_cached(t, {0x5B01, (uintptr_t)A, (uintptr_t)b, (uintptr_t)x, (uintptr_t)R, (uintptr_t)P,
+                                           (uintptr_t)rs, (uintptr_t)PS, (uintptr_t)XS, (uintptr_t)SP, (uintptr_t)st,
+                                           (uintptr_t)c, (uintptr_t)part, (uintptr_t)ns, (uintptr_t)l,
+                                           (uintptr_t)sizeof(PT)}, s, K, [&]() {
         for (int j = 0; j < l; j++) {
             hipLaunchKernelGGL(k_sh_p, dim3(G), dim3(256), 0, s, n, j, P, (const double *)R, c);
             sh_alpha_spmv(s, A, P + j * n, P + (j + 1) * n, rs, c, part, st);
-            if (ns > 0) sh_shift_step<PT>(j, g2, s, n, l, (const double *)P, (const double *)R, x, PS, st, c);
+            if (ns > 0) sh_shift_step<PT>(j, g, s, n, (const double *)P, (const double *)R, XS, PS, st, c);
             hipLaunchKernelGGL(k_sh_seed_r, dim3(G), dim3(256), 0, s, n, j, (const double *)P, R, x, c);
             if (j < l - 1)
                 launch_spmv_dot_w<double, double>(s, A, A->val, R + j * n, R + (j + 1) * n, rs, c, part,
@@ -462,13 +916,19 @@ static int sh_core(ThreadCtx *t, const bkh_matrix *A, int l, const double *b, do
             else
                 sh_gram_dispatch(l, s, A, R, n, c, part, st);
         }
-        if (ns > 0) sh_shift_upd<PT>(l, g2, s, n, (const double *)R, x, PS, st, c);
+        if (ns > 0) sh_shift_upd<PT>(l, g, s, n, (const double *)R, XS, PS, st, c);
         hipLaunchKernelGGL(k_sh_seed_upd, dim3(G), dim3(256), 0, s, n, l, P, R, x, (const double *)rs, c, part);
         launch_finalize<2>(s, part, G, c, ShUpdFin());
     });
     if (!ge) { bkh_set_error("hipGraph capture of the shifted BiCGSTAB(l) iteration failed"); return BKH_DEVICE_ERROR; }
     int rc = run_graph_until_done(t, ge, c, max_it / K + 2);
     return rc;
+}
+
+static int pow2_at_least(int v) {
+    int p = 4;
+    while (p < v) p <<= 1;
+    return p;
 }
 
 extern "C" int bkh_shbicgstabl(bkh_matrix *A, const double *b, int l, const double *sigma, int ns,
@@ -494,33 +954,70 @@ extern "C" int bkh_shbicgstabl(bkh_matrix *A, const double *b, int l, const doub
     const int nv = (l + 1) * (l + 2) / 2;
     const int nparts = std::max(spmv_grid(A), vec_grid(n, 256)) * std::max(nv, 2);
     const bool p32 = (o.precision == BKH_FP32_IR);
-    const size_t psb = (size_t)ns * (l + 1) * n * (p32 ? sizeof(float) : sizeof(double));
-    size_t bytes = carve_size(sizeof(Ctl)) + carve_size(sizeof(ShState)) + carve_size(sizeof(double) * nparts) +
-                   (2 * (l + 1) + 1) * carve_size(sizeof(double) * n) + carve_size(psb);
+    const int SP = pow2_at_least(std::max(ns, 1));        // interleaved width (power of two, >= 4)
+    const size_t pts = p32 ? sizeof(float) : sizeof(double);
+    const size_t blk = (size_t)SP * n;                     // one interleaved block
+    const bool shifts = ns > 0;
+    size_t bytes = carve_size(sizeof(Ctl)) * 2 + carve_size(sizeof(ShState)) + carve_size(sizeof(double) * nparts) +
+                   (2 * (l + 1) + 1) * carve_size(sizeof(double) * n) + carve_size(sizeof(MsCol) * SP) +
+                   carve_size(sizeof(double) * (size_t)MS_NB * 2 * SP);
+    if (shifts) {
+        bytes += carve_size(pts * blk * (l + 1)) + carve_size(pts * blk);       // p_shift, x_k
+        if (p32) bytes += carve_size(sizeof(double) * blk) + 7 * carve_size(sizeof(float) * blk);
+    }
     void *ws = bkh_workspace(bytes);
     if (!ws) return BKH_DEVICE_ERROR;
     Carve cv(ws);
-    Ctl *c = cv.take<Ctl>(1);
+    Ctl *c = cv.take<Ctl>(1), *c2 = cv.take<Ctl>(1);
     ShState *st = cv.take<ShState>(1);
     double *part = cv.take<double>(nparts);
     double *R = cv.take<double>((l + 1) * n), *P = cv.take<double>((l + 1) * n), *rs = cv.take<double>(n);
-    char *PS = cv.take<char>(psb);
-    HIP_OK(hipEventRecord(t->ev0, t->stream));
-    int rc = p32 ? sh_core<float>(t, A, l, b, x, R, P, rs, (float *)PS, st, sigma, ns, c, part, tol, max_iter, K)
-                 : sh_core<double>(t, A, l, b, x, R, P, rs, (double *)PS, st, sigma, ns, c, part, tol, max_iter, K);
+    MsCol *mc = cv.take<MsCol>(SP);
+    double *part2 = cv.take<double>((size_t)MS_NB * 2 * SP);
+    char *PS = shifts ? cv.take<char>(pts * blk * (l + 1)) : nullptr;
+    char *XS = shifts ? cv.take<char>(pts * blk) : nullptr;
+    double *XS64 = p32 ? nullptr : (double *)XS;
+    MsBufs B;
+    memset(&B, 0, sizeof B);
+    if (shifts && p32) {
+        XS64 = cv.take<double>(blk);
+        B.R = cv.take<float>(blk); B.RH = cv.take<float>(blk); B.D = cv.take<float>(blk);
+        B.P = cv.take<float>(blk); B.V = cv.take<float>(blk); B.S = cv.take<float>(blk); B.T = cv.take<float>(blk);
+    }
+    hipStream_t s = t->stream;
+    HIP_OK(hipEventRecord(t->ev0, s));
+    int rc = p32 ? sh_core<float>(t, A, l, b, x, R, P, rs, (float *)PS, (float *)XS, SP, st, sigma, ns, c, part, tol,
+                                  max_iter, K)
+                 : sh_core<double>(t, A, l, b, x, R, P, rs, (double *)PS, (double *)XS, SP, st, sigma, ns, c, part,
+                                   tol, max_iter, K);
     if (rc) return rc;
+    const double fnb = t->ctl_host->fnb;
+    if (shifts && p32) {
+        // FP64 refinement of every shift (the FP32 p_shift limits them to ~kappa eps32)
+        hipLaunchKernelGGL(k_to_f64<float>, dim3(vec_grid((long long)blk, 256)), dim3(256), 0, s, (long long)blk,
+                           (const float *)XS, XS64);
+        LAUNCH_OK();
+        if (fnb > 0.0) {
+            const int max_inner = o.max_inner > 0 ? o.max_inner : 1000;
+            if (ms_refine(t, A, b, XS64, SP, ns, st, mc, c2, part2, B, fnb, tol, max_inner, 4, &inf.passes,
+                          &inf.inner_iters))
+                return BKH_DEVICE_ERROR;
+        }
+    }
+    if (shifts) {
+        hipLaunchKernelGGL(k_xs_out<double>, dim3((unsigned)((n + 31) / 32), (unsigned)((SP + 31) / 32)), dim3(256), 0,
+                           s, n, ns, SP, (const double *)XS64, x);
+        LAUNCH_OK();
+    }
+    if (ctl_fetch(t, c)) return BKH_DEVICE_ERROR;          // the seed's control block for finish_solve
     int ret = finish_solve(t, A, 0.0, b, x, &inf, info);
     if (shift_true_res) {
-        shift_true_res[0] = inf.true_res;
-        for (int k = 0; k < ns; k++) {
-            double tr = 0.0;
-            if (true_residual(t, A, sigma[k], b, x + (size_t)(k + 1) * n, &tr)) return BKH_DEVICE_ERROR;
-            shift_true_res[k + 1] = tr;
-        }
-        if (!info) {
-            double tr = 0.0;
-            if (true_residual(t, A, 0.0, b, x, &tr)) return BKH_DEVICE_ERROR;
-            shift_true_res[0] = tr;
+        if (info) shift_true_res[0] = inf.true_res;
+        else if (true_residual(t, A, 0.0, b, x, &shift_true_res[0])) return BKH_DEVICE_ERROR;
+        if (shifts) {
+            std::vector<double> rel(SP);
+            if (fnb > 0.0 && ms_true_res(t, A, b, XS64, SP, st, mc, part2, fnb, ns, rel.data())) return BKH_DEVICE_ERROR;
+            for (int k = 0; k < ns; k++) shift_true_res[k + 1] = fnb > 0.0 ? rel[k] : 0.0;
         }
     }
     return ret;

@@ -85,6 +85,33 @@ def test_cfg3_full_size(prec):
     assert tr[0] < 2e-8
     if prec == 0:
         assert np.all(tr < 2e-8), tr.max()
+    else:
+        # FP32 p_shift/x_k, then FP64 refinement of every shift to tol
+        assert np.all(tr[1:] < 1e-8), tr.max()
+        assert info.passes >= 1
+
+
+@pytest.mark.parametrize("ns", [64, 5, 1])
+def test_cfg3_small_fp32_refined(ns):
+    """FP32-IR shifted solve at 20^3: every shifted system reaches tol through the
+    multi-shift FP64 refinement; the solutions agree with the oracle's FP64 ones to
+    what both tolerances allow (cond * tol)."""
+    n, rp, ci, v, b, sig = cfg3(20)
+    sig = sig[:ns] if ns < 64 else sig
+    M = dv.Matrix(n, rp, ci, v, fp32=True)
+    db, dx = dv.DeviceArray.from_numpy(b), dv.DeviceArray.from_numpy(np.zeros(n * (len(sig) + 1)))
+    info, tr = dv.shbicgstabl(M, db, 4, sig, dx, 1e-8, 1000, precision=dv.BKH_FP32_IR)
+    cret, cX = po.shbicgstabl(n, rp, ci, v, b, 4, sig, 1e-8, 1000)
+    print(f"ns={len(sig)}: outer {info.ret} (oracle {cret}), refinement passes {info.passes}, "
+          f"inner {info.inner_iters}, max shifted true res {tr[1:].max():.2e}")
+    assert info.ret >= 0 and abs(info.ret - cret) <= max(1, int(0.05 * cret))
+    assert np.all(tr[1:] < 1e-8), tr
+    X = dx.numpy().reshape(len(sig) + 1, n)
+    S = to_scipy(n, rp, ci, v)
+    for q, sgm in enumerate(sig):
+        r = b - S @ X[q + 1] - sgm * X[q + 1]
+        assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-8
+        assert rel(X[q + 1], cX[q + 1]) < 1e-6
 
 
 def test_dropin_shifted_bicg_reference_scenario(pde2961):
