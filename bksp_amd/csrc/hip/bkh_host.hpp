@@ -5,6 +5,8 @@
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
+#include <functional>
+#include <memory>
 #include <vector>
 #include <utility>
 #include "bkh_core.hpp"
@@ -97,6 +99,8 @@ struct ThreadCtx {
     double *red2 = nullptr;            // second-level partial sums (RED2_CAP doubles)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<std::pair<std::vector<uintptr_t>, hipGraphExec_t>> graphs;   // graph cache
+    // eager mode: the last few iteration bodies, run as plain launches
+    std::vector<std::unique_ptr<std::pair<std::function<void()>, int>>> eager;
 };
 ThreadCtx *bkh_tctx();                 // nullptr (with error set) if init failed
 #define RED2_CAP (1 << 17)
@@ -150,8 +154,17 @@ void graph_cache_put(ThreadCtx *t, const GraphKey &key, hipGraphExec_t ge);
 uintptr_t bkh_graph_epoch();
 void bkh_graph_epoch_bump();
 
+// Eager mode (environment variable BKH_EAGER=1): an iteration body is kept as a
+// function and run_graph_until_done issues its launches directly on the stream
+// instead of replaying a graph -- the same kernels in the same order, for tools that
+// do not follow graph launches. The handle is valid for the rest of the calling
+// solve (the body refers to the caller's locals); the last few are kept per thread.
+bool bkh_eager();
+hipGraphExec_t eager_handle(ThreadCtx *t, std::function<void()> body, int K);
+
 template <class Body>
 hipGraphExec_t capture_cached(ThreadCtx *t, GraphKey key, hipStream_t s, int K, Body body) {
+    if (bkh_eager()) return eager_handle(t, std::function<void()>(body), K);
     key.push_back(bkh_graph_epoch());
     key.push_back((uintptr_t)t->ws);
     key.push_back((uintptr_t)t->ws_bytes);

@@ -31,6 +31,10 @@ struct BlkMats {
 
 enum { FIN_QR1 = 0, FIN_QR2_INIT = 1, FIN_QR2_ITER = 2, FIN_ALPHA = 3, FIN_FNB = 4 };
 
+__global__ __launch_bounds__(256) void k_sh_fill_f(float *p, long long m) {
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < m; e += (long long)gridDim.x * 256) p[e] = 0.0f;
+}
+
 // One workgroup: sums the s x s Gram partials of `nblocks` workgroups in a fixed
 // order, then runs the block solver's small dense step `op`.
 template <int S, int OP>
@@ -263,6 +267,111 @@ static int block_true_res(ThreadCtx *t, const bkh_matrix *A, const double *Bt, c
     return 0;
 }
 
+// R = B - AX in FP64 (AX holds A X on entry, R may alias it), its FP32 copy, and
+// partials of |R|_F^2 and |B|_F^2.
+__global__ __launch_bounds__(256) void k_blk_resid32(long long ms, const double *B, const double *AX, double *R,
+                                                     float *R32, double *part) {
+    double v[2] = {0.0, 0.0};
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < ms; e += (long long)gridDim.x * 256) {
+        const double r = B[e] - AX[e];
+        R[e] = r;
+        R32[e] = (float)r;
+        v[0] += r * r;
+        v[1] += B[e] * B[e];
+    }
+    block_partial<2, 256>(v, part);
+}
+
+__global__ __launch_bounds__(256) void k_blk_addx(long long ms, double *X, const float *D) {
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < ms; e += (long long)gridDim.x * 256)
+        X[e] += (double)D[e];
+}
+
+struct BlkResFin {     // s[0] = |R|_F / |B|_F, s[1] = |R|_F, s[2] = |B|_F
+    __device__ void operator()(Ctl *c, const double *tot) const {
+        c->s[1] = sqrt(tot[0]);
+        c->s[2] = sqrt(tot[1]);
+        c->s[0] = c->s[2] > 0.0 ? c->s[1] / c->s[2] : 0.0;
+    }
+};
+
+// BKH_FP32_IR block CG: restarted FP64 iterative refinement around the FP32 block CG
+// (FP32 values and blocks; FP64 Gram accumulation, small s x s algebra and
+// convergence test): R = B - A X (FP64), solve A D = R to a relative inner
+// tolerance with block CG in FP32, X += D (FP64), until |B - A X|_F / |B|_F < tol.
+// Each pass starts a new block Krylov space from R with the same first-direction
+// rule as the FP64 path (bkh_opts.fix_blcg_p0).
+template <int S>
+static int blcg_ir(ThreadCtx *t, bkh_matrix *A, const double *B, double *X, double tol, int max_iter,
+                   const bkh_opts &o, bkh_info *inf) {
+    if (!A->val32) { bkh_set_error("FP32_IR needs a matrix created with want_fp32"); return BKH_BAD_ARGUMENT; }
+    const long long m = A->n, ms = m * S;
+    const int G = block_grid(m), Gv = vec_grid(ms, 256);
+    const int K = o.graph_iters > 0 ? o.graph_iters : std::max(2, default_graph_iters(m) / 4);
+    const int nparts = std::max(G * S * S, 2 * Gv);
+    size_t bytes = 2 * carve_size(sizeof(Ctl)) + carve_size(sizeof(BlkMats)) + carve_size(sizeof(double) * nparts) +
+                   3 * carve_size(sizeof(double) * ms) + 5 * carve_size(sizeof(float) * ms);
+    void *ws = bkh_workspace(bytes);
+    if (!ws) return BKH_DEVICE_ERROR;
+    Carve cv(ws);
+    Ctl *c = cv.take<Ctl>(1), *co = cv.take<Ctl>(1);
+    BlkMats *mt = cv.take<BlkMats>(1);
+    double *part = cv.take<double>(nparts);
+    double *Bt = cv.take<double>(ms), *Xt = cv.take<double>(ms), *R = cv.take<double>(ms);
+    float *R32 = cv.take<float>(ms), *D32 = cv.take<float>(ms), *V32 = cv.take<float>(ms), *P32 = cv.take<float>(ms),
+          *AP32 = cv.take<float>(ms);
+    hipStream_t s = t->stream;
+    HIP_OK(hipEventRecord(t->ev0, s));
+    hipLaunchKernelGGL((k_cm2rm<double, double>), dim3(Gv), dim3(256), 0, s, m, S, B, Bt);
+    hipLaunchKernelGGL((k_cm2rm<double, double>), dim3(Gv), dim3(256), 0, s, m, S, (const double *)X, Xt);
+    LAUNCH_OK();
+    const int max_inner = o.max_inner > 0 ? o.max_inner : max_iter;
+    int total = 0, passes = 0, rc = 0, st = ST_MAXIT;
+    double rel = 0.0;
+    for (;;) {
+        if (ctl_init(t, co, 0.0, 1)) return BKH_DEVICE_ERROR;
+        launch_spmm<S>(s, A, A->val, (const double *)Xt, R, 1.0, 0.0, (const Ctl *)nullptr);
+        hipLaunchKernelGGL(k_blk_resid32, dim3(Gv), dim3(256), 0, s, ms, (const double *)Bt, (const double *)R, R,
+                           R32, part);
+        launch_finalize<2>(s, part, Gv, co, BlkResFin());
+        LAUNCH_OK();
+        if (ctl_fetch(t, co)) return BKH_DEVICE_ERROR;
+        rel = t->ctl_host->s[0];
+        if (rel < tol) { st = ST_CONVERGED; break; }
+        if (!(rel == rel) || rel > 1e300) { st = ST_BREAKDOWN; break; }
+        if (total >= max_iter) { st = ST_MAXIT; break; }
+        double itol = o.inner_tol > 0 ? o.inner_tol : 1e-5;
+        itol = std::max(itol, 0.5 * tol / rel);
+        hipLaunchKernelGGL((k_sh_fill_f), dim3(Gv), dim3(256), 0, s, D32, ms);
+        LAUNCH_OK();
+        rc = blcg_core<S, float, float>(t, A, A->val32, (const float *)R32, D32, V32, P32, AP32, mt, c, part, itol,
+                                        std::min(max_inner, max_iter - total), o.fix_blcg_p0, K);
+        if (rc) return rc;
+        const Ctl hc = *t->ctl_host;
+        total += hc.it;
+        passes++;
+        if (hc.status == ST_BREAKDOWN) { st = ST_BREAKDOWN; break; }
+        hipLaunchKernelGGL(k_blk_addx, dim3(Gv), dim3(256), 0, s, ms, Xt, (const float *)D32);
+        LAUNCH_OK();
+    }
+    hipLaunchKernelGGL((k_rm2cm<double, double>), dim3(Gv), dim3(256), 0, s, m, S, (const double *)Xt, X);
+    LAUNCH_OK();
+    HIP_OK(hipEventRecord(t->ev1, s));
+    HIP_OK(hipEventSynchronize(t->ev1));
+    float msec = 0.f;
+    HIP_OK(hipEventElapsedTime(&msec, t->ev0, t->ev1));
+    inf->time_ms = msec;
+    bkh_note_solve_ms(msec);
+    inf->status = st;
+    inf->inner_iters = total;
+    inf->passes = passes;
+    inf->iters = total > 0 ? total - 1 : 0;
+    inf->rec_res = rel;
+    inf->true_res = rel;
+    inf->ret = st == ST_CONVERGED ? inf->iters : (st == ST_BREAKDOWN ? -2 : -1);
+    return inf->ret;
+}
+
 template <int S>
 static int blcg_dispatch(ThreadCtx *t, bkh_matrix *A, const double *B, double *X, double tol,
                          int max_iter, const bkh_opts &o, bkh_info *inf, bool want_true) {
@@ -325,6 +434,18 @@ extern "C" int bkh_blcg(bkh_matrix *A, int s, const double *B, double *X, double
         return 0;
     }
     int ret;
+    if (o.precision == BKH_FP32_IR) {
+        switch (s) {
+#define CASE(S_) case S_: ret = blcg_ir<S_>(t, A, B, X, tol, max_iter, o, &inf); break;
+            CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+            CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
+#undef CASE
+        default: return -3;
+        }
+        if (ret < -2) return ret;
+        if (info) *info = inf;
+        return ret;
+    }
     switch (s) {
 #define CASE(S_) case S_: ret = blcg_dispatch<S_>(t, A, B, X, tol, max_iter, o, &inf, info != nullptr); break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
