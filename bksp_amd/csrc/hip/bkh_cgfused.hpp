@@ -249,7 +249,7 @@ __global__ __launch_bounds__(512) void k_cgf_b(long long n, T *x, T *r, const T 
 }
 
 // End of a graph replay of K iterations: test of iteration base+K-1, base += K.
-__global__ __launch_bounds__(64) void k_cgf_tail(Ctl *c, int K, const double *partB, int npB) {
+[[maybe_unused]] static __global__ __launch_bounds__(64) void k_cgf_tail(Ctl *c, int K, const double *partB, int npB) {
     if (ctl_stopped(c)) return;
     const int k = c->base + K;
     const double rr = wave_sum_parts(partB, npB);
@@ -394,7 +394,7 @@ __global__ __launch_bounds__(256) void k_cg1_sell(int n, int nsl, int chunk, con
 }
 
 // End of a graph replay of K single-kernel iterations: test of iteration base+K.
-__global__ __launch_bounds__(256) void k_cg1_tail(Ctl *c, int K, const double *partPrev, int np) {
+[[maybe_unused]] static __global__ __launch_bounds__(256) void k_cg1_tail(Ctl *c, int K, const double *partPrev, int np) {
     if (ctl_stopped(c)) return;
     const int k = c->base + K;
     double ad, bd;

@@ -21,6 +21,7 @@
 // w_p = G^T z_p; dL/dg_ij = (2/(P T)) sum_p z_p[i] (A w_p)_j - [i == j] 2/(n g_ii).
 #include <math.h>
 #include <vector>
+#include "bkh_cgfused.hpp"
 #include "bkh_host.hpp"
 #include "bkh_solver_common.hpp"
 #include "bkh_spmv.hpp"
@@ -517,10 +518,24 @@ extern "C" bkh_fsai_field *bkh_fsai_field_create(int n, const int *rpl, const in
 }
 
 // G's values from the model (G and G^T, FP64 and FP32 copies); g_out (host) optional.
+// CSR-order values -> the SELL-64 copy (entry k of row r sits at so[r/64] + 64 k + r%64).
+__global__ __launch_bounds__(256) void k_csr_to_sell(int n, const int *rp, const int *so, const float *v32,
+                                                     float *sv32) {
+    for (int r = blockIdx.x * 256 + threadIdx.x; r < n; r += gridDim.x * 256) {
+        const int base = so[r >> 6] + (r & 63), k0 = rp[r], len = rp[r + 1] - k0;
+        for (int k = 0; k < len; k++) sv32[base + 64 * k] = v32[k0 + k];
+    }
+}
+
 static int fs_factor(ThreadCtx *t, const bkh_fsai_model *M, bkh_fsai_field *F) {
     hipLaunchKernelGGL((k_fsai_mlp<false>), dim3(fs_grid(F->nnzl)), dim3(256), 0, t->stream, F->n, F->nnzl, F->erow,
                        F->cil, F->vl, F->rows, M->params, F->perm, F->G->val, F->G->val32, F->G->T->val,
                        F->G->T->val32, (const double *)nullptr, (float *)nullptr);
+    // the fused PCG applies G and G^T through their SELL copies (FP32 values)
+    for (const bkh_matrix *Gm : {(const bkh_matrix *)F->G, (const bkh_matrix *)F->G->T})
+        if (Gm->nsl > 0 && Gm->sval32)
+            hipLaunchKernelGGL(k_csr_to_sell, dim3(vec_grid(F->n, 256, 1)), dim3(256), 0, t->stream, F->n, Gm->rp,
+                               Gm->sso, (const float *)Gm->val32, Gm->sval32);
     LAUNCH_OK();
     return 0;
 }
@@ -736,8 +751,15 @@ static void fs_apply(hipStream_t s, const bkh_matrix *G, const double *r, double
 // Preconditioned CG: q = A p, alpha = r.z / p.q, x += alpha p, r -= alpha q, stop on
 // ||r|| / ||b|| < tol (the index convention of cg.c:56-59), z = M^-1 r, beta = r.z'/r.z,
 // p = z + beta p.
+static int pcg_fused(ThreadCtx *t, const bkh_matrix *A, const bkh_matrix *G, int kind, const double *b, double *x,
+                     double tol, int max_iter, const bkh_opts &o, bkh_info *inf);
+
 static int pcg_run(ThreadCtx *t, const bkh_matrix *A, const bkh_matrix *G, int kind, const double *b, double *x,
                    double tol, int max_iter, const bkh_opts &o, bkh_info *inf) {
+    if (o.path != 1) {
+        const int rc = pcg_fused(t, A, G, kind, b, x, tol, max_iter, o, inf);
+        if (rc != 1) return rc;
+    }
     hipStream_t s = t->stream;
     const long long n = A->n;
     const int Gv = vec_grid(n, 256);
@@ -780,6 +802,322 @@ static int pcg_run(ThreadCtx *t, const bkh_matrix *A, const bkh_matrix *G, int k
         hipLaunchKernelGGL(k_pcg_p, dim3(Gv), dim3(256), 0, s, n, p, (const double *)z, (const Ctl *)c, 0);
     });
     if (!ge) { bkh_set_error("hipGraph capture of the PCG iteration failed"); return BKH_DEVICE_ERROR; }
+    int rc = run_graph_until_done(t, ge, c, max_iter / K + 2);
+    if (rc) return rc;
+    Ctl hc = *t->ctl_host;
+    HIP_OK(hipEventRecord(t->ev1, s));
+    HIP_OK(hipEventSynchronize(t->ev1));
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, t->ev0, t->ev1));
+    memset(inf, 0, sizeof *inf);
+    inf->time_ms = ms;
+    bkh_note_solve_ms(ms);
+    inf->ret = ctl_result(&hc);
+    inf->status = hc.status;
+    inf->iters = hc.result_it;
+    inf->rec_res = hc.err;
+    double tr = 0.0;
+    if (true_residual(t, A, 0.0, b, x, &tr) == 0) inf->true_res = tr;
+    return inf->ret;
+}
+
+// ------------------------------------------------ fused PCG (SELL-64 matrices)
+//
+// PCG in two (Jacobi, none) or three (FSAI) kernels per iteration and no finalize
+// kernels, on the pattern of the fused CG: every workgroup of a kernel sums the
+// previous kernel's partials itself in a fixed order.
+//   PA(k): convergence test of iteration k-1 on r_k.r_k (cg.c:56-59 convention),
+//          beta = (r_k.z_k)/(r_{k-1}.z_{k-1}); p_k = z_k + beta p_{k-1} formed in the
+//          gather; q = A p_k; partials p_k.q.
+//   PB(k): alpha = (r_k.z_k)/(p_k.q); x += alpha p_k; r_{k+1} = r_k - alpha q;
+//          Jacobi/none: z = D^-1 r_{k+1} (or r_{k+1}), partials {r.z, r.r};
+//          FSAI: y = G r_{k+1}, r_{k+1} formed in the gather from r_k and q.
+//   PD(k): (FSAI) z = G^T y, partials {r_{k+1}.z, r_{k+1}.r_{k+1}}.
+// r and p alternate between two buffers by the parity of k; r.z of iteration k lives
+// in Ctl::s[PF_RZ + (k & 1)]. All three matrices are streamed through their SELL-64
+// copies (lane per row, contiguous 64-entry loads); workgroup g owns `chunk` slices.
+enum { PF_RZ = 20 };
+
+// Workgroups of PF_BS threads (PF_BS / 64 waves, each taking every PF_W-th slice of
+// the workgroup's range): few, large workgroups keep the partial arrays that every
+// workgroup of the next kernel re-sums short (<= 512 entries).
+#define PF_BS 1024
+#define PF_W (PF_BS / 64)
+#define PF_NPART 512
+#define PF_SLICES(sl, row)                                                           \
+    const int g_ = xcd_block(blockIdx.x, gridDim.x);                                 \
+    const int w_ = threadIdx.x >> 6, l_ = threadIdx.x & 63;                          \
+    const int send_ = min(nsl, (g_ + 1) * chunk);                                    \
+    for (int sl = g_ * chunk + w_; sl < send_; sl += PF_W)                           \
+        for (int row = sl * 64 + l_; row < n; row = n)
+
+template <typename IT>
+__device__ __forceinline__ int sell_col(const IT *col, int e, int row) {
+    return sizeof(IT) == 2 ? row + (int)col[e] : (int)col[e];
+}
+
+// Scalars at the start of PA(k): false when the solve stops at k.
+__device__ __forceinline__ bool pf_beta(Ctl *c, int k, const double *partPrev, int np, bool writer, double &beta) {
+    if (k == c->first) {
+        beta = 0.0;
+        return true;
+    }
+    double d[2];
+    {
+        __shared__ double sh[2 * PF_W + 2];
+        double a0 = 0.0, a1 = 0.0;
+        for (int i = threadIdx.x; i < np; i += blockDim.x) { a0 += partPrev[2 * i]; a1 += partPrev[2 * i + 1]; }
+        a0 = wave_sum(a0);
+        a1 = wave_sum(a1);
+        const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+        if ((threadIdx.x & 63) == 0) { sh[w] = a0; sh[PF_W + w] = a1; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t0 = sh[0], t1 = sh[PF_W];
+            for (int q = 1; q < nw; q++) { t0 += sh[q]; t1 += sh[PF_W + q]; }
+            sh[2 * PF_W] = t0;
+            sh[2 * PF_W + 1] = t1;
+        }
+        __syncthreads();
+        d[0] = sh[2 * PF_W];
+        d[1] = sh[2 * PF_W + 1];
+    }
+    if (cgf_check(c, k, d[1], writer) != ST_RUNNING) return false;
+    const double rz_old = c->s[PF_RZ + ((k + 1) & 1)];
+    if (bad(rz_old)) {
+        if (writer) { c->status = ST_BREAKDOWN; c->result_it = k - 1; c->done = 1; }
+        return false;
+    }
+    if (writer) c->s[PF_RZ + (k & 1)] = d[0];
+    beta = d[0] / rz_old;
+    return true;
+}
+
+template <typename IT>
+__global__ __launch_bounds__(PF_BS) void k_pf_a(int n, int nsl, int chunk, const int *__restrict__ rp,
+                                              const int *__restrict__ so, const double *__restrict__ val,
+                                              const IT *__restrict__ col, const double *z, double *P0, double *P1,
+                                              double *q, Ctl *c, int j, const double *partPrev, int np,
+                                              double *partA) {
+    if (ctl_stopped(c)) return;
+    const int k = c->base + j;
+    const bool writer = blockIdx.x == 0 && threadIdx.x == 0;
+    double be;
+    if (!pf_beta(c, k, partPrev, np, writer, be)) return;
+    const double *pold = (k & 1) ? P1 : P0;
+    double *pnew = (k & 1) ? P0 : P1;
+    double dv[1] = {0.0};
+    PF_SLICES(sl, row) {
+        const int len = rp[row + 1] - rp[row], base = so[sl] + (row & 63);
+        double acc = 0.0;
+#pragma unroll 4
+        for (int kk = 0; kk < len; kk++) {
+            const int e = base + 64 * kk, cc = sell_col(col, e, row);
+            acc = fma(val[e], fma(be, pold[cc], z[cc]), acc);
+        }
+        q[row] = acc;
+        const double pn = fma(be, pold[row], z[row]);
+        pnew[row] = pn;
+        dv[0] += pn * acc;
+    }
+    block_partial<1, PF_BS>(dv, partA);
+}
+
+__device__ __forceinline__ bool pf_alpha(Ctl *c, int k, const double *partA, int npA, double &al) {
+    const double pq = block_sum_parts<PF_BS>(partA, npA);
+    if (bad(pq)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { c->status = ST_BREAKDOWN; c->result_it = k; c->done = 1; }
+        return false;
+    }
+    al = c->s[PF_RZ + (k & 1)] / pq;
+    return true;
+}
+
+// PB(k) for Jacobi (diag) or none (diag == nullptr).
+__global__ __launch_bounds__(PF_BS) void k_pf_b_vec(int n, int nsl, int chunk, double *x, double *R0, double *R1,
+                                                  const double *P0, const double *P1, const double *q, double *z,
+                                                  const double *diag, Ctl *c, int j, const double *partA, int npA,
+                                                  double *partB) {
+    if (ctl_stopped(c)) return;
+    const int k = c->base + j;
+    double al;
+    if (!pf_alpha(c, k, partA, npA, al)) return;
+    const double *p = (k & 1) ? P0 : P1, *rk = (k & 1) ? R1 : R0;
+    double *rn = (k & 1) ? R0 : R1;
+    double dv[2] = {0.0, 0.0};
+    PF_SLICES(sl, row) {
+        x[row] = fma(al, p[row], x[row]);
+        const double r = fma(-al, q[row], rk[row]);
+        rn[row] = r;
+        const double zz = diag ? r / diag[row] : r;
+        z[row] = zz;
+        dv[0] += r * zz;
+        dv[1] += r * r;
+    }
+    block_partial<2, PF_BS>(dv, partB);
+}
+
+// PB(k) for FSAI: x, r_{k+1} own; y = G r_{k+1} with r_{k+1}[j] = r_k[j] - alpha q[j].
+template <typename IT>
+__global__ __launch_bounds__(PF_BS) void k_pf_b_g(int n, int nsl, int chunk, const int *__restrict__ rp,
+                                                const int *__restrict__ so, const float *__restrict__ val,
+                                                const IT *__restrict__ col, double *x, double *R0, double *R1,
+                                                const double *P0, const double *P1, const double *q, double *y,
+                                                Ctl *c, int j, const double *partA, int npA) {
+    if (ctl_stopped(c)) return;
+    const int k = c->base + j;
+    double al;
+    if (!pf_alpha(c, k, partA, npA, al)) return;
+    const double *p = (k & 1) ? P0 : P1, *rk = (k & 1) ? R1 : R0;
+    double *rn = (k & 1) ? R0 : R1;
+    PF_SLICES(sl, row) {
+        const int len = rp[row + 1] - rp[row], base = so[sl] + (row & 63);
+        double acc = 0.0;
+#pragma unroll 4
+        for (int kk = 0; kk < len; kk++) {
+            const int e = base + 64 * kk, cc = sell_col(col, e, row);
+            acc = fma((double)val[e], fma(-al, q[cc], rk[cc]), acc);
+        }
+        y[row] = acc;
+        x[row] = fma(al, p[row], x[row]);
+        rn[row] = fma(-al, q[row], rk[row]);
+    }
+}
+
+// PD(k): z = G^T y; partials {r_{k+1}.z, r_{k+1}.r_{k+1}}.
+template <typename IT>
+__global__ __launch_bounds__(PF_BS) void k_pf_d(int n, int nsl, int chunk, const int *__restrict__ rp,
+                                              const int *__restrict__ so, const float *__restrict__ val,
+                                              const IT *__restrict__ col, const double *y, const double *R0,
+                                              const double *R1, double *z, Ctl *c, int j, double *partD) {
+    if (ctl_stopped(c)) return;
+    const int k = c->base + j;
+    const double *rn = (k & 1) ? R0 : R1;
+    double dv[2] = {0.0, 0.0};
+    PF_SLICES(sl, row) {
+        const int len = rp[row + 1] - rp[row], base = so[sl] + (row & 63);
+        double acc = 0.0;
+#pragma unroll 4
+        for (int kk = 0; kk < len; kk++) {
+            const int e = base + 64 * kk, cc = sell_col(col, e, row);
+            acc = fma((double)val[e], y[cc], acc);
+        }
+        z[row] = acc;
+        const double r = rn[row];
+        dv[0] += r * acc;
+        dv[1] += r * r;
+    }
+    block_partial<2, PF_BS>(dv, partD);
+}
+
+// End of a replay of K iterations: the test of iteration base + K.
+__global__ __launch_bounds__(PF_BS) void k_pf_tail(Ctl *c, int K, const double *partPrev, int np) {
+    if (ctl_stopped(c)) return;
+    const int k = c->base + K;
+    double be;
+    if (!pf_beta(c, k, partPrev, np, threadIdx.x == 0, be)) return;
+    if (threadIdx.x == 0) c->base = k;
+}
+
+// Start of the fused loop: r.z of iteration 0 from the unfused init (PC_RZ).
+__global__ void k_pf_start(Ctl *c) {
+    if (threadIdx.x == 0 && !c->done) {
+        c->s[PF_RZ] = c->s[PC_RZ];
+        c->base = 0;
+        c->first = 0;
+    }
+}
+
+static bool pf_ok(const bkh_matrix *M) {
+    return M && M->nsl > 0 && (M->variant == SPMV_SELL || M->variant == SPMV_SELL32 || M->sval32);
+}
+
+template <typename IT>
+static const IT *sell_cols(const bkh_matrix *M) {
+    if constexpr (sizeof(IT) == 2) return (const IT *)M->scd;
+    else return (const IT *)M->scol;
+}
+
+template <typename ITA, typename ITG, typename ITT>
+static void pf_capture_body(hipStream_t s, const bkh_matrix *A, const bkh_matrix *G, int kind, int K, int GA,
+                            int chunk, double *x, double *R, double *P, double *q, double *z, double *y, Ctl *c,
+                            double *partA, double *partB) {
+    const int n = A->n, nsl = A->nsl;
+    for (int j = 0; j < K; j++) {
+        hipLaunchKernelGGL((k_pf_a<ITA>), dim3(GA), dim3(PF_BS), 0, s, n, nsl, chunk, A->rp, A->sso,
+                           (const double *)A->sval, sell_cols<ITA>(A), (const double *)z, P, P + n, q, c, j,
+                           (const double *)partB, GA, partA);
+        if (kind == PCK_FSAI) {
+            hipLaunchKernelGGL((k_pf_b_g<ITG>), dim3(GA), dim3(PF_BS), 0, s, n, nsl, chunk, G->rp, G->sso,
+                               (const float *)G->sval32, sell_cols<ITG>(G), x, R, R + n, (const double *)P,
+                               (const double *)(P + n), (const double *)q, y, c, j, (const double *)partA, GA);
+            const bkh_matrix *Gt = G->T;
+            hipLaunchKernelGGL((k_pf_d<ITT>), dim3(GA), dim3(PF_BS), 0, s, n, nsl, chunk, Gt->rp, Gt->sso,
+                               (const float *)Gt->sval32, sell_cols<ITT>(Gt), (const double *)y, (const double *)R,
+                               (const double *)(R + n), z, c, j, partB);
+        } else {
+            hipLaunchKernelGGL(k_pf_b_vec, dim3(GA), dim3(PF_BS), 0, s, n, nsl, chunk, x, R, R + n, (const double *)P,
+                               (const double *)(P + n), (const double *)q, z,
+                               kind == PCK_JACOBI ? (const double *)A->diag : (const double *)nullptr, c, j,
+                               (const double *)partA, GA, partB);
+        }
+    }
+    hipLaunchKernelGGL(k_pf_tail, dim3(1), dim3(PF_BS), 0, s, c, K, (const double *)partB, GA);
+}
+
+// Fused PCG; returns 1 when the matrices are not all on the SELL layout (the caller
+// then runs the unfused loop).
+static int pcg_fused(ThreadCtx *t, const bkh_matrix *A, const bkh_matrix *G, int kind, const double *b, double *x,
+                     double tol, int max_iter, const bkh_opts &o, bkh_info *inf) {
+    if (!pf_ok(A) || !A->sval || (kind == PCK_FSAI && (!pf_ok(G) || !G->sval32 || !pf_ok(G->T) || !G->T->sval32)))
+        return 1;
+    if (kind == PCK_FSAI && (G->nsl != A->nsl || G->T->nsl != A->nsl)) return 1;
+    hipStream_t s = t->stream;
+    const long long n = A->n;
+    const int Gv = vec_grid(n, 256);
+    const int GA = std::max(1, std::min((A->nsl + PF_W - 1) / PF_W, PF_NPART));
+    const int chunk = (A->nsl + GA - 1) / GA;
+    int nparts = std::max(spmv_grid(A), Gv);
+    if (G) nparts = std::max(nparts, std::max(spmv_grid(G), spmv_grid(G->T)));
+    nparts *= 2;
+    void *ws = bkh_workspace(carve_size(sizeof(Ctl)) + carve_size(sizeof(double) * nparts) +
+                             2 * carve_size(sizeof(double) * 2 * PF_NPART) + 8 * carve_size(sizeof(double) * n));
+    if (!ws) return BKH_DEVICE_ERROR;
+    Carve cv(ws);
+    Ctl *c = cv.take<Ctl>(1);
+    double *part = cv.take<double>(nparts);
+    double *partA = cv.take<double>(2 * PF_NPART), *partB = cv.take<double>(2 * PF_NPART);
+    double *R = cv.take<double>(2 * n), *P = cv.take<double>(2 * n), *q = cv.take<double>(n), *z = cv.take<double>(n),
+           *y = cv.take<double>(n);
+    int K = o.graph_iters > 0 ? o.graph_iters : default_graph_iters(n);
+    HIP_OK(hipEventRecord(t->ev0, s));
+    if (ctl_init(t, c, tol, max_iter)) return BKH_DEVICE_ERROR;
+    Epi<0> e0;
+    memset(&e0, 0, sizeof e0);
+    // r_0 = b - A x_0 (buffer 0), z_0, r_0.z_0 (the unfused init), then the loop
+    launch_spmv<double, double, 0>(s, A, A->val, x, R, 1.0, 0.0, e0, nullptr, nullptr, NoFin());
+    hipLaunchKernelGGL(k_pcg_init, dim3(Gv), dim3(256), 0, s, n, b, R, z, (const double *)A->diag, kind, part);
+    launch_finalize<2>(s, part, Gv, c, PcInitFin{(int)(kind == PCK_FSAI)});
+    if (kind == PCK_FSAI) fs_apply(s, G, R, y, z, c, part, PcRzFin{1});
+    // P0 = z_0 (finite; the first PA forms p_0 = z_0 + 0 * P0)
+    hipLaunchKernelGGL(k_pcg_p, dim3(Gv), dim3(256), 0, s, n, P, (const double *)z, (const Ctl *)c, 1);
+    hipLaunchKernelGGL(k_pf_start, dim3(1), dim3(64), 0, s, c);
+    LAUNCH_OK();
+    const bool a16 = A->scd != nullptr;
+    const bool g16 = kind == PCK_FSAI && G->scd != nullptr, t16 = kind == PCK_FSAI && G->T->scd != nullptr;
+    hipGraphExec_t ge = capture_cached(t, {0xF5A2, (uintptr_t)A, (uintptr_t)G, (uintptr_t)kind, (uintptr_t)x,
+                                           (uintptr_t)R, (uintptr_t)P, (uintptr_t)q, (uintptr_t)z, (uintptr_t)y,
+                                           (uintptr_t)c, (uintptr_t)partA, (uintptr_t)partB, (uintptr_t)K}, s, 1, [&]() {
+        if (a16 && (kind != PCK_FSAI || (g16 && t16)))
+            pf_capture_body<short, short, short>(s, A, G, kind, K, GA, chunk, x, R, P, q, z, y, c, partA, partB);
+        else if (!a16 && (kind != PCK_FSAI || (!g16 && !t16)))
+            pf_capture_body<int, int, int>(s, A, G, kind, K, GA, chunk, x, R, P, q, z, y, c, partA, partB);
+        else if (a16)
+            pf_capture_body<short, int, int>(s, A, G, kind, K, GA, chunk, x, R, P, q, z, y, c, partA, partB);
+        else
+            pf_capture_body<int, short, short>(s, A, G, kind, K, GA, chunk, x, R, P, q, z, y, c, partA, partB);
+    });
+    if (!ge) { bkh_set_error("hipGraph capture of the fused PCG iteration failed"); return BKH_DEVICE_ERROR; }
     int rc = run_graph_until_done(t, ge, c, max_iter / K + 2);
     if (rc) return rc;
     Ctl hc = *t->ctl_host;

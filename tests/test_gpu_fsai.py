@@ -88,3 +88,30 @@ def test_training_reduces_loss_and_beats_jacobi_on_held_out_field():
     print(res)
     assert res["learned"]["true_res"] < 1e-8 and res["jacobi"]["true_res"] < 1e-8
     assert res["learned"]["iters"] < 0.9 * res["jacobi"]["iters"]
+
+
+@pytest.mark.parametrize("kind", ["none", "jacobi", "fsai"])
+def test_fused_pcg_matches_unfused(kind):
+    """The fused PCG (two or three kernels per iteration, SELL matrices) against the
+    unfused loop (path 1) on the same field: iteration indices within 2 % and the
+    same solution to the tolerance's accuracy; both meet tol in the FP64 true residual."""
+    n, rp, ci, v, rpl, cil, vl = small_field(61, 20)
+    F = dv.FsaiField(n, rpl, cil, vl)
+    b = gen.normal(505, n)
+    db = dv.DeviceArray.from_numpy(b)
+    out = []
+    for path in (0, 1):
+        dx = dv.DeviceArray.from_numpy(np.zeros(n))
+        if kind == "fsai":
+            M = dv.FsaiModel(fsai.init_params(3) * 2)
+            F.factor(M)
+            info = F.pcg(db, dx, 1e-9, 100000, path=path)
+        else:
+            info = F.pcg_plain(0 if kind == "none" else 1, db, dx, 1e-9, 100000, path=path)
+        out.append((info, dx.numpy()))
+    (i0, x0), (i1, x1) = out
+    print(f"{kind}: fused {i0.ret} iters {i0.time_ms:.2f} ms, unfused {i1.ret} iters {i1.time_ms:.2f} ms")
+    assert i0.ret >= 0 and i1.ret >= 0
+    assert abs(i0.ret - i1.ret) <= max(1, int(0.02 * i1.ret)), (i0.ret, i1.ret)
+    assert i0.true_res < 1.5e-9 and i1.true_res < 1.5e-9
+    assert np.linalg.norm(x0 - x1) / np.linalg.norm(x1) < 1e-6
