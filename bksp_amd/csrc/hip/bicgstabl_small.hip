@@ -10,6 +10,7 @@
 // Vectors live in global memory (L2-resident at these sizes). The loop is bounded
 // by max_iter, so the kernel always terminates.
 #include <math.h>
+#include <stdlib.h>
 #include "bkh_host.hpp"
 #include "bkh_mr.hpp"
 #include "bkh_solver_common.hpp"
@@ -185,6 +186,300 @@ __global__ __launch_bounds__(SMALL_BS) void k_bicgstabl_small(int n, const int *
     }
 }
 
+// ------------------------------------------------ register-resident variant
+//
+// For n <= RPT * 1024 with few Krylov vectors: thread t owns rows t, t + 1024, ...
+// and keeps ITS entries of every Krylov vector (r_0..r_l, p_0..p_l, rs, x) in
+// registers for the whole solve. An SpMV stages its input vector in LDS (two buffers
+// used alternately, so one barrier per SpMV suffices: a buffer is rewritten only after
+// an intervening barrier has ordered every read of it) and each thread gathers its
+// rows from LDS in the reference's order; the matrix streams from L2. Dot products
+// are wave shuffles plus one LDS exchange (again double-buffered, one barrier). The
+// arithmetic of every element is that of k_bicgstabl_small.
+
+template <int NV, int BS>
+__device__ __forceinline__ void reg_sum(double (&v)[NV], double *sh) {
+    constexpr int REG_NW = BS / 64;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; i++) v[i] = wave_sum(v[i]);
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; i++) sh[i * REG_NW + w] = v[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+        double t = sh[i * REG_NW];
+#pragma unroll
+        for (int k = 1; k < REG_NW; k++) t += sh[i * REG_NW + k];
+        v[i] = t;
+    }
+}
+
+// Row r's entries: CSR [rb, re), or on the SELL-64 copy (so != null) entry k at
+// so[r / 64] + 64 k + r % 64 -- contiguous across a wave. The first REG_W entries of
+// every row are loaded together (predicated), then summed in stored order.
+#define REG_W 6
+template <int RPT, int REG_BS>
+__device__ __forceinline__ void reg_spmv(int n, const int (&rb)[RPT], const int (&re)[RPT], const int *ci,
+                                         const double *val, const int *so, const double *sval, const int *scol,
+                                         const double (&in)[RPT], double (&out)[RPT], double *xs) {
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+        const int i = threadIdx.x + q * REG_BS;
+        if (i < n) xs[i] = in[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+        const int i = threadIdx.x + q * REG_BS;
+        const int len = re[q] - rb[q];
+        double a[REG_W];
+        int cc[REG_W];
+        if (so) {
+            const int base = i < n ? so[i >> 6] + (i & 63) : 0;
+#pragma unroll
+            for (int k = 0; k < REG_W; k++) {
+                a[k] = k < len ? sval[base + 64 * k] : 0.0;
+                cc[k] = k < len ? scol[base + 64 * k] : 0;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < REG_W; k++) {
+                a[k] = k < len ? val[rb[q] + k] : 0.0;
+                cc[k] = k < len ? ci[rb[q] + k] : 0;
+            }
+        }
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < REG_W; k++)
+            if (k < len) s = add_rn(s, mul_rn(a[k], xs[cc[k]]));
+        for (int k = REG_W; k < len; k++) {
+            const int e = so ? so[i >> 6] + (i & 63) + 64 * k : rb[q] + k;
+            s = add_rn(s, mul_rn(so ? sval[e] : val[e], xs[so ? scol[e] : ci[e]]));
+        }
+        out[q] = s;
+    }
+}
+
+template <int L, int RPT, int REG_BS>
+__global__ __launch_bounds__(REG_BS) void k_bicgstabl_reg(int n, const int *rp, const int *ci, const double *val,
+                                                          const int *so, const double *sval, const int *scol,
+                                                          const double *b, double *xg, Ctl *c) {
+    constexpr int NG = (L + 1) * (L + 2) / 2;
+    constexpr int REG_NW = REG_BS / 64;
+    __shared__ double xs[2][RPT * REG_BS];
+    __shared__ double shr[2][REG_NW * (NG > 2 ? NG : 2)];
+    __shared__ double coef[3 * MR_D + 4];
+    __shared__ int flags[2];
+    __shared__ MrOut o;                                    // thread 0's MR-step workspace
+    __shared__ double Gs[MR_D][MR_D];
+    // rs and x are touched once per step: they live in LDS (each thread only ever
+    // reads its own slots, so no barrier guards them), the other vectors in registers
+    __shared__ double rs_s[RPT * REG_BS], x_s[RPT * REG_BS];
+    const int tid = threadIdx.x;
+    int rb[RPT], re[RPT];
+    double R[L + 1][RPT], P[L + 1][RPT];
+#define RS(q) rs_s[(q) * REG_BS + tid]
+#define XX(q) x_s[(q) * REG_BS + tid]
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+        const int i = tid + q * REG_BS;
+        const bool live = i < n;
+        rb[q] = live ? rp[i] : 0;
+        re[q] = live ? rp[i + 1] : 0;
+        XX(q) = live ? xg[i] : 0.0;
+    }
+    int sx = 0, sr = 0;                                   // alternating LDS buffers
+    const int max_it = c->max_it;
+    const double tol = c->tol;
+    // r0 = b - A x0; p0 = rs = r0 (bicgstabl.c:44-51)
+    {
+        double ax[RPT], x0[RPT];
+#pragma unroll
+        for (int q = 0; q < RPT; q++) x0[q] = XX(q);
+        reg_spmv<RPT, REG_BS>(n, rb, re, ci, val, so, sval, scol, x0, ax, xs[sx]);
+        sx ^= 1;
+        double v2[2] = {0.0, 0.0};
+#pragma unroll
+        for (int q = 0; q < RPT; q++) {
+            const int i = tid + q * REG_BS;
+            const double bi = i < n ? b[i] : 0.0;
+            const double ri = i < n ? bi - ax[q] : 0.0;
+            R[0][q] = ri; P[0][q] = ri; RS(q) = ri;
+            v2[0] += ri * ri;
+            v2[1] += bi * bi;
+        }
+        reg_sum<2, REG_BS>(v2, shr[sr]);
+        sr ^= 1;
+        double fnb = sqrt(v2[1]);
+        double alpha = 0.0, beta = 0.0, g0 = v2[0], oml = 1.0, err = 0.0;
+        int status = ST_MAXIT, it = 0, result = 0;
+        int mit = max_it;
+        if (!(fnb > 0.0)) { status = ST_BREAKDOWN; mit = 0; }
+        else if (v2[0] == 0.0) { status = ST_CONVERGED; result = 0; }
+        for (it = 0; it < mit && status != ST_CONVERGED; it++) {
+            bool stop = false;
+#pragma unroll
+            for (int j = 0; j < L; j++) {
+                if (!stop) {
+#pragma unroll
+                    for (int q = 0; q < RPT; q++)               // (:64-67)
+#pragma unroll
+                        for (int a = 0; a <= j; a++) P[a][q] = add_rn(mul_rn(beta, P[a][q]), R[a][q]);
+                    reg_spmv<RPT, REG_BS>(n, rb, re, ci, val, so, sval, scol, P[j], P[j + 1], xs[sx]);   // (:68-69)
+                    sx ^= 1;
+                    double d1[1] = {0.0};
+#pragma unroll
+                    for (int q = 0; q < RPT; q++) d1[0] += RS(q) * P[j + 1][q];
+                    reg_sum<1, REG_BS>(d1, shr[sr]);
+                    sr ^= 1;
+                    if (bad(d1[0])) {
+                        status = ST_BREAKDOWN;
+                        stop = true;
+                    } else {
+                        alpha = g0 / d1[0];                            // (:71)
+#pragma unroll
+                        for (int q = 0; q < RPT; q++) {                // (:73-75, :79)
+#pragma unroll
+                            for (int a = 0; a <= j; a++) R[a][q] = fma(-alpha, P[a + 1][q], R[a][q]);
+                            XX(q) = fma(alpha, P[0][q], XX(q));
+                        }
+                        reg_spmv<RPT, REG_BS>(n, rb, re, ci, val, so, sval, scol, R[j], R[j + 1], xs[sx]);   // (:76-77)
+                        sx ^= 1;
+                        if (j < L - 1) {
+                            double d2[1] = {0.0};
+#pragma unroll
+                            for (int q = 0; q < RPT; q++) d2[0] += RS(q) * R[j + 1][q];
+                            reg_sum<1, REG_BS>(d2, shr[sr]);
+                            sr ^= 1;
+                            if (bad(g0)) {
+                                status = ST_BREAKDOWN;
+                                stop = true;
+                            } else {
+                                beta = -alpha * d2[0] / g0;            // (:60-62)
+                                g0 = d2[0];
+                            }
+                        }
+                    }
+                }
+            }
+            if (stop) break;
+            // MR step from the Gram matrix of r_0..r_L (:84-109)
+            double gv[NG];
+#pragma unroll
+            for (int a = 0; a < NG; a++) gv[a] = 0.0;
+#pragma unroll
+            for (int q = 0; q < RPT; q++) {
+                int idx = 0;
+#pragma unroll
+                for (int a = 0; a <= L; a++)
+#pragma unroll
+                    for (int bb = a; bb <= L; bb++) gv[idx++] += R[a][q] * R[bb][q];
+            }
+            reg_sum<NG, REG_BS>(gv, shr[sr]);
+            sr ^= 1;
+            if (tid == 0) {
+                gram_unpack(L, gv, Gs);
+                flags[0] = 0;
+                if (Gs[0][0] == 0.0) {
+                    flags[0] = 1; flags[1] = ST_CONVERGED;
+                } else if (!mr_from_gram(L, Gs, o)) {
+                    flags[0] = 1; flags[1] = ST_BREAKDOWN;
+                } else {
+                    coef[0] = o.om[1];
+                    for (int k = 1; k < L; k++) {
+                        double t = 0.0;
+                        for (int i = k; i < L; i++) t += o.odd[i] * o.C[i][k];
+                        coef[k] = t;
+                    }
+                    for (int k = 1; k <= L; k++) {
+                        double t = 0.0;
+                        for (int i = k; i <= L; i++) t += o.od[i] * o.C[i][k];
+                        coef[MR_D + k] = t;
+                        coef[2 * MR_D + k] = o.om[k];
+                    }
+                    coef[3 * MR_D] = o.om[L];
+                }
+            }
+            __syncthreads();
+            if (flags[0]) {
+                status = flags[1];
+                result = it;
+                break;
+            }
+            double v3[2] = {0.0, 0.0};
+#pragma unroll
+            for (int q = 0; q < RPT; q++) {                             // (:113-120)
+                double xi = XX(q), r0 = R[0][q], p0 = P[0][q];
+                xi += coef[0] * r0;
+#pragma unroll
+                for (int k = 1; k <= L; k++) {
+                    const double rk = R[k][q];
+                    if (k < L) xi += coef[k] * rk;
+                    r0 -= coef[MR_D + k] * rk;
+                    p0 -= coef[2 * MR_D + k] * P[k][q];
+                }
+                XX(q) = xi; R[0][q] = r0; P[0][q] = p0;
+                v3[0] += r0 * r0;
+                v3[1] += RS(q) * r0;
+            }
+            oml = coef[3 * MR_D];
+            reg_sum<2, REG_BS>(v3, shr[sr]);
+            sr ^= 1;
+            err = sqrt(v3[0]) / fnb;                                    // (:122-125)
+            if (err < tol) { status = ST_CONVERGED; result = it; break; }
+            if (!(err == err) || err > 1e300) { status = ST_BREAKDOWN; result = it; break; }
+            const double gg = -oml * g0;                                // (:57-62)
+            if (bad(gg)) { status = ST_BREAKDOWN; result = it; break; }
+            beta = -alpha * v3[1] / gg;
+            g0 = v3[1];
+        }
+#pragma unroll
+        for (int q = 0; q < RPT; q++) {
+            const int i = tid + q * REG_BS;
+            if (i < n) xg[i] = XX(q);
+        }
+        if (tid == 0) {
+            c->fnb = fnb;
+            c->err = err;
+            c->status = status;
+            c->result_it = (status == ST_MAXIT) ? (it > 0 ? it - 1 : 0) : result;
+            c->it = it;
+            c->done = 1;
+        }
+    }
+}
+#undef RS
+#undef XX
+
+// Launch shapes that hold the vectors without register spills (from the compiler's
+// resource report): 1024 threads while a thread's share is small, else 512 threads
+// with up to 256 registers per lane. Larger systems take the global-memory kernel.
+template <int L>
+static bool reg_launch(hipStream_t s, int n, const bkh_matrix *A, const double *b, double *x, Ctl *c) {
+    const bool sell = A->nsl > 0 && A->sval && A->scol;
+    const int *so = sell ? A->sso : nullptr;
+    const double *sv = sell ? A->sval : nullptr;
+    const int *sc = sell ? A->scol : nullptr;
+#define REG_GO(RPT_, BS_) \
+    { hipLaunchKernelGGL((k_bicgstabl_reg<L, RPT_, BS_>), dim3(1), dim3(BS_), 0, s, n, A->rp, A->ci, A->val, so, sv, sc, \
+                         b, x, c); return true; }
+    if constexpr (L <= 2) {
+        if (n <= 1024) REG_GO(1, 1024)
+        if (n <= 2048) REG_GO(2, 1024)
+        if (n <= 4096) REG_GO(8, 512)
+    } else if constexpr (L == 3) {
+        if (n <= 1024) REG_GO(1, 1024)
+        if (n <= 2048) REG_GO(4, 512)
+    } else {
+        if (n <= 1024) REG_GO(2, 512)
+    }
+#undef REG_GO
+    return false;
+}
+
 // Returns true (and the solve's result in t->ctl_host) if the small path ran.
 bool bicgstabl_small(ThreadCtx *t, const bkh_matrix *A, int l, const double *b, double *x, double tol,
                      int max_iter, int *rc) {
@@ -198,7 +493,17 @@ bool bicgstabl_small(ThreadCtx *t, const bkh_matrix *A, int l, const double *b, 
     hipStream_t s = t->stream;
     *rc = ctl_init(t, c, tol, max_iter);
     if (*rc) return true;
-    switch (l) {
+    bool reg = false;
+    if (!getenv("BKH_SMALL_GLOBAL")) {
+        switch (l) {
+        case 1: reg = reg_launch<1>(s, (int)n, A, b, x, c); break;
+        case 2: reg = reg_launch<2>(s, (int)n, A, b, x, c); break;
+        case 3: reg = reg_launch<3>(s, (int)n, A, b, x, c); break;
+        case 4: reg = reg_launch<4>(s, (int)n, A, b, x, c); break;
+        default: break;
+        }
+    }
+    if (!reg) switch (l) {
 #define CASE(L_) case L_: hipLaunchKernelGGL((k_bicgstabl_small<L_>), dim3(1), dim3(SMALL_BS), 0, s, (int)n, \
                      (const int *)A->rp, (const int *)A->ci, (const double *)A->val, b, x, R, P, rs, c); break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)

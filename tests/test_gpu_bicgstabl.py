@@ -144,3 +144,31 @@ def test_cfg1_both_paths(path):
         dv.bicgstabl(M, db, 2, dx, 1e-300, k, path=path)
         _, ck = po.bicgstabl(n, rp, ci, v, b, 2, 1e-300, k)
         assert rel(dx.numpy(), ck) < 1e-10, (path, k)
+
+
+@pytest.mark.parametrize("l", [1, 2])
+def test_register_kernel_matches_global_memory_kernel(monkeypatch, l):
+    """The single-workgroup path keeps the Krylov vectors in registers for these
+    sizes; BKH_SMALL_GLOBAL=1 selects its global-memory twin. Same iterates (the
+    dot products are summed in a different order: 1e-12), same convergence index."""
+    n, rp, ci, v, xt, b = cfg1()
+    M = dv.Matrix(n, rp, ci, v)
+    db = dv.DeviceArray.from_numpy(b)
+    res = {}
+    for mode in ("reg", "global"):
+        if mode == "global":
+            monkeypatch.setenv("BKH_SMALL_GLOBAL", "1")
+        xs = []
+        for k in (1, 5):
+            dx = dv.DeviceArray.from_numpy(np.zeros(n))
+            dv.bicgstabl(M, db, l, dx, 1e-300, k, path=2)
+            xs.append(dx.numpy())
+        dx = dv.DeviceArray.from_numpy(np.zeros(n))
+        info = dv.bicgstabl(M, db, l, dx, 1e-10, 2000, path=2)
+        res[mode] = (xs, info)
+    (xr, ir), (xg, ig) = res["reg"], res["global"]
+    for a, c in zip(xr, xg):
+        assert rel(a, c) < 1e-12
+    assert ir.ret >= 0 and abs(ir.ret - ig.ret) <= 1 and ir.true_res < 2e-10
+    print(f"l={l}: register kernel {ir.ret} outer, {ir.time_ms:.3f} ms; global-memory kernel {ig.ret}, "
+          f"{ig.time_ms:.3f} ms")
