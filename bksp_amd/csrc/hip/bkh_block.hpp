@@ -189,3 +189,173 @@ static inline int block_grid(long long m) {
     if (g < 1) g = 1;
     return (int)g;
 }
+
+// ------------------------------------------------ LDS-staged 16-row tiles
+//
+// A 16-row tile of a row-major m x S block is one contiguous run of 16 S elements.
+// The staged helpers move it between global memory and a per-wave LDS tile (16 rows,
+// stride TS_LD doubles) with 16-byte vectors on every lane, and the MFMA operands and
+// results between LDS and registers -- instead of the 4- or 8-byte per-lane accesses
+// in MFMA layout (with S < 16 lanes idle) of tile_load / tile_mul / tile_store.
+// Values are converted to FP64 on the way in and rounded to T on the way out.
+#define TS_LD 17
+
+// Orders this wave's LDS writes before its other lanes' reads (and reads before a
+// later overwrite) -- no workgroup barrier needed for a wave-private tile.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int S, typename T>
+__device__ __forceinline__ void stage_fetch(const T *X, long long m, long long row0, double *tb) {
+    constexpr int V = 16 / sizeof(T);
+    constexpr int NV = (16 * S + V - 1) / V;          // vectors in the tile
+    typedef T tvec __attribute__((ext_vector_type(V)));
+    const int L = threadIdx.x & 63;
+    const long long e0 = row0 * S, emax = m * S;
+#pragma unroll
+    for (int pass = 0; pass * 64 < NV; pass++) {
+        const int lv = L + pass * 64;
+        if (lv < NV) {
+            const long long e = e0 + (long long)lv * V;
+            tvec v;
+            if (e + V <= emax) {
+                v = *(const tvec *)(X + e);
+            } else {
+#pragma unroll
+                for (int u = 0; u < V; u++) v[u] = (e + u < emax) ? X[e + u] : (T)0;
+            }
+#pragma unroll
+            for (int u = 0; u < V; u++) {
+                const int le = lv * V + u;
+                tb[(le / S) * TS_LD + le % S] = (double)v[u];
+            }
+        }
+    }
+}
+
+template <int S, typename T>
+__device__ __forceinline__ void stage_store(T *X, long long m, long long row0, const double *tb) {
+    constexpr int V = 16 / sizeof(T);
+    constexpr int NV = (16 * S + V - 1) / V;
+    typedef T tvec __attribute__((ext_vector_type(V)));
+    const int L = threadIdx.x & 63;
+    const long long e0 = row0 * S, emax = m * S;
+#pragma unroll
+    for (int pass = 0; pass * 64 < NV; pass++) {
+        const int lv = L + pass * 64;
+        if (lv < NV) {
+            const long long e = e0 + (long long)lv * V;
+            tvec v;
+#pragma unroll
+            for (int u = 0; u < V; u++) {
+                const int le = lv * V + u;
+                v[u] = (T)tb[(le / S) * TS_LD + le % S];
+            }
+            if (e + V <= emax) {
+                *(tvec *)(X + e) = v;
+            } else {
+#pragma unroll
+                for (int u = 0; u < V; u++)
+                    if (e + u < emax) X[e + u] = v[u];
+            }
+        }
+    }
+}
+
+// D-layout registers (lane L register r: row L/16 + 4r, column L%16) from / to the tile.
+template <int S>
+__device__ __forceinline__ mf4 ts_load_d(const double *tb) {
+    const int L = threadIdx.x & 63, j = L & 15;
+    mf4 v;
+#pragma unroll
+    for (int r = 0; r < 4; r++) v[r] = (j < S) ? tb[((L >> 4) + 4 * r) * TS_LD + j] : 0.0;
+    return v;
+}
+
+template <int S>
+__device__ __forceinline__ void ts_put_d(double *tb, mf4 v) {
+    const int L = threadIdx.x & 63, j = L & 15;
+    if (j < S) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) tb[((L >> 4) + 4 * r) * TS_LD + j] = v[r];
+    }
+}
+
+// D = tile * M (M s x s column-major in Ms, zero beyond s), A operands from the tile.
+template <int S>
+__device__ __forceinline__ mf4 ts_mul(const double *tb, const double *Ms) {
+    const int L = threadIdx.x & 63;
+    mf4 d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < (S + 3) / 4; q++) {
+        const int k = 4 * q + (L >> 4), j = L & 15;
+        const double a = (k < S) ? tb[(L & 15) * TS_LD + k] : 0.0;
+        const double bm = (k < S && j < S) ? Ms[k + j * S] : 0.0;
+        d = mfma16x16x4(a, bm, d);
+    }
+    return d;
+}
+
+// Rounds D-layout values as a store to T would (the Gram of what is stored).
+template <typename T>
+__device__ __forceinline__ mf4 as_stored(mf4 v) {
+    if constexpr (sizeof(T) != sizeof(double)) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) v[r] = (double)(T)v[r];
+    }
+    return v;
+}
+
+// G = X^T Y with staged tiles (same partial layout as k_gram).
+template <int S, typename T>
+__global__ __launch_bounds__(256) void k_gram_st(long long m, const T *X, const T *Y, const Ctl *ctl, double *part) {
+    if (ctl && ctl_stopped(ctl)) return;
+    __shared__ double sh[4 * 256];
+    __shared__ double tbs[4][2][16 * TS_LD];
+    const int w = threadIdx.x >> 6;
+    double *t0 = tbs[w][0], *t1 = tbs[w][1];
+    mf4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (long long row0 = ((long long)blockIdx.x * 4 + w) * 16; row0 < m; row0 += (long long)gridDim.x * 64) {
+        wave_lds_sync();
+        stage_fetch<S>(X, m, row0, t0);
+        if (X != Y) stage_fetch<S>(Y, m, row0, t1);
+        wave_lds_sync();
+        const mf4 a = ts_load_d<S>(t0);
+        const mf4 b = (X == Y) ? a : ts_load_d<S>(t1);
+        acc = tile_gram(a, b, acc);
+    }
+    gram_block_partial<S>(acc, sh, part);
+}
+
+// ------------------------------------------------------------ SELL SpMM
+
+// Y = alpha A X + beta Y on the SELL-64 copy of A: a lane owns a row and sums its
+// entries in stored order (the CSR-stream kernel's arithmetic), reading the S values
+// of each gathered row of X as contiguous vectors.
+template <int S, typename VT, typename T, typename IT>
+__global__ __launch_bounds__(256) void k_spmm_sell(int n, const int *__restrict__ rp, const int *__restrict__ so,
+                                                   const VT *__restrict__ val, const IT *__restrict__ col,
+                                                   const T *X, T *Y, double alpha, double beta, const Ctl *ctl) {
+    if (ctl && ctl_stopped(ctl)) return;
+    const int row = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (row >= n) return;
+    const int len = rp[row + 1] - rp[row];
+    const int base = so[row >> 6] + (row & 63);
+    const bool a1 = (alpha == 1.0);
+    T acc[S];
+#pragma unroll
+    for (int j = 0; j < S; j++) acc[j] = (beta != 0.0) ? mul_rn((T)beta, Y[(size_t)row * S + j]) : (T)0;
+    for (int kk = 0; kk < len; kk++) {
+        const int e = base + 64 * kk;
+        const int c = sizeof(IT) == 2 ? row + (int)col[e] : (int)col[e];
+        const T a = a1 ? (T)val[e] : (T)(alpha * (double)val[e]);
+        const T *xr = X + (size_t)c * S;
+#pragma unroll
+        for (int j = 0; j < S; j++) acc[j] = add_rn(acc[j], mul_rn(a, xr[j]));
+    }
+#pragma unroll
+    for (int j = 0; j < S; j++) Y[(size_t)row * S + j] = acc[j];
+}

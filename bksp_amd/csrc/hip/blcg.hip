@@ -131,25 +131,44 @@ __device__ __forceinline__ void load_small(double *dst, const double *src, int n
     for (int t = threadIdx.x; t < n; t += blockDim.x) dst[t] = src[t];
 }
 
+// Panel kernels: each wave streams 16-row tiles through its LDS tiles (stage_fetch /
+// stage_store: 16-byte vectors on every lane), multiplies on FP64 MFMA and, where the
+// next step needs it, accumulates the Gram matrix of the stored result.
+#define BLK_TILES __shared__ double tbs[4][2][16 * TS_LD]; \
+    double *t0 = tbs[threadIdx.x >> 6][0], *t1 = tbs[threadIdx.x >> 6][1]
+
 // X += P W; V -= AP alpha; Gram V^T V of the new V.
 template <int S, typename T>
 __global__ __launch_bounds__(256) void k_blk_xv(long long m, T *X, const T *P, T *V, const T *AP,
                                                 const BlkMats *mt, const Ctl *c, double *part) {
     if (ctl_stopped(c)) return;
     __shared__ double Wm[SM_N], Am[SM_N], sh[4 * 256];
+    BLK_TILES;
     load_small(Wm, mt->W, S * S);
     load_small(Am, mt->alpha, S * S);
     __syncthreads();
     const int w = threadIdx.x >> 6;
     mf4 acc = {0.0, 0.0, 0.0, 0.0};
     for (long long row0 = ((long long)blockIdx.x * 4 + w) * 16; row0 < m; row0 += (long long)gridDim.x * 64) {
-        mf4 dx = tile_mul<S>(P, m, row0, Wm);
-        mf4 xt = tile_load<S>(X, m, row0);
-        tile_store<S>(X, m, row0, xt + dx);
-        mf4 dv = tile_mul<S>(AP, m, row0, Am);
-        mf4 vt = tile_load<S>(V, m, row0) - dv;
-        tile_store<S>(V, m, row0, vt);
-        if constexpr (sizeof(T) != sizeof(double)) vt = tile_load<S>(V, m, row0);   // stored values
+        wave_lds_sync();
+        stage_fetch<S>(P, m, row0, t0);
+        stage_fetch<S>(X, m, row0, t1);
+        wave_lds_sync();
+        const mf4 xt = ts_load_d<S>(t1) + ts_mul<S>(t0, Wm);
+        wave_lds_sync();
+        ts_put_d<S>(t1, xt);
+        stage_fetch<S>(AP, m, row0, t0);
+        wave_lds_sync();
+        stage_store<S>(X, m, row0, t1);
+        const mf4 dv = ts_mul<S>(t0, Am);
+        wave_lds_sync();
+        stage_fetch<S>(V, m, row0, t1);
+        wave_lds_sync();
+        const mf4 vt = as_stored<T>(ts_load_d<S>(t1) - dv);
+        wave_lds_sync();
+        ts_put_d<S>(t1, vt);
+        wave_lds_sync();
+        stage_store<S>(V, m, row0, t1);
         acc = tile_gram(vt, vt, acc);
     }
     gram_block_partial<S>(acc, sh, part);
@@ -161,14 +180,21 @@ __global__ __launch_bounds__(256) void k_blk_vr(long long m, T *V, const BlkMats
                                                 double *part) {
     if (ctl_stopped(c)) return;
     __shared__ double Rm[SM_N], sh[4 * 256];
+    BLK_TILES;
+    (void)t1;
     load_small(Rm, mt->R1i, S * S);
     __syncthreads();
     const int w = threadIdx.x >> 6;
     mf4 acc = {0.0, 0.0, 0.0, 0.0};
     for (long long row0 = ((long long)blockIdx.x * 4 + w) * 16; row0 < m; row0 += (long long)gridDim.x * 64) {
-        mf4 vt = tile_mul<S>(V, m, row0, Rm);
-        tile_store<S>(V, m, row0, vt);
-        if constexpr (sizeof(T) != sizeof(double)) vt = tile_load<S>(V, m, row0);
+        wave_lds_sync();
+        stage_fetch<S>(V, m, row0, t0);
+        wave_lds_sync();
+        const mf4 vt = as_stored<T>(ts_mul<S>(t0, Rm));
+        wave_lds_sync();
+        ts_put_d<S>(t0, vt);
+        wave_lds_sync();
+        stage_store<S>(V, m, row0, t0);
         acc = tile_gram(vt, vt, acc);
     }
     gram_block_partial<S>(acc, sh, part);
@@ -180,22 +206,47 @@ __global__ __launch_bounds__(256) void k_blk_vp(long long m, T *V, T *P, const B
                                                 double pscale, const Ctl *c) {
     if (ctl_stopped(c)) return;
     __shared__ double Rm[SM_N], Bm[SM_N];
+    BLK_TILES;
     load_small(Rm, mt->R2i, S * S);
     load_small(Bm, mt->betaT, S * S);
     __syncthreads();
     const int w = threadIdx.x >> 6;
     for (long long row0 = ((long long)blockIdx.x * 4 + w) * 16; row0 < m; row0 += (long long)gridDim.x * 64) {
-        mf4 vt = tile_mul<S>(V, m, row0, Rm);
-        tile_store<S>(V, m, row0, vt);
-        if constexpr (sizeof(T) != sizeof(double)) vt = tile_load<S>(V, m, row0);
-        mf4 pt = (pscale != 0.0) ? vt * pscale : vt + tile_mul<S>(P, m, row0, Bm);
-        tile_store<S>(P, m, row0, pt);
+        wave_lds_sync();
+        stage_fetch<S>(V, m, row0, t0);
+        if (pscale == 0.0) stage_fetch<S>(P, m, row0, t1);
+        wave_lds_sync();
+        const mf4 vt = as_stored<T>(ts_mul<S>(t0, Rm));
+        const mf4 pt = (pscale != 0.0) ? vt * pscale : vt + ts_mul<S>(t1, Bm);
+        wave_lds_sync();
+        ts_put_d<S>(t0, vt);
+        ts_put_d<S>(t1, pt);
+        wave_lds_sync();
+        stage_store<S>(V, m, row0, t0);
+        stage_store<S>(P, m, row0, t1);
     }
 }
 
+// Y = alpha A X + beta Y: the SELL-64 kernel when the matrix has that copy in the
+// values' precision, else CSR-stream.
 template <int S, typename VT, typename T>
 static void launch_spmm(hipStream_t s, const bkh_matrix *A, const VT *val, const T *X, T *Y,
                         double alpha, double beta, const Ctl *c) {
+    const VT *sv = nullptr;
+    if (A->nsl > 0) {
+        if constexpr (sizeof(VT) == sizeof(float)) sv = (val == (const VT *)A->val32) ? (const VT *)A->sval32 : nullptr;
+        else sv = (val == (const VT *)A->val) ? (const VT *)A->sval : nullptr;
+    }
+    if (sv && (A->scd || A->scol)) {
+        const int G = (A->n + 255) / 256;
+        if (A->scd)
+            hipLaunchKernelGGL((k_spmm_sell<S, VT, T, short>), dim3(G), dim3(256), 0, s, A->n, A->rp, A->sso, sv,
+                               (const short *)A->scd, X, Y, alpha, beta, c);
+        else
+            hipLaunchKernelGGL((k_spmm_sell<S, VT, T, int>), dim3(G), dim3(256), 0, s, A->n, A->rp, A->sso, sv,
+                               (const int *)A->scol, X, Y, alpha, beta, c);
+        return;
+    }
     hipLaunchKernelGGL((k_spmm<S, VT, T>), dim3(A->nblk), dim3(SPMV_BS), 0, s, A->rp, A->ci, val, A->blk,
                        X, Y, alpha, beta, c);
 }
@@ -218,7 +269,7 @@ static int blcg_core(ThreadCtx *t, const bkh_matrix *A, const VT *val, const T *
     launch_spmm<S>(s, A, val, (const T *)Xt, AP, 1.0, 0.0, c);
     hipLaunchKernelGGL(k_blk_resid<T>, dim3(Gv), dim3(256), 0, s, m * S, Bt, (const T *)AP, V, part);
     launch_blk_fin<S, FIN_FNB>(s, part, Gv, mt, c);
-    hipLaunchKernelGGL((k_gram<S, T>), dim3(G), dim3(256), 0, s, m, (const T *)V, (const T *)V, c, part);
+    hipLaunchKernelGGL((k_gram_st<S, T>), dim3(G), dim3(256), 0, s, m, (const T *)V, (const T *)V, c, part);
     launch_blk_fin<S, FIN_QR1>(s, part, G, mt, c);
     hipLaunchKernelGGL((k_blk_vr<S, T>), dim3(G), dim3(256), 0, s, m, V, mt, c, part);
     launch_blk_fin<S, FIN_QR2_INIT>(s, part, G, mt, c);
@@ -230,7 +281,7 @@ static int blcg_core(ThreadCtx *t, const bkh_matrix *A, const VT *val, const T *
                                            (uintptr_t)P, (uintptr_t)AP, (uintptr_t)mt, (uintptr_t)c, (uintptr_t)part,
                                            (uintptr_t)S, (uintptr_t)sizeof(T)}, s, K, [&]() {
         launch_spmm<S>(s, A, val, (const T *)P, AP, 1.0, 0.0, c);
-        hipLaunchKernelGGL((k_gram<S, T>), dim3(G), dim3(256), 0, s, m, (const T *)P, (const T *)AP, c, part);
+        hipLaunchKernelGGL((k_gram_st<S, T>), dim3(G), dim3(256), 0, s, m, (const T *)P, (const T *)AP, c, part);
         launch_blk_fin<S, FIN_ALPHA>(s, part, G, mt, c);
         hipLaunchKernelGGL((k_blk_xv<S, T>), dim3(G), dim3(256), 0, s, m, Xt, (const T *)P, V,
                            (const T *)AP, mt, c, part);
@@ -372,6 +423,131 @@ static int blcg_ir(ThreadCtx *t, bkh_matrix *A, const double *B, double *X, doub
     return inf->ret;
 }
 
+__global__ void k_blk_setfnb(Ctl *c, const Ctl *co) {
+    if (threadIdx.x == 0) c->fnb = co->s[2];
+}
+
+__global__ void k_blk_resume(Ctl *c, double target) {
+    if (threadIdx.x == 0) { c->done = 0; c->status = ST_RUNNING; c->tol = target; }
+}
+
+__global__ __launch_bounds__(256) void k_blk_fold(long long ms, double *X, float *X32) {
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < ms; e += (long long)gridDim.x * 256) {
+        X[e] += (double)X32[e];
+        X32[e] = 0.0f;
+    }
+}
+
+// BKH_FP32_IR block CG with reliable updates (the block form of the CG driver): the
+// FP32 block CG runs on a correction X32 (FP64 Gram accumulation and s x s algebra,
+// its recursive residual measured against |B|_F); whenever |C|_F has fallen by a
+// factor delta, the host folds X += X32 and replaces the residual basis by the QR of
+// the FP64 true residual B - A X (V, C), then continues with the SAME directions P
+// (P = V + P beta^T with the last step's beta), so the block Krylov information is
+// kept across replacements.
+template <int S>
+static int blcg_ru(ThreadCtx *t, bkh_matrix *A, const double *B, double *X, double tol, int max_iter,
+                   const bkh_opts &o, bkh_info *inf) {
+    if (!A->val32) { bkh_set_error("FP32_IR needs a matrix created with want_fp32"); return BKH_BAD_ARGUMENT; }
+    const long long m = A->n, ms = m * S;
+    const int G = block_grid(m), Gv = vec_grid(ms, 256);
+    const int K = o.graph_iters > 0 ? o.graph_iters : std::max(2, default_graph_iters(m) / 4);
+    const int nparts = std::max(G * S * S, 2 * Gv);
+    const double delta = o.inner_tol > 0 ? o.inner_tol : 0.01;
+    size_t bytes = 2 * carve_size(sizeof(Ctl)) + carve_size(sizeof(BlkMats)) + carve_size(sizeof(double) * nparts) +
+                   3 * carve_size(sizeof(double) * ms) + 4 * carve_size(sizeof(float) * ms);
+    void *ws = bkh_workspace(bytes);
+    if (!ws) return BKH_DEVICE_ERROR;
+    Carve cv(ws);
+    Ctl *c = cv.take<Ctl>(1), *co = cv.take<Ctl>(1);
+    BlkMats *mt = cv.take<BlkMats>(1);
+    double *part = cv.take<double>(nparts);
+    double *Bt = cv.take<double>(ms), *Xt = cv.take<double>(ms), *R = cv.take<double>(ms);
+    float *X32 = cv.take<float>(ms), *V32 = cv.take<float>(ms), *P32 = cv.take<float>(ms), *AP32 = cv.take<float>(ms);
+    hipStream_t s = t->stream;
+    const float *val = A->val32;
+    HIP_OK(hipEventRecord(t->ev0, s));
+    hipLaunchKernelGGL((k_cm2rm<double, double>), dim3(Gv), dim3(256), 0, s, m, S, B, Bt);
+    hipLaunchKernelGGL((k_cm2rm<double, double>), dim3(Gv), dim3(256), 0, s, m, S, (const double *)X, Xt);
+    hipLaunchKernelGGL((k_sh_fill_f), dim3(Gv), dim3(256), 0, s, X32, ms);
+    BlkMats h;
+    memset(&h, 0, sizeof h);
+    h.m = (int)m;
+    h.s = S;
+    HIP_OK(hipMemcpyAsync(mt, &h, sizeof h, hipMemcpyHostToDevice, s));
+    if (ctl_init(t, c, 1.0, max_iter)) return BKH_DEVICE_ERROR;
+    LAUNCH_OK();
+    hipGraphExec_t ge = capture_cached(t, {0xB1C7, (uintptr_t)A, (uintptr_t)X32, (uintptr_t)V32, (uintptr_t)P32,
+                                           (uintptr_t)AP32, (uintptr_t)mt, (uintptr_t)c, (uintptr_t)part,
+                                           (uintptr_t)S}, s, K, [&]() {
+        launch_spmm<S>(s, A, val, (const float *)P32, AP32, 1.0, 0.0, c);
+        hipLaunchKernelGGL((k_gram_st<S, float>), dim3(G), dim3(256), 0, s, m, (const float *)P32, (const float *)AP32,
+                           c, part);
+        launch_blk_fin<S, FIN_ALPHA>(s, part, G, mt, c);
+        hipLaunchKernelGGL((k_blk_xv<S, float>), dim3(G), dim3(256), 0, s, m, X32, (const float *)P32, V32,
+                           (const float *)AP32, mt, c, part);
+        launch_blk_fin<S, FIN_QR1>(s, part, G, mt, c);
+        hipLaunchKernelGGL((k_blk_vr<S, float>), dim3(G), dim3(256), 0, s, m, V32, mt, c, part);
+        launch_blk_fin<S, FIN_QR2_ITER>(s, part, G, mt, c);
+        hipLaunchKernelGGL((k_blk_vp<S, float>), dim3(G), dim3(256), 0, s, m, V32, P32, mt, 0.0, c);
+    });
+    if (!ge) { bkh_set_error("hipGraph capture of the FP32 block CG iteration failed"); return BKH_DEVICE_ERROR; }
+    int passes = 0, st = ST_MAXIT, used = 0;
+    double rel = 0.0;
+    for (;;) {
+        // X += X32 (FP64); R = B - A X in FP64, its FP32 copy straight into V32
+        if (passes > 0) hipLaunchKernelGGL(k_blk_fold, dim3(Gv), dim3(256), 0, s, ms, Xt, X32);
+        if (ctl_init(t, co, 0.0, 1)) return BKH_DEVICE_ERROR;
+        launch_spmm<S>(s, A, A->val, (const double *)Xt, R, 1.0, 0.0, (const Ctl *)nullptr);
+        hipLaunchKernelGGL(k_blk_resid32, dim3(Gv), dim3(256), 0, s, ms, (const double *)Bt, (const double *)R, R,
+                           V32, part);
+        launch_finalize<2>(s, part, Gv, co, BlkResFin());
+        LAUNCH_OK();
+        if (ctl_fetch(t, co)) return BKH_DEVICE_ERROR;
+        rel = t->ctl_host->s[0];
+        if (rel < tol) { st = ST_CONVERGED; break; }
+        if (!(rel == rel) || rel > 1e300) { st = ST_BREAKDOWN; break; }
+        if (passes > 0) {
+            if (ctl_fetch(t, c)) return BKH_DEVICE_ERROR;
+            used = t->ctl_host->it;
+            if (t->ctl_host->status == ST_BREAKDOWN) { st = ST_BREAKDOWN; break; }
+            if (used >= max_iter) { st = ST_MAXIT; break; }
+        }
+        // the new residual basis: V, C = qr(R); then P = V + P beta^T (the first pass:
+        // P = V or 2V as the FP64 path's first direction)
+        hipLaunchKernelGGL(k_blk_resume, dim3(1), dim3(64), 0, s, c, std::max(0.5 * tol, delta * rel));
+        if (passes == 0) hipLaunchKernelGGL(k_blk_setfnb, dim3(1), dim3(64), 0, s, c, (const Ctl *)co);
+        hipLaunchKernelGGL((k_gram_st<S, float>), dim3(G), dim3(256), 0, s, m, (const float *)V32, (const float *)V32,
+                           c, part);
+        launch_blk_fin<S, FIN_QR1>(s, part, G, mt, c);
+        hipLaunchKernelGGL((k_blk_vr<S, float>), dim3(G), dim3(256), 0, s, m, V32, mt, c, part);
+        launch_blk_fin<S, FIN_QR2_INIT>(s, part, G, mt, c);
+        hipLaunchKernelGGL((k_blk_vp<S, float>), dim3(G), dim3(256), 0, s, m, V32, P32, mt,
+                           passes == 0 ? (o.fix_blcg_p0 ? 1.0 : 2.0) : 0.0, c);
+        LAUNCH_OK();
+        if (run_graph_until_done(t, ge, c, (max_iter - used) / K + 2)) return BKH_DEVICE_ERROR;
+        passes++;
+    }
+    if (ctl_fetch(t, c)) return BKH_DEVICE_ERROR;
+    const int total = t->ctl_host->it;
+    hipLaunchKernelGGL((k_rm2cm<double, double>), dim3(Gv), dim3(256), 0, s, m, S, (const double *)Xt, X);
+    LAUNCH_OK();
+    HIP_OK(hipEventRecord(t->ev1, s));
+    HIP_OK(hipEventSynchronize(t->ev1));
+    float msec = 0.f;
+    HIP_OK(hipEventElapsedTime(&msec, t->ev0, t->ev1));
+    inf->time_ms = msec;
+    bkh_note_solve_ms(msec);
+    inf->status = st;
+    inf->inner_iters = total;
+    inf->passes = passes;
+    inf->iters = total > 0 ? total - 1 : 0;
+    inf->rec_res = rel;
+    inf->true_res = rel;
+    inf->ret = st == ST_CONVERGED ? inf->iters : (st == ST_BREAKDOWN ? -2 : -1);
+    return inf->ret;
+}
+
 template <int S>
 static int blcg_dispatch(ThreadCtx *t, bkh_matrix *A, const double *B, double *X, double tol,
                          int max_iter, const bkh_opts &o, bkh_info *inf, bool want_true) {
@@ -436,7 +612,8 @@ extern "C" int bkh_blcg(bkh_matrix *A, int s, const double *B, double *X, double
     int ret;
     if (o.precision == BKH_FP32_IR) {
         switch (s) {
-#define CASE(S_) case S_: ret = blcg_ir<S_>(t, A, B, X, tol, max_iter, o, &inf); break;
+#define CASE(S_) case S_: ret = (o.path == 1) ? blcg_ir<S_>(t, A, B, X, tol, max_iter, o, &inf) \
+                                             : blcg_ru<S_>(t, A, B, X, tol, max_iter, o, &inf); break;
             CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
             CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
 #undef CASE

@@ -90,21 +90,22 @@ def test_cfg2_full_size_blcg_s8():
           f"P0=V {info1.ret} iters {info1.time_ms:.1f} ms")
 
 
-@pytest.mark.parametrize("m,s,fix_p0", [(24, 8, 0), (24, 8, 1), (20, 3, 0)])
-def test_fp32_ir_small(m, s, fix_p0):
-    """FP32 block CG inside FP64 iterative refinement reaches the FP64 tolerance;
-    the solution agrees with the oracle's FP64 one to cond(A) * tol."""
+@pytest.mark.parametrize("m,s,fix_p0,path", [(24, 8, 0, 0), (24, 8, 1, 0), (20, 3, 0, 0), (24, 8, 0, 1)])
+def test_fp32_ir_small(m, s, fix_p0, path):
+    """FP32 block CG with FP64 reliable updates (path 0; path 1: restarted
+    refinement) reaches the FP64 tolerance; the solution agrees with the oracle's
+    FP64 one to cond(A) * tol."""
     n, rp, ci, v = gen.poisson3d(m, lower=True)
     B = gen.normal(202, n * s).reshape(s, n).T.copy()
     M = dv.Matrix(n, rp, ci, v, symmetric=True, fp32=True)
     dB = dv.DeviceArray.from_numpy(np.ascontiguousarray(B.T))
     dX = dv.DeviceArray.from_numpy(np.zeros((s, n)))
-    info = dv.blcg(M, s, dB, dX, 1e-8, 3000, precision=dv.BKH_FP32_IR, fix_blcg_p0=fix_p0)
+    info = dv.blcg(M, s, dB, dX, 1e-8, 3000, precision=dv.BKH_FP32_IR, fix_blcg_p0=fix_p0, path=path)
     X = dX.numpy().T
     cret, cX = po.blcg(n, rp, ci, v, B, 1e-8, 3000, p0_mode=fix_p0)
     Af = expand_lower(n, rp, ci, v)
     res = np.linalg.norm(B - Af @ X) / np.linalg.norm(B)
-    print(f"m={m} s={s} p0={fix_p0}: fp32-ir inner {info.inner_iters} passes {info.passes} "
+    print(f"m={m} s={s} p0={fix_p0} path={path}: fp32-ir inner {info.inner_iters} passes {info.passes} "
           f"(fp64 oracle {cret}) true res {res:.2e}")
     assert info.ret >= 0 and info.passes >= 1
     assert res < 1e-8 and abs(info.true_res - res) < 1e-10
