@@ -236,6 +236,53 @@ __device__ __forceinline__ void stage_fetch(const T *X, long long m, long long r
     }
 }
 
+// The same tile in two steps: stage_load into registers (one 16-byte vector per lane
+// and pass) and stage_put into LDS, so the global load of the next tile can be issued
+// before the current one is processed.
+template <int S, typename T>
+struct StageRegs {
+    static constexpr int V = 16 / sizeof(T);
+    static constexpr int NV = (16 * S + V - 1) / V;
+    static constexpr int NP = (NV + 63) / 64;
+    typedef T tvec __attribute__((ext_vector_type(V)));
+    tvec v[NP];
+};
+
+template <int S, typename T>
+__device__ __forceinline__ void stage_load(const T *X, long long m, long long row0, StageRegs<S, T> &r) {
+    typedef StageRegs<S, T> R;
+    const int L = threadIdx.x & 63;
+    const long long e0 = row0 * S, emax = m * S;
+#pragma unroll
+    for (int pass = 0; pass < R::NP; pass++) {
+        const int lv = L + pass * 64;
+        const long long e = e0 + (long long)lv * R::V;
+        if (lv < R::NV && e + R::V <= emax) {
+            r.v[pass] = *(const typename R::tvec *)(X + e);
+        } else {
+#pragma unroll
+            for (int u = 0; u < R::V; u++) r.v[pass][u] = (lv < R::NV && e + u < emax) ? X[e + u] : (T)0;
+        }
+    }
+}
+
+template <int S, typename T>
+__device__ __forceinline__ void stage_put(const StageRegs<S, T> &r, double *tb) {
+    typedef StageRegs<S, T> R;
+    const int L = threadIdx.x & 63;
+#pragma unroll
+    for (int pass = 0; pass < R::NP; pass++) {
+        const int lv = L + pass * 64;
+        if (lv < R::NV) {
+#pragma unroll
+            for (int u = 0; u < R::V; u++) {
+                const int le = lv * R::V + u;
+                tb[(le / S) * TS_LD + le % S] = (double)r.v[pass][u];
+            }
+        }
+    }
+}
+
 template <int S, typename T>
 __device__ __forceinline__ void stage_store(T *X, long long m, long long row0, const double *tb) {
     constexpr int V = 16 / sizeof(T);
@@ -309,7 +356,8 @@ __device__ __forceinline__ mf4 as_stored(mf4 v) {
     return v;
 }
 
-// G = X^T Y with staged tiles (same partial layout as k_gram).
+// G = X^T Y with staged tiles (same partial layout as k_gram); the next tile's rows
+// are loaded while the current one is multiplied.
 template <int S, typename T>
 __global__ __launch_bounds__(256) void k_gram_st(long long m, const T *X, const T *Y, const Ctl *ctl, double *part) {
     if (ctl && ctl_stopped(ctl)) return;
@@ -317,14 +365,26 @@ __global__ __launch_bounds__(256) void k_gram_st(long long m, const T *X, const 
     __shared__ double tbs[4][2][16 * TS_LD];
     const int w = threadIdx.x >> 6;
     double *t0 = tbs[w][0], *t1 = tbs[w][1];
+    const bool same = (X == Y);
+    const long long step = (long long)gridDim.x * 64;
+    long long row0 = ((long long)blockIdx.x * 4 + w) * 16;
     mf4 acc = {0.0, 0.0, 0.0, 0.0};
-    for (long long row0 = ((long long)blockIdx.x * 4 + w) * 16; row0 < m; row0 += (long long)gridDim.x * 64) {
+    StageRegs<S, T> rx, ry;
+    if (row0 < m) {
+        stage_load<S>(X, m, row0, rx);
+        if (!same) stage_load<S>(Y, m, row0, ry);
+    }
+    for (; row0 < m; row0 += step) {
         wave_lds_sync();
-        stage_fetch<S>(X, m, row0, t0);
-        if (X != Y) stage_fetch<S>(Y, m, row0, t1);
+        stage_put<S>(rx, t0);
+        if (!same) stage_put<S>(ry, t1);
+        if (row0 + step < m) {
+            stage_load<S>(X, m, row0 + step, rx);
+            if (!same) stage_load<S>(Y, m, row0 + step, ry);
+        }
         wave_lds_sync();
         const mf4 a = ts_load_d<S>(t0);
-        const mf4 b = (X == Y) ? a : ts_load_d<S>(t1);
+        const mf4 b = same ? a : ts_load_d<S>(t1);
         acc = tile_gram(a, b, acc);
     }
     gram_block_partial<S>(acc, sh, part);

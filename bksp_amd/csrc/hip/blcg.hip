@@ -148,21 +148,34 @@ __global__ __launch_bounds__(256) void k_blk_xv(long long m, T *X, const T *P, T
     load_small(Am, mt->alpha, S * S);
     __syncthreads();
     const int w = threadIdx.x >> 6;
+    const long long step = (long long)gridDim.x * 64;
+    long long row0 = ((long long)blockIdx.x * 4 + w) * 16;
     mf4 acc = {0.0, 0.0, 0.0, 0.0};
-    for (long long row0 = ((long long)blockIdx.x * 4 + w) * 16; row0 < m; row0 += (long long)gridDim.x * 64) {
+    StageRegs<S, T> rp, rx, ra, rv;
+    if (row0 < m) {
+        stage_load<S>(P, m, row0, rp);
+        stage_load<S>(X, m, row0, rx);
+        stage_load<S>(AP, m, row0, ra);
+        stage_load<S>(V, m, row0, rv);
+    }
+    for (; row0 < m; row0 += step) {
+        const long long nx = row0 + step;
         wave_lds_sync();
-        stage_fetch<S>(P, m, row0, t0);
-        stage_fetch<S>(X, m, row0, t1);
+        stage_put<S>(rp, t0);
+        stage_put<S>(rx, t1);
+        if (nx < m) { stage_load<S>(P, m, nx, rp); stage_load<S>(X, m, nx, rx); }
         wave_lds_sync();
         const mf4 xt = ts_load_d<S>(t1) + ts_mul<S>(t0, Wm);
         wave_lds_sync();
         ts_put_d<S>(t1, xt);
-        stage_fetch<S>(AP, m, row0, t0);
+        stage_put<S>(ra, t0);
+        if (nx < m) stage_load<S>(AP, m, nx, ra);
         wave_lds_sync();
         stage_store<S>(X, m, row0, t1);
         const mf4 dv = ts_mul<S>(t0, Am);
         wave_lds_sync();
-        stage_fetch<S>(V, m, row0, t1);
+        stage_put<S>(rv, t1);
+        if (nx < m) stage_load<S>(V, m, nx, rv);
         wave_lds_sync();
         const mf4 vt = as_stored<T>(ts_load_d<S>(t1) - dv);
         wave_lds_sync();
@@ -185,10 +198,15 @@ __global__ __launch_bounds__(256) void k_blk_vr(long long m, T *V, const BlkMats
     load_small(Rm, mt->R1i, S * S);
     __syncthreads();
     const int w = threadIdx.x >> 6;
+    const long long step = (long long)gridDim.x * 64;
+    long long row0 = ((long long)blockIdx.x * 4 + w) * 16;
     mf4 acc = {0.0, 0.0, 0.0, 0.0};
-    for (long long row0 = ((long long)blockIdx.x * 4 + w) * 16; row0 < m; row0 += (long long)gridDim.x * 64) {
+    StageRegs<S, T> rv;
+    if (row0 < m) stage_load<S>(V, m, row0, rv);
+    for (; row0 < m; row0 += step) {
         wave_lds_sync();
-        stage_fetch<S>(V, m, row0, t0);
+        stage_put<S>(rv, t0);
+        if (row0 + step < m) stage_load<S>(V, m, row0 + step, rv);
         wave_lds_sync();
         const mf4 vt = as_stored<T>(ts_mul<S>(t0, Rm));
         wave_lds_sync();
@@ -211,13 +229,26 @@ __global__ __launch_bounds__(256) void k_blk_vp(long long m, T *V, T *P, const B
     load_small(Bm, mt->betaT, S * S);
     __syncthreads();
     const int w = threadIdx.x >> 6;
-    for (long long row0 = ((long long)blockIdx.x * 4 + w) * 16; row0 < m; row0 += (long long)gridDim.x * 64) {
+    const long long step = (long long)gridDim.x * 64;
+    long long row0 = ((long long)blockIdx.x * 4 + w) * 16;
+    const bool withp = (pscale == 0.0);
+    StageRegs<S, T> rv, rp;
+    if (row0 < m) {
+        stage_load<S>(V, m, row0, rv);
+        if (withp) stage_load<S>(P, m, row0, rp);
+    }
+    for (; row0 < m; row0 += step) {
+        const long long nx = row0 + step;
         wave_lds_sync();
-        stage_fetch<S>(V, m, row0, t0);
-        if (pscale == 0.0) stage_fetch<S>(P, m, row0, t1);
+        stage_put<S>(rv, t0);
+        if (withp) stage_put<S>(rp, t1);
+        if (nx < m) {
+            stage_load<S>(V, m, nx, rv);
+            if (withp) stage_load<S>(P, m, nx, rp);
+        }
         wave_lds_sync();
         const mf4 vt = as_stored<T>(ts_mul<S>(t0, Rm));
-        const mf4 pt = (pscale != 0.0) ? vt * pscale : vt + ts_mul<S>(t1, Bm);
+        const mf4 pt = withp ? vt + ts_mul<S>(t1, Bm) : vt * pscale;
         wave_lds_sync();
         ts_put_d<S>(t0, vt);
         ts_put_d<S>(t1, pt);
