@@ -120,8 +120,9 @@ __global__ __launch_bounds__(256) void k_blk_resid(long long ms, const T *B, con
                                                    double *part) {
     double v[1] = {0.0};
     for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < ms; e += (long long)gridDim.x * 256) {
-        V[e] = B[e] - AX[e];
-        v[0] += (double)B[e] * (double)B[e];
+        const T bv = B[e];                 // read once: V may alias B (and AX)
+        V[e] = bv - AX[e];
+        v[0] += (double)bv * (double)bv;
     }
     block_partial<1, 256>(v, part);
 }

@@ -128,7 +128,7 @@ static hipGraphExec_t cgf_capture(ThreadCtx *t, hipStream_t s, const bkh_matrix 
     GraphKey key = {0xC6F0, (uintptr_t)A, (uintptr_t)val, (uintptr_t)cd, (uintptr_t)x, (uintptr_t)r, (uintptr_t)P0,
                     (uintptr_t)P1, (uintptr_t)q, (uintptr_t)c, (uintptr_t)partA, (uintptr_t)partB,
                     (uintptr_t)sizeof(T), (uintptr_t)K};
-    return capture_cached(t, key, s, 1, [&]() {
+    return capture_cached(t, key, s, 1, [=]() {   // by value: eager mode runs the body after return
         for (int j = 0; j < K; j++) {
             if (g.sell) {
                 const VT *sv;
@@ -320,7 +320,7 @@ static hipGraphExec_t cg1_capture(ThreadCtx *t, hipStream_t s, const bkh_matrix 
     const int np = g.GA;
     GraphKey key = {0xC6F3, (uintptr_t)A, (uintptr_t)x32, (uintptr_t)R, (uintptr_t)Q, (uintptr_t)P, (uintptr_t)c,
                     (uintptr_t)part2, (uintptr_t)K};
-    return capture_cached(t, key, s, 1, [&]() {
+    return capture_cached(t, key, s, 1, [=]() {   // by value: eager mode runs the body after return
         for (int j = 0; j < K; j++) {
             // kernel base+j reads the partials of base+j-1: parity of j (K even)
             const double *pin = part2 + (size_t)((j + 1) & 1) * 3 * CGF_NPART;

@@ -7,8 +7,9 @@
  * There is no CPU solve path: if the device layer fails, the call returns
  * BKH_DEVICE_ERROR (-5) and bkh_last_error() says why.
  *
- * Setting BKSP_PRECISION=fp32ir in the environment makes these entry points use
- * the FP32-inner / FP64-refinement path (default: plain FP64).
+ * Setting BKSP_PRECISION=fp32ir in the environment makes the CG, block CG,
+ * BiCGSTAB(l) and shifted BiCGSTAB(l) entry points use the FP32-inner /
+ * FP64-refinement path (default: plain FP64).
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -160,7 +161,7 @@ int dblcg(dspmat *A, dmat *B, double tol, int max_iter, dmat *X) {
     bkh_opts o = default_opts();
     upload_t u;
     size_t ms = (size_t)m * s;
-    int rc = upload_system_sym(A, B->value, ms, X->value, ms, 0, &u);
+    int rc = upload_system_sym(A, B->value, ms, X->value, ms, o.precision == BKH_FP32_IR, &u);
     if (rc == 0) {
         rc = bkh_blcg(u.M, s, u.d_b, u.d_x, tol, max_iter, &o, NULL);
         if (rc > BKH_DEVICE_ERROR && download_x(&u, X->value)) rc = BKH_DEVICE_ERROR;

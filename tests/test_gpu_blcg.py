@@ -72,6 +72,10 @@ def test_poisson_parity(m, s):
     assert info.ret >= 0 and cret >= 0
     assert close_index(info.ret, cret), (info.ret, cret)
     assert info.true_res < 2e-8
+    # the reported FP64 true residual is the host's |B - A X|_F / |B|_F
+    Af = expand_lower(n, rp, ci, v)
+    res = np.linalg.norm(B - Af @ X) / np.linalg.norm(B)
+    assert res > 0 and abs(info.true_res - res) <= 1e-6 * res, (info.true_res, res)
     if info.ret == cret:
         assert rel(X, cX) < 1e-8
 
@@ -135,3 +139,19 @@ def test_codes():
     assert bksp.dblcg(A, bksp.Mat(B), 1e-8, 0, bksp.Mat(np.zeros((n, 2)))) == 0
     assert bksp.dblcg(A, bksp.Mat(np.ones((n, 17))), 1e-8, 5, bksp.Mat(np.zeros((n, 17)))) \
         == bksp.INVALID_PARAMETERS
+
+
+def test_dropin_fp32ir_environment(monkeypatch):
+    """BKSP_PRECISION=fp32ir routes the drop-in dblcg and bksp_dcg through the FP32
+    inner / FP64 refinement path; both still meet tol in the FP64 true residual."""
+    monkeypatch.setenv("BKSP_PRECISION", "fp32ir")
+    n, rp, ci, v = gen.poisson3d(20, lower=True)
+    Af = expand_lower(n, rp, ci, v)
+    A = bksp.dspmat(n, rp, ci, v, symmetric=True)
+    B = gen.normal(7, n * 4).reshape(4, n).T.copy()
+    Xm = bksp.Mat(np.zeros((n, 4)))
+    assert bksp.dblcg(A, bksp.Mat(B), 1e-9, 2000, Xm) >= 0
+    assert np.linalg.norm(B - Af @ Xm.a) / np.linalg.norm(B) < 1e-9
+    x = np.zeros(n)
+    assert bksp.bksp_dcg(A, B[:, 0].copy(), 1e-9, 2000, x) >= 0
+    assert np.linalg.norm(B[:, 0] - Af @ x) / np.linalg.norm(B[:, 0]) < 1e-9

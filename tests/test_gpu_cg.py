@@ -172,3 +172,32 @@ def test_cg_fp32_one_kernel_repeatable():
         assert info.ret >= 0 and info.true_res < 1e-9
         xs.append(dx.numpy())
     assert np.array_equal(xs[0], xs[1])
+
+
+def test_eager_mode_same_results():
+    """BKH_EAGER=1 (plain launches instead of graph replays) runs the same kernels in
+    the same order: bitwise the same FP64 and FP32-IR solutions, in a child process
+    (the mode is read once per process)."""
+    import subprocess, sys, os
+    code = (
+        "import sys; sys.path.insert(0, %r)\n"
+        "import numpy as np\n"
+        "from bksp_amd import device as dv, gen\n"
+        "n, rp, ci, v = gen.poisson3d(24, lower=True)\n"
+        "M = dv.Matrix(n, rp, ci, v, symmetric=True, fp32=True)\n"
+        "db = dv.DeviceArray.from_numpy(gen.normal(3, n))\n"
+        "out = []\n"
+        "for prec in (0, 1):\n"
+        "    dx = dv.DeviceArray.from_numpy(np.zeros(n))\n"
+        "    dv.cg(M, db, dx, 1e-9, 5000, precision=prec)\n"
+        "    out.append(dx.numpy())\n"
+        "np.save(sys.argv[1], np.stack(out))\n") % os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    import tempfile
+    res = []
+    with tempfile.TemporaryDirectory() as d:
+        for eager in ("0", "1"):
+            f = os.path.join(d, f"x{eager}.npy")
+            env = dict(os.environ, BKH_EAGER=eager)
+            subprocess.run([sys.executable, "-c", code, f], env=env, check=True, timeout=120)
+            res.append(np.load(f))
+    assert np.array_equal(res[0], res[1])
