@@ -73,22 +73,29 @@ def bench_cfg1(dv, gen, po):
 
 
 def bench_cfg2_block(dv, gen, m=128, s=8):
-    """Block CG (s = 8) on the 128^3 Poisson matrix, tol 1e-8, FP64 (P0 as the reference)."""
+    """Block CG (s = 8) on the 128^3 Poisson matrix, tol 1e-8: plain FP64 and FP32 inner
+    iterations with FP64 reliable updates, with the reference's first direction P0 = 2V
+    and with P0 = V."""
     n, rp, ci, v = gen.poisson3d(m, lower=True)
     B = gen.normal(202, n * s).reshape(s, n)            # column-major m x s: column j = B[j]
-    M = dv.Matrix(n, rp, ci, v, symmetric=True)
-    dB, dX = dv.DeviceArray.from_numpy(B), dv.DeviceArray.from_numpy(np.zeros((s, n)))
-    def run(fix):
+    M = dv.Matrix(n, rp, ci, v, symmetric=True, fp32=True)
+    dB, dX = dv.DeviceArray.from_numpy(B), dv.DeviceArray((s, n))
+
+    def run(fix, prec=0):
         dX.zero()
-        return dv.blcg(M, s, dB, dX, 1e-8, 5000, fix_blcg_p0=fix)
+        return dv.blcg(M, s, dB, dX, 1e-8, 5000, fix_blcg_p0=fix, precision=prec)
 
     run(0)
-    info, ms = _timed(lambda: run(0), 2)
-    info_f, ms_f = _timed(lambda: run(1), 2)
-    return {"workload": f"block CG s={s}, Poisson {m}^3, tol 1e-8, FP64",
-            "ms": round(ms, 2), "iters": info.ret, "true_res": info.true_res,
-            "ms_per_iter": round(ms / max(1, info.ret + 1), 3),
-            "fixed_p0": {"ms": round(ms_f, 2), "iters": info_f.ret, "true_res": info_f.true_res}}
+    out = {"workload": f"block CG s={s}, Poisson {m}^3, tol 1e-8"}
+    for fix, key in ((0, "p0_2v_as_reference"), (1, "p0_v")):
+        info, ms = _timed(lambda: run(fix), 1)
+        run(fix, dv.BKH_FP32_IR)
+        i32, ms32 = _timed(lambda: run(fix, dv.BKH_FP32_IR), 1)
+        out[key] = {"fp64_ms": round(ms, 2), "iters": info.ret, "true_res": info.true_res,
+                    "ms_per_iter": round(ms / max(1, info.ret + 1), 3),
+                    "fp32_ir": {"ms": round(ms32, 2), "inner_iters": i32.inner_iters, "passes": i32.passes,
+                                "true_res": i32.true_res, "speedup_vs_fp64": round(ms / ms32, 3)}}
+    return out
 
 
 def bench_cfg3(dv, gen, m=96):

@@ -114,7 +114,9 @@ static CgfGrid cgf_grid(const bkh_matrix *A) {
         g.GA = std::max(1, std::min((A->nwblk + CGF_WPB - 1) / CGF_WPB, CGF_NPART));
         g.chunk = (A->nwblk + g.GA - 1) / g.GA;
     }
-    g.GB = (int)std::max(1LL, std::min((long long)((A->n + 2047) / 2048), (long long)CGF_NPART));
+    // kernel B: 4096 rows per 512-thread workgroup (measured ~2 % faster per FP64 solve
+    // than 2048: half the partials for every workgroup of kernel A to re-sum)
+    g.GB = (int)std::max(1LL, std::min((long long)((A->n + 4095) / 4096), (long long)CGF_NPART));
     return g;
 }
 
