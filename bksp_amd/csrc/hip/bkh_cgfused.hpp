@@ -267,15 +267,16 @@ __global__ __launch_bounds__(512) void k_cgf_b(long long n, T *x, T *r, const T 
 //   rho_k       = rho_{k-1} - 2 alpha r_{k-1}.q_{k-1} + alpha^2 q_{k-1}.q_{k-1}
 //                 (= |r_{k-1} - alpha q_{k-1}|^2, FP64 sums of the stored FP32 values)
 //   beta_{k-1}  = rho_k / rho_{k-1}
-//   r_k = r_{k-1} - alpha q_{k-1},  p_k = r_k + beta p_{k-1}   (formed in the gather)
-//   q_k = A p_k,  x += alpha p_{k-1}  (the deferred update of iteration k-1)
+//   r_k = r_{k-1} - alpha q_{k-1}         (formed in the gather at every column)
+//   q_k = A r_k + beta q_{k-1},  p_k = r_k + beta p_{k-1}   (q_k = A p_k in exact arithmetic)
+//   x += alpha p_{k-1}  (the deferred update of iteration k-1)
 // and writes r_k, p_k, q_k (parity k&1) and its partials (p_k.q_k, r_k.q_k,
-// q_k.q_k). Same recurrence as the two-kernel form in exact arithmetic; one global
-// synchronisation per iteration instead of two. rho_k is the norm of the residual
-// that the next kernel materialises; the periodic FP64 residual replacement of the
-// reliable-update driver bounds any drift between it and the true residual.
-// At k == Ctl::first (a pass start) alpha = 0 (x, r already hold the replaced
-// values) and beta = rho_k / rho_{k-1} from the control block (0 when fresh).
+// q_k.q_k). Only r and q are gathered (the SpMV multiplies r_k), one global
+// synchronisation per iteration. The first kernel of a pass (k == Ctl::first: alpha = 0,
+// x and r hold the replaced values) forms q_k = A p_k explicitly from p_k = r + beta
+// p_{k-1} in its gather, so the recursively updated q starts every pass consistent;
+// the periodic FP64 residual replacement of the reliable-update driver bounds the
+// drift of r and rho in between.
 enum { CG1_ALPHA = 12 };              // Ctl::s slot: alpha_{k-1} at a stop
 
 // Fixed-order sum of np rows of 3 partials by a workgroup of BS threads,
@@ -357,6 +358,7 @@ __global__ __launch_bounds__(256) void k_cg1_sell(int n, int nsl, int chunk, con
     const bool writer = blockIdx.x == 0 && threadIdx.x == 0;
     double ad, bd;
     if (!cg1_scalars<256>(c, k, partPrev, np, writer, ad, bd)) return;
+    const bool start = (k == c->first);            // q = A p explicitly
     const float al = (float)ad, be = (float)bd;
     const size_t po = (size_t)((k + 1) & 1) * n, pn = (size_t)(k & 1) * n;
     const float *rp_ = R + po, *qp_ = Q + po, *pp_ = P + po;
@@ -371,24 +373,32 @@ __global__ __launch_bounds__(256) void k_cg1_sell(int n, int nsl, int chunk, con
         const int len = rp[row + 1] - rp[row];
         const int base = so[sl] + l;
         float acc = 0.0f;
+        if (start) {
 #pragma unroll 4
-        for (int kk = 0; kk < len; kk++) {
-            const int e = base + 64 * kk;
-            const int cc = sizeof(IT) == 2 ? row + (int)col[e] : (int)col[e];
-            const float rc = fmaf(-al, qp_[cc], rp_[cc]);
-            const float pc = fmaf(be, pp_[cc], rc);
-            acc = add_rn(acc, mul_rn((float)val[e], pc));
+            for (int kk = 0; kk < len; kk++) {
+                const int e = base + 64 * kk;
+                const int cc = sizeof(IT) == 2 ? row + (int)col[e] : (int)col[e];
+                acc = add_rn(acc, mul_rn((float)val[e], fmaf(be, pp_[cc], rp_[cc])));
+            }
+        } else {
+#pragma unroll 4
+            for (int kk = 0; kk < len; kk++) {
+                const int e = base + 64 * kk;
+                const int cc = sizeof(IT) == 2 ? row + (int)col[e] : (int)col[e];
+                acc = add_rn(acc, mul_rn((float)val[e], fmaf(-al, qp_[cc], rp_[cc])));
+            }
         }
-        const float pold = pp_[row];
-        const float rr = fmaf(-al, qp_[row], rp_[row]);
+        const float pold = pp_[row], qold = qp_[row];
+        const float rr = fmaf(-al, qold, rp_[row]);
         const float pnew = fmaf(be, pold, rr);
+        const float qnew = start ? acc : fmaf(be, qold, acc);
         x[row] = fmaf(al, pold, x[row]);
         rn_[row] = rr;
         pn_[row] = pnew;
-        qn_[row] = acc;
-        dv[0] += (double)pnew * (double)acc;
-        dv[1] += (double)rr * (double)acc;
-        dv[2] += (double)acc * (double)acc;
+        qn_[row] = qnew;
+        dv[0] += (double)pnew * (double)qnew;
+        dv[1] += (double)rr * (double)qnew;
+        dv[2] += (double)qnew * (double)qnew;
     }
     block_partial<3, 256>(dv, partOut);
 }

@@ -19,8 +19,9 @@
 // FP64, r32 = r, x32 = 0, and the FP32 iteration continues with its search
 // direction, so no Krylov information is discarded (a restarted refinement would
 // lose it at every pass). On SELL matrices the FP32 iteration is one kernel per
-// iteration (k_cg1_sell: residual norm from the alpha recurrence, x update deferred
-// into the next kernel), elsewhere the two-kernel form of the FP64 path.
+// iteration (k_cg1_sell: residual norm from the alpha recurrence, q = A p by
+// recurrence so only r and q are gathered, x update deferred into the next kernel),
+// elsewhere the two-kernel form of the FP64 path.
 #include <math.h>
 #include "bkh_cgfused.hpp"
 #include "bkh_host.hpp"
