@@ -292,7 +292,7 @@ def main():
 
     t_cga = load_traffic("cg_fused_poisson128_traffic.json")
     t_spmv = load_traffic("spmv_fp64_poisson128_traffic.json")
-    roofline = {"bound": "hbm", "kernel": "k_cgf_a_sell<double,double,short> (CG: SELL-64 SpMV with p = r + beta p "
+    roofline = {"bound": "hbm", "kernel": "k_cgf_a_sell<double,double,short,1024> (CG: SELL-64 SpMV with p = r + beta p "
                                           "in the gather)",
                 "achieved": round(cga_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(cga_gbs / HBM_PEAK_GBS, 4),

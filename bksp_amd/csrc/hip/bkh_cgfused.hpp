@@ -177,8 +177,8 @@ __global__ __launch_bounds__(64 * CGF_WPB) void k_cgf_a(int nwblk, int chunk, co
 // A(k) on the SELL-64 layout: a lane owns a row and forms p_k at each column of it
 // in the gather; workgroup g owns `chunk` consecutive slices, its waves taking every
 // fourth. IT: int columns or short (column - row).
-template <typename VT, typename T, typename IT>
-__global__ __launch_bounds__(256) void k_cgf_a_sell(int n, int nsl, int chunk, const int *__restrict__ rp,
+template <typename VT, typename T, typename IT, int BS>
+__global__ __launch_bounds__(BS) void k_cgf_a_sell(int n, int nsl, int chunk, const int *__restrict__ rp,
                                                     const int *__restrict__ so, const VT *__restrict__ val,
                                                     const IT *__restrict__ col, const T *r, T *P0, T *P1, T *q,
                                                     Ctl *c, int j, const double *partB, int npB, double *partA) {
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(256) void k_cgf_a_sell(int n, int nsl, int chunk, c
     if (k == c->first) {
         rr = c->s[CGF_RR + (k & 1)];
     } else {
-        rr = block_sum_parts<256>(partB, npB);
+        rr = block_sum_parts<BS>(partB, npB);
         if (cgf_check(c, k, rr, writer) != ST_RUNNING) return;
         if (writer) c->s[CGF_RR + (k & 1)] = rr;
     }
@@ -201,7 +201,7 @@ __global__ __launch_bounds__(256) void k_cgf_a_sell(int n, int nsl, int chunk, c
     const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int send = min(nsl, (g + 1) * chunk);
     double dv[1] = {0.0};
-    for (int sl = g * chunk + w; sl < send; sl += 4) {
+    for (int sl = g * chunk + w; sl < send; sl += BS / 64) {
         const int row = sl * 64 + l;
         if (row >= n) continue;
         const int len = rp[row + 1] - rp[row];
@@ -219,7 +219,7 @@ __global__ __launch_bounds__(256) void k_cgf_a_sell(int n, int nsl, int chunk, c
         pnew[row] = pn;
         dv[0] += (double)pn * (double)acc;
     }
-    block_partial<1, 256>(dv, partA);
+    block_partial<1, BS>(dv, partA);
 }
 
 // B(k): alpha_k = (r_k.r_k) / (p_k.q); x += alpha p_k, r -= alpha q; r.r partials.
@@ -347,8 +347,8 @@ __device__ __forceinline__ bool cg1_scalars(Ctl *c, int k, const double *partPre
 
 // Kernel k on the SELL-64 layout (lane per row, workgroup g owns `chunk` slices).
 // R, Q, P hold two buffers each (parity), n entries apart; x32 is updated in place.
-template <typename VT, typename IT>
-__global__ __launch_bounds__(256) void k_cg1_sell(int n, int nsl, int chunk, const int *__restrict__ rp,
+template <typename VT, typename IT, int BS>
+__global__ __launch_bounds__(BS) void k_cg1_sell(int n, int nsl, int chunk, const int *__restrict__ rp,
                                                   const int *__restrict__ so, const VT *__restrict__ val,
                                                   const IT *__restrict__ col, float *x, float *R, float *Q,
                                                   float *P, Ctl *c, int j, const double *partPrev,
@@ -357,7 +357,7 @@ __global__ __launch_bounds__(256) void k_cg1_sell(int n, int nsl, int chunk, con
     const int k = c->base + j;
     const bool writer = blockIdx.x == 0 && threadIdx.x == 0;
     double ad, bd;
-    if (!cg1_scalars<256>(c, k, partPrev, np, writer, ad, bd)) return;
+    if (!cg1_scalars<BS>(c, k, partPrev, np, writer, ad, bd)) return;
     const bool start = (k == c->first);            // q = A p explicitly
     const float al = (float)ad, be = (float)bd;
     const size_t po = (size_t)((k + 1) & 1) * n, pn = (size_t)(k & 1) * n;
@@ -367,7 +367,7 @@ __global__ __launch_bounds__(256) void k_cg1_sell(int n, int nsl, int chunk, con
     const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int send = min(nsl, (g + 1) * chunk);
     double dv[3] = {0.0, 0.0, 0.0};
-    for (int sl = g * chunk + w; sl < send; sl += 4) {
+    for (int sl = g * chunk + w; sl < send; sl += BS / 64) {
         const int row = sl * 64 + l;
         if (row >= n) continue;
         const int len = rp[row + 1] - rp[row];
@@ -400,7 +400,7 @@ __global__ __launch_bounds__(256) void k_cg1_sell(int n, int nsl, int chunk, con
         dv[1] += (double)rr * (double)qnew;
         dv[2] += (double)qnew * (double)qnew;
     }
-    block_partial<3, 256>(dv, partOut);
+    block_partial<3, BS>(dv, partOut);
 }
 
 // End of a graph replay of K single-kernel iterations: test of iteration base+K.

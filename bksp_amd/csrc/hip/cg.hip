@@ -103,13 +103,21 @@ static inline bool cgf_sell(const bkh_matrix *A) {
 
 static inline bool cg_can_fuse(const bkh_matrix *A) { return cgf_sell(A) || A->max_row <= SPMV_WCAP; }
 
-struct CgfGrid { int GA, chunk, GB; bool sell; };
+struct CgfGrid { int GA, chunk, GB, BSA; bool sell; };
 
 static CgfGrid cgf_grid(const bkh_matrix *A) {
     CgfGrid g;
     g.sell = cgf_sell(A);
+    g.BSA = 256;
     if (g.sell) {
-        g.GA = std::max(1, std::min((A->nsl + 3) / 4, CGF_NPART));
+        // 1024-thread workgroups, at most 1024 of them, once there are 512 or more
+        // (kernel B re-sums A's partials: 28.0 ms per FP64 128^3 solve against 29.4 ms
+        // with 2048 workgroups of 256 threads)
+        if (A->nsl / 16 >= 512) {
+            g.BSA = 1024;
+            g.GA = std::min(A->nsl / 16, 1024);
+        } else
+            g.GA = std::max(1, std::min((A->nsl + 3) / 4, CGF_NPART));
         g.chunk = (A->nsl + g.GA - 1) / g.GA;
     } else {
         g.GA = std::max(1, std::min((A->nwblk + CGF_WPB - 1) / CGF_WPB, CGF_NPART));
@@ -121,6 +129,22 @@ static CgfGrid cgf_grid(const bkh_matrix *A) {
     return g;
 }
 
+template <typename VT, typename T, int BS>
+static void launch_cgf_a_sell(hipStream_t s, const bkh_matrix *A, const CgfGrid &g, T *r, T *P0, T *P1, T *q,
+                              Ctl *c, int j, double *partB, double *partA) {
+    const VT *sv;
+    if constexpr (sizeof(VT) == sizeof(float)) sv = (const VT *)A->sval32;
+    else sv = (const VT *)A->sval;
+    if (A->scd && A->variant == SPMV_SELL)
+        hipLaunchKernelGGL((k_cgf_a_sell<VT, T, short, BS>), dim3(g.GA), dim3(BS), 0, s, (int)A->n, A->nsl,
+                           g.chunk, A->rp, A->sso, sv, (const short *)A->scd, (const T *)r, P0, P1, q, c, j,
+                           (const double *)partB, g.GB, partA);
+    else
+        hipLaunchKernelGGL((k_cgf_a_sell<VT, T, int, BS>), dim3(g.GA), dim3(BS), 0, s, (int)A->n, A->nsl,
+                           g.chunk, A->rp, A->sso, sv, (const int *)A->scol, (const T *)r, P0, P1, q, c, j,
+                           (const double *)partB, g.GB, partA);
+}
+
 // Graph of K fused iterations plus the tail, on vectors T with values VT
 // (cd: 16-bit column deltas or null).
 template <typename VT, typename T>
@@ -130,21 +154,12 @@ static hipGraphExec_t cgf_capture(ThreadCtx *t, hipStream_t s, const bkh_matrix 
     const long long n = A->n;
     GraphKey key = {0xC6F0, (uintptr_t)A, (uintptr_t)val, (uintptr_t)cd, (uintptr_t)x, (uintptr_t)r, (uintptr_t)P0,
                     (uintptr_t)P1, (uintptr_t)q, (uintptr_t)c, (uintptr_t)partA, (uintptr_t)partB,
-                    (uintptr_t)sizeof(T), (uintptr_t)K};
+                    (uintptr_t)sizeof(T), (uintptr_t)K, (uintptr_t)(g.BSA * 65536 + g.GA)};
     return capture_cached(t, key, s, 1, [=]() {   // by value: eager mode runs the body after return
         for (int j = 0; j < K; j++) {
             if (g.sell) {
-                const VT *sv;
-                if constexpr (sizeof(VT) == sizeof(float)) sv = (const VT *)A->sval32;
-                else sv = (const VT *)A->sval;
-                if (A->scd && A->variant == SPMV_SELL)
-                    hipLaunchKernelGGL((k_cgf_a_sell<VT, T, short>), dim3(g.GA), dim3(256), 0, s, (int)n, A->nsl,
-                                       g.chunk, A->rp, A->sso, sv, (const short *)A->scd, (const T *)r, P0, P1, q,
-                                       c, j, (const double *)partB, g.GB, partA);
-                else
-                    hipLaunchKernelGGL((k_cgf_a_sell<VT, T, int>), dim3(g.GA), dim3(256), 0, s, (int)n, A->nsl,
-                                       g.chunk, A->rp, A->sso, sv, (const int *)A->scol, (const T *)r, P0, P1, q,
-                                       c, j, (const double *)partB, g.GB, partA);
+                if (g.BSA == 1024) launch_cgf_a_sell<VT, T, 1024>(s, A, g, r, P0, P1, q, c, j, partB, partA);
+                else launch_cgf_a_sell<VT, T, 256>(s, A, g, r, P0, P1, q, c, j, partB, partA);
             } else if (cd)
                 hipLaunchKernelGGL((k_cgf_a<VT, T, true>), dim3(g.GA), dim3(64 * CGF_WPB), 0, s, A->nwblk, g.chunk,
                                    A->rp, A->wblk, A->ci, cd, val, (const T *)r, P0, P1, q, c, j,
@@ -316,29 +331,56 @@ struct Cg1ResumeFin {
     }
 };
 
+// Launch shape of the single-kernel FP32 iteration. Every workgroup first re-sums
+// all partials of the previous kernel (3 per workgroup), so large matrices run
+// CG1_NPART workgroups of 512 threads (8 waves, about two slices per wave at 128^3):
+// measured 15.0 ms per 128^3 solve against 16.2 ms with 2048 workgroups of 256
+// threads, whose partials cost ~100 MB of L2 reads per kernel; 1024 workgroups of
+// 1024 threads were slower again (17.6 ms).
+#define CG1_NPART 1024
+struct Cg1Grid { int G, BS, chunk; };
+
+static Cg1Grid cg1_grid(const bkh_matrix *A) {
+    Cg1Grid g;
+    if (A->nsl >= 8 * CG1_NPART) {
+        g.BS = 512;
+        g.G = CG1_NPART;
+    } else {
+        g.BS = 256;
+        g.G = std::max(1, std::min((A->nsl + 3) / 4, CGF_NPART));
+    }
+    g.chunk = (A->nsl + g.G - 1) / g.G;
+    return g;
+}
+
+template <int BS>
+static void launch_cg1(hipStream_t s, const bkh_matrix *A, const Cg1Grid &g, float *x32, float *R, float *Q,
+                       float *P, Ctl *c, int j, const double *pin, double *pout) {
+    if (A->scd)
+        hipLaunchKernelGGL((k_cg1_sell<float, short, BS>), dim3(g.G), dim3(BS), 0, s, (int)A->n, A->nsl, g.chunk,
+                           A->rp, A->sso, (const float *)A->sval32, (const short *)A->scd, x32, R, Q, P, c, j,
+                           pin, g.G, pout);
+    else
+        hipLaunchKernelGGL((k_cg1_sell<float, int, BS>), dim3(g.G), dim3(BS), 0, s, (int)A->n, A->nsl, g.chunk,
+                           A->rp, A->sso, (const float *)A->sval32, (const int *)A->scol, x32, R, Q, P, c, j,
+                           pin, g.G, pout);
+}
+
 static hipGraphExec_t cg1_capture(ThreadCtx *t, hipStream_t s, const bkh_matrix *A, float *x32, float *R,
                                   float *Q, float *P, Ctl *c, double *part2, int K) {
-    const CgfGrid g = cgf_grid(A);
-    const long long n = A->n;
-    const int np = g.GA;
+    const Cg1Grid g = cg1_grid(A);
     GraphKey key = {0xC6F3, (uintptr_t)A, (uintptr_t)x32, (uintptr_t)R, (uintptr_t)Q, (uintptr_t)P, (uintptr_t)c,
-                    (uintptr_t)part2, (uintptr_t)K};
+                    (uintptr_t)part2, (uintptr_t)K, (uintptr_t)(g.BS * 65536 + g.G)};
     return capture_cached(t, key, s, 1, [=]() {   // by value: eager mode runs the body after return
         for (int j = 0; j < K; j++) {
             // kernel base+j reads the partials of base+j-1: parity of j (K even)
             const double *pin = part2 + (size_t)((j + 1) & 1) * 3 * CGF_NPART;
             double *pout = part2 + (size_t)(j & 1) * 3 * CGF_NPART;
-            if (A->scd)
-                hipLaunchKernelGGL((k_cg1_sell<float, short>), dim3(g.GA), dim3(256), 0, s, (int)n, A->nsl, g.chunk,
-                                   A->rp, A->sso, (const float *)A->sval32, (const short *)A->scd, x32, R, Q, P, c,
-                                   j, pin, np, pout);
-            else
-                hipLaunchKernelGGL((k_cg1_sell<float, int>), dim3(g.GA), dim3(256), 0, s, (int)n, A->nsl, g.chunk,
-                                   A->rp, A->sso, (const float *)A->sval32, (const int *)A->scol, x32, R, Q, P, c,
-                                   j, pin, np, pout);
+            if (g.BS == 512) launch_cg1<512>(s, A, g, x32, R, Q, P, c, j, pin, pout);
+            else launch_cg1<256>(s, A, g, x32, R, Q, P, c, j, pin, pout);
         }
         hipLaunchKernelGGL(k_cg1_tail, dim3(1), dim3(256), 0, s, c, K,
-                           (const double *)(part2 + (size_t)((K + 1) & 1) * 3 * CGF_NPART), np);
+                           (const double *)(part2 + (size_t)((K + 1) & 1) * 3 * CGF_NPART), g.G);
     });
 }
 
@@ -557,27 +599,13 @@ extern "C" int bkh_bench_cg_kernel(bkh_matrix *A, int fp32, int reps, double *av
     const CgfGrid g = cgf_grid(A);
     auto launch = [&]() {
         if (fp32) {
-            if (A->scd && A->variant == SPMV_SELL)
-                hipLaunchKernelGGL((k_cgf_a_sell<float, float, short>), dim3(g.GA), dim3(256), 0, s, (int)n, A->nsl,
-                                   g.chunk, A->rp, A->sso, (const float *)A->sval32, (const short *)A->scd,
-                                   (const float *)r, (float *)P, (float *)P + n, (float *)q, c, 0,
-                                   (const double *)partB, g.GB, partA);
-            else
-                hipLaunchKernelGGL((k_cgf_a_sell<float, float, int>), dim3(g.GA), dim3(256), 0, s, (int)n, A->nsl,
-                                   g.chunk, A->rp, A->sso, (const float *)A->sval32, (const int *)A->scol,
-                                   (const float *)r, (float *)P, (float *)P + n, (float *)q, c, 0,
-                                   (const double *)partB, g.GB, partA);
+            float *r_ = (float *)r, *p0 = (float *)P, *p1 = (float *)P + n, *q_ = (float *)q;
+            if (g.BSA == 1024) launch_cgf_a_sell<float, float, 1024>(s, A, g, r_, p0, p1, q_, c, 0, partB, partA);
+            else launch_cgf_a_sell<float, float, 256>(s, A, g, r_, p0, p1, q_, c, 0, partB, partA);
         } else {
-            if (A->scd && A->variant == SPMV_SELL)
-                hipLaunchKernelGGL((k_cgf_a_sell<double, double, short>), dim3(g.GA), dim3(256), 0, s, (int)n,
-                                   A->nsl, g.chunk, A->rp, A->sso, (const double *)A->sval, (const short *)A->scd,
-                                   (const double *)r, (double *)P, (double *)P + n, (double *)q, c, 0,
-                                   (const double *)partB, g.GB, partA);
-            else
-                hipLaunchKernelGGL((k_cgf_a_sell<double, double, int>), dim3(g.GA), dim3(256), 0, s, (int)n,
-                                   A->nsl, g.chunk, A->rp, A->sso, (const double *)A->sval, (const int *)A->scol,
-                                   (const double *)r, (double *)P, (double *)P + n, (double *)q, c, 0,
-                                   (const double *)partB, g.GB, partA);
+            double *r_ = (double *)r, *p0 = (double *)P, *p1 = (double *)P + n, *q_ = (double *)q;
+            if (g.BSA == 1024) launch_cgf_a_sell<double, double, 1024>(s, A, g, r_, p0, p1, q_, c, 0, partB, partA);
+            else launch_cgf_a_sell<double, double, 256>(s, A, g, r_, p0, p1, q_, c, 0, partB, partA);
         }
     };
     launch();
