@@ -65,6 +65,7 @@ _SIGS = {
     "bkh_matrix_blocks": (c_int, [c_vp]),
     "bkh_matrix_set_spmv": (c_int, [c_vp, c_int]),
     "bkh_spmv": (c_int, [c_vp, ctypes.c_char, c_dbl, c_vp, c_dbl, c_vp]),
+    "bkh_spmm": (c_int, [c_vp, c_int, c_dbl, c_vp, c_dbl, c_vp]),
     "bkh_spmv32": (c_int, [c_vp, c_vp, c_vp]),
     "bkh_dot": (c_int, [c_ll, c_vp, c_vp, c_dp]),
     "bkh_qr": (c_int, [c_int, c_int, c_vp, c_dp]),

@@ -419,3 +419,58 @@ __global__ __launch_bounds__(256) void k_spmm_sell(int n, const int *__restrict_
 #pragma unroll
     for (int j = 0; j < S; j++) Y[(size_t)row * S + j] = acc[j];
 }
+
+// Lanes per row of k_spmm_sell_v: each lane owns 16 bytes of a row of X and Y
+// (S * sizeof(T) = 16 * LPR), 0 when S does not split that way.
+template <int S, typename T>
+constexpr int spmm_lpr() {
+    constexpr int b = S * (int)sizeof(T);
+    return (b == 16 || b == 32 || b == 64 || b == 128) ? b / 16 : 0;
+}
+
+// Y = alpha A X + beta Y on the SELL-64 copy of A with LPR lanes per row: lane `sub`
+// of a row owns its columns [sub*CPL, (sub+1)*CPL), so every gather of a neighbour's
+// row of X is one 16-byte load per lane and a wave's loads cover 64/LPR whole rows
+// (contiguous for neighbouring rows) instead of a quarter of each of 64 rows' cache
+// lines. Per column the sum runs over the row's entries in stored order with the same
+// roundings as k_spmm_sell, so both give identical results. X, Y 16-byte aligned.
+template <int S, typename VT, typename T, typename IT>
+__global__ __launch_bounds__(256) void k_spmm_sell_v(int n, const int *__restrict__ rp, const int *__restrict__ so,
+                                                     const VT *__restrict__ val, const IT *__restrict__ col,
+                                                     const T *X, T *Y, double alpha, double beta, const Ctl *ctl) {
+    constexpr int LPR = spmm_lpr<S, T>();
+    static_assert(LPR > 0, "k_spmm_sell_v needs S * sizeof(T) in {16, 32, 64, 128}");
+    constexpr int CPL = S / LPR, RPB = 256 / LPR;
+    typedef T tvec __attribute__((ext_vector_type(CPL)));
+    if (ctl && ctl_stopped(ctl)) return;
+    const int row = xcd_block(blockIdx.x, gridDim.x) * RPB + (int)threadIdx.x / LPR;
+    const int sub = (int)threadIdx.x % LPR;
+    if (row >= n) return;
+    const int len = rp[row + 1] - rp[row];
+    const int base = so[row >> 6] + (row & 63);
+    const bool a1 = (alpha == 1.0);
+    T acc[CPL];
+    if (beta != 0.0) {
+        const tvec y = *(const tvec *)(Y + (size_t)row * S + sub * CPL);
+#pragma unroll
+        for (int j = 0; j < CPL; j++) acc[j] = mul_rn((T)beta, y[j]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < CPL; j++) acc[j] = (T)0;
+    }
+#pragma unroll 2
+    for (int kk = 0; kk < len; kk++) {
+        const int e = base + 64 * kk;
+        const IT ce = __builtin_nontemporal_load(col + e);     // the matrix streams once:
+        const VT ve = __builtin_nontemporal_load(val + e);     // keep X's lines in L2
+        const int c = sizeof(IT) == 2 ? row + (int)ce : (int)ce;
+        const T a = a1 ? (T)ve : (T)(alpha * (double)ve);
+        const tvec xv = *(const tvec *)(X + (size_t)c * S + sub * CPL);
+#pragma unroll
+        for (int j = 0; j < CPL; j++) acc[j] = add_rn(acc[j], mul_rn(a, xv[j]));
+    }
+    tvec out;
+#pragma unroll
+    for (int j = 0; j < CPL; j++) out[j] = acc[j];
+    __builtin_nontemporal_store(out, (tvec *)(Y + (size_t)row * S + sub * CPL));
+}

@@ -35,6 +35,30 @@ __global__ __launch_bounds__(256) void k_sh_fill_f(float *p, long long m) {
     for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < m; e += (long long)gridDim.x * 256) p[e] = 0.0f;
 }
 
+// Fixed-order sum of nblocks rows of E partials by a 256-thread workgroup: thread
+// t < Q*E (Q = 256/E groups) sums rows q, q+Q, q+2Q, ... of entry e = t % E, q = t / E
+// (independent loads in flight), then the Q group sums are added in order.
+// red: Q*E <= 256 doubles of LDS; out[e], e < E, valid in every thread on return.
+template <int E>
+__device__ __forceinline__ void fin_sum_parts(const double *part, int nblocks, double *red, double *out) {
+    constexpr int Q = 256 / E;
+    const int t = threadIdx.x;
+    if (t < Q * E) {
+        const int e = t % E;
+        double acc = 0.0;
+#pragma unroll 8
+        for (int b = t / E; b < nblocks; b += Q) acc += part[(size_t)b * E + e];
+        red[t] = acc;
+    }
+    __syncthreads();
+    if (t < E) {
+        double v = red[t];
+        for (int q = 1; q < Q; q++) v += red[q * E + t];
+        out[t] = v;
+    }
+    __syncthreads();
+}
+
 // One workgroup: sums the s x s Gram partials of `nblocks` workgroups in a fixed
 // order, then runs the block solver's small dense step `op`.
 template <int S, int OP>
@@ -46,21 +70,15 @@ __global__ __launch_bounds__(256) void k_blk_fin(const double *part, int nblocks
     __shared__ double red;
     const int t = threadIdx.x;
     if (OP == FIN_FNB) {
+        fin_sum_parts<1>(part, nblocks, work, G);
         if (t == 0) {
-            double acc = 0.0;
-            for (int b = 0; b < nblocks; b++) acc += part[b];
-            mt->fnb2 = acc;
-            c->fnb = sqrt(acc);
+            mt->fnb2 = G[0];
+            c->fnb = sqrt(G[0]);
             if (!(c->fnb > 0.0)) ctl_breakdown(c);
         }
         return;
     }
-    if (t < S * S) {
-        double acc = 0.0;
-        for (int b = 0; b < nblocks; b++) acc += part[(size_t)b * S * S + t];
-        G[t] = acc;
-    }
-    __syncthreads();
+    fin_sum_parts<S * S>(part, nblocks, work, G);
     if (OP == FIN_ALPHA) {
         // alpha = (P^T A P)^-1 (dgetrf/dgetri, blcg.c:81-82), W = alpha C (:85-86)
         if (!sm_inverse(S, G, Ri, work, flag)) { if (t == 0) ctl_breakdown(c); return; }
@@ -106,9 +124,16 @@ __global__ __launch_bounds__(256) void k_blk_fin(const double *part, int nblocks
     if (t == 0) ctl_check(c, f2);
 }
 
+// Partial rows the fin kernel sums itself; more go through k_part_reduce first.
+#define FIN_DIRECT 256
+
 template <int S, int OP>
 static void launch_blk_fin(hipStream_t s, const double *part, int nblocks, BlkMats *mt, Ctl *c) {
     const int E = (OP == FIN_FNB) ? 1 : S * S;
+    if (nblocks <= FIN_DIRECT) {
+        hipLaunchKernelGGL((k_blk_fin<S, OP>), dim3(1), dim3(256), 0, s, part, nblocks, mt, c);
+        return;
+    }
     double *r2 = bkh_red2();
     const int nb2 = (nblocks + RED_CHUNK - 1) / RED_CHUNK;
     hipLaunchKernelGGL(k_part_reduce, dim3(nb2), dim3(256), 0, s, part, nblocks, E, r2);
@@ -259,6 +284,512 @@ __global__ __launch_bounds__(256) void k_blk_vp(long long m, T *V, T *P, const B
     }
 }
 
+// ----------------------------------------------- row panel kernels (s <= 8)
+//
+// A lane owns a row of the m x S row-major blocks: the products with the s x s
+// matrices (column-major, read with uniform addresses) run in FP64 with explicit
+// fma, results are rounded to T as stored, and the Gram matrix of the stored
+// result is accumulated per lane in FP64 (upper triangle, S(S+1)/2 values), then
+// reduced once per workgroup (wave shuffles, LDS across the 8 waves) and written
+// as a full s x s partial. Streaming kernels without LDS phases: for s <= 8 the
+// FP64 flops per row are far below what the HBM time of its 2-6 row vectors
+// allows. 512-thread workgroups (up to 256 VGPRs per lane: no spills of the FP64
+// Gram accumulators), at most FIN_DIRECT of them, so the fin kernel
+// sums the partials itself.
+#define RP_BS 512
+
+static inline int rp_grid(long long m) {
+    long long g = (m + RP_BS - 1) / RP_BS;
+    return (int)std::max(1LL, std::min(g, (long long)FIN_DIRECT));
+}
+
+// A row of S values of T in registers (vector loads of 16, 8 or sizeof(T) bytes).
+template <int S, typename T>
+struct RowRaw {
+    static constexpr int B = S * (int)sizeof(T);
+    static constexpr int NE = (B % 16 == 0) ? 16 / (int)sizeof(T) : (B % 8 == 0) ? 8 / (int)sizeof(T) : 1;
+    T v[S];
+    __device__ __forceinline__ void load(const T *__restrict__ p) {
+        if constexpr (NE == 1) {
+#pragma unroll
+            for (int j = 0; j < S; j++) v[j] = p[j];
+        } else {
+            typedef T vt __attribute__((ext_vector_type(NE)));
+#pragma unroll
+            for (int q = 0; q < S / NE; q++) {
+                const vt w = *(const vt *)(p + q * NE);
+#pragma unroll
+                for (int u = 0; u < NE; u++) v[q * NE + u] = w[u];
+            }
+        }
+    }
+    __device__ __forceinline__ void get(double (&r)[S]) const {
+#pragma unroll
+        for (int j = 0; j < S; j++) r[j] = (double)v[j];
+    }
+};
+
+template <int S, typename T>
+__device__ __forceinline__ void row_store(T *__restrict__ p, const double (&r)[S]) {
+    constexpr int NE = RowRaw<S, T>::NE;
+    if constexpr (NE == 1) {
+#pragma unroll
+        for (int j = 0; j < S; j++) p[j] = (T)r[j];
+    } else {
+        typedef T vt __attribute__((ext_vector_type(NE)));
+#pragma unroll
+        for (int q = 0; q < S / NE; q++) {
+            vt v;
+#pragma unroll
+            for (int u = 0; u < NE; u++) v[u] = (T)r[q * NE + u];
+            *(vt *)(p + q * NE) = v;
+        }
+    }
+}
+
+// o = r M (M column-major S x S), rounded to T when `round`.
+template <int S, typename T, bool ROUND>
+__device__ __forceinline__ void row_mul(const double (&r)[S], const double *__restrict__ M, double (&o)[S]) {
+#pragma unroll
+    for (int j = 0; j < S; j++) {
+        double a = 0.0;
+#pragma unroll
+        for (int k = 0; k < S; k++) a = fma(r[k], M[k + j * S], a);
+        o[j] = ROUND ? (double)(T)a : a;
+    }
+}
+
+template <int S>
+__device__ __forceinline__ void row_gram(const double (&x)[S], const double (&y)[S], double (&g)[S * (S + 1) / 2]) {
+    int q = 0;
+#pragma unroll
+    for (int i = 0; i < S; i++)
+#pragma unroll
+        for (int j = i; j < S; j++, q++) g[q] = fma(x[i], y[j], g[q]);
+}
+
+// Workgroup total of the per-lane upper-triangle Grams, written as the full
+// (mirrored) s x s partial of this workgroup.
+template <int S>
+__device__ __forceinline__ void row_gram_partial(double (&g)[S * (S + 1) / 2], double *part) {
+    constexpr int NG = S * (S + 1) / 2, NW = RP_BS / 64;
+    __shared__ double sh[NW][NG];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NG; q++) g[q] = wave_sum(g[q]);
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < NG; q++) sh[w][q] = g[q];
+    }
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < NG) {
+        double v = sh[0][t];
+        for (int k = 1; k < NW; k++) v += sh[k][t];
+        int i = 0, r = t;                      // t -> (i, j), i <= j, row-wise upper triangle
+        while (r >= S - i) { r -= S - i; i++; }
+        const int j = i + r;
+        double *pb = part + (size_t)blockIdx.x * S * S;
+        pb[i + j * S] = v;
+        pb[j + i * S] = v;
+    }
+}
+
+// Each lane's next row is loaded before the current one is processed (register
+// double buffering), so a wave keeps its loads in flight through the arithmetic.
+#define RP_LOOP_BEGIN                                                         \
+    const long long step = (long long)gridDim.x * RP_BS;                      \
+    long long row = (long long)blockIdx.x * RP_BS + threadIdx.x;
+
+// X += P W; V -= AP alpha; Gram V^T V of the new V.
+template <int S, typename T>
+__global__ __launch_bounds__(RP_BS) void k_blkr_xv(long long m, T *__restrict__ X, const T *__restrict__ P,
+                                                   T *__restrict__ V, const T *__restrict__ AP,
+                                                   const BlkMats *__restrict__ mt, const Ctl *c,
+                                                   double *__restrict__ part) {
+    if (ctl_stopped(c)) return;
+    double g[S * (S + 1) / 2];
+#pragma unroll
+    for (int q = 0; q < S * (S + 1) / 2; q++) g[q] = 0.0;
+    RP_LOOP_BEGIN
+    RowRaw<S, T> rp, rx, ra, rv;
+    if (row < m) { rp.load(P + row * S); rx.load(X + row * S); ra.load(AP + row * S); rv.load(V + row * S); }
+    for (; row < m; row += step) {
+        double p[S], x[S], a[S], v[S], d[S];
+        rp.get(p); rx.get(x); ra.get(a); rv.get(v);
+        const long long nr = row + step;
+        if (nr < m) { rp.load(P + nr * S); rx.load(X + nr * S); ra.load(AP + nr * S); rv.load(V + nr * S); }
+        row_mul<S, T, false>(p, mt->W, d);
+#pragma unroll
+        for (int j = 0; j < S; j++) x[j] += d[j];
+        row_store<S>(X + row * S, x);
+        row_mul<S, T, false>(a, mt->alpha, d);
+#pragma unroll
+        for (int j = 0; j < S; j++) v[j] = (double)(T)(v[j] - d[j]);
+        row_store<S>(V + row * S, v);
+        row_gram<S>(v, v, g);
+    }
+    row_gram_partial<S>(g, part);
+}
+
+// V = V R1^-1; Gram V^T V.
+template <int S, typename T>
+__global__ __launch_bounds__(RP_BS) void k_blkr_vr(long long m, T *__restrict__ V, const BlkMats *__restrict__ mt,
+                                                   const Ctl *c, double *__restrict__ part) {
+    if (ctl_stopped(c)) return;
+    double g[S * (S + 1) / 2];
+#pragma unroll
+    for (int q = 0; q < S * (S + 1) / 2; q++) g[q] = 0.0;
+    RP_LOOP_BEGIN
+    RowRaw<S, T> rv;
+    if (row < m) rv.load(V + row * S);
+    for (; row < m; row += step) {
+        double v[S], o[S];
+        rv.get(v);
+        const long long nr = row + step;
+        if (nr < m) rv.load(V + nr * S);
+        row_mul<S, T, true>(v, mt->R1i, o);
+        row_store<S>(V + row * S, o);
+        row_gram<S>(o, o, g);
+    }
+    row_gram_partial<S>(g, part);
+}
+
+// V = V R2^-1; P = V + P beta^T (or P = pscale V at the start).
+template <int S, typename T>
+__global__ __launch_bounds__(RP_BS) void k_blkr_vp(long long m, T *__restrict__ V, T *__restrict__ P,
+                                                   const BlkMats *__restrict__ mt, double pscale, const Ctl *c) {
+    if (ctl_stopped(c)) return;
+    const bool withp = (pscale == 0.0);
+    RP_LOOP_BEGIN
+    RowRaw<S, T> rv, rp;
+    if (row < m) { rv.load(V + row * S); if (withp) rp.load(P + row * S); }
+    for (; row < m; row += step) {
+        double v[S], o[S], p[S];
+        rv.get(v);
+        if (withp) rp.get(p);
+        const long long nr = row + step;
+        if (nr < m) { rv.load(V + nr * S); if (withp) rp.load(P + nr * S); }
+        row_mul<S, T, true>(v, mt->R2i, o);
+        row_store<S>(V + row * S, o);
+        if (withp) {
+            double d[S];
+            row_mul<S, T, false>(p, mt->betaT, d);
+#pragma unroll
+            for (int j = 0; j < S; j++) o[j] += d[j];
+        } else {
+#pragma unroll
+            for (int j = 0; j < S; j++) o[j] *= pscale;
+        }
+        row_store<S>(P + row * S, o);
+    }
+}
+
+// G = X^T Y (upper triangle, mirrored: X^T Y is symmetric where block CG uses it,
+// P^T A P and V^T V).
+template <int S, typename T>
+__global__ __launch_bounds__(RP_BS) void k_blkr_gram(long long m, const T *__restrict__ X, const T *__restrict__ Y,
+                                                     const Ctl *c, double *__restrict__ part) {
+    if (c && ctl_stopped(c)) return;
+    double g[S * (S + 1) / 2];
+#pragma unroll
+    for (int q = 0; q < S * (S + 1) / 2; q++) g[q] = 0.0;
+    const bool same = (X == Y);
+    RP_LOOP_BEGIN
+    RowRaw<S, T> rx, ry;
+    if (row < m) { rx.load(X + row * S); if (!same) ry.load(Y + row * S); }
+    for (; row < m; row += step) {
+        double x[S], y[S];
+        rx.get(x);
+        if (!same) ry.get(y);
+        const long long nr = row + step;
+        if (nr < m) { rx.load(X + nr * S); if (!same) ry.load(Y + nr * S); }
+        if (same) row_gram<S>(x, x, g);
+        else row_gram<S>(x, y, g);
+    }
+    row_gram_partial<S>(g, part);
+}
+
+// ------------------------------------ lane-pair row kernels (FP64, s = 8)
+//
+// Two lanes per row, each loading and storing one 32-byte half of it (a wave's
+// accesses then cover 32 whole rows; a lane per 64-byte FP64 row strides its 16-byte
+// loads across four times as many cache lines). The halves are swapped between the
+// pair with DPP, both lanes form the full-row products (uniform matrix reads), each
+// keeps its own columns, and each accumulates the Gram columns of its half against
+// the whole row: G[i][j], j in its half, all i (the mirrored entries are the same
+// products summed in the same order, so the partial is exactly symmetric).
+__device__ __forceinline__ double pair_swap(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0xB1, 0xF, 0xF, false);   // quad_perm(1,0,3,2)
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0xB1, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+template <int S>
+__device__ __forceinline__ void pair_full(const double (&mine)[S / 2], bool hi, double (&full)[S]) {
+#pragma unroll
+    for (int u = 0; u < S / 2; u++) {
+        const double o = pair_swap(mine[u]);
+        full[u] = hi ? o : mine[u];
+        full[S / 2 + u] = hi ? mine[u] : o;
+    }
+}
+
+template <int S>
+__device__ __forceinline__ void pair_mul(const double (&r)[S], const double *__restrict__ M, bool hi, bool round32,
+                                         double (&o)[S / 2]) {
+    double f[S];
+    row_mul<S, double, false>(r, M, f);
+#pragma unroll
+    for (int u = 0; u < S / 2; u++) {
+        const double v = hi ? f[S / 2 + u] : f[u];
+        o[u] = round32 ? (double)(float)v : v;
+    }
+}
+
+template <int S>
+__device__ __forceinline__ void pair_gram(const double (&full)[S], const double (&mine)[S / 2], double (&g)[S][S / 2]) {
+#pragma unroll
+    for (int i = 0; i < S; i++)
+#pragma unroll
+        for (int u = 0; u < S / 2; u++) g[i][u] = fma(full[i], mine[u], g[i][u]);
+}
+
+template <int S>
+__device__ __forceinline__ void pair_gram_partial(double (&g)[S][S / 2], bool hi, double *part) {
+    constexpr int NW = RP_BS / 64;
+    __shared__ double sh[NW][S * S];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < S; i++)
+#pragma unroll
+        for (int u = 0; u < S / 2; u++) {
+            double v = g[i][u];
+#pragma unroll
+            for (int o = 32; o >= 2; o >>= 1) v += __shfl_xor(v, o, 64);
+            g[i][u] = v;
+        }
+    if (lane < 2) {
+#pragma unroll
+        for (int i = 0; i < S; i++)
+#pragma unroll
+            for (int u = 0; u < S / 2; u++) sh[w][i + ((hi ? S / 2 : 0) + u) * S] = g[i][u];
+    }
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < S * S) {
+        double v = sh[0][t];
+        for (int k = 1; k < NW; k++) v += sh[k][t];
+        part[(size_t)blockIdx.x * S * S + t] = v;
+    }
+}
+
+#define RP2_LOOP_BEGIN                                                        \
+    const long long step = (long long)gridDim.x * (RP_BS / 2);                \
+    const bool hi = (threadIdx.x & 1) != 0;                                   \
+    const int off = hi ? S / 2 : 0;                                           \
+    long long row = (long long)blockIdx.x * (RP_BS / 2) + (threadIdx.x >> 1);
+
+template <int S>
+__global__ __launch_bounds__(RP_BS) void k_blkr2_xv(long long m, double *__restrict__ X, const double *__restrict__ P,
+                                                    double *__restrict__ V, const double *__restrict__ AP,
+                                                    const BlkMats *__restrict__ mt, const Ctl *c,
+                                                    double *__restrict__ part) {
+    if (ctl_stopped(c)) return;
+    double g[S][S / 2];
+#pragma unroll
+    for (int i = 0; i < S; i++)
+#pragma unroll
+        for (int u = 0; u < S / 2; u++) g[i][u] = 0.0;
+    RP2_LOOP_BEGIN
+    RowRaw<S / 2, double> rp, rx, ra, rv;
+    if (row < m) {
+        rp.load(P + row * S + off); rx.load(X + row * S + off); ra.load(AP + row * S + off); rv.load(V + row * S + off);
+    }
+    for (; row < m; row += step) {
+        double p[S / 2], x[S / 2], a[S / 2], v[S / 2], pf[S], af[S], vf[S], d[S / 2];
+        rp.get(p); rx.get(x); ra.get(a); rv.get(v);
+        const long long nr = row + step;
+        if (nr < m) {
+            rp.load(P + nr * S + off); rx.load(X + nr * S + off); ra.load(AP + nr * S + off); rv.load(V + nr * S + off);
+        }
+        pair_full<S>(p, hi, pf);
+        pair_mul<S>(pf, mt->W, hi, false, d);
+#pragma unroll
+        for (int u = 0; u < S / 2; u++) x[u] += d[u];
+        row_store<S / 2>(X + row * S + off, x);
+        pair_full<S>(a, hi, af);
+        pair_mul<S>(af, mt->alpha, hi, false, d);
+#pragma unroll
+        for (int u = 0; u < S / 2; u++) v[u] = v[u] - d[u];
+        row_store<S / 2>(V + row * S + off, v);
+        pair_full<S>(v, hi, vf);
+        pair_gram<S>(vf, v, g);
+    }
+    pair_gram_partial<S>(g, hi, part);
+}
+
+template <int S>
+__global__ __launch_bounds__(RP_BS) void k_blkr2_vr(long long m, double *__restrict__ V,
+                                                    const BlkMats *__restrict__ mt, const Ctl *c,
+                                                    double *__restrict__ part) {
+    if (ctl_stopped(c)) return;
+    double g[S][S / 2];
+#pragma unroll
+    for (int i = 0; i < S; i++)
+#pragma unroll
+        for (int u = 0; u < S / 2; u++) g[i][u] = 0.0;
+    RP2_LOOP_BEGIN
+    RowRaw<S / 2, double> rv;
+    if (row < m) rv.load(V + row * S + off);
+    for (; row < m; row += step) {
+        double v[S / 2], vf[S], o[S / 2], of[S];
+        rv.get(v);
+        const long long nr = row + step;
+        if (nr < m) rv.load(V + nr * S + off);
+        pair_full<S>(v, hi, vf);
+        pair_mul<S>(vf, mt->R1i, hi, false, o);
+        row_store<S / 2>(V + row * S + off, o);
+        pair_full<S>(o, hi, of);
+        pair_gram<S>(of, o, g);
+    }
+    pair_gram_partial<S>(g, hi, part);
+}
+
+template <int S>
+__global__ __launch_bounds__(RP_BS) void k_blkr2_vp(long long m, double *__restrict__ V, double *__restrict__ P,
+                                                    const BlkMats *__restrict__ mt, double pscale, const Ctl *c) {
+    if (ctl_stopped(c)) return;
+    const bool withp = (pscale == 0.0);
+    RP2_LOOP_BEGIN
+    RowRaw<S / 2, double> rv, rp;
+    if (row < m) { rv.load(V + row * S + off); if (withp) rp.load(P + row * S + off); }
+    for (; row < m; row += step) {
+        double v[S / 2], vf[S], o[S / 2], p[S / 2];
+        rv.get(v);
+        if (withp) rp.get(p);
+        const long long nr = row + step;
+        if (nr < m) { rv.load(V + nr * S + off); if (withp) rp.load(P + nr * S + off); }
+        pair_full<S>(v, hi, vf);
+        pair_mul<S>(vf, mt->R2i, hi, false, o);
+        row_store<S / 2>(V + row * S + off, o);
+        if (withp) {
+            double pf[S], d[S / 2];
+            pair_full<S>(p, hi, pf);
+            pair_mul<S>(pf, mt->betaT, hi, false, d);
+#pragma unroll
+            for (int u = 0; u < S / 2; u++) o[u] += d[u];
+        } else {
+#pragma unroll
+            for (int u = 0; u < S / 2; u++) o[u] *= pscale;
+        }
+        row_store<S / 2>(P + row * S + off, o);
+    }
+}
+
+template <int S>
+__global__ __launch_bounds__(RP_BS) void k_blkr2_gram(long long m, const double *__restrict__ X,
+                                                      const double *__restrict__ Y, const Ctl *c,
+                                                      double *__restrict__ part) {
+    if (c && ctl_stopped(c)) return;
+    double g[S][S / 2];
+#pragma unroll
+    for (int i = 0; i < S; i++)
+#pragma unroll
+        for (int u = 0; u < S / 2; u++) g[i][u] = 0.0;
+    const bool same = (X == Y);
+    RP2_LOOP_BEGIN
+    RowRaw<S / 2, double> rx, ry;
+    if (row < m) { rx.load(X + row * S + off); if (!same) ry.load(Y + row * S + off); }
+    for (; row < m; row += step) {
+        double x[S / 2], y[S / 2], xf[S];
+        rx.get(x);
+        if (!same) ry.get(y);
+        const long long nr = row + step;
+        if (nr < m) { rx.load(X + nr * S + off); if (!same) ry.load(Y + nr * S + off); }
+        pair_full<S>(x, hi, xf);
+        if (same) pair_gram<S>(xf, x, g);
+        else pair_gram<S>(xf, y, g);
+    }
+    pair_gram_partial<S>(g, hi, part);
+}
+
+// Launchers: lane-pair kernels for FP64 s = 8, row kernels for other s <= 8,
+// LDS-tile MFMA kernels above; each returns the number of partial rows it wrote
+// (for launch_blk_fin).
+template <typename T, int S>
+constexpr bool panel_pair() { return S == 8 && sizeof(T) == sizeof(double); }
+
+static inline int rp2_grid(long long m) {
+    return (int)std::max(1LL, std::min((m + RP_BS / 2 - 1) / (RP_BS / 2), (long long)FIN_DIRECT));
+}
+
+template <int S, typename T>
+static int panel_gram(hipStream_t s, long long m, const T *X, const T *Y, const Ctl *c, double *part) {
+    if constexpr (panel_pair<T, S>()) {
+        const int G = rp2_grid(m);
+        hipLaunchKernelGGL((k_blkr2_gram<S>), dim3(G), dim3(RP_BS), 0, s, m, X, Y, c, part);
+        return G;
+    } else if constexpr (S <= 8) {
+        const int G = rp_grid(m);
+        hipLaunchKernelGGL((k_blkr_gram<S, T>), dim3(G), dim3(RP_BS), 0, s, m, X, Y, c, part);
+        return G;
+    } else {
+        const int G = block_grid(m);
+        hipLaunchKernelGGL((k_gram_st<S, T>), dim3(G), dim3(256), 0, s, m, X, Y, c, part);
+        return G;
+    }
+}
+
+template <int S, typename T>
+static int panel_xv(hipStream_t s, long long m, T *X, const T *P, T *V, const T *AP, const BlkMats *mt,
+                    const Ctl *c, double *part) {
+    if constexpr (panel_pair<T, S>()) {
+        const int G = rp2_grid(m);
+        hipLaunchKernelGGL((k_blkr2_xv<S>), dim3(G), dim3(RP_BS), 0, s, m, X, P, V, AP, mt, c, part);
+        return G;
+    } else if constexpr (S <= 8) {
+        const int G = rp_grid(m);
+        hipLaunchKernelGGL((k_blkr_xv<S, T>), dim3(G), dim3(RP_BS), 0, s, m, X, P, V, AP, mt, c, part);
+        return G;
+    } else {
+        const int G = block_grid(m);
+        hipLaunchKernelGGL((k_blk_xv<S, T>), dim3(G), dim3(256), 0, s, m, X, P, V, AP, mt, c, part);
+        return G;
+    }
+}
+
+template <int S, typename T>
+static int panel_vr(hipStream_t s, long long m, T *V, const BlkMats *mt, const Ctl *c, double *part) {
+    if constexpr (panel_pair<T, S>()) {
+        const int G = rp2_grid(m);
+        hipLaunchKernelGGL((k_blkr2_vr<S>), dim3(G), dim3(RP_BS), 0, s, m, V, mt, c, part);
+        return G;
+    } else if constexpr (S <= 8) {
+        const int G = rp_grid(m);
+        hipLaunchKernelGGL((k_blkr_vr<S, T>), dim3(G), dim3(RP_BS), 0, s, m, V, mt, c, part);
+        return G;
+    } else {
+        const int G = block_grid(m);
+        hipLaunchKernelGGL((k_blk_vr<S, T>), dim3(G), dim3(256), 0, s, m, V, mt, c, part);
+        return G;
+    }
+}
+
+template <int S, typename T>
+static void panel_vp(hipStream_t s, long long m, T *V, T *P, const BlkMats *mt, double pscale, const Ctl *c) {
+    // no partials: up to 4 FIN_DIRECT workgroups
+    if constexpr (panel_pair<T, S>()) {
+        const int G = (int)std::max(1LL, std::min((m + RP_BS / 2 - 1) / (RP_BS / 2), 4LL * FIN_DIRECT));
+        hipLaunchKernelGGL((k_blkr2_vp<S>), dim3(G), dim3(RP_BS), 0, s, m, V, P, mt, pscale, c);
+    } else if constexpr (S <= 8) {
+        const int G = (int)std::max(1LL, std::min((m + RP_BS - 1) / RP_BS, 4LL * FIN_DIRECT));
+        hipLaunchKernelGGL((k_blkr_vp<S, T>), dim3(G), dim3(RP_BS), 0, s, m, V, P, mt, pscale, c);
+    } else {
+        hipLaunchKernelGGL((k_blk_vp<S, T>), dim3(block_grid(m)), dim3(256), 0, s, m, V, P, mt, pscale, c);
+    }
+}
+
 // Y = alpha A X + beta Y: the SELL-64 kernel when the matrix has that copy in the
 // values' precision, else CSR-stream.
 template <int S, typename VT, typename T>
@@ -270,6 +801,19 @@ static void launch_spmm(hipStream_t s, const bkh_matrix *A, const VT *val, const
         else sv = (val == (const VT *)A->val) ? (const VT *)A->sval : nullptr;
     }
     if (sv && (A->scd || A->scol)) {
+        constexpr int LPR = spmm_lpr<S, T>();
+        if constexpr (LPR > 0) {
+            if ((((uintptr_t)X | (uintptr_t)Y) & 15) == 0) {
+                const int G = (int)((A->n + 256 / LPR - 1) / (256 / LPR));
+                if (A->scd)
+                    hipLaunchKernelGGL((k_spmm_sell_v<S, VT, T, short>), dim3(G), dim3(256), 0, s, A->n, A->rp,
+                                       A->sso, sv, (const short *)A->scd, X, Y, alpha, beta, c);
+                else
+                    hipLaunchKernelGGL((k_spmm_sell_v<S, VT, T, int>), dim3(G), dim3(256), 0, s, A->n, A->rp,
+                                       A->sso, sv, (const int *)A->scol, X, Y, alpha, beta, c);
+                return;
+            }
+        }
         const int G = (A->n + 255) / 256;
         if (A->scd)
             hipLaunchKernelGGL((k_spmm_sell<S, VT, T, short>), dim3(G), dim3(256), 0, s, A->n, A->rp, A->sso, sv,
@@ -290,7 +834,7 @@ static int blcg_core(ThreadCtx *t, const bkh_matrix *A, const VT *val, const T *
                      int p0_mode, int K) {
     hipStream_t s = t->stream;
     const long long m = A->n;
-    const int G = block_grid(m), Gv = vec_grid(m * S, 256);
+    const int Gv = vec_grid(m * S, 256);
     if (ctl_init(t, c, tol, max_it)) return BKH_DEVICE_ERROR;
     BlkMats h;
     memset(&h, 0, sizeof h);
@@ -301,26 +845,19 @@ static int blcg_core(ThreadCtx *t, const bkh_matrix *A, const VT *val, const T *
     launch_spmm<S>(s, A, val, (const T *)Xt, AP, 1.0, 0.0, c);
     hipLaunchKernelGGL(k_blk_resid<T>, dim3(Gv), dim3(256), 0, s, m * S, Bt, (const T *)AP, V, part);
     launch_blk_fin<S, FIN_FNB>(s, part, Gv, mt, c);
-    hipLaunchKernelGGL((k_gram_st<S, T>), dim3(G), dim3(256), 0, s, m, (const T *)V, (const T *)V, c, part);
-    launch_blk_fin<S, FIN_QR1>(s, part, G, mt, c);
-    hipLaunchKernelGGL((k_blk_vr<S, T>), dim3(G), dim3(256), 0, s, m, V, mt, c, part);
-    launch_blk_fin<S, FIN_QR2_INIT>(s, part, G, mt, c);
+    launch_blk_fin<S, FIN_QR1>(s, part, panel_gram<S, T>(s, m, V, V, c, part), mt, c);
+    launch_blk_fin<S, FIN_QR2_INIT>(s, part, panel_vr<S, T>(s, m, V, mt, c, part), mt, c);
     // P = V (then P = V + P beta^T with beta = I: P0 = 2V, blcg.c:59-71)
-    hipLaunchKernelGGL((k_blk_vp<S, T>), dim3(G), dim3(256), 0, s, m, V, P, mt,
-                       p0_mode ? 1.0 : 2.0, c);
+    panel_vp<S, T>(s, m, V, P, mt, p0_mode ? 1.0 : 2.0, c);
     LAUNCH_OK();
     hipGraphExec_t ge = capture_cached(t, {0xB1C6, (uintptr_t)A, (uintptr_t)val, (uintptr_t)Bt, (uintptr_t)Xt, (uintptr_t)V,
                                            (uintptr_t)P, (uintptr_t)AP, (uintptr_t)mt, (uintptr_t)c, (uintptr_t)part,
                                            (uintptr_t)S, (uintptr_t)sizeof(T)}, s, K, [&]() {
         launch_spmm<S>(s, A, val, (const T *)P, AP, 1.0, 0.0, c);
-        hipLaunchKernelGGL((k_gram_st<S, T>), dim3(G), dim3(256), 0, s, m, (const T *)P, (const T *)AP, c, part);
-        launch_blk_fin<S, FIN_ALPHA>(s, part, G, mt, c);
-        hipLaunchKernelGGL((k_blk_xv<S, T>), dim3(G), dim3(256), 0, s, m, Xt, (const T *)P, V,
-                           (const T *)AP, mt, c, part);
-        launch_blk_fin<S, FIN_QR1>(s, part, G, mt, c);
-        hipLaunchKernelGGL((k_blk_vr<S, T>), dim3(G), dim3(256), 0, s, m, V, mt, c, part);
-        launch_blk_fin<S, FIN_QR2_ITER>(s, part, G, mt, c);
-        hipLaunchKernelGGL((k_blk_vp<S, T>), dim3(G), dim3(256), 0, s, m, V, P, mt, 0.0, c);
+        launch_blk_fin<S, FIN_ALPHA>(s, part, panel_gram<S, T>(s, m, P, AP, c, part), mt, c);
+        launch_blk_fin<S, FIN_QR1>(s, part, panel_xv<S, T>(s, m, Xt, P, V, AP, mt, c, part), mt, c);
+        launch_blk_fin<S, FIN_QR2_ITER>(s, part, panel_vr<S, T>(s, m, V, mt, c, part), mt, c);
+        panel_vp<S, T>(s, m, V, P, mt, 0.0, c);
     });
     if (!ge) { bkh_set_error("hipGraph capture of the block CG iteration failed"); return BKH_DEVICE_ERROR; }
     int rc = run_graph_until_done(t, ge, c, max_it / K + 2);
@@ -513,15 +1050,10 @@ static int blcg_ru(ThreadCtx *t, bkh_matrix *A, const double *B, double *X, doub
                                            (uintptr_t)AP32, (uintptr_t)mt, (uintptr_t)c, (uintptr_t)part,
                                            (uintptr_t)S}, s, K, [&]() {
         launch_spmm<S>(s, A, val, (const float *)P32, AP32, 1.0, 0.0, c);
-        hipLaunchKernelGGL((k_gram_st<S, float>), dim3(G), dim3(256), 0, s, m, (const float *)P32, (const float *)AP32,
-                           c, part);
-        launch_blk_fin<S, FIN_ALPHA>(s, part, G, mt, c);
-        hipLaunchKernelGGL((k_blk_xv<S, float>), dim3(G), dim3(256), 0, s, m, X32, (const float *)P32, V32,
-                           (const float *)AP32, mt, c, part);
-        launch_blk_fin<S, FIN_QR1>(s, part, G, mt, c);
-        hipLaunchKernelGGL((k_blk_vr<S, float>), dim3(G), dim3(256), 0, s, m, V32, mt, c, part);
-        launch_blk_fin<S, FIN_QR2_ITER>(s, part, G, mt, c);
-        hipLaunchKernelGGL((k_blk_vp<S, float>), dim3(G), dim3(256), 0, s, m, V32, P32, mt, 0.0, c);
+        launch_blk_fin<S, FIN_ALPHA>(s, part, panel_gram<S, float>(s, m, P32, AP32, c, part), mt, c);
+        launch_blk_fin<S, FIN_QR1>(s, part, panel_xv<S, float>(s, m, X32, P32, V32, AP32, mt, c, part), mt, c);
+        launch_blk_fin<S, FIN_QR2_ITER>(s, part, panel_vr<S, float>(s, m, V32, mt, c, part), mt, c);
+        panel_vp<S, float>(s, m, V32, P32, mt, 0.0, c);
     });
     if (!ge) { bkh_set_error("hipGraph capture of the FP32 block CG iteration failed"); return BKH_DEVICE_ERROR; }
     int passes = 0, st = ST_MAXIT, used = 0;
@@ -549,13 +1081,9 @@ static int blcg_ru(ThreadCtx *t, bkh_matrix *A, const double *B, double *X, doub
         // P = V or 2V as the FP64 path's first direction)
         hipLaunchKernelGGL(k_blk_resume, dim3(1), dim3(64), 0, s, c, std::max(0.5 * tol, delta * rel));
         if (passes == 0) hipLaunchKernelGGL(k_blk_setfnb, dim3(1), dim3(64), 0, s, c, (const Ctl *)co);
-        hipLaunchKernelGGL((k_gram_st<S, float>), dim3(G), dim3(256), 0, s, m, (const float *)V32, (const float *)V32,
-                           c, part);
-        launch_blk_fin<S, FIN_QR1>(s, part, G, mt, c);
-        hipLaunchKernelGGL((k_blk_vr<S, float>), dim3(G), dim3(256), 0, s, m, V32, mt, c, part);
-        launch_blk_fin<S, FIN_QR2_INIT>(s, part, G, mt, c);
-        hipLaunchKernelGGL((k_blk_vp<S, float>), dim3(G), dim3(256), 0, s, m, V32, P32, mt,
-                           passes == 0 ? (o.fix_blcg_p0 ? 1.0 : 2.0) : 0.0, c);
+        launch_blk_fin<S, FIN_QR1>(s, part, panel_gram<S, float>(s, m, V32, V32, c, part), mt, c);
+        launch_blk_fin<S, FIN_QR2_INIT>(s, part, panel_vr<S, float>(s, m, V32, mt, c, part), mt, c);
+        panel_vp<S, float>(s, m, V32, P32, mt, passes == 0 ? (o.fix_blcg_p0 ? 1.0 : 2.0) : 0.0, c);
         LAUNCH_OK();
         if (run_graph_until_done(t, ge, c, (max_iter - used) / K + 2)) return BKH_DEVICE_ERROR;
         passes++;

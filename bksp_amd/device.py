@@ -114,6 +114,12 @@ class Matrix:
         if rc:
             raise BkspDeviceError(f"bkh_spmv -> {rc}: {last_error()}")
 
+    def spmm(self, s, d_X, d_Y, alpha=1.0, beta=0.0):
+        """Y = alpha A X + beta Y for s column-major columns of n entries."""
+        rc = load().bkh_spmm(self.handle, s, alpha, d_X.ptr, beta, d_Y.ptr)
+        if rc:
+            raise BkspDeviceError(f"bkh_spmm -> {rc}: {last_error()}")
+
     def spmv_bytes(self, fp32=False):
         """Bytes one SpMV with the current kernel moves (format bytes)."""
         return load().bkh_matrix_spmv_bytes(self.handle, int(fp32))
