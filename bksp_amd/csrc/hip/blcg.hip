@@ -35,19 +35,34 @@ __global__ __launch_bounds__(256) void k_sh_fill_f(float *p, long long m) {
     for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < m; e += (long long)gridDim.x * 256) p[e] = 0.0f;
 }
 
+// Partial rows the fin kernel sums itself; more go through k_part_reduce first.
+#define FIN_DIRECT 256
+
 // Fixed-order sum of nblocks rows of E partials by a 256-thread workgroup: thread
-// t < Q*E (Q = 256/E groups) sums rows q, q+Q, q+2Q, ... of entry e = t % E, q = t / E
-// (independent loads in flight), then the Q group sums are added in order.
+// t < Q*E (Q = 256/E groups) sums rows q, q+Q, q+2Q, ... of entry e = t % E, q = t / E,
+// then the Q group sums are added in order. The rows of a group are loaded RMAX at
+// a time into registers before they are added, so their loads (from another XCD's
+// writes, i.e. from memory) are in flight together instead of in dependent batches.
 // red: Q*E <= 256 doubles of LDS; out[e], e < E, valid in every thread on return.
 template <int E>
 __device__ __forceinline__ void fin_sum_parts(const double *part, int nblocks, double *red, double *out) {
     constexpr int Q = 256 / E;
+    constexpr int RMAX = (FIN_DIRECT + Q - 1) / Q < 64 ? (FIN_DIRECT + Q - 1) / Q : 64;
     const int t = threadIdx.x;
     if (t < Q * E) {
         const int e = t % E;
         double acc = 0.0;
-#pragma unroll 8
-        for (int b = t / E; b < nblocks; b += Q) acc += part[(size_t)b * E + e];
+        for (int b0 = t / E; b0 < nblocks; b0 += RMAX * Q) {
+            double v[RMAX];
+#pragma unroll
+            for (int u = 0; u < RMAX; u++) {
+                const int b = b0 + u * Q;
+                v[u] = b < nblocks ? part[(size_t)b * E + e] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < RMAX; u++)
+                if (b0 + u * Q < nblocks) acc += v[u];
+        }
         red[t] = acc;
     }
     __syncthreads();
@@ -123,9 +138,6 @@ __global__ __launch_bounds__(256) void k_blk_fin(const double *part, int nblocks
     const double f2 = sm_fro2(S, R, &red);
     if (t == 0) ctl_check(c, f2);
 }
-
-// Partial rows the fin kernel sums itself; more go through k_part_reduce first.
-#define FIN_DIRECT 256
 
 template <int S, int OP>
 static void launch_blk_fin(hipStream_t s, const double *part, int nblocks, BlkMats *mt, Ctl *c) {

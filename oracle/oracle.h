@@ -110,6 +110,25 @@ void   orc_fsai_backward(int n, const int *rpl, const int *cil, const double *vl
                          const double *excess, const float *params, int bf16, const double *h,
                          const double *dg, double *grad);
 
+/* ---- complex FP64 family (orc_complex.c); complex values are C99 double complex
+ *      (interleaved re, im pairs), passed as double* in this header (orc_complex.c
+ *      itself defines them on double complex and skips these declarations). ---- */
+#ifndef ORC_COMPLEX_IMPL
+void   orc_zdotc_sub(int n, const double *x, const double *y, double *out);
+double orc_dznrm2(int n, const double *x);
+int    orc_zcsrgemm0(char type, int m, int n, const double *A, const int *Ai, const int *Aj,
+                     const double *B, double *C);
+int    orc_zcsrgemm1(char type, const double *alpha, int m, int n, const double *A, const int *Ai,
+                     const int *Aj, const double *B, const double *beta, double *C);
+int    orc_zhorner(const double *gamma, int l, const double *sigma, double *out);
+int    orc_zbicg(int n, const int *rp, const int *ci, const double *v, const double *b,
+                 double tol, int max_iter, double *x);
+int    orc_zbicgstabl(int n, const int *rp, const int *ci, const double *v, const double *b,
+                      int l, double tol, int max_iter, double *x);
+int    orc_zshbicgstabl(int n, const int *rp, const int *ci, const double *v, const double *b,
+                        int l, const double *sigma, int nsigma, double tol, int max_iter, double *x);
+#endif
+
 /* Time one call of a solver in seconds (CLOCK_MONOTONIC), for the CPU baseline. */
 double orc_wall_seconds(void);
 

@@ -35,7 +35,7 @@ def lib():
         if not os.path.exists(ORACLE_SO):
             raise RuntimeError("oracle/liboracle.so missing: run `make -C oracle`")
         L = ctypes.CDLL(ORACLE_SO)
-        for name in ("orc_ddot", "orc_dnrm2", "orc_sp_norm_f", "orc_wall_seconds"):
+        for name in ("orc_ddot", "orc_dnrm2", "orc_sp_norm_f", "orc_wall_seconds", "orc_dznrm2"):
             getattr(L, name).restype = c_dbl
         _o = L
     return _o
@@ -176,6 +176,84 @@ def wall():
     return lib().orc_wall_seconds()
 
 
+# ---------------------------------------------------------- complex family
+
+def _zp(a):
+    """Pointer to a complex128 array's interleaved (re, im) doubles."""
+    return a.ctypes.data_as(DP)
+
+
+def _zcsr(rp, ci, v):
+    return (np.ascontiguousarray(rp, np.int32), np.ascontiguousarray(ci, np.int32),
+            np.ascontiguousarray(v, np.complex128))
+
+
+def zdotc(x, y):
+    x = np.ascontiguousarray(x, np.complex128); y = np.ascontiguousarray(y, np.complex128)
+    out = np.zeros(1, np.complex128)
+    lib().orc_zdotc_sub(len(x), _zp(x), _zp(y), _zp(out))
+    return complex(out[0])
+
+
+def dznrm2(x):
+    x = np.ascontiguousarray(x, np.complex128)
+    return lib().orc_dznrm2(len(x), _zp(x))
+
+
+def zcsrgemm0(trans, n, rp, ci, v, X):
+    """C = op(A) X for X of n x s (column-major), op 'n' or 't' (conjugate transpose)."""
+    rp, ci, v = _zcsr(rp, ci, v)
+    X = np.asfortranarray(X, np.complex128).reshape(n, -1, order="F")
+    C = np.zeros(X.shape, np.complex128, order="F")
+    lib().orc_zcsrgemm0(c_char(trans.encode()), n, X.shape[1], _zp(v), _ip(rp), _ip(ci), _zp(X), _zp(C))
+    return C
+
+
+def zcsrgemm1(trans, alpha, n, rp, ci, v, X, beta, C):
+    """C = beta C + alpha op(A) X with 0-based CSR in (shifted to 1-based here)."""
+    rp, ci, v = _zcsr(rp, ci, v)
+    rp1, ci1 = rp + 1, ci + 1
+    X = np.asfortranarray(X, np.complex128).reshape(n, -1, order="F")
+    C = np.array(C, np.complex128, order="F", copy=True).reshape(n, -1, order="F")
+    al, be = np.array([alpha], np.complex128), np.array([beta], np.complex128)
+    lib().orc_zcsrgemm1(c_char(trans.encode()), _zp(al), n, X.shape[1], _zp(v), _ip(rp1), _ip(ci1), _zp(X),
+                        _zp(be), _zp(C))
+    return C
+
+
+def zhorner(gamma, l, sigma):
+    g = np.ascontiguousarray(gamma, np.complex128)
+    out = np.zeros(l + 1, np.complex128)
+    sg = np.array([sigma], np.complex128)
+    lib().orc_zhorner(_zp(g), l, _zp(sg), _zp(out))
+    return out
+
+
+def _zsolve(fn, n, rp, ci, v, b, x0, *args):
+    rp, ci, v = _zcsr(rp, ci, v)
+    b = np.ascontiguousarray(b, np.complex128)
+    x = np.array(x0 if x0 is not None else np.zeros(len(b)), dtype=np.complex128, copy=True)
+    ret = fn(n, _ip(rp), _ip(ci), _zp(v), _zp(b), *args, _zp(x))
+    return ret, x
+
+
+def zbicg(n, rp, ci, v, b, tol, max_iter, x0=None):
+    return _zsolve(lib().orc_zbicg, n, rp, ci, v, b, x0, c_dbl(tol), max_iter)
+
+
+def zbicgstabl(n, rp, ci, v, b, l, tol, max_iter, x0=None):
+    return _zsolve(lib().orc_zbicgstabl, n, rp, ci, v, b, x0, l, c_dbl(tol), max_iter)
+
+
+def zshbicgstabl(n, rp, ci, v, b, l, sigma, tol, max_iter):
+    rp, ci, v = _zcsr(rp, ci, v)
+    sig = np.ascontiguousarray(sigma, np.complex128)
+    x = np.zeros(n * (len(sig) + 1), np.complex128)
+    ret = lib().orc_zshbicgstabl(n, _ip(rp), _ip(ci), _zp(v), _zp(np.ascontiguousarray(b, np.complex128)),
+                                 l, _zp(sig), len(sig), c_dbl(tol), max_iter, _zp(x))
+    return ret, x.reshape(len(sig) + 1, n)
+
+
 # ------------------------------------------------------ reference kernels
 
 def ref_csrgemv(trans, n, rp, ci, v, x):
@@ -208,6 +286,29 @@ def ref_csrgemm1(n, s, alpha, rp1, ci1, v, B, beta, C):
     C = np.array(C, dtype=np.float64, order="F", copy=True)
     R.dcsrgemm1(c_char(b"n"), c_dbl(alpha), n, s, _dp(v), _ip(rp1), _ip(ci1), int(rp1[-1] - 1),
                 _dp(B), c_dbl(beta), _dp(C))
+    return C
+
+
+def ref_zcsrgemm0(trans, n, rp, ci, v, X):
+    """The reference's zcsrgemm0 (src/matvec.c:55-98) on 0-based CSR, X n x s column-major."""
+    R = ref()
+    rp, ci, v = _zcsr(rp, ci, v)
+    X = np.asfortranarray(X, np.complex128).reshape(n, -1, order="F")
+    C = np.zeros(X.shape, np.complex128, order="F")
+    R.zcsrgemm0(c_char(trans.encode()), n, X.shape[1], _zp(v), _ip(rp), _ip(ci), int(rp[-1]), _zp(X), _zp(C))
+    return C
+
+
+def ref_zcsrgemm1(trans, alpha, n, rp, ci, v, X, beta, C):
+    """The reference's zcsrgemm1 (src/matvec.c:174-217); 0-based CSR in, 1-based passed."""
+    R = ref()
+    rp, ci, v = _zcsr(rp, ci, v)
+    rp1, ci1 = rp + 1, ci + 1
+    X = np.asfortranarray(X, np.complex128).reshape(n, -1, order="F")
+    C = np.array(C, np.complex128, order="F", copy=True).reshape(n, -1, order="F")
+    al, be = np.array([alpha], np.complex128), np.array([beta], np.complex128)
+    R.zcsrgemm1(c_char(trans.encode()), _zp(al), n, X.shape[1], _zp(v), _ip(rp1), _ip(ci1), int(rp[-1]), _zp(X),
+                _zp(be), _zp(C))
     return C
 
 

@@ -58,3 +58,24 @@ def expand_lower(n, rp, ci, v):
 def to_scipy(n, rp, ci, v):
     import scipy.sparse as sp
     return sp.csr_matrix((v, ci, rp), shape=(n, n))
+
+
+def zsystem(m, seed=0, delta=0.3, offim=0.2, vx=10.0, vy=20.0):
+    """A complex nonsymmetric test system: the m x m 2D convection-diffusion
+    operator with an imaginary diagonal shift i*delta and seeded imaginary parts
+    (|.| <= offim times the real part) on its off-diagonal entries.
+    Returns (n, row_ptr, col, complex val)."""
+    from bksp_amd import gen
+    n, rp, ci, v = gen.convdiff2d(m, vx, vy)
+    rng = np.random.default_rng(seed)
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    z = v.astype(np.complex128)
+    off = ci != rows
+    z[off] += 1j * offim * v[off] * rng.uniform(-1.0, 1.0, int(off.sum()))
+    z[~off] += 1j * delta * np.abs(v[~off])
+    return n, rp.astype(np.int32), ci.astype(np.int32), z
+
+
+def zcsr_dense(n, rp, ci, v):
+    import scipy.sparse as sp
+    return sp.csr_matrix((v, ci, rp), shape=(n, n))
