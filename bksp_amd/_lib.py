@@ -66,6 +66,13 @@ _SIGS = {
     "bkh_matrix_set_spmv": (c_int, [c_vp, c_int]),
     "bkh_spmv": (c_int, [c_vp, ctypes.c_char, c_dbl, c_vp, c_dbl, c_vp]),
     "bkh_spmm": (c_int, [c_vp, c_int, c_dbl, c_vp, c_dbl, c_vp]),
+    "bkh_zmatrix_create": (c_vp, [c_int, c_vp, c_vp, c_vp]),
+    "bkh_zmatrix_destroy": (None, [c_vp]),
+    "bkh_zspmv": (c_int, [c_vp, ctypes.c_char, c_vp, c_vp]),
+    "bkh_zspmm1": (c_int, [c_vp, ctypes.c_char, c_vp, c_int, c_vp, c_vp, c_vp]),
+    "bkh_zbicg": (c_int, [c_vp, c_vp, c_vp, c_dbl, c_int, c_vp]),
+    "bkh_zbicgstabl": (c_int, [c_vp, c_vp, c_int, c_vp, c_dbl, c_int, c_vp]),
+    "bkh_zshbicgstabl": (c_int, [c_vp, c_vp, c_int, c_dp, c_int, c_vp, c_dbl, c_int, c_vp, c_dp]),
     "bkh_spmv32": (c_int, [c_vp, c_vp, c_vp]),
     "bkh_dot": (c_int, [c_ll, c_vp, c_vp, c_dp]),
     "bkh_qr": (c_int, [c_int, c_int, c_vp, c_dp]),

@@ -1,0 +1,1058 @@
+// zkrylov.hip -- the complex FP64 family on the device:
+//   complex CSR products  y = A x, y = A^H x      (replace src/matvec.c:55-98, 174-217)
+//   BiCG                                          (replaces src/bicg.c:91-177)
+//   BiCGSTAB(l)                                   (replaces src/bicgstabl.c:141-276)
+//   shifted BiCGSTAB(l), complex shifts           (replaces src/shbicgstabl.c:288-592)
+//
+// Complex values are double2 (re, im), the layout of C99 double _Complex, so host
+// arrays of the reference's zspmat / double _Complex vectors upload as they are.
+// Products follow the reference's C99 arithmetic for finite operands (separate
+// multiply and add: re = ac - bd, im = ad + bc); every row of A x and A^H x sums its
+// entries in the reference's order (A^H through a transposed copy whose rows list
+// the scatter's contributions in source-row order), so the products are bitwise
+// equal to src/matvec.c. Scalar divisions use Smith's algorithm.
+//
+// The solvers follow the reference's BLAS call sequence with the vector operations
+// of one step fused into one kernel, every scalar kept on the device (Ctl::s and a
+// per-shift array) and one outer iteration captured as a hipGraph. Dot products are
+// deterministic (fixed-order partials per workgroup, summed by one workgroup);
+// dznrm2 is the square root of that sum of |.|^2.
+//
+// The reference's zbicgstabl passes a real `one` where cblas_zaxpy reads a complex
+// alpha (bicgstabl.c:147, :177, :194); this build uses the intended 1 + 0i.
+#include <math.h>
+#include <vector>
+#include "bkh_host.hpp"
+#include "bkh_solver_common.hpp"
+
+typedef double2 zd;
+
+__host__ __device__ __forceinline__ zd zmk(double r, double i) { return make_double2(r, i); }
+__device__ __forceinline__ zd zmul(zd a, zd b) {
+    const double ac = a.x * b.x, bd = a.y * b.y, ad = a.x * b.y, bc = a.y * b.x;
+    return zmk(ac - bd, ad + bc);
+}
+__device__ __forceinline__ zd zadd(zd a, zd b) { return zmk(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ zd zsub(zd a, zd b) { return zmk(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ zd zneg(zd a) { return zmk(-a.x, -a.y); }
+__device__ __forceinline__ zd zconj(zd a) { return zmk(a.x, -a.y); }
+// a / b by Smith's algorithm (scaled by the larger component of b).
+__device__ __forceinline__ zd zdiv(zd a, zd b) {
+    if (fabs(b.x) >= fabs(b.y)) {
+        const double r = b.y / b.x, d = b.x + b.y * r;
+        return zmk((a.x + a.y * r) / d, (a.y - a.x * r) / d);
+    }
+    const double r = b.x / b.y, d = b.x * r + b.y;
+    return zmk((a.x * r + a.y) / d, (a.y * r - a.x) / d);
+}
+__device__ __forceinline__ bool zbad(zd a) {
+    const double m = fabs(a.x) + fabs(a.y);
+    return !(m == m) || m == 0.0 || m > 1.0e300;
+}
+__device__ __forceinline__ zd zload(const double *s, int i) { return zmk(s[2 * i], s[2 * i + 1]); }
+__device__ __forceinline__ void zstore(double *s, int i, zd v) { s[2 * i] = v.x; s[2 * i + 1] = v.y; }
+
+#define ZLOOP(i, n) \
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < (n); i += (long long)gridDim.x * 256)
+
+// ------------------------------------------------------------------ matrix
+
+// Row-sliced copy of a CSR matrix: slices of 64 rows, entry k of the slice's row l
+// at so[slice] + 64 k + l, padded to the slice's longest row; rl[row] = row length.
+struct ZSell {
+    int nsl = 0;
+    int *so = nullptr, *rl = nullptr, *col = nullptr;
+    zd *val = nullptr;
+};
+
+struct bkh_zmatrix {
+    int n = 0;
+    long long nnz = 0;
+    ZSell A, AT;       // A and A^T (values stored unconjugated; A^H conjugates at use)
+};
+
+static void zsell_free(ZSell &s) {
+    hipFree(s.so); hipFree(s.rl); hipFree(s.col); hipFree(s.val);
+    s = ZSell();
+}
+
+static bool zsell_upload(ZSell &s, int n, const std::vector<int> &rp, const std::vector<int> &ci,
+                         const std::vector<zd> &v) {
+    const int nsl = (n + 63) / 64;
+    std::vector<int> so((size_t)nsl + 1, 0), rl((size_t)std::max(n, 1), 0);
+    long long tot = 0;
+    for (int q = 0; q < nsl; q++) {
+        int w = 0;
+        for (int r = q * 64; r < std::min(n, q * 64 + 64); r++) {
+            rl[r] = rp[r + 1] - rp[r];
+            w = std::max(w, rl[r]);
+        }
+        tot += 64LL * w;
+        if (tot >= (1LL << 31) - 64) { bkh_set_error("complex matrix too large for 32-bit slice offsets"); return false; }
+        so[q + 1] = (int)tot;
+    }
+    std::vector<int> col((size_t)std::max(tot, 1LL), 0);
+    std::vector<zd> val((size_t)std::max(tot, 1LL), zmk(0.0, 0.0));
+    for (int q = 0; q < nsl; q++)
+        for (int r = q * 64; r < std::min(n, q * 64 + 64); r++) {
+            const int w = (so[q + 1] - so[q]) / 64;
+            for (int k = 0; k < w; k++) {
+                const size_t e = (size_t)so[q] + 64 * (size_t)k + (r & 63);
+                if (k < rl[r]) { col[e] = ci[rp[r] + k]; val[e] = v[rp[r] + k]; }
+                else col[e] = r;
+            }
+        }
+    s.nsl = nsl;
+    if (hipMalloc(&s.so, so.size() * sizeof(int)) != hipSuccess ||
+        hipMalloc(&s.rl, rl.size() * sizeof(int)) != hipSuccess ||
+        hipMalloc(&s.col, col.size() * sizeof(int)) != hipSuccess ||
+        hipMalloc(&s.val, val.size() * sizeof(zd)) != hipSuccess) {
+        bkh_set_error("hipMalloc failed for a complex matrix");
+        return false;
+    }
+    return hipMemcpy(s.so, so.data(), so.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+           hipMemcpy(s.rl, rl.data(), rl.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+           hipMemcpy(s.col, col.data(), col.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+           hipMemcpy(s.val, val.data(), val.size() * sizeof(zd), hipMemcpyHostToDevice) == hipSuccess;
+}
+
+extern "C" bkh_zmatrix *bkh_zmatrix_create(int n, const int *row_ptr, const int *col, const double *val) {
+    if (n < 1 || !row_ptr || !col || !val) { bkh_set_error("bkh_zmatrix_create: bad arguments"); return nullptr; }
+    if (!bkh_tctx()) return nullptr;
+    const long long nnz = row_ptr[n];
+    std::vector<int> rp(row_ptr, row_ptr + n + 1), ci(col, col + nnz);
+    std::vector<zd> v((size_t)nnz);
+    for (long long k = 0; k < nnz; k++) {
+        if (ci[k] < 0 || ci[k] >= n) { bkh_set_error("bkh_zmatrix_create: column out of range"); return nullptr; }
+        v[k] = zmk(val[2 * k], val[2 * k + 1]);
+    }
+    // A^T: a stable counting sort by column lists, for every output row c, the
+    // scatter's contributions (src/matvec.c:85-90) in source-row order.
+    std::vector<int> rpt((size_t)n + 1, 0), cit((size_t)nnz);
+    std::vector<zd> vt((size_t)nnz);
+    for (long long k = 0; k < nnz; k++) rpt[ci[k] + 1]++;
+    for (int i = 0; i < n; i++) rpt[i + 1] += rpt[i];
+    std::vector<int> fill(rpt.begin(), rpt.end() - 1);
+    for (int j = 0; j < n; j++)
+        for (int k = rp[j]; k < rp[j + 1]; k++) {
+            const int d = fill[ci[k]]++;
+            cit[d] = j;
+            vt[d] = v[k];
+        }
+    bkh_zmatrix *A = new bkh_zmatrix();
+    A->n = n;
+    A->nnz = nnz;
+    if (!zsell_upload(A->A, n, rp, ci, v) || !zsell_upload(A->AT, n, rpt, cit, vt)) {
+        zsell_free(A->A); zsell_free(A->AT);
+        delete A;
+        return nullptr;
+    }
+    return A;
+}
+
+extern "C" void bkh_zmatrix_destroy(bkh_zmatrix *A) {
+    if (!A) return;
+    hipDeviceSynchronize();
+    zsell_free(A->A);
+    zsell_free(A->AT);
+    delete A;
+    bkh_graph_epoch_bump();
+}
+
+// y = op(A) x, lane per row. CONJ: the A^T copy with conjugated values (A^H).
+template <bool CONJ>
+__global__ __launch_bounds__(256) void k_zspmv(int n, const int *__restrict__ so, const int *__restrict__ rl,
+                                               const int *__restrict__ col, const zd *__restrict__ val,
+                                               const zd *x, zd *y, const Ctl *c) {
+    if (c && ctl_stopped(c)) return;
+    const int row = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (row >= n) return;
+    const int len = rl[row];
+    const int base = so[row >> 6] + (row & 63);
+    zd acc = zmk(0.0, 0.0);
+    for (int k = 0; k < len; k++) {
+        const int e = base + 64 * k;
+        zd a = val[e];
+        if (CONJ) a = zconj(a);
+        acc = zadd(acc, zmul(a, x[col[e]]));
+    }
+    y[row] = acc;
+}
+
+static void launch_zspmv(hipStream_t s, const bkh_zmatrix *A, bool herm, const zd *x, zd *y, const Ctl *c) {
+    const ZSell &M = herm ? A->AT : A->A;
+    const int G = (A->n + 255) / 256;
+    if (herm)
+        hipLaunchKernelGGL((k_zspmv<true>), dim3(G), dim3(256), 0, s, A->n, M.so, M.rl, M.col, M.val, x, y, c);
+    else
+        hipLaunchKernelGGL((k_zspmv<false>), dim3(G), dim3(256), 0, s, A->n, M.so, M.rl, M.col, M.val, x, y, c);
+}
+
+// Y = beta Y + alpha op(A) X for s columns (column-major, ld n), the reference's
+// zcsrgemm1 order: per row, beta*Y first, then + (alpha*a)*x per entry.
+template <bool CONJ>
+__global__ __launch_bounds__(256) void k_zspmm1(int n, int s, const int *__restrict__ so, const int *__restrict__ rl,
+                                                const int *__restrict__ col, const zd *__restrict__ val,
+                                                zd alpha, zd beta, const zd *X, zd *Y) {
+    const int row = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (row >= n) return;
+    const int len = rl[row];
+    const int base = so[row >> 6] + (row & 63);
+    for (int j = 0; j < s; j++) {
+        const zd *x = X + (size_t)j * n;
+        zd acc = zmul(beta, Y[(size_t)j * n + row]);
+        for (int k = 0; k < len; k++) {
+            const int e = base + 64 * k;
+            zd a = val[e];
+            if (CONJ) a = zconj(a);
+            acc = zadd(acc, zmul(zmul(alpha, a), x[col[e]]));
+        }
+        Y[(size_t)j * n + row] = acc;
+    }
+}
+
+extern "C" int bkh_zspmv(const bkh_zmatrix *A, char op, const double *d_x, double *d_y) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (!A || !d_x || !d_y || (op != 'n' && op != 't')) return BKH_BAD_ARGUMENT;
+    launch_zspmv(t->stream, A, op == 't', (const zd *)d_x, (zd *)d_y, nullptr);
+    LAUNCH_OK();
+    HIP_OK(hipStreamSynchronize(t->stream));
+    return 0;
+}
+
+extern "C" int bkh_zspmm1(const bkh_zmatrix *A, char op, const double *alpha, int s, const double *d_X,
+                          const double *beta, double *d_Y) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (!A || !alpha || !beta || !d_X || !d_Y || s < 1 || (op != 'n' && op != 't')) return BKH_BAD_ARGUMENT;
+    const ZSell &M = op == 't' ? A->AT : A->A;
+    const int G = (A->n + 255) / 256;
+    const zd al = zmk(alpha[0], alpha[1]), be = zmk(beta[0], beta[1]);
+    if (op == 't')
+        hipLaunchKernelGGL((k_zspmm1<true>), dim3(G), dim3(256), 0, t->stream, A->n, s, M.so, M.rl, M.col, M.val, al,
+                           be, (const zd *)d_X, (zd *)d_Y);
+    else
+        hipLaunchKernelGGL((k_zspmm1<false>), dim3(G), dim3(256), 0, t->stream, A->n, s, M.so, M.rl, M.col, M.val, al,
+                           be, (const zd *)d_X, (zd *)d_Y);
+    LAUNCH_OK();
+    HIP_OK(hipStreamSynchronize(t->stream));
+    return 0;
+}
+
+// ------------------------------------------------------------- dot products
+
+// Up to ZD_MAX conj(x_d).y_d sums per launch; partial (re, im) pairs per workgroup.
+#define ZD_MAX 4
+struct ZDots {
+    const zd *x[ZD_MAX];
+    const zd *y[ZD_MAX];
+};
+
+template <int ND>
+__global__ __launch_bounds__(256) void k_zdots(long long n, ZDots a, const Ctl *c, double *part) {
+    if (c && ctl_stopped(c)) return;
+    double v[2 * ND];
+#pragma unroll
+    for (int d = 0; d < 2 * ND; d++) v[d] = 0.0;
+    ZLOOP(i, n) {
+#pragma unroll
+        for (int d = 0; d < ND; d++) {
+            const zd p = zmul(zconj(a.x[d][i]), a.y[d][i]);
+            v[2 * d] += p.x;
+            v[2 * d + 1] += p.y;
+        }
+    }
+    block_partial<2 * ND, 256>(v, part);
+}
+
+template <int ND>
+static void launch_zdots(hipStream_t s, long long n, const ZDots &a, const Ctl *c, double *part) {
+    hipLaunchKernelGGL((k_zdots<ND>), dim3(vec_grid(n, 256)), dim3(256), 0, s, n, a, c, part);
+}
+
+static ZDots zdots(const zd *x0, const zd *y0, const zd *x1 = nullptr, const zd *y1 = nullptr) {
+    ZDots a;
+    memset(&a, 0, sizeof a);
+    a.x[0] = x0; a.y[0] = y0; a.x[1] = x1; a.y[1] = y1;
+    return a;
+}
+
+// ------------------------------------------------------------------ solvers
+
+// Ctl::s slots (complex values take two doubles).
+enum {
+    ZS_ALPHA = 0, ZS_BETA = 2, ZS_RDOT = 4, ZS_G0 = 6, ZS_G1 = 8, ZS_ALOLD = 10, ZS_RHOOLD = 12,
+    ZS_OMEGA = 20,      // omega[0..l]        (up to 17 values)
+    ZS_OMD = 56,        // omega_d[0..l]
+    ZS_OMDD = 92,       // omega_dd[0..l]
+    ZS_SIG = 128,       // sigma[0..l] (MR norms, real parts used)
+};
+#define ZS_LMAX 8
+
+static inline int zbl_ok(int l) { return l >= 1 && l <= ZS_LMAX; }
+
+// r = -(A x) + 1 b (zdscal(-1) then zaxpy(one), bicg.c:106-109); copies to the
+// shadow / direction vectors; partial conj(sr).r and |b|^2.
+__global__ __launch_bounds__(256) void k_zinit(long long n, const zd *b, const zd *ax, zd *r, zd *c1, zd *c2, zd *c3,
+                                               double *part) {
+    double v[3] = {0.0, 0.0, 0.0};
+    ZLOOP(i, n) {
+        const zd t = zmk(-1.0 * ax[i].x, -1.0 * ax[i].y);
+        const zd ri = zadd(t, zmul(zmk(1.0, 0.0), b[i]));
+        r[i] = ri;
+        if (c1) c1[i] = ri;
+        if (c2) c2[i] = ri;
+        if (c3) c3[i] = ri;
+        const zd d = zmul(zconj(ri), ri);
+        v[0] += d.x;
+        v[1] += d.y;
+        v[2] += b[i].x * b[i].x + b[i].y * b[i].y;
+    }
+    block_partial<3, 256>(v, part);
+}
+
+// |r0|^2 of the initial residual and |b|^2 -> Ctl::fnb; slot RDOT = conj(r).r.
+struct ZInitFin {
+    __device__ void operator()(Ctl *c, const double *tot) const {
+        c->s[ZS_RDOT] = tot[0];
+        c->s[ZS_RDOT + 1] = tot[1];
+        c->fnb = sqrt(tot[2]);
+        if (!(c->fnb > 0.0)) ctl_breakdown(c);
+    }
+};
+
+// ---- BiCG (bicg.c:91-177)
+
+struct ZbgAlphaFin {   // alpha = r_dot_old / conj(sp).ap   (:125-126)
+    __device__ void operator()(Ctl *c, const double *tot) const {
+        const zd d = zmk(tot[0], tot[1]);
+        if (zbad(d)) { ctl_breakdown(c); return; }
+        zstore(c->s, ZS_ALPHA / 2, zdiv(zload(c->s, ZS_RDOT / 2), d));
+    }
+};
+
+// x += alpha p; r += (-alpha) ap; sr += (-conj(alpha)) atp; partials conj(sr).r, |r|^2.
+__global__ __launch_bounds__(256) void k_zbg_upd(long long n, zd *x, zd *r, zd *sr, const zd *p, const zd *ap,
+                                                 const zd *atp, const Ctl *c, double *part) {
+    if (ctl_stopped(c)) return;
+    const zd al = zload(c->s, ZS_ALPHA / 2), mal = zneg(al), mcal = zneg(zconj(al));
+    double v[3] = {0.0, 0.0, 0.0};
+    ZLOOP(i, n) {
+        x[i] = zadd(x[i], zmul(al, p[i]));
+        const zd ri = zadd(r[i], zmul(mal, ap[i]));
+        r[i] = ri;
+        const zd si = zadd(sr[i], zmul(mcal, atp[i]));
+        sr[i] = si;
+        const zd d = zmul(zconj(si), ri);
+        v[0] += d.x;
+        v[1] += d.y;
+        v[2] += ri.x * ri.x + ri.y * ri.y;
+    }
+    block_partial<3, 256>(v, part);
+}
+
+struct ZbgBetaFin {    // beta = r_dot / r_dot_old (:141-144); error test (:156-159)
+    __device__ void operator()(Ctl *c, const double *tot) const {
+        const zd rd = zmk(tot[0], tot[1]), old = zload(c->s, ZS_RDOT / 2);
+        zstore(c->s, ZS_BETA / 2, zdiv(rd, old));
+        zstore(c->s, ZS_RDOT / 2, rd);
+        ctl_check(c, tot[2]);
+    }
+};
+
+// p = beta p + 1 r; sp = conj(beta) sp + 1 sr (zscal then zaxpy, :146-153).
+__global__ __launch_bounds__(256) void k_zbg_p(long long n, zd *p, zd *sp, const zd *r, const zd *sr, const Ctl *c) {
+    if (ctl_stopped(c)) return;
+    const zd be = zload(c->s, ZS_BETA / 2), cbe = zconj(be), one = zmk(1.0, 0.0);
+    ZLOOP(i, n) {
+        p[i] = zadd(zmul(be, p[i]), zmul(one, r[i]));
+        sp[i] = zadd(zmul(cbe, sp[i]), zmul(one, sr[i]));
+    }
+}
+
+struct ZResFin {
+    __device__ void operator()(Ctl *c, const double *tot) const {
+        c->s[0] = tot[1] > 0.0 ? sqrt(tot[0]) / sqrt(tot[1]) : 0.0;
+    }
+};
+
+// ||b - (A + sigma I) x|| / ||b|| on the device into co->s[0] (co: a scratch control block).
+__global__ __launch_bounds__(256) void k_zres2_shift(long long n, const zd *b, const zd *ax, const zd *x, zd sigma,
+                                                     double *part) {
+    double v[2] = {0.0, 0.0};
+    ZLOOP(i, n) {
+        const zd r = zsub(zsub(b[i], ax[i]), zmul(sigma, x[i]));
+        v[0] += r.x * r.x + r.y * r.y;
+        v[1] += b[i].x * b[i].x + b[i].y * b[i].y;
+    }
+    block_partial<2, 256>(v, part);
+}
+
+static int ztrue_res(ThreadCtx *t, const bkh_zmatrix *A, const zd *b, const zd *x, zd sigma, zd *tmp, Ctl *co,
+                     double *part, double *out) {
+    hipStream_t s = t->stream;
+    const long long n = A->n;
+    if (ctl_init(t, co, 0.0, 1)) return BKH_DEVICE_ERROR;
+    launch_zspmv(s, A, false, x, tmp, nullptr);
+    hipLaunchKernelGGL(k_zres2_shift, dim3(vec_grid(n, 256)), dim3(256), 0, s, n, b, (const zd *)tmp, x, sigma, part);
+    launch_finalize<2>(s, part, vec_grid(n, 256), co, ZResFin());
+    LAUNCH_OK();
+    if (ctl_fetch(t, co)) return BKH_DEVICE_ERROR;
+    *out = t->ctl_host->s[0];
+    return 0;
+}
+
+// Fills `inf` from the solve's control block (fetched into t->ctl_host by the caller).
+static int zresult(ThreadCtx *t, const Ctl *hc, bkh_info *inf) {
+    inf->status = hc->status;
+    inf->iters = hc->result_it;
+    inf->rec_res = hc->err;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, t->ev0, t->ev1) == hipSuccess) inf->time_ms = ms;
+    bkh_note_solve_ms(ms);
+    inf->ret = (hc->status == ST_CONVERGED) ? hc->result_it : (hc->status == ST_BREAKDOWN ? -2 : -1);
+    return inf->ret;
+}
+
+static int zsolve_end(ThreadCtx *t, const bkh_zmatrix *A, const zd *b, const zd *x, Ctl *c, Ctl *co, zd *tmp,
+                      double *part, bkh_info *info) {
+    HIP_OK(hipEventRecord(t->ev1, t->stream));
+    if (ctl_fetch(t, c)) return BKH_DEVICE_ERROR;
+    bkh_info inf;
+    memset(&inf, 0, sizeof inf);
+    const Ctl hc = *t->ctl_host;
+    HIP_OK(hipEventSynchronize(t->ev1));
+    int ret = zresult(t, &hc, &inf);
+    if (info) {
+        if (ztrue_res(t, A, b, x, zmk(0.0, 0.0), tmp, co, part, &inf.true_res)) return BKH_DEVICE_ERROR;
+        *info = inf;
+    }
+    return ret;
+}
+
+extern "C" int bkh_zbicg(const bkh_zmatrix *A, const double *d_b, double *d_x, double tol, int max_iter,
+                         bkh_info *info) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (!A || !d_b || !d_x) return BKH_BAD_ARGUMENT;
+    if (max_iter <= 0) {                 // no iteration: the reference's error test sees 0
+        if (info) { memset(info, 0, sizeof *info); info->status = ST_CONVERGED; }
+        return 0;
+    }
+    const long long n = A->n;
+    const int G = vec_grid(n, 256);
+    void *ws = bkh_workspace(2 * carve_size(sizeof(Ctl)) + carve_size(sizeof(double) * 6 * G) +
+                             6 * carve_size(sizeof(zd) * n));
+    if (!ws) return BKH_DEVICE_ERROR;
+    Carve cv(ws);
+    Ctl *c = cv.take<Ctl>(1), *co = cv.take<Ctl>(1);
+    double *part = cv.take<double>(6 * G);
+    zd *r = cv.take<zd>(n), *sr = cv.take<zd>(n), *p = cv.take<zd>(n), *sp = cv.take<zd>(n);
+    zd *ap = cv.take<zd>(n), *atp = cv.take<zd>(n);
+    const zd *b = (const zd *)d_b;
+    zd *x = (zd *)d_x;
+    hipStream_t s = t->stream;
+    HIP_OK(hipEventRecord(t->ev0, s));
+    if (ctl_init(t, c, tol, max_iter)) return BKH_DEVICE_ERROR;
+    launch_zspmv(s, A, false, x, ap, nullptr);                 // bicg.c:106-115
+    hipLaunchKernelGGL(k_zinit, dim3(G), dim3(256), 0, s, n, b, (const zd *)ap, r, sr, p, sp, part);
+    launch_finalize<3>(s, part, G, c, ZInitFin());
+    LAUNCH_OK();
+    const int K = default_graph_iters(n);
+    hipGraphExec_t ge = capture_cached(t, {0x2B16, (uintptr_t)A, (uintptr_t)b, (uintptr_t)x, (uintptr_t)c,
+                                           (uintptr_t)part, (uintptr_t)r}, s, K, [=]() {
+        launch_zspmv(s, A, false, p, ap, c);
+        launch_zspmv(s, A, true, sp, atp, c);
+        launch_zdots<1>(s, n, zdots(sp, ap), c, part);
+        launch_finalize<2>(s, part, G, c, ZbgAlphaFin());
+        hipLaunchKernelGGL(k_zbg_upd, dim3(G), dim3(256), 0, s, n, x, r, sr, (const zd *)p, (const zd *)ap,
+                           (const zd *)atp, (const Ctl *)c, part);
+        launch_finalize<3>(s, part, G, c, ZbgBetaFin());
+        hipLaunchKernelGGL(k_zbg_p, dim3(G), dim3(256), 0, s, n, p, sp, (const zd *)r, (const zd *)sr,
+                           (const Ctl *)c);
+    });
+    if (!ge) { bkh_set_error("hipGraph capture of the complex BiCG iteration failed"); return BKH_DEVICE_ERROR; }
+    if (run_graph_until_done(t, ge, c, max_iter / K + 2)) return BKH_DEVICE_ERROR;
+    return zsolve_end(t, A, b, x, c, co, ap, part, info);
+}
+
+// ---- BiCGSTAB(l) (bicgstabl.c:141-276)
+//
+// Vectors: R[0..l], P[0..l] (n each), rs; scalars in Ctl::s (complex pairs).
+
+// BiCG step j, first half: gamma1 = conj(rs).r_j; beta = (-alpha) gamma1 / gamma0;
+// gamma0 = gamma1 (:189-191). On j = 0 the outer gamma0 = -omega_l gamma0 (:186)
+// comes first.
+struct ZblBetaFin {
+    int j, l;
+    __device__ void operator()(Ctl *c, const double *tot) const {
+        if (j == 0) zstore(c->s, ZS_G0 / 2, zmul(zneg(zload(c->s, ZS_OMEGA / 2 + l)), zload(c->s, ZS_G0 / 2)));
+        const zd g1 = zmk(tot[0], tot[1]), g0 = zload(c->s, ZS_G0 / 2);
+        if (zbad(g0)) { ctl_breakdown(c); return; }
+        zstore(c->s, ZS_BETA / 2, zdiv(zmul(zneg(zload(c->s, ZS_ALPHA / 2)), g1), g0));
+        zstore(c->s, ZS_G0 / 2, g1);
+        zstore(c->s, ZS_G1 / 2, g1);
+    }
+};
+
+// p_i = beta p_i + 1 r_i, i <= j (:193-196).
+__global__ __launch_bounds__(256) void k_zbl_p(long long n, int j, zd *P, const zd *R, const Ctl *c) {
+    if (ctl_stopped(c)) return;
+    const zd be = zload(c->s, ZS_BETA / 2), one = zmk(1.0, 0.0);
+    ZLOOP(e, n) {
+        for (int i = 0; i <= j; i++) {
+            const size_t k = (size_t)i * n + e;
+            P[k] = zadd(zmul(be, P[k]), zmul(one, R[k]));
+        }
+    }
+}
+
+struct ZblAlphaFin {   // alpha = gamma1 / conj(rs).p_{j+1} (:200-201)
+    __device__ void operator()(Ctl *c, const double *tot) const {
+        const zd d = zmk(tot[0], tot[1]);
+        if (zbad(d)) { ctl_breakdown(c); return; }
+        zstore(c->s, ZS_ALPHA / 2, zdiv(zload(c->s, ZS_G1 / 2), d));
+    }
+};
+
+// r_i += (-alpha) p_{i+1}, i <= j (:203-206); x += alpha p_0 (:209).
+__global__ __launch_bounds__(256) void k_zbl_r(long long n, int j, zd *R, const zd *P, zd *x, const Ctl *c) {
+    if (ctl_stopped(c)) return;
+    const zd al = zload(c->s, ZS_ALPHA / 2), mal = zneg(al);
+    ZLOOP(e, n) {
+        for (int i = 0; i <= j; i++) {
+            const size_t k = (size_t)i * n + e;
+            R[k] = zadd(R[k], zmul(mal, P[k + n]));
+        }
+        x[e] = zadd(x[e], zmul(al, P[e]));
+    }
+}
+
+// MR part, modified Gram-Schmidt (:214-224). tau and sigma indices follow the
+// reference (tau stride l). Slot layout: tau in `tau` (device array, l2*l2 pairs).
+struct ZblTauFin {     // tau[j*l+i] = conj(r_i).r_j / sigma_i
+    int i, j, l;
+    double *tau;
+    __device__ void operator()(Ctl *c, const double *tot) const {
+        const zd sg = zload(c->s, ZS_SIG / 2 + i);
+        if (zbad(sg)) { ctl_breakdown(c); return; }
+        zstore(tau, j * l + i, zdiv(zmk(tot[0], tot[1]), sg));
+    }
+};
+
+// r_j += (-tau[j*l+i]) r_i.
+__global__ __launch_bounds__(256) void k_zbl_mgs(long long n, zd *rj, const zd *ri, const double *tau, int q,
+                                                 const Ctl *c) {
+    if (ctl_stopped(c)) return;
+    const zd m = zneg(zload(tau, q));
+    ZLOOP(e, n) rj[e] = zadd(rj[e], zmul(m, ri[e]));
+}
+
+// sigma_j = conj(r_j).r_j; omega_d[j] = conj(r_j).r_0 / sigma_j; on j == l the
+// omega / omega_dd recurrences (:226-242).
+struct ZblSigFin {
+    int j, l;
+    const double *tau;
+    __device__ void operator()(Ctl *c, const double *tot) const {
+        const zd sg = zmk(tot[0], tot[1]);
+        if (zbad(sg)) { ctl_breakdown(c); return; }
+        zstore(c->s, ZS_SIG / 2 + j, sg);
+        zstore(c->s, ZS_OMD / 2 + j, zdiv(zmk(tot[2], tot[3]), sg));
+        if (j < l) return;
+        zd *om = (zd *)&c->s[ZS_OMEGA], *omd = (zd *)&c->s[ZS_OMD], *omdd = (zd *)&c->s[ZS_OMDD];
+        om[l] = omd[l];
+        for (int jj = l - 1; jj > 0; jj--) {
+            zd t = zmk(0.0, 0.0);
+            for (int i = jj + 1; i <= l; i++) t = zadd(t, zmul(zload(tau, i * l + jj), om[i]));
+            om[jj] = zsub(omd[jj], t);
+        }
+        for (int jj = 1; jj < l; jj++) {
+            zd t = zmk(0.0, 0.0);
+            for (int i = jj + 1; i < l; i++) t = zadd(t, zmul(zload(tau, i * l + jj), om[i + 1]));
+            omdd[jj] = zadd(om[jj + 1], t);
+        }
+    }
+};
+
+// Update (:245-256): x += omega_1 r_0, r_0 += (-omega_d_l) r_l, p_0 += (-omega_l) p_l,
+// then for i = 1..l-1: p_0 += (-omega_i) p_i, r_0 += (-omega_d_i) r_i, x += omega_dd_i r_i;
+// partial |r_0|^2 for the error test.
+__global__ __launch_bounds__(256) void k_zbl_upd(long long n, int l, zd *R, zd *P, zd *x, const Ctl *c, double *part) {
+    if (ctl_stopped(c)) return;
+    const zd *om = (const zd *)&c->s[ZS_OMEGA], *omd = (const zd *)&c->s[ZS_OMD], *omdd = (const zd *)&c->s[ZS_OMDD];
+    double v[1] = {0.0};
+    ZLOOP(e, n) {
+        zd xe = zadd(x[e], zmul(om[1], R[e]));
+        zd r0 = zadd(R[e], zmul(zneg(omd[l]), R[(size_t)l * n + e]));
+        zd p0 = zadd(P[e], zmul(zneg(om[l]), P[(size_t)l * n + e]));
+        for (int i = 1; i < l; i++) {
+            const zd ri = R[(size_t)i * n + e];
+            p0 = zadd(p0, zmul(zneg(om[i]), P[(size_t)i * n + e]));
+            r0 = zadd(r0, zmul(zneg(omd[i]), ri));
+            xe = zadd(xe, zmul(omdd[i], ri));
+        }
+        x[e] = xe;
+        R[e] = r0;
+        P[e] = p0;
+        v[0] += r0.x * r0.x + r0.y * r0.y;
+    }
+    block_partial<1, 256>(v, part);
+}
+
+struct ZCheckFin {
+    __device__ void operator()(Ctl *c, const double *tot) const { ctl_check(c, tot[0]); }
+};
+
+__global__ void k_zbl_setup(Ctl *c, int l) {
+    zstore(c->s, ZS_OMEGA / 2 + l, zmk(1.0, 0.0));      // omega[l] = 1 (:168)
+    zstore(c->s, ZS_G0 / 2, zmk(1.0, 0.0));             // gamma0 = 1
+    zstore(c->s, ZS_ALPHA / 2, zmk(0.0, 0.0));          // alpha = 0
+}
+
+extern "C" int bkh_zbicgstabl(const bkh_zmatrix *A, const double *d_b, int l, double *d_x, double tol,
+                              int max_iter, bkh_info *info) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (!A || !d_b || !d_x || !zbl_ok(l)) return BKH_BAD_ARGUMENT;
+    if (max_iter <= 0) {                 // no iteration: the reference's error test sees 0
+        if (info) { memset(info, 0, sizeof *info); info->status = ST_CONVERGED; }
+        return 0;
+    }
+    const long long n = A->n;
+    const int G = vec_grid(n, 256), l2 = l + 1;
+    void *ws = bkh_workspace(2 * carve_size(sizeof(Ctl)) + carve_size(sizeof(double) * 8 * G) +
+                             carve_size(sizeof(zd) * l2 * l2) + (2 * l2 + 2) * carve_size(sizeof(zd) * n));
+    if (!ws) return BKH_DEVICE_ERROR;
+    Carve cv(ws);
+    Ctl *c = cv.take<Ctl>(1), *co = cv.take<Ctl>(1);
+    double *part = cv.take<double>(8 * G);
+    double *tau = (double *)cv.take<zd>((size_t)l2 * l2);
+    zd *R = cv.take<zd>((size_t)l2 * n), *P = cv.take<zd>((size_t)l2 * n), *rs = cv.take<zd>(n), *tmp = cv.take<zd>(n);
+    const zd *b = (const zd *)d_b;
+    zd *x = (zd *)d_x;
+    hipStream_t s = t->stream;
+    HIP_OK(hipEventRecord(t->ev0, s));
+    if (ctl_init(t, c, tol, max_iter)) return BKH_DEVICE_ERROR;
+    HIP_OK(hipMemsetAsync(P, 0, sizeof(zd) * l2 * n, s));
+    HIP_OK(hipMemsetAsync(R, 0, sizeof(zd) * l2 * n, s));
+    hipLaunchKernelGGL(k_zbl_setup, dim3(1), dim3(1), 0, s, c, l);
+    launch_zspmv(s, A, false, x, tmp, nullptr);                // :175-181
+    hipLaunchKernelGGL(k_zinit, dim3(G), dim3(256), 0, s, n, b, (const zd *)tmp, R, P, rs, (zd *)nullptr, part);
+    launch_finalize<3>(s, part, G, c, ZInitFin());
+    LAUNCH_OK();
+    hipGraphExec_t ge = capture_cached(t, {0x2B17, (uintptr_t)A, (uintptr_t)b, (uintptr_t)x, (uintptr_t)c,
+                                           (uintptr_t)part, (uintptr_t)R, (uintptr_t)l}, s, 1, [=]() {
+        for (int j = 0; j < l; j++) {
+            launch_zdots<1>(s, n, zdots(rs, R + (size_t)j * n), c, part);
+            launch_finalize<2>(s, part, G, c, ZblBetaFin{j, l});
+            hipLaunchKernelGGL(k_zbl_p, dim3(G), dim3(256), 0, s, n, j, P, (const zd *)R, (const Ctl *)c);
+            launch_zspmv(s, A, false, P + (size_t)j * n, P + (size_t)(j + 1) * n, c);
+            launch_zdots<1>(s, n, zdots(rs, P + (size_t)(j + 1) * n), c, part);
+            launch_finalize<2>(s, part, G, c, ZblAlphaFin());
+            hipLaunchKernelGGL(k_zbl_r, dim3(G), dim3(256), 0, s, n, j, R, (const zd *)P, x, (const Ctl *)c);
+            launch_zspmv(s, A, false, R + (size_t)j * n, R + (size_t)(j + 1) * n, c);
+        }
+        for (int j = 1; j <= l; j++) {
+            zd *rj = R + (size_t)j * n;
+            for (int i = 1; i < j; i++) {
+                const zd *ri = R + (size_t)i * n;
+                launch_zdots<1>(s, n, zdots(ri, rj), c, part);
+                launch_finalize<2>(s, part, G, c, ZblTauFin{i, j, l, tau});
+                hipLaunchKernelGGL(k_zbl_mgs, dim3(G), dim3(256), 0, s, n, rj, ri, (const double *)tau, j * l + i,
+                                   (const Ctl *)c);
+            }
+            launch_zdots<2>(s, n, zdots(rj, rj, rj, R), c, part);
+            launch_finalize<4>(s, part, G, c, ZblSigFin{j, l, tau});
+        }
+        hipLaunchKernelGGL(k_zbl_upd, dim3(G), dim3(256), 0, s, n, l, R, P, x, (const Ctl *)c, part);
+        launch_finalize<1>(s, part, G, c, ZCheckFin());
+    });
+    if (!ge) { bkh_set_error("hipGraph capture of the complex BiCGSTAB(l) iteration failed"); return BKH_DEVICE_ERROR; }
+    if (run_graph_until_done(t, ge, c, max_iter + 2)) return BKH_DEVICE_ERROR;
+    return zsolve_end(t, A, b, x, c, co, tmp, part, info);
+}
+
+// ---- shifted BiCGSTAB(l) (shbicgstabl.c:288-592)
+//
+// Per-shift scalars in a device array (one thread per shift in the scalar kernels);
+// p_shift[k][i] at PS + (k (l+1) + i) n, x_k at x + (k+1) n as in the reference;
+// the per-shift vector kernels run a 2D grid (rows x shifts).
+struct ZSh {
+    zd sigma, pi, pi_new, pi_old, theta, alpha_s, beta_s;
+    zd cA, cB, cC, d1, d2;           // coefficients of the BiCG step's p_shift updates
+    zd cx[ZS_LMAX];                  // x_k coefficients of the MR update
+    zd cpsi;                         // 1 / psi
+};
+
+__global__ void k_zsh_setup(Ctl *c, ZSh *sh, const double *sigma, int ns, int l) {
+    if (threadIdx.x == 0) {
+        zstore(c->s, ZS_OMEGA / 2 + l - 1, zmk(1.0, 0.0));   // omega[l-1] = 1 (:315)
+        zstore(c->s, ZS_ALPHA / 2, zmk(0.0, 0.0));           // alpha = 0, alpha_old = 1,
+        zstore(c->s, ZS_ALOLD / 2, zmk(1.0, 0.0));           // rho_old = 1 (:330-332)
+        zstore(c->s, ZS_RHOOLD / 2, zmk(1.0, 0.0));
+    }
+    for (int k = threadIdx.x; k < ns; k += blockDim.x) {
+        ZSh z;
+        memset(&z, 0, sizeof z);
+        z.sigma = zmk(sigma[2 * k], sigma[2 * k + 1]);
+        z.pi = z.pi_old = z.theta = zmk(1.0, 0.0);          // :366-370
+        sh[k] = z;
+    }
+}
+
+// Seed step j: rho = conj(rs).r_j, beta = alpha rho / rho_old (:390-391); on j = 0
+// first rho_old = -omega[l-1] rho_old (:387).
+struct ZshBetaFin {
+    int j, l;
+    __device__ void operator()(Ctl *c, const double *tot) const {
+        if (j == 0)
+            zstore(c->s, ZS_RHOOLD / 2, zmul(zneg(zload(c->s, ZS_OMEGA / 2 + l - 1)), zload(c->s, ZS_RHOOLD / 2)));
+        const zd rho = zmk(tot[0], tot[1]), ro = zload(c->s, ZS_RHOOLD / 2);
+        if (zbad(ro)) { ctl_breakdown(c); return; }
+        zstore(c->s, ZS_BETA / 2, zdiv(zmul(zload(c->s, ZS_ALPHA / 2), rho), ro));
+        zstore(c->s, ZS_G1 / 2, rho);
+    }
+};
+
+// p_i = (-beta) p_i + 1 r_i, i <= j (:394-398).
+__global__ __launch_bounds__(256) void k_zsh_p(long long n, int j, zd *P, const zd *R, const Ctl *c) {
+    if (ctl_stopped(c)) return;
+    const zd mb = zneg(zload(c->s, ZS_BETA / 2)), one = zmk(1.0, 0.0);
+    ZLOOP(e, n) {
+        for (int i = 0; i <= j; i++) {
+            const size_t k = (size_t)i * n + e;
+            P[k] = zadd(zmul(mb, P[k]), zmul(one, R[k]));
+        }
+    }
+}
+
+struct ZshAlphaFin {   // alpha = rho / conj(rs).p_{j+1} (:405-406)
+    __device__ void operator()(Ctl *c, const double *tot) const {
+        const zd d = zmk(tot[0], tot[1]);
+        if (zbad(d)) { ctl_breakdown(c); return; }
+        zstore(c->s, ZS_ALPHA / 2, zdiv(zload(c->s, ZS_G1 / 2), d));
+    }
+};
+
+// x += alpha p_0 (:409).
+__global__ __launch_bounds__(256) void k_zsh_x(long long n, zd *x, const zd *P, const Ctl *c) {
+    if (ctl_stopped(c)) return;
+    const zd al = zload(c->s, ZS_ALPHA / 2);
+    ZLOOP(e, n) x[e] = zadd(x[e], zmul(al, P[e]));
+}
+
+// Per-shift scalars of step j (:414-424) and the coefficients of both p_shift updates.
+__global__ void k_zsh_sc1(int ns, ZSh *sh, const Ctl *c) {
+    if (ctl_stopped(c)) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ns) return;
+    ZSh z = sh[k];
+    const zd al = zload(c->s, ZS_ALPHA / 2), be = zload(c->s, ZS_BETA / 2), ao = zload(c->s, ZS_ALOLD / 2);
+    const zd one = zmk(1.0, 0.0);
+    const zd as = zmul(al, z.sigma);
+    const zd t1 = zmul(zmk(1.0 + as.x, as.y), z.pi);
+    const zd t2 = zmul(zdiv(zmul(al, be), ao), zsub(z.pi_old, z.pi));
+    z.pi_new = zadd(t1, t2);
+    const zd q = zdiv(z.pi_old, z.pi);
+    z.beta_s = zmul(zmul(q, q), be);
+    z.alpha_s = zmul(zdiv(z.pi, z.pi_new), al);
+    z.cA = zneg(z.beta_s);
+    z.cB = zdiv(one, zmul(z.pi, z.theta));
+    z.cC = zdiv(one, z.pi);
+    z.d1 = zdiv(zmk(-1.0, 0.0), z.pi_new);
+    z.d2 = zdiv(one, zmul(z.theta, z.alpha_s));
+    sh[k] = z;
+}
+
+// p_shift updates of step j before the seed's r update (:426-437): p_s,i = cA p_s,i +
+// cB r_i (i <= j); p_s,j+1 = cC r_j; x_k += alpha_s p_s,0.
+__global__ __launch_bounds__(256) void k_zsh_v1(long long n, int j, int l2, zd *PS, zd *XS, const zd *R,
+                                                const ZSh *sh, const Ctl *c) {
+    if (ctl_stopped(c)) return;
+    const int k = blockIdx.y;
+    const ZSh z = sh[k];
+    zd *ps = PS + (size_t)k * l2 * n, *xk = XS + (size_t)(k + 1) * n;
+    ZLOOP(e, n) {
+        for (int i = 0; i <= j; i++) {
+            const size_t q = (size_t)i * n + e;
+            ps[q] = zadd(zmul(z.cA, ps[q]), zmul(z.cB, R[q]));
+        }
+        ps[(size_t)(j + 1) * n + e] = zmul(z.cC, R[(size_t)j * n + e]);
+        xk[e] = zadd(xk[e], zmul(z.alpha_s, ps[e]));
+    }
+}
+
+// r_i += (-alpha) p_{i+1}, i <= j (:443-446).
+__global__ __launch_bounds__(256) void k_zsh_r(long long n, int j, zd *R, const zd *P, const Ctl *c) {
+    if (ctl_stopped(c)) return;
+    const zd mal = zneg(zload(c->s, ZS_ALPHA / 2));
+    ZLOOP(e, n) {
+        for (int i = 0; i <= j; i++) {
+            const size_t k = (size_t)i * n + e;
+            R[k] = zadd(R[k], zmul(mal, P[k + n]));
+        }
+    }
+}
+
+// p_s,j+1 = d2 (p_s,j+1 + d1 r_j) + (-sigma) p_s,j (:452-461).
+__global__ __launch_bounds__(256) void k_zsh_v2(long long n, int j, int l2, zd *PS, const zd *R, const ZSh *sh,
+                                                const Ctl *c) {
+    if (ctl_stopped(c)) return;
+    const int k = blockIdx.y;
+    const ZSh z = sh[k];
+    const zd ms = zneg(z.sigma);
+    zd *ps = PS + (size_t)k * l2 * n;
+    ZLOOP(e, n) {
+        const size_t q = (size_t)(j + 1) * n + e;
+        zd v = zadd(ps[q], zmul(z.d1, R[(size_t)j * n + e]));
+        v = zmul(z.d2, v);
+        ps[q] = zadd(v, zmul(ms, ps[(size_t)j * n + e]));
+    }
+}
+
+// alpha_old = alpha, rho_old = rho; pi_old = pi, pi = pi_new (:464-469).
+__global__ void k_zsh_rot(int ns, ZSh *sh, Ctl *c) {
+    if (ctl_stopped(c)) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) {
+        zstore(c->s, ZS_ALOLD / 2, zload(c->s, ZS_ALPHA / 2));
+        zstore(c->s, ZS_RHOOLD / 2, zload(c->s, ZS_G1 / 2));
+    }
+    if (k < ns) {
+        sh[k].pi_old = sh[k].pi;
+        sh[k].pi = sh[k].pi_new;
+    }
+}
+
+// MR part (:475-502): tau[j*l+i] = conj(r_{i+1}).r_{j+1} / s_i; s_j; omega_d_j; and on
+// the last j the omega / omega_dd recurrences.
+struct ZshSigFin {
+    int j, l;
+    const double *tau;
+    __device__ void operator()(Ctl *c, const double *tot) const {
+        const zd sg = zmk(tot[0], tot[1]);
+        if (zbad(sg)) { ctl_breakdown(c); return; }
+        zstore(c->s, ZS_SIG / 2 + j, sg);
+        zstore(c->s, ZS_OMD / 2 + j, zdiv(zmk(tot[2], tot[3]), sg));
+        if (j < l - 1) return;
+        zd *om = (zd *)&c->s[ZS_OMEGA], *omd = (zd *)&c->s[ZS_OMD], *omdd = (zd *)&c->s[ZS_OMDD];
+        om[l - 1] = omd[l - 1];
+        for (int jj = l - 2; jj >= 0; jj--) {
+            zd t = zmk(0.0, 0.0);
+            for (int i = jj + 1; i < l; i++) t = zadd(t, zmul(zload(tau, i * l + jj), om[i]));
+            om[jj] = zsub(omd[jj], t);
+        }
+        for (int jj = 0; jj < l - 1; jj++) {
+            zd t = zmk(0.0, 0.0);
+            for (int i = jj + 1; i < l - 1; i++) t = zadd(t, zmul(zload(tau, i * l + jj), om[i + 1]));
+            omdd[jj] = zadd(om[jj + 1], t);
+        }
+    }
+};
+
+// Seed update except the residual (:505-511): x += omega_0 r_0; p_0 += (-omega_{l-1}) p_l;
+// for i = 1..l-1: p_0 += (-omega_{i-1}) p_i, x += omega_dd_{i-1} r_i.
+__global__ __launch_bounds__(256) void k_zsh_seed(long long n, int l, zd *R, zd *P, zd *x, const Ctl *c) {
+    if (ctl_stopped(c)) return;
+    const zd *om = (const zd *)&c->s[ZS_OMEGA], *omdd = (const zd *)&c->s[ZS_OMDD];
+    ZLOOP(e, n) {
+        zd xe = zadd(x[e], zmul(om[0], R[e]));
+        zd p0 = zadd(P[e], zmul(zneg(om[l - 1]), P[(size_t)l * n + e]));
+        for (int i = 1; i < l; i++) {
+            p0 = zadd(p0, zmul(zneg(om[i - 1]), P[(size_t)i * n + e]));
+            xe = zadd(xe, zmul(omdd[i - 1], R[(size_t)i * n + e]));
+        }
+        x[e] = xe;
+        P[e] = p0;
+    }
+}
+
+__device__ double zsh_comb(int n, int r) {         // binomial coefficient (util_shift.c:55-63)
+    if (n < 0 || r < 0 || n < r) return 0.0;
+    double v = 1.0;
+    for (int i = 1; i <= r; i++) v = v * (double)(n - r + i) / (double)i;
+    return v;
+}
+
+// Per-shift MR scalars (:515-549): Horner re-expansion of omega at sigma_k (zhorner,
+// util_shift.c:29-53), e = theta pi, theta *= psi, omega_shift_d via the binomial
+// matrix M (:374-380, powers by repeated products), omega_shift_dd via tau.
+__global__ void k_zsh_sc2(int ns, int l, ZSh *sh, const double *tau, const Ctl *c) {
+    if (ctl_stopped(c)) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ns) return;
+    ZSh z = sh[k];
+    const zd *om = (const zd *)&c->s[ZS_OMEGA];
+    const zd sg = z.sigma, msg = zneg(sg);
+    zd out[ZS_LMAX + 1];
+    out[l] = zneg(om[l - 1]);
+    for (int i = l - 1; i > 0; i--) out[i] = zsub(zmul(msg, out[i + 1]), om[i - 1]);
+    {
+        const zd t = zmul(msg, out[1]);
+        out[0] = zmk(t.x + 1.0, t.y);
+    }
+    for (int i = 1; i < l; i++)
+        for (int j = l - 1; j >= i; j--) out[j] = zadd(out[j], zmul(msg, out[j + 1]));
+    for (int i = 1; i <= l; i++) out[i] = zdiv(zneg(out[i]), out[0]);
+    const zd psi = out[0];
+    const zd e = zmul(z.theta, z.pi);
+    z.theta = zmul(z.theta, psi);
+    zd pw[ZS_LMAX];                                  // sigma^d, d = 0..l-1
+    pw[0] = zmk(1.0, 0.0);
+    for (int d = 1; d < l; d++) pw[d] = zmul(pw[d - 1], sg);
+    zd osd[ZS_LMAX];
+    for (int j = 0; j < l; j++) {
+        zd acc = zmk(0.0, 0.0);
+        for (int i = j; i < l; i++) {
+            const double cb = zsh_comb(i, j);
+            const zd m = zmk(pw[i - j].x * cb, pw[i - j].y * cb);
+            acc = zadd(acc, zmul(m, out[i + 1]));
+        }
+        osd[j] = acc;
+    }
+    z.cx[0] = zdiv(osd[0], e);
+    for (int j = 0; j < l - 1; j++) {
+        zd acc = zmk(0.0, 0.0);
+        for (int i = j + 1; i < l - 1; i++) acc = zadd(acc, zmul(zload(tau, i * l + j), osd[i + 1]));
+        z.cx[j + 1] = zdiv(zadd(osd[j + 1], acc), e);
+    }
+    z.cpsi = zdiv(zmk(1.0, 0.0), psi);
+    sh[k] = z;
+}
+
+// Shifted MR update (:551-557): x_k += cx_0 r_0; p_s,0 += (-omega_{l-1}) p_s,l;
+// for i = 1..l-1: p_s,0 += (-omega_{i-1}) p_s,i, x_k += cx_i r_i; p_s,0 *= 1/psi.
+__global__ __launch_bounds__(256) void k_zsh_v3(long long n, int l, zd *PS, zd *XS, const zd *R, const ZSh *sh,
+                                                const Ctl *c) {
+    if (ctl_stopped(c)) return;
+    const int k = blockIdx.y;
+    const int l2 = l + 1;
+    const ZSh z = sh[k];
+    const zd *om = (const zd *)&c->s[ZS_OMEGA];
+    zd *ps = PS + (size_t)k * l2 * n, *xk = XS + (size_t)(k + 1) * n;
+    ZLOOP(e, n) {
+        zd xe = zadd(xk[e], zmul(z.cx[0], R[e]));
+        zd p0 = zadd(ps[e], zmul(zneg(om[l - 1]), ps[(size_t)l * n + e]));
+        for (int i = 1; i < l; i++) {
+            p0 = zadd(p0, zmul(zneg(om[i - 1]), ps[(size_t)i * n + e]));
+            xe = zadd(xe, zmul(z.cx[i], R[(size_t)i * n + e]));
+        }
+        xk[e] = xe;
+        ps[e] = zmul(z.cpsi, p0);
+    }
+}
+
+// Seed residual (:561-564): r_0 += (-omega_d_{i-1}) r_i, i = 1..l; partial |r_0|^2.
+__global__ __launch_bounds__(256) void k_zsh_r0(long long n, int l, zd *R, const Ctl *c, double *part) {
+    if (ctl_stopped(c)) return;
+    const zd *omd = (const zd *)&c->s[ZS_OMD];
+    double v[1] = {0.0};
+    ZLOOP(e, n) {
+        zd r0 = R[e];
+        for (int i = 1; i <= l; i++) r0 = zadd(r0, zmul(zneg(omd[i - 1]), R[(size_t)i * n + e]));
+        R[e] = r0;
+        v[0] += r0.x * r0.x + r0.y * r0.y;
+    }
+    block_partial<1, 256>(v, part);
+}
+
+extern "C" int bkh_zshbicgstabl(const bkh_zmatrix *A, const double *d_b, int l, const double *sigma, int ns,
+                                double *d_x, double tol, int max_iter, bkh_info *info, double *shift_true_res) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (!A || !d_b || !d_x || !zbl_ok(l) || ns < 0 || ns > 65535 || (ns > 0 && !sigma)) return BKH_BAD_ARGUMENT;
+    if (max_iter <= 0) {                 // no iteration: the reference's error test sees 0
+        if (info) { memset(info, 0, sizeof *info); info->status = ST_CONVERGED; }
+        return 0;
+    }
+    const long long n = A->n;
+    const int G = vec_grid(n, 256), l2 = l + 1;
+    const int Gs = std::max(1, (int)std::min<long long>(G, std::max(1LL, 65536LL / std::max(ns, 1))));
+    void *ws = bkh_workspace(2 * carve_size(sizeof(Ctl)) + carve_size(sizeof(double) * 8 * G) +
+                             carve_size(sizeof(zd) * l * l) + carve_size(sizeof(ZSh) * std::max(ns, 1)) +
+                             carve_size(sizeof(double) * 2 * std::max(ns, 1)) +
+                             (2 * l2 + 2) * carve_size(sizeof(zd) * n) +
+                             carve_size(sizeof(zd) * (size_t)n * l2 * std::max(ns, 1)));
+    if (!ws) return BKH_DEVICE_ERROR;
+    Carve cv(ws);
+    Ctl *c = cv.take<Ctl>(1), *co = cv.take<Ctl>(1);
+    double *part = cv.take<double>(8 * G);
+    double *tau = (double *)cv.take<zd>((size_t)l * l);
+    ZSh *sh = cv.take<ZSh>(std::max(ns, 1));
+    double *dsig = cv.take<double>(2 * std::max(ns, 1));
+    zd *R = cv.take<zd>((size_t)l2 * n), *P = cv.take<zd>((size_t)l2 * n), *rs = cv.take<zd>(n), *tmp = cv.take<zd>(n);
+    zd *PS = cv.take<zd>((size_t)n * l2 * std::max(ns, 1));
+    const zd *b = (const zd *)d_b;
+    zd *x = (zd *)d_x;
+    hipStream_t s = t->stream;
+    HIP_OK(hipEventRecord(t->ev0, s));
+    if (ctl_init(t, c, tol, max_iter)) return BKH_DEVICE_ERROR;
+    if (ns > 0) HIP_OK(hipMemcpyAsync(dsig, sigma, sizeof(double) * 2 * ns, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemsetAsync(P, 0, sizeof(zd) * l2 * n, s));
+    HIP_OK(hipMemsetAsync(R, 0, sizeof(zd) * l2 * n, s));
+    HIP_OK(hipMemsetAsync(tau, 0, sizeof(zd) * l * l, s));
+    HIP_OK(hipMemsetAsync(PS, 0, sizeof(zd) * (size_t)n * l2 * std::max(ns, 1), s));
+    hipLaunchKernelGGL(k_zsh_setup, dim3(1), dim3(256), 0, s, c, sh, (const double *)dsig, ns, l);
+    launch_zspmv(s, A, false, x, tmp, nullptr);                // :322-329
+    hipLaunchKernelGGL(k_zinit, dim3(G), dim3(256), 0, s, n, b, (const zd *)tmp, R, P, rs, (zd *)nullptr, part);
+    launch_finalize<3>(s, part, G, c, ZInitFin());
+    LAUNCH_OK();
+    const int SB = (ns + 63) / 64;
+    hipGraphExec_t ge = capture_cached(t, {0x2B18, (uintptr_t)A, (uintptr_t)b, (uintptr_t)x, (uintptr_t)c,
+                                           (uintptr_t)part, (uintptr_t)R, (uintptr_t)l, (uintptr_t)ns,
+                                           (uintptr_t)PS}, s, 1, [=]() {
+        for (int j = 0; j < l; j++) {
+            launch_zdots<1>(s, n, zdots(rs, R + (size_t)j * n), c, part);
+            launch_finalize<2>(s, part, G, c, ZshBetaFin{j, l});
+            hipLaunchKernelGGL(k_zsh_p, dim3(G), dim3(256), 0, s, n, j, P, (const zd *)R, (const Ctl *)c);
+            launch_zspmv(s, A, false, P + (size_t)j * n, P + (size_t)(j + 1) * n, c);
+            launch_zdots<1>(s, n, zdots(rs, P + (size_t)(j + 1) * n), c, part);
+            launch_finalize<2>(s, part, G, c, ZshAlphaFin());
+            hipLaunchKernelGGL(k_zsh_x, dim3(G), dim3(256), 0, s, n, x, (const zd *)P, (const Ctl *)c);
+            if (ns > 0) {
+                hipLaunchKernelGGL(k_zsh_sc1, dim3(SB), dim3(64), 0, s, ns, sh, (const Ctl *)c);
+                hipLaunchKernelGGL(k_zsh_v1, dim3(Gs, ns), dim3(256), 0, s, n, j, l2, PS, x, (const zd *)R,
+                                   (const ZSh *)sh, (const Ctl *)c);
+            }
+            hipLaunchKernelGGL(k_zsh_r, dim3(G), dim3(256), 0, s, n, j, R, (const zd *)P, (const Ctl *)c);
+            launch_zspmv(s, A, false, R + (size_t)j * n, R + (size_t)(j + 1) * n, c);
+            if (ns > 0)
+                hipLaunchKernelGGL(k_zsh_v2, dim3(Gs, ns), dim3(256), 0, s, n, j, l2, PS, (const zd *)R,
+                                   (const ZSh *)sh, (const Ctl *)c);
+            hipLaunchKernelGGL(k_zsh_rot, dim3(std::max(SB, 1)), dim3(64), 0, s, ns, sh, c);
+        }
+        for (int j = 0; j < l; j++) {
+            zd *rj = R + (size_t)(j + 1) * n;
+            for (int i = 0; i < j; i++) {
+                const zd *ri = R + (size_t)(i + 1) * n;
+                launch_zdots<1>(s, n, zdots(ri, rj), c, part);
+                launch_finalize<2>(s, part, G, c, ZblTauFin{i, j, l, tau});
+                hipLaunchKernelGGL(k_zbl_mgs, dim3(G), dim3(256), 0, s, n, rj, ri, (const double *)tau, j * l + i,
+                                   (const Ctl *)c);
+            }
+            launch_zdots<2>(s, n, zdots(rj, rj, rj, R), c, part);
+            launch_finalize<4>(s, part, G, c, ZshSigFin{j, l, tau});
+        }
+        hipLaunchKernelGGL(k_zsh_seed, dim3(G), dim3(256), 0, s, n, l, R, P, x, (const Ctl *)c);
+        if (ns > 0) {
+            hipLaunchKernelGGL(k_zsh_sc2, dim3(SB), dim3(64), 0, s, ns, l, sh, (const double *)tau, (const Ctl *)c);
+            hipLaunchKernelGGL(k_zsh_v3, dim3(Gs, ns), dim3(256), 0, s, n, l, PS, x, (const zd *)R,
+                               (const ZSh *)sh, (const Ctl *)c);
+        }
+        hipLaunchKernelGGL(k_zsh_r0, dim3(G), dim3(256), 0, s, n, l, R, (const Ctl *)c, part);
+        launch_finalize<1>(s, part, G, c, ZCheckFin());
+    });
+    if (!ge) { bkh_set_error("hipGraph capture of the complex shifted BiCGSTAB(l) iteration failed"); return BKH_DEVICE_ERROR; }
+    if (run_graph_until_done(t, ge, c, max_iter + 2)) return BKH_DEVICE_ERROR;
+    int ret = zsolve_end(t, A, b, x, c, co, tmp, part, info);
+    if (ret < -4) return ret;
+    if (shift_true_res) {
+        if (ztrue_res(t, A, b, x, zmk(0.0, 0.0), tmp, co, part, &shift_true_res[0])) return BKH_DEVICE_ERROR;
+        for (int k = 0; k < ns; k++)
+            if (ztrue_res(t, A, b, x + (size_t)(k + 1) * n, zmk(sigma[2 * k], sigma[2 * k + 1]), tmp, co, part,
+                          &shift_true_res[k + 1]))
+                return BKH_DEVICE_ERROR;
+    }
+    return ret;
+}

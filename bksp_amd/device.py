@@ -178,6 +178,71 @@ def bicgstabl(M, d_b, l, d_x, tol, max_iter, **kw):
     return info
 
 
+class ZMatrix:
+    """A device-resident complex general CSR (bkh_zmatrix); values complex128."""
+
+    def __init__(self, n, row_ptr, col, val):
+        rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+        ci = np.ascontiguousarray(col, dtype=np.int32)
+        v = np.ascontiguousarray(val, dtype=np.complex128)
+        self.n = int(n)
+        self.handle = load().bkh_zmatrix_create(self.n, _ptr(rp), _ptr(ci), _ptr(v))
+        if not self.handle:
+            raise BkspDeviceError("bkh_zmatrix_create failed: " + last_error())
+
+    def destroy(self):
+        if self.handle:
+            load().bkh_zmatrix_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def spmv(self, d_x, d_y, op="n"):
+        """y = A x (op 'n') or A^H x (op 't')."""
+        rc = load().bkh_zspmv(self.handle, op.encode(), d_x.ptr, d_y.ptr)
+        if rc:
+            raise BkspDeviceError(f"bkh_zspmv -> {rc}: {last_error()}")
+
+    def spmm1(self, s, d_X, d_Y, alpha=1.0, beta=0.0, op="n"):
+        """Y = beta Y + alpha op(A) X for s column-major columns."""
+        al = np.array([alpha], np.complex128)
+        be = np.array([beta], np.complex128)
+        rc = load().bkh_zspmm1(self.handle, op.encode(), _ptr(al), s, d_X.ptr, _ptr(be), d_Y.ptr)
+        if rc:
+            raise BkspDeviceError(f"bkh_zspmm1 -> {rc}: {last_error()}")
+
+
+def zbicg(M, d_b, d_x, tol, max_iter):
+    info = BkhInfo()
+    rc = load().bkh_zbicg(M.handle, d_b.ptr, d_x.ptr, tol, max_iter, ctypes.byref(info))
+    _check(rc, "bkh_zbicg")
+    return info
+
+
+def zbicgstabl(M, d_b, l, d_x, tol, max_iter):
+    info = BkhInfo()
+    rc = load().bkh_zbicgstabl(M.handle, d_b.ptr, l, d_x.ptr, tol, max_iter, ctypes.byref(info))
+    _check(rc, "bkh_zbicgstabl")
+    return info
+
+
+def zshbicgstabl(M, d_b, l, sigma, d_x, tol, max_iter):
+    """Complex shifted BiCGSTAB(l); d_x holds n*(len(sigma)+1) complex entries (seed
+    first). Returns (info, per-system FP64 true residuals)."""
+    info = BkhInfo()
+    sig = np.ascontiguousarray(sigma, dtype=np.complex128)
+    tr = np.zeros(len(sig) + 1)
+    rc = load().bkh_zshbicgstabl(M.handle, d_b.ptr, l, sig.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                 len(sig), d_x.ptr, tol, max_iter, ctypes.byref(info),
+                                 tr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    _check(rc, "bkh_zshbicgstabl")
+    return info, tr
+
+
 def blcg(M, s, d_B, d_X, tol, max_iter, **kw):
     """Block CG; d_B, d_X hold m x s column-major blocks (Fortran order)."""
     opts = make_opts(**kw)

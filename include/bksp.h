@@ -95,6 +95,47 @@ int dshbicg(dspmat *A, double *b, double *sigma, int sigma_size,
 int dshbicgstabl(dspmat *A, double *b, int l, double *sigma,
                  int sigma_size, double tol, int max_iter, double *x);
 
+/* ---- complex numbers (include/bksp.h:109-190 of the reference) ---- */
+#ifndef __cplusplus
+typedef struct {
+    int *row;
+    int *col;
+    int *nnz;
+    int *row_size;
+    int *col_size;
+    double _Complex *value;
+    char *format;
+    char *type;
+} zspmat;
+
+typedef struct {
+    int row_size;
+    int col_size;
+    double _Complex *value;
+} zmat;
+
+static inline int zspmat_rowsize(zspmat *mat) { return *mat->row_size; }
+static inline int zspmat_colsize(zspmat *mat) { return *mat->col_size; }
+static inline int zmat_set(zmat *mat, int m, int n, double _Complex *num) {
+    mat->value[n * mat->row_size + m] = *num;
+    return 0;
+}
+/* returns double as in the reference (bksp.h:147-149): the real part */
+static inline double zmat_get(zmat *mat, int m, int n) { return mat->value[n * mat->row_size + m]; }
+static inline int zmat_rowsize(zmat *mat) { return mat->row_size; }
+static inline int zmat_colsize(zmat *mat) { return mat->col_size; }
+static inline double _Complex *zmat_value(zmat *mat) { return mat->value; }
+
+zspmat *zread_matrix_market(char *filename);
+zmat *zmat_create(int m, int n);
+int zmat_free(zmat *mat);
+
+int zbicg(zspmat *A, double _Complex *b, double tol, int max_iter, double _Complex *x);
+int zbicgstabl(zspmat *A, double _Complex *b, int ell, double tol, int max_iter, double _Complex *x);
+int zshbicgstabl(zspmat *A, double _Complex *b, int l, double _Complex *sigma,
+                 int sigma_size, double tol, int max_iter, double _Complex *x);
+#endif
+
 #ifdef __cplusplus
 }
 #endif

@@ -157,6 +157,28 @@ int bkh_shbicgstabl(bkh_matrix *A, const double *d_b, int l, const double *sigma
                     double *d_x, double tol, int max_iter, const bkh_opts *opts,
                     bkh_info *info, double *shift_true_res);
 
+/* ---- complex FP64 family. Complex values are interleaved (re, im) doubles, the
+ * layout of double _Complex; matrices are general CSR (0-based). */
+typedef struct bkh_zmatrix bkh_zmatrix;
+/* val holds 2*nnz doubles. A transposed copy is kept for the conjugate transpose. */
+bkh_zmatrix *bkh_zmatrix_create(int n, const int *row_ptr, const int *col, const double *val);
+void bkh_zmatrix_destroy(bkh_zmatrix *A);
+/* y = A x (op 'n') or A^H x (op 't'), each row summed in the order of src/matvec.c:55-98
+ * (replaces zcsrgemm0 through mkl_cspblas_zcsrgemv, include/bksp_internal.h:54-59). */
+int bkh_zspmv(const bkh_zmatrix *A, char op, const double *d_x, double *d_y);
+/* Y = beta Y + alpha op(A) X for s columns (replaces zcsrgemm1, src/matvec.c:174-217). */
+int bkh_zspmm1(const bkh_zmatrix *A, char op, const double *alpha, int s, const double *d_X,
+               const double *beta, double *d_Y);
+/* Replace zbicg (src/bicg.c:91-177), zbicgstabl (src/bicgstabl.c:141-276) and
+ * zshbicgstabl (src/shbicgstabl.c:288-592; sigma: 2*ns host doubles, d_x: (ns+1)*n
+ * complex entries, seed first, shifts entering as 0). Return values as bksp.h. */
+int bkh_zbicg(const bkh_zmatrix *A, const double *d_b, double *d_x, double tol, int max_iter,
+              bkh_info *info);
+int bkh_zbicgstabl(const bkh_zmatrix *A, const double *d_b, int l, double *d_x, double tol,
+                   int max_iter, bkh_info *info);
+int bkh_zshbicgstabl(const bkh_zmatrix *A, const double *d_b, int l, const double *sigma, int ns,
+                     double *d_x, double tol, int max_iter, bkh_info *info, double *shift_true_res);
+
 /* ---- batches of independent systems (no reference counterpart: the reference
  * solves one system per call, so a batch is that many sequential dbicgstabl calls) */
 typedef struct bkh_batch bkh_batch;
