@@ -119,6 +119,64 @@ def bench_cfg3(dv, gen, m=96):
                         "speedup_vs_fp64": round(ms / ms32, 3)}}
 
 
+def bench_complex(dv, gen, po, m=96, ns=16):
+    """Complex FP64 family: shifted BiCGSTAB(4) and BiCGSTAB(4) on the 96^3
+    convection-diffusion operator with an imaginary diagonal shift (i 0.05 a_ii),
+    16 complex shifts sigma_k = 10^(-2 + 2k/15) (1 + 0.5i), RHS N(0,1)+iN(0,1) seeds
+    303/304, tol 1e-8; complex SpMV roofline; oracle CPU baseline on a bounded sample."""
+    n, rp, ci, v = gen.zconvdiff3d(m)
+    nnz = int(rp[-1])
+    b = gen.normal(303, n) + 1j * gen.normal(304, n)
+    sig = 10.0 ** (-2 + 2 * np.arange(ns) / (ns - 1.0)) * (1 + 0.5j)
+    M = dv.ZMatrix(n, rp, ci, v)
+    db = dv.DeviceArray.from_numpy(b)
+    dxs = dv.DeviceArray((n * (ns + 1),), np.complex128)
+    dx = dv.DeviceArray((n,), np.complex128)
+
+    def shifted():
+        dxs.zero()
+        return dv.zshbicgstabl(M, db, 4, sig, dxs, 1e-8, 1000)
+
+    def single():
+        dx.zero()
+        return dv.zbicgstabl(M, db, 4, dx, 1e-8, 1000)
+
+    shifted()
+    (sinfo, tr), sms = _timed(shifted, 1)
+    single()
+    info, ms = _timed(single, 1)
+    M.bench_spmv(reps=3)
+    spmv_ms = M.bench_spmv(reps=20)
+    alg = nnz * (16 + 4) + 4 * (n + 1) + 16 * n + 16 * n
+    gbs = alg / (spmv_ms * 1e-3) / 1e9
+    out = {"workload": f"complex shifted BiCGSTAB(4) and BiCGSTAB(4), convection-diffusion {m}^3 + i 0.05 diag, "
+                       f"n={n}, nnz={nnz}, {ns} complex shifts, tol 1e-8",
+           "zshbicgstabl_ms": round(sms, 2), "zshbicgstabl_outer_iters": sinfo.ret,
+           "zshbicgstabl_max_true_res": float(tr.max()),
+           "zbicgstabl_ms": round(ms, 2), "zbicgstabl_outer_iters": info.ret, "zbicgstabl_true_res": info.true_res,
+           "roofline": {"bound": "hbm", "kernel": "k_zspmv<false> (complex SELL-64, lane per row)",
+                        "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes": alg,
+                        "avg_launch_us": round(spmv_ms * 1e3, 2)}}
+    if po is not None:
+        # bounded sample: 2 outer iterations of the oracle's restatements, scaled to
+        # the device's outer iteration counts
+        rpi, cii = np.ascontiguousarray(rp, np.int32), np.ascontiguousarray(ci, np.int32)
+        t0 = time.perf_counter()
+        po.zbicgstabl(n, rpi, cii, v, b, 4, 1e-300, 2)
+        per1 = (time.perf_counter() - t0) / 2
+        t0 = time.perf_counter()
+        po.zshbicgstabl(n, rpi, cii, v, b, 4, sig, 1e-300, 1)
+        pers = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "zbicgstabl_ms": round(per1 * (info.ret + 1) * 1e3, 1), "zshbicgstabl_ms": round(pers * (sinfo.ret + 1) * 1e3, 1),
+            "unit": "ms", "cores": 1, "kind": "port",
+            "sample": f"oracle (C restatement of src/bicgstabl.c:141-276 and src/shbicgstabl.c:288-592, gcc -O3, "
+                      f"1 thread): 2 outer iterations of zbicgstabl ({per1 * 1e3:.0f} ms each) and 1 of "
+                      f"zshbicgstabl ({pers * 1e3:.0f} ms), scaled to the device's outer iteration counts"}
+    return out
+
+
 def bench_cfg4(dv, gen):
     """BiCGSTAB(2) on 512 independent uneven systems, tol 1e-8: batched (one launch per
     phase, autotuned layout) FP32-IR and FP64 against 512 sequential plain FP64 solves."""
@@ -225,7 +283,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--m", type=int, default=128)
-    ap.add_argument("--configs", default="cfg1,cfg2b,cfg3,cfg4,cfg5",
+    ap.add_argument("--configs", default="cfg1,cfg2b,cfg3,cfg4,cfg5,complex",
                     help="extra workloads reported beside the main line ('' for none)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -332,7 +390,7 @@ def main():
         del M, db, dx, dy, db32, dy32
         for name, fn in (("cfg1", lambda: bench_cfg1(dv, gen, po)), ("cfg2_block", lambda: bench_cfg2_block(dv, gen)),
                          ("cfg3", lambda: bench_cfg3(dv, gen)), ("cfg4", lambda: bench_cfg4(dv, gen)),
-                         ("cfg5", lambda: bench_cfg5(dv, gen))):
+                         ("cfg5", lambda: bench_cfg5(dv, gen)), ("complex", lambda: bench_complex(dv, gen, po))):
             key = {"cfg2_block": "cfg2b"}.get(name, name)
             if key in want:
                 try:

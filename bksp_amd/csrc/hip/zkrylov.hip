@@ -1056,3 +1056,28 @@ extern "C" int bkh_zshbicgstabl(const bkh_zmatrix *A, const double *d_b, int l, 
     }
     return ret;
 }
+
+// Average duration of `reps` back-to-back complex SpMV launches (HIP events on the
+// stream they run on), x = 1; for the roofline of the complex product.
+extern "C" int bkh_zspmv_bench(const bkh_zmatrix *A, char op, int reps, double *avg_ms) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (!A || !avg_ms || reps < 1 || (op != 'n' && op != 't')) return BKH_BAD_ARGUMENT;
+    const long long n = A->n;
+    void *ws = bkh_workspace(2 * carve_size(sizeof(zd) * n));
+    if (!ws) return BKH_DEVICE_ERROR;
+    Carve cv(ws);
+    zd *x = cv.take<zd>(n), *y = cv.take<zd>(n);
+    hipStream_t s = t->stream;
+    HIP_OK(hipMemsetAsync(x, 0, sizeof(zd) * n, s));
+    launch_zspmv(s, A, op == 't', x, y, nullptr);
+    HIP_OK(hipEventRecord(t->ev0, s));
+    for (int k = 0; k < reps; k++) launch_zspmv(s, A, op == 't', x, y, nullptr);
+    HIP_OK(hipEventRecord(t->ev1, s));
+    LAUNCH_OK();
+    HIP_OK(hipEventSynchronize(t->ev1));
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, t->ev0, t->ev1));
+    *avg_ms = ms / reps;
+    return 0;
+}

@@ -207,6 +207,12 @@ class ZMatrix:
         if rc:
             raise BkspDeviceError(f"bkh_zspmv -> {rc}: {last_error()}")
 
+    def bench_spmv(self, reps=20, op="n"):
+        """Average ms of one complex SpMV launch (HIP events, back-to-back launches)."""
+        ms = ctypes.c_double()
+        _check(load().bkh_zspmv_bench(self.handle, op.encode(), reps, ctypes.byref(ms)), "bkh_zspmv_bench")
+        return ms.value
+
     def spmm1(self, s, d_X, d_Y, alpha=1.0, beta=0.0, op="n"):
         """Y = beta Y + alpha op(A) X for s column-major columns."""
         al = np.array([alpha], np.complex128)

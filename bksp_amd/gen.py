@@ -94,3 +94,16 @@ def cfg4_batch(count=512, seed=SEEDS["cfg4_struct"], lo2=10.0, hi2=17.0):
     random_rows(cfg4_system_seed(s), n_s). Returns a list of (n, row_ptr, col, val)."""
     sizes = batch_sizes(seed, count, lo2, hi2)
     return [random_rows(cfg4_system_seed(q, seed), int(n)) for q, n in enumerate(sizes)]
+
+
+def zconvdiff3d(m, delta=0.05, pmax=0.4):
+    """Complex test operator: the m^3 convection-diffusion matrix of cfg3_peclet
+    with an imaginary diagonal shift i*delta*a_ii (a complex nonsymmetric system of
+    the shifted-Helmholtz kind). Returns (n, row_ptr, col, complex val)."""
+    px, py, pz = cfg3_peclet(pmax)
+    n, rp, ci, v = convdiff3d(m, px, py, pz)
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(rp))
+    z = v.astype(np.complex128)
+    d = ci == rows
+    z[d] += 1j * delta * v[d]
+    return n, rp, ci, z

@@ -11,7 +11,8 @@ graphs run.
 Workloads: cg / blcg (128^3 Poisson, RHS N(0,1) seed 202, s = 8 for block CG),
 shifted (96^3 convection-diffusion, 64 shifts, l = 4), pcg (128^3 exp(GRF) diffusion
 field 6000, Jacobi or an untrained learned factor), bicgstabl (64x64 convection-
-diffusion, l = 2, tol 1e-10).
+diffusion, l = 2, tol 1e-10), zbicgstabl / zshbicgstabl (the complex 96^3
+convection-diffusion + i 0.05 diag operator, l = 4, 16 complex shifts).
 """
 import argparse
 import json
@@ -25,7 +26,8 @@ from . import gen
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--solver", choices=["cg", "blcg", "shifted", "pcg", "bicgstabl"], default="cg")
+    ap.add_argument("--solver", choices=["cg", "blcg", "shifted", "pcg", "bicgstabl", "zbicgstabl", "zshbicgstabl"],
+                    default="cg")
     ap.add_argument("--precision", choices=["fp64", "fp32ir"], default="fp64")
     ap.add_argument("--precond", choices=["jacobi", "fsai"], default="jacobi")
     ap.add_argument("--m", type=int, default=0)
@@ -76,6 +78,24 @@ def main():
             def run():
                 dx.zero()
                 return F.pcg_plain(1, db, dx, 1e-8, 100000)
+    elif args.solver in ("zbicgstabl", "zshbicgstabl"):
+        m = args.m or 96
+        n, rp, ci, v = gen.zconvdiff3d(m)
+        M = dv.ZMatrix(n, rp, ci, v)
+        db = dv.DeviceArray.from_numpy(gen.normal(303, n) + 1j * gen.normal(304, n))
+        sig = 10.0 ** (-2 + 2 * np.arange(16) / 15.0) * (1 + 0.5j)
+        if args.solver == "zbicgstabl":
+            dx = dv.DeviceArray((n,), np.complex128)
+
+            def run():
+                dx.zero()
+                return dv.zbicgstabl(M, db, 4, dx, 1e-8, 1000)
+        else:
+            dx = dv.DeviceArray((n * 17,), np.complex128)
+
+            def run():
+                dx.zero()
+                return dv.zshbicgstabl(M, db, 4, sig, dx, 1e-8, 1000)[0]
     else:
         m = args.m or 64
         n, rp, ci, v = gen.convdiff2d(m, 10.0, 20.0)

@@ -178,6 +178,8 @@ int bkh_zbicgstabl(const bkh_zmatrix *A, const double *d_b, int l, double *d_x, 
                    int max_iter, bkh_info *info);
 int bkh_zshbicgstabl(const bkh_zmatrix *A, const double *d_b, int l, const double *sigma, int ns,
                      double *d_x, double tol, int max_iter, bkh_info *info, double *shift_true_res);
+/* Average ms of reps back-to-back complex SpMV launches (op 'n' or 't'), HIP events. */
+int bkh_zspmv_bench(const bkh_zmatrix *A, char op, int reps, double *avg_ms);
 
 /* ---- batches of independent systems (no reference counterpart: the reference
  * solves one system per call, so a batch is that many sequential dbicgstabl calls) */
