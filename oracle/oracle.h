@@ -127,6 +127,9 @@ int    orc_zbicgstabl(int n, const int *rp, const int *ci, const double *v, cons
                       int l, double tol, int max_iter, double *x);
 int    orc_zshbicgstabl(int n, const int *rp, const int *ci, const double *v, const double *b,
                         int l, const double *sigma, int nsigma, double tol, int max_iter, double *x);
+int    orc_zqreco(int m, int n, double *mat, double *r);
+int    orc_zblbicgstabl(int m, const int *rp, const int *ci, const double *v, int n, const double *B,
+                        int l, double tol, int max_iter, double *X, double *true_res);
 #endif
 
 /* Time one call of a solver in seconds (CLOCK_MONOTONIC), for the CPU baseline. */

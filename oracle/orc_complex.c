@@ -8,6 +8,8 @@
  *   orc_zbicg       src/bicg.c:91-177
  *   orc_zbicgstabl  src/bicgstabl.c:141-276
  *   orc_zshbicgstabl src/shbicgstabl.c:288-592
+ *   orc_zqreco      src/myqr.c:41-76        (zgeqrf + zungqr restated below)
+ *   orc_zblbicgstabl src/blbicgstabl.c:255-486
  * and the CBLAS level-1 routines they call, from the Netlib reference algorithms
  * (zdotc: sum of conj(x_i) y_i in index order; dznrm2: the scaled sum of squares
  * over real and imaginary parts; zaxpy: quick return when alpha == 0).
@@ -385,5 +387,279 @@ int orc_zshbicgstabl(int n, const int *rp, const int *ci, const double complex *
     free(tmp); free(rs);
     for (i = 0; i < l2; i++) { free(r[i]); free(p[i]); }
     if (error >= tol) return NOT_CONVERGED_CODE;
+    return iter;
+}
+
+/* ------------------------------------------------ LAPACK-level restatements
+ * From the Netlib reference algorithms (zgemm, zgetf2/zgetrs/zgetri/zgesv,
+ * zgeqr2 + zlarfg + zlarf, zung2r); the safe-minimum rescaling branches of
+ * zlarfg/zgetf2 are omitted (they only act on values near underflow) and ZLADIV
+ * is C99 complex division. Pinned by invariants (Q^H Q = I, QR = V, LU solves). */
+
+/* C = alpha op(A) op(B) + beta C, op 'N' or 'T' (transpose without conjugation). */
+void orc_zgemm(char ta, char tb, int m, int n, int k, double complex alpha, const double complex *A, int lda,
+               const double complex *B, int ldb, double complex beta, double complex *C, int ldc) {
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < m; i++) {
+            double complex s = 0.0;
+            for (int l = 0; l < k; l++) {
+                const double complex a = (ta == 'N') ? A[i + (size_t)l * lda] : A[l + (size_t)i * lda];
+                const double complex b = (tb == 'N') ? B[l + (size_t)j * ldb] : B[j + (size_t)l * ldb];
+                s += a * b;
+            }
+            double complex *c = &C[i + (size_t)j * ldc];
+            *c = (beta == 0.0) ? alpha * s : alpha * s + beta * *c;
+        }
+}
+
+static double cabs1(double complex z) { return fabs(creal(z)) + fabs(cimag(z)); }
+
+int orc_zgetrf(int n, double complex *A, int lda, int *ipiv) {
+    int info = 0;
+    for (int j = 0; j < n; j++) {
+        int p = j;
+        double best = cabs1(A[j + (size_t)j * lda]);
+        for (int i = j + 1; i < n; i++)
+            if (cabs1(A[i + (size_t)j * lda]) > best) { best = cabs1(A[i + (size_t)j * lda]); p = i; }
+        ipiv[j] = p + 1;
+        if (A[p + (size_t)j * lda] != 0.0) {
+            if (p != j)
+                for (int c = 0; c < n; c++) {
+                    double complex t = A[j + (size_t)c * lda];
+                    A[j + (size_t)c * lda] = A[p + (size_t)c * lda];
+                    A[p + (size_t)c * lda] = t;
+                }
+            const double complex r = 1.0 / A[j + (size_t)j * lda];
+            for (int i = j + 1; i < n; i++) A[i + (size_t)j * lda] *= r;
+        } else if (info == 0) {
+            info = j + 1;
+        }
+        for (int c = j + 1; c < n; c++) {
+            const double complex u = A[j + (size_t)c * lda];
+            for (int i = j + 1; i < n; i++) A[i + (size_t)c * lda] -= A[i + (size_t)j * lda] * u;
+        }
+    }
+    return info;
+}
+
+int orc_zgetrs(int n, int nrhs, const double complex *LU, int lda, const int *ipiv, double complex *B, int ldb) {
+    for (int c = 0; c < nrhs; c++) {
+        double complex *b = B + (size_t)c * ldb;
+        for (int i = 0; i < n; i++) {
+            const int p = ipiv[i] - 1;
+            if (p != i) { double complex t = b[i]; b[i] = b[p]; b[p] = t; }
+        }
+        for (int j = 0; j < n; j++)
+            for (int i = j + 1; i < n; i++) b[i] -= LU[i + (size_t)j * lda] * b[j];
+        for (int j = n - 1; j >= 0; j--) {
+            b[j] /= LU[j + (size_t)j * lda];
+            for (int i = 0; i < j; i++) b[i] -= LU[i + (size_t)j * lda] * b[j];
+        }
+    }
+    return 0;
+}
+
+int orc_zgetri(int n, double complex *A, int lda, const int *ipiv) {
+    double complex *lu = zalloc((size_t)n * n);
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) lu[i + (size_t)j * n] = A[i + (size_t)j * lda];
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) A[i + (size_t)j * lda] = (i == j) ? 1.0 : 0.0;
+    orc_zgetrs(n, n, lu, n, ipiv, A, lda);
+    free(lu);
+    return 0;
+}
+
+int orc_zgesv(int n, int nrhs, double complex *A, int lda, int *ipiv, double complex *B, int ldb) {
+    const int info = orc_zgetrf(n, A, lda, ipiv);
+    if (info) return info;
+    return orc_zgetrs(n, nrhs, A, lda, ipiv, B, ldb);
+}
+
+static double dlapy3(double x, double y, double z) {
+    const double ax = fabs(x), ay = fabs(y), az = fabs(z);
+    double w = ax > ay ? ax : ay;
+    w = w > az ? w : az;
+    if (w == 0.0) return ax + ay + az;
+    return w * sqrt((ax / w) * (ax / w) + (ay / w) * (ay / w) + (az / w) * (az / w));
+}
+
+/* zlarfg: H^H (alpha; x) = (beta; 0), beta real, H = I - tau v v^H, v = (1; x scaled). */
+static void zlarfg(int n, double complex *alpha, double complex *x, double complex *tau) {
+    if (n <= 0) { *tau = 0.0; return; }
+    const double xnorm = orc_dznrm2(n - 1, x);
+    const double ar = creal(*alpha), ai = cimag(*alpha);
+    if (xnorm == 0.0 && ai == 0.0) { *tau = 0.0; return; }
+    const double beta = -copysign(dlapy3(ar, ai, xnorm), ar);
+    *tau = CMPLX((beta - ar) / beta, -ai / beta);
+    orc_zscal(n - 1, 1.0 / (*alpha - beta), x);
+    *alpha = beta;
+}
+
+/* zlarf('L'): C (m x n, ldc) := (I - tau v v^H) C. */
+static void zlarf_left(int m, int n, const double complex *v, double complex tau, double complex *C, int ldc) {
+    if (tau == 0.0) return;
+    for (int j = 0; j < n; j++) {
+        double complex *c = C + (size_t)j * ldc;
+        double complex w = 0.0;
+        for (int i = 0; i < m; i++) w += conj(c[i]) * v[i];
+        for (int i = 0; i < m; i++) c[i] -= tau * v[i] * conj(w);
+    }
+}
+
+int orc_zgeqrf(int m, int n, double complex *A, int lda, double complex *tau) {
+    const int k = m < n ? m : n;
+    for (int i = 0; i < k; i++) {
+        double complex *aii = &A[i + (size_t)i * lda];
+        zlarfg(m - i, aii, &A[(i + 1 < m ? i + 1 : m - 1) + (size_t)i * lda], &tau[i]);
+        if (i < n - 1) {
+            const double complex keep = *aii;
+            *aii = 1.0;
+            zlarf_left(m - i, n - i - 1, aii, conj(tau[i]), &A[i + (size_t)(i + 1) * lda], lda);
+            *aii = keep;
+        }
+    }
+    return 0;
+}
+
+int orc_zungqr(int m, int n, int k, double complex *A, int lda, const double complex *tau) {
+    for (int j = k; j < n; j++) {
+        for (int l = 0; l < m; l++) A[l + (size_t)j * lda] = 0.0;
+        A[j + (size_t)j * lda] = 1.0;
+    }
+    for (int i = k - 1; i >= 0; i--) {
+        double complex *aii = &A[i + (size_t)i * lda];
+        if (i < n - 1) {
+            *aii = 1.0;
+            zlarf_left(m - i, n - i - 1, aii, tau[i], &A[i + (size_t)(i + 1) * lda], lda);
+        }
+        if (i < m - 1) orc_zscal(m - i - 1, -tau[i], &A[i + 1 + (size_t)i * lda]);
+        *aii = 1.0 - tau[i];
+        for (int l = 0; l < i; l++) A[l + (size_t)i * lda] = 0.0;
+    }
+    return 0;
+}
+
+/* zqreco (myqr.c:41-76): thin QR of an m x n column-major block, Q in place, R (n x n). */
+int orc_zqreco(int m, int n, double complex *mat, double complex *r) {
+    double complex *tau = zalloc((size_t)n * n);
+    orc_zgeqrf(m, n, mat, m, tau);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) r[i * n + j] = (j > i) ? 0.0 : mat[i * m + j];
+    orc_zungqr(m, n, n, mat, m, tau);
+    free(tau);
+    return 0;
+}
+
+/* mkl_zcsrmm 'G' (bksp_internal.h:81-95) on 1-based CSR: C = beta C + alpha A B. */
+static void zgemm1(int m, int n, double complex alpha, const double complex *v, const int *rp1, const int *ci1,
+                   const double complex *B, double complex beta, double complex *C) {
+    orc_zcsrgemm1('n', &alpha, m, n, v, rp1, ci1, B, &beta, C);
+}
+
+int orc_zblbicgstabl(int m, const int *rp, const int *ci, const double complex *v, int n,
+                     const double complex *B, int l, double tol, int max_iter, double complex *X,
+                     double *true_res) {
+    int iter, i, j;
+    double fnb, error = 0;
+    const size_t mn = (size_t)m * n;
+    double complex *alpha = zalloc((size_t)n * n), *beta = zalloc((size_t)n * n), *t = zalloc((size_t)n * n);
+    double complex *xi = zalloc((size_t)n * n), *gamma0 = zalloc((size_t)n * n), *gamma1 = zalloc((size_t)n * n);
+    double complex *omega = zalloc(l), *omega_d = zalloc(l), *sigma = zalloc(l), *tau = zalloc((size_t)l * l);
+    double complex *Rs = zalloc(mn), *R_base = zalloc((l + 1) * mn), *PV_base = zalloc(2 * (l + 1) * mn);
+    double complex *tmp = zalloc(mn), t_local;
+    double complex **P = (double complex **)malloc((l + 1) * sizeof(double complex *));
+    double complex **V = (double complex **)malloc((l + 1) * sizeof(double complex *));
+    double complex **R = (double complex **)malloc((l + 1) * sizeof(double complex *));
+    int *ipiv = (int *)malloc((size_t)n * n * sizeof(int));
+    int *rp1 = (int *)malloc(sizeof(int) * (m + 1)), *ci1 = (int *)malloc(sizeof(int) * (rp[m] > 0 ? rp[m] : 1));
+    for (i = 0; i <= m; i++) rp1[i] = rp[i] + 1;               /* blbicgstabl.c:299-304 */
+    for (i = 0; i < rp[m]; i++) ci1[i] = ci[i] + 1;
+    omega[l - 1] = 1;                                          /* :283 */
+    for (i = 0; i <= l; i++) {                                 /* :305-312 */
+        P[i] = PV_base + (size_t)(2 * i) * mn;
+        V[i] = PV_base + (size_t)(2 * i + 1) * mn;
+        R[i] = R_base + (size_t)i * mn;
+    }
+    for (i = 0; i < n; i++) gamma0[i * n + i] = 1.0;
+
+    zgemm1(m, n, -1, v, rp1, ci1, X, 0, tmp);                  /* :322-330 */
+    orc_zdscal((int)mn, -1, tmp);
+    orc_zaxpy((int)mn, 1, B, tmp);
+    orc_zcopy((int)mn, tmp, V[0]);
+    orc_zqreco(m, n, V[0], xi);
+    orc_zcopy((int)mn, V[0], P[0]);
+    orc_zcopy((int)mn, V[0], Rs);
+    fnb = orc_dznrm2((int)mn, B);
+
+    for (iter = 0; iter < max_iter; iter++) {
+        orc_zscal(n * n, -omega[l - 1], gamma0);               /* :339-340 */
+        for (j = 0; j < l; j++) {
+            orc_zgemm('T', 'N', n, n, m, 1, Rs, m, V[j], m, 0, gamma1, n);   /* :344-350 */
+            orc_zcopy(n * n, gamma1, tmp);
+            orc_zgesv(n, n, gamma0, n, ipiv, tmp, n);
+            orc_zgemm('N', 'N', n, n, n, 1, alpha, n, tmp, n, 0, beta, n);
+            orc_zcopy(n * n, gamma1, gamma0);
+            for (i = 0; i <= j; i++) {                         /* :353-358 */
+                orc_zgemm('N', 'N', m, n, n, -1, P[i], m, beta, n, 0, tmp, m);
+                orc_zaxpy((int)mn, 1, V[i], tmp);
+                orc_zcopy((int)mn, tmp, P[i]);
+            }
+            zgemm1(m, n, 1, v, rp1, ci1, P[j], 0, P[j + 1]);
+            orc_zcopy(n * n, gamma1, alpha);                   /* :363-366 */
+            orc_zgemm('T', 'N', n, n, m, 1, Rs, m, P[j + 1], m, 0, tmp, n);
+            orc_zgesv(n, n, tmp, n, ipiv, alpha, n);
+            for (i = 0; i <= j; i++) {                         /* :369-400 */
+                if (i == 0) {
+                    orc_zgemm('N', 'N', m, n, n, 1, P[0], m, alpha, n, 0, tmp, m);
+                    zgemm1(m, n, -1, v, rp1, ci1, tmp, 1, V[0]);
+                    orc_zqreco(m, n, V[0], t);
+                    orc_zgemm('N', 'N', m, n, n, 1, tmp, m, xi, n, 1, X, m);
+                    orc_zgemm('N', 'N', n, n, n, 1, t, n, xi, n, 0, tmp, n);
+                    orc_zcopy(n * n, tmp, xi);
+                    orc_zgetrf(n, t, n, ipiv);
+                    orc_zgetri(n, t, n, ipiv);
+                } else {
+                    orc_zgemm('N', 'N', m, n, n, -1, P[i + 1], m, alpha, n, 1, V[i], m);
+                    orc_zgemm('N', 'N', m, n, n, 1, V[i], m, t, n, 0, tmp, m);
+                    orc_zcopy((int)mn, tmp, V[i]);
+                }
+            }
+            zgemm1(m, n, 1, v, rp1, ci1, V[j], 0, V[j + 1]);   /* :402-403 */
+        }
+        for (i = 0; i <= l; i++)                                /* :406-409 */
+            orc_zgemm('N', 'N', m, n, n, 1, V[i], m, xi, n, 0, R[i], m);
+        for (j = 0; j < l; j++) {                              /* MR :413-423 */
+            for (i = 0; i < j; i++) {
+                tau[j * l + i] = orc_zdotc((int)mn, R[i + 1], R[j + 1]) / sigma[i];
+                orc_zaxpy((int)mn, -tau[j * l + i], R[i + 1], R[j + 1]);
+            }
+            sigma[j] = orc_zdotc((int)mn, R[j + 1], R[j + 1]);
+            omega_d[j] = orc_zdotc((int)mn, R[j + 1], R[0]) / sigma[j];
+        }
+        omega[l - 1] = omega_d[l - 1];                         /* :424-432 */
+        for (j = l - 2; j >= 0; j--) {
+            t_local = 0;
+            for (i = j + 1; i < l; i++) t_local += tau[i * l + j] * omega[i];
+            omega[j] = omega_d[j] - t_local;
+        }
+        orc_zdscal((int)mn, 0, tmp);                           /* :442-452 */
+        for (i = 0; i < l; i++) {
+            orc_zaxpy((int)mn, omega[i], V[i], tmp);
+            orc_zaxpy((int)mn, -omega[i], P[i + 1], P[0]);
+        }
+        zgemm1(m, n, -1, v, rp1, ci1, tmp, 1, V[0]);
+        orc_zgemm('N', 'N', m, n, n, 1, tmp, m, xi, n, 1, X, m);
+        error = orc_dznrm2(n * n, xi) / fnb;                   /* :454-457 */
+        if (error < tol || isnan(error)) break;
+    }
+    zgemm1(m, n, -1, v, rp1, ci1, X, 0, tmp);                  /* :461-469 */
+    orc_zaxpy((int)mn, 1, B, tmp);
+    if (true_res) *true_res = orc_dznrm2((int)mn, tmp) / fnb;
+
+    free(ci1); free(rp1); free(ipiv); free(R); free(V); free(P); free(tmp);
+    free(PV_base); free(R_base); free(Rs); free(tau); free(sigma); free(omega_d); free(omega);
+    free(gamma1); free(gamma0); free(xi); free(t); free(beta); free(alpha);
+    if (error >= tol || isnan(error)) return NOT_CONVERGED_CODE;
     return iter;
 }

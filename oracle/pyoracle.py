@@ -245,6 +245,25 @@ def zbicgstabl(n, rp, ci, v, b, l, tol, max_iter, x0=None):
     return _zsolve(lib().orc_zbicgstabl, n, rp, ci, v, b, x0, l, c_dbl(tol), max_iter)
 
 
+def zqreco(V):
+    V = np.array(V, dtype=np.complex128, order="F", copy=True)
+    m, s = V.shape
+    R = np.zeros((s, s), np.complex128, order="F")
+    lib().orc_zqreco(m, s, _zp(V), _zp(R))
+    return V, R
+
+
+def zblbicgstabl(n, rp, ci, v, B, l, tol, max_iter, X0=None):
+    rp, ci, v = _zcsr(rp, ci, v)
+    B = np.asfortranarray(B, np.complex128)
+    s = B.shape[1]
+    X = np.array(X0 if X0 is not None else np.zeros((n, s)), dtype=np.complex128, order="F", copy=True)
+    tr = ctypes.c_double()
+    ret = lib().orc_zblbicgstabl(n, _ip(rp), _ip(ci), _zp(v), s, _zp(B), l, c_dbl(tol), max_iter, _zp(X),
+                                 ctypes.byref(tr))
+    return ret, X, tr.value
+
+
 def zshbicgstabl(n, rp, ci, v, b, l, sigma, tol, max_iter):
     rp, ci, v = _zcsr(rp, ci, v)
     sig = np.ascontiguousarray(sigma, np.complex128)

@@ -134,3 +134,32 @@ def test_zshbicgstabl_every_shift_solves_its_system():
         assert true_res(A, X[0], b) < 1e-10
         for k, s in enumerate(sig):
             assert true_res(A, X[k + 1], b, s) < 1e-8, (l, k, true_res(A, X[k + 1], b, s))
+
+
+def test_zqreco_invariants():
+    rng = np.random.default_rng(13)
+    V = zrand(rng, 300, 6)
+    Q, R = po.zqreco(V)
+    assert np.allclose(Q.conj().T @ Q, np.eye(6), atol=1e-14)
+    assert np.allclose(Q @ R, V, atol=1e-13)
+    assert np.allclose(np.tril(R, -1), 0) and np.abs(np.diag(R).imag).max() == 0.0   # zgeqrf: real diagonal
+    Qn, Rn = np.linalg.qr(V)          # LAPACK zgeqrf/zungqr: the same reflector convention
+    assert np.allclose(R, Rn, atol=1e-12) and np.allclose(Q, Qn, atol=1e-12)
+
+
+def test_zblbicgstabl_real_data_and_true_residual():
+    """On real data the complex block solver performs the real block solver's
+    operations (zgemm 'T' is dgemm 'T', zqreco is myqr); on a complex system every
+    column reaches the true residual."""
+    n, rp, ci, v = real_system()
+    B = np.random.default_rng(14).standard_normal((n, 3))
+    ret, X, tr = po.blbicgstabl(n, rp, ci, v, B, 2, 1e-10, 200)
+    zret, ZX, ztr = po.zblbicgstabl(n, rp, ci, v.astype(complex), B.astype(complex), 2, 1e-10, 200)
+    assert ret == zret and ret > 0 and np.abs(ZX.real - X).max() <= 1e-10 * np.abs(X).max()
+    n, rp, ci, v = zsystem(20, seed=15)
+    A = zcsr_dense(n, rp, ci, v)
+    B = zrand(np.random.default_rng(16), n, 4)
+    for l in (1, 2, 4):
+        ret, X, tr = po.zblbicgstabl(n, rp, ci, v, B, l, 1e-10, 500)
+        res = np.linalg.norm(B - A @ X) / np.linalg.norm(B)
+        assert ret >= 0 and res < 1e-8 and abs(tr - res) <= 1e-6 * res, (l, ret, res, tr)
