@@ -74,6 +74,8 @@ _SIGS = {
     "bkh_zbicgstabl": (c_int, [c_vp, c_vp, c_int, c_vp, c_dbl, c_int, c_vp]),
     "bkh_zshbicgstabl": (c_int, [c_vp, c_vp, c_int, c_dp, c_int, c_vp, c_dbl, c_int, c_vp, c_dp]),
     "bkh_zspmv_bench": (c_int, [c_vp, ctypes.c_char, c_int, c_dp]),
+    "bkh_zblbicgstabl": (c_int, [c_vp, c_int, c_vp, c_int, c_vp, c_dbl, c_int, c_vp]),
+    "bkh_zqr": (c_int, [c_int, c_int, c_vp, c_vp]),
     "bkh_spmv32": (c_int, [c_vp, c_vp, c_vp]),
     "bkh_dot": (c_int, [c_ll, c_vp, c_vp, c_dp]),
     "bkh_qr": (c_int, [c_int, c_int, c_vp, c_dp]),

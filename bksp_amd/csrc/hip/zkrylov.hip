@@ -1081,3 +1081,481 @@ extern "C" int bkh_zspmv_bench(const bkh_zmatrix *A, char op, int reps, double *
     *avg_ms = ms / reps;
     return 0;
 }
+
+// ---- block BiCGSTAB(l) (blbicgstabl.c:255-486) and its thin QR (zqreco, myqr.c:41-76)
+//
+// Blocks are m x s column-major (ld m), as zmat. The reference's Gram products are
+// bilinear (cblas_zgemm CblasTrans: Rs^T V); the QR's Gram is V^H V. zqreco is
+// CholeskyQR2 here: R has a real positive diagonal where zgeqrf's R has a real
+// diagonal of either sign, so the two differ by Q -> Q D, R -> D R with D = diag(+-1),
+// under which the block iterates are invariant (as for the real block solvers). The
+// s x s algebra (zgesv, zgetrf/zgetri, products, Cholesky) runs in one workgroup.
+#define ZB_SMAX 16
+#define ZB_SN (ZB_SMAX * ZB_SMAX)
+
+struct ZBState {
+    zd alpha[ZB_SN], beta[ZB_SN], t[ZB_SN], xi[ZB_SN], g0[ZB_SN], g1[ZB_SN];
+    zd R1[ZB_SN], R1i[ZB_SN], R2i[ZB_SN], work[2 * ZB_SN];
+    double fnb;
+};
+
+// C = A B (s x s, column-major, LDS or global), one thread per entry.
+__device__ void zsm_mul(int s, const zd *A, const zd *B, zd *C) {
+    const int t = threadIdx.x;
+    if (t < s * s) {
+        const int i = t % s, j = t / s;
+        zd acc = zmk(0.0, 0.0);
+        for (int k = 0; k < s; k++) acc = zadd(acc, zmul(A[i + k * s], B[k + j * s]));
+        C[t] = acc;
+    }
+    __syncthreads();
+}
+
+__device__ void zsm_copy(int s, const zd *A, zd *B) {
+    if ((int)threadIdx.x < s * s) B[threadIdx.x] = A[threadIdx.x];
+    __syncthreads();
+}
+
+// X = A^-1 B (Gauss-Jordan with partial pivoting on [A | B], max |re|+|im| pivot);
+// work: 2 s^2 entries, piv: 2 ints in LDS. Returns false (uniformly) if singular.
+__device__ bool zsm_solve(int s, const zd *A, const zd *B, zd *X, zd *work, int *piv) {
+    const int t = threadIdx.x;
+    zd *M = work;                                  // s x 2s, column-major, ld s
+    if (t < s * s) { M[t] = A[t]; M[s * s + t] = B[t]; }
+    if (t == 0) piv[1] = 1;
+    __syncthreads();
+    for (int k = 0; k < s; k++) {
+        if (t == 0) {
+            int p = k;
+            double best = fabs(M[k + k * s].x) + fabs(M[k + k * s].y);
+            for (int i = k + 1; i < s; i++) {
+                const double a = fabs(M[i + k * s].x) + fabs(M[i + k * s].y);
+                if (a > best) { best = a; p = i; }
+            }
+            piv[0] = p;
+            if (!(best > 0.0)) piv[1] = 0;
+        }
+        __syncthreads();
+        const int p = piv[0];
+        if (p != k && t < 2 * s) {
+            const zd a = M[k + t * s], b = M[p + t * s];
+            M[k + t * s] = b;
+            M[p + t * s] = a;
+        }
+        __syncthreads();
+        const zd inv = piv[1] ? zdiv(zmk(1.0, 0.0), M[k + k * s]) : zmk(0.0, 0.0);
+        __syncthreads();
+        if (t < 2 * s) M[k + t * s] = zmul(M[k + t * s], inv);
+        __syncthreads();
+        for (int e = t; e < s * 2 * s; e += blockDim.x) {
+            const int i = e % s, c = e / s;
+            if (i != k && c != k) M[i + c * s] = zsub(M[i + c * s], zmul(M[i + k * s], M[k + c * s]));
+        }
+        __syncthreads();
+        if (t < s && t != k) M[t + k * s] = zmk(0.0, 0.0);
+        __syncthreads();
+    }
+    if (t < s * s) X[t] = M[s * s + t];
+    __syncthreads();
+    return piv[1] != 0;
+}
+
+// Upper Cholesky factor of a Hermitian positive definite G (G = R^H R).
+__device__ bool zsm_chol(int s, const zd *G, zd *R, int *flag) {
+    const int t = threadIdx.x;
+    if (t == 0) *flag = 1;
+    if (t < s * s) {
+        const int i = t % s, j = t / s;
+        R[t] = (i <= j) ? G[t] : zmk(0.0, 0.0);
+    }
+    __syncthreads();
+    for (int k = 0; k < s; k++) {
+        if (t == 0) {
+            const double d = R[k + k * s].x;
+            if (!(d > 0.0)) *flag = 0;
+            R[k + k * s] = zmk(d > 0.0 ? sqrt(d) : 1.0, 0.0);
+        }
+        __syncthreads();
+        const double rkk = R[k + k * s].x;
+        if (t > k && t < s) R[k + t * s] = zmk(R[k + t * s].x / rkk, R[k + t * s].y / rkk);
+        __syncthreads();
+        if (t < s * s) {
+            const int i = t % s, j = t / s;
+            if (i > k && j >= i) R[i + j * s] = zsub(R[i + j * s], zmul(zconj(R[k + i * s]), R[k + j * s]));
+        }
+        __syncthreads();
+    }
+    return *flag != 0;
+}
+
+// Inverse of an upper-triangular R (column j by back substitution, thread j).
+__device__ void zsm_triu_inv(int s, const zd *R, zd *Ri) {
+    const int t = threadIdx.x;
+    if (t < s * s) Ri[t] = zmk(0.0, 0.0);
+    __syncthreads();
+    if (t < s) {
+        const int j = t;
+        for (int i = j; i >= 0; i--) {
+            zd acc = (i == j) ? zmk(1.0, 0.0) : zmk(0.0, 0.0);
+            for (int k = i + 1; k <= j; k++) acc = zsub(acc, zmul(R[i + k * s], Ri[k + j * s]));
+            Ri[i + j * s] = zdiv(acc, R[i + i * s]);
+        }
+    }
+    __syncthreads();
+}
+
+// Partial Gram of one workgroup's rows: G[i][j] = sum_r f(X[r,i]) Y[r,j], f = conj when
+// CONJ; 64-row tiles staged in LDS, one thread per entry.
+template <bool CONJ>
+__global__ __launch_bounds__(256) void k_zgram(long long m, int s, const zd *X, const zd *Y, const Ctl *c,
+                                               double *part) {
+    if (c && ctl_stopped(c)) return;
+    __shared__ zd tx[64 * ZB_SMAX], ty[64 * ZB_SMAX];
+    const int t = threadIdx.x, ss = s * s;
+    const int ei = t % s, ej = t / s;
+    zd acc = zmk(0.0, 0.0);
+    for (long long r0 = (long long)blockIdx.x * 64; r0 < m; r0 += (long long)gridDim.x * 64) {
+        const int rows = (int)min(64LL, m - r0);
+        for (int e = t; e < 64 * s; e += 256) {
+            const int r = e % 64, q = e / 64;
+            zd a = zmk(0.0, 0.0), b = zmk(0.0, 0.0);
+            if (r < rows) { a = X[(size_t)q * m + r0 + r]; b = Y[(size_t)q * m + r0 + r]; }
+            tx[q * 64 + r] = CONJ ? zconj(a) : a;
+            ty[q * 64 + r] = b;
+        }
+        __syncthreads();
+        if (t < ss)
+            for (int r = 0; r < rows; r++) acc = zadd(acc, zmul(tx[ei * 64 + r], ty[ej * 64 + r]));
+        __syncthreads();
+    }
+    if (t < ss) {
+        part[((size_t)blockIdx.x * ss + t) * 2] = acc.x;
+        part[((size_t)blockIdx.x * ss + t) * 2 + 1] = acc.y;
+    }
+}
+
+static inline int zgram_grid(long long m) { return (int)std::max(1LL, std::min((m + 63) / 64, 256LL)); }
+
+enum { ZB_QR1 = 0, ZB_QR2_INIT = 1, ZB_QR2_ITER = 2, ZB_BETA = 3, ZB_ALPHA = 4, ZB_XI = 5 };
+
+// Sums the Gram partials (fixed order) and runs the s x s step `op`:
+//   ZB_BETA (j): on j == 0 first g0 = (-omega[l-1]) g0 (:339-340); g1 = Rs^T V_j;
+//               beta = alpha (g0^-1 g1); g0 = g1                          (:344-350)
+//   ZB_ALPHA:   alpha = (Rs^T P_{j+1})^-1 g1                              (:363-366)
+//   ZB_QR1:     R1 = chol(V^H V), R1i = R1^-1
+//   ZB_QR2_*:   R2 = chol(V^H V), R2i = R2^-1; R = R2 R1 (zqreco's R);
+//               INIT: xi = R (:327); ITER: t = R (:373)
+//   ZB_XI:      xi = t xi, t = t^-1 (zgetrf/zgetri of the triangular t, :376-380);
+//               no partials
+__global__ __launch_bounds__(256) void k_zblk_fin(const double *part, int nblocks, int s, int op, int j, int l,
+                                                  ZBState *st, Ctl *c) {
+    if (ctl_stopped(c)) return;
+    __shared__ zd G[ZB_SN], T1[ZB_SN], T2[ZB_SN], W[2 * ZB_SN];
+    __shared__ int flag[2];
+    const int t = threadIdx.x, ss = s * s;
+    if (op == ZB_XI) {
+        zsm_mul(s, st->t, st->xi, G);
+        zsm_copy(s, G, st->xi);
+        zsm_copy(s, st->t, T1);
+        zsm_triu_inv(s, T1, st->t);
+        return;
+    }
+    if (t < ss) {
+        double re = 0.0, im = 0.0;
+        for (int b = 0; b < nblocks; b++) {
+            re += part[((size_t)b * ss + t) * 2];
+            im += part[((size_t)b * ss + t) * 2 + 1];
+        }
+        G[t] = zmk(re, im);
+    }
+    __syncthreads();
+    if (op == ZB_BETA) {
+        if (j == 0 && t < ss) st->g0[t] = zmul(zneg(zload(c->s, ZS_OMEGA / 2 + l - 1)), st->g0[t]);
+        __syncthreads();
+        zsm_copy(s, G, st->g1);
+        if (!zsm_solve(s, st->g0, G, T1, W, flag)) { if (t == 0) ctl_breakdown(c); return; }
+        zsm_mul(s, st->alpha, T1, st->beta);
+        zsm_copy(s, G, st->g0);
+        return;
+    }
+    if (op == ZB_ALPHA) {
+        if (!zsm_solve(s, G, st->g1, T1, W, flag)) { if (t == 0) ctl_breakdown(c); return; }
+        zsm_copy(s, T1, st->alpha);
+        return;
+    }
+    if (!zsm_chol(s, G, T1, flag)) { if (t == 0) ctl_breakdown(c); return; }
+    if (op == ZB_QR1) {
+        zsm_copy(s, T1, st->R1);
+        zsm_triu_inv(s, T1, st->R1i);
+        return;
+    }
+    zsm_triu_inv(s, T1, st->R2i);
+    zsm_mul(s, T1, st->R1, T2);                    // R = R2 R1
+    zsm_copy(s, T2, op == ZB_QR2_INIT ? st->xi : st->t);
+}
+
+// One row per thread, all s columns: O[r,:] = a (X[r,:] M) (+ b Y[r,:]); M from the
+// state (s x s). X may alias O only when Y is null or Y == O (row-local).
+__global__ __launch_bounds__(256) void k_zpan(long long m, int s, const zd *X, const zd *M, zd a, const zd *Y, zd b,
+                                              zd *O, const Ctl *c) {
+    if (c && ctl_stopped(c)) return;
+    ZLOOP(r, m) {
+        zd xr[ZB_SMAX];
+        for (int l = 0; l < s; l++) xr[l] = X[(size_t)l * m + r];
+        for (int q = 0; q < s; q++) {
+            zd acc = zmk(0.0, 0.0);
+            for (int l = 0; l < s; l++) acc = zadd(acc, zmul(xr[l], M[l + q * s]));
+            zd o = zmul(a, acc);
+            if (Y) o = zadd(o, zmul(b, Y[(size_t)q * m + r]));
+            O[(size_t)q * m + r] = o;
+        }
+    }
+}
+
+// V_i = ((-1) (P_{i+1} alpha) + 1 V_i) t, row-local (:392-396).
+__global__ __launch_bounds__(256) void k_zpan_vt(long long m, int s, zd *V, const zd *P1, const ZBState *st,
+                                                 const Ctl *c) {
+    if (ctl_stopped(c)) return;
+    ZLOOP(r, m) {
+        zd v[ZB_SMAX], p[ZB_SMAX];
+        for (int l = 0; l < s; l++) { p[l] = P1[(size_t)l * m + r]; v[l] = V[(size_t)l * m + r]; }
+        zd w[ZB_SMAX];
+        for (int q = 0; q < s; q++) {
+            zd acc = zmk(0.0, 0.0);
+            for (int l = 0; l < s; l++) acc = zadd(acc, zmul(p[l], st->alpha[l + q * s]));
+            w[q] = zadd(zmul(zmk(-1.0, 0.0), acc), zmul(zmk(1.0, 0.0), v[q]));
+        }
+        for (int q = 0; q < s; q++) {
+            zd acc = zmk(0.0, 0.0);
+            for (int l = 0; l < s; l++) acc = zadd(acc, zmul(w[l], st->t[l + q * s]));
+            V[(size_t)q * m + r] = zmul(zmk(1.0, 0.0), acc);
+        }
+    }
+}
+
+// tmp = sum_i omega_i V_i (from 0: zdscal(0) then zaxpy); P_0 += (-omega_i) P_{i+1},
+// i = 0..l-1 (:442-447), over all m s entries.
+__global__ __launch_bounds__(256) void k_zblk_upd(long long ms, int l, const zd *V, zd *P, zd *tmp, const Ctl *c) {
+    if (ctl_stopped(c)) return;
+    const zd *om = (const zd *)&c->s[ZS_OMEGA];
+    ZLOOP(e, ms) {
+        zd tv = zmk(0.0, 0.0), p0 = P[e];
+        for (int i = 0; i < l; i++) {
+            tv = zadd(tv, zmul(om[i], V[(size_t)i * ms + e]));
+            p0 = zadd(p0, zmul(zneg(om[i]), P[(size_t)(i + 1) * ms + e]));
+        }
+        tmp[e] = tv;
+        P[e] = p0;
+    }
+}
+
+// error = ||xi||_F / ||B||_F (:454-457): the block solvers' stop test, NaN stops.
+__global__ void k_zblk_check(int s, const ZBState *st, Ctl *c) {
+    if (ctl_stopped(c)) return;
+    if (threadIdx.x != 0) return;
+    double acc = 0.0;
+    for (int e = 0; e < s * s; e++) acc += st->xi[e].x * st->xi[e].x + st->xi[e].y * st->xi[e].y;
+    ctl_check(c, acc);
+}
+
+// MR part of the block solver (:413-432): as the shifted solver's (omega_dd unused).
+struct ZbSigFin {
+    int j, l;
+    const double *tau;
+    __device__ void operator()(Ctl *c, const double *tot) const {
+        ZshSigFin{j, l, tau}(c, tot);
+    }
+};
+
+__global__ void k_zblk_init(Ctl *c, ZBState *st, int s, int l) {
+    const int t = threadIdx.x;
+    if (t < s * s) {
+        st->g0[t] = (t % s == t / s) ? zmk(1.0, 0.0) : zmk(0.0, 0.0);
+        st->alpha[t] = zmk(0.0, 0.0);
+    }
+    if (t == 0) zstore(c->s, ZS_OMEGA / 2 + l - 1, zmk(1.0, 0.0));
+}
+
+// |B|_F^2 -> Ctl::fnb.
+__global__ __launch_bounds__(256) void k_znorm2(long long n, const zd *x, double *part) {
+    double v[1] = {0.0};
+    ZLOOP(i, n) v[0] += x[i].x * x[i].x + x[i].y * x[i].y;
+    block_partial<1, 256>(v, part);
+}
+
+struct ZFnbFin {
+    __device__ void operator()(Ctl *c, const double *tot) const {
+        c->fnb = sqrt(tot[0]);
+        if (!(c->fnb > 0.0)) ctl_breakdown(c);
+    }
+};
+
+static void launch_zspmm1(hipStream_t st, const bkh_zmatrix *A, int s, zd alpha, const zd *X, zd beta, zd *Y) {
+    const int G = (A->n + 255) / 256;
+    hipLaunchKernelGGL((k_zspmm1<false>), dim3(G), dim3(256), 0, st, A->n, s, A->A.so, A->A.rl, A->A.col, A->A.val,
+                       alpha, beta, X, Y);
+}
+
+// CholeskyQR2 of V (m x s) in place; op2 = ZB_QR2_INIT or ZB_QR2_ITER.
+static void zqr2(hipStream_t st, long long m, int s, zd *V, ZBState *bs, Ctl *c, double *part, int op2) {
+    const int Gg = zgram_grid(m), Gp = vec_grid(m, 256);
+    hipLaunchKernelGGL((k_zgram<true>), dim3(Gg), dim3(256), 0, st, m, s, (const zd *)V, (const zd *)V,
+                       (const Ctl *)c, part);
+    hipLaunchKernelGGL(k_zblk_fin, dim3(1), dim3(256), 0, st, (const double *)part, Gg, s, (int)ZB_QR1, 0, 0, bs, c);
+    hipLaunchKernelGGL(k_zpan, dim3(Gp), dim3(256), 0, st, m, s, (const zd *)V, (const zd *)bs->R1i, zmk(1.0, 0.0),
+                       (const zd *)nullptr, zmk(0.0, 0.0), V, (const Ctl *)c);
+    hipLaunchKernelGGL((k_zgram<true>), dim3(Gg), dim3(256), 0, st, m, s, (const zd *)V, (const zd *)V,
+                       (const Ctl *)c, part);
+    hipLaunchKernelGGL(k_zblk_fin, dim3(1), dim3(256), 0, st, (const double *)part, Gg, s, op2, 0, 0, bs, c);
+    hipLaunchKernelGGL(k_zpan, dim3(Gp), dim3(256), 0, st, m, s, (const zd *)V, (const zd *)bs->R2i, zmk(1.0, 0.0),
+                       (const zd *)nullptr, zmk(0.0, 0.0), V, (const Ctl *)c);
+}
+
+extern "C" int bkh_zblbicgstabl(const bkh_zmatrix *A, int s, const double *d_B, int l, double *d_X, double tol,
+                                int max_iter, bkh_info *info) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (!A || !d_B || !d_X || !zbl_ok(l) || s < 1 || s > ZB_SMAX) return BKH_BAD_ARGUMENT;
+    if (max_iter <= 0) {
+        if (info) { memset(info, 0, sizeof *info); info->status = ST_CONVERGED; }
+        return 0;
+    }
+    const long long m = A->n, ms = m * s;
+    const int Gv = vec_grid(ms, 256), Gg = zgram_grid(m), Gp = vec_grid(m, 256), l2 = l + 1;
+    const int nparts = std::max(Gg * ZB_SN * 2, 8 * Gv);
+    void *ws = bkh_workspace(2 * carve_size(sizeof(Ctl)) + carve_size(sizeof(ZBState)) +
+                             carve_size(sizeof(double) * nparts) + carve_size(sizeof(zd) * l * l) +
+                             (3 * l2 + 2) * carve_size(sizeof(zd) * ms));
+    if (!ws) return BKH_DEVICE_ERROR;
+    Carve cv(ws);
+    Ctl *c = cv.take<Ctl>(1), *co = cv.take<Ctl>(1);
+    ZBState *bs = cv.take<ZBState>(1);
+    double *part = cv.take<double>(nparts);
+    double *tau = (double *)cv.take<zd>((size_t)l * l);
+    zd *Pb = cv.take<zd>((size_t)l2 * ms), *Vb = cv.take<zd>((size_t)l2 * ms), *Rb = cv.take<zd>((size_t)l2 * ms);
+    zd *Rs = cv.take<zd>(ms), *tmp = cv.take<zd>(ms);
+    const zd *B = (const zd *)d_B;
+    zd *X = (zd *)d_X;
+    auto Pi = [=](int i) { return Pb + (size_t)i * ms; };
+    auto Vi = [=](int i) { return Vb + (size_t)i * ms; };
+    auto Ri = [=](int i) { return Rb + (size_t)i * ms; };
+    const zd one = zmk(1.0, 0.0), mone = zmk(-1.0, 0.0), zero = zmk(0.0, 0.0);
+    hipStream_t st = t->stream;
+    HIP_OK(hipEventRecord(t->ev0, st));
+    if (ctl_init(t, c, tol, max_iter)) return BKH_DEVICE_ERROR;
+    HIP_OK(hipMemsetAsync(Pb, 0, sizeof(zd) * l2 * ms, st));
+    HIP_OK(hipMemsetAsync(Vb, 0, sizeof(zd) * l2 * ms, st));
+    HIP_OK(hipMemsetAsync(tau, 0, sizeof(zd) * l * l, st));
+    hipLaunchKernelGGL(k_zblk_init, dim3(1), dim3(256), 0, st, c, bs, s, l);
+    // V0 = B - A X0 (:322-326: zcsrmm with -1, zdscal(-1), zaxpy(one, B))
+    HIP_OK(hipMemsetAsync(tmp, 0, sizeof(zd) * ms, st));
+    launch_zspmm1(st, A, s, mone, X, zero, tmp);
+    hipLaunchKernelGGL(k_zinit, dim3(Gv), dim3(256), 0, st, ms, B, (const zd *)tmp, Vi(0), (zd *)nullptr,
+                       (zd *)nullptr, (zd *)nullptr, part);
+    hipLaunchKernelGGL(k_znorm2, dim3(Gv), dim3(256), 0, st, ms, B, part);
+    launch_finalize<1>(st, part, Gv, c, ZFnbFin());
+    zqr2(st, m, s, Vi(0), bs, c, part, ZB_QR2_INIT);           // zqreco(V0, xi) (:327)
+    HIP_OK(hipMemcpyAsync(Pi(0), Vi(0), sizeof(zd) * ms, hipMemcpyDeviceToDevice, st));
+    HIP_OK(hipMemcpyAsync(Rs, Vi(0), sizeof(zd) * ms, hipMemcpyDeviceToDevice, st));
+    LAUNCH_OK();
+    hipGraphExec_t ge = capture_cached(t, {0x2B19, (uintptr_t)A, (uintptr_t)B, (uintptr_t)X, (uintptr_t)c,
+                                           (uintptr_t)part, (uintptr_t)Pb, (uintptr_t)l, (uintptr_t)s}, st, 1, [=]() {
+        for (int j = 0; j < l; j++) {
+            hipLaunchKernelGGL((k_zgram<false>), dim3(Gg), dim3(256), 0, st, m, s, (const zd *)Rs, (const zd *)Vi(j),
+                               (const Ctl *)c, part);
+            hipLaunchKernelGGL(k_zblk_fin, dim3(1), dim3(256), 0, st, (const double *)part, Gg, s, (int)ZB_BETA, j, l,
+                               bs, c);
+            for (int i = 0; i <= j; i++)                         // P_i = (-1) P_i beta + 1 V_i (:353-358)
+                hipLaunchKernelGGL(k_zpan, dim3(Gp), dim3(256), 0, st, m, s, (const zd *)Pi(i), (const zd *)bs->beta,
+                                   mone, (const zd *)Vi(i), one, Pi(i), (const Ctl *)c);
+            launch_zspmm1(st, A, s, one, Pi(j), zero, Pi(j + 1));
+            hipLaunchKernelGGL((k_zgram<false>), dim3(Gg), dim3(256), 0, st, m, s, (const zd *)Rs,
+                               (const zd *)Pi(j + 1), (const Ctl *)c, part);
+            hipLaunchKernelGGL(k_zblk_fin, dim3(1), dim3(256), 0, st, (const double *)part, Gg, s, (int)ZB_ALPHA, j,
+                               l, bs, c);
+            // i = 0 (:371-381): tmp = P0 alpha; V0 = V0 - A tmp; qr(V0) -> t; X += tmp xi;
+            // xi = t xi; t = t^-1
+            hipLaunchKernelGGL(k_zpan, dim3(Gp), dim3(256), 0, st, m, s, (const zd *)Pi(0), (const zd *)bs->alpha, one,
+                               (const zd *)nullptr, zero, tmp, (const Ctl *)c);
+            launch_zspmm1(st, A, s, mone, tmp, one, Vi(0));
+            zqr2(st, m, s, Vi(0), bs, c, part, ZB_QR2_ITER);
+            hipLaunchKernelGGL(k_zpan, dim3(Gp), dim3(256), 0, st, m, s, (const zd *)tmp, (const zd *)bs->xi, one,
+                               (const zd *)X, one, X, (const Ctl *)c);
+            hipLaunchKernelGGL(k_zblk_fin, dim3(1), dim3(256), 0, st, (const double *)part, 0, s, (int)ZB_XI, j,
+                               l, bs, c);
+            for (int i = 1; i <= j; i++)                         // :392-396
+                hipLaunchKernelGGL(k_zpan_vt, dim3(Gp), dim3(256), 0, st, m, s, Vi(i), (const zd *)Pi(i + 1),
+                                   (const ZBState *)bs, (const Ctl *)c);
+            launch_zspmm1(st, A, s, one, Vi(j), zero, Vi(j + 1));
+        }
+        for (int i = 0; i <= l; i++)                             // R_i = V_i xi (:406-409)
+            hipLaunchKernelGGL(k_zpan, dim3(Gp), dim3(256), 0, st, m, s, (const zd *)Vi(i), (const zd *)bs->xi, one,
+                               (const zd *)nullptr, zero, Ri(i), (const Ctl *)c);
+        for (int j = 0; j < l; j++) {                            // MR (:413-432)
+            zd *rj = Ri(j + 1);
+            for (int i = 0; i < j; i++) {
+                const zd *ri = Ri(i + 1);
+                launch_zdots<1>(st, ms, zdots(ri, rj), c, part);
+                launch_finalize<2>(st, part, Gv, c, ZblTauFin{i, j, l, tau});
+                hipLaunchKernelGGL(k_zbl_mgs, dim3(Gv), dim3(256), 0, st, ms, rj, ri, (const double *)tau, j * l + i,
+                                   (const Ctl *)c);
+            }
+            launch_zdots<2>(st, ms, zdots(rj, rj, rj, Ri(0)), c, part);
+            launch_finalize<4>(st, part, Gv, c, ZbSigFin{j, l, tau});
+        }
+        hipLaunchKernelGGL(k_zblk_upd, dim3(Gv), dim3(256), 0, st, ms, l, (const zd *)Vb, Pb, tmp, (const Ctl *)c);
+        launch_zspmm1(st, A, s, mone, tmp, one, Vi(0));
+        hipLaunchKernelGGL(k_zpan, dim3(Gp), dim3(256), 0, st, m, s, (const zd *)tmp, (const zd *)bs->xi, one,
+                           (const zd *)X, one, X, (const Ctl *)c);
+        hipLaunchKernelGGL(k_zblk_check, dim3(1), dim3(64), 0, st, s, (const ZBState *)bs, c);
+    });
+    if (!ge) { bkh_set_error("hipGraph capture of the complex block BiCGSTAB(l) iteration failed"); return BKH_DEVICE_ERROR; }
+    if (run_graph_until_done(t, ge, c, max_iter + 2)) return BKH_DEVICE_ERROR;
+    HIP_OK(hipEventRecord(t->ev1, st));
+    if (ctl_fetch(t, c)) return BKH_DEVICE_ERROR;
+    const Ctl hc = *t->ctl_host;
+    HIP_OK(hipEventSynchronize(t->ev1));
+    bkh_info inf;
+    memset(&inf, 0, sizeof inf);
+    const int ret = zresult(t, &hc, &inf);
+    if (info) {
+        // Frobenius true residual ||B - A X||_F / ||B||_F (:461-469)
+        if (ctl_init(t, co, 0.0, 1)) return BKH_DEVICE_ERROR;
+        HIP_OK(hipMemcpyAsync(tmp, B, sizeof(zd) * ms, hipMemcpyDeviceToDevice, st));
+        launch_zspmm1(st, A, s, mone, X, one, tmp);
+        hipLaunchKernelGGL(k_znorm2, dim3(Gv), dim3(256), 0, st, ms, (const zd *)tmp, part);
+        launch_finalize<1>(st, part, Gv, co, ZFnbFin());
+        LAUNCH_OK();
+        if (ctl_fetch(t, co)) return BKH_DEVICE_ERROR;
+        inf.true_res = t->ctl_host->fnb / hc.fnb;
+        *info = inf;
+    }
+    return ret;
+}
+
+// Thin QR of an m x s complex panel on the device (CholeskyQR2): d_V <- Q, R to the
+// host (s x s column-major, real positive diagonal). Replaces zqreco (myqr.c:41-76).
+extern "C" int bkh_zqr(int m, int s, double *d_V, double *R) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (!d_V || !R || m < 1 || s < 1 || s > ZB_SMAX || m < s) return BKH_BAD_ARGUMENT;
+    const int Gg = zgram_grid(m);
+    void *ws = bkh_workspace(carve_size(sizeof(Ctl)) + carve_size(sizeof(ZBState)) +
+                             carve_size(sizeof(double) * Gg * ZB_SN * 2));
+    if (!ws) return BKH_DEVICE_ERROR;
+    Carve cv(ws);
+    Ctl *c = cv.take<Ctl>(1);
+    ZBState *bs = cv.take<ZBState>(1);
+    double *part = cv.take<double>((size_t)Gg * ZB_SN * 2);
+    hipStream_t st = t->stream;
+    if (ctl_init(t, c, 0.0, 1)) return BKH_DEVICE_ERROR;
+    zqr2(st, m, s, (zd *)d_V, bs, c, part, ZB_QR2_INIT);
+    LAUNCH_OK();
+    if (ctl_fetch(t, c)) return BKH_DEVICE_ERROR;
+    if (t->ctl_host->status == ST_BREAKDOWN) {
+        bkh_set_error("bkh_zqr: the panel is numerically rank deficient");
+        return -2;
+    }
+    HIP_OK(hipMemcpy(R, bs->xi, sizeof(zd) * s * s, hipMemcpyDeviceToHost));
+    return 0;
+}

@@ -5,6 +5,8 @@
  *   zbicg          reference src/bicg.c:91-177
  *   zbicgstabl     reference src/bicgstabl.c:141-276
  *   zshbicgstabl   reference src/shbicgstabl.c:288-592
+ *   zblbicgstabl   reference src/blbicgstabl.c:255-486 (prints its summary, :461-469)
+ *   zqreco         reference src/myqr.c:41-76 (CholeskyQR2 on the GPU)
  *   zmat_create / zmat_free   src/util.c:72-88
  *   zhorner        src/util_shift.c:29-53 (host, scalar)
  *   zcsrgemm0 / zcsrgemm1     src/matvec.c:55-98, 174-217 (on the GPU)
@@ -86,6 +88,40 @@ int zshbicgstabl(zspmat *A, double _Complex *b, int l, double _Complex *sigma, i
     }
     zrelease(&u);
     return ret;
+}
+
+int zblbicgstabl(zspmat *A, zmat *B, int l, double tol, int max_iter, zmat *X) {
+    int m = *A->row_size, s = B->col_size;
+    if (s < 1 || s > 16 || l < 1 || l > 8) return INVALID_PARAMETERS;
+    size_t ms = (size_t)m * s;
+    bkh_info info;
+    memset(&info, 0, sizeof info);
+    zupload_t u;
+    int ret = zupload(A, B->value, ms, X->value, ms, &u);
+    if (ret == 0) {
+        ret = bkh_zblbicgstabl(u.M, s, u.d_b, l, u.d_x, tol, max_iter, &info);
+        if (ret > BKH_DEVICE_ERROR && bkh_d2h(X->value, u.d_x, sizeof(double _Complex) * ms)) ret = BKH_DEVICE_ERROR;
+    }
+    zrelease(&u);
+    if (ret > BKH_DEVICE_ERROR)
+        printf("Block BiCGSTAB(%d) has converged at:\n"
+               "        The number of iteration ... %d\n"
+               "        Frobenius norm of the relative residual ... %e\n"
+               "        Frobenius norm of the true residual ... %e\n",
+               l, max_iter <= 0 ? 0 : info.iters, info.rec_res, info.true_res);
+    fflush(stdout);
+    return ret;
+}
+
+int zqreco(int m, int n, double _Complex *mat, double _Complex *r) {
+    size_t bytes = sizeof(double _Complex) * (size_t)m * n;
+    double *d = (double *)bkh_malloc(bytes);
+    if (!d) return BKH_DEVICE_ERROR;
+    int rc = bkh_h2d(d, mat, bytes);
+    if (!rc) rc = bkh_zqr(m, n, d, (double *)r);
+    if (!rc) rc = bkh_d2h(mat, d, bytes);
+    bkh_free(d);
+    return rc;
 }
 
 zmat *zmat_create(int m, int n) {

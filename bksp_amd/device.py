@@ -236,6 +236,23 @@ def zbicgstabl(M, d_b, l, d_x, tol, max_iter):
     return info
 
 
+def zblbicgstabl(M, s, d_B, l, d_X, tol, max_iter):
+    """Complex block BiCGSTAB(l); d_B, d_X hold m x s column-major complex blocks."""
+    info = BkhInfo()
+    rc = load().bkh_zblbicgstabl(M.handle, s, d_B.ptr, l, d_X.ptr, tol, max_iter, ctypes.byref(info))
+    _check(rc, "bkh_zblbicgstabl")
+    return info
+
+
+def zqr(d_V, m, s):
+    """Thin QR of an m x s complex panel (column-major) in place; returns R."""
+    R = np.zeros((s, s), np.complex128, order="F")
+    rc = load().bkh_zqr(m, s, d_V.ptr, R.ctypes.data_as(ctypes.c_void_p))
+    if rc:
+        raise BkspDeviceError(f"bkh_zqr -> {rc}: {last_error()}")
+    return R
+
+
 def zshbicgstabl(M, d_b, l, sigma, d_x, tol, max_iter):
     """Complex shifted BiCGSTAB(l); d_x holds n*(len(sigma)+1) complex entries (seed
     first). Returns (info, per-system FP64 true residuals)."""

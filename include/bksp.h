@@ -134,6 +134,8 @@ int zbicg(zspmat *A, double _Complex *b, double tol, int max_iter, double _Compl
 int zbicgstabl(zspmat *A, double _Complex *b, int ell, double tol, int max_iter, double _Complex *x);
 int zshbicgstabl(zspmat *A, double _Complex *b, int l, double _Complex *sigma,
                  int sigma_size, double tol, int max_iter, double _Complex *x);
+int zblbicgstabl(zspmat *A, zmat *b, int ell, double tol, int max_iter, zmat *x);
+int zqreco(int m, int n, double _Complex *mat, double _Complex *r);
 #endif
 
 #ifdef __cplusplus

@@ -178,6 +178,13 @@ int bkh_zbicgstabl(const bkh_zmatrix *A, const double *d_b, int l, double *d_x, 
                    int max_iter, bkh_info *info);
 int bkh_zshbicgstabl(const bkh_zmatrix *A, const double *d_b, int l, const double *sigma, int ns,
                      double *d_x, double tol, int max_iter, bkh_info *info, double *shift_true_res);
+/* Replaces zblbicgstabl (src/blbicgstabl.c:255-486): B, X m x s column-major (s <= 16),
+ * Frobenius stop test on xi; info->true_res = ||B - A X||_F / ||B||_F. */
+int bkh_zblbicgstabl(const bkh_zmatrix *A, int s, const double *d_B, int l, double *d_X, double tol,
+                     int max_iter, bkh_info *info);
+/* Thin QR of an m x s (s <= 16) complex panel, CholeskyQR2: d_V <- Q, R (host, s x s
+ * column-major, real positive diagonal). Replaces zqreco (src/myqr.c:41-76). */
+int bkh_zqr(int m, int s, double *d_V, double *R);
 /* Average ms of reps back-to-back complex SpMV launches (op 'n' or 't'), HIP events. */
 int bkh_zspmv_bench(const bkh_zmatrix *A, char op, int reps, double *avg_ms);
 

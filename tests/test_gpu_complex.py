@@ -198,3 +198,69 @@ def test_dropin_zsolvers_and_kernels():
     assert lib.zcsrgemm1(ctypes.c_char(b"n"), vp(al), n, 2, vp(v), vp(rp1), vp(ci1), len(v), vp(B), vp(be),
                          vp(C1)) == 0
     assert np.array_equal(C1, po.zcsrgemm1("n", al[0], n, rp, ci, v, B, be[0], C0))
+
+
+def test_zqr_invariants():
+    """CholeskyQR2 on the device: Q^H Q = I, Q R = V, R upper triangular with a real
+    positive diagonal, equal to LAPACK's R up to the sign of each row."""
+    m, s = 500, 6
+    V = zrand(np.random.default_rng(17), m, s)
+    dV = dv.DeviceArray.from_numpy(V.ravel(order="F"))
+    R = dv.zqr(dV, m, s)
+    Q = dV.numpy().reshape(s, m).T
+    assert np.allclose(Q.conj().T @ Q, np.eye(s), atol=1e-13)
+    assert np.allclose(Q @ R, V, atol=1e-12)
+    assert np.allclose(np.tril(R, -1), 0) and np.all(np.diag(R).real > 0) and np.all(np.diag(R).imag == 0)
+    _, Rn = po.zqreco(V)
+    d = np.sign(np.diag(Rn).real)
+    assert np.allclose(R, d[:, None] * Rn, atol=1e-12)
+
+
+@pytest.mark.parametrize("l,s", [(1, 3), (2, 4), (4, 2)])
+def test_zblbicgstabl_parity(l, s):
+    n, rp, ci, v = zsystem(32, seed=18)
+    A = zcsr_dense(n, rp, ci, v)
+    B = zrand(np.random.default_rng(19), n, s)
+    M = dv.ZMatrix(n, rp, ci, v)
+    dB = dv.DeviceArray.from_numpy(np.asfortranarray(B).ravel(order="F"))
+    dX = dv.DeviceArray((n * s,), np.complex128)
+    dX.zero()
+    info = dv.zblbicgstabl(M, s, dB, l, dX, 1e-10, 500)
+    X = dX.numpy().reshape(s, n).T
+    cret, CX, ctr = po.zblbicgstabl(n, rp, ci, v, B, l, 1e-10, 500)
+    assert info.ret >= 0 and cret >= 0 and close_index(info.ret, cret), (l, s, info.ret, cret)
+    res = np.linalg.norm(B - A @ X) / np.linalg.norm(B)
+    assert res < 1e-8 and abs(info.true_res - res) <= 1e-6 * res, (res, info.true_res)
+    if info.ret == cret:
+        assert rel(X, CX) < 1e-7
+
+
+class ZMat(ctypes.Structure):                    # include/bksp.h zmat
+    _fields_ = [("row_size", ctypes.c_int), ("col_size", ctypes.c_int), ("value", ctypes.c_void_p)]
+
+
+def test_dropin_zblbicgstabl_and_zqreco(capfd):
+    n, rp, ci, v = zsystem(20, seed=20)
+    A = zcsr_dense(n, rp, ci, v)
+    rp = np.ascontiguousarray(rp, np.int32); ci = np.ascontiguousarray(ci, np.int32)
+    v = np.ascontiguousarray(v, np.complex128)
+    ints = [np.array([x], np.int32) for x in (len(v), n, n)]
+    IP = ctypes.POINTER(ctypes.c_int)
+    Z = ZSpMat(rp.ctypes.data_as(IP), ci.ctypes.data_as(IP), ints[0].ctypes.data_as(IP),
+               ints[1].ctypes.data_as(IP), ints[2].ctypes.data_as(IP), v.ctypes.data, b"csr", b"unsymmetric")
+    B = np.asfortranarray(zrand(np.random.default_rng(21), n, 3))
+    X = np.zeros((n, 3), np.complex128, order="F")
+    Bm, Xm = ZMat(n, 3, B.ctypes.data), ZMat(n, 3, X.ctypes.data)
+    lib = load()
+    lib.zblbicgstabl.restype = ctypes.c_int
+    lib.zblbicgstabl.argtypes = [ctypes.POINTER(ZSpMat), ctypes.POINTER(ZMat), ctypes.c_int, ctypes.c_double,
+                                 ctypes.c_int, ctypes.POINTER(ZMat)]
+    ret = lib.zblbicgstabl(ctypes.byref(Z), ctypes.byref(Bm), 2, 1e-10, 500, ctypes.byref(Xm))
+    assert ret >= 0 and np.linalg.norm(B - A @ X) < 1e-8 * np.linalg.norm(B)
+    assert "Block BiCGSTAB(2) has converged at:" in capfd.readouterr().out
+    V = np.asfortranarray(zrand(np.random.default_rng(22), 200, 4))
+    R = np.zeros((4, 4), np.complex128, order="F")
+    V0 = V.copy(order="F")
+    lib.zqreco.restype = ctypes.c_int
+    assert lib.zqreco(200, 4, ctypes.c_void_p(V.ctypes.data), ctypes.c_void_p(R.ctypes.data)) == 0
+    assert np.allclose(V @ R, V0, atol=1e-12) and np.allclose(V.conj().T @ V, np.eye(4), atol=1e-13)
