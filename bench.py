@@ -120,7 +120,8 @@ def bench_cfg3(dv, gen, m=96):
 
 
 def bench_complex(dv, gen, po, m=96, ns=16):
-    """Complex FP64 family: shifted BiCGSTAB(4) and BiCGSTAB(4) on the 96^3
+    """Complex FP64 family: shifted BiCGSTAB(4), BiCGSTAB(4) and block BiCGSTAB(4) (s = 4,
+    RHS seeds 305/306) on the 96^3
     convection-diffusion operator with an imaginary diagonal shift (i 0.05 a_ii),
     16 complex shifts sigma_k = 10^(-2 + 2k/15) (1 + 0.5i), RHS N(0,1)+iN(0,1) seeds
     303/304, tol 1e-8; complex SpMV roofline; oracle CPU baseline on a bounded sample."""
@@ -145,15 +146,28 @@ def bench_complex(dv, gen, po, m=96, ns=16):
     (sinfo, tr), sms = _timed(shifted, 1)
     single()
     info, ms = _timed(single, 1)
+    sb = 4
+    B = (gen.normal(305, n * sb) + 1j * gen.normal(306, n * sb)).reshape(sb, n)     # column-major n x sb
+    dB = dv.DeviceArray.from_numpy(B.ravel())
+    dX = dv.DeviceArray((n * sb,), np.complex128)
+
+    def block():
+        dX.zero()
+        return dv.zblbicgstabl(M, sb, dB, 4, dX, 1e-8, 1000)
+
+    block()
+    binfo, bms = _timed(block, 1)
     M.bench_spmv(reps=3)
     spmv_ms = M.bench_spmv(reps=20)
     alg = nnz * (16 + 4) + 4 * (n + 1) + 16 * n + 16 * n
     gbs = alg / (spmv_ms * 1e-3) / 1e9
-    out = {"workload": f"complex shifted BiCGSTAB(4) and BiCGSTAB(4), convection-diffusion {m}^3 + i 0.05 diag, "
-                       f"n={n}, nnz={nnz}, {ns} complex shifts, tol 1e-8",
+    out = {"workload": f"complex shifted BiCGSTAB(4), BiCGSTAB(4) and block BiCGSTAB(4) (s={sb}), convection-diffusion "
+                       f"{m}^3 + i 0.05 diag, n={n}, nnz={nnz}, {ns} complex shifts, tol 1e-8",
            "zshbicgstabl_ms": round(sms, 2), "zshbicgstabl_outer_iters": sinfo.ret,
            "zshbicgstabl_max_true_res": float(tr.max()),
            "zbicgstabl_ms": round(ms, 2), "zbicgstabl_outer_iters": info.ret, "zbicgstabl_true_res": info.true_res,
+           "zblbicgstabl_s4_ms": round(bms, 2), "zblbicgstabl_outer_iters": binfo.ret,
+           "zblbicgstabl_true_res": binfo.true_res,
            "roofline": {"bound": "hbm", "kernel": "k_zspmv<false> (complex SELL-64, lane per row)",
                         "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes": alg,
@@ -168,12 +182,18 @@ def bench_complex(dv, gen, po, m=96, ns=16):
         t0 = time.perf_counter()
         po.zshbicgstabl(n, rpi, cii, v, b, 4, sig, 1e-300, 1)
         pers = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        po.zblbicgstabl(n, rpi, cii, v, B.T, 4, 1e-300, 1)
+        perb = time.perf_counter() - t0
         out["cpu_baseline"] = {
             "zbicgstabl_ms": round(per1 * (info.ret + 1) * 1e3, 1), "zshbicgstabl_ms": round(pers * (sinfo.ret + 1) * 1e3, 1),
+            "zblbicgstabl_s4_ms": round(perb * (binfo.ret + 1) * 1e3, 1),
             "unit": "ms", "cores": 1, "kind": "port",
-            "sample": f"oracle (C restatement of src/bicgstabl.c:141-276 and src/shbicgstabl.c:288-592, gcc -O3, "
-                      f"1 thread): 2 outer iterations of zbicgstabl ({per1 * 1e3:.0f} ms each) and 1 of "
-                      f"zshbicgstabl ({pers * 1e3:.0f} ms), scaled to the device's outer iteration counts"}
+            "sample": f"oracle (C restatement of src/bicgstabl.c:141-276, src/shbicgstabl.c:288-592 and "
+                      f"src/blbicgstabl.c:255-486, gcc -O3, "
+                      f"1 thread): 2 outer iterations of zbicgstabl ({per1 * 1e3:.0f} ms each), 1 of "
+                      f"zshbicgstabl ({pers * 1e3:.0f} ms) and 1 of zblbicgstabl s=4 ({perb * 1e3:.0f} ms), "
+                      f"scaled to the device's outer iteration counts"}
     return out
 
 
