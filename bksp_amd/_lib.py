@@ -31,6 +31,18 @@ class DMat(ctypes.Structure):
     _fields_ = [("row_size", c_int), ("col_size", c_int), ("value", c_dp)]
 
 
+class ZSpMat(ctypes.Structure):
+    """The reference's ``zspmat`` (include/bksp.h:113-122): CSR with double _Complex values."""
+    _fields_ = [("row", c_ip), ("col", c_ip), ("nnz", c_ip), ("row_size", c_ip),
+                ("col_size", c_ip), ("value", c_vp), ("format", ctypes.c_char_p),
+                ("type", ctypes.c_char_p)]
+
+
+class ZMat(ctypes.Structure):
+    """The reference's ``zmat`` (include/bksp.h:124-128): column-major complex block."""
+    _fields_ = [("row_size", c_int), ("col_size", c_int), ("value", c_vp)]
+
+
 class BkhOpts(ctypes.Structure):
     _fields_ = [("precision", c_int), ("graph_iters", c_int), ("fix_blcg_p0", c_int),
                 ("max_inner", c_int), ("inner_tol", c_dbl), ("path", c_int)]
@@ -139,6 +151,11 @@ _SIGS = {
     "dblbicgstab": (c_int, [ctypes.POINTER(DSpMat), ctypes.POINTER(DMat), c_dbl, c_int, ctypes.POINTER(DMat)]),
     "dshbicg": (c_int, [ctypes.POINTER(DSpMat), c_dp, c_dp, c_int, c_dbl, c_int, c_dp]),
     "dshbicgstabl": (c_int, [ctypes.POINTER(DSpMat), c_dp, c_int, c_dp, c_int, c_dbl, c_int, c_dp]),
+    "zbicg": (c_int, [ctypes.POINTER(ZSpMat), c_vp, c_dbl, c_int, c_vp]),
+    "zbicgstabl": (c_int, [ctypes.POINTER(ZSpMat), c_vp, c_int, c_dbl, c_int, c_vp]),
+    "zshbicgstabl": (c_int, [ctypes.POINTER(ZSpMat), c_vp, c_int, c_vp, c_int, c_dbl, c_int, c_vp]),
+    "zblbicgstabl": (c_int, [ctypes.POINTER(ZSpMat), ctypes.POINTER(ZMat), c_int, c_dbl, c_int,
+                             ctypes.POINTER(ZMat)]),
     "read_matrix_market": (ctypes.POINTER(DSpMat), [ctypes.c_char_p]),
     "myqr": (c_int, [c_int, c_int, c_dp, c_dp]),
     "sp_norm_f": (c_dbl, [ctypes.POINTER(DSpMat)]),

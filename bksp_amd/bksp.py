@@ -10,7 +10,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import DSpMat, load
+from ._lib import DSpMat, ZMat, ZSpMat, load
 
 NOT_CONVERGED = -1
 DIVERGENCE = -2
@@ -119,3 +119,72 @@ def dshbicgstabl(A, b, l, sigma, tol, max_iter, x):
     entries: seed first, then shift k at x[(k+1)*n:]."""
     sig = np.ascontiguousarray(sigma, dtype=np.float64)
     return load().dshbicgstabl(A.ref, _dp(b), l, _dp(sig), len(sig), tol, max_iter, _dp(x))
+
+
+# ------------------------------------------------------------- complex family
+
+class ZSpMatrix:
+    """Owns the arrays behind a ``zspmat`` (include/bksp.h:113-122); values complex128."""
+
+    def __init__(self, n, row_ptr, col, val, mtype="unsymmetric"):
+        self.rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+        self.ci = np.ascontiguousarray(col, dtype=np.int32)
+        self.val = np.ascontiguousarray(val, dtype=np.complex128)
+        self._n = ctypes.c_int(int(n))
+        self._m = ctypes.c_int(int(n))
+        self._nnz = ctypes.c_int(int(self.rp[-1]))
+        self._fmt = b"csr"
+        self._type = mtype.encode()
+        self.s = ZSpMat(self.rp.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                        self.ci.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                        ctypes.pointer(self._nnz), ctypes.pointer(self._n), ctypes.pointer(self._m),
+                        self.val.ctypes.data, self._fmt, self._type)
+        self.n = int(n)
+
+    @property
+    def ref(self):
+        return ctypes.byref(self.s)
+
+
+class ZMatrix:
+    """Owns the column-major complex array behind a ``zmat`` (include/bksp.h:124-128)."""
+
+    def __init__(self, a):
+        self.a = np.asfortranarray(a, dtype=np.complex128)
+        self.s = ZMat(self.a.shape[0], self.a.shape[1], self.a.ctypes.data)
+
+    @property
+    def ref(self):
+        return ctypes.byref(self.s)
+
+
+def zspmat(n, row_ptr, col, val):
+    return ZSpMatrix(n, row_ptr, col, val)
+
+
+def _zp(a):
+    assert a.dtype == np.complex128 and a.flags["C_CONTIGUOUS"]
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def zbicg(A, b, tol, max_iter, x):
+    """Complex BiCG (reference src/bicg.c:91-177)."""
+    return load().zbicg(A.ref, _zp(b), tol, max_iter, _zp(x))
+
+
+def zbicgstabl(A, b, l, tol, max_iter, x):
+    """Complex BiCGSTAB(l) (reference src/bicgstabl.c:141-276); returns the outer index."""
+    return load().zbicgstabl(A.ref, _zp(b), l, tol, max_iter, _zp(x))
+
+
+def zshbicgstabl(A, b, l, sigma, tol, max_iter, x):
+    """Complex shifted BiCGSTAB(l) (reference src/shbicgstabl.c:288-592); x has
+    n*(len(sigma)+1) entries, seed first."""
+    sig = np.ascontiguousarray(sigma, dtype=np.complex128)
+    return load().zshbicgstabl(A.ref, _zp(b), l, _zp(sig), len(sig), tol, max_iter, _zp(x))
+
+
+def zblbicgstabl(A, B, l, tol, max_iter, X):
+    """Complex block BiCGSTAB(l) (reference src/blbicgstabl.c:255-486); prints the
+    reference's summary. B, X are ``ZMatrix`` (X overwritten)."""
+    return load().zblbicgstabl(A.ref, B.ref, l, tol, max_iter, X.ref)
