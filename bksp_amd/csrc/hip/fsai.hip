@@ -756,7 +756,11 @@ static int pcg_fused(ThreadCtx *t, const bkh_matrix *A, const bkh_matrix *G, int
 
 static int pcg_run(ThreadCtx *t, const bkh_matrix *A, const bkh_matrix *G, int kind, const double *b, double *x,
                    double tol, int max_iter, const bkh_opts &o, bkh_info *inf) {
-    if (o.path != 1) {
+    // The fused loop pays off for FSAI (three kernels instead of six launches); with a
+    // Jacobi or no preconditioner the separate SpMV and vector kernels measured faster
+    // (77 against 80 us per iteration at 128^3), so those run fused only on request.
+    const bool fused = (o.path == 0 && kind == PCK_FSAI) || o.path == 2;
+    if (fused) {
         const int rc = pcg_fused(t, A, G, kind, b, x, tol, max_iter, o, inf);
         if (rc != 1) return rc;
     }
@@ -893,10 +897,10 @@ __device__ __forceinline__ bool pf_beta(Ctl *c, int k, const double *partPrev, i
     return true;
 }
 
-template <typename IT>
+template <typename IT, typename ZT>
 __global__ __launch_bounds__(PF_BS) void k_pf_a(int n, int nsl, int chunk, const int *__restrict__ rp,
                                               const int *__restrict__ so, const double *__restrict__ val,
-                                              const IT *__restrict__ col, const double *z, double *P0, double *P1,
+                                              const IT *__restrict__ col, const ZT *z, double *P0, double *P1,
                                               double *q, Ctl *c, int j, const double *partPrev, int np,
                                               double *partA) {
     if (ctl_stopped(c)) return;
@@ -913,10 +917,10 @@ __global__ __launch_bounds__(PF_BS) void k_pf_a(int n, int nsl, int chunk, const
 #pragma unroll 4
         for (int kk = 0; kk < len; kk++) {
             const int e = base + 64 * kk, cc = sell_col(col, e, row);
-            acc = fma(val[e], fma(be, pold[cc], z[cc]), acc);
+            acc = fma(val[e], fma(be, pold[cc], (double)z[cc]), acc);
         }
         q[row] = acc;
-        const double pn = fma(be, pold[row], z[row]);
+        const double pn = fma(be, pold[row], (double)z[row]);
         pnew[row] = pn;
         dv[0] += pn * acc;
     }
@@ -958,11 +962,13 @@ __global__ __launch_bounds__(PF_BS) void k_pf_b_vec(int n, int nsl, int chunk, d
 }
 
 // PB(k) for FSAI: x, r_{k+1} own; y = G r_{k+1} with r_{k+1}[j] = r_k[j] - alpha q[j].
+// y is stored in FP32: the factor G itself is held in FP32, so the preconditioner
+// z = G^T G r is an FP32-accurate operator either way (r, x, p, q stay FP64).
 template <typename IT>
 __global__ __launch_bounds__(PF_BS) void k_pf_b_g(int n, int nsl, int chunk, const int *__restrict__ rp,
                                                 const int *__restrict__ so, const float *__restrict__ val,
                                                 const IT *__restrict__ col, double *x, double *R0, double *R1,
-                                                const double *P0, const double *P1, const double *q, double *y,
+                                                const double *P0, const double *P1, const double *q, float *y,
                                                 Ctl *c, int j, const double *partA, int npA) {
     if (ctl_stopped(c)) return;
     const int k = c->base + j;
@@ -978,18 +984,19 @@ __global__ __launch_bounds__(PF_BS) void k_pf_b_g(int n, int nsl, int chunk, con
             const int e = base + 64 * kk, cc = sell_col(col, e, row);
             acc = fma((double)val[e], fma(-al, q[cc], rk[cc]), acc);
         }
-        y[row] = acc;
+        y[row] = (float)acc;
         x[row] = fma(al, p[row], x[row]);
         rn[row] = fma(-al, q[row], rk[row]);
     }
 }
 
-// PD(k): z = G^T y; partials {r_{k+1}.z, r_{k+1}.r_{k+1}}.
+// PD(k): z = G^T y (FP32 storage, as y); partials {r_{k+1}.z, r_{k+1}.r_{k+1}} with the
+// stored z, so beta and the direction update see the same values.
 template <typename IT>
 __global__ __launch_bounds__(PF_BS) void k_pf_d(int n, int nsl, int chunk, const int *__restrict__ rp,
                                               const int *__restrict__ so, const float *__restrict__ val,
-                                              const IT *__restrict__ col, const double *y, const double *R0,
-                                              const double *R1, double *z, Ctl *c, int j, double *partD) {
+                                              const IT *__restrict__ col, const float *y, const double *R0,
+                                              const double *R1, float *z, Ctl *c, int j, double *partD) {
     if (ctl_stopped(c)) return;
     const int k = c->base + j;
     const double *rn = (k & 1) ? R0 : R1;
@@ -1000,11 +1007,12 @@ __global__ __launch_bounds__(PF_BS) void k_pf_d(int n, int nsl, int chunk, const
 #pragma unroll 4
         for (int kk = 0; kk < len; kk++) {
             const int e = base + 64 * kk, cc = sell_col(col, e, row);
-            acc = fma((double)val[e], y[cc], acc);
+            acc = fma((double)val[e], (double)y[cc], acc);
         }
-        z[row] = acc;
+        const float zs = (float)acc;
+        z[row] = zs;
         const double r = rn[row];
-        dv[0] += r * acc;
+        dv[0] += r * (double)zs;
         dv[1] += r * r;
     }
     block_partial<2, PF_BS>(dv, partD);
@@ -1017,6 +1025,11 @@ __global__ __launch_bounds__(PF_BS) void k_pf_tail(Ctl *c, int K, const double *
     double be;
     if (!pf_beta(c, k, partPrev, np, threadIdx.x == 0, be)) return;
     if (threadIdx.x == 0) c->base = k;
+}
+
+__global__ __launch_bounds__(256) void k_pf_d2f(long long n, const double *src, float *dst) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        dst[i] = (float)src[i];
 }
 
 // Start of the fused loop: r.z of iteration 0 from the unfused init (PC_RZ).
@@ -1044,18 +1057,23 @@ static void pf_capture_body(hipStream_t s, const bkh_matrix *A, const bkh_matrix
                             double *partA, double *partB) {
     const int n = A->n, nsl = A->nsl;
     for (int j = 0; j < K; j++) {
-        hipLaunchKernelGGL((k_pf_a<ITA>), dim3(GA), dim3(PF_BS), 0, s, n, nsl, chunk, A->rp, A->sso,
-                           (const double *)A->sval, sell_cols<ITA>(A), (const double *)z, P, P + n, q, c, j,
-                           (const double *)partB, GA, partA);
         if (kind == PCK_FSAI) {
+            // z and y in FP32 (their buffers hold n doubles each; the first 4n bytes are used)
+            hipLaunchKernelGGL((k_pf_a<ITA, float>), dim3(GA), dim3(PF_BS), 0, s, n, nsl, chunk, A->rp, A->sso,
+                               (const double *)A->sval, sell_cols<ITA>(A), (const float *)z, P, P + n, q, c, j,
+                               (const double *)partB, GA, partA);
             hipLaunchKernelGGL((k_pf_b_g<ITG>), dim3(GA), dim3(PF_BS), 0, s, n, nsl, chunk, G->rp, G->sso,
                                (const float *)G->sval32, sell_cols<ITG>(G), x, R, R + n, (const double *)P,
-                               (const double *)(P + n), (const double *)q, y, c, j, (const double *)partA, GA);
+                               (const double *)(P + n), (const double *)q, (float *)y, c, j, (const double *)partA,
+                               GA);
             const bkh_matrix *Gt = G->T;
             hipLaunchKernelGGL((k_pf_d<ITT>), dim3(GA), dim3(PF_BS), 0, s, n, nsl, chunk, Gt->rp, Gt->sso,
-                               (const float *)Gt->sval32, sell_cols<ITT>(Gt), (const double *)y, (const double *)R,
-                               (const double *)(R + n), z, c, j, partB);
+                               (const float *)Gt->sval32, sell_cols<ITT>(Gt), (const float *)y, (const double *)R,
+                               (const double *)(R + n), (float *)z, c, j, partB);
         } else {
+            hipLaunchKernelGGL((k_pf_a<ITA, double>), dim3(GA), dim3(PF_BS), 0, s, n, nsl, chunk, A->rp, A->sso,
+                               (const double *)A->sval, sell_cols<ITA>(A), (const double *)z, P, P + n, q, c, j,
+                               (const double *)partB, GA, partA);
             hipLaunchKernelGGL(k_pf_b_vec, dim3(GA), dim3(PF_BS), 0, s, n, nsl, chunk, x, R, R + n, (const double *)P,
                                (const double *)(P + n), (const double *)q, z,
                                kind == PCK_JACOBI ? (const double *)A->diag : (const double *)nullptr, c, j,
@@ -1101,6 +1119,10 @@ static int pcg_fused(ThreadCtx *t, const bkh_matrix *A, const bkh_matrix *G, int
     if (kind == PCK_FSAI) fs_apply(s, G, R, y, z, c, part, PcRzFin{1});
     // P0 = z_0 (finite; the first PA forms p_0 = z_0 + 0 * P0)
     hipLaunchKernelGGL(k_pcg_p, dim3(Gv), dim3(256), 0, s, n, P, (const double *)z, (const Ctl *)c, 1);
+    if (kind == PCK_FSAI) {            // the fused FSAI loop keeps z (and y) in FP32
+        hipLaunchKernelGGL(k_pf_d2f, dim3(Gv), dim3(256), 0, s, (long long)n, (const double *)z, (float *)y);
+        HIP_OK(hipMemcpyAsync(z, y, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+    }
     hipLaunchKernelGGL(k_pf_start, dim3(1), dim3(64), 0, s, c);
     LAUNCH_OK();
     const bool a16 = A->scd != nullptr;
