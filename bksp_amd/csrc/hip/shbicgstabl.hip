@@ -457,6 +457,11 @@ struct MsCol {
     const int k0 = (int)(threadIdx.x % G) * 4;                                     \
     const size_t SP = (size_t)G * 4
 #define MS_ROWS(i) for (long long i = tid_ / G; i < n; i += rstride_)
+// A group of 4 shifts whose shifts are all frozen keeps its d and r whatever the
+// kernels do (alpha = omega = 0), so its rows are skipped (no loads or stores); its
+// partials stay 0, which the finalize ignores for inactive shifts.
+#define MS_LIVE(mc) ((mc)[k0].act | (mc)[k0 + 1].act | (mc)[k0 + 2].act | (mc)[k0 + 3].act)
+#define MS_LIVE_ROWS(i, mc) if (MS_LIVE(mc)) MS_ROWS(i)
 
 // Sums NV x 4 per-shift values over the threads of one shift group (gq = lane % G,
 // G a power of two <= 64) and writes part[(blockIdx.x * NV + v) * SP + k], fixed order.
@@ -529,7 +534,7 @@ __global__ __launch_bounds__(256) void k_ms_k0(long long n, int G, const float *
     fvec be, om;
 #pragma unroll
     for (int u = 0; u < 4; u++) { be[u] = (float)mc[k0 + u].beta; om[u] = (float)mc[k0 + u].omega; }
-    MS_ROWS(i) {
+    MS_LIVE_ROWS(i, mc) {
         const size_t o = (size_t)i * SP + k0;
         *(fvec *)(Pv + o) = *(const fvec *)(R + o) + be * (*(const fvec *)(Pv + o) - om * *(const fvec *)(Vv + o));
     }
@@ -542,7 +547,7 @@ template <typename VT, int NV>
 __global__ __launch_bounds__(256) void k_ms_spmv(long long n, int G, const int *__restrict__ rp,
                                                  const int *__restrict__ ci, const VT *__restrict__ val,
                                                  const float *X, float *Y, const float *W, const ShState *st,
-                                                 const Ctl *c, double *part) {
+                                                 const MsCol *mc, const Ctl *c, double *part) {
     if (ctl_stopped(c)) return;
     typedef float fvec __attribute__((ext_vector_type(4)));
     MS_ROWS_SETUP;
@@ -554,7 +559,7 @@ __global__ __launch_bounds__(256) void k_ms_spmv(long long n, int G, const int *
     for (int a = 0; a < NV; a++)
 #pragma unroll
         for (int u = 0; u < 4; u++) dv[a][u] = 0.0;
-    MS_ROWS(i) {
+    MS_LIVE_ROWS(i, mc) {
         fvec acc = {0.0f, 0.0f, 0.0f, 0.0f};
         for (int e = rp[i]; e < rp[i + 1]; e++)
             acc += (float)val[e] * *(const fvec *)(X + (size_t)ci[e] * SP + k0);
@@ -586,7 +591,7 @@ __global__ __launch_bounds__(256) void k_ms_s(long long n, int G, const float *R
     fvec al;
 #pragma unroll
     for (int u = 0; u < 4; u++) al[u] = (float)mc[k0 + u].alpha;
-    MS_ROWS(i) {
+    MS_LIVE_ROWS(i, mc) {
         const size_t o = (size_t)i * SP + k0;
         *(fvec *)(S + o) = *(const fvec *)(R + o) - al * *(const fvec *)(Vv + o);
     }
@@ -603,7 +608,7 @@ __global__ __launch_bounds__(256) void k_ms_k3(long long n, int G, float *D, flo
 #pragma unroll
     for (int u = 0; u < 4; u++) { al[u] = (float)mc[k0 + u].alpha; om[u] = (float)mc[k0 + u].omega; }
     double dv[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
-    MS_ROWS(i) {
+    MS_LIVE_ROWS(i, mc) {
         const size_t o = (size_t)i * SP + k0;
         const fvec sv = *(const fvec *)(S + o);
         *(fvec *)(D + o) = *(const fvec *)(D + o) + al * *(const fvec *)(Pv + o) + om * sv;
@@ -744,22 +749,22 @@ static int ms_refine(ThreadCtx *t, const bkh_matrix *A, const double *b, double 
                                (const float *)B.V, (const MsCol *)mc, (const Ctl *)c);
             if (v32)
                 hipLaunchKernelGGL((k_ms_spmv<float, 1>), dim3(g.blocks), dim3(256), 0, s, n, g.G, A->rp, A->ci, v32,
-                                   (const float *)B.P, B.V, (const float *)B.RH, st, (const Ctl *)c, part);
+                                   (const float *)B.P, B.V, (const float *)B.RH, st, (const MsCol *)mc, (const Ctl *)c, part);
             else
                 hipLaunchKernelGGL((k_ms_spmv<double, 1>), dim3(g.blocks), dim3(256), 0, s, n, g.G, A->rp, A->ci,
                                    (const double *)A->val, (const float *)B.P, B.V, (const float *)B.RH, st,
-                                   (const Ctl *)c, part);
+                                   (const MsCol *)mc, (const Ctl *)c, part);
             hipLaunchKernelGGL((k_ms_fin<1, MsAlphaOp>), dim3(SP), dim3(256), 0, s, (const double *)part, g.blocks,
                                SP, mc, (const Ctl *)c, MsAlphaOp());
             hipLaunchKernelGGL(k_ms_s, dim3(g.blocks), dim3(256), 0, s, n, g.G, (const float *)B.R,
                                (const float *)B.V, B.S, (const MsCol *)mc, (const Ctl *)c);
             if (v32)
                 hipLaunchKernelGGL((k_ms_spmv<float, 2>), dim3(g.blocks), dim3(256), 0, s, n, g.G, A->rp, A->ci, v32,
-                                   (const float *)B.S, B.T, (const float *)nullptr, st, (const Ctl *)c, part);
+                                   (const float *)B.S, B.T, (const float *)nullptr, st, (const MsCol *)mc, (const Ctl *)c, part);
             else
                 hipLaunchKernelGGL((k_ms_spmv<double, 2>), dim3(g.blocks), dim3(256), 0, s, n, g.G, A->rp, A->ci,
                                    (const double *)A->val, (const float *)B.S, B.T, (const float *)nullptr, st,
-                                   (const Ctl *)c, part);
+                                   (const MsCol *)mc, (const Ctl *)c, part);
             hipLaunchKernelGGL((k_ms_fin<2, MsOmegaOp>), dim3(SP), dim3(256), 0, s, (const double *)part, g.blocks,
                                SP, mc, (const Ctl *)c, MsOmegaOp());
             hipLaunchKernelGGL(k_ms_k3, dim3(g.blocks), dim3(256), 0, s, n, g.G, B.D, B.R, (const float *)B.P,
