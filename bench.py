@@ -350,7 +350,13 @@ def main():
     M.bench_cg_kernel(reps=5)
     cga_ms = M.bench_cg_kernel(reps=50)
     cga_alg = 12 * nnz + 4 * (n + 1) + 32 * n           # A, row pointers, p_old and r read, q and p_new written
-    cga_fmt = M.spmv_bytes() - 16 * n + 32 * n            # as stored (SELL entries, 16-bit column deltas)
+    if M.dia_entries > 0:   # as stored: slice-diagonal values, per-slice offsets and slice offsets
+        cga_fmt = 8 * M.dia_entries + 4 * (M.dia_entries // 64) + 4 * ((n + 63) // 64 + 1) + 32 * n
+        cga_kernel = ("k_cgf_a_sell<double,double,int,1024,DIA> (CG: SpMV on the slice-diagonal copy with "
+                      "p = r + beta p in the gather)")
+    else:                   # as stored (SELL entries, 16-bit column deltas)
+        cga_fmt = M.spmv_bytes() - 16 * n + 32 * n
+        cga_kernel = "k_cgf_a_sell<double,double,short,1024> (CG: SELL-64 SpMV with p = r + beta p in the gather)"
     cga_gbs = cga_alg / (cga_ms * 1e-3) / 1e9
     dy = dv.DeviceArray((n,))
     dv.bench_spmv(M, db, dy, reps=5)
@@ -370,8 +376,7 @@ def main():
 
     t_cga = load_traffic("cg_fused_poisson128_traffic.json")
     t_spmv = load_traffic("spmv_fp64_poisson128_traffic.json")
-    roofline = {"bound": "hbm", "kernel": "k_cgf_a_sell<double,double,short,1024> (CG: SELL-64 SpMV with p = r + beta p "
-                                          "in the gather)",
+    roofline = {"bound": "hbm", "kernel": cga_kernel,
                 "achieved": round(cga_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(cga_gbs / HBM_PEAK_GBS, 4),
                 "traffic": t_cga.get("bytes_per_launch") if t_cga else None,

@@ -73,6 +73,7 @@ _SIGS = {
     "bkh_matrix_create": (c_vp, [c_int, c_vp, c_vp, c_vp, c_int, c_int, c_int]),
     "bkh_matrix_destroy": (None, [c_vp]),
     "bkh_matrix_nnz": (c_ll, [c_vp]),
+    "bkh_matrix_dia_entries": (c_ll, [c_vp]),
     "bkh_matrix_rows": (c_int, [c_vp]),
     "bkh_matrix_blocks": (c_int, [c_vp]),
     "bkh_matrix_set_spmv": (c_int, [c_vp, c_int]),

@@ -174,10 +174,35 @@ __global__ __launch_bounds__(64 * CGF_WPB) void k_cgf_a(int nwblk, int chunk, co
     block_partial<1, 64 * CGF_WPB>(dv, partA);
 }
 
-// A(k) on the SELL-64 layout: a lane owns a row and forms p_k at each column of it
-// in the gather; workgroup g owns `chunk` consecutive slices, its waves taking every
-// fourth. IT: int columns or short (column - row).
-template <typename VT, typename T, typename IT, int BS>
+// Calls f(e, column) for the stored entries of row `row`, lane l of slice sl (sl the
+// same on every lane of the wave), in the row's order. SELL-64: the row's length from
+// rp, IT int columns or short (column - row). Slice-diagonal (DIA, IT int): the
+// slice's width from so, column = row + the slice's offset of the position (uniform,
+// scalar loads), clamped into [0, n) for the value-0 positions a row does not have.
+template <typename IT, bool DIA, typename F>
+__device__ __forceinline__ void sell_row_entries(int n, int row, int l, int sl, const int *__restrict__ rp,
+                                                 const int *__restrict__ so, const IT *__restrict__ col, F &&f) {
+    if constexpr (DIA) {
+        const int s0 = so[sl], w = (so[sl + 1] - s0) >> 6;
+        const int *o = (const int *)col + (s0 >> 6);
+        const int base = s0 + l;
+#pragma unroll 4
+        for (int kk = 0; kk < w; kk++) f(base + 64 * kk, min(max(row + o[kk], 0), n - 1));
+    } else {
+        const int len = rp[row + 1] - rp[row];
+        const int base = so[sl] + l;
+#pragma unroll 4
+        for (int kk = 0; kk < len; kk++) {
+            const int e = base + 64 * kk;
+            f(e, sizeof(IT) == 2 ? row + (int)col[e] : (int)col[e]);
+        }
+    }
+}
+
+// A(k) on the SELL-64 layout (or its slice-diagonal copy, DIA): a lane owns a row and
+// forms p_k at each column of it in the gather; workgroup g owns `chunk` consecutive
+// slices, its waves taking every fourth. IT: int columns or short (column - row).
+template <typename VT, typename T, typename IT, int BS, bool DIA = false>
 __global__ __launch_bounds__(BS) void k_cgf_a_sell(int n, int nsl, int chunk, const int *__restrict__ rp,
                                                     const int *__restrict__ so, const VT *__restrict__ val,
                                                     const IT *__restrict__ col, const T *r, T *P0, T *P1, T *q,
@@ -198,22 +223,17 @@ __global__ __launch_bounds__(BS) void k_cgf_a_sell(int n, int nsl, int chunk, co
     const T *pold = (k & 1) ? P1 : P0;
     T *pnew = (k & 1) ? P0 : P1;
     const int g = xcd_block(blockIdx.x, gridDim.x);
-    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int l = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int send = min(nsl, (g + 1) * chunk);
     double dv[1] = {0.0};
     for (int sl = g * chunk + w; sl < send; sl += BS / 64) {
         const int row = sl * 64 + l;
         if (row >= n) continue;
-        const int len = rp[row + 1] - rp[row];
-        const int base = so[sl] + l;
         T acc = (T)0;
-#pragma unroll 4
-        for (int kk = 0; kk < len; kk++) {
-            const int e = base + 64 * kk;
-            const int cc = sizeof(IT) == 2 ? row + (int)col[e] : (int)col[e];
+        sell_row_entries<IT, DIA>(n, row, l, sl, rp, so, col, [&](int e, int cc) {
             const T pc = add_rn(mul_rn(be, pold[cc]), r[cc]);
             acc = add_rn(acc, mul_rn((T)val[e], pc));
-        }
+        });
         q[row] = acc;
         const T pn = add_rn(mul_rn(be, pold[row]), r[row]);
         pnew[row] = pn;
@@ -345,9 +365,10 @@ __device__ __forceinline__ bool cg1_scalars(Ctl *c, int k, const double *partPre
     return true;
 }
 
-// Kernel k on the SELL-64 layout (lane per row, workgroup g owns `chunk` slices).
-// R, Q, P hold two buffers each (parity), n entries apart; x32 is updated in place.
-template <typename VT, typename IT, int BS>
+// Kernel k on the SELL-64 layout or its slice-diagonal copy (DIA; lane per row,
+// workgroup g owns `chunk` slices). R, Q, P hold two buffers each (parity), n entries
+// apart; x32 is updated in place.
+template <typename VT, typename IT, int BS, bool DIA = false>
 __global__ __launch_bounds__(BS) void k_cg1_sell(int n, int nsl, int chunk, const int *__restrict__ rp,
                                                   const int *__restrict__ so, const VT *__restrict__ val,
                                                   const IT *__restrict__ col, float *x, float *R, float *Q,
@@ -364,30 +385,21 @@ __global__ __launch_bounds__(BS) void k_cg1_sell(int n, int nsl, int chunk, cons
     const float *rp_ = R + po, *qp_ = Q + po, *pp_ = P + po;
     float *rn_ = R + pn, *qn_ = Q + pn, *pn_ = P + pn;
     const int g = xcd_block(blockIdx.x, gridDim.x);
-    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int l = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int send = min(nsl, (g + 1) * chunk);
     double dv[3] = {0.0, 0.0, 0.0};
     for (int sl = g * chunk + w; sl < send; sl += BS / 64) {
         const int row = sl * 64 + l;
         if (row >= n) continue;
-        const int len = rp[row + 1] - rp[row];
-        const int base = so[sl] + l;
         float acc = 0.0f;
-        if (start) {
-#pragma unroll 4
-            for (int kk = 0; kk < len; kk++) {
-                const int e = base + 64 * kk;
-                const int cc = sizeof(IT) == 2 ? row + (int)col[e] : (int)col[e];
+        if (start)
+            sell_row_entries<IT, DIA>(n, row, l, sl, rp, so, col, [&](int e, int cc) {
                 acc = add_rn(acc, mul_rn((float)val[e], fmaf(be, pp_[cc], rp_[cc])));
-            }
-        } else {
-#pragma unroll 4
-            for (int kk = 0; kk < len; kk++) {
-                const int e = base + 64 * kk;
-                const int cc = sizeof(IT) == 2 ? row + (int)col[e] : (int)col[e];
+            });
+        else
+            sell_row_entries<IT, DIA>(n, row, l, sl, rp, so, col, [&](int e, int cc) {
                 acc = add_rn(acc, mul_rn((float)val[e], fmaf(-al, qp_[cc], rp_[cc])));
-            }
-        }
+            });
         const float pold = pp_[row], qold = qp_[row];
         const float rr = fmaf(-al, qold, rp_[row]);
         const float pnew = fmaf(be, pold, rr);

@@ -61,6 +61,14 @@ void bkh_set_error(const char *fmt, ...);
 // stays below SELL_MAX_PAD of the nonzeros.
 #define SELL_MAX_PAD 1.2
 enum { SPMV_BLOCK = 0, SPMV_WAVE = 1, SPMV_SELL = 2, SPMV_SELL32 = 3 };
+// Slice-diagonal copy (banded matrices, e.g. stencils): the SELL-64 slices with every
+// stored position k of a slice carrying one column offset o_k for all 64 rows
+// (column = row + o_k), the offsets of a slice the ascending union of its rows'
+// column - row values, absent entries stored as value 0 at a column clamped into
+// [0, n). No per-entry index stream; built when every row's columns ascend (so the
+// ascending offsets keep each row's summation order) and the copy stays below
+// DIA_MAX_PAD of the nonzeros. Used by the CG iteration kernels.
+#define DIA_MAX_PAD 1.1
 
 // Device-resident CSR. Rows are stored in the reference's accumulation order
 // (see bkh_matrix_create), so a sequential per-row sum reproduces src/matvec.c.
@@ -84,6 +92,11 @@ struct bkh_matrix {
     float *sval32 = nullptr;           // FP32 copy
     int *scol = nullptr;               // SELL columns (padding: the row itself)
     short *scd = nullptr;              // column - row, when every delta fits in 16 bits
+    long long dia_entries = 0;         // slice-diagonal copy (0: none)
+    int *dso = nullptr;                // its slice offsets in entries, nsl + 1
+    double *dval = nullptr;            // its values (absent entries 0)
+    float *dval32 = nullptr;           // FP32 copy
+    int *doff = nullptr;               // column offset of stored position k of slice q at dso[q] / 64 + k
     int variant = SPMV_WAVE;           // kernel used by every SpMV on this matrix
     int max_row = 0;                   // longest row (nonzeros)
     bkh_matrix *T = nullptr;           // A^T in the same format (BiCG)

@@ -101,6 +101,12 @@ static inline bool cgf_sell(const bkh_matrix *A) {
     return A->nsl > 0 && (A->variant == SPMV_SELL || A->variant == SPMV_SELL32);
 }
 
+// The CG iteration kernels run on the slice-diagonal copy when the matrix has one and
+// the default SELL-64 SpMV is selected (SELL32 keeps the int-column kernels).
+static inline bool cg_dia(const bkh_matrix *A) {
+    return A->dia_entries > 0 && A->nsl > 0 && A->variant == SPMV_SELL;
+}
+
 static inline bool cg_can_fuse(const bkh_matrix *A) { return cgf_sell(A) || A->max_row <= SPMV_WCAP; }
 
 struct CgfGrid { int GA, chunk, GB, BSA; bool sell; };
@@ -135,7 +141,14 @@ static void launch_cgf_a_sell(hipStream_t s, const bkh_matrix *A, const CgfGrid 
     const VT *sv;
     if constexpr (sizeof(VT) == sizeof(float)) sv = (const VT *)A->sval32;
     else sv = (const VT *)A->sval;
-    if (A->scd && A->variant == SPMV_SELL)
+    const VT *dv;
+    if constexpr (sizeof(VT) == sizeof(float)) dv = (const VT *)A->dval32;
+    else dv = (const VT *)A->dval;
+    if (cg_dia(A) && dv)
+        hipLaunchKernelGGL((k_cgf_a_sell<VT, T, int, BS, true>), dim3(g.GA), dim3(BS), 0, s, (int)A->n, A->nsl,
+                           g.chunk, A->rp, A->dso, dv, (const int *)A->doff, (const T *)r, P0, P1, q, c, j,
+                           (const double *)partB, g.GB, partA);
+    else if (A->scd && A->variant == SPMV_SELL)
         hipLaunchKernelGGL((k_cgf_a_sell<VT, T, short, BS>), dim3(g.GA), dim3(BS), 0, s, (int)A->n, A->nsl,
                            g.chunk, A->rp, A->sso, sv, (const short *)A->scd, (const T *)r, P0, P1, q, c, j,
                            (const double *)partB, g.GB, partA);
@@ -356,7 +369,11 @@ static Cg1Grid cg1_grid(const bkh_matrix *A) {
 template <int BS>
 static void launch_cg1(hipStream_t s, const bkh_matrix *A, const Cg1Grid &g, float *x32, float *R, float *Q,
                        float *P, Ctl *c, int j, const double *pin, double *pout) {
-    if (A->scd)
+    if (cg_dia(A) && A->dval32)
+        hipLaunchKernelGGL((k_cg1_sell<float, int, BS, true>), dim3(g.G), dim3(BS), 0, s, (int)A->n, A->nsl, g.chunk,
+                           A->rp, A->dso, (const float *)A->dval32, (const int *)A->doff, x32, R, Q, P, c, j,
+                           pin, g.G, pout);
+    else if (A->scd)
         hipLaunchKernelGGL((k_cg1_sell<float, short, BS>), dim3(g.G), dim3(BS), 0, s, (int)A->n, A->nsl, g.chunk,
                            A->rp, A->sso, (const float *)A->sval32, (const short *)A->scd, x32, R, Q, P, c, j,
                            pin, g.G, pout);

@@ -368,6 +368,7 @@ static void free_matrix(bkh_matrix *A) {
     hipFree(A->rp); hipFree(A->ci); hipFree(A->val); hipFree(A->val32);
     hipFree(A->diag); hipFree(A->blk); hipFree(A->wblk); hipFree(A->cd16);
     hipFree(A->sso); hipFree(A->sval); hipFree(A->sval32); hipFree(A->scol); hipFree(A->scd);
+    hipFree(A->dso); hipFree(A->dval); hipFree(A->dval32); hipFree(A->doff);
     if (A->T) free_matrix(A->T);
     delete A;
 }
@@ -430,6 +431,66 @@ static bool upload_sell(bkh_matrix *A, int n, const std::vector<int> &rp, const 
     A->nsl = nsl;
     A->sell_entries = tot;
     A->variant = SPMV_SELL;
+    return true;
+}
+
+// Slice-diagonal copy of a SELL matrix (see bkh_host.hpp) when it qualifies;
+// BKH_DIA=0 in the environment turns it off. Returns false only on a device error.
+static bool upload_dia(bkh_matrix *A, int n, const std::vector<int> &rp, const std::vector<int> &ci,
+                       const std::vector<double> &v, int want_fp32, hipStream_t s) {
+    const char *env = getenv("BKH_DIA");
+    if (env && env[0] == '0') return true;
+    const int nsl = A->nsl;
+    if (nsl <= 0 || A->nnz == 0) return true;
+    for (int r = 0; r < n; r++)
+        for (int k = rp[r] + 1; k < rp[r + 1]; k++)
+            if (ci[k] <= ci[k - 1]) return true;                   // a row not in ascending order
+    std::vector<int> so((size_t)nsl + 1, 0), offs;
+    std::vector<int> u;
+    long long tot = 0;
+    const long long cap = (long long)(DIA_MAX_PAD * (double)A->nnz) + 64;
+    for (int q = 0; q < nsl; q++) {
+        u.clear();
+        for (int r = q * 64; r < std::min(n, q * 64 + 64); r++)
+            for (int k = rp[r]; k < rp[r + 1]; k++) u.push_back(ci[k] - r);
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        tot += 64LL * (long long)u.size();
+        if (tot > cap || tot >= (1LL << 31) - 64) return true;
+        so[q + 1] = (int)tot;
+        offs.insert(offs.end(), u.begin(), u.end());
+    }
+    std::vector<double> dv((size_t)tot + 4, 0.0);
+    for (int q = 0; q < nsl; q++) {
+        const int *o = offs.data() + so[q] / 64;
+        for (int r = q * 64; r < std::min(n, q * 64 + 64); r++) {
+            int p = 0;
+            for (int k = rp[r]; k < rp[r + 1]; k++) {
+                while (o[p] != ci[k] - r) p++;                     // both ascend; o holds every offset
+                dv[(size_t)so[q] + 64 * (size_t)p + (r & 63)] = v[k];
+                p++;
+            }
+        }
+    }
+    const size_t ne = (size_t)tot + 4;
+    bool ok = hipMalloc(&A->dso, sizeof(int) * (nsl + 1)) == hipSuccess &&
+              hipMalloc(&A->dval, sizeof(double) * ne) == hipSuccess &&
+              hipMalloc(&A->doff, sizeof(int) * (offs.size() + 1)) == hipSuccess;
+    if (ok && want_fp32) ok = hipMalloc(&A->dval32, sizeof(float) * ne) == hipSuccess;
+    if (!ok) return false;
+    offs.push_back(0);
+    ok = hipMemcpyAsync(A->dso, so.data(), sizeof(int) * (nsl + 1), hipMemcpyHostToDevice, s) == hipSuccess &&
+         hipMemcpyAsync(A->dval, dv.data(), sizeof(double) * ne, hipMemcpyHostToDevice, s) == hipSuccess &&
+         hipMemcpyAsync(A->doff, offs.data(), sizeof(int) * offs.size(), hipMemcpyHostToDevice, s) == hipSuccess;
+    std::vector<float> d32;
+    if (ok && want_fp32) {
+        d32.resize(ne);
+        for (size_t e = 0; e < ne; e++) d32[e] = (float)dv[e];
+        ok = hipMemcpyAsync(A->dval32, d32.data(), sizeof(float) * ne, hipMemcpyHostToDevice, s) == hipSuccess;
+    }
+    ok = ok && hipStreamSynchronize(s) == hipSuccess;
+    if (!ok) return false;
+    A->dia_entries = tot;
     return true;
 }
 
@@ -502,6 +563,7 @@ static bkh_matrix *upload(int n, const std::vector<int> &rp, const std::vector<i
         ok = hipMemcpyAsync(A->val32, v32.data(), sizeof(float) * A->nnz, hipMemcpyHostToDevice, s) == hipSuccess;
     }
     if (ok) ok = upload_sell(A, n, rp, ci, v, want_fp32, s);
+    if (ok) ok = upload_dia(A, n, rp, ci, v, want_fp32, s);
     if (!ok || hipStreamSynchronize(s) != hipSuccess) {
         bkh_set_error("matrix upload failed");
         free_matrix(A);
@@ -545,6 +607,7 @@ extern "C" void bkh_matrix_destroy(bkh_matrix *A) {
 }
 
 extern "C" long long bkh_matrix_nnz(const bkh_matrix *A) { return A ? A->nnz : -1; }
+extern "C" long long bkh_matrix_dia_entries(const bkh_matrix *A) { return A ? A->dia_entries : -1; }
 extern "C" int bkh_matrix_rows(const bkh_matrix *A) { return A ? A->n : -1; }
 extern "C" int bkh_matrix_blocks(const bkh_matrix *A) { return A ? spmv_grid(A) : -1; }
 

@@ -88,6 +88,7 @@ class Matrix:
         if not self.handle:
             raise BkspDeviceError("bkh_matrix_create failed: " + last_error())
         self.nnz = load().bkh_matrix_nnz(self.handle)
+        self.dia_entries = load().bkh_matrix_dia_entries(self.handle)
         self.nblocks = load().bkh_matrix_blocks(self.handle)
         del self._rp, self._ci, self._v
 

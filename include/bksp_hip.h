@@ -85,6 +85,9 @@ bkh_matrix *bkh_matrix_create(int n, const int *row_ptr, const int *col,
                               int want_transpose, int want_fp32);
 void        bkh_matrix_destroy(bkh_matrix *A);
 long long   bkh_matrix_nnz(const bkh_matrix *A);
+/* Stored entries of the slice-diagonal copy the CG iteration kernels use (banded
+ * matrices whose rows ascend; 0: none; BKH_DIA=0 at creation turns it off). */
+long long   bkh_matrix_dia_entries(const bkh_matrix *A);
 int         bkh_matrix_rows(const bkh_matrix *A);
 int         bkh_matrix_blocks(const bkh_matrix *A);
 /* SpMV kernel variant: 0 = workgroup CSR-stream (256 rows / 2048 nonzeros per

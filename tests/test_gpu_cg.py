@@ -201,3 +201,71 @@ def test_eager_mode_same_results():
             subprocess.run([sys.executable, "-c", code, f], env=env, check=True, timeout=120)
             res.append(np.load(f))
     assert np.array_equal(res[0], res[1])
+
+
+def _matrix_without_dia(*args, **kw):
+    """The same matrix built with the slice-diagonal copy turned off (BKH_DIA=0 is
+    read when a matrix is created)."""
+    import os
+    old = os.environ.get("BKH_DIA")
+    os.environ["BKH_DIA"] = "0"
+    try:
+        return dv.Matrix(*args, **kw)
+    finally:
+        if old is None:
+            del os.environ["BKH_DIA"]
+        else:
+            os.environ["BKH_DIA"] = old
+
+
+@pytest.mark.parametrize("m", [15, 40])
+def test_cg_dia_layout_bitwise_equal_to_sell(m):
+    """The CG kernels on the slice-diagonal copy (per-slice column offsets, value-0
+    positions for the entries a row lacks, columns clamped into [0, n)) give the
+    solutions of the SELL-64 kernels bit for bit, FP64 and FP32-IR: the positions keep
+    each row's ascending order and a 0 product leaves a sum unchanged. m = 15: slices
+    straddle grid lines and the ends of the matrix (n = 3375, not a multiple of 64)."""
+    n, rp, ci, v = gen.poisson3d(m, lower=True)
+    b = gen.normal(31, n)
+    A = dv.Matrix(n, rp, ci, v, symmetric=True, fp32=True)
+    B = _matrix_without_dia(n, rp, ci, v, symmetric=True, fp32=True)
+    assert A.dia_entries >= A.nnz and A.dia_entries <= 1.1 * A.nnz + 64
+    assert B.dia_entries == 0
+    db = dv.DeviceArray.from_numpy(b)
+    for prec in (0, dv.BKH_FP32_IR):
+        out = []
+        for M in (A, B):
+            dx = dv.DeviceArray.from_numpy(np.zeros(n))
+            info = dv.cg(M, db, dx, 1e-9, 5000, precision=prec)
+            assert info.ret >= 0 and info.true_res < 1e-9
+            out.append((info.ret, info.inner_iters, dx.numpy()))
+        assert out[0][:2] == out[1][:2]
+        assert np.array_equal(out[0][2], out[1][2])
+
+
+def test_dia_layout_only_for_banded_ascending_rows():
+    """Random columns (a union of offsets far wider than a row) get no slice-diagonal
+    copy; the CG solve then runs on the SELL-64 kernels and still matches the oracle."""
+    rng = np.random.default_rng(4)
+    n = 2000
+    rows = []
+    for i in range(n):
+        cols = sorted(set(rng.integers(0, i + 1, size=3).tolist()) - {i})
+        rows.append(cols)
+    rp, ci, v = [0], [], []
+    for i, cols in enumerate(rows):
+        for j in cols:
+            ci.append(j); v.append(-0.1)
+        ci.append(i); v.append(4.0)
+        rp.append(len(ci))
+    rp, ci, v = np.array(rp, np.int32), np.array(ci, np.int32), np.array(v)
+    M = dv.Matrix(n, rp, ci, v, symmetric=True, fp32=True)
+    assert M.dia_entries == 0
+    b = gen.normal(3, n)
+    cret, cx = po.cg(n, rp, ci, v, b, 1e-10, 500)
+    db, dx = dv.DeviceArray.from_numpy(b), dv.DeviceArray.from_numpy(np.zeros(n))
+    info = dv.cg(M, db, dx, 1e-10, 500)
+    assert cret >= 0 and info.ret >= 0 and close_index(info.ret, cret), (info.ret, cret)
+    assert info.true_res < 1.5e-10
+    if info.ret == cret:
+        assert rel(dx.numpy(), cx) < 1e-8
