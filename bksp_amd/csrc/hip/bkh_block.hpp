@@ -434,7 +434,11 @@ constexpr int spmm_lpr() {
 // (contiguous for neighbouring rows) instead of a quarter of each of 64 rows' cache
 // lines. Per column the sum runs over the row's entries in stored order with the same
 // roundings as k_spmm_sell, so both give identical results. X, Y 16-byte aligned.
-template <int S, typename VT, typename T, typename IT>
+// DIA: the slice-diagonal copy (so, val, col = its slice offsets, values and per-slice
+// column offsets; a wave's rows lie in one slice, so the offsets are uniform loads);
+// its value-0 positions add 0 products at clamped columns, which leaves every sum as
+// on the SELL copy for finite X.
+template <int S, typename VT, typename T, typename IT, bool DIA = false>
 __global__ __launch_bounds__(256) void k_spmm_sell_v(int n, const int *__restrict__ rp, const int *__restrict__ so,
                                                      const VT *__restrict__ val, const IT *__restrict__ col,
                                                      const T *X, T *Y, double alpha, double beta, const Ctl *ctl) {
@@ -446,8 +450,18 @@ __global__ __launch_bounds__(256) void k_spmm_sell_v(int n, const int *__restric
     const int row = xcd_block(blockIdx.x, gridDim.x) * RPB + (int)threadIdx.x / LPR;
     const int sub = (int)threadIdx.x % LPR;
     if (row >= n) return;
-    const int len = rp[row + 1] - rp[row];
-    const int base = so[row >> 6] + (row & 63);
+    int len, base;
+    const int *dof = nullptr;
+    if constexpr (DIA) {
+        const int sl = __builtin_amdgcn_readfirstlane(row >> 6);
+        const int s0 = so[sl];
+        len = (so[sl + 1] - s0) >> 6;
+        base = s0 + (row & 63);
+        dof = (const int *)col + (s0 >> 6);
+    } else {
+        len = rp[row + 1] - rp[row];
+        base = so[row >> 6] + (row & 63);
+    }
     const bool a1 = (alpha == 1.0);
     T acc[CPL];
     if (beta != 0.0) {
@@ -461,9 +475,14 @@ __global__ __launch_bounds__(256) void k_spmm_sell_v(int n, const int *__restric
 #pragma unroll 2
     for (int kk = 0; kk < len; kk++) {
         const int e = base + 64 * kk;
-        const IT ce = __builtin_nontemporal_load(col + e);     // the matrix streams once:
-        const VT ve = __builtin_nontemporal_load(val + e);     // keep X's lines in L2
-        const int c = sizeof(IT) == 2 ? row + (int)ce : (int)ce;
+        int c;
+        if constexpr (DIA) {
+            c = min(max(row + dof[kk], 0), n - 1);
+        } else {
+            const IT ce = __builtin_nontemporal_load(col + e);  // the matrix streams once:
+            c = sizeof(IT) == 2 ? row + (int)ce : (int)ce;        // keep X's lines in L2
+        }
+        const VT ve = __builtin_nontemporal_load(val + e);
         const T a = a1 ? (T)ve : (T)(alpha * (double)ve);
         const tvec xv = *(const tvec *)(X + (size_t)c * S + sub * CPL);
 #pragma unroll

@@ -174,27 +174,45 @@ __global__ __launch_bounds__(64 * CGF_WPB) void k_cgf_a(int nwblk, int chunk, co
     block_partial<1, 64 * CGF_WPB>(dv, partA);
 }
 
-// Calls f(e, column) for the stored entries of row `row`, lane l of slice sl (sl the
-// same on every lane of the wave), in the row's order. SELL-64: the row's length from
-// rp, IT int columns or short (column - row). Slice-diagonal (DIA, IT int): the
-// slice's width from so, column = row + the slice's offset of the position (uniform,
-// scalar loads), clamped into [0, n) for the value-0 positions a row does not have.
-template <typename IT, bool DIA, typename F>
+// Calls f(e, column, live) for the stored entries of row `row`, lane l of slice sl
+// (sl the same on every lane of the wave), in the row's order; an entry with live false
+// is to be added as 0. SELL-64: the row's length from rp, IT int columns or short
+// (column - row), every entry live. Slice-diagonal (DIA, IT int): the slice's width w
+// from so, column = row + the slice's offset of the position (uniform, scalar loads),
+// clamped into [0, n) for the value-0 positions a row does not have. With GRP = 8 the
+// positions run in groups of 8 with unconditional loads (so a group's loads are all in
+// flight at once; FP32 rows are latency-bound): positions past w read the last entry
+// of the slice with live false, and the 8 offsets of a group are read as one block (the
+// offset table has 8 entries of padding past its end; an offset read past w only picks
+// a clamped column). GRP = 0: one position at a time, unrolled by 4, all live.
+template <typename IT, bool DIA, int GRP = 0, typename F>
 __device__ __forceinline__ void sell_row_entries(int n, int row, int l, int sl, const int *__restrict__ rp,
                                                  const int *__restrict__ so, const IT *__restrict__ col, F &&f) {
-    if constexpr (DIA) {
+    static_assert(GRP == 0 || GRP == 8, "position groups of 8 (the offset table's padding)");
+    if constexpr (DIA && GRP == 0) {
         const int s0 = so[sl], w = (so[sl + 1] - s0) >> 6;
         const int *o = (const int *)col + (s0 >> 6);
         const int base = s0 + l;
 #pragma unroll 4
-        for (int kk = 0; kk < w; kk++) f(base + 64 * kk, min(max(row + o[kk], 0), n - 1));
+        for (int kk = 0; kk < w; kk++) f(base + 64 * kk, min(max(row + o[kk], 0), n - 1), true);
+    } else if constexpr (DIA) {
+        const int s0 = so[sl], w = (so[sl + 1] - s0) >> 6;
+        const int *o = (const int *)col + (s0 >> 6);
+        const int base = s0 + l;
+        for (int k0 = 0; k0 < w; k0 += 8) {
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int kk = min(k0 + u, w - 1);
+                f(base + 64 * kk, min(max(row + o[k0 + u], 0), n - 1), k0 + u < w);
+            }
+        }
     } else {
         const int len = rp[row + 1] - rp[row];
         const int base = so[sl] + l;
 #pragma unroll 4
         for (int kk = 0; kk < len; kk++) {
             const int e = base + 64 * kk;
-            f(e, sizeof(IT) == 2 ? row + (int)col[e] : (int)col[e]);
+            f(e, sizeof(IT) == 2 ? row + (int)col[e] : (int)col[e], true);
         }
     }
 }
@@ -230,9 +248,10 @@ __global__ __launch_bounds__(BS) void k_cgf_a_sell(int n, int nsl, int chunk, co
         const int row = sl * 64 + l;
         if (row >= n) continue;
         T acc = (T)0;
-        sell_row_entries<IT, DIA>(n, row, l, sl, rp, so, col, [&](int e, int cc) {
+        sell_row_entries<IT, DIA>(n, row, l, sl, rp, so, col, [&](int e, int cc, bool live) {
             const T pc = add_rn(mul_rn(be, pold[cc]), r[cc]);
-            acc = add_rn(acc, mul_rn((T)val[e], pc));
+            const T v = (T)val[e];
+            acc = add_rn(acc, mul_rn(live ? v : (T)0, pc));
         });
         q[row] = acc;
         const T pn = add_rn(mul_rn(be, pold[row]), r[row]);
@@ -393,12 +412,14 @@ __global__ __launch_bounds__(BS) void k_cg1_sell(int n, int nsl, int chunk, cons
         if (row >= n) continue;
         float acc = 0.0f;
         if (start)
-            sell_row_entries<IT, DIA>(n, row, l, sl, rp, so, col, [&](int e, int cc) {
-                acc = add_rn(acc, mul_rn((float)val[e], fmaf(be, pp_[cc], rp_[cc])));
+            sell_row_entries<IT, DIA, 8>(n, row, l, sl, rp, so, col, [&](int e, int cc, bool live) {
+                const float v = (float)val[e];
+                acc = add_rn(acc, mul_rn(live ? v : 0.0f, fmaf(be, pp_[cc], rp_[cc])));
             });
         else
-            sell_row_entries<IT, DIA>(n, row, l, sl, rp, so, col, [&](int e, int cc) {
-                acc = add_rn(acc, mul_rn((float)val[e], fmaf(-al, qp_[cc], rp_[cc])));
+            sell_row_entries<IT, DIA, 8>(n, row, l, sl, rp, so, col, [&](int e, int cc, bool live) {
+                const float v = (float)val[e];
+                acc = add_rn(acc, mul_rn(live ? v : 0.0f, fmaf(-al, qp_[cc], rp_[cc])));
             });
         const float pold = pp_[row], qold = qp_[row];
         const float rr = fmaf(-al, qold, rp_[row]);

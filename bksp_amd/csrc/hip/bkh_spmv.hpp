@@ -211,8 +211,11 @@ __global__ __launch_bounds__(256) void k_spmv_wave(SpmvArgs<VT, YT> a, int nwblk
 // a wave loads 64 consecutive values and columns (entry k of each of its rows) and
 // gathers x at those columns; each lane adds its row's products in stored order, so
 // the result is the sequential row sum of src/matvec.c:22-25 bit for bit. IT is int
-// (columns) or short (column - row).
-template <typename VT, typename YT, typename IT, int NV, class Fin>
+// (columns) or short (column - row). DIA: the slice-diagonal copy (so, val, col = its
+// slice offsets, values and per-slice column offsets, uniform over the wave's slice);
+// its value-0 positions add 0 products at clamped columns, which leaves every sum as on
+// the SELL copy for finite x (the solvers' own products).
+template <typename VT, typename YT, typename IT, int NV, class Fin, bool DIA = false>
 __global__ __launch_bounds__(256) void k_spmv_sell(const int *__restrict__ rp, const int *__restrict__ so,
                                                    const VT *__restrict__ val, const IT *__restrict__ col, int n,
                                                    const YT *x, YT *y, double alpha, double beta, int alpha_one,
@@ -223,15 +226,26 @@ __global__ __launch_bounds__(256) void k_spmv_sell(const int *__restrict__ rp, c
 #pragma unroll
     for (int i = 0; i < NV; i++) dv[i] = 0.0;
     if (row < n) {
-        const int sl = row >> 6;
-        const int len = rp[row + 1] - rp[row];
-        const int base = so[sl] + (row & 63);
+        int len, base;
+        const int *dof = nullptr;
+        if constexpr (DIA) {
+            const int sl = __builtin_amdgcn_readfirstlane(row >> 6);
+            const int s0 = so[sl];
+            len = (so[sl + 1] - s0) >> 6;
+            base = s0 + (row & 63);
+            dof = (const int *)col + (s0 >> 6);
+        } else {
+            len = rp[row + 1] - rp[row];
+            base = so[row >> 6] + (row & 63);
+        }
         const YT al = (YT)alpha;
         YT s = has_beta ? mul_rn((YT)beta, y[row]) : (YT)0;
 #pragma unroll 4
         for (int k = 0; k < len; k++) {
             const int ei = base + 64 * k;
-            const int c = sizeof(IT) == 2 ? row + (int)col[ei] : (int)col[ei];
+            int c;
+            if constexpr (DIA) c = min(max(row + dof[k], 0), n - 1);
+            else c = sizeof(IT) == 2 ? row + (int)col[ei] : (int)col[ei];
             const YT v = (YT)val[ei];
             s = add_rn(s, alpha_one ? mul_rn(v, x[c]) : mul_rn(mul_rn(al, v), x[c]));
         }
@@ -250,15 +264,28 @@ static inline int spmv_grid(const bkh_matrix *A) {
 }
 
 // Launches the SpMV kernel only (partials written, no finalize); returns the
-// number of workgroups, i.e. of partial rows.
+// number of workgroups, i.e. of partial rows. dia: the solvers' own products (finite
+// x) run on the slice-diagonal copy when the matrix has one and the default SELL-64
+// kernel is selected; the public SpMV entry points pass false.
 template <typename VT, typename YT, int NV, class Fin>
 static inline int launch_spmv_kernel(hipStream_t s, const bkh_matrix *A, const VT *val, const YT *x,
                                      YT *y, double alpha, double beta, const Epi<NV, YT> &e, Ctl *ctl,
-                                     double *part, Fin fin) {
+                                     double *part, Fin fin, bool dia = true) {
     SpmvArgs<VT, YT> a;
     a.rp = A->rp; a.ci = A->ci; a.blk = A->blk; a.val = val; a.x = x; a.y = y;
     a.alpha = alpha; a.beta = beta; a.alpha_one = (alpha == 1.0); a.has_beta = (beta != 0.0);
     const int G = spmv_grid(A);
+    if (dia && A->dia_entries > 0 && A->nsl > 0 && A->variant == SPMV_SELL) {
+        const VT *dvv;
+        if constexpr (sizeof(VT) == sizeof(float)) dvv = (const VT *)A->dval32;
+        else dvv = (const VT *)A->dval;
+        if (dvv) {
+            hipLaunchKernelGGL((k_spmv_sell<VT, YT, int, NV, Fin, true>), dim3(G), dim3(256), 0, s, A->rp, A->dso,
+                               dvv, (const int *)A->doff, A->n, x, y, alpha, beta, (int)(alpha == 1.0),
+                               (int)(beta != 0.0), e, ctl, part, fin);
+            return G;
+        }
+    }
     if (A->variant == SPMV_SELL || A->variant == SPMV_SELL32) {
         const VT *sv;
         if constexpr (sizeof(VT) == sizeof(float)) sv = (const VT *)A->sval32;
@@ -289,8 +316,8 @@ static inline int launch_spmv_kernel(hipStream_t s, const bkh_matrix *A, const V
 template <typename VT, typename YT, int NV, class Fin>
 static inline void launch_spmv(hipStream_t s, const bkh_matrix *A, const VT *val, const YT *x,
                                YT *y, double alpha, double beta, const Epi<NV, YT> &e, Ctl *ctl,
-                               double *part, Fin fin) {
-    const int G = launch_spmv_kernel<VT, YT, NV>(s, A, val, x, y, alpha, beta, e, ctl, part, fin);
+                               double *part, Fin fin, bool dia = true) {
+    const int G = launch_spmv_kernel<VT, YT, NV>(s, A, val, x, y, alpha, beta, e, ctl, part, fin, dia);
     if constexpr (NV > 0) launch_finalize<NV>(s, part, G, ctl, fin);
 }
 

@@ -803,10 +803,25 @@ static void panel_vp(hipStream_t s, long long m, T *V, T *P, const BlkMats *mt, 
 }
 
 // Y = alpha A X + beta Y: the SELL-64 kernel when the matrix has that copy in the
-// values' precision, else CSR-stream.
+// values' precision, else CSR-stream. dia: the solvers' own products (finite X) may
+// run on the slice-diagonal copy.
 template <int S, typename VT, typename T>
 static void launch_spmm(hipStream_t s, const bkh_matrix *A, const VT *val, const T *X, T *Y,
-                        double alpha, double beta, const Ctl *c) {
+                        double alpha, double beta, const Ctl *c, bool dia = false) {
+    constexpr int LPRD = spmm_lpr<S, T>();
+    if constexpr (LPRD > 0) {
+        const VT *dv = nullptr;
+        if (dia && A->dia_entries > 0 && A->nsl > 0 && A->variant == SPMV_SELL) {
+            if constexpr (sizeof(VT) == sizeof(float)) dv = (val == (const VT *)A->val32) ? (const VT *)A->dval32 : nullptr;
+            else dv = (val == (const VT *)A->val) ? (const VT *)A->dval : nullptr;
+        }
+        if (dv && (((uintptr_t)X | (uintptr_t)Y) & 15) == 0) {
+            const int G = (int)((A->n + 256 / LPRD - 1) / (256 / LPRD));
+            hipLaunchKernelGGL((k_spmm_sell_v<S, VT, T, int, true>), dim3(G), dim3(256), 0, s, A->n, A->rp, A->dso,
+                               dv, (const int *)A->doff, X, Y, alpha, beta, c);
+            return;
+        }
+    }
     const VT *sv = nullptr;
     if (A->nsl > 0) {
         if constexpr (sizeof(VT) == sizeof(float)) sv = (val == (const VT *)A->val32) ? (const VT *)A->sval32 : nullptr;
@@ -854,7 +869,7 @@ static int blcg_core(ThreadCtx *t, const bkh_matrix *A, const VT *val, const T *
     h.s = S;
     HIP_OK(hipMemcpyAsync(mt, &h, sizeof h, hipMemcpyHostToDevice, s));
     // R0 = B - A X0 (blcg.c:54-57), V, C = qr(R0) (:58), fnb = ||B||_F (:60)
-    launch_spmm<S>(s, A, val, (const T *)Xt, AP, 1.0, 0.0, c);
+    launch_spmm<S>(s, A, val, (const T *)Xt, AP, 1.0, 0.0, c, true);
     hipLaunchKernelGGL(k_blk_resid<T>, dim3(Gv), dim3(256), 0, s, m * S, Bt, (const T *)AP, V, part);
     launch_blk_fin<S, FIN_FNB>(s, part, Gv, mt, c);
     launch_blk_fin<S, FIN_QR1>(s, part, panel_gram<S, T>(s, m, V, V, c, part), mt, c);
@@ -865,7 +880,7 @@ static int blcg_core(ThreadCtx *t, const bkh_matrix *A, const VT *val, const T *
     hipGraphExec_t ge = capture_cached(t, {0xB1C6, (uintptr_t)A, (uintptr_t)val, (uintptr_t)Bt, (uintptr_t)Xt, (uintptr_t)V,
                                            (uintptr_t)P, (uintptr_t)AP, (uintptr_t)mt, (uintptr_t)c, (uintptr_t)part,
                                            (uintptr_t)S, (uintptr_t)sizeof(T)}, s, K, [&]() {
-        launch_spmm<S>(s, A, val, (const T *)P, AP, 1.0, 0.0, c);
+        launch_spmm<S>(s, A, val, (const T *)P, AP, 1.0, 0.0, c, true);
         launch_blk_fin<S, FIN_ALPHA>(s, part, panel_gram<S, T>(s, m, P, AP, c, part), mt, c);
         launch_blk_fin<S, FIN_QR1>(s, part, panel_xv<S, T>(s, m, Xt, P, V, AP, mt, c, part), mt, c);
         launch_blk_fin<S, FIN_QR2_ITER>(s, part, panel_vr<S, T>(s, m, V, mt, c, part), mt, c);
@@ -884,7 +899,7 @@ static int block_true_res(ThreadCtx *t, const bkh_matrix *A, const double *Bt, c
     const long long m = A->n;
     const int Gv = vec_grid(m * S, 256);
     if (ctl_init(t, c, 0.0, 1)) return BKH_DEVICE_ERROR;
-    launch_spmm<S>(s, A, A->val, Xt, tmp, 1.0, 0.0, (const Ctl *)nullptr);
+    launch_spmm<S>(s, A, A->val, Xt, tmp, 1.0, 0.0, (const Ctl *)nullptr, true);
     hipLaunchKernelGGL(k_blk_resid<double>, dim3(Gv), dim3(256), 0, s, m * S, Bt, (const double *)tmp, tmp, part);
     // ||R||^2 from a second pass (tmp now holds R)
     hipLaunchKernelGGL(k_blk_resid<double>, dim3(Gv), dim3(256), 0, s, m * S, (const double *)tmp,
@@ -962,7 +977,7 @@ static int blcg_ir(ThreadCtx *t, bkh_matrix *A, const double *B, double *X, doub
     double rel = 0.0;
     for (;;) {
         if (ctl_init(t, co, 0.0, 1)) return BKH_DEVICE_ERROR;
-        launch_spmm<S>(s, A, A->val, (const double *)Xt, R, 1.0, 0.0, (const Ctl *)nullptr);
+        launch_spmm<S>(s, A, A->val, (const double *)Xt, R, 1.0, 0.0, (const Ctl *)nullptr, true);
         hipLaunchKernelGGL(k_blk_resid32, dim3(Gv), dim3(256), 0, s, ms, (const double *)Bt, (const double *)R, R,
                            R32, part);
         launch_finalize<2>(s, part, Gv, co, BlkResFin());
@@ -1061,7 +1076,7 @@ static int blcg_ru(ThreadCtx *t, bkh_matrix *A, const double *B, double *X, doub
     hipGraphExec_t ge = capture_cached(t, {0xB1C7, (uintptr_t)A, (uintptr_t)X32, (uintptr_t)V32, (uintptr_t)P32,
                                            (uintptr_t)AP32, (uintptr_t)mt, (uintptr_t)c, (uintptr_t)part,
                                            (uintptr_t)S}, s, K, [&]() {
-        launch_spmm<S>(s, A, val, (const float *)P32, AP32, 1.0, 0.0, c);
+        launch_spmm<S>(s, A, val, (const float *)P32, AP32, 1.0, 0.0, c, true);
         launch_blk_fin<S, FIN_ALPHA>(s, part, panel_gram<S, float>(s, m, P32, AP32, c, part), mt, c);
         launch_blk_fin<S, FIN_QR1>(s, part, panel_xv<S, float>(s, m, X32, P32, V32, AP32, mt, c, part), mt, c);
         launch_blk_fin<S, FIN_QR2_ITER>(s, part, panel_vr<S, float>(s, m, V32, mt, c, part), mt, c);
@@ -1074,7 +1089,7 @@ static int blcg_ru(ThreadCtx *t, bkh_matrix *A, const double *B, double *X, doub
         // X += X32 (FP64); R = B - A X in FP64, its FP32 copy straight into V32
         if (passes > 0) hipLaunchKernelGGL(k_blk_fold, dim3(Gv), dim3(256), 0, s, ms, Xt, X32);
         if (ctl_init(t, co, 0.0, 1)) return BKH_DEVICE_ERROR;
-        launch_spmm<S>(s, A, A->val, (const double *)Xt, R, 1.0, 0.0, (const Ctl *)nullptr);
+        launch_spmm<S>(s, A, A->val, (const double *)Xt, R, 1.0, 0.0, (const Ctl *)nullptr, true);
         hipLaunchKernelGGL(k_blk_resid32, dim3(Gv), dim3(256), 0, s, ms, (const double *)Bt, (const double *)R, R,
                            V32, part);
         launch_finalize<2>(s, part, Gv, co, BlkResFin());
@@ -1270,7 +1285,7 @@ extern "C" int bkh_spmm(const bkh_matrix *A, int s, double alpha, const double *
         memset(&e, 0, sizeof e);
         for (int j = 0; j < s; j++)
             launch_spmv<double, double, 0>(st, A, A->val, X + (size_t)j * m, Y + (size_t)j * m, alpha, beta, e,
-                                           nullptr, nullptr, NoFin());
+                                           nullptr, nullptr, NoFin(), false);
         LAUNCH_OK();
         HIP_OK(hipStreamSynchronize(st));
         return 0;

@@ -475,10 +475,10 @@ static bool upload_dia(bkh_matrix *A, int n, const std::vector<int> &rp, const s
     const size_t ne = (size_t)tot + 4;
     bool ok = hipMalloc(&A->dso, sizeof(int) * (nsl + 1)) == hipSuccess &&
               hipMalloc(&A->dval, sizeof(double) * ne) == hipSuccess &&
-              hipMalloc(&A->doff, sizeof(int) * (offs.size() + 1)) == hipSuccess;
+              hipMalloc(&A->doff, sizeof(int) * (offs.size() + 8)) == hipSuccess;
     if (ok && want_fp32) ok = hipMalloc(&A->dval32, sizeof(float) * ne) == hipSuccess;
     if (!ok) return false;
-    offs.push_back(0);
+    offs.resize(offs.size() + 8, 0);                              // read-ahead padding of the kernels
     ok = hipMemcpyAsync(A->dso, so.data(), sizeof(int) * (nsl + 1), hipMemcpyHostToDevice, s) == hipSuccess &&
          hipMemcpyAsync(A->dval, dv.data(), sizeof(double) * ne, hipMemcpyHostToDevice, s) == hipSuccess &&
          hipMemcpyAsync(A->doff, offs.data(), sizeof(int) * offs.size(), hipMemcpyHostToDevice, s) == hipSuccess;
@@ -647,7 +647,7 @@ extern "C" int bkh_spmv(const bkh_matrix *A, char op, double alpha, const double
     }
     Epi<0> e;
     memset(&e, 0, sizeof e);
-    launch_spmv<double, double, 0>(t->stream, M, M->val, x, y, alpha, beta, e, nullptr, nullptr, NoFin());
+    launch_spmv<double, double, 0>(t->stream, M, M->val, x, y, alpha, beta, e, nullptr, nullptr, NoFin(), false);
     LAUNCH_OK();
     HIP_OK(hipStreamSynchronize(t->stream));
     return 0;
@@ -662,7 +662,7 @@ extern "C" int bkh_spmv32(const bkh_matrix *A, const float *x, float *y) {
     }
     Epi<0, float> e;
     memset(&e, 0, sizeof e);
-    launch_spmv<float, float, 0>(t->stream, A, A->val32, x, y, 1.0, 0.0, e, nullptr, nullptr, NoFin());
+    launch_spmv<float, float, 0>(t->stream, A, A->val32, x, y, 1.0, 0.0, e, nullptr, nullptr, NoFin(), false);
     LAUNCH_OK();
     HIP_OK(hipStreamSynchronize(t->stream));
     return 0;
@@ -811,12 +811,12 @@ extern "C" int bkh_bench_spmv(const bkh_matrix *A, int fp32, const void *x, void
             Epi<0, float> e;
             memset(&e, 0, sizeof e);
             launch_spmv<float, float, 0>(t->stream, A, A->val32, (const float *)x, (float *)y, 1.0, 0.0,
-                                         e, nullptr, nullptr, NoFin());
+                                         e, nullptr, nullptr, NoFin(), false);
         } else {
             Epi<0> e;
             memset(&e, 0, sizeof e);
             launch_spmv<double, double, 0>(t->stream, A, A->val, (const double *)x, (double *)y, 1.0,
-                                           0.0, e, nullptr, nullptr, NoFin());
+                                           0.0, e, nullptr, nullptr, NoFin(), false);
         }
     }
     LAUNCH_OK();

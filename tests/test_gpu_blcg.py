@@ -155,3 +155,30 @@ def test_dropin_fp32ir_environment(monkeypatch):
     x = np.zeros(n)
     assert bksp.bksp_dcg(A, B[:, 0].copy(), 1e-9, 2000, x) >= 0
     assert np.linalg.norm(B[:, 0] - Af @ x) / np.linalg.norm(B[:, 0]) < 1e-9
+
+
+@pytest.mark.parametrize("m,s", [(15, 8), (24, 4), (20, 3)])
+def test_blcg_dia_layout_bitwise_equal_to_sell(m, s):
+    """The block SpMM on the slice-diagonal copy (s = 4, 8: 16-byte row segments per
+    lane) gives the solutions of the SELL-64 SpMM bit for bit, FP64 and FP32-IR; s = 3
+    has no such split and runs the SELL kernels on both matrices."""
+    import os
+    n, rp, ci, v = gen.poisson3d(m, lower=True)
+    B = gen.normal(77, n * s).reshape(s, n).T.copy()
+    A = dv.Matrix(n, rp, ci, v, symmetric=True, fp32=True)
+    os.environ["BKH_DIA"] = "0"
+    try:
+        C = dv.Matrix(n, rp, ci, v, symmetric=True, fp32=True)
+    finally:
+        del os.environ["BKH_DIA"]
+    assert A.dia_entries > 0 and C.dia_entries == 0
+    dB = dv.DeviceArray.from_numpy(np.ascontiguousarray(B.T))
+    for prec in (0, dv.BKH_FP32_IR):
+        out = []
+        for M in (A, C):
+            dX = dv.DeviceArray.from_numpy(np.zeros((s, n)))
+            info = dv.blcg(M, s, dB, dX, 1e-9, 3000, precision=prec)
+            assert info.ret >= 0 and info.true_res < 1e-9
+            out.append((info.ret, info.inner_iters, dX.numpy()))
+        assert out[0][:2] == out[1][:2]
+        assert np.array_equal(out[0][2], out[1][2])
