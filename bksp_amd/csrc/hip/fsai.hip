@@ -897,7 +897,10 @@ __device__ __forceinline__ bool pf_beta(Ctl *c, int k, const double *partPrev, i
     return true;
 }
 
-template <typename IT, typename ZT>
+// PA(k): q = A p with p = z + beta p formed in the gather, p.q partials. DIA: on the
+// slice-diagonal copy of A (so, val, col = its slice offsets, values and per-slice
+// column offsets); its value-0 positions add fma(0, p, acc) = acc.
+template <typename IT, typename ZT, bool DIA = false>
 __global__ __launch_bounds__(PF_BS) void k_pf_a(int n, int nsl, int chunk, const int *__restrict__ rp,
                                               const int *__restrict__ so, const double *__restrict__ val,
                                               const IT *__restrict__ col, const ZT *z, double *P0, double *P1,
@@ -912,11 +915,25 @@ __global__ __launch_bounds__(PF_BS) void k_pf_a(int n, int nsl, int chunk, const
     double *pnew = (k & 1) ? P0 : P1;
     double dv[1] = {0.0};
     PF_SLICES(sl, row) {
-        const int len = rp[row + 1] - rp[row], base = so[sl] + (row & 63);
+        int len, base;
+        const int *dof = nullptr;
+        if constexpr (DIA) {
+            const int su = __builtin_amdgcn_readfirstlane(sl);
+            const int s0 = so[su];
+            len = (so[su + 1] - s0) >> 6;
+            base = s0 + (row & 63);
+            dof = (const int *)col + (s0 >> 6);
+        } else {
+            len = rp[row + 1] - rp[row];
+            base = so[sl] + (row & 63);
+        }
         double acc = 0.0;
 #pragma unroll 4
         for (int kk = 0; kk < len; kk++) {
-            const int e = base + 64 * kk, cc = sell_col(col, e, row);
+            const int e = base + 64 * kk;
+            int cc;
+            if constexpr (DIA) cc = min(max(row + dof[kk], 0), n - 1);
+            else cc = sell_col(col, e, row);
             acc = fma(val[e], fma(be, pold[cc], (double)z[cc]), acc);
         }
         q[row] = acc;
@@ -1045,6 +1062,12 @@ static bool pf_ok(const bkh_matrix *M) {
     return M && M->nsl > 0 && (M->variant == SPMV_SELL || M->variant == SPMV_SELL32 || M->sval32);
 }
 
+// The system matrix's products run on its slice-diagonal copy when it has one and the
+// default SELL-64 kernel is selected.
+static bool pf_dia(const bkh_matrix *A) {
+    return A->dia_entries > 0 && A->dval && A->variant == SPMV_SELL;
+}
+
 template <typename IT>
 static const IT *sell_cols(const bkh_matrix *M) {
     if constexpr (sizeof(IT) == 2) return (const IT *)M->scd;
@@ -1059,9 +1082,14 @@ static void pf_capture_body(hipStream_t s, const bkh_matrix *A, const bkh_matrix
     for (int j = 0; j < K; j++) {
         if (kind == PCK_FSAI) {
             // z and y in FP32 (their buffers hold n doubles each; the first 4n bytes are used)
-            hipLaunchKernelGGL((k_pf_a<ITA, float>), dim3(GA), dim3(PF_BS), 0, s, n, nsl, chunk, A->rp, A->sso,
-                               (const double *)A->sval, sell_cols<ITA>(A), (const float *)z, P, P + n, q, c, j,
-                               (const double *)partB, GA, partA);
+            if (pf_dia(A))
+                hipLaunchKernelGGL((k_pf_a<int, float, true>), dim3(GA), dim3(PF_BS), 0, s, n, nsl, chunk, A->rp,
+                                   A->dso, (const double *)A->dval, (const int *)A->doff, (const float *)z, P, P + n,
+                                   q, c, j, (const double *)partB, GA, partA);
+            else
+                hipLaunchKernelGGL((k_pf_a<ITA, float>), dim3(GA), dim3(PF_BS), 0, s, n, nsl, chunk, A->rp, A->sso,
+                                   (const double *)A->sval, sell_cols<ITA>(A), (const float *)z, P, P + n, q, c, j,
+                                   (const double *)partB, GA, partA);
             hipLaunchKernelGGL((k_pf_b_g<ITG>), dim3(GA), dim3(PF_BS), 0, s, n, nsl, chunk, G->rp, G->sso,
                                (const float *)G->sval32, sell_cols<ITG>(G), x, R, R + n, (const double *)P,
                                (const double *)(P + n), (const double *)q, (float *)y, c, j, (const double *)partA,
@@ -1071,9 +1099,14 @@ static void pf_capture_body(hipStream_t s, const bkh_matrix *A, const bkh_matrix
                                (const float *)Gt->sval32, sell_cols<ITT>(Gt), (const float *)y, (const double *)R,
                                (const double *)(R + n), (float *)z, c, j, partB);
         } else {
-            hipLaunchKernelGGL((k_pf_a<ITA, double>), dim3(GA), dim3(PF_BS), 0, s, n, nsl, chunk, A->rp, A->sso,
-                               (const double *)A->sval, sell_cols<ITA>(A), (const double *)z, P, P + n, q, c, j,
-                               (const double *)partB, GA, partA);
+            if (pf_dia(A))
+                hipLaunchKernelGGL((k_pf_a<int, double, true>), dim3(GA), dim3(PF_BS), 0, s, n, nsl, chunk, A->rp,
+                                   A->dso, (const double *)A->dval, (const int *)A->doff, (const double *)z, P,
+                                   P + n, q, c, j, (const double *)partB, GA, partA);
+            else
+                hipLaunchKernelGGL((k_pf_a<ITA, double>), dim3(GA), dim3(PF_BS), 0, s, n, nsl, chunk, A->rp, A->sso,
+                                   (const double *)A->sval, sell_cols<ITA>(A), (const double *)z, P, P + n, q, c, j,
+                                   (const double *)partB, GA, partA);
             hipLaunchKernelGGL(k_pf_b_vec, dim3(GA), dim3(PF_BS), 0, s, n, nsl, chunk, x, R, R + n, (const double *)P,
                                (const double *)(P + n), (const double *)q, z,
                                kind == PCK_JACOBI ? (const double *)A->diag : (const double *)nullptr, c, j,

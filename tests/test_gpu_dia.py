@@ -108,3 +108,29 @@ def test_public_spmv_keeps_sell_semantics():
     refs = {i for i in range(n) if j in set(ci[rp[i]:rp[i + 1]].tolist())}
     bad = set(np.nonzero(~np.isfinite(y))[0].tolist())
     assert bad == refs
+
+
+@pytest.mark.parametrize("path", [2, 1])
+def test_fsai_pcg_dia_bitwise(path):
+    """Learned-FSAI PCG on a field whose matrix has the slice-diagonal copy against one
+    built without it (the factor G keeps the SELL-64 / CSR kernels either way)."""
+    from bksp_amd import fsai
+    m = 16
+    n, (rpl, cil, vl) = fsai.field_system(29, m)
+    F1 = dv.FsaiField(n, rpl, cil, vl)
+    os.environ["BKH_DIA"] = "0"
+    try:
+        F2 = dv.FsaiField(n, rpl, cil, vl)
+    finally:
+        del os.environ["BKH_DIA"]
+    M = dv.FsaiModel(fsai.init_params(3) * 2)
+    db = dv.DeviceArray.from_numpy(gen.normal(12, n))
+    out = []
+    for F in (F1, F2):
+        F.factor(M)
+        dx = dv.DeviceArray.from_numpy(np.zeros(n))
+        info = F.pcg(db, dx, 1e-9, 100000, path=path)
+        out.append((info.ret, info.true_res, dx.numpy()))
+    assert out[0][:2] == out[1][:2]
+    assert np.array_equal(out[0][2], out[1][2])
+    assert out[0][0] >= 0 and out[0][1] < 1.5e-9
