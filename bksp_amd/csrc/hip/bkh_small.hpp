@@ -4,14 +4,28 @@
 //   dgetrf/dgetri (blcg.c:81-82, blbicgstabl.c:147-148)  -> sm_inverse
 //   dgesv (blbicgstabl.c:105, :124)                       -> sm_inverse + sm_mul
 //   dgeqrf+dorgqr R factor (myqr.c)                       -> sm_chol_upper (CholQR)
-// Every routine ends with __syncthreads() so results are visible to the block.
+// Every routine ends with a barrier so results are visible to the block. W = true:
+// the routine runs on the first wave only (the other waves have left; s <= 8, so every
+// index fits in 64 lanes) and its barriers are wave barriers with LDS fences.
 #pragma once
 #include "bkh_core.hpp"
 
 #define SM_MAX 16
 #define SM_N (SM_MAX * SM_MAX)
 
+template <bool W>
+__device__ __forceinline__ void sm_sync() {
+    if constexpr (W) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 // C = op(A) * op(B) (s x s); ta/tb: 0 = as stored, 1 = transposed. C may not alias A/B.
+template <bool W = false>
 __device__ __forceinline__ void sm_mul(int s, const double *A, int ta, const double *B, int tb,
                                        double *C) {
     const int t = threadIdx.x;
@@ -25,23 +39,26 @@ __device__ __forceinline__ void sm_mul(int s, const double *A, int ta, const dou
         }
         C[i + j * s] = acc;
     }
-    __syncthreads();
+    sm_sync<W>();
 }
 
+template <bool W = false>
 __device__ __forceinline__ void sm_copy(int s, const double *A, double *B) {
     const int t = threadIdx.x;
     if (t < s * s) B[t] = A[t];
-    __syncthreads();
+    sm_sync<W>();
 }
 
+template <bool W = false>
 __device__ __forceinline__ void sm_eye(int s, double *A, double diag) {
     const int t = threadIdx.x;
     if (t < s * s) A[t] = (t % s == t / s) ? diag : 0.0;
-    __syncthreads();
+    sm_sync<W>();
 }
 
 // Upper Cholesky factor R of SPD G (G = R^T R). `shift` is added to the diagonal
 // first. Returns false (uniformly) if a pivot is not positive.
+template <bool W = false>
 __device__ __forceinline__ bool sm_chol_upper(int s, const double *G, double shift, double *R,
                                               int *flag) {
     const int t = threadIdx.x;
@@ -50,31 +67,32 @@ __device__ __forceinline__ bool sm_chol_upper(int s, const double *G, double shi
         const int i = t % s, j = t / s;
         R[t] = (i <= j) ? G[t] + ((i == j) ? shift : 0.0) : 0.0;
     }
-    __syncthreads();
+    sm_sync<W>();
     for (int k = 0; k < s; k++) {
         if (t == 0) {
             double d = R[k + k * s];
             if (!(d > 0.0)) *flag = 0;
             R[k + k * s] = d > 0.0 ? sqrt(d) : 1.0;
         }
-        __syncthreads();
+        sm_sync<W>();
         const double rkk = R[k + k * s];
         if (t > k && t < s) R[k + t * s] /= rkk;                 // row k of R
-        __syncthreads();
+        sm_sync<W>();
         if (t < s * s) {                                          // trailing update
             const int i = t % s, j = t / s;
             if (i > k && j >= i) R[i + j * s] -= R[k + i * s] * R[k + j * s];
         }
-        __syncthreads();
+        sm_sync<W>();
     }
     return *flag != 0;
 }
 
 // Inverse of an upper-triangular R (column j by back substitution, thread j).
+template <bool W = false>
 __device__ __forceinline__ void sm_triu_inv(int s, const double *R, double *Ri) {
     const int t = threadIdx.x;
     if (t < s * s) Ri[t] = 0.0;
-    __syncthreads();
+    sm_sync<W>();
     if (t < s) {
         const int j = t;
         for (int i = j; i >= 0; i--) {
@@ -83,11 +101,12 @@ __device__ __forceinline__ void sm_triu_inv(int s, const double *R, double *Ri) 
             Ri[i + j * s] = acc / R[i + i * s];
         }
     }
-    __syncthreads();
+    sm_sync<W>();
 }
 
 // General inverse by Gauss-Jordan elimination with partial pivoting (works on a
 // copy in `work`, 2*s*s doubles). Returns false (uniformly) if singular.
+template <bool W = false>
 __device__ __forceinline__ bool sm_inverse(int s, const double *A, double *Ai, double *work,
                                            int *piv) {
     const int t = threadIdx.x;
@@ -97,7 +116,7 @@ __device__ __forceinline__ bool sm_inverse(int s, const double *A, double *Ai, d
         M[s * s + t] = (t % s == t / s) ? 1.0 : 0.0;
     }
     if (t == 0) piv[1] = 1;
-    __syncthreads();
+    sm_sync<W>();
     for (int k = 0; k < s; k++) {
         if (t == 0) {
             int p = k;
@@ -107,37 +126,38 @@ __device__ __forceinline__ bool sm_inverse(int s, const double *A, double *Ai, d
             piv[0] = p;
             if (!(best > 0.0)) piv[1] = 0;
         }
-        __syncthreads();
+        sm_sync<W>();
         const int p = piv[0];
         if (p != k && t < 2 * s) {
             double a = M[k + t * s], b = M[p + t * s];
             M[k + t * s] = b; M[p + t * s] = a;
         }
-        __syncthreads();
+        sm_sync<W>();
         const double inv = piv[1] ? 1.0 / M[k + k * s] : 0.0;
         if (t < 2 * s) M[k + t * s] *= inv;
-        __syncthreads();
+        sm_sync<W>();
         // eliminate column k from the other rows: thread (i, c)
-        for (int e = t; e < s * 2 * s; e += blockDim.x) {
+        for (int e = t; e < s * 2 * s; e += W ? 64 : (int)blockDim.x) {
             const int i = e % s, c = e / s;
             if (i != k && c != k) M[i + c * s] -= M[i + k * s] * M[k + c * s];
         }
-        __syncthreads();
+        sm_sync<W>();
         if (t < s && t != k) M[t + k * s] = 0.0;
-        __syncthreads();
+        sm_sync<W>();
     }
     if (t < s * s) Ai[t] = M[s * s + t];
-    __syncthreads();
+    sm_sync<W>();
     return piv[1] != 0;
 }
 
 // Frobenius norm squared (thread 0 result, broadcast through `red`).
+template <bool W = false>
 __device__ __forceinline__ double sm_fro2(int s, const double *A, double *red) {
     if (threadIdx.x == 0) {
         double acc = 0.0;
         for (int k = 0; k < s * s; k++) acc += A[k] * A[k];
         *red = acc;
     }
-    __syncthreads();
+    sm_sync<W>();
     return *red;
 }

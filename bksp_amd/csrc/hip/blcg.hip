@@ -94,48 +94,53 @@ __global__ __launch_bounds__(256) void k_blk_fin(const double *part, int nblocks
         return;
     }
     fin_sum_parts<S * S>(part, nblocks, work, G);
+    // s <= 8: the small dense step runs on the first wave alone, with wave barriers
+    // instead of workgroup barriers (block CG s = 8 at 128^3: 350.3 -> 348.5 ms FP64,
+    // 192.9 -> 190.2 ms FP32, the same iterates)
+    constexpr bool W = S * S <= 64;
+    if (W && t >= 64) return;
     if (OP == FIN_ALPHA) {
         // alpha = (P^T A P)^-1 (dgetrf/dgetri, blcg.c:81-82), W = alpha C (:85-86)
-        if (!sm_inverse(S, G, Ri, work, flag)) { if (t == 0) ctl_breakdown(c); return; }
-        sm_copy(S, Ri, mt->alpha);
-        sm_copy(S, mt->C, T1);
-        sm_mul(S, Ri, 0, T1, 0, T2);
-        sm_copy(S, T2, mt->W);
+        if (!sm_inverse<W>(S, G, Ri, work, flag)) { if (t == 0) ctl_breakdown(c); return; }
+        sm_copy<W>(S, Ri, mt->alpha);
+        sm_copy<W>(S, mt->C, T1);
+        sm_mul<W>(S, Ri, 0, T1, 0, T2);
+        sm_copy<W>(S, T2, mt->W);
         return;
     }
     // CholQR pass: G = R^T R. Pass 1 retries with a diagonal shift when G is
     // numerically singular (shifted CholeskyQR, Fukaya et al. 2020).
-    bool ok = sm_chol_upper(S, G, 0.0, R, flag);
+    bool ok = sm_chol_upper<W>(S, G, 0.0, R, flag);
     if (!ok && OP == FIN_QR1) {
         double tr = 0.0;
         if (t == 0) {
             for (int k = 0; k < S; k++) tr += G[k + k * S];
             red = 11.0 * ((double)mt->m * S + S * (S + 1)) * 1.1102230246251565e-16 * tr;
         }
-        __syncthreads();
-        ok = sm_chol_upper(S, G, red, R, flag);
+        sm_sync<W>();
+        ok = sm_chol_upper<W>(S, G, red, R, flag);
     }
     if (!ok) { if (t == 0) ctl_breakdown(c); return; }
-    sm_triu_inv(S, R, Ri);
+    sm_triu_inv<W>(S, R, Ri);
     if (OP == FIN_QR1) {
-        sm_copy(S, R, mt->R1);
-        sm_copy(S, Ri, mt->R1i);
+        sm_copy<W>(S, R, mt->R1);
+        sm_copy<W>(S, Ri, mt->R1i);
         return;
     }
     // pass 2: full R of the QR = R2 R1
-    sm_copy(S, mt->R1, T1);
-    sm_mul(S, R, 0, T1, 0, T2);                  // T2 = beta = R2 R1
-    sm_copy(S, Ri, mt->R2i);
+    sm_copy<W>(S, mt->R1, T1);
+    sm_mul<W>(S, R, 0, T1, 0, T2);                  // T2 = beta = R2 R1
+    sm_copy<W>(S, Ri, mt->R2i);
     if (OP == FIN_QR2_INIT) {                     // R0 = V C  (blcg.c:58)
-        sm_copy(S, T2, mt->C);
+        sm_copy<W>(S, T2, mt->C);
         return;
     }
     // C = beta C; error = ||C||_F / ||B||_F (blcg.c:98-105); next P uses beta^T
-    sm_copy(S, mt->C, T1);
-    sm_mul(S, T2, 0, T1, 0, R);                  // R <- beta C
-    sm_copy(S, R, mt->C);
+    sm_copy<W>(S, mt->C, T1);
+    sm_mul<W>(S, T2, 0, T1, 0, R);                  // R <- beta C
+    sm_copy<W>(S, R, mt->C);
     if (t < S * S) mt->betaT[(t / S) + (t % S) * S] = T2[t];
-    const double f2 = sm_fro2(S, R, &red);
+    const double f2 = sm_fro2<W>(S, R, &red);
     if (t == 0) ctl_check(c, f2);
 }
 
