@@ -316,7 +316,9 @@ __global__ __launch_bounds__(512) void k_cgf_b(long long n, T *x, T *r, const T 
 // p_{k-1} in its gather, so the recursively updated q starts every pass consistent;
 // the periodic FP64 residual replacement of the reliable-update driver bounds the
 // drift of r and rho in between.
-enum { CG1_ALPHA = 12 };              // Ctl::s slot: alpha_{k-1} at a stop
+// Ctl::s slots: alpha_{k-1} at a stop; the alpha of the x update an even kernel
+// deferred to the next (odd) one, 0 when none is pending
+enum { CG1_ALPHA = 12, CG1_ALPHAD = 13 };
 
 // Fixed-order sum of np rows of 3 partials by a workgroup of BS threads,
 // broadcast to every thread.
@@ -386,7 +388,10 @@ __device__ __forceinline__ bool cg1_scalars(Ctl *c, int k, const double *partPre
 
 // Kernel k on the SELL-64 layout or its slice-diagonal copy (DIA; lane per row,
 // workgroup g owns `chunk` slices). R, Q, P hold two buffers each (parity), n entries
-// apart; x32 is updated in place.
+// apart; x32 is updated in place every second kernel: kernel k owes x the update
+// alpha_{k-1} p_{k-1}; an even kernel defers it (alpha in CG1_ALPHAD) and the next odd
+// kernel applies both, reading p_{k-2} from the buffer it is about to overwrite (the
+// same two roundings in the same order, one x read and write per two iterations).
 template <typename VT, typename IT, int BS, bool DIA = false>
 __global__ __launch_bounds__(BS) void k_cg1_sell(int n, int nsl, int chunk, const int *__restrict__ rp,
                                                   const int *__restrict__ so, const VT *__restrict__ val,
@@ -400,6 +405,10 @@ __global__ __launch_bounds__(BS) void k_cg1_sell(int n, int nsl, int chunk, cons
     if (!cg1_scalars<BS>(c, k, partPrev, np, writer, ad, bd)) return;
     const bool start = (k == c->first);            // q = A p explicitly
     const float al = (float)ad, be = (float)bd;
+    const bool odd = (k & 1) != 0;
+    // alpha_{k-2} of the update kernel k-1 deferred (none at a pass's first kernel)
+    const float ald = (odd && !start) ? (float)c->s[CG1_ALPHAD] : 0.0f;
+    if (!odd && writer) c->s[CG1_ALPHAD] = ad;
     const size_t po = (size_t)((k + 1) & 1) * n, pn = (size_t)(k & 1) * n;
     const float *rp_ = R + po, *qp_ = Q + po, *pp_ = P + po;
     float *rn_ = R + pn, *qn_ = Q + pn, *pn_ = P + pn;
@@ -425,7 +434,11 @@ __global__ __launch_bounds__(BS) void k_cg1_sell(int n, int nsl, int chunk, cons
         const float rr = fmaf(-al, qold, rp_[row]);
         const float pnew = fmaf(be, pold, rr);
         const float qnew = start ? acc : fmaf(be, qold, acc);
-        x[row] = fmaf(al, pold, x[row]);
+        if (odd) {
+            float xv = x[row];
+            if (ald != 0.0f) xv = fmaf(ald, pn_[row], xv);   // p_{k-2}, before p_k replaces it
+            x[row] = fmaf(al, pold, xv);
+        }
         rn_[row] = rr;
         pn_[row] = pnew;
         qn_[row] = qnew;

@@ -295,14 +295,19 @@ __global__ void k_ctl_resume_gate(Ctl *c) {
 
 // ---- single-kernel FP32 inner iteration (SELL matrices)
 
-// x += x32 + alpha_{k-1} p_{k-1} (the update the stopped kernel k had deferred), x32 = 0.
+// x += x32 + the updates the stopped kernel k owed x: alpha_{k-1} p_{k-1}, and for an
+// odd k also alpha_{k-2} p_{k-2}, which kernel k-1 deferred (p_{k-2} is still in the
+// buffer kernel k would have overwritten); x32 = 0.
 __global__ __launch_bounds__(256) void k_cg1_fold(long long n, double *x, float *x32, const float *P,
                                                   const Ctl *c) {
     const int k = c->it;
-    const double al = (c->status == ST_RUNNING) ? 0.0 : c->s[CG1_ALPHA];
-    const float *p = P + (size_t)((k + 1) & 1) * n;
+    const bool stopped = c->status != ST_RUNNING;
+    const double al = stopped ? c->s[CG1_ALPHA] : 0.0;
+    const double ad = (stopped && (k & 1) && k != c->first) ? c->s[CG1_ALPHAD] : 0.0;
+    const float *p = P + (size_t)((k + 1) & 1) * n, *p2 = P + (size_t)(k & 1) * n;
     ROWLOOP(i, n) {
-        x[i] += (double)x32[i] + al * (double)p[i];
+        const double d = ad != 0.0 ? ad * (double)p2[i] : 0.0;
+        x[i] += (double)x32[i] + d + al * (double)p[i];
         x32[i] = 0.0f;
     }
 }
@@ -335,6 +340,7 @@ struct Cg1ResumeFin {
         c->s[CGF_RR + (k & 1)] = tot[0];
         c->s[CGF_FRESH] = first ? 1.0 : 0.0;
         c->s[CG1_ALPHA] = 0.0;            // nothing deferred until the pass's first kernel
+        c->s[CG1_ALPHAD] = 0.0;
         c->base = k;
         c->first = k;
         c->fnb = co->s[2];
