@@ -483,6 +483,8 @@ extern "C" bkh_fsai_field *bkh_fsai_field_create(int n, const int *rpl, const in
     // CSR-stream kernel (no SELL copy to keep in step)
     F->G->variant = SPMV_WAVE;
     F->G->T->variant = SPMV_WAVE;
+    matrix_drop_dia(F->G);
+    matrix_drop_dia(F->G->T);
     std::vector<int> erow((size_t)nnzl), perm((size_t)nnzl), cnt((size_t)n + 1, 0);
     for (int i = 0; i < n; i++)
         for (int k = rpl[i]; k < rpl[i + 1]; k++) { erow[k] = i; cnt[cil[k] + 1]++; }
