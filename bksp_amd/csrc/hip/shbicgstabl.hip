@@ -652,7 +652,10 @@ struct MsStartOp {      // a pass: r0 = b - (A + sigma I) x, targets, r^ = r0
         m.rho = a[0];
         m.alpha = 0.0; m.omega = 0.0; m.beta = 0.0; m.it = 0;
         m.act = (k < ns && rel == rel && !(rel < tol) && a[0] > 0.0) ? 1 : 0;
-        double itol = 0.25 * tol / rel;
+        // half of tol: the pass's FP64 true residual check starts another pass for a
+        // shift the recurrence misjudged (96^3 convection-diffusion, 64 shifts: 24 inner
+        // iterations, 124 ms, against 29 and 133 ms with a quarter)
+        double itol = 0.5 * tol / rel;
         if (itol < itol_min) itol = itol_min;
         if (itol > 0.5) itol = 0.5;
         m.tgt2 = itol * itol * a[0];
