@@ -38,15 +38,15 @@ __global__ __launch_bounds__(256) void k_sh_fill_f(float *p, long long m) {
 // Partial rows the fin kernel sums itself; more go through k_part_reduce first.
 #define FIN_DIRECT 256
 
-// Fixed-order sum of nblocks rows of E partials by a 256-thread workgroup: thread
-// t < Q*E (Q = 256/E groups) sums rows q, q+Q, q+2Q, ... of entry e = t % E, q = t / E,
+// Fixed-order sum of nblocks rows of E partials by a BS-thread workgroup: thread
+// t < Q*E (Q = BS/E groups) sums rows q, q+Q, q+2Q, ... of entry e = t % E, q = t / E,
 // then the Q group sums are added in order. The rows of a group are loaded RMAX at
 // a time into registers before they are added, so their loads (from another XCD's
 // writes, i.e. from memory) are in flight together instead of in dependent batches.
-// red: Q*E <= 256 doubles of LDS; out[e], e < E, valid in every thread on return.
-template <int E>
+// red: Q*E <= BS doubles of LDS; out[e], e < E, valid in every thread on return.
+template <int E, int BS>
 __device__ __forceinline__ void fin_sum_parts(const double *part, int nblocks, double *red, double *out) {
-    constexpr int Q = 256 / E;
+    constexpr int Q = BS / E;
     constexpr int RMAX = (FIN_DIRECT + Q - 1) / Q < 64 ? (FIN_DIRECT + Q - 1) / Q : 64;
     const int t = threadIdx.x;
     if (t < Q * E) {
@@ -76,16 +76,16 @@ __device__ __forceinline__ void fin_sum_parts(const double *part, int nblocks, d
 
 // One workgroup: sums the s x s Gram partials of `nblocks` workgroups in a fixed
 // order, then runs the block solver's small dense step `op`.
-template <int S, int OP>
-__global__ __launch_bounds__(256) void k_blk_fin(const double *part, int nblocks, BlkMats *mt,
-                                                 Ctl *c) {
+template <int S, int OP, int BS>
+__global__ __launch_bounds__(BS) void k_blk_fin(const double *part, int nblocks, BlkMats *mt,
+                                                Ctl *c) {
     if (ctl_stopped(c)) return;
-    __shared__ double G[SM_N], R[SM_N], Ri[SM_N], T1[SM_N], T2[SM_N], work[2 * SM_N];
+    __shared__ double G[SM_N], R[SM_N], Ri[SM_N], T1[SM_N], T2[SM_N], work[BS > 2 * SM_N ? BS : 2 * SM_N];
     __shared__ int flag[2];
     __shared__ double red;
     const int t = threadIdx.x;
     if (OP == FIN_FNB) {
-        fin_sum_parts<1>(part, nblocks, work, G);
+        fin_sum_parts<1, BS>(part, nblocks, work, G);
         if (t == 0) {
             mt->fnb2 = G[0];
             c->fnb = sqrt(G[0]);
@@ -93,7 +93,7 @@ __global__ __launch_bounds__(256) void k_blk_fin(const double *part, int nblocks
         }
         return;
     }
-    fin_sum_parts<S * S>(part, nblocks, work, G);
+    fin_sum_parts<S * S, BS>(part, nblocks, work, G);
     // s <= 8: the small dense step runs on the first wave alone, with wave barriers
     // instead of workgroup barriers (block CG s = 8 at 128^3: 350.3 -> 348.5 ms FP64,
     // 192.9 -> 190.2 ms FP32, the same iterates)
@@ -147,14 +147,18 @@ __global__ __launch_bounds__(256) void k_blk_fin(const double *part, int nblocks
 template <int S, int OP>
 static void launch_blk_fin(hipStream_t s, const double *part, int nblocks, BlkMats *mt, Ctl *c) {
     const int E = (OP == FIN_FNB) ? 1 : S * S;
+    // s <= 8: 1024 threads, each summing a quarter as many partial rows as with 256
+    // (block CG s = 8 at 128^3: 349.7 -> 346.6 ms FP64, 193.3 -> 190.1 ms FP32); larger s
+    // keep 256 (their 64 partial loads per thread would not fit 128 registers)
+    constexpr int FBS = S <= 8 ? 1024 : 256;
     if (nblocks <= FIN_DIRECT) {
-        hipLaunchKernelGGL((k_blk_fin<S, OP>), dim3(1), dim3(256), 0, s, part, nblocks, mt, c);
+        hipLaunchKernelGGL((k_blk_fin<S, OP, FBS>), dim3(1), dim3(FBS), 0, s, part, nblocks, mt, c);
         return;
     }
     double *r2 = bkh_red2();
     const int nb2 = (nblocks + RED_CHUNK - 1) / RED_CHUNK;
     hipLaunchKernelGGL(k_part_reduce, dim3(nb2), dim3(256), 0, s, part, nblocks, E, r2);
-    hipLaunchKernelGGL((k_blk_fin<S, OP>), dim3(1), dim3(256), 0, s, (const double *)r2, nb2, mt, c);
+    hipLaunchKernelGGL((k_blk_fin<S, OP, 256>), dim3(1), dim3(256), 0, s, (const double *)r2, nb2, mt, c);
 }
 
 template <typename T>
