@@ -430,6 +430,7 @@ static int cg_reliable1(ThreadCtx *t, const bkh_matrix *A, const double *b, doub
     hipGraphExec_t ge = cg1_capture(t, s, A, x32, R, Q, P, ci, part2, K);
     if (!ge) { bkh_set_error("hipGraph capture of the FP32 CG iteration failed"); return BKH_DEVICE_ERROR; }
     int rc = 0, updates = 0;
+    int used = 0, inner_status = ST_RUNNING;   // the inner control after the last pass
     double rel = 0.0;
     for (;;) {
         if (updates > 0)
@@ -438,16 +439,17 @@ static int cg_reliable1(ThreadCtx *t, const bkh_matrix *A, const double *b, doub
         if ((rc = ctl_fetch(t, co))) break;
         rel = t->ctl_host->s[0];
         if (rel < tol || !(rel == rel)) break;
-        if ((rc = ctl_fetch(t, ci))) break;
-        const int used = t->ctl_host->it;
-        if (updates > 0 && (t->ctl_host->status == ST_BREAKDOWN || used >= max_iter)) break;
+        if (updates > 0 && (inner_status == ST_BREAKDOWN || used >= max_iter)) break;
         const double target = std::max(0.5 * tol, delta * rel);
         hipLaunchKernelGGL(k_cg1_load, dim3(G), dim3(256), 0, s, n, (const double *)r64, x32, R, Q, P,
                            (int)(updates == 0), (const Ctl *)ci, part);
         hipLaunchKernelGGL(k_ctl_resume_gate, dim3(1), dim3(64), 0, s, ci);
         launch_finalize<1>(s, part, G, ci, Cg1ResumeFin{target, updates == 0, co});
         LAUNCH_OK();
+        // leaves the inner control block in t->ctl_host (fold and resid do not change it)
         if ((rc = run_graph_until_done(t, ge, ci, (max_iter - used) / K + 2))) break;
+        used = t->ctl_host->it;
+        inner_status = t->ctl_host->status;
         updates++;
     }
     if (rc) return rc;

@@ -282,8 +282,7 @@ int run_graph_until_done(ThreadCtx *t, hipGraphExec_t ge, Ctl *ctl_dev, int max_
         HIP_OK(hipEventSynchronize(ev));
         if (t->ctl_host->done) break;
     }
-    HIP_OK(hipStreamSynchronize(t->stream));
-    return ctl_fetch(t, ctl_dev);
+    return ctl_fetch(t, ctl_dev);        // its copy follows the queued replay on the stream
 }
 
 // --------------------------------------------------------------- matrices
