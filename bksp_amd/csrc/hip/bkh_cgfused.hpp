@@ -262,7 +262,7 @@ __global__ __launch_bounds__(BS) void k_cgf_a_sell(int n, int nsl, int chunk, co
 }
 
 // B(k): alpha_k = (r_k.r_k) / (p_k.q); x += alpha p_k, r -= alpha q; r.r partials.
-template <typename T>
+template <typename T, bool VEC = false>
 __global__ __launch_bounds__(512) void k_cgf_b(long long n, T *x, T *r, const T *P0, const T *P1,
                                                const T *q, Ctl *c, int j, const double *partA, int npA,
                                                double *partB) {
@@ -278,11 +278,37 @@ __global__ __launch_bounds__(512) void k_cgf_b(long long n, T *x, T *r, const T 
     const T al = (T)(c->s[CGF_RR + (k & 1)] / pq);
     const T *p = (k & 1) ? P0 : P1;
     double v[1] = {0.0};
-    for (long long i = (long long)blockIdx.x * 512 + threadIdx.x; i < n; i += (long long)gridDim.x * 512) {
-        x[i] = fma(al, p[i], x[i]);
-        const T ri = fma(-al, q[i], r[i]);
-        r[i] = ri;
-        v[0] += (double)ri * (double)ri;
+    const long long tid = (long long)blockIdx.x * 512 + threadIdx.x, stride = (long long)gridDim.x * 512;
+    if constexpr (VEC) {
+        // 16-byte vectors (every array 16-byte aligned), the odd last entry by thread 0
+        constexpr int NV = 16 / (int)sizeof(T);
+        typedef T tv __attribute__((ext_vector_type(NV)));
+        const long long nv = n / NV;
+        for (long long iv = tid; iv < nv; iv += stride) {
+            tv xv = ((const tv *)x)[iv], rv = ((const tv *)r)[iv];
+            const tv pv = ((const tv *)p)[iv], qv = ((const tv *)q)[iv];
+#pragma unroll
+            for (int u = 0; u < NV; u++) {
+                xv[u] = fma(al, pv[u], xv[u]);
+                rv[u] = fma(-al, qv[u], rv[u]);
+                v[0] += (double)rv[u] * (double)rv[u];
+            }
+            ((tv *)x)[iv] = xv;
+            ((tv *)r)[iv] = rv;
+        }
+        for (long long i = nv * NV + tid; i < n; i += stride) {
+            x[i] = fma(al, p[i], x[i]);
+            const T ri = fma(-al, q[i], r[i]);
+            r[i] = ri;
+            v[0] += (double)ri * (double)ri;
+        }
+    } else {
+        for (long long i = tid; i < n; i += stride) {
+            x[i] = fma(al, p[i], x[i]);
+            const T ri = fma(-al, q[i], r[i]);
+            r[i] = ri;
+            v[0] += (double)ri * (double)ri;
+        }
     }
     block_partial<1, 512>(v, partB);
 }

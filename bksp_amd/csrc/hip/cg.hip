@@ -169,6 +169,8 @@ static hipGraphExec_t cgf_capture(ThreadCtx *t, hipStream_t s, const bkh_matrix 
     GraphKey key = {0xC6F0, (uintptr_t)A, (uintptr_t)val, (uintptr_t)cd, (uintptr_t)x, (uintptr_t)r, (uintptr_t)P0,
                     (uintptr_t)P1, (uintptr_t)q, (uintptr_t)c, (uintptr_t)partA, (uintptr_t)partB,
                     (uintptr_t)sizeof(T), (uintptr_t)K, (uintptr_t)(g.BSA * 65536 + g.GA)};
+    // kernel B on 16-byte vectors when every array it streams allows (x is the caller's)
+    const bool vec = (((uintptr_t)x | (uintptr_t)r | (uintptr_t)P0 | (uintptr_t)P1 | (uintptr_t)q) & 15) == 0;
     return capture_cached(t, key, s, 1, [=]() {   // by value: eager mode runs the body after return
         for (int j = 0; j < K; j++) {
             if (g.sell) {
@@ -182,8 +184,12 @@ static hipGraphExec_t cgf_capture(ThreadCtx *t, hipStream_t s, const bkh_matrix 
                 hipLaunchKernelGGL((k_cgf_a<VT, T, false>), dim3(g.GA), dim3(64 * CGF_WPB), 0, s, A->nwblk, g.chunk,
                                    A->rp, A->wblk, A->ci, cd, val, (const T *)r, P0, P1, q, c, j,
                                    (const double *)partB, g.GB, partA);
-            hipLaunchKernelGGL(k_cgf_b<T>, dim3(g.GB), dim3(512), 0, s, n, x, r, (const T *)P0, (const T *)P1,
-                               (const T *)q, c, j, (const double *)partA, g.GA, partB);
+            if (vec)
+                hipLaunchKernelGGL((k_cgf_b<T, true>), dim3(g.GB), dim3(512), 0, s, n, x, r, (const T *)P0,
+                                   (const T *)P1, (const T *)q, c, j, (const double *)partA, g.GA, partB);
+            else
+                hipLaunchKernelGGL((k_cgf_b<T, false>), dim3(g.GB), dim3(512), 0, s, n, x, r, (const T *)P0,
+                                   (const T *)P1, (const T *)q, c, j, (const double *)partA, g.GA, partB);
         }
         hipLaunchKernelGGL(k_cgf_tail, dim3(1), dim3(64), 0, s, c, K, (const double *)partB, g.GB);
     });
