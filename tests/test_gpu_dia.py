@@ -134,3 +134,28 @@ def test_fsai_pcg_dia_bitwise(path):
     assert out[0][:2] == out[1][:2]
     assert np.array_equal(out[0][2], out[1][2])
     assert out[0][0] >= 0 and out[0][1] < 1.5e-9
+
+
+@pytest.mark.parametrize("m", [8, 12])
+def test_small_matrices_dia_bitwise(m):
+    """n = 512, 1728: slices straddling several grid lines, every offset of the rows
+    near both ends clamped. CG (FP64 and FP32-IR) on the copy matches the SELL-64
+    kernels bit for bit."""
+    n, rp, ci, v = gen.poisson3d(m, lower=True)
+    A, B = _pair(n, rp, ci, v, symmetric=True, fp32=True)
+    db = dv.DeviceArray.from_numpy(gen.normal(40 + m, n))
+    for prec in (0, dv.BKH_FP32_IR):
+        ret, _, tr, _ = _same(lambda M, dx: dv.cg(M, db, dx, 1e-10, 500, precision=prec), A, B, n)
+        assert ret >= 0 and tr < 1e-10
+
+
+def test_tiny_matrix_has_no_copy():
+    """n = 8: SELL-64 padding (one 64-row slice) exceeds its bound, so there is neither a
+    SELL nor a slice-diagonal copy; the solve runs on the CSR kernels."""
+    n, rp, ci, v = gen.poisson3d(2, lower=True)
+    M = dv.Matrix(n, rp, ci, v, symmetric=True, fp32=True)
+    assert M.dia_entries == 0
+    db = dv.DeviceArray.from_numpy(gen.normal(41, n))
+    dx = dv.DeviceArray.from_numpy(np.zeros(n))
+    info = dv.cg(M, db, dx, 1e-12, 100)
+    assert info.ret >= 0 and info.true_res < 1e-12
