@@ -45,8 +45,8 @@ typedef struct {
                             workgroup (systems up to 16384 rows / 400000 nonzeros).
                             CG FP32-IR: 0 auto (one kernel per iteration on SELL
                             matrices), 1 two kernels per iteration.
-                            PCG: 0 auto (fused loop for FSAI, separate kernels for
-                            Jacobi / none), 1 separate kernels, 2 fused loop */
+                            PCG: 0 auto (the fused loop where the matrices allow),
+                            1 separate kernels, 2 fused loop */
 } bkh_opts;
 
 typedef struct {
