@@ -43,9 +43,6 @@ static inline void launch_spmv_dot_w(hipStream_t s, const bkh_matrix *A, const V
 // SpMV output. `part` needs max(spmv_grid(A), 4096) * 2 doubles.
 int resid_fp64(ThreadCtx *t, const bkh_matrix *A, double sigma, const double *b, const double *x,
                double *r, Ctl *c, double *part);
-// Frees the slice-diagonal copy of a matrix whose values are rewritten after upload
-// (the copy would go stale); its kernels then run on SELL-64 / CSR.
-void matrix_drop_dia(bkh_matrix *A);
 // Allocating, synchronous FP64 true residual ||b - (A + sigma I) x|| / ||b||.
 int true_residual(ThreadCtx *t, const bkh_matrix *A, double sigma, const double *b,
                   const double *x, double *out);

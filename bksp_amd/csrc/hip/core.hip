@@ -605,14 +605,6 @@ extern "C" void bkh_matrix_destroy(bkh_matrix *A) {
     free_matrix(A);
 }
 
-void matrix_drop_dia(bkh_matrix *A) {
-    if (!A || A->dia_entries == 0) return;
-    hipFree(A->dso); hipFree(A->dval); hipFree(A->dval32); hipFree(A->doff);
-    A->dso = nullptr; A->dval = nullptr; A->dval32 = nullptr; A->doff = nullptr;
-    A->dia_entries = 0;
-    bkh_graph_epoch_bump();
-}
-
 extern "C" long long bkh_matrix_nnz(const bkh_matrix *A) { return A ? A->nnz : -1; }
 extern "C" long long bkh_matrix_dia_entries(const bkh_matrix *A) { return A ? A->dia_entries : -1; }
 extern "C" int bkh_matrix_rows(const bkh_matrix *A) { return A ? A->n : -1; }

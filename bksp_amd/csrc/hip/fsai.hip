@@ -483,8 +483,6 @@ extern "C" bkh_fsai_field *bkh_fsai_field_create(int n, const int *rpl, const in
     // CSR-stream kernel (no SELL copy to keep in step)
     F->G->variant = SPMV_WAVE;
     F->G->T->variant = SPMV_WAVE;
-    matrix_drop_dia(F->G);
-    matrix_drop_dia(F->G->T);
     std::vector<int> erow((size_t)nnzl), perm((size_t)nnzl), cnt((size_t)n + 1, 0);
     for (int i = 0; i < n; i++)
         for (int k = rpl[i]; k < rpl[i + 1]; k++) { erow[k] = i; cnt[cil[k] + 1]++; }
@@ -529,6 +527,30 @@ __global__ __launch_bounds__(256) void k_csr_to_sell(int n, const int *rp, const
     }
 }
 
+// The fused PCG applies G and G^T through their slice-diagonal copies (FP32 values,
+// refreshed after every factor) when both have one.
+static bool pf_gdia(const bkh_matrix *G) {
+    return G && G->dia_entries > 0 && G->dval32 && G->T && G->T->dia_entries > 0 && G->T->dval32;
+}
+
+// CSR-order values -> the slice-diagonal copy: entry k of row r (column c) sits at the
+// position of offset c - r in its slice's ascending offset list (rows ascend, so one
+// forward scan finds every entry's position); positions no entry holds stay 0.
+__global__ __launch_bounds__(256) void k_csr_to_dia(int n, const int *rp, const int *ci, const int *dso,
+                                                    const int *doff, const float *v32, float *dv32) {
+    for (int r = blockIdx.x * 256 + threadIdx.x; r < n; r += gridDim.x * 256) {
+        const int s0 = dso[r >> 6], w = (dso[(r >> 6) + 1] - s0) >> 6;
+        const int *o = doff + (s0 >> 6);
+        int p = 0;
+        for (int k = rp[r]; k < rp[r + 1]; k++) {
+            const int off = ci[k] - r;
+            while (p < w - 1 && o[p] != off) p++;
+            dv32[s0 + 64 * p + (r & 63)] = v32[k];
+            p++;
+        }
+    }
+}
+
 static int fs_factor(ThreadCtx *t, const bkh_fsai_model *M, bkh_fsai_field *F) {
     hipLaunchKernelGGL((k_fsai_mlp<false>), dim3(fs_grid(F->nnzl)), dim3(256), 0, t->stream, F->n, F->nnzl, F->erow,
                        F->cil, F->vl, F->rows, M->params, F->perm, F->G->val, F->G->val32, F->G->T->val,
@@ -538,6 +560,11 @@ static int fs_factor(ThreadCtx *t, const bkh_fsai_model *M, bkh_fsai_field *F) {
         if (Gm->nsl > 0 && Gm->sval32)
             hipLaunchKernelGGL(k_csr_to_sell, dim3(vec_grid(F->n, 256, 1)), dim3(256), 0, t->stream, F->n, Gm->rp,
                                Gm->sso, (const float *)Gm->val32, Gm->sval32);
+    // and through their slice-diagonal copies when both have one (FP32 values)
+    if (pf_gdia(F->G))
+        for (const bkh_matrix *Gm : {(const bkh_matrix *)F->G, (const bkh_matrix *)F->G->T})
+            hipLaunchKernelGGL(k_csr_to_dia, dim3(vec_grid(F->n, 256, 1)), dim3(256), 0, t->stream, F->n, Gm->rp,
+                               Gm->ci, Gm->dso, Gm->doff, (const float *)Gm->val32, Gm->dval32);
     LAUNCH_OK();
     return 0;
 }
@@ -983,7 +1010,7 @@ __global__ __launch_bounds__(PF_BS) void k_pf_b_vec(int n, int nsl, int chunk, d
 // PB(k) for FSAI: x, r_{k+1} own; y = G r_{k+1} with r_{k+1}[j] = r_k[j] - alpha q[j].
 // y is stored in FP32: the factor G itself is held in FP32, so the preconditioner
 // z = G^T G r is an FP32-accurate operator either way (r, x, p, q stay FP64).
-template <typename IT>
+template <typename IT, bool DIA = false>
 __global__ __launch_bounds__(PF_BS) void k_pf_b_g(int n, int nsl, int chunk, const int *__restrict__ rp,
                                                 const int *__restrict__ so, const float *__restrict__ val,
                                                 const IT *__restrict__ col, double *x, double *R0, double *R1,
@@ -996,13 +1023,9 @@ __global__ __launch_bounds__(PF_BS) void k_pf_b_g(int n, int nsl, int chunk, con
     const double *p = (k & 1) ? P0 : P1, *rk = (k & 1) ? R1 : R0;
     double *rn = (k & 1) ? R0 : R1;
     PF_SLICES(sl, row) {
-        const int len = rp[row + 1] - rp[row], base = so[sl] + (row & 63);
         double acc = 0.0;
-#pragma unroll 4
-        for (int kk = 0; kk < len; kk++) {
-            const int e = base + 64 * kk, cc = sell_col(col, e, row);
-            acc = fma((double)val[e], fma(-al, q[cc], rk[cc]), acc);
-        }
+        sell_row_entries<IT, DIA>(n, row, row & 63, __builtin_amdgcn_readfirstlane(sl), rp, so, col,
+                                  [&](int e, int cc, bool) { acc = fma((double)val[e], fma(-al, q[cc], rk[cc]), acc); });
         y[row] = (float)acc;
         x[row] = fma(al, p[row], x[row]);
         rn[row] = fma(-al, q[row], rk[row]);
@@ -1011,7 +1034,7 @@ __global__ __launch_bounds__(PF_BS) void k_pf_b_g(int n, int nsl, int chunk, con
 
 // PD(k): z = G^T y (FP32 storage, as y); partials {r_{k+1}.z, r_{k+1}.r_{k+1}} with the
 // stored z, so beta and the direction update see the same values.
-template <typename IT>
+template <typename IT, bool DIA = false>
 __global__ __launch_bounds__(PF_BS) void k_pf_d(int n, int nsl, int chunk, const int *__restrict__ rp,
                                               const int *__restrict__ so, const float *__restrict__ val,
                                               const IT *__restrict__ col, const float *y, const double *R0,
@@ -1021,13 +1044,9 @@ __global__ __launch_bounds__(PF_BS) void k_pf_d(int n, int nsl, int chunk, const
     const double *rn = (k & 1) ? R0 : R1;
     double dv[2] = {0.0, 0.0};
     PF_SLICES(sl, row) {
-        const int len = rp[row + 1] - rp[row], base = so[sl] + (row & 63);
         double acc = 0.0;
-#pragma unroll 4
-        for (int kk = 0; kk < len; kk++) {
-            const int e = base + 64 * kk, cc = sell_col(col, e, row);
-            acc = fma((double)val[e], (double)y[cc], acc);
-        }
+        sell_row_entries<IT, DIA>(n, row, row & 63, __builtin_amdgcn_readfirstlane(sl), rp, so, col,
+                                  [&](int e, int cc, bool) { acc = fma((double)val[e], (double)y[cc], acc); });
         const float zs = (float)acc;
         z[row] = zs;
         const double r = rn[row];
@@ -1092,14 +1111,24 @@ static void pf_capture_body(hipStream_t s, const bkh_matrix *A, const bkh_matrix
                 hipLaunchKernelGGL((k_pf_a<ITA, float>), dim3(GA), dim3(PF_BS), 0, s, n, nsl, chunk, A->rp, A->sso,
                                    (const double *)A->sval, sell_cols<ITA>(A), (const float *)z, P, P + n, q, c, j,
                                    (const double *)partB, GA, partA);
-            hipLaunchKernelGGL((k_pf_b_g<ITG>), dim3(GA), dim3(PF_BS), 0, s, n, nsl, chunk, G->rp, G->sso,
-                               (const float *)G->sval32, sell_cols<ITG>(G), x, R, R + n, (const double *)P,
-                               (const double *)(P + n), (const double *)q, (float *)y, c, j, (const double *)partA,
-                               GA);
             const bkh_matrix *Gt = G->T;
-            hipLaunchKernelGGL((k_pf_d<ITT>), dim3(GA), dim3(PF_BS), 0, s, n, nsl, chunk, Gt->rp, Gt->sso,
-                               (const float *)Gt->sval32, sell_cols<ITT>(Gt), (const float *)y, (const double *)R,
-                               (const double *)(R + n), (float *)z, c, j, partB);
+            if (pf_gdia(G)) {
+                hipLaunchKernelGGL((k_pf_b_g<int, true>), dim3(GA), dim3(PF_BS), 0, s, n, nsl, chunk, G->rp, G->dso,
+                                   (const float *)G->dval32, (const int *)G->doff, x, R, R + n, (const double *)P,
+                                   (const double *)(P + n), (const double *)q, (float *)y, c, j,
+                                   (const double *)partA, GA);
+                hipLaunchKernelGGL((k_pf_d<int, true>), dim3(GA), dim3(PF_BS), 0, s, n, nsl, chunk, Gt->rp, Gt->dso,
+                                   (const float *)Gt->dval32, (const int *)Gt->doff, (const float *)y,
+                                   (const double *)R, (const double *)(R + n), (float *)z, c, j, partB);
+            } else {
+                hipLaunchKernelGGL((k_pf_b_g<ITG>), dim3(GA), dim3(PF_BS), 0, s, n, nsl, chunk, G->rp, G->sso,
+                                   (const float *)G->sval32, sell_cols<ITG>(G), x, R, R + n, (const double *)P,
+                                   (const double *)(P + n), (const double *)q, (float *)y, c, j,
+                                   (const double *)partA, GA);
+                hipLaunchKernelGGL((k_pf_d<ITT>), dim3(GA), dim3(PF_BS), 0, s, n, nsl, chunk, Gt->rp, Gt->sso,
+                                   (const float *)Gt->sval32, sell_cols<ITT>(Gt), (const float *)y,
+                                   (const double *)R, (const double *)(R + n), (float *)z, c, j, partB);
+            }
         } else {
             if (pf_dia(A))
                 hipLaunchKernelGGL((k_pf_a<int, double, true>), dim3(GA), dim3(PF_BS), 0, s, n, nsl, chunk, A->rp,
