@@ -190,6 +190,7 @@ class ZMatrix:
         self.handle = load().bkh_zmatrix_create(self.n, _ptr(rp), _ptr(ci), _ptr(v))
         if not self.handle:
             raise BkspDeviceError("bkh_zmatrix_create failed: " + last_error())
+        self.dia_entries = load().bkh_zmatrix_dia_entries(self.handle)
 
     def destroy(self):
         if self.handle:

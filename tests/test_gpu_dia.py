@@ -159,3 +159,37 @@ def test_tiny_matrix_has_no_copy():
     dx = dv.DeviceArray.from_numpy(np.zeros(n))
     info = dv.cg(M, db, dx, 1e-12, 100)
     assert info.ret >= 0 and info.true_res < 1e-12
+
+
+def test_complex_solvers_dia_bitwise():
+    """The complex solvers' SpMV on the slice-diagonal copies of A and A^T (a value-0
+    position adds (0 + 0i) x) gives the iterates of the SELL-64 kernels bit for bit:
+    zbicg (A and A^H), zbicgstabl, zshbicgstabl; the public complex SpMV keeps SELL-64."""
+    from conftest import zsystem
+    n, rp, ci, v = zsystem(40, seed=6)
+    A = dv.ZMatrix(n, rp, ci, v)
+    os.environ["BKH_DIA"] = "0"
+    try:
+        B = dv.ZMatrix(n, rp, ci, v)
+    finally:
+        del os.environ["BKH_DIA"]
+    assert A.dia_entries > 0 and B.dia_entries == 0
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    db = dv.DeviceArray.from_numpy(b)
+    sig = np.array([0.1 + 0.2j, 0.5, 1.0 - 0.3j])
+
+    def run(M, which):
+        if which == "shifted":
+            dx = dv.DeviceArray.from_numpy(np.zeros(n * 4, np.complex128))
+            info, tr = dv.zshbicgstabl(M, db, 2, sig, dx, 1e-10, 1000)
+            return info.ret, dx.numpy(), tr
+        dx = dv.DeviceArray.from_numpy(np.zeros(n, np.complex128))
+        info = dv.zbicg(M, db, dx, 1e-10, 2000) if which == "bicg" else dv.zbicgstabl(M, db, 4, dx, 1e-10, 2000)
+        return info.ret, dx.numpy(), np.array([info.true_res])
+
+    for which in ("bicg", "bicgstabl", "shifted"):
+        ra, xa, ta = run(A, which)
+        rb, xb, tb = run(B, which)
+        assert ra == rb and ra >= 0, (which, ra, rb)
+        assert np.array_equal(xa, xb) and np.array_equal(ta, tb), which
