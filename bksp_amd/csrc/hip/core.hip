@@ -262,7 +262,14 @@ static std::pair<std::function<void()>, int> *eager_find(ThreadCtx *t, hipGraphE
 }
 
 int run_graph_until_done(ThreadCtx *t, hipGraphExec_t ge, Ctl *ctl_dev, int max_launches) {
-    if (auto *eb = eager_find(t, ge)) {
+    if (bkh_eager()) {
+        // in eager mode every handle is an iteration body, never a graph: a handle that
+        // is no longer kept must not reach hipGraphLaunch
+        auto *eb = eager_find(t, ge);
+        if (!eb) {
+            bkh_set_error("eager iteration body no longer held (more than 8 live bodies on this thread)");
+            return BKH_DEVICE_ERROR;
+        }
         for (int l = 1; l <= max_launches; l++) {
             for (int k = 0; k < eb->second; k++) eb->first();
             LAUNCH_OK();

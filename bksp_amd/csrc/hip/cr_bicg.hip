@@ -273,3 +273,148 @@ extern "C" int bkh_bicg(bkh_matrix *A, const double *b, double *x, double tol, i
     if (rc) return rc;
     return finish_solve(t, A, 0.0, b, x, &inf, info);
 }
+
+// ------------------------------------------------------------------- BiCGSTAB
+// Replaces src/bicgstab.c:5-72 step for step. Per iteration:
+//   SpMV      ap = A p,  epilogue rs.ap -> alpha = rho / (rs.ap)         (bicgstab.c:34-36)
+//   k_bs_s    s = r - alpha ap                                           (:38-39)
+//   SpMV      as = A s,  epilogue as.s, as.as -> omega                   (:41-43)
+//   k_bs_upd  x += alpha p, x += omega s, r = s - omega as; rs.r, r.r    (:45-53, :60)
+//             -> beta = alpha / omega * rho_new / rho (left to right), error
+//   k_bs_p    p = (p - omega ap) * beta + r                              (:55-58)
+
+enum { BS_RHO = 0, BS_ALPHA = 1, BS_OMEGA = 2, BS_BETA = 3 };
+
+struct BsInitFin {    // rho = rs.r (rs = r), ||b||                         (bicgstab.c:28-29)
+    __device__ void operator()(Ctl *c, const double *tot) const {
+        c->s[BS_RHO] = tot[0];
+        c->fnb = sqrt(tot[1]);
+        if (!(c->fnb > 0.0)) ctl_breakdown(c);
+    }
+};
+
+struct BsAlphaFin {
+    __device__ void operator()(Ctl *c, const double *tot) const {
+        if (bad(tot[0])) { ctl_breakdown(c); return; }
+        c->s[BS_ALPHA] = c->s[BS_RHO] / tot[0];
+    }
+};
+
+struct BsOmegaFin {   // omega = (as.s) / (as.as)
+    __device__ void operator()(Ctl *c, const double *tot) const {
+        if (bad(tot[1])) { ctl_breakdown(c); return; }
+        c->s[BS_OMEGA] = tot[0] / tot[1];
+    }
+};
+
+struct BsBetaFin {    // beta = alpha / omega * rho_new / rho; error = ||r|| / ||b||
+    __device__ void operator()(Ctl *c, const double *tot) const {
+        const double om = c->s[BS_OMEGA], rho = c->s[BS_RHO];
+        if (bad(om) || bad(rho)) {
+            // a zero r ends the solve as converged (lucky breakdown) through ctl_check
+            if (tot[1] == 0.0) ctl_check(c, 0.0);
+            else ctl_breakdown(c);
+            return;
+        }
+        c->s[BS_BETA] = c->s[BS_ALPHA] / om * tot[0] / rho;
+        c->s[BS_RHO] = tot[0];
+        ctl_check(c, tot[1]);
+    }
+};
+
+__global__ __launch_bounds__(256) void k_bs_init(long long n, const double *b, const double *q, double *r,
+                                                 double *rs, double *p, double *part) {
+    double v[2] = {0.0, 0.0};
+    ROWLOOP(i, n) {
+        double ri = b[i] - q[i];
+        r[i] = ri; rs[i] = ri; p[i] = ri;
+        v[0] += ri * ri;
+        v[1] += b[i] * b[i];
+    }
+    block_partial<2, 256>(v, part);
+}
+
+__global__ __launch_bounds__(256) void k_bs_s(long long n, const double *r, const double *ap, double *sv,
+                                              const Ctl *c) {
+    if (ctl_stopped(c)) return;
+    const double al = c->s[BS_ALPHA];
+    ROWLOOP(i, n) sv[i] = fma(-al, ap[i], r[i]);
+}
+
+__global__ __launch_bounds__(256) void k_bs_upd(long long n, double *x, double *r, const double *rs,
+                                                const double *p, const double *sv, const double *as,
+                                                const Ctl *c, double *part) {
+    if (ctl_stopped(c)) return;
+    const double al = c->s[BS_ALPHA], om = c->s[BS_OMEGA];
+    double v[2] = {0.0, 0.0};
+    ROWLOOP(i, n) {
+        const double si = sv[i];
+        x[i] = fma(om, si, fma(al, p[i], x[i]));
+        const double ri = fma(-om, as[i], si);
+        r[i] = ri;
+        v[0] += rs[i] * ri;
+        v[1] += ri * ri;
+    }
+    block_partial<2, 256>(v, part);
+}
+
+__global__ __launch_bounds__(256) void k_bs_p(long long n, double *p, const double *ap, const double *r,
+                                              const Ctl *c) {
+    if (ctl_stopped(c)) return;
+    const double om = c->s[BS_OMEGA], be = c->s[BS_BETA];
+    ROWLOOP(i, n) p[i] = add_rn(mul_rn(fma(-om, ap[i], p[i]), be), r[i]);
+}
+
+extern "C" int bkh_bicgstab(bkh_matrix *A, const double *b, double *x, double tol, int max_iter,
+                            const bkh_opts *opts, bkh_info *info) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (!A || !b || !x) return BKH_BAD_ARGUMENT;
+    bkh_opts o = opts ? *opts : bkh_opts{};
+    bkh_info inf;
+    memset(&inf, 0, sizeof inf);
+    if (max_iter <= 0) {                 // bicgstab.c:9,67-71: error = 0 -> returns 0
+        inf.status = ST_CONVERGED;
+        if (info) *info = inf;
+        return 0;
+    }
+    const long long n = A->n;
+    const int G = vec_grid(n, 256);
+    const int K = o.graph_iters > 0 ? o.graph_iters : default_graph_iters(n);
+    const int nparts = std::max(spmv_grid(A), G) * 2;
+    void *ws = bkh_workspace(carve_size(sizeof(Ctl)) + carve_size(sizeof(double) * nparts) +
+                             6 * carve_size(sizeof(double) * n));
+    if (!ws) return BKH_DEVICE_ERROR;
+    Carve cv(ws);
+    Ctl *c = cv.take<Ctl>(1);
+    double *part = cv.take<double>(nparts);
+    double *r = cv.take<double>(n), *rs = cv.take<double>(n), *sv = cv.take<double>(n),
+           *p = cv.take<double>(n), *ap = cv.take<double>(n), *as = cv.take<double>(n);
+    hipStream_t s = t->stream;
+    HIP_OK(hipEventRecord(t->ev0, s));
+    if (ctl_init(t, c, tol, max_iter)) return BKH_DEVICE_ERROR;
+    Epi<0> e0;
+    memset(&e0, 0, sizeof e0);
+    launch_spmv<double, double, 0>(s, A, A->val, x, ap, 1.0, 0.0, e0, nullptr, nullptr, NoFin());
+    hipLaunchKernelGGL(k_bs_init, dim3(G), dim3(256), 0, s, n, b, ap, r, rs, p, part);
+    launch_finalize<2>(s, part, G, c, BsInitFin());
+    LAUNCH_OK();
+    hipGraphExec_t ge = capture_cached(t, {0xB5AB, (uintptr_t)A, (uintptr_t)b, (uintptr_t)x, (uintptr_t)r,
+                                           (uintptr_t)rs, (uintptr_t)sv, (uintptr_t)p, (uintptr_t)ap, (uintptr_t)as,
+                                           (uintptr_t)c, (uintptr_t)part}, s, K, [&]() {
+        launch_spmv_dot_w<double, double>(s, A, A->val, p, ap, rs, c, part, BsAlphaFin());
+        hipLaunchKernelGGL(k_bs_s, dim3(G), dim3(256), 0, s, n, (const double *)r, (const double *)ap, sv,
+                           (const Ctl *)c);
+        Epi<2> e2 = make_epi<2>({sv}, {{0, EPI_Y}, {EPI_Y, EPI_Y}});
+        launch_spmv<double, double, 2>(s, A, A->val, sv, as, 1.0, 0.0, e2, c, part, BsOmegaFin());
+        hipLaunchKernelGGL(k_bs_upd, dim3(G), dim3(256), 0, s, n, x, r, (const double *)rs, (const double *)p,
+                           (const double *)sv, (const double *)as, (const Ctl *)c, part);
+        launch_finalize<2>(s, part, G, c, BsBetaFin());
+        hipLaunchKernelGGL(k_bs_p, dim3(G), dim3(256), 0, s, n, p, (const double *)ap, (const double *)r,
+                           (const Ctl *)c);
+    });
+    if (!ge) { bkh_set_error("hipGraph capture of the BiCGSTAB iteration failed"); return BKH_DEVICE_ERROR; }
+    int rc = run_graph_until_done(t, ge, c, max_iter / K + 2);
+    if (rc) return rc;
+    return finish_solve(t, A, 0.0, b, x, &inf, info);
+}

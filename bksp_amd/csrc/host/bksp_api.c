@@ -196,25 +196,33 @@ int dblbicgstabl(dspmat *A, dmat *B, int l, double tol, int max_iter, dmat *X) {
     return rc;
 }
 
-/* reference src/bicgstab.c:5-73: BiCGSTAB is BiCGSTAB(1) (the same iterates in
- * exact arithmetic; both return the loop index at convergence). */
+/* reference src/bicgstab.c:5-72, step for step on the device (bkh_bicgstab). */
 int dbicgstab(dspmat *A, double *b, double tol, int max_iter, double *x) {
-    return dbicgstabl(A, b, 1, tol, max_iter, x);
+    if (max_iter <= 0) return 0;                     /* bicgstab.c:9, 67-71 */
+    bkh_opts o = default_opts();
+    o.precision = BKH_FP64;
+    int n = *A->row_size;
+    upload_t u;
+    int rc = upload_system(A, b, n, x, n, 0, 0, &u);
+    if (rc == 0) {
+        rc = bkh_bicgstab(u.M, u.d_b, u.d_x, tol, max_iter, &o, NULL);
+        if (rc > BKH_DEVICE_ERROR && download_x(&u, x)) rc = BKH_DEVICE_ERROR;
+    }
+    release(&u);
+    return rc;
 }
 
-/* reference src/blbicgstab.c:11-124: block BiCGSTAB, here as block BiCGSTAB(1) with
- * residual QR (the same block Krylov iterates in exact arithmetic; no summary). */
+/* reference src/blbicgstab.c:13-123 (block BiCGSTAB with an orthonormalised direction
+ * block), step for step on the device (bkh_blbicgstab); it prints nothing. */
 int dblbicgstab(dspmat *A, dmat *B, double tol, int max_iter, dmat *X) {
     int m = *A->row_size, s = B->col_size;
     if (s < 1 || s > 16) return INVALID_PARAMETERS;
     if (max_iter <= 0) return 0;
-    bkh_opts o = default_opts();
-    o.precision = BKH_FP64;
     upload_t u;
     size_t ms = (size_t)m * s;
     int rc = upload_system(A, B->value, ms, X->value, ms, 0, 0, &u);
     if (rc == 0) {
-        rc = bkh_blbicgstabl(u.M, s, u.d_b, 1, u.d_x, tol, max_iter, &o, NULL);
+        rc = bkh_blbicgstab(u.M, s, u.d_b, u.d_x, tol, max_iter);
         if (rc > BKH_DEVICE_ERROR && download_x(&u, X->value)) rc = BKH_DEVICE_ERROR;
     }
     release(&u);

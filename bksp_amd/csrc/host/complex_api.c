@@ -6,7 +6,7 @@
  *   zbicgstabl     reference src/bicgstabl.c:141-276
  *   zshbicgstabl   reference src/shbicgstabl.c:288-592
  *   zblbicgstabl   reference src/blbicgstabl.c:255-486 (prints its summary, :461-469)
- *   zqreco         reference src/myqr.c:41-76 (CholeskyQR2 on the GPU)
+ *   zqreco         reference src/myqr.c:41-76 (Householder QR on the GPU, zgeqr2 + zung2r)
  *   zmat_create / zmat_free   src/util.c:72-88
  *   zhorner        src/util_shift.c:29-53 (host, scalar)
  *   zcsrgemm0 / zcsrgemm1     src/matvec.c:55-98, 174-217 (on the GPU)
@@ -114,11 +114,14 @@ int zblbicgstabl(zspmat *A, zmat *B, int l, double tol, int max_iter, zmat *X) {
 }
 
 int zqreco(int m, int n, double _Complex *mat, double _Complex *r) {
+    if (m < 0) return -1;                      /* zungqr's info */
+    if (n < 0 || n > m) return -2;
+    if (n == 0) return 0;
     size_t bytes = sizeof(double _Complex) * (size_t)m * n;
     double *d = (double *)bkh_malloc(bytes);
     if (!d) return BKH_DEVICE_ERROR;
     int rc = bkh_h2d(d, mat, bytes);
-    if (!rc) rc = bkh_zqr(m, n, d, (double *)r);
+    if (!rc) rc = bkh_zqr_householder(m, n, d, (double *)r);
     if (!rc) rc = bkh_d2h(mat, d, bytes);
     bkh_free(d);
     return rc;

@@ -93,13 +93,17 @@ double sp_norm_f(dspmat *A) {
     return out;
 }
 
-/* reference src/myqr.c:4-39 on the GPU (CholeskyQR2). R's rows may differ in sign
- * from LAPACK's Householder R; returns -3 for n > 16. */
+/* reference src/myqr.c:4-39 on the GPU: Householder QR in LAPACK's convention
+ * (dgeqr2 + dorg2r, what dgeqrf + dorgqr run below 32 columns), so Q and R carry
+ * LAPACK's signs; returns dorgqr's info (0, or -1/-2 for bad m/n). */
 int myqr(int m, int n, double *mat, double *r) {
+    if (m < 0) return -1;
+    if (n < 0 || n > m) return -2;
+    if (n == 0) return 0;
     double *d = (double *)bkh_malloc(sizeof(double) * (size_t)m * n);
     if (!d) return BKH_DEVICE_ERROR;
     int rc = bkh_h2d(d, mat, sizeof(double) * (size_t)m * n);
-    if (!rc) rc = bkh_qr(m, n, d, r);
+    if (!rc) rc = bkh_qr_householder(m, n, d, r);
     if (!rc) rc = bkh_d2h(mat, d, sizeof(double) * (size_t)m * n);
     bkh_free(d);
     return rc;

@@ -110,6 +110,17 @@ int bkh_spmm(const bkh_matrix *A, int s, double alpha, const double *d_X,
 /* Thin QR of an m x s column-major device panel, s <= 16 (CholeskyQR2): d_V <- Q,
  * R (host, s x s column-major, positive diagonal). Replaces src/myqr.c:4-39. */
 int bkh_qr(int m, int s, double *d_V, double *R);
+/* Thin QR of an m x n column-major device panel (n <= m) with LAPACK's Householder
+ * convention, the unblocked dgeqr2 + dorg2r that dgeqrf + dorgqr run for n < 32:
+ * d_V <- Q, R (host, n x n column-major, upper triangular; R(k,k) = -sign(alpha_k)
+ * times the reflected column norm, so it may be negative, and Q's columns carry the
+ * matching signs). Rank-deficient panels factor as LAPACK factors them. Returns 0,
+ * -1 (m < 0) or -2 (n < 0 or n > m) as dorgqr's info. Replaces the
+ * LAPACKE_dgeqrf/dorgqr pair of src/myqr.c:9,30. */
+int bkh_qr_householder(int m, int n, double *d_V, double *R);
+/* The same for a complex panel (interleaved re, im): zgeqr2 + zung2r, R with a real
+ * diagonal of either sign. Replaces LAPACKE_zgeqrf/zungqr of src/myqr.c:46,67. */
+int bkh_zqr_householder(int m, int n, double *d_V, double *R);
 /* ||A||_F of host CSR arrays, stored-triangle off-diagonals counted twice when
  * symmetric = 1 (replaces src/norm.c:5-47). */
 int bkh_sp_norm_f(int n, const int *row_ptr, const int *col, const double *val,
@@ -129,6 +140,9 @@ int bkh_cr(bkh_matrix *A, const double *d_b, double *d_x, double tol, int max_it
 /* BiCG (replaces src/bicg.c:7-88); A must be created with want_transpose = 1. */
 int bkh_bicg(bkh_matrix *A, const double *d_b, double *d_x, double tol, int max_iter,
              const bkh_opts *opts, bkh_info *info);
+/* BiCGSTAB (replaces src/bicgstab.c:5-72, step for step). */
+int bkh_bicgstab(bkh_matrix *A, const double *d_b, double *d_x, double tol, int max_iter,
+                 const bkh_opts *opts, bkh_info *info);
 /* BiCGSTAB(l), 1 <= l <= 8 (replaces src/bicgstabl.c:13-138). */
 int bkh_bicgstabl(bkh_matrix *A, const double *d_b, int l, double *d_x, double tol,
                   int max_iter, const bkh_opts *opts, bkh_info *info);
@@ -146,6 +160,11 @@ int bkh_blcg(bkh_matrix *A, int s, const double *d_B, double *d_X, double tol, i
 int bkh_blbicgstabl(bkh_matrix *A, int s, const double *d_B, int l, double *d_X, double tol,
                     int max_iter, const bkh_opts *opts, bkh_info *info);
 
+/* Block BiCGSTAB with an orthonormalised direction block (replaces
+ * src/blbicgstab.c:13-123, step for step; host-driven, for link parity). B, X: m x s
+ * column-major device blocks, 1 <= s <= 16. Returns the break index, -1, or -2 when
+ * Rs^T V is singular. */
+int bkh_blbicgstab(bkh_matrix *A, int s, const double *d_B, double *d_X, double tol, int max_iter);
 /* Shifted BiCG (replaces src/shbicg.c:4-130); host-driven, for link parity. A must be
  * created with want_transpose = 1; d_x holds n*(ns+1) entries (seed first). */
 int bkh_shbicg(bkh_matrix *A, const double *d_b, const double *sigma, int ns, double *d_x,

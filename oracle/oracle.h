@@ -81,6 +81,12 @@ int orc_bicgstabl(int n, const int *rp, const int *ci, const double *v,
 int orc_shbicgstabl(int n, const int *rp, const int *ci, const double *v,
                     const double *b, int l, const double *sigma, int nsigma,
                     double tol, int max_iter, double *x);
+/* src/bicgstab.c:5-72 (BiCGSTAB). */
+int orc_bicgstab(int n, const int *rp, const int *ci, const double *v,
+                 const double *b, double tol, int max_iter, double *x);
+/* src/shbicg.c:5-129 (shifted BiCG); x: n*(nsigma+1), seed first. */
+int orc_shbicg(int n, const int *rp, const int *ci, const double *v, const double *b,
+               const double *sigma, int nsigma, double tol, int max_iter, double *x);
 /* p0_mode 0: calloc reading of blcg.c's uninitialised beta (P0 = 2V);
  * p0_mode 1: corrected P0 = V. */
 int orc_blcg(int n, const int *rp, const int *ci, const double *v,
@@ -91,6 +97,9 @@ int orc_blcg(int n, const int *rp, const int *ci, const double *v,
 int orc_blbicgstabl(int n, const int *rp, const int *ci, const double *v,
                     int s, const double *B, int l, double tol, int max_iter,
                     double *X, double *true_res);
+/* src/blbicgstab.c:13-123 (block BiCGSTAB, orthonormalised direction block). */
+int orc_blbicgstab(int n, const int *rp, const int *ci, const double *v,
+                   int s, const double *B, double tol, int max_iter, double *X);
 
 /* ---- learned FSAI preconditioner (orc_fsai.c); no reference counterpart ---- */
 int    orc_fsai_nparam(void);

@@ -4,6 +4,7 @@
  * Block solvers, restated call for call from the reference:
  *   orc_blcg        src/blcg.c:15-119
  *   orc_blbicgstabl src/blbicgstabl.c:16-252
+ *   orc_blbicgstab  src/blbicgstab.c:13-123
  * Dense blocks are m x s column-major (bksp.h:23-27, dmat).
  *
  * Documented readings of reference quirks:
@@ -197,4 +198,56 @@ int orc_blbicgstabl(int m, const int *rp, const int *ci, const double *v,
     free(omega); free(gamma1); free(gamma0); free(xi); free(t); free(beta); free(alpha);
     if (error >= tol) return NOT_CONVERGED_CODE;
     return iter;
+}
+
+/* src/blbicgstab.c:13-123: block BiCGSTAB with an orthonormalised direction block.
+ * R0 = B - A X0 (:49-54); per iteration V = A P, alpha = (Rs^T V)^-1 Rs^T R by dgesv
+ * (whose LU of Rs^T V stays in tmp for beta's dgetrs, :65-69, :92-94), R -= V alpha,
+ * T = A R, omega = <T,R>/<T,T> over all m*n entries, X += P alpha + omega R,
+ * R -= omega T, beta = -(Rs^T V)^-1 Rs^T T, P = (P - omega V) beta + R, then
+ * myqr(P) (:104) keeps only P's orthonormal basis. The stop test is ||R||_F/||B||_F. */
+int orc_blbicgstab(int m, const int *rp, const int *ci, const double *v,
+                   int n, const double *B, double tol, int max_iter, double *X) {
+    int i;
+    double omega, fnb, error = 0;
+    size_t mn = (size_t)m * n;
+    double *base0 = dalloc((size_t)n * n * 2);
+    double *alpha = base0, *beta = base0 + (size_t)n * n;
+    double *base1 = dalloc(mn * 6);
+    double *P = base1, *R = base1 + mn, *Rs = base1 + 2 * mn, *V = base1 + 3 * mn, *T = base1 + 4 * mn;
+    double *tmp = base1 + 5 * mn;
+    int *ipiv = (int *)malloc((size_t)n * n * sizeof(int));
+    int *rp1, *ci1;
+    one_based(m, rp, ci, &rp1, &ci1);
+
+    gemm1(m, n, -1, v, rp1, ci1, X, 0, tmp);                /* blbicgstab.c:49-55 */
+    orc_daxpy((int)mn, 1, B, tmp);
+    orc_dcopy((int)mn, tmp, R);
+    orc_dcopy((int)mn, R, P);
+    orc_dcopy((int)mn, R, Rs);
+    fnb = orc_dnrm2((int)mn, B);
+    for (i = 0; i < max_iter; i++) {                         /* :58-110 */
+        gemm1(m, n, 1, v, rp1, ci1, P, 0, V);
+        orc_dgemm('T', 'N', n, n, m, 1, Rs, m, V, m, 0, tmp, n);
+        orc_dgemm('T', 'N', n, n, m, 1, Rs, m, R, m, 0, alpha, n);
+        orc_dgesv(n, n, tmp, n, ipiv, alpha, n);
+        orc_dgemm('N', 'N', m, n, n, -1, V, m, alpha, n, 1, R, m);
+        gemm1(m, n, 1, v, rp1, ci1, R, 0, T);
+        omega = orc_ddot((int)mn, T, R) / orc_ddot((int)mn, T, T);
+        orc_dgemm('N', 'N', m, n, n, 1, P, m, alpha, n, 1, X, m);
+        orc_daxpy((int)mn, omega, R, X);
+        orc_daxpy((int)mn, -omega, T, R);
+        orc_dgemm('T', 'N', n, n, m, -1, Rs, m, T, m, 0, beta, n);
+        orc_dgetrs(n, n, tmp, n, ipiv, beta, n);
+        orc_daxpy((int)mn, -omega, V, P);
+        orc_dgemm('N', 'N', m, n, n, 1, P, m, beta, n, 0, tmp, m);
+        orc_daxpy((int)mn, 1, R, tmp);
+        orc_dcopy((int)mn, tmp, P);
+        orc_myqr(m, n, P, tmp);
+        error = orc_dnrm2((int)mn, R) / fnb;
+        if (error < tol) break;
+    }
+    free(ci1); free(rp1); free(ipiv); free(base1); free(base0);
+    if (error >= tol) return NOT_CONVERGED_CODE;
+    return i;
 }

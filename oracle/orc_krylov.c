@@ -7,6 +7,8 @@
  *   orc_bicg        src/bicg.c:7-88
  *   orc_bicgstabl   src/bicgstabl.c:13-138
  *   orc_shbicgstabl src/shbicgstabl.c:7-285
+ *   orc_bicgstab    src/bicgstab.c:5-72
+ *   orc_shbicg      src/shbicg.c:5-129
  * Each returns the 0-based loop index at the break, or -1 (NOT_CONVERGED), with
  * the reference's final test `error >= tol` (so a NaN error returns max_iter).
  */
@@ -326,4 +328,102 @@ int orc_shbicgstabl(int n, const int *rp, const int *ci, const double *v,
     for (i = 0; i < l2; i++) { free(r[i]); free(p[i]); }
     if (error >= tol) return NOT_CONVERGED_CODE;
     return iter;
+}
+
+/* src/bicgstab.c:5-72: BiCGSTAB. beta = alpha / omega * rho_new / rho evaluated
+ * left to right (:56); x gets alpha p and omega s as two separate updates (:49-50). */
+int orc_bicgstab(int n, const int *rp, const int *ci, const double *v,
+                 const double *b, double tol, int max_iter, double *x) {
+    int i;
+    double error = 0, alpha, beta, omega, rho, rho_new, norm_b;
+    double *base = dalloc((size_t)n * 7);
+    double *r = base, *rs = base + n, *s = base + 2 * (size_t)n, *p = base + 3 * (size_t)n;
+    double *ap = base + 4 * (size_t)n, *as = base + 5 * (size_t)n, *tmp = base + 6 * (size_t)n;
+    orc_csrgemv('n', n, v, rp, ci, x, tmp);                  /* bicgstab.c:22-30 */
+    orc_dscal(n, -1, tmp);
+    orc_daxpy(n, 1, b, tmp);
+    orc_dcopy(n, tmp, r);
+    orc_dcopy(n, r, p);
+    orc_dcopy(n, r, rs);
+    norm_b = orc_dnrm2(n, b);
+    rho = orc_ddot(n, rs, r);
+    for (i = 0; i < max_iter; i++) {                         /* bicgstab.c:33-65 */
+        orc_csrgemv('n', n, v, rp, ci, p, ap);
+        alpha = rho / orc_ddot(n, rs, ap);
+        orc_dcopy(n, r, s);
+        orc_daxpy(n, -alpha, ap, s);
+        orc_csrgemv('n', n, v, rp, ci, s, as);
+        omega = orc_ddot(n, as, s) / orc_ddot(n, as, as);
+        orc_daxpy(n, alpha, p, x);
+        orc_daxpy(n, omega, s, x);
+        orc_dcopy(n, s, r);
+        orc_daxpy(n, -omega, as, r);
+        rho_new = orc_ddot(n, rs, r);
+        beta = alpha / omega * rho_new / rho;
+        rho = rho_new;
+        orc_daxpy(n, -omega, ap, p);
+        orc_dscal(n, beta, p);
+        orc_daxpy(n, 1, r, p);
+        error = orc_dnrm2(n, r) / norm_b;
+        if (error < tol) break;
+    }
+    free(base);
+    if (error >= tol) return NOT_CONVERGED_CODE;
+    return i;
+}
+
+/* src/shbicg.c:5-129: shifted BiCG. x holds n*(ns+1) entries, the seed first and
+ * shift k at x + (k+1) n; the seed loop is BiCG with beta = -rho/rho_old (:53) and
+ * p = -beta p + r (:56-57); each shift k carries pi_k, p_k and x_k (:78-96). */
+int orc_shbicg(int n, const int *rp, const int *ci, const double *v, const double *b,
+               const double *sigma, int ns, double tol, int max_iter, double *x) {
+    int i, k;
+    double error = 0, alpha, beta, alpha_old, rho, rho_old, fnb;
+    double *base = dalloc((size_t)n * 7);
+    double *r = base, *sr = base + n, *p = base + 2 * (size_t)n, *sp = base + 3 * (size_t)n;
+    double *ap = base + 4 * (size_t)n, *atp = base + 5 * (size_t)n, *tmp = base + 6 * (size_t)n;
+    double *shb = dalloc((size_t)ns * 5);
+    double *alpha_shift = shb, *beta_shift = shb + ns, *pi = shb + 2 * ns, *pi_new = shb + 3 * ns,
+           *pi_old = shb + 4 * ns;
+    double *ps = dalloc((size_t)n * ns);
+    orc_csrgemv('n', n, v, rp, ci, x, tmp);                  /* shbicg.c:21-29 */
+    orc_dscal(n, -1, tmp);
+    orc_daxpy(n, 1, b, tmp);
+    orc_dcopy(n, tmp, r);
+    orc_dcopy(n, tmp, sr);
+    fnb = orc_dnrm2(n, b);
+    alpha_old = 1;
+    rho_old = 1;
+    for (k = 0; k < ns; k++) { pi_old[k] = 1; pi[k] = 1; }  /* :41-46 */
+    for (i = 0; i < max_iter; i++) {                         /* :49-116 */
+        rho = orc_ddot(n, r, sr);
+        beta = -rho / rho_old;
+        orc_dscal(n, -beta, p);
+        orc_daxpy(n, 1, r, p);
+        orc_dscal(n, -beta, sp);
+        orc_daxpy(n, 1, sr, sp);
+        orc_csrgemv('n', n, v, rp, ci, p, ap);
+        orc_csrgemv('t', n, v, rp, ci, sp, atp);
+        alpha = rho / orc_ddot(n, ap, sr);
+        orc_daxpy(n, alpha, p, x);
+        for (k = 0; k < ns; k++) {
+            double *pk = ps + (size_t)k * n, *xk = x + (size_t)(k + 1) * n;
+            pi_new[k] = (1 + alpha * sigma[k]) * pi[k] + alpha * beta / alpha_old * (pi_old[k] - pi[k]);
+            beta_shift[k] = (pi_old[k] / pi[k]) * (pi_old[k] / pi[k]) * beta;
+            alpha_shift[k] = pi[k] / pi_new[k] * alpha;
+            orc_dscal(n, -beta_shift[k], pk);
+            orc_daxpy(n, 1.0 / pi[k], r, pk);
+            orc_daxpy(n, alpha_shift[k], pk, xk);
+        }
+        orc_daxpy(n, -alpha, ap, r);
+        orc_daxpy(n, -alpha, atp, sr);
+        alpha_old = alpha;
+        rho_old = rho;
+        for (k = 0; k < ns; k++) { pi_old[k] = pi[k]; pi[k] = pi_new[k]; }
+        error = orc_dnrm2(n, r) / fnb;
+        if (error < tol) break;
+    }
+    free(ps); free(shb); free(base);
+    if (error >= tol) return NOT_CONVERGED_CODE;
+    return i;
 }

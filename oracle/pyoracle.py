@@ -152,6 +152,29 @@ def shbicgstabl(n, rp, ci, v, b, l, sigma, tol, max_iter):
     return ret, x.reshape(len(sig) + 1, n)
 
 
+def bicgstab(n, rp, ci, v, b, tol, max_iter, x0=None):
+    return _solve(lib().orc_bicgstab, n, rp, ci, v, b, x0, c_dbl(tol), max_iter)
+
+
+def shbicg(n, rp, ci, v, b, sigma, tol, max_iter):
+    """src/shbicg.c: returns (ret, X) with X[0] the seed and X[k+1] shift k."""
+    rp, ci, v = _csr(rp, ci, v)
+    sig = np.ascontiguousarray(sigma, np.float64)
+    x = np.zeros(n * (len(sig) + 1))
+    ret = lib().orc_shbicg(n, _ip(rp), _ip(ci), _dp(v), _dp(np.ascontiguousarray(b, np.float64)),
+                           _dp(sig), len(sig), c_dbl(tol), max_iter, _dp(x))
+    return ret, x.reshape(len(sig) + 1, n)
+
+
+def blbicgstab(n, rp, ci, v, B, tol, max_iter, X0=None):
+    rp, ci, v = _csr(rp, ci, v)
+    B = np.asfortranarray(B, np.float64)
+    s = B.shape[1]
+    X = np.array(X0 if X0 is not None else np.zeros((n, s)), dtype=np.float64, order="F", copy=True)
+    ret = lib().orc_blbicgstab(n, _ip(rp), _ip(ci), _dp(v), s, _dp(B), c_dbl(tol), max_iter, _dp(X))
+    return ret, X
+
+
 def blcg(n, rp, ci, v, B, tol, max_iter, X0=None, p0_mode=0):
     rp, ci, v = _csr(rp, ci, v)
     B = np.asfortranarray(B, np.float64)

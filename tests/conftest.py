@@ -36,6 +36,27 @@ def read_mtx(path):
     return M, rp, c[order], v[order], sym
 
 
+def oracle_at_index(call, k):
+    """The oracle's iterate after loop index k, where the device broke.
+
+    ``call(tol, max_iter)`` runs an oracle solver and returns (ret, x, ...). Every
+    reference loop runs ``for (i = start; i < max_iter; i++)`` and tests
+    ``error < tol`` after the iteration's update (cg.c:33-60, cr.c:37-68,
+    bicg.c:35-73, bicgstabl.c:53-118, blcg.c:66-106, blbicgstabl.c:96-212,
+    shbicgstabl.c:83-247), so with tol 0 the break never fires and max_iter = k + 1
+    performs exactly the iterations start..k that the device performed before
+    breaking at k. Returns the oracle's x at that index."""
+    assert k >= 0, k
+    return call(0.0, k + 1)[1]
+
+
+def count_index_mismatch(ret, cret):
+    """Record (in the test log) a device/oracle break-index difference; the x check
+    runs at the device's index either way."""
+    if ret != cret:
+        print(f"break index: device {ret}, oracle {cret} (x compared at the device's index)")
+
+
 @pytest.fixture(scope="session")
 def bfwb398():
     return read_mtx(os.path.join(GOLDEN, "testmat", "bfwb398.mtx"))

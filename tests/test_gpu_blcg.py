@@ -9,7 +9,7 @@ import pytest
 from bksp_amd import bksp
 from bksp_amd import device as dv
 from bksp_amd import gen
-from conftest import expand_lower
+from conftest import count_index_mismatch, expand_lower, oracle_at_index
 from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
@@ -43,8 +43,14 @@ def test_reference_scenario_16_unit_rhs(bfwb398):
     Af = expand_lower(n, rp, ci, v)
     res = np.linalg.norm(B - Af @ Xm.a) / np.linalg.norm(B)
     assert res < 1e-9, (ret, cret, res)
-    if ret >= 0 and cret >= 0:
-        assert abs(ret - cret) <= max(2, int(0.1 * cret)), (ret, cret)
+    assert ret >= 0 and cret >= 0, (ret, cret)
+    assert abs(ret - cret) <= max(2, int(0.1 * cret)), (ret, cret)
+    # both reach 1e-14: X agrees with the oracle's converged X, and with its
+    # iterate at the device's break index
+    assert rel(Xm.a, cX) < 1e-8
+    count_index_mismatch(ret, cret)
+    Xk = oracle_at_index(lambda t, k: po.blcg(n, rp, ci, v, B, t, k), ret)
+    assert rel(Xm.a, Xk) < 1e-8
 
 
 @pytest.mark.parametrize("fix_p0", [0, 1])
@@ -76,8 +82,9 @@ def test_poisson_parity(m, s):
     Af = expand_lower(n, rp, ci, v)
     res = np.linalg.norm(B - Af @ X) / np.linalg.norm(B)
     assert res > 0 and abs(info.true_res - res) <= 1e-6 * res, (info.true_res, res)
-    if info.ret == cret:
-        assert rel(X, cX) < 1e-8
+    count_index_mismatch(info.ret, cret)
+    Xk = oracle_at_index(lambda t, k: po.blcg(n, rp, ci, v, B, t, k), info.ret)
+    assert rel(X, Xk) < 1e-8
 
 
 def test_cfg2_full_size_blcg_s8():
@@ -92,6 +99,25 @@ def test_cfg2_full_size_blcg_s8():
     assert info1.ret < info.ret
     print(f"cfg2 block CG s=8: P0=2V {info.ret} iters {info.time_ms:.1f} ms; "
           f"P0=V {info1.ret} iters {info1.time_ms:.1f} ms")
+
+
+def test_cfg2_full_size_blcg_first_iterates():
+    """cfg2's block CG at its own size (128^3 Poisson, s = 8, RHS N(0,1) seed 202,
+    P0 = 2V as the reference): the device iterates after 1 and 5 iterations equal the
+    oracle's (src/blcg.c:66-106) to rounding."""
+    n, rp, ci, v = gen.poisson3d(128, lower=True)
+    B = gen.normal(202, n * 8).reshape(8, n).T.copy()
+    M = dv.Matrix(n, rp, ci, v, symmetric=True)
+    dB = dv.DeviceArray.from_numpy(np.ascontiguousarray(B.T))
+    for k in (1, 5):
+        dX = dv.DeviceArray.from_numpy(np.zeros((8, n)))
+        info = dv.blcg(M, 8, dB, dX, 1e-300, k)
+        assert info.ret == -1
+        X = dX.numpy().T
+        _, cX = po.blcg(n, rp, ci, v, B, 1e-300, k)
+        err = rel(X, cX)
+        print(f"cfg2 block CG s=8 after {k} iterations: |X - X_oracle|_F / |X_oracle|_F = {err:.2e}")
+        assert err < 1e-10, (k, err)
 
 
 @pytest.mark.parametrize("m,s,fix_p0,path", [(24, 8, 0, 0), (24, 8, 1, 0), (20, 3, 0, 0), (24, 8, 0, 1)])

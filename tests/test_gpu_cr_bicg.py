@@ -1,12 +1,12 @@
 """GPU CR (dcr) and BiCG (dbicg) against the oracle's restatements of src/cr.c and
-src/bicg.c: loop index within +-5% (at least +-1), x within 1e-8 relative at equal
-indices, true FP64 residual at the tolerance."""
+src/bicg.c: loop index within +-5% (at least +-1), x within 1e-8 relative of the
+oracle's iterate at the device's break index, true FP64 residual at the tolerance."""
 import numpy as np
 import pytest
 
 from bksp_amd import bksp
 from bksp_amd import gen
-from conftest import expand_lower, to_scipy
+from conftest import count_index_mismatch, expand_lower, oracle_at_index, to_scipy
 from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
@@ -41,6 +41,9 @@ def test_cr_reference_scenario(bfwb398):
     A = expand_lower(n, rp, ci, v)
     assert np.linalg.norm(b - A @ x) < 1e-10
     assert rel(x, cx) < 1e-8
+    k = ret if ret >= 0 else 499
+    xk = oracle_at_index(lambda t, m: po.cr(n, rp, ci, v, b, t, m), k)
+    assert rel(x, xk) < 1e-8, rel(x, xk)
 
 
 def test_cr_poisson_parity_and_codes():
@@ -52,8 +55,9 @@ def test_cr_poisson_parity_and_codes():
     cret, cx = po.cr(n, rp, ci, v, b, 1e-8, 2000)
     assert ret >= 1 and close_index(ret, cret), (ret, cret)
     assert np.linalg.norm(b - expand_lower(n, rp, ci, v) @ x) / np.linalg.norm(b) < 1.5e-8
-    if ret == cret:
-        assert rel(x, cx) < 1e-8
+    count_index_mismatch(ret, cret)
+    xk = oracle_at_index(lambda t, k: po.cr(n, rp, ci, v, b, t, k), ret)
+    assert rel(x, xk) < 1e-8
     assert bksp.dcr(A, b, 1e-8, 1, np.zeros(n)) == 1                   # cr.c:37,76-79
     assert bksp.dcr(bksp.dspmat(n, rp, ci, v), b, 1e-8, 10, np.zeros(n)) == bksp.INVALID_MATRIX_TYPE
     assert bksp.dcr(A, b, 1e-30, 3, np.zeros(n)) == bksp.NOT_CONVERGED
@@ -71,6 +75,9 @@ def test_bicg_reference_scenario(pde2961):
         assert close_index_fixture(ret, cret), (ret, cret)
     assert np.linalg.norm(b - to_scipy(n, rp, ci, v) @ x) < 1e-9
     assert rel(x, cx) < 1e-6
+    k = ret if ret >= 0 else 499
+    xk = oracle_at_index(lambda t, m: po.bicg(n, rp, ci, v, b, t, m), k)
+    assert rel(x, xk) < 1e-8, rel(x, xk)
 
 
 def test_bicg_convdiff_parity():
@@ -82,6 +89,9 @@ def test_bicg_convdiff_parity():
     assert ret >= 0 and close_index(ret, cret), (ret, cret)
     assert np.linalg.norm(b - to_scipy(n, rp, ci, v) @ x) / np.linalg.norm(b) < 2e-10
     assert rel(x, cx) < 1e-6
+    count_index_mismatch(ret, cret)
+    xk = oracle_at_index(lambda t, k: po.bicg(n, rp, ci, v, b, t, k), ret)
+    assert rel(x, xk) < 1e-8
     # the first iterates agree to rounding
     for k in (1, 5):
         xk = np.zeros(n)

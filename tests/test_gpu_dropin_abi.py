@@ -71,18 +71,66 @@ def test_csr_kernel_symbols_bitwise(pde2961, bfwb398):
     assert load().dcsrgemm0(b"x", n, 1, dp(v), ip(rp), ip(ci), 0, dp(B), dp(C)) == -1
 
 
-@pytest.mark.parametrize("s", [1, 5, 8, 16])
-def test_myqr(s):
+def _myqr(V):
+    m, s = V.shape
+    Q = np.asfortranarray(V.copy())
+    R = np.full((s, s), np.nan, order="F")
+    rc = load().myqr(m, s, dp(Q), dp(R))
+    return rc, Q, R
+
+
+@pytest.mark.parametrize("m,s", [(1000, 1), (1000, 5), (1000, 8), (1000, 16), (300, 40), (50, 50), (70000, 8)])
+def test_myqr_lapack_convention(m, s):
+    """src/myqr.c:4-39: dgeqrf + dorgqr. The device Householder QR returns LAPACK's Q
+    and R, signs included (numpy's qr is LAPACK dgeqrf/dorgqr), to 1e-12; the oracle's
+    restatement agrees too. Square and wider-than-16 panels included."""
     rng = np.random.default_rng(52 + s)
-    V = np.asfortranarray(rng.standard_normal((1000, s)))
-    Q = V.copy(order="F")
-    R = np.zeros((s, s), order="F")
-    assert load().myqr(1000, s, dp(Q), dp(R)) == 0
-    assert np.allclose(Q.T @ Q, np.eye(s), atol=1e-13)
-    assert np.allclose(Q @ R, V, atol=1e-12)
-    assert np.allclose(np.tril(R, -1), 0) and np.all(np.diag(R) > 0)
-    _, Ro = po.myqr(V)      # LAPACK signs: |R| agrees
-    assert np.allclose(np.abs(R), np.abs(Ro), atol=1e-11)
+    V = np.asfortranarray(rng.standard_normal((m, s)))
+    rc, Q, R = _myqr(V)
+    assert rc == 0
+    Qn, Rn = np.linalg.qr(V)
+    assert np.abs(R - Rn).max() <= 1e-12 * np.abs(Rn).max()
+    assert np.abs(Q - Qn).max() <= 1e-12
+    assert np.array_equal(np.tril(R, -1), np.zeros((s, s)))
+    Qo, Ro = po.myqr(V)
+    assert np.abs(R - Ro).max() <= 1e-12 * np.abs(Ro).max() and np.abs(Q - Qo).max() <= 1e-12
+
+
+def test_myqr_ill_conditioned_and_rank_deficient():
+    """LAPACK's Householder QR returns 0 for any panel; so does the device one. cond(V)
+    = 1e10: Q orthonormal, QR = V, R equal to LAPACK's (diagonal signs included). A
+    repeated column: R's diagonal entry for it is at rounding level, Q stays orthonormal
+    and QR = V (a Cholesky-based QR breaks down on both)."""
+    rng = np.random.default_rng(7)
+    m, s = 2000, 8
+    U, _ = np.linalg.qr(rng.standard_normal((m, s)))
+    W, _ = np.linalg.qr(rng.standard_normal((s, s)))
+    V = np.asfortranarray(U @ np.diag(np.logspace(0, -10, s)) @ W.T)
+    rc, Q, R = _myqr(V)
+    assert rc == 0
+    Qn, Rn = np.linalg.qr(V)
+    assert np.abs(Q.T @ Q - np.eye(s)).max() < 1e-13
+    assert np.abs(Q @ R - V).max() <= 1e-14 * np.abs(V).max() * 10
+    assert np.abs(R - Rn).max() <= 1e-12 * np.abs(Rn).max()
+    assert np.array_equal(np.sign(np.diag(R)), np.sign(np.diag(Rn)))
+    V = np.asfortranarray(rng.standard_normal((m, s)))
+    V[:, 5] = V[:, 2]
+    rc, Q, R = _myqr(V)
+    assert rc == 0
+    assert abs(R[5, 5]) < 1e-12 * np.abs(R).max()
+    assert np.abs(Q.T @ Q - np.eye(s)).max() < 1e-13
+    assert np.abs(Q @ R - V).max() < 1e-12
+    Qn, Rn = np.linalg.qr(V)
+    assert np.abs(R[:5] - Rn[:5]).max() <= 1e-12 * np.abs(Rn).max()
+
+
+def test_myqr_argument_codes():
+    """dorgqr's info for n > m and m < 0 (myqr returns it, src/myqr.c:30-38)."""
+    V = np.asfortranarray(np.ones((3, 4)))
+    R = np.zeros((4, 4), order="F")
+    assert load().myqr(3, 4, dp(V), dp(R)) == -2
+    assert load().myqr(-1, 2, dp(V), dp(R)) == -1
+    assert load().myqr(3, 0, dp(V), dp(R)) == 0
 
 
 def test_dhorner_comb_and_dmat():

@@ -193,3 +193,93 @@ def test_reference_scenarios_true_residual(bfwb398, pde2961):
     b = np.zeros(n); b[0] = 1
     for ret, x in (po.bicg(n, rp, ci, v, b, 1e-14, 500), po.bicgstabl(n, rp, ci, v, b, 4, 1e-14, 500)):
         assert np.linalg.norm(b - A @ x) < 1e-9
+
+
+# ---- link-parity solvers: src/bicgstab.c, src/shbicg.c, src/blbicgstab.c
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 6])
+def test_bicgstab_equals_bicgstabl_one(k):
+    """BiCGSTAB is BiCGSTAB(1) (van der Vorst 1992; Sleijpen-Fokkema 1993): the same
+    iterates in exact arithmetic, through two different recurrences (bicgstab.c's
+    rho/omega form, bicgstabl.c's BiCG step + degree-1 MR polynomial, itself pinned by
+    its closed form above)."""
+    n = 40
+    A, rp, ci, v = gen_dense(n, 90 + k)
+    b = np.random.default_rng(k + 13).standard_normal(n)
+    ret, x = po.bicgstab(n, rp, ci, v, b, TINY, k)
+    assert ret == -1
+    _, xl = po.bicgstabl(n, rp, ci, v, b, 1, TINY, k)
+    assert rel(x, xl) < 1e-9
+
+
+def shifted_csr(n, rp, ci, v, sigma):
+    """A + sigma I as CSR (scipy; the diagonal entry added or inserted)."""
+    import scipy.sparse as sp
+    S = (to_scipy(n, rp, ci, v) + sigma * sp.identity(n, format="csr")).tocsr()
+    S.sort_indices()
+    return S.indptr.astype(np.int32), S.indices.astype(np.int32), S.data.copy()
+
+
+@pytest.mark.parametrize("k", [1, 3, 8])
+def test_shbicg_seed_is_bicg_and_shifts_are_bicg_on_shifted_systems(pde2961, k):
+    """Shifted BiCG (Frommer 2003): with x0 = 0 the shifted residuals stay collinear
+    with the seed's, so the seed iterate is BiCG's on A and shift k's iterate is BiCG's
+    on A + sigma_k I, iteration by iteration (orc_bicg pinned by its Petrov-Galerkin
+    closed form above). A wrong pi recurrence or shift scaling breaks the second."""
+    n, rp, ci, v, _ = pde2961
+    b = np.random.default_rng(5).standard_normal(n)
+    sig = [0.0, 0.2, 0.4, 0.8]
+    ret, X = po.shbicg(n, rp, ci, v, b, sig, TINY, k)
+    assert ret == -1
+    _, x0 = po.bicg(n, rp, ci, v, b, TINY, k)
+    assert rel(X[0], x0) < 1e-10
+    for q, s in enumerate(sig):
+        rps, cis, vs = shifted_csr(n, rp, ci, v, s)
+        _, xs = po.bicg(n, rps, cis, vs, b, TINY, k)
+        assert rel(X[q + 1], xs) < 1e-8, (q, s, rel(X[q + 1], xs))
+
+
+def test_shbicg_reference_scenario_solves_every_system(pde2961):
+    """tests/test.c:243-262 scenario: each (A + sigma_k I) x_k = b to the tolerance's scale."""
+    n, rp, ci, v, _ = pde2961
+    A = to_scipy(n, rp, ci, v)
+    b = np.zeros(n); b[0] = 1.0
+    sig = [0.0, 0.2, 0.4, 0.8]
+    ret, X = po.shbicg(n, rp, ci, v, b, sig, 1e-14, 500)
+    assert ret > 0
+    for q, s in enumerate([0.0] + sig):
+        assert np.linalg.norm(b - A @ X[q] - s * X[q]) < 1e-12
+
+
+@pytest.mark.parametrize("k", [1, 2, 5])
+def test_blbicgstab_one_column_is_bicgstab(k):
+    """With one right-hand side the block method is BiCGSTAB: alpha, beta are 1 x 1
+    and the orthonormalisation of P only rescales it, which changes neither P alpha
+    nor the next direction (both invariant under P -> P d)."""
+    n = 40
+    A, rp, ci, v = gen_dense(n, 110 + k)
+    b = np.random.default_rng(k + 17).standard_normal(n)
+    ret, X = po.blbicgstab(n, rp, ci, v, b[:, None], TINY, k)
+    assert ret == -1
+    _, x = po.bicgstab(n, rp, ci, v, b, TINY, k)
+    assert rel(X[:, 0], x) < 1e-9
+
+
+def test_blbicgstab_first_step_and_convergence():
+    """First iterate: the block Petrov-Galerkin step X = P0 alpha with
+    Rs^T (R0 - A P0 alpha) = 0 (P0 = Rs = R0 = B), then the Frobenius minimal-residual
+    step omega = argmin ||S - omega A S||_F on S = R0 - A P0 alpha; and a converged
+    block solve has a small true residual for every column."""
+    n, s = 50, 3
+    A, rp, ci, v = gen_dense(n, 130)
+    B = np.random.default_rng(131).standard_normal((n, s))
+    ret, X = po.blbicgstab(n, rp, ci, v, B, TINY, 1)
+    alpha = np.linalg.solve(B.T @ (A @ B), B.T @ B)
+    S = B - A @ B @ alpha
+    AS = A @ S
+    omega = np.sum(AS * S) / np.sum(AS * AS)
+    assert rel(X, B @ alpha + omega * S) < 1e-10
+    ret, X = po.blbicgstab(n, rp, ci, v, B, 1e-12, 200)
+    assert ret >= 0
+    assert np.linalg.norm(B - A @ X) / np.linalg.norm(B) < 1e-10
