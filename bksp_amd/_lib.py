@@ -98,6 +98,8 @@ _SIGS = {
     "bkh_bench_spmv": (c_int, [c_vp, c_int, c_vp, c_vp, c_int, c_dp]),
     "bkh_bench_copy": (c_int, [c_vp, c_vp, ctypes.c_size_t, c_int, c_dp]),
     # solvers (device vectors)
+    "bkh_cg_noalloc": (c_int, [c_vp, c_vp, c_vp, c_dbl, c_int, ctypes.POINTER(BkhInfo)]),
+    "bkh_cg_noalloc_max_rows": (c_ll, []),
     "bkh_cg": (c_int, [c_vp, c_vp, c_vp, c_dbl, c_int, ctypes.POINTER(BkhOpts),
                        ctypes.POINTER(BkhInfo)]),
     "bkh_cr": (c_int, [c_vp, c_vp, c_vp, c_dbl, c_int, ctypes.POINTER(BkhOpts),

@@ -23,6 +23,8 @@
 // recurrence so only r and q are gathered, x update deferred into the next kernel),
 // elsewhere the two-kernel form of the FP64 path.
 #include <math.h>
+#include <chrono>
+#include <mutex>
 #include "bkh_cgfused.hpp"
 #include "bkh_host.hpp"
 #include "bkh_solver_common.hpp"
@@ -159,39 +161,47 @@ static void launch_cgf_a_sell(hipStream_t s, const bkh_matrix *A, const CgfGrid 
                            (const double *)partB, g.GB, partA);
 }
 
-// Graph of K fused iterations plus the tail, on vectors T with values VT
-// (cd: 16-bit column deltas or null).
+// Issues K fused iterations plus the tail on stream s, on vectors T with values VT
+// (cd: 16-bit column deltas or null): the body of the iteration graph, and the
+// iteration of the allocation-free entry point, which launches it directly.
+template <typename VT, typename T>
+static void cgf_issue(hipStream_t s, const bkh_matrix *A, const CgfGrid &g, const VT *val, const short *cd, T *x,
+                      T *r, T *P0, T *P1, T *q, Ctl *c, double *partA, double *partB, int K) {
+    const long long n = A->n;
+    // kernel B on 16-byte vectors when every array it streams allows (x is the caller's)
+    const bool vec = (((uintptr_t)x | (uintptr_t)r | (uintptr_t)P0 | (uintptr_t)P1 | (uintptr_t)q) & 15) == 0;
+    for (int j = 0; j < K; j++) {
+        if (g.sell) {
+            if (g.BSA == 1024) launch_cgf_a_sell<VT, T, 1024>(s, A, g, r, P0, P1, q, c, j, partB, partA);
+            else launch_cgf_a_sell<VT, T, 256>(s, A, g, r, P0, P1, q, c, j, partB, partA);
+        } else if (cd)
+            hipLaunchKernelGGL((k_cgf_a<VT, T, true>), dim3(g.GA), dim3(64 * CGF_WPB), 0, s, A->nwblk, g.chunk,
+                               A->rp, A->wblk, A->ci, cd, val, (const T *)r, P0, P1, q, c, j,
+                               (const double *)partB, g.GB, partA);
+        else
+            hipLaunchKernelGGL((k_cgf_a<VT, T, false>), dim3(g.GA), dim3(64 * CGF_WPB), 0, s, A->nwblk, g.chunk,
+                               A->rp, A->wblk, A->ci, cd, val, (const T *)r, P0, P1, q, c, j,
+                               (const double *)partB, g.GB, partA);
+        if (vec)
+            hipLaunchKernelGGL((k_cgf_b<T, true>), dim3(g.GB), dim3(512), 0, s, n, x, r, (const T *)P0,
+                               (const T *)P1, (const T *)q, c, j, (const double *)partA, g.GA, partB);
+        else
+            hipLaunchKernelGGL((k_cgf_b<T, false>), dim3(g.GB), dim3(512), 0, s, n, x, r, (const T *)P0,
+                               (const T *)P1, (const T *)q, c, j, (const double *)partA, g.GA, partB);
+    }
+    hipLaunchKernelGGL(k_cgf_tail, dim3(1), dim3(64), 0, s, c, K, (const double *)partB, g.GB);
+}
+
+// Graph of K fused iterations plus the tail.
 template <typename VT, typename T>
 static hipGraphExec_t cgf_capture(ThreadCtx *t, hipStream_t s, const bkh_matrix *A, const VT *val, const short *cd,
                                   T *x, T *r, T *P0, T *P1, T *q, Ctl *c, double *partA, double *partB, int K) {
     const CgfGrid g = cgf_grid(A);
-    const long long n = A->n;
     GraphKey key = {0xC6F0, (uintptr_t)A, (uintptr_t)val, (uintptr_t)cd, (uintptr_t)x, (uintptr_t)r, (uintptr_t)P0,
                     (uintptr_t)P1, (uintptr_t)q, (uintptr_t)c, (uintptr_t)partA, (uintptr_t)partB,
                     (uintptr_t)sizeof(T), (uintptr_t)K, (uintptr_t)(g.BSA * 65536 + g.GA)};
-    // kernel B on 16-byte vectors when every array it streams allows (x is the caller's)
-    const bool vec = (((uintptr_t)x | (uintptr_t)r | (uintptr_t)P0 | (uintptr_t)P1 | (uintptr_t)q) & 15) == 0;
     return capture_cached(t, key, s, 1, [=]() {   // by value: eager mode runs the body after return
-        for (int j = 0; j < K; j++) {
-            if (g.sell) {
-                if (g.BSA == 1024) launch_cgf_a_sell<VT, T, 1024>(s, A, g, r, P0, P1, q, c, j, partB, partA);
-                else launch_cgf_a_sell<VT, T, 256>(s, A, g, r, P0, P1, q, c, j, partB, partA);
-            } else if (cd)
-                hipLaunchKernelGGL((k_cgf_a<VT, T, true>), dim3(g.GA), dim3(64 * CGF_WPB), 0, s, A->nwblk, g.chunk,
-                                   A->rp, A->wblk, A->ci, cd, val, (const T *)r, P0, P1, q, c, j,
-                                   (const double *)partB, g.GB, partA);
-            else
-                hipLaunchKernelGGL((k_cgf_a<VT, T, false>), dim3(g.GA), dim3(64 * CGF_WPB), 0, s, A->nwblk, g.chunk,
-                                   A->rp, A->wblk, A->ci, cd, val, (const T *)r, P0, P1, q, c, j,
-                                   (const double *)partB, g.GB, partA);
-            if (vec)
-                hipLaunchKernelGGL((k_cgf_b<T, true>), dim3(g.GB), dim3(512), 0, s, n, x, r, (const T *)P0,
-                                   (const T *)P1, (const T *)q, c, j, (const double *)partA, g.GA, partB);
-            else
-                hipLaunchKernelGGL((k_cgf_b<T, false>), dim3(g.GB), dim3(512), 0, s, n, x, r, (const T *)P0,
-                                   (const T *)P1, (const T *)q, c, j, (const double *)partA, g.GA, partB);
-        }
-        hipLaunchKernelGGL(k_cgf_tail, dim3(1), dim3(64), 0, s, c, K, (const double *)partB, g.GB);
+        cgf_issue<VT, T>(s, A, g, val, cd, x, r, P0, P1, q, c, partA, partB, K);
     });
 }
 
@@ -650,4 +660,130 @@ extern "C" int bkh_bench_cg_kernel(bkh_matrix *A, int fp32, int reps, double *av
     HIP_OK(hipEventElapsedTime(&ms, t->ev0, t->ev1));
     *avg_ms = ms / reps;
     return 0;
+}
+
+// ------------------------------------------------------------ allocation-free CG
+//
+// bkh_cg_noalloc: plain FP64 CG (the fused two-kernel iteration above) whose whole
+// workspace -- control blocks, partial sums, r, the two p buffers, q -- lives in the
+// file-scope device array `scratch` below, so the call allocates nothing: no
+// hipMalloc/hipFree, no graph instantiation, no stream or event creation. The kernels
+// are issued K iterations at a time on the null stream and the host reads the
+// control block between groups (hipMemcpy into a stack copy). Every call on one GPU
+// uses that GPU's single copy of the array: calls that overlap on one GPU race, and
+// each call initialises everything it reads. Rows are limited to CGNA_MAX_ROWS.
+
+#define CGNA_MAX_ROWS (1 << 20)
+#define CGNA_MAX_PARTS 16384            // doubles for the initial reductions' partials
+#define CGNA_MAX_DEV 64
+static constexpr size_t CGNA_BYTES =
+    2 * (((sizeof(Ctl) + 255) & ~(size_t)255) + 256) + (sizeof(double) * CGNA_MAX_PARTS + 256) +
+    2 * (sizeof(double) * CGF_NPART + 256) + 4 * (sizeof(double) * (size_t)CGNA_MAX_ROWS + 256) + 256;
+static __device__ __align__(256) double scratch[CGNA_BYTES / sizeof(double)];
+
+__global__ void k_ctl_init(Ctl *c, double tol, int max_it, int it0);
+__global__ __launch_bounds__(256) void k_resid(long long n, const double *b, const double *x, double sigma,
+                                               double *r, Ctl *c, double *part);
+
+struct CgnaResidFin {   // ||b - A x|| / ||b|| into s[0]
+    __device__ void operator()(Ctl *c, const double *tot) const { c->s[0] = sqrt(tot[0]) / sqrt(tot[1]); }
+};
+
+// The device address of `scratch` on the current device, looked up once per device
+// (hipGetSymbolAddress with that device current); nullptr with the error set.
+static double *cgna_scratch() {
+    static std::mutex mu;
+    static double *addr[CGNA_MAX_DEV];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= CGNA_MAX_DEV) {
+        bkh_set_error("allocation-free CG: no current HIP device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            bkh_set_error("hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+            return nullptr;
+        }
+        addr[dev] = (double *)p;
+    }
+    return addr[dev];
+}
+
+extern "C" long long bkh_cg_noalloc_max_rows(void) { return CGNA_MAX_ROWS; }
+
+extern "C" int bkh_cg_noalloc(const bkh_matrix *A, const double *b, double *x, double tol, int max_iter,
+                              bkh_info *info) {
+    if (!A || !b || !x) return BKH_BAD_ARGUMENT;
+    const long long n = A->n;
+    if (n > CGNA_MAX_ROWS) {
+        bkh_set_error("allocation-free CG: %lld rows exceed the scratch limit of %d", n, CGNA_MAX_ROWS);
+        return BKH_BAD_ARGUMENT;
+    }
+    if (!cg_can_fuse(A)) {
+        bkh_set_error("allocation-free CG: needs the fused iteration (SELL-64 layout or rows of <= %d nonzeros)",
+                      SPMV_WCAP);
+        return BKH_BAD_ARGUMENT;
+    }
+    bkh_info inf;
+    memset(&inf, 0, sizeof inf);
+    if (max_iter <= 0) {                     // cg.c:15,67-70: error = 0 -> returns 0
+        inf.status = ST_CONVERGED;
+        if (info) *info = inf;
+        return 0;
+    }
+    const int G = vec_grid(n, 256);
+    const int nparts = std::max(spmv_grid(A), G) * 2;
+    if (nparts > CGNA_MAX_PARTS) {
+        bkh_set_error("allocation-free CG: %d partial sums exceed the scratch limit", nparts);
+        return BKH_BAD_ARGUMENT;
+    }
+    double *base = cgna_scratch();
+    if (!base) return BKH_DEVICE_ERROR;
+    Carve cv(base);
+    Ctl *c = cv.take<Ctl>(1), *cr = cv.take<Ctl>(1);
+    double *part = cv.take<double>(CGNA_MAX_PARTS);
+    double *partA = cv.take<double>(CGF_NPART), *partB = cv.take<double>(CGF_NPART);
+    double *r = cv.take<double>(n), *P = cv.take<double>(2 * n), *q = cv.take<double>(n);
+    if (cv.off > CGNA_BYTES) {
+        bkh_set_error("allocation-free CG: scratch layout overflow");
+        return BKH_DEVICE_ERROR;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    const hipStream_t s = 0;
+    const int K = default_graph_iters(n);
+    const CgfGrid g = cgf_grid(A);
+    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, s, c, tol, max_iter, 0);
+    Epi<0> e0;
+    memset(&e0, 0, sizeof e0);
+    launch_spmv<double, double, 0>(s, A, A->val, x, q, 1.0, 0.0, e0, nullptr, nullptr, NoFin());
+    hipLaunchKernelGGL(k_cg_init<double>, dim3(G), dim3(256), 0, s, n, b, (const double *)q, r, P, 1, part);
+    launch_finalize<2>(s, part, G, c, CgfInitFin());
+    LAUNCH_OK();
+    Ctl h;
+    const int max_groups = max_iter / K + 2;
+    for (int grp = 0; grp < max_groups; grp++) {
+        cgf_issue<double, double>(s, A, g, A->val, nullptr, x, r, P, P + n, q, c, partA, partB, K);
+        LAUNCH_OK();
+        HIP_OK(hipMemcpy(&h, c, sizeof(Ctl), hipMemcpyDeviceToHost));
+        if (h.done) break;
+    }
+    // FP64 true residual into q (the iteration is over)
+    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, s, cr, 0.0, 1, 0);
+    launch_spmv<double, double, 0>(s, A, A->val, x, q, 1.0, 0.0, e0, nullptr, nullptr, NoFin());
+    hipLaunchKernelGGL(k_resid, dim3(G), dim3(256), 0, s, n, b, (const double *)x, 0.0, q, cr, part);
+    launch_finalize<2>(s, part, G, cr, CgnaResidFin());
+    LAUNCH_OK();
+    Ctl hr;
+    HIP_OK(hipMemcpy(&hr, cr, sizeof(Ctl), hipMemcpyDeviceToHost));
+    inf.ret = ctl_result(&h);
+    inf.status = h.done ? h.status : ST_MAXIT;
+    inf.iters = h.result_it;
+    inf.rec_res = h.err;
+    inf.true_res = hr.s[0];
+    inf.time_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (info) *info = inf;
+    return inf.ret;
 }

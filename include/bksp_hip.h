@@ -134,6 +134,19 @@ int bkh_gram(int m, int s, const double *d_X, const double *d_Y, double *G);
  * The return value is what the matching bksp.h entry point returns. */
 int bkh_cg(bkh_matrix *A, const double *d_b, double *d_x, double tol, int max_iter,
            const bkh_opts *opts, bkh_info *info);
+/* Allocation-free CG. No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The plain FP64 iteration of bkh_cg (same
+ * kernels, bitwise the same x) with its whole workspace in a static device array of
+ * the library (about 34 MB per GPU, held until the process ends): no hipMalloc,
+ * hipFree, graph, stream or event is created. A must exist already (bkh_matrix_create
+ * allocates), must use the fused iteration (SELL-64, or rows of <= 512 nonzeros) and
+ * have at most bkh_cg_noalloc_max_rows() rows (2^20); otherwise BKH_BAD_ARGUMENT.
+ * Runs on the null stream; calls that overlap on one GPU share the array and race.
+ * info->time_ms is host wall time. */
+int bkh_cg_noalloc(const bkh_matrix *A, const double *d_b, double *d_x, double tol, int max_iter,
+                   bkh_info *info);
+long long bkh_cg_noalloc_max_rows(void);
 /* Conjugate residual (replaces src/cr.c:6-80); the loop index starts at 1. */
 int bkh_cr(bkh_matrix *A, const double *d_b, double *d_x, double tol, int max_iter,
            const bkh_opts *opts, bkh_info *info);
