@@ -2,8 +2,10 @@
 (2,097,152 rows, 14,581,760 expanded nonzeros, passed as its 8,339,456-entry
 lower triangle), RHS N(0,1) seed 202, x0 = 0, on one MI355X.
 
-One step = one complete solve from x0 = 0 to tol = 1e-8 (plain FP64 GPU path),
-inputs already resident in HBM. Prints one JSON line.
+One step = one complete solve from x0 = 0 to tol = 1e-8 (plain FP64 GPU path,
+bkh_cg on a matrix and vectors already resident in HBM). The bksp_dcg drop-in
+(host arrays: upload, layout, solve, download) is timed beside it and is not
+`value`. Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -33,6 +35,37 @@ def load_traffic(name):
         with open(p) as f:
             return json.load(f)
     return None
+
+
+def _sha256(*arrays):
+    import hashlib
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _host_info():
+    """CPU model and the cores a threaded CPU run may use on this host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", avail))
+    return model, min(threads, avail), avail
 
 
 def _timed(fn, reps=1):
@@ -89,12 +122,14 @@ def bench_cfg2_block(dv, gen, m=128, s=8):
     out = {"workload": f"block CG s={s}, Poisson {m}^3, tol 1e-8"}
     for fix, key in ((0, "p0_2v_as_reference"), (1, "p0_v")):
         info, ms = _timed(lambda: run(fix), 1)
+        X64 = dX.numpy()
         run(fix, dv.BKH_FP32_IR)
         i32, ms32 = _timed(lambda: run(fix, dv.BKH_FP32_IR), 1)
         out[key] = {"fp64_ms": round(ms, 2), "iters": info.ret, "true_res": info.true_res,
                     "ms_per_iter": round(ms / max(1, info.ret + 1), 3),
                     "fp32_ir": {"ms": round(ms32, 2), "inner_iters": i32.inner_iters, "passes": i32.passes,
-                                "true_res": i32.true_res, "speedup_vs_fp64": round(ms / ms32, 3)}}
+                                "true_res": i32.true_res, "speedup_vs_fp64": round(ms / ms32, 3),
+                                "x_rel_err_vs_fp64": _rel(dX.numpy(), X64)}}
     return out
 
 
@@ -112,11 +147,14 @@ def bench_cfg3(dv, gen, m=96):
 
     run()
     (info, tr), ms = _timed(run, 1)
+    X64 = dx.numpy().reshape(65, n)
     (info32, tr32), ms32 = _timed(lambda: run(precision=dv.BKH_FP32_IR), 1)
+    X32 = dx.numpy().reshape(65, n)
+    err = max(_rel(X32[q], X64[q]) for q in range(65))
     return {"workload": f"shifted BiCGSTAB(4), convection-diffusion {m}^3, n={n}, 64 shifts, tol 1e-8",
             "fp64_ms": round(ms, 2), "outer_iters": info.ret, "max_true_res": float(tr.max()),
             "fp32_ir": {"ms": round(ms32, 2), "outer_iters": info32.ret, "max_true_res": float(tr32.max()),
-                        "speedup_vs_fp64": round(ms / ms32, 3)}}
+                        "speedup_vs_fp64": round(ms / ms32, 3), "max_x_rel_err_vs_fp64": err}}
 
 
 def bench_complex(dv, gen, po, m=96, ns=16):
@@ -212,7 +250,11 @@ def bench_cfg4(dv, gen):
 
     run(precision=dv.BKH_FP32_IR)
     (i32, r32, t32), ms32 = _timed(lambda: run(precision=dv.BKH_FP32_IR), 2)
+    x32 = dx.numpy()
     (i64, r64, t64), ms64 = _timed(lambda: run(), 2)
+    x64 = dx.numpy()
+    off0 = Bt.offsets
+    err32 = max(_rel(x32[off0[q]:off0[q + 1]], x64[off0[q]:off0[q + 1]]) for q in range(len(sizes)))
     # the plain FP64 path: one bkh_bicgstabl per system, in sequence
     mats, bs, xs = [], [], []
     off = Bt.offsets
@@ -236,6 +278,7 @@ def bench_cfg4(dv, gen):
             "sequential_fp64_ms": round(ms_seq, 1), "sequential_max_true_res": worst_seq,
             "batched_fp64_ms": round(ms64, 1), "batched_fp64_max_true_res": float(t64.max()),
             "batched_fp32_ir_ms": round(ms32, 1), "batched_fp32_ir_max_true_res": float(t32.max()),
+            "batched_fp32_ir_max_x_rel_err_vs_fp64": err32,
             "fp32_ir_passes": i32.passes, "converged": int((r32 >= 0).sum()),
             "speedup_vs_sequential_fp64": round(ms_seq / ms32, 2),
             "speedup_vs_batched_fp64": round(ms64 / ms32, 3),
@@ -243,11 +286,12 @@ def bench_cfg4(dv, gen):
                                  "tile_rows": layout[2], "fp32_spmv_ms": round(tune_ms, 3)}}
 
 
-def bench_cfg5(dv, gen, n_train=200, m_train=64, n_test=8, m_test=128, n_plain=2, epochs=4):
+def bench_cfg5(dv, gen, n_train=200, m_train=64, n_test=50, m_test=128, n_plain=2, epochs=4):
     """Learned-FSAI PCG on variable-coefficient 3D diffusion (kappa = exp(GRF), sigma 2,
     correlation 8 cells): train the edge MLP on the device over n_train seeded fields
-    (m_train^3 grids, same statistics), then solve held-out m_test^3 fields with
-    Jacobi PCG and learned-FSAI PCG (and unpreconditioned CG on n_plain of them)."""
+    (m_train^3 grids, same statistics), then solve the n_test held-out m_test^3 fields
+    with Jacobi PCG, the untrained network's factor and the trained network's factor
+    (and unpreconditioned CG on n_plain of them)."""
     from concurrent.futures import ThreadPoolExecutor
     from bksp_amd import fsai
     t0 = time.perf_counter()
@@ -267,9 +311,17 @@ def bench_cfg5(dv, gen, n_train=200, m_train=64, n_test=8, m_test=128, n_plain=2
     edges = fields[0].nnzl
     for f in fields:
         f.destroy()
+    untrained = dv.FsaiModel(fsai.init_params())
     rows = []
-    for q in range(n_test):
-        rows.append(fsai.solve_compare(model, 6000 + q, m_test, plain=q < n_plain))
+    with ThreadPoolExecutor(8) as ex:                  # held-out fields generated ahead on host threads
+        futs = [ex.submit(fsai.field_system, 6000 + q, m_test) for q in range(n_test)]
+        for q in range(n_test):
+            system = futs[q].result()
+            futs[q] = None
+            rows.append(fsai.solve_compare(model, 6000 + q, m_test, plain=q < n_plain, system=system,
+                                           untrained=untrained))
+            del system
+    ui = np.array([r["untrained"]["iters"] for r in rows]); ums = np.array([r["untrained"]["ms"] for r in rows])
     jac = np.array([r["jacobi"]["ms"] for r in rows]); lea = np.array([r["learned"]["ms"] for r in rows])
     ji = np.array([r["jacobi"]["iters"] for r in rows]); li = np.array([r["learned"]["iters"] for r in rows])
     plain = [r["plain_cg"] for r in rows if "plain_cg" in r]
@@ -291,6 +343,10 @@ def bench_cfg5(dv, gen, n_train=200, m_train=64, n_test=8, m_test=128, n_plain=2
             "plain_cg": {"fields": len(plain), "mean_ms": round(pl_ms, 1) if pl_ms else None,
                          "mean_iters": float(np.mean([p["iters"] for p in plain])) if plain else None,
                          "max_true_res": max(p["true_res"] for p in plain) if plain else None},
+            "untrained": {"mean_iters": float(ui.mean()), "mean_ms": round(float(ums.mean()), 2),
+                          "max_true_res": max(r["untrained"]["true_res"] for r in rows)},
+            "learned_iters_vs_untrained": round(float(li.mean() / ui.mean()), 3),
+            "speedup_learned_vs_untrained": round(float(ums.mean() / lea.mean()), 3),
             "learned_iters_vs_jacobi": round(float(li.mean() / ji.mean()), 3),
             "speedup_learned_vs_jacobi": round(float(jac.mean() / lea.mean()), 3),
             "speedup_learned_vs_plain_fp64_cg": round(pl_ms / lea_p, 2) if plain else None}
@@ -340,6 +396,7 @@ def main():
     wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
     info = infos[-1]
     dev_ms = float(np.mean([i.time_ms for i in infos]))
+    x64 = dx.numpy()
 
     # FP32-inner / FP64-refinement path on the same system (secondary line): the
     # median of at least 5 solves (its reliable updates wait on the host 4-5 times per
@@ -352,6 +409,28 @@ def main():
         ir_each.append((time.perf_counter() - t1) * 1e3)
     ir_wall = float(np.median(ir_each))
     fp64_median = float(np.median(each))
+    x32 = dx.numpy()
+
+    # the bksp_dcg drop-in on host arrays: the first call uploads A and builds its device
+    # layout (expansion of the stored triangle, SELL-64 and slice-diagonal copies); later
+    # calls with the same dspmat hit the per-thread device-copy cache (contents hashed)
+    # and pay the hash, the b / x transfers and the solve
+    from bksp_amd import bksp
+    Ah = bksp.dspmat(n, rp, ci, v, symmetric=True)
+    xh = np.zeros(n)
+    t1 = time.perf_counter()
+    dret = bksp.bksp_dcg(Ah, b, tol, 20000, xh)
+    drop_first = (time.perf_counter() - t1) * 1e3
+    drop_each = []
+    for _ in range(3):
+        xh[:] = 0.0
+        t1 = time.perf_counter()
+        bksp.bksp_dcg(Ah, b, tol, 20000, xh)
+        drop_each.append((time.perf_counter() - t1) * 1e3)
+    dropin = {"first_call_ms": round(drop_first, 2), "cached_call_ms": round(float(np.median(drop_each)), 2),
+              "ret": dret, "x_equals_device_path": bool(np.array_equal(xh, x64)),
+              "note": "host arrays in and out (PCIe-inclusive); not value"}
+    del Ah, xh
 
     # dominant kernel of the timed solve: CG's fused kernel A (SELL SpMV with the
     # direction update p = r + beta p in its gather), timed with HIP events on its
@@ -368,6 +447,24 @@ def main():
         cga_fmt = M.spmv_bytes() - 16 * n + 32 * n
         cga_kernel = "k_cgf_a_sell<double,double,short,1024> (CG: SELL-64 SpMV with p = r + beta p in the gather)"
     cga_gbs = cga_alg / (cga_ms * 1e-3) / 1e9
+    # the same kernel on a matrix whose working set exceeds the 256 MB Infinity Cache
+    # (192^3 Poisson: 395 MB of values alone), so repeated launches stream from HBM
+    nb_, rpb, cib, vb = gen.poisson3d(192, lower=True)
+    Mb = dv.Matrix(nb_, rpb, cib, vb, symmetric=True)
+    del rpb, cib, vb
+    Mb.bench_cg_kernel(reps=5)
+    big_ms = Mb.bench_cg_kernel(reps=50)
+    big_alg = 12 * int(Mb.nnz) + 4 * (nb_ + 1) + 32 * nb_
+    big_fmt = (8 * Mb.dia_entries + 4 * (Mb.dia_entries // 64) + 4 * ((nb_ + 63) // 64 + 1) + 32 * nb_
+               if Mb.dia_entries > 0 else None)
+    beyond = {"workload": f"Poisson 192^3, n={nb_}, nnz={int(Mb.nnz)} (working set > 256 MB Infinity Cache)",
+              "avg_launch_us": round(big_ms * 1e3, 2), "algorithmic_bytes": big_alg,
+              "achieved": round(big_alg / (big_ms * 1e-3) / 1e9, 1),
+              "frac": round(big_alg / (big_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+              "format_bytes": big_fmt,
+              "format_frac": round(big_fmt / (big_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if big_fmt else None}
+    Mb.destroy()
+    del Mb
     dy = dv.DeviceArray((n,))
     dv.bench_spmv(M, db, dy, reps=5)
     spmv_ms = dv.bench_spmv(M, db, dy, reps=50)
@@ -399,6 +496,8 @@ def main():
                          "stores, traffic the HBM bytes rocprof counts per launch")
                 if M.dia_entries > 0 else None,
                 "measured_copy_GBs": round(copy_gbs, 1),
+                "frac_of_measured_copy": round(cga_gbs / copy_gbs, 4),
+                "beyond_cache": beyond,
                 "spmv": {"kernel": "k_spmv_sell<double,double,short> (SELL-64, 16-bit column deltas)",
                          "achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
                          "algorithmic_bytes": bytes64, "avg_launch_us": round(spmv_ms * 1e3, 2),
@@ -412,15 +511,23 @@ def main():
     cpu = None
     if not args.no_cpu:
         from oracle import pyoracle as po
-        k = 60
+        model, threads, avail = _host_info()
         t0 = po.wall()
-        po.cg(n, rp, ci, v, b, 1e-300, k)
-        per_it = (po.wall() - t0) / k
-        iters = info.ret + 1
-        cpu = {"value": round(per_it * iters * 1e3, 1), "unit": "ms", "cores": 1, "kind": "port",
-               "sample": f"{k} iterations of the oracle's CG (C restatement of src/cg.c, gcc -O3, "
-                         f"1 thread) on the same system, {per_it * 1e3:.1f} ms/iteration, "
-                         f"scaled to the {iters} iterations the solve takes"}
+        cret, cx = po.cg(n, rp, ci, v, b, tol, 20000)
+        cpu_s = po.wall() - t0
+        cpu = {"value": round(cpu_s * 1e3, 1), "unit": "ms", "cores": 1, "kind": "port",
+               "sample": f"one complete solve of the same system by the oracle's CG (C restatement of "
+                         f"src/cg.c, gcc -O3, one thread), {cret} iterations to 1e-8",
+               "cpu_model": model, "host_cores_available": avail,
+               "x_rel_diff_vs_gpu": _rel(x64, cx), "iterations": cret}
+        if po.lib_mt() is not None:
+            os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+            t0 = po.wall()
+            mret, _ = po.cg_mt(n, rp, ci, v, b, tol, 20000)
+            cpu["threaded"] = {"value": round((po.wall() - t0) * 1e3, 1), "unit": "ms", "cores": threads,
+                               "iterations": mret,
+                               "sample": "the same solve with the BLAS-1 restatements threaded (OpenMP) and "
+                                         "the SpMV serial, as the reference runs with a multithreaded BLAS"}
 
     extra = {}
     want = [c for c in args.configs.split(",") if c]
@@ -445,14 +552,18 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall_ms, 3),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"CG (bksp_dcg) to 1e-8 on 3D 7-point Poisson {m}^3, n={n}, "
+        "config": {"workload": f"CG to 1e-8 on 3D 7-point Poisson {m}^3 (bkh_cg on HBM-resident inputs; "
+                               f"bksp_dcg drop-in in 'dropin'), n={n}, "
                                f"nnz={nnz} expanded ({int(rp[-1])} stored lower), RHS N(0,1) seed 202, x0=0",
-                   "n": n, "nnz": nnz, "tol": tol},
+                   "n": n, "nnz": nnz, "tol": tol,
+                   "inputs_sha256": _sha256(rp, ci, v, b)},
         "iterations": info.ret, "true_res": info.true_res, "device_ms": round(dev_ms, 3),
         "fp64_median_ms": round(fp64_median, 3),
         "fp32_ir": {"ms": round(ir_wall, 3), "mean_ms": round(float(np.mean(ir_each)), 3), "solves": len(ir_each),
                     "inner_iters": ir[-1].inner_iters, "passes": ir[-1].passes,
-                    "true_res": ir[-1].true_res, "speedup_vs_fp64": round(fp64_median / ir_wall, 3)},
+                    "true_res": ir[-1].true_res, "speedup_vs_fp64": round(fp64_median / ir_wall, 3),
+                    "x_rel_err_vs_fp64": _rel(x32, x64)},
+        "dropin": dropin,
         "roofline": roofline, "cpu_baseline": cpu,
         "configs": extra,
     }

@@ -32,16 +32,105 @@ typedef struct {
     bkh_matrix *M;
     double *d_b, *d_x;
     size_t n_b, n_x;
+    int cached;            /* M belongs to the matrix cache; b and x to the buffer cache */
 } upload_t;
+
+/*
+ * Device-copy cache. The reference builds nothing per matrix, so a caller that solves
+ * repeatedly with one dspmat pays the upload and device layout (expansion of a stored
+ * triangle, SELL-64 and slice-diagonal copies) on every call unless it is kept. Each
+ * thread keeps its last MCACHE_N matrices, keyed by the dspmat's array pointers, n,
+ * nnz, the creation flags and a 64-bit hash of row, col and value contents, so a
+ * caller that rewrites values (or reuses the arrays for another matrix) never hits a
+ * stale copy. The b / x device buffers are kept per thread too (grow-only), so the
+ * solvers' cached iteration graphs, keyed by buffer addresses, are replayed as well.
+ * BKSP_CACHE=0 in the environment turns both off (upload and free on every call, as
+ * the reference mallocs and frees its workspace per call).
+ */
+#define MCACHE_N 2
+typedef struct {
+    const int *row, *col;
+    const double *val;
+    int n, nnz, triangle, want_T, want_fp32;
+    unsigned long long hash, stamp;
+    bkh_matrix *M;
+} mcache_ent;
+static __thread mcache_ent g_mcache[MCACHE_N];
+static __thread unsigned long long g_mstamp;
+static __thread double *g_bbuf, *g_xbuf;
+static __thread size_t g_bcap, g_xcap;
+
+static int cache_on(void) {
+    const char *e = getenv("BKSP_CACHE");
+    return !(e && e[0] == '0');
+}
+
+/* 64-bit multiply-xorshift hash of 8-byte words (contents, not addresses). */
+static unsigned long long hash_bytes(unsigned long long h, const void *p, size_t bytes) {
+    const unsigned char *c = (const unsigned char *)p;
+    size_t i = 0;
+    for (; i + 8 <= bytes; i += 8) {
+        unsigned long long w;
+        memcpy(&w, c + i, 8);
+        h = (h ^ w) * 0x9E3779B97F4A7C15ULL;
+        h ^= h >> 29;
+    }
+    for (; i < bytes; i++) h = (h ^ c[i]) * 0x100000001B3ULL;
+    return h;
+}
+
+static bkh_matrix *cached_matrix(dspmat *A, int triangle, int want_T, int want_fp32, int *is_cached) {
+    const int n = *A->row_size, nnz = A->row[n];
+    *is_cached = 0;
+    if (!cache_on()) return bkh_matrix_create(n, A->row, A->col, A->value, triangle, want_T, want_fp32);
+    unsigned long long h = hash_bytes(0xCBF29CE484222325ULL, A->row, sizeof(int) * ((size_t)n + 1));
+    h = hash_bytes(h, A->col, sizeof(int) * (size_t)nnz);
+    h = hash_bytes(h, A->value, sizeof(double) * (size_t)nnz);
+    int victim = 0;
+    for (int k = 0; k < MCACHE_N; k++) {
+        mcache_ent *e = &g_mcache[k];
+        if (e->M && e->row == A->row && e->col == A->col && e->val == A->value && e->n == n && e->nnz == nnz &&
+            e->triangle == triangle && e->want_T >= want_T && e->want_fp32 >= want_fp32 && e->hash == h) {
+            e->stamp = ++g_mstamp;
+            *is_cached = 1;
+            return e->M;
+        }
+    }
+    for (int k = 1; k < MCACHE_N; k++)        /* an empty slot, else the least recently used */
+        if (g_mcache[victim].M && (!g_mcache[k].M || g_mcache[k].stamp < g_mcache[victim].stamp)) victim = k;
+    bkh_matrix *M = bkh_matrix_create(n, A->row, A->col, A->value, triangle, want_T, want_fp32);
+    if (!M) return NULL;
+    mcache_ent *e = &g_mcache[victim];
+    if (e->M) bkh_matrix_destroy(e->M);
+    e->row = A->row; e->col = A->col; e->val = A->value;
+    e->n = n; e->nnz = nnz; e->triangle = triangle; e->want_T = want_T; e->want_fp32 = want_fp32;
+    e->hash = h; e->stamp = ++g_mstamp; e->M = M;
+    *is_cached = 1;
+    return M;
+}
+
+static double *grow(double **buf, size_t *cap, size_t n) {
+    if (*cap >= n) return *buf;
+    if (*buf) bkh_free(*buf);
+    *buf = (double *)bkh_malloc(sizeof(double) * n);
+    *cap = *buf ? n : 0;
+    return *buf;
+}
 
 static int upload_general(dspmat *A, const double *b, size_t nb, const double *x, size_t nx,
                           int triangle, int want_T, int want_fp32, upload_t *u) {
     memset(u, 0, sizeof *u);
-    int n = *A->row_size;
-    u->M = bkh_matrix_create(n, A->row, A->col, A->value, triangle, want_T, want_fp32);
+    int cached = 0;
+    u->M = cached_matrix(A, triangle, want_T, want_fp32, &cached);
     if (!u->M) return BKH_DEVICE_ERROR;
-    u->d_b = (double *)bkh_malloc(sizeof(double) * nb);
-    u->d_x = (double *)bkh_malloc(sizeof(double) * nx);
+    u->cached = cached;
+    if (cached) {
+        u->d_b = grow(&g_bbuf, &g_bcap, nb);
+        u->d_x = grow(&g_xbuf, &g_xcap, nx);
+    } else {
+        u->d_b = (double *)bkh_malloc(sizeof(double) * nb);
+        u->d_x = (double *)bkh_malloc(sizeof(double) * nx);
+    }
     if (!u->d_b || !u->d_x) return BKH_DEVICE_ERROR;
     u->n_b = nb;
     u->n_x = nx;
@@ -67,6 +156,7 @@ static int download_x(upload_t *u, double *x) {
 }
 
 static void release(upload_t *u) {
+    if (u->cached) return;                  /* kept for the next call on this thread */
     bkh_free(u->d_b);
     bkh_free(u->d_x);
     if (u->M) bkh_matrix_destroy(u->M);

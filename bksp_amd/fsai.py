@@ -85,9 +85,12 @@ def train(model, fields, epochs, lr=3e-3, lr_final=3e-4, log_every=0):
     return hist
 
 
-def solve_compare(model, seed, m, tol=1e-8, max_iter=200000, b_seed=gen.SEEDS["cfg5_rhs"], plain=True):
-    """Unpreconditioned CG, Jacobi PCG and learned-FSAI PCG on one held-out field."""
-    n, lower = field_system(seed, m)
+def solve_compare(model, seed, m, tol=1e-8, max_iter=200000, b_seed=gen.SEEDS["cfg5_rhs"], plain=True,
+                  system=None, untrained=None):
+    """Unpreconditioned CG, Jacobi PCG and learned-FSAI PCG on one held-out field
+    (`system`: the field's (n, lower CSR) if already generated; `untrained`: a second
+    model, e.g. the initial weights, whose factor is solved with as well)."""
+    n, lower = system if system is not None else field_system(seed, m)
     F = dv.FsaiField(n, *lower)
     b = gen.normal(b_seed, n)
     db, dx = dv.DeviceArray.from_numpy(b), dv.DeviceArray((n,))
@@ -100,6 +103,9 @@ def solve_compare(model, seed, m, tol=1e-8, max_iter=200000, b_seed=gen.SEEDS["c
         out[name] = {"iters": info.ret, "ms": info.time_ms, "true_res": info.true_res}
 
     build = {}
+    if untrained is not None:
+        F.factor(untrained)
+        run("untrained", lambda: F.pcg(db, dx, tol, max_iter))
     F.factor(model)
     build["ms"] = F.bench_forward(model, reps=3)
     run("jacobi", lambda: F.pcg_plain(1, db, dx, tol, max_iter))

@@ -18,6 +18,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--m", type=int, default=128)
     ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=0, help="untimed launches first (bench.py uses 5)")
     ap.add_argument("--fp32", action="store_true")
     ap.add_argument("--kernel", choices=["spmv", "cg"], default="spmv")
     args = ap.parse_args()
@@ -27,8 +28,12 @@ def main():
     dx = dv.DeviceArray.from_numpy(gen.normal(202, n).astype(dt))
     dy = dv.DeviceArray((n,), dt)
     if args.kernel == "cg":
+        if args.warmup:
+            M.bench_cg_kernel(reps=args.warmup, fp32=args.fp32)
         ms = M.bench_cg_kernel(reps=args.reps, fp32=args.fp32)
     else:
+        if args.warmup:
+            dv.bench_spmv(M, dx, dy, reps=args.warmup, fp32=args.fp32)
         ms = dv.bench_spmv(M, dx, dy, reps=args.reps, fp32=args.fp32)
     nbytes = 1 << 28                                   # 256 MiB copy: 256 MiB read + 256 MiB written
     a, b = dv.DeviceArray((nbytes // 8,)), dv.DeviceArray((nbytes // 8,))

@@ -13,6 +13,12 @@
  *                              (LAPACK's dtrtri route differs only in rounding).
  * One deliberate convention: dscal with a == 0 writes exact zeros (the reference
  * relies on it to clear malloc'd buffers, bksp_internal.h:39 and :73).
+ *
+ * Built with -DORC_THREADS -fopenmp (liboracle_mt.so) the vector routines run on all
+ * host cores, as a multithreaded BLAS runs the reference's BLAS-1 calls, while the
+ * CSR kernels stay serial as the reference's matvec.c does (its Makefile's -openmp is
+ * parsed as an output-file option); the reduction order then depends on the thread
+ * count. That build is only a timing baseline; the checker is the serial build.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -20,27 +26,62 @@
 #include <time.h>
 #include "oracle.h"
 
+#ifdef ORC_THREADS
+#define ORC_PAR _Pragma("omp parallel for schedule(static) if (n > 32768)")
+#define ORC_PAR_SUM _Pragma("omp parallel for schedule(static) reduction(+ : s) if (n > 32768)")
+#else
+#define ORC_PAR
+#define ORC_PAR_SUM
+#endif
+
 double orc_ddot(int n, const double *x, const double *y) {
     double s = 0.0;
+    ORC_PAR_SUM
     for (int i = 0; i < n; i++) s += x[i] * y[i];
     return s;
 }
 
 void orc_daxpy(int n, double a, const double *x, double *y) {
     if (a == 0.0) return;                 /* Netlib daxpy quick return */
+    ORC_PAR
     for (int i = 0; i < n; i++) y[i] += a * x[i];
 }
 
 void orc_dscal(int n, double a, double *x) {
-    if (a == 0.0) { for (int i = 0; i < n; i++) x[i] = 0.0; return; }
+    if (a == 0.0) {
+        ORC_PAR
+        for (int i = 0; i < n; i++) x[i] = 0.0;
+        return;
+    }
+    ORC_PAR
     for (int i = 0; i < n; i++) x[i] = a * x[i];
 }
 
 void orc_dcopy(int n, const double *x, double *y) {
+#ifdef ORC_THREADS
+    if (x + n <= y || y + n <= x) {
+        ORC_PAR
+        for (int i = 0; i < n; i++) y[i] = x[i];
+        return;
+    }
+#endif
     memmove(y, x, (size_t)n * sizeof(double));
 }
 
 double orc_dnrm2(int n, const double *x) {
+#ifdef ORC_THREADS
+    if (n > 32768) {                      /* max-scaled two-pass norm */
+        double mx = 0.0;
+        _Pragma("omp parallel for schedule(static) reduction(max : mx)")
+        for (int i = 0; i < n; i++) mx = fmax(mx, fabs(x[i]));
+        if (mx == 0.0) return 0.0;
+        double s = 0.0;
+        const double inv = 1.0 / mx;
+        _Pragma("omp parallel for schedule(static) reduction(+ : s)")
+        for (int i = 0; i < n; i++) s += (x[i] * inv) * (x[i] * inv);
+        return mx * sqrt(s);
+    }
+#endif
     if (n < 1) return 0.0;
     if (n == 1) return fabs(x[0]);
     double scale = 0.0, ssq = 1.0;

@@ -41,6 +41,27 @@ def lib():
     return _o
 
 
+def lib_mt():
+    """The timing build (liboracle_mt.so: BLAS-1 threaded with OpenMP, SpMV serial), or
+    None when it was not built. Never used as the checker."""
+    p = os.path.join(HERE, "liboracle_mt.so")
+    if not os.path.exists(p):
+        return None
+    L = ctypes.CDLL(p)
+    L.orc_wall_seconds.restype = c_dbl
+    return L
+
+
+def cg_mt(n, rp, ci, v, b, tol, max_iter):
+    """CG of the threaded timing build (same algorithm, reduction order per thread count)."""
+    L = lib_mt()
+    rp, ci, v = _csr(rp, ci, v)
+    x = np.zeros(n)
+    ret = L.orc_cg(n, _ip(rp), _ip(ci), _dp(v), _dp(np.ascontiguousarray(b, np.float64)), c_dbl(tol), max_iter,
+                   _dp(x))
+    return ret, x
+
+
 def ref():
     """The reference's compiled kernels, or None when they were not built."""
     global _r

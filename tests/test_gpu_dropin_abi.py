@@ -7,7 +7,7 @@ import os
 import numpy as np
 import pytest
 
-from bksp_amd import load
+from bksp_amd import gen, load
 from bksp_amd._lib import DMat
 from conftest import GOLDEN, read_mtx
 from oracle import pyoracle as po
@@ -142,3 +142,32 @@ def test_dhorner_comb_and_dmat():
     M = load().dmat_create(4, 3).contents
     assert M.row_size == 4 and M.col_size == 3
     assert all(M.value[i] == 0.0 for i in range(12))
+
+
+def test_dropin_device_copy_cache(monkeypatch):
+    """bksp_dcg keeps the device copy of A per thread, keyed by the dspmat's arrays and
+    a hash of their contents: repeated calls give the same x as an uncached call, and a
+    caller that rewrites values in place gets a solve of the new matrix."""
+    from bksp_amd import bksp
+    n, rp, ci, v = gen.poisson3d(24, lower=True)
+    v = v.copy()
+    b = gen.normal(3, n)
+    A = bksp.dspmat(n, rp, ci, v, symmetric=True)
+    xs = []
+    for _ in range(3):
+        x = np.zeros(n)
+        assert bksp.bksp_dcg(A, b, 1e-10, 2000, x) >= 0
+        xs.append(x)
+    monkeypatch.setenv("BKSP_CACHE", "0")
+    x0 = np.zeros(n)
+    assert bksp.bksp_dcg(A, b, 1e-10, 2000, x0) >= 0
+    monkeypatch.delenv("BKSP_CACHE")
+    assert np.array_equal(xs[0], xs[1]) and np.array_equal(xs[1], xs[2]) and np.array_equal(xs[0], x0)
+    # rewrite the diagonal in place (the same arrays, new contents): the cache must miss
+    diag = np.array([k for i in range(n) for k in range(rp[i], rp[i + 1]) if ci[k] == i])
+    A.val[diag] *= 2.0
+    x = np.zeros(n)
+    ret = bksp.bksp_dcg(A, b, 1e-10, 2000, x)
+    cret, cx = po.cg(n, rp, ci, A.val, b, 1e-10, 2000)
+    assert ret >= 0 and abs(ret - cret) <= max(1, int(0.05 * cret)), (ret, cret)
+    assert np.linalg.norm(x - cx) / np.linalg.norm(cx) < 1e-8
