@@ -493,3 +493,135 @@ __global__ __launch_bounds__(256) void k_spmm_sell_v(int n, const int *__restric
     for (int j = 0; j < CPL; j++) out[j] = acc[j];
     __builtin_nontemporal_store(out, (tvec *)(Y + (size_t)row * S + sub * CPL));
 }
+
+// AP = A P with the Gram matrix P^T AP in the same pass (block CG, src/blcg.c:74-79):
+// the SpMM of k_spmm_sell_v (LPR lanes per row, alpha = 1, beta = 0, each row summed
+// in stored order, so AP is bitwise the plain SpMM's) over a contiguous range of
+// slices per workgroup, and while a wave's rows of P and AP are still in registers
+// they go through a wave-private LDS tile into v_mfma_f64_16x16x4f64: A operand
+// P^T (lane L: P[row 4q + L/16][L%16]), B operand AP (lane L: AP[row 4q + L/16][L%16]),
+// columns >= S zero. The 16 x 16 accumulator holds G[a][b] = sum P_a AP_b in lane
+// L register r at a = L/16 + 4r, b = L%16; one G (column-major S x S) per workgroup
+// goes to part. This replaces the separate Gram pass, which re-read P and AP (16 m S
+// bytes) right after the SpMM wrote AP. T = float: products in FP32 as the plain
+// SpMM, the Gram of the stored FP32 values accumulated in FP64.
+template <int S, typename VT, typename T, typename IT, bool DIA>
+__device__ __forceinline__ void spmm_gram_row(int n, int sl, int row, int sub, bool live, const int *__restrict__ rp,
+                                              const int *__restrict__ so, const VT *__restrict__ val,
+                                              const IT *__restrict__ col, int &len, int &base, const int *&dof) {
+    len = 0;
+    base = 0;
+    dof = nullptr;
+    if (!live) return;
+    if constexpr (DIA) {
+        const int s0 = so[sl];
+        len = (so[sl + 1] - s0) >> 6;
+        base = s0 + (row & 63);
+        dof = (const int *)col + (s0 >> 6);
+    } else {
+        len = rp[row + 1] - rp[row];
+        base = so[row >> 6] + (row & 63);
+    }
+}
+
+// UNR rows per thread (from slices SPB apart) are multiplied together, so a wave has
+// UNR rows' gathers in flight between two MFMA steps.
+// (8 waves per SIMD: two 1024-thread workgroups per CU; the FP32 variants would
+// otherwise take 66 registers and fit one.)
+template <int S, typename VT, typename T, typename IT, bool DIA, int BS, int UNR>
+__global__ __launch_bounds__(BS, S <= 8 ? 8 : 4) void k_spmm_gram(int n, int nsl, int chunk, const int *__restrict__ rp,
+                                                  const int *__restrict__ so, const VT *__restrict__ val,
+                                                  const IT *__restrict__ col, const T *X, T *Y, const Ctl *ctl,
+                                                  double *__restrict__ part) {
+    constexpr int LPR = spmm_lpr<S, T>();
+    static_assert(LPR > 0 && S <= 16, "k_spmm_gram needs S * sizeof(T) in {16, 32, 64, 128} and S <= 16");
+    constexpr int CPL = S / LPR, RW = 64 / LPR, NWV = BS / 64;   // RW: rows per wave and row slot
+    constexpr int SPB = BS / (64 * LPR);                           // slices per row slot of the workgroup
+    static_assert(SPB >= 1 && BS % (64 * LPR) == 0, "a pass must cover whole slices");
+    typedef T tvec __attribute__((ext_vector_type(CPL)));
+    if (ctl && ctl_stopped(ctl)) return;
+    __shared__ T tp[NWV][UNR * RW * S], ta[NWV][UNR * RW * S];
+    __shared__ double red[NWV][S * S];
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int sub = tid % LPR, rp_ = tid / LPR;                 // row within the slot
+    const int rw = (tid % 64) / LPR;                             // row within the wave's tile
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int q0 = lb * chunk, q1 = min(nsl, q0 + chunk);
+    mf4 g = {0.0, 0.0, 0.0, 0.0};
+    for (int qb = q0; qb < q1; qb += UNR * SPB) {
+        int row[UNR], len[UNR], base[UNR];
+        const int *dof[UNR];
+        bool live[UNR];
+        T acc[UNR][CPL];
+        int L = 0;
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const int sl = qb + u * SPB + rp_ / 64;              // uniform per wave
+            row[u] = sl * 64 + (rp_ & 63);
+            live[u] = sl < q1 && row[u] < n;
+            spmm_gram_row<S, VT, T, IT, DIA>(n, sl, row[u], sub, live[u], rp, so, val, col, len[u], base[u], dof[u]);
+            L = max(L, len[u]);
+#pragma unroll
+            for (int j = 0; j < CPL; j++) acc[u][j] = (T)0;
+        }
+        for (int kk = 0; kk < L; kk++) {
+#pragma unroll
+            for (int u = 0; u < UNR; u++) {
+                if (kk < len[u]) {
+                    const int e = base[u] + 64 * kk;
+                    int c;
+                    if constexpr (DIA) {
+                        c = min(max(row[u] + dof[u][kk], 0), n - 1);
+                    } else {
+                        const IT ce = __builtin_nontemporal_load(col + e);
+                        c = sizeof(IT) == 2 ? row[u] + (int)ce : (int)ce;
+                    }
+                    const T a = (T)__builtin_nontemporal_load(val + e);
+                    const tvec xv = *(const tvec *)(X + (size_t)c * S + sub * CPL);
+#pragma unroll
+                    for (int j = 0; j < CPL; j++) acc[u][j] = add_rn(acc[u][j], mul_rn(a, xv[j]));
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            tvec pv;
+#pragma unroll
+            for (int j = 0; j < CPL; j++) pv[j] = (T)0;
+            if (live[u]) {
+                tvec out;
+#pragma unroll
+                for (int j = 0; j < CPL; j++) out[j] = acc[u][j];
+                __builtin_nontemporal_store(out, (tvec *)(Y + (size_t)row[u] * S + sub * CPL));
+                pv = *(const tvec *)(X + (size_t)row[u] * S + sub * CPL);   // P's own row (just gathered: L2)
+            }
+#pragma unroll
+            for (int j = 0; j < CPL; j++) {
+                tp[w][(u * RW + rw) * S + sub * CPL + j] = pv[j];
+                ta[w][(u * RW + rw) * S + sub * CPL + j] = acc[u][j];
+            }
+        }
+        wave_lds_sync();
+        const int a = lane & 15;
+#pragma unroll
+        for (int q = 0; q < UNR * RW / 4; q++) {
+            const int r = 4 * q + (lane >> 4);
+            const double av = a < S ? (double)tp[w][r * S + a] : 0.0;
+            const double bv = a < S ? (double)ta[w][r * S + a] : 0.0;
+            g = mfma16x16x4(av, bv, g);
+        }
+        wave_lds_sync();
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int a = (lane >> 4) + 4 * r, b = lane & 15;
+        if (a < S && b < S) red[w][a + b * S] = g[r];
+    }
+    __syncthreads();
+    for (int t = tid; t < S * S; t += BS) {
+        double v = red[0][t];
+#pragma unroll
+        for (int k = 1; k < NWV; k++) v += red[k][t];
+        part[(size_t)lb * S * S + t] = v;
+    }
+}

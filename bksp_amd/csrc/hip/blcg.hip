@@ -35,8 +35,11 @@ __global__ __launch_bounds__(256) void k_sh_fill_f(float *p, long long m) {
     for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < m; e += (long long)gridDim.x * 256) p[e] = 0.0f;
 }
 
-// Partial rows the fin kernel sums itself; more go through k_part_reduce first.
+// Partial rows the fin kernel sums itself; more go through k_part_reduce first. The
+// 1024-thread fin kernels (s <= 8) take up to FIN_DIRECT_WIDE rows (16 groups of
+// threads, 32 rows each at most).
 #define FIN_DIRECT 256
+#define FIN_DIRECT_WIDE 512
 
 // Fixed-order sum of nblocks rows of E partials by a BS-thread workgroup: thread
 // t < Q*E (Q = BS/E groups) sums rows q, q+Q, q+2Q, ... of entry e = t % E, q = t / E,
@@ -151,7 +154,7 @@ static void launch_blk_fin(hipStream_t s, const double *part, int nblocks, BlkMa
     // (block CG s = 8 at 128^3: 349.7 -> 346.6 ms FP64, 193.3 -> 190.1 ms FP32); larger s
     // keep 256 (their 64 partial loads per thread would not fit 128 registers)
     constexpr int FBS = S <= 8 ? 1024 : 256;
-    if (nblocks <= FIN_DIRECT) {
+    if (nblocks <= (S <= 8 ? FIN_DIRECT_WIDE : FIN_DIRECT)) {
         hipLaunchKernelGGL((k_blk_fin<S, OP, FBS>), dim3(1), dim3(FBS), 0, s, part, nblocks, mt, c);
         return;
     }
@@ -863,6 +866,49 @@ static void launch_spmm(hipStream_t s, const bkh_matrix *A, const VT *val, const
                        X, Y, alpha, beta, c);
 }
 
+// AP = A P with the Gram P^T AP fused on MFMA (k_spmm_gram) when the matrix has a
+// SELL-64 copy in the values' precision and the blocks allow 16-byte rows: returns the
+// partial rows written (at most FIN_DIRECT), or 0 when the caller must run the SpMM
+// and the Gram pass separately. BKH_GRAM_FUSE=0 in the environment keeps them apart.
+template <int S, typename VT, typename T>
+static int launch_spmm_gram(hipStream_t s, const bkh_matrix *A, const VT *val, const T *P, T *AP, const Ctl *c,
+                            double *part) {
+    constexpr int LPR = spmm_lpr<S, T>();
+    if constexpr (LPR == 0 || S > 16) {
+        return 0;
+    } else {
+        static const bool off = [] { const char *e = getenv("BKH_GRAM_FUSE"); return e && e[0] == '0'; }();
+        if (off || A->nsl == 0 || A->variant != SPMV_SELL || (((uintptr_t)P | (uintptr_t)AP) & 15)) return 0;
+        constexpr int BSG = 1024, UNR = 1, SPB = UNR * BSG / (64 * LPR);   // slices per pass
+        const int npass = (A->nsl + SPB - 1) / SPB;
+        // 512 workgroups of 16 waves (8 per SIMD): 139.8 us per FP64 s = 8 launch at 128^3
+        // against 113.4 + 47.4 us for the separate SpMM and Gram passes; 256 workgroups
+        // (4 waves per SIMD) took 179 us, 1024 of 8 waves 158 us, two slices per thread
+        // per pass 162 us
+        const int G0 = std::max(1, std::min(S <= 8 ? FIN_DIRECT_WIDE : FIN_DIRECT, npass));
+        const int chunk = ((npass + G0 - 1) / G0) * SPB;           // slices per workgroup
+        const int G = (A->nsl + chunk - 1) / chunk;
+        const VT *dv = nullptr, *sv = nullptr;
+        if constexpr (sizeof(VT) == sizeof(float)) {
+            if (val == (const VT *)A->val32) { dv = (const VT *)A->dval32; sv = (const VT *)A->sval32; }
+        } else {
+            if (val == (const VT *)A->val) { dv = (const VT *)A->dval; sv = (const VT *)A->sval; }
+        }
+        if (A->dia_entries > 0 && dv)
+            hipLaunchKernelGGL((k_spmm_gram<S, VT, T, int, true, BSG, UNR>), dim3(G), dim3(BSG), 0, s, (int)A->n, A->nsl,
+                               chunk, A->rp, A->dso, dv, (const int *)A->doff, P, AP, c, part);
+        else if (sv && A->scd)
+            hipLaunchKernelGGL((k_spmm_gram<S, VT, T, short, false, BSG, UNR>), dim3(G), dim3(BSG), 0, s, (int)A->n,
+                               A->nsl, chunk, A->rp, A->sso, sv, (const short *)A->scd, P, AP, c, part);
+        else if (sv && A->scol)
+            hipLaunchKernelGGL((k_spmm_gram<S, VT, T, int, false, BSG, UNR>), dim3(G), dim3(BSG), 0, s, (int)A->n,
+                               A->nsl, chunk, A->rp, A->sso, sv, (const int *)A->scol, P, AP, c, part);
+        else
+            return 0;
+        return G;
+    }
+}
+
 // Block CG on row-major device blocks: Bt (m x s), Xt (in/out).
 template <int S, typename VT, typename T>
 static int blcg_core(ThreadCtx *t, const bkh_matrix *A, const VT *val, const T *Bt, T *Xt, T *V,
@@ -889,8 +935,12 @@ static int blcg_core(ThreadCtx *t, const bkh_matrix *A, const VT *val, const T *
     hipGraphExec_t ge = capture_cached(t, {0xB1C6, (uintptr_t)A, (uintptr_t)val, (uintptr_t)Bt, (uintptr_t)Xt, (uintptr_t)V,
                                            (uintptr_t)P, (uintptr_t)AP, (uintptr_t)mt, (uintptr_t)c, (uintptr_t)part,
                                            (uintptr_t)S, (uintptr_t)sizeof(T)}, s, K, [&]() {
-        launch_spmm<S>(s, A, val, (const T *)P, AP, 1.0, 0.0, c, true);
-        launch_blk_fin<S, FIN_ALPHA>(s, part, panel_gram<S, T>(s, m, P, AP, c, part), mt, c);
+        int ng = launch_spmm_gram<S>(s, A, val, (const T *)P, AP, c, part);
+        if (!ng) {
+            launch_spmm<S>(s, A, val, (const T *)P, AP, 1.0, 0.0, c, true);
+            ng = panel_gram<S, T>(s, m, P, AP, c, part);
+        }
+        launch_blk_fin<S, FIN_ALPHA>(s, part, ng, mt, c);
         launch_blk_fin<S, FIN_QR1>(s, part, panel_xv<S, T>(s, m, Xt, P, V, AP, mt, c, part), mt, c);
         launch_blk_fin<S, FIN_QR2_ITER>(s, part, panel_vr<S, T>(s, m, V, mt, c, part), mt, c);
         panel_vp<S, T>(s, m, V, P, mt, 0.0, c);
@@ -1085,8 +1135,12 @@ static int blcg_ru(ThreadCtx *t, bkh_matrix *A, const double *B, double *X, doub
     hipGraphExec_t ge = capture_cached(t, {0xB1C7, (uintptr_t)A, (uintptr_t)X32, (uintptr_t)V32, (uintptr_t)P32,
                                            (uintptr_t)AP32, (uintptr_t)mt, (uintptr_t)c, (uintptr_t)part,
                                            (uintptr_t)S}, s, K, [&]() {
-        launch_spmm<S>(s, A, val, (const float *)P32, AP32, 1.0, 0.0, c, true);
-        launch_blk_fin<S, FIN_ALPHA>(s, part, panel_gram<S, float>(s, m, P32, AP32, c, part), mt, c);
+        int ng = launch_spmm_gram<S>(s, A, val, (const float *)P32, AP32, c, part);
+        if (!ng) {
+            launch_spmm<S>(s, A, val, (const float *)P32, AP32, 1.0, 0.0, c, true);
+            ng = panel_gram<S, float>(s, m, P32, AP32, c, part);
+        }
+        launch_blk_fin<S, FIN_ALPHA>(s, part, ng, mt, c);
         launch_blk_fin<S, FIN_QR1>(s, part, panel_xv<S, float>(s, m, X32, P32, V32, AP32, mt, c, part), mt, c);
         launch_blk_fin<S, FIN_QR2_ITER>(s, part, panel_vr<S, float>(s, m, V32, mt, c, part), mt, c);
         panel_vp<S, float>(s, m, V32, P32, mt, 0.0, c);

@@ -39,6 +39,8 @@
 #define FS_WAVES (FS_BLOCKS * 4)
 
 typedef short s4v __attribute__((ext_vector_type(4)));
+typedef short s8v __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 
 static const double FS_C0 = 0.54132485461291810;   // log(e - 1)
@@ -58,15 +60,13 @@ struct bkh_fsai_field {
     int *rpl = nullptr, *cil = nullptr, *erow = nullptr, *perm = nullptr;
     double *vl = nullptr;
     double4 *rows = nullptr;      // sqrt(a_ii), coupling sum, lower count, diagonal excess
+    double2 *rowx = nullptr;      // 1 / sqrt(a_ii), log(a_ii): per-row parts of the edge features
     double *Z = nullptr, *W = nullptr, *U = nullptr, *gE = nullptr;
     int P = 0;
 };
 
-__device__ __forceinline__ short f2bf(float x) {
-    unsigned u = __float_as_uint(x);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (short)(u >> 16);
-}
+// FP32 -> BF16 with round-to-nearest-even (gfx950's v_cvt_pk_bf16_f32).
+__device__ __forceinline__ short f2bf(float x) { return __builtin_bit_cast(short, (__bf16)x); }
 
 __device__ __forceinline__ float bf_round(float x) { return __uint_as_float(((unsigned)(unsigned short)f2bf(x)) << 16); }
 
@@ -74,9 +74,39 @@ __device__ __forceinline__ f4v mfma_bf16(s4v a, s4v b, f4v c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// gfx950's v_mfma_f32_16x16x32_bf16: A lane L holds A[L%16][8(L/16) .. 8(L/16)+7],
+// B lane L holds B[8(L/16) .. 8(L/16)+7][L%16], D as the 16x16x16 form.
+__device__ __forceinline__ f4v mfma_bf16_k32(s8v a, s8v b, f4v c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8v, a), __builtin_bit_cast(bf8v, b), c, 0, 0, 0);
+}
+
+// tanh in FP32 from the hardware exp2 and reciprocal: tanh x = 1 - 2 / (exp(2x) + 1),
+// and the series x - x^3/3 + 2x^5/15 below |x| = 1/16, where the subtraction would
+// cancel (series error < 17/315 x^7). Absolute error ~2e-7: below half a BF16 step of
+// every activation the matrix cores consume, and 5 to 8 instructions where the libm
+// tanhf spends about 40 (the MLP evaluates 32 per lane per 16-edge tile).
+// softplus and the logistic function in FP32 from the hardware exp2 / log2 (the factor
+// is stored in FP32 for the fused PCG and the MLP's output h is FP32 already).
+__device__ __forceinline__ float softplus_f(float x) {
+    const float e = __builtin_amdgcn_exp2f(1.4426950408889634f * x);
+    const float l = 0.69314718055994531f * __builtin_amdgcn_logf(1.0f + e);
+    return x > 20.0f ? x : (x < -15.0f ? e : l);
+}
+__device__ __forceinline__ float sigmoid_f(float x) {
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.4426950408889634f * x));
+}
+
+__device__ __forceinline__ float tanh_fast(float x) {
+    const float e = __builtin_amdgcn_exp2f(2.8853900817779268f * x);   // exp(2x) = 2^(2x / ln 2)
+    const float t = 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+    const float x2 = x * x;
+    const float s = x * (1.0f - x2 * (0.33333333f - 0.13333333f * x2));
+    return fabsf(x) < 0.0625f ? s : t;
+}
+
 // Row data from the expanded matrix (the oracle's orc_fsai_rows).
 __global__ __launch_bounds__(256) void k_fsai_rows(int n, const int *rp, const int *ci, const double *v,
-                                                   const double *diag, double4 *rows) {
+                                                   const double *diag, double4 *rows, double2 *rowx) {
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         double acc = 0.0, asum = 0.0;
         int nl = 0;
@@ -89,28 +119,46 @@ __global__ __launch_bounds__(256) void k_fsai_rows(int n, const int *rp, const i
             if (j < i) nl++;
         }
         rows[i] = make_double4(sqrt(di), acc, (double)nl, (di - asum) / di);
+        rowx[i] = make_double2(1.0 / sqrt(di), log(di));
     }
 }
 
-// Weights staged in LDS as BF16: W2T[o][i] and W2[i][o] (A operands of the forward
-// layer 2 and of dH1 = W2 dZ2).
+// Weights staged in LDS as BF16: W2T[o][i] and W2[i][o] (A operands of the
+// backward's transposes and of dH1 = W2 dZ2), and W2T with its input index permuted
+// for the forward layer 2 on the K = 32 matrix-core form: lane group g of layer 1's
+// output holds, for row blocks m = 2q and 2q + 1, hidden units 16m + 4g + r (r < 4),
+// which are exactly the eight B-operand values of K-block q when K index 8g + j of
+// that block stands for unit 32q + 16(j / 4) + 4g + j % 4. w2p[o][32q + 8g + j] is W2
+// at that unit, so the activations stay in registers between the layers.
+// The forward-only kernel keeps w2p alone (8 KB of LDS, so occupancy is set by
+// registers); the arrays only the backward reads are one element there.
+template <bool BWD>
 struct alignas(16) FsSmem {
-    short w2t[FS_H * FS_H];
-    short w2[FS_H * FS_H];
+    short w2t[BWD ? FS_H * FS_H : 1];
+    short w2[BWD ? FS_H * FS_H : 1];
+    short w2p[FS_H * FS_H];
 };
 
-// The four features of lane group gq (0: f0..f3, 1: f4..f7, 2-3: zero padding).
-__device__ __forceinline__ void fs_features(int n, int i, int j, double aij, const double4 *rows, int gq,
-                                            float (&f)[4]) {
-    const double4 ri = rows[i], rj = rows[j];
+__device__ __forceinline__ int fs_k32_unit(int kk) {
+    const int q = kk >> 5, g = (kk >> 3) & 3, j = kk & 7;
+    return 32 * q + 16 * (j >> 2) + 4 * g + (j & 3);
+}
+
+// The four features of lane group gq (0: f0..f3, 1: f4..f7, 2-3: zero padding); the
+// logarithms of the diagonal are per row (rowx), the index distance's log2 in FP32
+// (the feature is rounded to FP32 and then to BF16 either way).
+__device__ __forceinline__ void fs_features(int i, int j, double aij, const double4 *rows, const double2 *rowx,
+                                            float inv_log2n1, int gq, float (&f)[4]) {
     if (gq == 0) {
+        const double2 xi = rowx[i], xj = rowx[j];
         f[0] = (float)(i == j ? 1.0 : 0.0);
-        f[1] = (float)(aij / (ri.x * rj.x));
-        f[2] = (float)(0.25 * log((ri.x * ri.x) / (rj.x * rj.x)));
-        f[3] = (float)ri.y;
+        f[1] = (float)(aij * xi.x * xj.x);
+        f[2] = (float)(0.25 * (xi.y - xj.y));
+        f[3] = (float)rows[i].y;
     } else if (gq == 1) {
-        f[0] = (float)rj.y;
-        f[1] = (float)(log2(1.0 + (double)(i - j)) / log2(1.0 + (double)n));
+        const double4 ri = rows[i];
+        f[0] = (float)rows[j].y;
+        f[1] = __builtin_amdgcn_logf(1.0f + (float)(i - j)) * inv_log2n1;
         f[2] = (float)(0.25 * ri.z);
         f[3] = (float)ri.w;
     } else {
@@ -118,21 +166,29 @@ __device__ __forceinline__ void fs_features(int n, int i, int j, double aij, con
     }
 }
 
+// 1 / log2(1 + n), the scale of the index-distance feature.
+static inline float fs_inv_log2n1(int n) { return (float)(1.0 / log2(1.0 + (double)n)); }
+
 // Forward (BWD = false: writes the factor values) or forward + backward (BWD = true:
 // per-wave gradient partials). Persistent waves stride over 16-edge tiles.
 template <bool BWD>
-__global__ __launch_bounds__(256) void k_fsai_mlp(int n, long long nnzl, const int *erow, const int *cil,
-                                                  const double *vl, const double4 *rows, const float *params,
+__global__ __launch_bounds__(256, BWD ? 1 : 4) void k_fsai_mlp(int n, long long nnzl, const int *erow, const int *cil,
+                                                  const double *vl, const double4 *rows, const double2 *rowx,
+                                                  float inv_log2n1, const float *params,
                                                   const int *perm, double *gval, float *gval32, double *gtval,
                                                   float *gtval32, const double *gE, float *part) {
-    __shared__ FsSmem sm;
-    __shared__ __attribute__((aligned(16))) short lds_all[4][3 * FS_H * 16 + 16 * 16];
+    __shared__ FsSmem<BWD> sm;
+    __shared__ __attribute__((aligned(16))) short lds_all[4][BWD ? 3 * FS_H * 16 + 16 * 16 : 1];
     const int L = threadIdx.x & 63, w = threadIdx.x >> 6, gq = L >> 4, el = L & 15;
     for (int e = threadIdx.x; e < FS_H * FS_H; e += 256) {
         const int i = e / FS_H, o = e % FS_H;
         const short b = f2bf(params[FS_W2 + e]);
-        sm.w2[e] = b;                         // [i][o]
-        sm.w2t[o * FS_H + i] = b;             // [o][i]
+        if constexpr (BWD) {
+            sm.w2[e] = b;                     // [i][o]
+            sm.w2t[o * FS_H + i] = b;         // [o][i]
+        }
+        const int o2 = e / FS_H, kk = e % FS_H;
+        sm.w2p[e] = f2bf(params[FS_W2 + fs_k32_unit(kk) * FS_H + o2]);   // [o][kk]
     }
     __syncthreads();
     // layer-1 A operand: W1^T[16m + L%16][4gq + j] = W1[4gq + j][16m + L%16]
@@ -175,7 +231,7 @@ __global__ __launch_bounds__(256) void k_fsai_mlp(int n, long long nnzl, const i
         const int i = valid ? erow[e] : 0, j = valid ? cil[e] : 0;
         const double aij = valid ? vl[e] : 0.0;
         float f[4];
-        if (valid) fs_features(n, i, j, aij, rows, gq, f);
+        if (valid) fs_features(i, j, aij, rows, rowx, inv_log2n1, gq, f);
         else f[0] = f[1] = f[2] = f[3] = 0.0f;
         s4v xb;
 #pragma unroll
@@ -188,22 +244,30 @@ __global__ __launch_bounds__(256) void k_fsai_mlp(int n, long long nnzl, const i
             const f4v d = mfma_bf16(A1[m], xb, f4v{0.f, 0.f, 0.f, 0.f});
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                a1[m][r] = tanhf(d[r] + b1[m][r]);
+                a1[m][r] = tanh_fast(d[r] + b1[m][r]);
                 B2[m][r] = f2bf(a1[m][r]);
             }
         }
-        // layer 2: H2^T = W2^T H1^T
+        // layer 2: H2^T = W2^T H1^T, two K = 32 steps per 16 output units
+        s8v B2k[2];
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                B2k[q][j] = B2[2 * q][j];
+                B2k[q][4 + j] = B2[2 * q + 1][j];
+            }
         float a2[4][4];
 #pragma unroll
         for (int mo = 0; mo < 4; mo++) {
             f4v acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const s4v A2 = *(const s4v *)&sm.w2t[(16 * mo + el) * FS_H + 16 * q + 4 * gq];
-                acc = mfma_bf16(A2, B2[q], acc);
+            for (int q = 0; q < 2; q++) {
+                const s8v A2 = *(const s8v *)&sm.w2p[(16 * mo + el) * FS_H + 32 * q + 8 * gq];
+                acc = mfma_bf16_k32(A2, B2k[q], acc);
             }
 #pragma unroll
-            for (int r = 0; r < 4; r++) a2[mo][r] = tanhf(acc[r] + b2[mo][r]);
+            for (int r = 0; r < 4; r++) a2[mo][r] = tanh_fast(acc[r] + b2[mo][r]);
         }
         // layer 3: h = w3 . H2 + b3 (reduce the four lane groups of each edge)
         float ps = 0.0f;
@@ -214,11 +278,10 @@ __global__ __launch_bounds__(256) void k_fsai_mlp(int n, long long nnzl, const i
         ps += __shfl_xor(ps, 16, 64);
         ps += __shfl_xor(ps, 32, 64);
         const float h = ps + b3;
-        if (!BWD) {
+        if constexpr (!BWD) {
             if (valid && gq == 0) {
                 const double hd = (double)h;
-                const double g = (i == j) ? (hd + FS_C0 > 30.0 ? hd + FS_C0 : log1p(exp(hd + FS_C0))) / rows[i].x
-                                          : hd / rows[j].x;
+                const double g = (i == j) ? (double)softplus_f(h + (float)FS_C0) * rowx[i].x : hd * rowx[j].x;
                 gval[e] = g;
                 gval32[e] = (float)g;
                 const int pe = perm[e];
@@ -230,8 +293,8 @@ __global__ __launch_bounds__(256) void k_fsai_mlp(int n, long long nnzl, const i
         // ---- backward: dL/dh from dL/dg through the parametrisation
         float dh = 0.0f;
         if (valid) {
-            const double hd = (double)h, dg = gE[e];
-            dh = (float)((i == j) ? dg / (1.0 + exp(-(hd + FS_C0))) / rows[i].x : dg / rows[j].x);
+            const double dg = gE[e];
+            dh = (float)((i == j) ? dg * (double)sigmoid_f(h + (float)FS_C0) * rowx[i].x : dg * rowx[j].x);
         }
         if (gq == 0) gb3 += dh;
         float dz2[4][4];
@@ -387,10 +450,15 @@ __global__ __launch_bounds__(256) void k_fsai_adam(float *p, float *m, float *v,
 
 // ---------------------------------------------------------------- host side
 
+// Forward (factor build) grid: 1024 workgroups of 4 waves, the 4 waves per SIMD its
+// 128 registers allow, each wave striding over 16-edge tiles (with 256 workgroups, one
+// wave per SIMD, the MLP's dependent MFMA / transcendental / load chains had nothing
+// to overlap with: SQ_WAVES 1024 and 2.7 % MFMA busy at 128^3). The backward keeps
+// FS_BLOCKS workgroups: one gradient partial per wave.
 static int fs_grid(long long nnzl) {
     long long tiles = (nnzl + 15) / 16;
     long long b = (tiles + 3) / 4;
-    return (int)std::max(1LL, std::min(b, (long long)FS_BLOCKS));
+    return (int)std::max(1LL, std::min(b, 1024LL));
 }
 
 extern "C" bkh_fsai_model *bkh_fsai_model_create(const float *init) {
@@ -447,7 +515,7 @@ extern "C" int bkh_fsai_nparam(void) { return FS_NP; }
 static void fs_field_free(bkh_fsai_field *F) {
     if (F->A) bkh_matrix_destroy(F->A);
     if (F->G) bkh_matrix_destroy(F->G);
-    hipFree(F->rpl); hipFree(F->cil); hipFree(F->erow); hipFree(F->perm); hipFree(F->vl); hipFree(F->rows);
+    hipFree(F->rpl); hipFree(F->cil); hipFree(F->erow); hipFree(F->perm); hipFree(F->vl); hipFree(F->rows); hipFree(F->rowx);
     hipFree(F->Z); hipFree(F->W); hipFree(F->U); hipFree(F->gE);
     delete F;
 }
@@ -496,6 +564,7 @@ extern "C" bkh_fsai_field *bkh_fsai_field_create(int n, const int *rpl, const in
               hipMalloc(&F->perm, sizeof(int) * nnzl) == hipSuccess &&
               hipMalloc(&F->vl, sizeof(double) * nnzl) == hipSuccess &&
               hipMalloc(&F->rows, sizeof(double4) * n) == hipSuccess &&
+              hipMalloc(&F->rowx, sizeof(double2) * n) == hipSuccess &&
               hipMalloc(&F->Z, sizeof(double) * np) == hipSuccess &&
               hipMalloc(&F->W, sizeof(double) * np) == hipSuccess &&
               hipMalloc(&F->U, sizeof(double) * np) == hipSuccess &&
@@ -510,7 +579,7 @@ extern "C" bkh_fsai_field *bkh_fsai_field_create(int n, const int *rpl, const in
     if (ok) {
         const bkh_matrix *A = F->A;
         hipLaunchKernelGGL(k_fsai_rows, dim3(vec_grid(n, 256, 1)), dim3(256), 0, s, n, A->rp, A->ci, A->val, A->diag,
-                           F->rows);
+                           F->rows, F->rowx);
         ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(s) == hipSuccess;
     }
     if (!ok) { bkh_set_error("FSAI field upload failed"); fs_field_free(F); return nullptr; }
@@ -553,7 +622,8 @@ __global__ __launch_bounds__(256) void k_csr_to_dia(int n, const int *rp, const 
 
 static int fs_factor(ThreadCtx *t, const bkh_fsai_model *M, bkh_fsai_field *F) {
     hipLaunchKernelGGL((k_fsai_mlp<false>), dim3(fs_grid(F->nnzl)), dim3(256), 0, t->stream, F->n, F->nnzl, F->erow,
-                       F->cil, F->vl, F->rows, M->params, F->perm, F->G->val, F->G->val32, F->G->T->val,
+                       F->cil, F->vl, F->rows, F->rowx, fs_inv_log2n1(F->n), M->params, F->perm, F->G->val,
+                       F->G->val32, F->G->T->val,
                        F->G->T->val32, (const double *)nullptr, (float *)nullptr);
     // the fused PCG applies G and G^T through their SELL copies (FP32 values)
     for (const bkh_matrix *Gm : {(const bkh_matrix *)F->G, (const bkh_matrix *)F->G->T})
@@ -616,7 +686,8 @@ static int fs_loss_grad(ThreadCtx *t, bkh_fsai_model *M, bkh_fsai_field *F, doub
                        (const double *)F->G->val, F->P, (const double *)F->Z, (const double *)F->U, (const Ctl *)c,
                        F->gE);
     hipLaunchKernelGGL((k_fsai_mlp<true>), dim3(FS_BLOCKS), dim3(256), 0, s, n, F->nnzl, F->erow, F->cil, F->vl,
-                       F->rows, M->params, F->perm, (double *)nullptr, (float *)nullptr, (double *)nullptr,
+                       F->rows, F->rowx, fs_inv_log2n1(n), M->params, F->perm, (double *)nullptr, (float *)nullptr,
+                       (double *)nullptr,
                        (float *)nullptr, (const double *)F->gE, M->part);
     hipLaunchKernelGGL(k_fsai_reduce, dim3((FS_NP + 255) / 256), dim3(256), 0, s, (const float *)M->part, FS_WAVES,
                        M->grad);
