@@ -26,7 +26,8 @@ from . import gen
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--solver", choices=["cg", "blcg", "shifted", "pcg", "bicgstabl", "zbicgstabl", "zshbicgstabl"],
+    ap.add_argument("--solver", choices=["cg", "blcg", "shifted", "pcg", "bicgstabl", "zbicgstabl", "zshbicgstabl",
+                                         "batched", "batch_spmv"],
                     default="cg")
     ap.add_argument("--precision", choices=["fp64", "fp32ir"], default="fp64")
     ap.add_argument("--precond", choices=["jacobi", "fsai"], default="jacobi")
@@ -78,6 +79,29 @@ def main():
             def run():
                 dx.zero()
                 return F.pcg_plain(1, db, dx, 1e-8, 100000)
+    elif args.solver in ("batched", "batch_spmv"):
+        systems = gen.cfg4_batch()
+        Bt = dv.Batch(systems, fp32=True)
+        del systems
+        Bt.tune(fp32=prec == dv.BKH_FP32_IR)
+        n = Bt.N
+        db, dx = dv.DeviceArray.from_numpy(gen.normal(405, n)), dv.DeviceArray((n,))
+        if args.solver == "batch_spmv":            # 20 SpMVs in the solve's precision
+            dt = np.float32 if prec == dv.BKH_FP32_IR else np.float64
+            xs, ys = dv.DeviceArray.from_numpy(gen.normal(406, n).astype(dt)), dv.DeviceArray((n,), dt)
+
+            class _Info:
+                ret = inner_iters = passes = 0
+                time_ms = true_res = 0.0
+
+            def run():
+                for _ in range(20):
+                    Bt.spmv(xs, ys, fp32=prec == dv.BKH_FP32_IR)
+                return _Info()
+        else:
+            def run():
+                dx.zero()
+                return Bt.bicgstabl(db, 2, dx, 1e-8, 1000, precision=prec)[0]
     elif args.solver in ("zbicgstabl", "zshbicgstabl"):
         m = args.m or 96
         n, rp, ci, v = gen.zconvdiff3d(m)
