@@ -268,7 +268,8 @@ __global__ __launch_bounds__(256) void k_blk_vr(long long m, T *V, const BlkMats
     gram_block_partial<S>(acc, sh, part);
 }
 
-// V = V R2^-1; P = V + P beta^T (or P = pscale V at the start).
+// V = (V R1^-1) R2^-1 (pass 1's product recomputed, see k_blkr_vr); P = V + P beta^T
+// (or P = pscale V at the start).
 template <int S, typename T>
 __global__ __launch_bounds__(256) void k_blk_vp(long long m, T *V, T *P, const BlkMats *mt,
                                                 double pscale, const Ctl *c) {
@@ -456,10 +457,14 @@ __global__ __launch_bounds__(RP_BS) void k_blkr_xv(long long m, T *__restrict__ 
     row_gram_partial<S>(g, part);
 }
 
-// V = V R1^-1; Gram V^T V.
+// CholeskyQR2 pass 1 without its store: the Gram of V R1^-1 (each row rounded to T as
+// a stored copy would be). k_blkr_vp recomputes V R1^-1 from the same V with the same
+// roundings before applying R2^-1, so the iterates are bitwise those of storing the
+// intermediate block, for one fewer write of V per iteration.
 template <int S, typename T>
-__global__ __launch_bounds__(RP_BS) void k_blkr_vr(long long m, T *__restrict__ V, const BlkMats *__restrict__ mt,
-                                                   const Ctl *c, double *__restrict__ part) {
+__global__ __launch_bounds__(RP_BS) void k_blkr_vr(long long m, const T *__restrict__ V,
+                                                   const BlkMats *__restrict__ mt, const Ctl *c,
+                                                   double *__restrict__ part) {
     if (ctl_stopped(c)) return;
     double g[S * (S + 1) / 2];
 #pragma unroll
@@ -473,7 +478,6 @@ __global__ __launch_bounds__(RP_BS) void k_blkr_vr(long long m, T *__restrict__ 
         const long long nr = row + step;
         if (nr < m) rv.load(V + nr * S);
         row_mul<S, T, true>(v, mt->R1i, o);
-        row_store<S>(V + row * S, o);
         row_gram<S>(o, o, g);
     }
     row_gram_partial<S>(g, part);
@@ -494,7 +498,9 @@ __global__ __launch_bounds__(RP_BS) void k_blkr_vp(long long m, T *__restrict__ 
         if (withp) rp.get(p);
         const long long nr = row + step;
         if (nr < m) { rv.load(V + nr * S); if (withp) rp.load(P + nr * S); }
-        row_mul<S, T, true>(v, mt->R2i, o);
+        double o1[S];
+        row_mul<S, T, true>(v, mt->R1i, o1);
+        row_mul<S, T, true>(o1, mt->R2i, o);
         row_store<S>(V + row * S, o);
         if (withp) {
             double d[S];
@@ -653,8 +659,9 @@ __global__ __launch_bounds__(RP_BS) void k_blkr2_xv(long long m, double *__restr
     pair_gram_partial<S>(g, hi, part);
 }
 
+// Pass 1 of CholeskyQR2 as k_blkr_vr (Gram of V R1^-1, nothing stored).
 template <int S>
-__global__ __launch_bounds__(RP_BS) void k_blkr2_vr(long long m, double *__restrict__ V,
+__global__ __launch_bounds__(RP_BS) void k_blkr2_vr(long long m, const double *__restrict__ V,
                                                     const BlkMats *__restrict__ mt, const Ctl *c,
                                                     double *__restrict__ part) {
     if (ctl_stopped(c)) return;
@@ -673,7 +680,6 @@ __global__ __launch_bounds__(RP_BS) void k_blkr2_vr(long long m, double *__restr
         if (nr < m) rv.load(V + nr * S + off);
         pair_full<S>(v, hi, vf);
         pair_mul<S>(vf, mt->R1i, hi, false, o);
-        row_store<S / 2>(V + row * S + off, o);
         pair_full<S>(o, hi, of);
         pair_gram<S>(of, o, g);
     }
@@ -689,13 +695,15 @@ __global__ __launch_bounds__(RP_BS) void k_blkr2_vp(long long m, double *__restr
     RowRaw<S / 2, double> rv, rp;
     if (row < m) { rv.load(V + row * S + off); if (withp) rp.load(P + row * S + off); }
     for (; row < m; row += step) {
-        double v[S / 2], vf[S], o[S / 2], p[S / 2];
+        double v[S / 2], vf[S], o1[S / 2], o1f[S], o[S / 2], p[S / 2];
         rv.get(v);
         if (withp) rp.get(p);
         const long long nr = row + step;
         if (nr < m) { rv.load(V + nr * S + off); if (withp) rp.load(P + nr * S + off); }
         pair_full<S>(v, hi, vf);
-        pair_mul<S>(vf, mt->R2i, hi, false, o);
+        pair_mul<S>(vf, mt->R1i, hi, false, o1);         // pass 1's V R1^-1, recomputed
+        pair_full<S>(o1, hi, o1f);
+        pair_mul<S>(o1f, mt->R2i, hi, false, o);
         row_store<S / 2>(V + row * S + off, o);
         if (withp) {
             double pf[S], d[S / 2];
@@ -881,10 +889,10 @@ static int launch_spmm_gram(hipStream_t s, const bkh_matrix *A, const VT *val, c
         if (off || A->nsl == 0 || A->variant != SPMV_SELL || (((uintptr_t)P | (uintptr_t)AP) & 15)) return 0;
         constexpr int BSG = 1024, UNR = 1, SPB = UNR * BSG / (64 * LPR);   // slices per pass
         const int npass = (A->nsl + SPB - 1) / SPB;
-        // 512 workgroups of 16 waves (8 per SIMD): 139.8 us per FP64 s = 8 launch at 128^3
-        // against 113.4 + 47.4 us for the separate SpMM and Gram passes; 256 workgroups
-        // (4 waves per SIMD) took 179 us, 1024 of 8 waves 158 us, two slices per thread
-        // per pass 162 us
+        // 512 workgroups of 16 waves (8 per SIMD): 156.8 us per FP64 s = 8 launch at 128^3
+        // against 113.1 + 47.3 us for the separate SpMM and Gram passes on the same box
+        // (another box: 139.8 against 113.4 + 47.4); 256 workgroups (4 waves per SIMD)
+        // were 20 % slower, 1024 of 8 waves and two slices per thread per pass 10 %
         const int G0 = std::max(1, std::min(S <= 8 ? FIN_DIRECT_WIDE : FIN_DIRECT, npass));
         const int chunk = ((npass + G0 - 1) / G0) * SPB;           // slices per workgroup
         const int G = (A->nsl + chunk - 1) / chunk;
