@@ -1,5 +1,5 @@
 /*
- * bksp.h -- public API of the MI355X-native bksp drop-in (real FP64 part).
+ * bksp.h -- public API of the MI355X-native bksp drop-in (real and complex FP64).
  *
  * Same declarations as the reference's public header, so a C caller written
  * against it compiles and links unchanged:
@@ -17,7 +17,8 @@
  * fell below tol (outer iterations for the BiCGSTAB(l) family); negative values
  * are the codes below. Device failures use the extension codes of bksp_hip.h.
  *
- * The complex (z*) family of the reference header is not provided.
+ * The complex (z*) family (reference include/bksp.h:109-190) is declared in the
+ * "complex numbers" section below; values are C99 double _Complex, as there.
  */
 #ifndef BKSP_H
 #define BKSP_H
