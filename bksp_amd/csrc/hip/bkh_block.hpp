@@ -420,12 +420,94 @@ __global__ __launch_bounds__(256) void k_spmm_sell(int n, const int *__restrict_
     for (int j = 0; j < S; j++) Y[(size_t)row * S + j] = acc[j];
 }
 
+#ifndef BKH_SPMM_KB
+#define BKH_SPMM_KB 8      // entries per row loaded before use in k_spmm_sell_v
+#endif
+
 // Lanes per row of k_spmm_sell_v: each lane owns 16 bytes of a row of X and Y
 // (S * sizeof(T) = 16 * LPR), 0 when S does not split that way.
 template <int S, typename T>
 constexpr int spmm_lpr() {
     constexpr int b = S * (int)sizeof(T);
     return (b == 16 || b == 32 || b == 64 || b == 128) ? b / 16 : 0;
+}
+
+// Lane k of the calling lane's group of G consecutive lanes (G a power of two, k < G,
+// k a constant after unrolling): DPP quad permutations for G = 2 and 4, ds_bpermute
+// above. Every lane of a group must be active.
+template <int CTRL>
+__device__ __forceinline__ int dpp_mov(int v) {
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+    return __int_as_float(dpp_mov<CTRL>(__float_as_int(v)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+    return __hiloint2double(dpp_mov<CTRL>(__double2hiint(v)), dpp_mov<CTRL>(__double2loint(v)));
+}
+template <int G, typename V>
+__device__ __forceinline__ V grp_bcast(V v, int k) {
+    if constexpr (G == 1) {
+        return v;
+    } else if constexpr (G == 2) {       // quad_perm (k, k, 2 + k, 2 + k)
+        return (k & 1) ? dpp_mov<0xF5>(v) : dpp_mov<0xA0>(v);
+    } else if constexpr (G == 4) {       // quad_perm (k, k, k, k)
+        switch (k & 3) {
+            case 0: return dpp_mov<0x00>(v);
+            case 1: return dpp_mov<0x55>(v);
+            case 2: return dpp_mov<0xAA>(v);
+            default: return dpp_mov<0xFF>(v);
+        }
+    } else {
+        return __shfl(v, (int)(threadIdx.x & 63 & ~(G - 1)) + k, 64);
+    }
+}
+
+// KB entries k0 .. k0 + KB - 1 of one SELL / slice-diagonal row (entry kk at
+// val[base + 64 kk]; last = the row's final entry), accumulated into the calling
+// lane's CPL = S / LPR columns in entry order. Lane `sub` of the row's LPR lanes loads
+// the values (and SELL columns) of entries k0 + sub + LPR i, and the row's lanes take
+// them from it by lane broadcast, so a row costs ceil(KB / LPR) value loads per block
+// instead of KB; then all KB gathers of X are issued before the first is used.
+// Entries past `last` re-read the final one (in bounds) and are not accumulated.
+template <int S, int LPR, int KB, typename VT, typename T, typename IT, bool DIA>
+__device__ __forceinline__ void spmm_row_block(int n, int row, int sub, int k0, int last, int base,
+                                               const int *dof, const VT *__restrict__ val,
+                                               const IT *__restrict__ col, const T *X, bool a1,
+                                               double alpha, T (&acc)[S / LPR]) {
+    constexpr int CPL = S / LPR, NV = (KB + LPR - 1) / LPR;
+    typedef T tvec __attribute__((ext_vector_type(CPL)));
+    VT vl[NV];
+    int cl[NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+        const int e = base + 64 * min(k0 + sub + LPR * i, last);
+        vl[i] = __builtin_nontemporal_load(val + e);          // the matrix streams once:
+        if constexpr (!DIA) cl[i] = (int)__builtin_nontemporal_load(col + e);   // keep X in L2
+    }
+    VT av[KB];
+    tvec xv[KB];
+#pragma unroll
+    for (int q = 0; q < KB; q++) {
+        av[q] = grp_bcast<LPR>(vl[q / LPR], q % LPR);
+        int c;
+        if constexpr (DIA) {
+            c = min(max(row + dof[min(k0 + q, last)], 0), n - 1);
+        } else {
+            const int ce = grp_bcast<LPR>(cl[q / LPR], q % LPR);
+            c = sizeof(IT) == 2 ? row + ce : ce;
+        }
+        xv[q] = *(const tvec *)(X + (size_t)c * S + sub * CPL);
+    }
+#pragma unroll
+    for (int q = 0; q < KB; q++)
+        if (k0 + q <= last) {
+            const T a = a1 ? (T)av[q] : (T)(alpha * (double)av[q]);
+#pragma unroll
+            for (int j = 0; j < CPL; j++) acc[j] = add_rn(acc[j], mul_rn(a, xv[q][j]));
+        }
 }
 
 // Y = alpha A X + beta Y on the SELL-64 copy of A with LPR lanes per row: lane `sub`
@@ -438,7 +520,7 @@ constexpr int spmm_lpr() {
 // column offsets; a wave's rows lie in one slice, so the offsets are uniform loads);
 // its value-0 positions add 0 products at clamped columns, which leaves every sum as
 // on the SELL copy for finite X.
-template <int S, typename VT, typename T, typename IT, bool DIA = false>
+template <int S, typename VT, typename T, typename IT, bool DIA = false, int KB = BKH_SPMM_KB>
 __global__ __launch_bounds__(256) void k_spmm_sell_v(int n, const int *__restrict__ rp, const int *__restrict__ so,
                                                      const VT *__restrict__ val, const IT *__restrict__ col,
                                                      const T *X, T *Y, double alpha, double beta, const Ctl *ctl) {
@@ -472,22 +554,8 @@ __global__ __launch_bounds__(256) void k_spmm_sell_v(int n, const int *__restric
 #pragma unroll
         for (int j = 0; j < CPL; j++) acc[j] = (T)0;
     }
-#pragma unroll 2
-    for (int kk = 0; kk < len; kk++) {
-        const int e = base + 64 * kk;
-        int c;
-        if constexpr (DIA) {
-            c = min(max(row + dof[kk], 0), n - 1);
-        } else {
-            const IT ce = __builtin_nontemporal_load(col + e);  // the matrix streams once:
-            c = sizeof(IT) == 2 ? row + (int)ce : (int)ce;        // keep X's lines in L2
-        }
-        const VT ve = __builtin_nontemporal_load(val + e);
-        const T a = a1 ? (T)ve : (T)(alpha * (double)ve);
-        const tvec xv = *(const tvec *)(X + (size_t)c * S + sub * CPL);
-#pragma unroll
-        for (int j = 0; j < CPL; j++) acc[j] = add_rn(acc[j], mul_rn(a, xv[j]));
-    }
+    for (int k0 = 0; k0 < len; k0 += KB)
+        spmm_row_block<S, LPR, KB, VT, T, IT, DIA>(n, row, sub, k0, len - 1, base, dof, val, col, X, a1, alpha, acc);
     tvec out;
 #pragma unroll
     for (int j = 0; j < CPL; j++) out[j] = acc[j];
@@ -528,7 +596,7 @@ __device__ __forceinline__ void spmm_gram_row(int n, int sl, int row, int sub, b
 // UNR rows' gathers in flight between two MFMA steps.
 // (8 waves per SIMD: two 1024-thread workgroups per CU; the FP32 variants would
 // otherwise take 66 registers and fit one.)
-template <int S, typename VT, typename T, typename IT, bool DIA, int BS, int UNR>
+template <int S, typename VT, typename T, typename IT, bool DIA, int BS, int UNR, int KB>
 __global__ __launch_bounds__(BS, S <= 8 ? 8 : 4) void k_spmm_gram(int n, int nsl, int chunk, const int *__restrict__ rp,
                                                   const int *__restrict__ so, const VT *__restrict__ val,
                                                   const IT *__restrict__ col, const T *X, T *Y, const Ctl *ctl,
@@ -556,7 +624,7 @@ __global__ __launch_bounds__(BS, S <= 8 ? 8 : 4) void k_spmm_gram(int n, int nsl
         int L = 0;
 #pragma unroll
         for (int u = 0; u < UNR; u++) {
-            const int sl = qb + u * SPB + rp_ / 64;              // uniform per wave
+            const int sl = __builtin_amdgcn_readfirstlane(qb + u * SPB + rp_ / 64);   // uniform per wave
             row[u] = sl * 64 + (rp_ & 63);
             live[u] = sl < q1 && row[u] < n;
             spmm_gram_row<S, VT, T, IT, DIA>(n, sl, row[u], sub, live[u], rp, so, val, col, len[u], base[u], dof[u]);
@@ -564,24 +632,12 @@ __global__ __launch_bounds__(BS, S <= 8 ? 8 : 4) void k_spmm_gram(int n, int nsl
 #pragma unroll
             for (int j = 0; j < CPL; j++) acc[u][j] = (T)0;
         }
-        for (int kk = 0; kk < L; kk++) {
+        for (int k0 = 0; k0 < L; k0 += KB) {
 #pragma unroll
-            for (int u = 0; u < UNR; u++) {
-                if (kk < len[u]) {
-                    const int e = base[u] + 64 * kk;
-                    int c;
-                    if constexpr (DIA) {
-                        c = min(max(row[u] + dof[u][kk], 0), n - 1);
-                    } else {
-                        const IT ce = __builtin_nontemporal_load(col + e);
-                        c = sizeof(IT) == 2 ? row[u] + (int)ce : (int)ce;
-                    }
-                    const T a = (T)__builtin_nontemporal_load(val + e);
-                    const tvec xv = *(const tvec *)(X + (size_t)c * S + sub * CPL);
-#pragma unroll
-                    for (int j = 0; j < CPL; j++) acc[u][j] = add_rn(acc[u][j], mul_rn(a, xv[j]));
-                }
-            }
+            for (int u = 0; u < UNR; u++)
+                if (k0 < len[u])                                  // rows that are not live have len 0
+                    spmm_row_block<S, LPR, KB, VT, T, IT, DIA>(n, row[u], sub, k0, len[u] - 1, base[u], dof[u], val,
+                                                               col, X, true, 1.0, acc[u]);
         }
 #pragma unroll
         for (int u = 0; u < UNR; u++) {
