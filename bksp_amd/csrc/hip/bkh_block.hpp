@@ -592,12 +592,11 @@ __device__ __forceinline__ void spmm_gram_row(int n, int sl, int row, int sub, b
     }
 }
 
-// UNR rows per thread (from slices SPB apart) are multiplied together, so a wave has
-// UNR rows' gathers in flight between two MFMA steps.
-// (8 waves per SIMD: two 1024-thread workgroups per CU; the FP32 variants would
-// otherwise take 66 registers and fit one.)
-template <int S, typename VT, typename T, typename IT, bool DIA, int BS, int UNR, int KB>
-__global__ __launch_bounds__(BS, S <= 8 ? 8 : 4) void k_spmm_gram(int n, int nsl, int chunk, const int *__restrict__ rp,
+// UNR rows per thread (from slices SPB apart) are multiplied together; KB entries of
+// each row are in flight at once (spmm_row_block). MINW: waves per SIMD the register
+// budget is set for (8: two 1024-thread workgroups per CU, 4: one).
+template <int S, typename VT, typename T, typename IT, bool DIA, int BS, int UNR, int KB, int MINW>
+__global__ __launch_bounds__(BS, MINW) void k_spmm_gram(int n, int nsl, int chunk, const int *__restrict__ rp,
                                                   const int *__restrict__ so, const VT *__restrict__ val,
                                                   const IT *__restrict__ col, const T *X, T *Y, const Ctl *ctl,
                                                   double *__restrict__ part) {

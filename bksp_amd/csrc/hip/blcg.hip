@@ -40,9 +40,6 @@ __global__ __launch_bounds__(256) void k_sh_fill_f(float *p, long long m) {
 // threads, 32 rows each at most).
 #define FIN_DIRECT 256
 #define FIN_DIRECT_WIDE 512
-#ifndef BKH_GRAM_KB
-#define BKH_GRAM_KB 4      // SpMM entries loaded per row before use in k_spmm_gram
-#endif
 
 // Fixed-order sum of nblocks rows of E partials by a BS-thread workgroup: thread
 // t < Q*E (Q = BS/E groups) sums rows q, q+Q, q+2Q, ... of entry e = t % E, q = t / E,
@@ -890,13 +887,15 @@ static int launch_spmm_gram(hipStream_t s, const bkh_matrix *A, const VT *val, c
     } else {
         static const bool off = [] { const char *e = getenv("BKH_GRAM_FUSE"); return e && e[0] == '0'; }();
         if (off || A->nsl == 0 || A->variant != SPMV_SELL || (((uintptr_t)P | (uintptr_t)AP) & 15)) return 0;
-        constexpr int BSG = 1024, UNR = 1, KB = BKH_GRAM_KB, SPB = UNR * BSG / (64 * LPR);   // slices per pass
+        // FP64 rows of s <= 8: eight entries per row in flight (88 registers, one 16-wave
+        // workgroup per CU, 256 workgroups): 120.5 us per s = 8 launch at 128^3 against
+        // 129.3 us for four entries at 8 waves per SIMD (512 workgroups; 1024: 129.2),
+        // 154-157 us with two slices per thread per pass. Otherwise four entries.
+        constexpr bool WIDE = sizeof(T) == 8 && S <= 8;
+        constexpr int BSG = 1024, UNR = 1, KB = WIDE ? 8 : 4, MINW = WIDE ? 4 : (S <= 8 ? 8 : 4);
+        constexpr int SPB = UNR * BSG / (64 * LPR);                  // slices per pass
         const int npass = (A->nsl + SPB - 1) / SPB;
-        // 512 workgroups of 16 waves (8 per SIMD): 156.8 us per FP64 s = 8 launch at 128^3
-        // against 113.1 + 47.3 us for the separate SpMM and Gram passes on the same box
-        // (another box: 139.8 against 113.4 + 47.4); 256 workgroups (4 waves per SIMD)
-        // were 20 % slower, 1024 of 8 waves and two slices per thread per pass 10 %
-        const int G0 = std::max(1, std::min(S <= 8 ? FIN_DIRECT_WIDE : FIN_DIRECT, npass));
+        const int G0 = std::max(1, std::min(WIDE ? 256 : S <= 8 ? FIN_DIRECT_WIDE : FIN_DIRECT, npass));
         const int chunk = ((npass + G0 - 1) / G0) * SPB;           // slices per workgroup
         const int G = (A->nsl + chunk - 1) / chunk;
         const VT *dv = nullptr, *sv = nullptr;
@@ -906,13 +905,13 @@ static int launch_spmm_gram(hipStream_t s, const bkh_matrix *A, const VT *val, c
             if (val == (const VT *)A->val) { dv = (const VT *)A->dval; sv = (const VT *)A->sval; }
         }
         if (A->dia_entries > 0 && dv)
-            hipLaunchKernelGGL((k_spmm_gram<S, VT, T, int, true, BSG, UNR, KB>), dim3(G), dim3(BSG), 0, s, (int)A->n, A->nsl,
+            hipLaunchKernelGGL((k_spmm_gram<S, VT, T, int, true, BSG, UNR, KB, MINW>), dim3(G), dim3(BSG), 0, s, (int)A->n, A->nsl,
                                chunk, A->rp, A->dso, dv, (const int *)A->doff, P, AP, c, part);
         else if (sv && A->scd)
-            hipLaunchKernelGGL((k_spmm_gram<S, VT, T, short, false, BSG, UNR, KB>), dim3(G), dim3(BSG), 0, s, (int)A->n,
+            hipLaunchKernelGGL((k_spmm_gram<S, VT, T, short, false, BSG, UNR, KB, MINW>), dim3(G), dim3(BSG), 0, s, (int)A->n,
                                A->nsl, chunk, A->rp, A->sso, sv, (const short *)A->scd, P, AP, c, part);
         else if (sv && A->scol)
-            hipLaunchKernelGGL((k_spmm_gram<S, VT, T, int, false, BSG, UNR, KB>), dim3(G), dim3(BSG), 0, s, (int)A->n,
+            hipLaunchKernelGGL((k_spmm_gram<S, VT, T, int, false, BSG, UNR, KB, MINW>), dim3(G), dim3(BSG), 0, s, (int)A->n,
                                A->nsl, chunk, A->rp, A->sso, sv, (const int *)A->scol, P, AP, c, part);
         else
             return 0;
