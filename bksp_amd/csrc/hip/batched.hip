@@ -53,9 +53,11 @@ struct bkh_batch {
 
 // ------------------------------------------------------------------ kernels
 
+// Every phase is a device function of the unit / tile / system it serves, so the
+// same code runs as one launch per phase (graph path) and inside the persistent
+// kernel below. `sh` is workgroup scratch for the partial sums (>= 4 NV doubles).
 template <int NV>
-__device__ __forceinline__ void unit_partial(double (&v)[NV], double *part, int u) {
-    __shared__ double sh[NV * 4 > 0 ? NV * 4 : 1];
+__device__ __forceinline__ void unit_partial(double (&v)[NV], double *part, int u, double *sh) {
     block_sum<NV, 256>(v, sh);
     if (threadIdx.x == 0) {
 #pragma unroll
@@ -63,15 +65,15 @@ __device__ __forceinline__ void unit_partial(double (&v)[NV], double *part, int 
     }
 }
 
-// y = A x over the batch; epilogue dot products of y with the vectors in e, per unit.
+// y = A x over unit u's four wave blocks; epilogue dot products of y with the vectors
+// in e (one partial row per unit). prod: the workgroup's 4 x SPMV_WCAP products.
 template <typename VT, typename T, int NV>
-__global__ __launch_bounds__(256) void k_bt_spmv(const int *__restrict__ rp, const int *__restrict__ ci,
-                                                 const VT *__restrict__ val, const int *__restrict__ wblk,
-                                                 const int *__restrict__ usys, const BtSys *sy, const T *x, T *y,
-                                                 Epi<NV, T> e, double *part) {
-    const int u = xcd_block(blockIdx.x, gridDim.x);
+__device__ __forceinline__ void bt_spmv_unit(int u, const int *__restrict__ rp, const int *__restrict__ ci,
+                                             const VT *__restrict__ val, const int *__restrict__ wblk,
+                                             const int *__restrict__ usys, const BtSys *sy, const T *x, T *y,
+                                             const Epi<NV, T> &e, double *part, T (*prod_all)[SPMV_WCAP],
+                                             double *sh) {
     if (sy && sy[usys[u]].done) return;
-    __shared__ T prod_all[4][SPMV_WCAP];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     T *prod = prod_all[w];
     const int b = u * 4 + w;
@@ -116,23 +118,36 @@ __global__ __launch_bounds__(256) void k_bt_spmv(const int *__restrict__ rp, con
             for (int i = 0; i < NV; i++)
                 dv[i] = epi_operand(e, e.a[i], row, (double)s) * epi_operand(e, e.b[i], row, (double)s);
         }
+        // the next unit of this wave rewrites prod only after every lane has read it
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    if constexpr (NV > 0) unit_partial<NV>(dv, part, u);
+    if constexpr (NV > 0) unit_partial<NV>(dv, part, u, sh);
 }
 
-#define TILE_ROWS(i)                                                                  \
-    const int t_ = blockIdx.x;                                                        \
+template <typename VT, typename T, int NV>
+__global__ __launch_bounds__(256) void k_bt_spmv(const int *__restrict__ rp, const int *__restrict__ ci,
+                                                 const VT *__restrict__ val, const int *__restrict__ wblk,
+                                                 const int *__restrict__ usys, const BtSys *sy, const T *x, T *y,
+                                                 Epi<NV, T> e, double *part) {
+    __shared__ T prod_all[4][SPMV_WCAP];
+    __shared__ double sh[NV * 4 > 0 ? NV * 4 : 1];
+    bt_spmv_unit<VT, T, NV>(xcd_block(blockIdx.x, gridDim.x), rp, ci, val, wblk, usys, sy, x, y, e, part, prod_all,
+                            sh);
+}
+
+#define TILE_ROWS(i, t_)                                                              \
     const int sys = tsys[t_];                                                         \
     for (long long i = (long long)trow[t_] + threadIdx.x; i < trow[t_ + 1]; i += 256)
 
 // r0 = b - A x (A x in r0 on entry); p0 = rs = r0; partials {r0.r0, b.b}.
 template <typename T>
-__global__ __launch_bounds__(256) void k_bt_init(const int *trow, const int *tsys, long long N, const T *b,
-                                                 T *R, T *P, T *rs, const BtSys *sy, double *part) {
+__device__ __forceinline__ void bt_init_tile(int t, const int *trow, const int *tsys, const T *b, T *R, T *P,
+                                             T *rs, const BtSys *sy, double *part, double *sh) {
     double v[2] = {0.0, 0.0};
-    bool live = !sy[tsys[blockIdx.x]].done;
-    if (live) {
-        TILE_ROWS(i) {
+    if (!sy[tsys[t]].done) {
+        TILE_ROWS(i, t) {
             const T ri = b[i] - R[i];
             R[i] = ri; P[i] = ri; rs[i] = ri;
             v[0] += (double)ri * (double)ri;
@@ -140,42 +155,58 @@ __global__ __launch_bounds__(256) void k_bt_init(const int *trow, const int *tsy
         }
         (void)sys;
     }
-    unit_partial<2>(v, part, blockIdx.x);
+    unit_partial<2>(v, part, t, sh);
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_bt_init(const int *trow, const int *tsys, long long N, const T *b,
+                                                 T *R, T *P, T *rs, const BtSys *sy, double *part) {
+    __shared__ double sh[8];
+    bt_init_tile<T>(blockIdx.x, trow, tsys, b, R, P, rs, sy, part, sh);
 }
 
 // p_q = beta p_q + r_q, q <= j (bicgstabl.c:64-67)
 template <typename T>
-__global__ __launch_bounds__(256) void k_bt_p(const int *trow, const int *tsys, long long N, int j, T *P,
-                                              const T *R, const BtSys *sy) {
-    if (sy[tsys[blockIdx.x]].done) return;
-    TILE_ROWS(i) {
+__device__ __forceinline__ void bt_p_tile(int t, const int *trow, const int *tsys, long long N, int j, T *P,
+                                          const T *R, const BtSys *sy) {
+    if (sy[tsys[t]].done) return;
+    TILE_ROWS(i, t) {
         const T be = (T)sy[sys].beta;
         for (int q = 0; q <= j; q++) P[q * N + i] = add_rn(mul_rn(be, P[q * N + i]), R[q * N + i]);
     }
 }
+template <typename T>
+__global__ __launch_bounds__(256) void k_bt_p(const int *trow, const int *tsys, long long N, int j, T *P,
+                                              const T *R, const BtSys *sy) {
+    bt_p_tile<T>(blockIdx.x, trow, tsys, N, j, P, R, sy);
+}
 
 // r_q -= alpha p_{q+1}, q <= j; x += alpha p_0 (:73-75, :79)
 template <typename T>
-__global__ __launch_bounds__(256) void k_bt_r(const int *trow, const int *tsys, long long N, int j, const T *P,
-                                              T *R, T *x, const BtSys *sy) {
-    if (sy[tsys[blockIdx.x]].done) return;
-    TILE_ROWS(i) {
+__device__ __forceinline__ void bt_r_tile(int t, const int *trow, const int *tsys, long long N, int j, const T *P,
+                                          T *R, T *x, const BtSys *sy) {
+    if (sy[tsys[t]].done) return;
+    TILE_ROWS(i, t) {
         const T al = (T)sy[sys].alpha;
         for (int q = 0; q <= j; q++) R[q * N + i] = fma(-al, P[(q + 1) * N + i], R[q * N + i]);
         x[i] = fma(al, P[i], x[i]);
     }
 }
+template <typename T>
+__global__ __launch_bounds__(256) void k_bt_r(const int *trow, const int *tsys, long long N, int j, const T *P,
+                                              T *R, T *x, const BtSys *sy) {
+    bt_r_tile<T>(blockIdx.x, trow, tsys, N, j, P, R, x, sy);
+}
 
 // MR update of x, r_0, p_0 with partials {r0.r0, rs.r0} (:113-125)
 template <typename T>
-__global__ __launch_bounds__(256) void k_bt_upd(const int *trow, const int *tsys, long long N, int l, T *P, T *R,
-                                                T *x, const T *rs, const BtSys *sy, double *part) {
+__device__ __forceinline__ void bt_upd_tile(int t, const int *trow, const int *tsys, long long N, int l, T *P,
+                                            T *R, T *x, const T *rs, const BtSys *sy, double *part, double *sh) {
     double v[2] = {0.0, 0.0};
-    const int sy_i = tsys[blockIdx.x];
+    const int sy_i = tsys[t];
     if (!sy[sy_i].done) {
         double cx[MR_D], cr[MR_D], cp[MR_D];
         for (int k = 0; k <= l; k++) { cx[k] = sy[sy_i].cx[k]; cr[k] = sy[sy_i].cr[k]; cp[k] = sy[sy_i].cp[k]; }
-        TILE_ROWS(i) {
+        TILE_ROWS(i, t) {
             double xi = (double)x[i], r0 = (double)R[i], p0 = (double)P[i];
             xi += cx[0] * r0;
             for (int k = 1; k <= l; k++) {
@@ -191,7 +222,13 @@ __global__ __launch_bounds__(256) void k_bt_upd(const int *trow, const int *tsys
         }
         (void)sys;
     }
-    unit_partial<2>(v, part, blockIdx.x);
+    unit_partial<2>(v, part, t, sh);
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_bt_upd(const int *trow, const int *tsys, long long N, int l, T *P, T *R,
+                                                T *x, const T *rs, const BtSys *sy, double *part) {
+    __shared__ double sh[8];
+    bt_upd_tile<T>(blockIdx.x, trow, tsys, N, l, P, R, x, rs, sy, part, sh);
 }
 
 enum { BT_INIT = 0, BT_ALPHA = 1, BT_GAMMA = 2, BT_MR = 3, BT_UPD = 4, BT_RESID = 5 };
@@ -200,46 +237,46 @@ __device__ __forceinline__ void bt_stop(BtSys &s, int status) {
     s.done = 1; s.status = status; s.result = s.it;
 }
 
-// One workgroup per system: the system's partial rows [rng[s], rng[s+1]) summed in a
-// fixed order, then its scalar step (the same recurrences as the single-system path).
+// System q's finalize: its partial rows [rng[q], rng[q+1]) summed in a fixed order by
+// the workgroup, then its scalar step in thread 0 (the same recurrences as the
+// single-system path).
 template <int OP, int NV>
-__global__ __launch_bounds__(256) void k_bt_fin(const double *part, const int *rng, BtSys *sy, int l) {
-    BtSys &S = sy[blockIdx.x];
+__device__ __forceinline__ void bt_fin_sys(int q, const double *part, const int *rng, BtSys *sy, int l, double *sh) {
+    BtSys &S = sy[q];
     if (OP != BT_RESID && S.done) return;
-    __shared__ double sh[NV * 4];
     double acc[NV];
 #pragma unroll
     for (int i = 0; i < NV; i++) acc[i] = 0.0;
-    for (int r = rng[blockIdx.x] + threadIdx.x; r < rng[blockIdx.x + 1]; r += 256)
+    for (int r = rng[q] + threadIdx.x; r < rng[q + 1]; r += 256)
 #pragma unroll
         for (int i = 0; i < NV; i++) acc[i] += part[(size_t)r * NV + i];
     block_sum<NV, 256>(acc, sh);
     if (threadIdx.x != 0) return;
-    if (OP == BT_RESID) {                         // FP64 true residual: ||r||, ||b||, ratio
+    if constexpr (OP == BT_RESID) {                         // FP64 true residual: ||r||, ||b||, ratio
         S.err = sqrt(acc[0]);
         S.fnb = sqrt(acc[1]);
         S.rel = acc[1] > 0.0 ? sqrt(acc[0] / acc[1]) : 0.0;
         return;
     }
-    if (OP == BT_INIT) {
+    if constexpr (OP == BT_INIT) {
         S.fnb = sqrt(acc[1]);
         if (!(S.fnb > 0.0)) { bt_stop(S, ST_BREAKDOWN); return; }
         if (acc[0] == 0.0) { S.done = 1; S.status = ST_CONVERGED; S.result = 0; return; }
         S.alpha = 0.0; S.beta = 0.0; S.g0 = acc[0];
         return;
     }
-    if (OP == BT_ALPHA) {
+    if constexpr (OP == BT_ALPHA) {
         if (bad(acc[0])) { bt_stop(S, ST_BREAKDOWN); return; }
         S.alpha = S.g0 / acc[0];
         return;
     }
-    if (OP == BT_GAMMA) {
+    if constexpr (OP == BT_GAMMA) {
         if (bad(S.g0)) { bt_stop(S, ST_BREAKDOWN); return; }
         S.beta = -S.alpha * acc[0] / S.g0;
         S.g0 = acc[0];
         return;
     }
-    if (OP == BT_MR) {
+    if constexpr (OP == BT_MR) {
         double G[MR_D][MR_D];
         gram_unpack(l, acc, G);
         if (G[0][0] == 0.0) { S.done = 1; S.status = ST_CONVERGED; S.result = S.it; return; }
@@ -261,17 +298,226 @@ __global__ __launch_bounds__(256) void k_bt_fin(const double *part, const int *r
         return;
     }
     // BT_UPD: error = ||r0|| / ||b|| (:122-125), then the next first beta (:57-62)
-    const double err = sqrt(acc[0]) / S.fnb;
-    S.err = err;
-    const int it = S.it;
-    S.it = it + 1;
-    if (err < S.tol) { S.done = 1; S.status = ST_CONVERGED; S.result = it; return; }
-    if (!(err == err) || err > 1.0e300) { S.done = 1; S.status = ST_BREAKDOWN; S.result = it; return; }
-    if (it + 1 >= S.max_it) { S.done = 1; S.status = ST_MAXIT; S.result = it; return; }
-    const double g0 = -S.oml * S.g0;
-    if (bad(g0)) { bt_stop(S, ST_BREAKDOWN); return; }
-    S.beta = -S.alpha * acc[1] / g0;
-    S.g0 = acc[1];
+    if constexpr (OP == BT_UPD) {
+        const double err = sqrt(acc[0]) / S.fnb;
+        S.err = err;
+        const int it = S.it;
+        S.it = it + 1;
+        if (err < S.tol) { S.done = 1; S.status = ST_CONVERGED; S.result = it; return; }
+        if (!(err == err) || err > 1.0e300) { S.done = 1; S.status = ST_BREAKDOWN; S.result = it; return; }
+        if (it + 1 >= S.max_it) { S.done = 1; S.status = ST_MAXIT; S.result = it; return; }
+        const double g0 = -S.oml * S.g0;
+        if (bad(g0)) { bt_stop(S, ST_BREAKDOWN); return; }
+        S.beta = -S.alpha * acc[1] / g0;
+        S.g0 = acc[1];
+    }
+}
+
+// One workgroup per system.
+template <int OP, int NV>
+__global__ __launch_bounds__(256) void k_bt_fin(const double *part, const int *rng, BtSys *sy, int l) {
+    __shared__ double sh[NV * 4];
+    bt_fin_sys<OP, NV>(blockIdx.x, part, rng, sy, l, sh);
+}
+
+// ------------------------------------------------------- persistent solve
+//
+// The whole batched BiCGSTAB(L) -- initial residual and every outer iteration of
+// every system -- as ONE launch: a resident grid (cooperative launch, so every
+// workgroup is on the device at once) steps through the phases of the graph path,
+// each phase spread over the grid, with a grid barrier between phases. Units, tiles
+// and systems are dealt to workgroups XCD by XCD (workgroup g runs on XCD g % 8 and
+// takes the g / 8-th share of that XCD's contiguous range), so a system's units share
+// one L2 as in the per-phase launches. The loop ends when no system is active or
+// after max_it iterations, the same test k_bt_alldone makes.
+//
+// Grid barrier, two levels. The workgroups of one XCD (workgroup g runs on XCD g % 8,
+// checked against the XCC_ID hardware register at entry) arrive on that XCD's
+// monotone counter once their stores have completed (they sit in the XCD's shared
+// L2). The last arriver of the XCD is its leader for this barrier: an agent-scope
+// fence writes the XCD's L2 back, it arrives on the global counter (barrier k
+// completes at 8 k leader arrivals; the last publishes generation k), waits for it,
+// invalidates the XCD's L2 with a second agent-scope fence and publishes the XCD's
+// generation; the others wait for that and make an acquire-only fence. So each
+// barrier writes each XCD's L2 back once, as a kernel boundary does, instead of once
+// per workgroup. If any workgroup finds itself on another XCD than g % 8, every barrier
+// falls back to agent-scope fences in every workgroup. All waits are bounded: a wait
+// that exceeds its bound sets `abort`, every waiting workgroup leaves, and the host
+// reports a device error, so every wave exits.
+struct BtGrid {
+    unsigned arrive, gen;
+    int abort, iters;
+    unsigned xarrive[BKH_NXCD], xgen[BKH_NXCD];
+    int bad_map, pad;
+};
+
+#define BT_SPIN_MAX (1u << 24)
+
+// Waits until *w >= target; false (and `abort` set) if the bound is exceeded or
+// another workgroup aborted.
+__device__ __forceinline__ bool bt_wait(BtGrid *g, unsigned *w, unsigned target) {
+    unsigned spins = 0;
+    while (__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        if (__hip_atomic_load(&g->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || ++spins > BT_SPIN_MAX) {
+            __hip_atomic_store(&g->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    return true;
+}
+
+// XCC this wave runs on (hardware register XCC_ID, bits 3:0).
+__device__ __forceinline__ int bt_xcc_id() {
+    return __builtin_amdgcn_s_getreg((3 << 11) | 20) & 0xF;
+}
+
+// Barrier sequence state of a workgroup: barriers passed, and the global and per-XCD
+// arrival counts at the end of the last one (every workgroup passes the same
+// sequence of barriers in the same modes, so all agree on them).
+struct BtSeq {
+    unsigned gen = 0, arrived = 0, xarrived = 0;
+};
+
+__device__ __forceinline__ bool bt_grid_sync(BtGrid *g, unsigned nwg, BtSeq &q, int *flag, bool hier) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned target = q.gen + 1;
+        if (!hier) {
+            __threadfence();
+            if (atomicAdd(&g->arrive, 1u) == q.arrived + nwg - 1)
+                __hip_atomic_store(&g->gen, target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            else
+                bt_wait(g, &g->gen, target);
+            __threadfence();
+        } else {
+            const int x = (int)blockIdx.x % BKH_NXCD;
+            const unsigned nx = nwg / BKH_NXCD;
+            __builtin_amdgcn_s_waitcnt(0);                 // this workgroup's stores are in L2
+            if (atomicAdd(&g->xarrive[x], 1u) == q.xarrived + nx - 1) {
+                __threadfence();                            // write the XCD's L2 back
+                if (atomicAdd(&g->arrive, 1u) == q.arrived + BKH_NXCD - 1)
+                    __hip_atomic_store(&g->gen, target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                else
+                    bt_wait(g, &g->gen, target);
+                __threadfence();                            // invalidate the XCD's L2 (and L1)
+                __hip_atomic_store(&g->xgen[x], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                bt_wait(g, &g->xgen[x], target);
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // acquire only: no L2 write-back
+            }
+        }
+        *flag = __hip_atomic_load(&g->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    q.gen++;
+    if (hier) {
+        q.arrived += BKH_NXCD;
+        q.xarrived += nwg / BKH_NXCD;
+    } else {
+        q.arrived += nwg;
+    }
+    return *flag == 0;
+}
+
+// Work item k of `count` for workgroup g of G (G a multiple of BKH_NXCD): the XCD's
+// contiguous share of [0, count), strided by its G / BKH_NXCD workgroups.
+#define BT_DEAL(k, count)                                                                    \
+    for (int x_ = (int)blockIdx.x % BKH_NXCD, per_ = ((count) + BKH_NXCD - 1) / BKH_NXCD,      \
+             k = x_ * per_ + (int)blockIdx.x / BKH_NXCD,                                     \
+             end_ = min((count), (x_ + 1) * per_);                                            \
+         k < end_; k += (int)gridDim.x / BKH_NXCD)
+
+template <typename VT, typename T>
+struct BtPArgs {
+    const int *rp, *ci, *wblk, *usys, *ub, *trow, *tsys, *tb;
+    const VT *val;
+    const T *b, *rs_in;
+    T *x, *R, *P, *rs;
+    BtSys *sy;
+    double *part;
+    BtGrid *grid;
+    long long N;
+    int nunit, ntile, ns, max_it;
+};
+
+template <int L, typename VT, typename T>
+__global__ __launch_bounds__(256) void k_bt_persist(BtPArgs<VT, T> a) {
+    constexpr int NG = (L + 1) * (L + 2) / 2;
+    __shared__ T prod_all[4][SPMV_WCAP];
+    __shared__ double sh[NG * 4];
+    __shared__ int flag, active;
+    const unsigned nwg = gridDim.x;
+    const long long N = a.N;
+    BtSeq seq;
+    Epi<0, T> e0;
+    Epi<1, T> e1;
+    Epi<NG, T> eg;
+    e1.w[0] = a.rs;
+    e1.a[0] = 0;
+    e1.b[0] = EPI_Y;
+    for (int k = 0; k < L; k++) eg.w[k] = a.R + k * N;
+    {
+        int idx = 0;
+        for (int p = 0; p <= L; p++)
+            for (int q = p; q <= L; q++) {
+                eg.a[idx] = (unsigned char)(p == L ? EPI_Y : p);
+                eg.b[idx] = (unsigned char)(q == L ? EPI_Y : q);
+                idx++;
+            }
+    }
+    if (threadIdx.x == 0 && bt_xcc_id() != (int)blockIdx.x % BKH_NXCD) atomicOr(&a.grid->bad_map, 1);
+    bool hier = false;
+#define BT_SYNC() if (!bt_grid_sync(a.grid, nwg, seq, &flag, hier)) return
+    // r0 = b - A x, p0 = rs = r0, then each system's initial scalars
+    BT_DEAL(u, a.nunit) bt_spmv_unit<VT, T, 0>(u, a.rp, a.ci, a.val, a.wblk, a.usys, a.sy, a.x, a.R, e0, nullptr,
+                                               prod_all, sh);
+    BT_SYNC();                                   // full fences: publishes bad_map too
+    hier = __hip_atomic_load(&a.grid->bad_map, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+    BT_DEAL(t, a.ntile) bt_init_tile<T>(t, a.trow, a.tsys, a.b, a.R, a.P, a.rs, a.sy, a.part, sh);
+    BT_SYNC();
+    BT_DEAL(q, a.ns) bt_fin_sys<BT_INIT, 2>(q, a.part, a.tb, a.sy, L, sh);
+    BT_SYNC();
+    int it = 0;
+    for (; it < a.max_it; it++) {
+        for (int j = 0; j < L; j++) {
+            BT_DEAL(t, a.ntile) bt_p_tile<T>(t, a.trow, a.tsys, N, j, a.P, a.R, a.sy);
+            BT_SYNC();
+            BT_DEAL(u, a.nunit) bt_spmv_unit<VT, T, 1>(u, a.rp, a.ci, a.val, a.wblk, a.usys, a.sy, a.P + j * N,
+                                                       a.P + (j + 1) * N, e1, a.part, prod_all, sh);
+            BT_SYNC();
+            BT_DEAL(q, a.ns) bt_fin_sys<BT_ALPHA, 1>(q, a.part, a.ub, a.sy, L, sh);
+            BT_SYNC();
+            BT_DEAL(t, a.ntile) bt_r_tile<T>(t, a.trow, a.tsys, N, j, a.P, a.R, a.x, a.sy);
+            BT_SYNC();
+            if (j < L - 1) {
+                BT_DEAL(u, a.nunit) bt_spmv_unit<VT, T, 1>(u, a.rp, a.ci, a.val, a.wblk, a.usys, a.sy, a.R + j * N,
+                                                           a.R + (j + 1) * N, e1, a.part, prod_all, sh);
+                BT_SYNC();
+                BT_DEAL(q, a.ns) bt_fin_sys<BT_GAMMA, 1>(q, a.part, a.ub, a.sy, L, sh);
+            } else {
+                BT_DEAL(u, a.nunit) bt_spmv_unit<VT, T, NG>(u, a.rp, a.ci, a.val, a.wblk, a.usys, a.sy,
+                                                            a.R + (L - 1) * N, a.R + L * N, eg, a.part, prod_all, sh);
+                BT_SYNC();
+                BT_DEAL(q, a.ns) bt_fin_sys<BT_MR, NG>(q, a.part, a.ub, a.sy, L, sh);
+            }
+            BT_SYNC();
+        }
+        BT_DEAL(t, a.ntile) bt_upd_tile<T>(t, a.trow, a.tsys, N, L, a.P, a.R, a.x, a.rs, a.sy, a.part, sh);
+        BT_SYNC();
+        BT_DEAL(q, a.ns) bt_fin_sys<BT_UPD, 2>(q, a.part, a.tb, a.sy, L, sh);
+        BT_SYNC();
+        // every workgroup reads the same settled flags, so all leave together
+        if (threadIdx.x == 0) active = 0;
+        __syncthreads();
+        int mine = 0;
+        for (int q = threadIdx.x; q < a.ns; q += 256) mine += a.sy[q].done ? 0 : 1;
+        if (mine) atomicAdd(&active, mine);
+        __syncthreads();
+        if (active == 0) { it++; break; }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.grid->iters = it;
+#undef BT_SYNC
 }
 
 // Stops the graph once no system is active.
@@ -311,20 +557,21 @@ __global__ void k_bt_sys_init(BtSys *sy, int ns, const double *tol_each, double 
 __global__ __launch_bounds__(256) void k_bt_resid(const int *trow, const int *tsys, const double *b, double *r,
                                                   double *part) {
     double v[2] = {0.0, 0.0};
-    TILE_ROWS(i) {
+    TILE_ROWS(i, blockIdx.x) {
         const double ri = b[i] - r[i];
         r[i] = ri;
         v[0] += ri * ri;
         v[1] += b[i] * b[i];
     }
     (void)sys;
-    unit_partial<2>(v, part, blockIdx.x);
+    __shared__ double sh[8];
+    unit_partial<2>(v, part, blockIdx.x, sh);
 }
 
 // r32 = r64 (scaled by 1/||r_s|| so every inner system starts at unit norm), d = 0.
 __global__ __launch_bounds__(256) void k_bt_ir_prep(const int *trow, const int *tsys, const double *r64,
                                                     float *r32, float *d, const double *scale) {
-    TILE_ROWS(i) {
+    TILE_ROWS(i, blockIdx.x) {
         r32[i] = (float)(r64[i] * scale[sys]);
         d[i] = 0.0f;
     }
@@ -333,7 +580,7 @@ __global__ __launch_bounds__(256) void k_bt_ir_prep(const int *trow, const int *
 // x += d / scale_s (FP64)
 __global__ __launch_bounds__(256) void k_bt_ir_update(const int *trow, const int *tsys, double *x, const float *d,
                                                       const double *scale) {
-    TILE_ROWS(i) x[i] += (double)d[i] / scale[sys];
+    TILE_ROWS(i, blockIdx.x) x[i] += (double)d[i] / scale[sys];
 }
 
 // --------------------------------------------------------------- building
@@ -524,10 +771,52 @@ static void bt_gram_spmv(hipStream_t s, const bkh_batch *B, const VT *val, BtSys
     bt_fin<BT_MR, NV>(s, B, part, true, sy, L);
 }
 
+static int g_bt_last_path = 0;      // 1 graph of per-phase launches, 2 persistent
+
+// The whole solve as one cooperative launch of k_bt_persist (see there). The grid is
+// what the device holds at once (occupancy x CUs), a multiple of the XCD count, and
+// no larger than the largest work list.
+template <int L, typename VT, typename T>
+static int bt_core_persist(ThreadCtx *t, const bkh_batch *B, const VT *val, const T *b, T *x, T *R, T *P, T *rs,
+                           BtSys *sy, double *part, BtGrid *grid, int max_it) {
+    hipStream_t s = t->stream;
+    static int G_res = [] {
+        int dev = 0, ncu = 0, per = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_bt_persist<L, VT, T>, 256, 0) != hipSuccess)
+            return 0;
+        return ncu * per;
+    }();
+    const int work = std::max(std::max(B->nunit, B->ntile), B->ns);
+    int G = std::min(G_res, (work + BKH_NXCD - 1) / BKH_NXCD * BKH_NXCD);
+    G -= G % BKH_NXCD;
+    if (G < BKH_NXCD) { bkh_set_error("persistent batched solve: no resident grid (occupancy %d)", G_res); return BKH_DEVICE_ERROR; }
+    HIP_OK(hipMemsetAsync(grid, 0, sizeof(BtGrid), s));
+    BtPArgs<VT, T> a;
+    a.rp = B->rp; a.ci = B->ci; a.wblk = B->wblk; a.usys = B->usys; a.ub = B->ub;
+    a.trow = B->trow; a.tsys = B->tsys; a.tb = B->tb;
+    a.val = val; a.b = b; a.rs_in = nullptr; a.x = x; a.R = R; a.P = P; a.rs = rs;
+    a.sy = sy; a.part = part; a.grid = grid; a.N = B->N;
+    a.nunit = B->nunit; a.ntile = B->ntile; a.ns = B->ns; a.max_it = max_it;
+    void *args[] = {&a};
+    const hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_bt_persist<L, VT, T>), dim3(G),
+                                                    dim3(256), args, 0, s);
+    if (e != hipSuccess) { bkh_set_error("cooperative launch of the batched solve: %s", hipGetErrorString(e)); return BKH_DEVICE_ERROR; }
+    BtGrid hg;
+    HIP_OK(hipMemcpyAsync(&hg, grid, sizeof hg, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    if (hg.abort) { bkh_set_error("persistent batched solve: grid barrier wait exceeded its bound"); return BKH_DEVICE_ERROR; }
+    g_bt_last_path = 2;
+    return 0;
+}
+
 // One batched BiCGSTAB(L) solve (all systems) on vectors T; sy initialised by caller.
 template <int L, typename VT, typename T>
 static int bt_core(ThreadCtx *t, const bkh_batch *B, const VT *val, const T *b, T *x, T *R, T *P, T *rs,
-                   BtSys *sy, Ctl *c, double *part, int max_it, int K) {
+                   BtSys *sy, Ctl *c, double *part, int max_it, int K, BtGrid *grid) {
+    if (grid) return bt_core_persist<L, VT, T>(t, B, val, b, x, R, P, rs, sy, part, grid, max_it);
+    g_bt_last_path = 1;
     hipStream_t s = t->stream;
     const long long N = B->N;
     if (ctl_init(t, c, 0.0, max_it)) return BKH_DEVICE_ERROR;
@@ -590,13 +879,20 @@ static int bt_solve(ThreadCtx *t, const bkh_batch *B, const double *b, double *x
     const bool ir = o.precision == BKH_FP32_IR;
     if (ir && !B->val32) { bkh_set_error("FP32_IR needs a batch created with want_fp32"); return BKH_BAD_ARGUMENT; }
     const size_t vbytes = ir ? sizeof(float) : sizeof(double);
-    size_t bytes = carve_size(sizeof(Ctl)) + carve_size(sizeof(BtSys) * ns) + carve_size(sizeof(double) * nparts) +
+    // path 0 / 1: one launch per phase in hipGraphs; 2: one persistent launch per solve.
+    // cfg4's 512 systems (14.8M rows, 251M nonzeros), BiCGSTAB(2) to 1e-8: graphs
+    // 173.9 ms FP64 and 146.7 ms FP32-IR, persistent 225.3 and 211.1 ms with the
+    // two-level barrier (467.0 and 307.0 ms with agent-scope fences in every
+    // workgroup); x is bitwise the same on both paths.
+    const bool persist = o.path == 2;
+    size_t bytes = carve_size(sizeof(Ctl)) + carve_size(sizeof(BtGrid)) + carve_size(sizeof(BtSys) * ns) + carve_size(sizeof(double) * nparts) +
                    carve_size(sizeof(double) * ns) + carve_size(sizeof(int) * ns) + carve_size(sizeof(double) * ns) +
                    carve_size(sizeof(double) * N) + (2 * (L + 1) + 3) * carve_size(vbytes * N);
     void *ws = bkh_workspace(bytes);
     if (!ws) return BKH_DEVICE_ERROR;
     Carve cv(ws);
     Ctl *c = cv.take<Ctl>(1);
+    BtGrid *grid = cv.take<BtGrid>(1);
     BtSys *sy = cv.take<BtSys>(ns);
     double *part = cv.take<double>(nparts);
     double *tol_d = cv.take<double>(ns);
@@ -611,7 +907,8 @@ static int bt_solve(ThreadCtx *t, const bkh_batch *B, const double *b, double *x
         double *R = cv.take<double>((L + 1) * N), *P = cv.take<double>((L + 1) * N), *rs = cv.take<double>(N);
         hipLaunchKernelGGL(k_bt_sys_init, dim3((ns + 255) / 256), dim3(256), 0, s, sy, ns, (const double *)nullptr,
                            tol, (const int *)nullptr, max_iter);
-        rc = bt_core<L, double, double>(t, B, B->val, b, x, R, P, rs, sy, c, part, max_iter, K);
+        rc = bt_core<L, double, double>(t, B, B->val, b, x, R, P, rs, sy, c, part, max_iter, K,
+                                        persist ? grid : nullptr);
         if (rc) return rc;
         HIP_OK(hipMemcpyAsync(hs.data(), sy, sizeof(BtSys) * ns, hipMemcpyDeviceToHost, s)); HIP_OK(hipStreamSynchronize(s));
         for (int q = 0; q < ns; q++) {
@@ -658,7 +955,8 @@ static int bt_solve(ThreadCtx *t, const bkh_batch *B, const double *b, double *x
                                r32, d32, (const double *)scale_d);
             hipLaunchKernelGGL(k_bt_sys_init, dim3((ns + 255) / 256), dim3(256), 0, s, sy, ns, (const double *)tol_d,
                                0.0, (const int *)frozen_d, max_iter);
-            rc = bt_core<L, float, float>(t, B, B->val32, r32, d32, R, P, rs, sy, c, part, max_iter, K);
+            rc = bt_core<L, float, float>(t, B, B->val32, r32, d32, R, P, rs, sy, c, part, max_iter, K,
+                                          persist ? grid : nullptr);
             if (rc) return rc;
             HIP_OK(hipMemcpyAsync(hs.data(), sy, sizeof(BtSys) * ns, hipMemcpyDeviceToHost, s)); HIP_OK(hipStreamSynchronize(s));
             for (int q = 0; q < ns; q++) {
@@ -720,6 +1018,8 @@ extern "C" int bkh_batch_bicgstabl(bkh_batch *B, const double *d_b, int l, doubl
     default: return bt_solve<4>(t, B, d_b, d_x, tol, max_iter, o, ret, true_res, info);
     }
 }
+
+extern "C" int bkh_batch_last_path(void) { return g_bt_last_path; }
 
 // y = A x over the whole batch (FP64, or FP32 values and vectors).
 extern "C" int bkh_batch_spmv(const bkh_batch *B, int fp32, const void *x, void *y) {

@@ -393,6 +393,11 @@ class Batch:
             raise BkspDeviceError(f"bkh_batch_bicgstabl -> {rc}: {last_error()}")
         return info, ret, tr
 
+    @staticmethod
+    def last_path():
+        """1: the last batched solve ran per-phase graphs; 2: one persistent launch."""
+        return load().bkh_batch_last_path()
+
 
 def pcg(M, kind, d_b, d_x, tol, max_iter, **kw):
     """Preconditioned CG: kind 0 = none, 1 = Jacobi."""

@@ -246,14 +246,20 @@ void bkh_batch_layout(const bkh_batch *B, int *rows_cap, int *nnz_cap, int *tile
 int bkh_batch_tune(bkh_batch *B, int fp32, double *best_ms);
 /* y = A x over the batch; x, y device vectors of N doubles (floats if fp32). */
 int bkh_batch_spmv(const bkh_batch *B, int fp32, const void *d_x, void *d_y);
-/* BiCGSTAB(l), 1 <= l <= 4, on every system at once (one launch per phase over the
- * whole batch). d_b, d_x: device vectors of N entries (system s at its offset); ret
- * (host, ns) receives each system's result as dbicgstabl returns it; true_res (host,
- * ns, may be NULL) each system's FP64 true residual. opts->precision = BKH_FP32_IR runs
- * FP32 inner solves inside per-system FP64 refinement (ret = inner iterations used). */
+/* BiCGSTAB(l), 1 <= l <= 4, on every system at once. opts->path: 0 or 1 = one launch
+ * per phase over the whole batch, replayed from hipGraphs (the faster path); 2 = the
+ * whole solve (every phase of every outer iteration of every system) as one
+ * persistent cooperative launch with grid barriers between phases (the same
+ * arithmetic: x is bitwise the same). d_b, d_x: device vectors of N entries (system s at its offset); ret (host,
+ * ns) receives each system's result as dbicgstabl returns it; true_res (host, ns, may
+ * be NULL) each system's FP64 true residual. opts->precision = BKH_FP32_IR runs FP32
+ * inner solves (one launch each) inside per-system FP64 refinement (ret = inner
+ * iterations used). */
 int bkh_batch_bicgstabl(bkh_batch *B, const double *d_b, int l, double *d_x, double tol,
                         int max_iter, const bkh_opts *opts, int *ret, double *true_res,
                         bkh_info *info);
+/* Path the last batched solve of the process ran: 1 per-phase graphs, 2 persistent. */
+int bkh_batch_last_path(void);
 
 /* ---- learned FSAI preconditioner and preconditioned CG (no reference counterpart:
  * the reference has no preconditioners) ----

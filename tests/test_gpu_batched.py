@@ -67,6 +67,26 @@ def test_batched_fp64_matches_oracle_per_system():
     assert info.ret == ret.max()
 
 
+@pytest.mark.parametrize("precision", [dv.BKH_FP64, dv.BKH_FP32_IR])
+def test_batched_persistent_equals_per_phase_graphs(precision):
+    """The persistent launch runs the per-phase kernels' arithmetic in the same order
+    (same units, tiles and partial rows), so x is bitwise the graph path's."""
+    systems = small_batch(5150)
+    B = dv.Batch(systems, fp32=True)
+    b = gen.normal(406, B.N)
+    db = dv.DeviceArray.from_numpy(b)
+    out = {}
+    for path in (1, 2, 0):
+        dx = dv.DeviceArray.from_numpy(np.zeros(B.N))
+        info, ret, tr = B.bicgstabl(db, 2, dx, 1e-9, 500, precision=precision, path=path)
+        assert dv.Batch.last_path() == (2 if path == 2 else 1), path
+        out[path] = (dx.numpy(), ret.copy(), tr.copy())
+    for path in (2, 0):
+        assert np.array_equal(out[path][0], out[1][0]), path
+        assert np.array_equal(out[path][1], out[1][1]), path
+    assert (out[2][1] >= 0).all() and out[2][2].max() < 1.5e-9
+
+
 def test_batched_fp32_refined_reaches_fp64_tolerance():
     systems = small_batch()
     B = dv.Batch(systems, fp32=True)
@@ -101,6 +121,12 @@ def test_cfg4_full_batch():
           f"max inner={info.inner_iters} max true res={tr.max():.2e} layout={B.layout()}")
     assert (ret >= 0).all()
     assert tr.max() < 1e-8
+    x_graph = dx.numpy()
+    dx.upload(np.zeros(B.N))
+    infop, retp, _ = B.bicgstabl(db, 2, dx, 1e-8, 1000, precision=dv.BKH_FP32_IR, path=2)
+    assert dv.Batch.last_path() == 2
+    print(f"cfg4 fp32ir persistent launch {infop.time_ms:.1f} ms")
+    assert np.array_equal(dx.numpy(), x_graph) and np.array_equal(retp, ret)
     dx.upload(np.zeros(B.N))
     info64, ret64, tr64 = B.bicgstabl(db, 2, dx, 1e-8, 1000)
     print(f"cfg4 fp64 batched {info64.time_ms:.1f} ms max outer={info64.iters} max true res={tr64.max():.2e}")
