@@ -1,4 +1,5 @@
-// batched.hip -- many independent systems in one launch per phase, with an autotuner.
+// batched.hip -- many independent systems in one launch per phase (or one persistent
+// launch), with an autotuner.
 //
 // The reference solves one system per call (src/bicgstabl.c:13-138); a batch of
 // uneven systems would be that many sequential solves, each a long chain of small
@@ -18,9 +19,13 @@
 // With BKH_FP32_IR the iteration runs in FP32 on a correction, inside per-system
 // FP64 iterative refinement (true residual b - A x in FP64 for every system at once).
 // bkh_batch_tune() times candidate SpMV blockings (rows per wavefront, nonzeros per
-// wavefront) and vector tile sizes on the batch itself and keeps the fastest.
+// wavefront from the batch's row-length histogram) and vector tile sizes on the batch
+// itself and keeps the fastest. With opts path 2 the whole solve is one persistent
+// cooperative launch instead (k_bt_persist), with the same arithmetic.
 #include <math.h>
 #include <algorithm>
+#include <array>
+#include <cmath>
 #include <vector>
 #include "bkh_host.hpp"
 #include "bkh_mr.hpp"
@@ -1042,8 +1047,9 @@ extern "C" int bkh_batch_spmv(const bkh_batch *B, int fp32, const void *x, void 
 }
 
 // Autotuner: times the batched SpMV for candidate blockings (rows per wavefront,
-// nonzeros per wavefront) and a vector kernel for candidate tile sizes, on this
-// batch, and keeps the fastest layout. fp32 selects the FP32 SpMV (the inner
+// nonzeros per wavefront, the latter derived from the batch's row-length histogram)
+// and a vector kernel for candidate tile sizes, on this batch, and keeps the fastest
+// layout. fp32 selects the FP32 SpMV (the inner
 // iteration of BKH_FP32_IR). best_ms receives the chosen SpMV's time.
 extern "C" int bkh_batch_tune(bkh_batch *B, int fp32, double *best_ms) {
     ThreadCtx *t = bkh_tctx();
@@ -1060,7 +1066,25 @@ extern "C" int bkh_batch_tune(bkh_batch *B, int fp32, double *best_ms) {
     HIP_OK(hipMemsetAsync(x, 0, vb * N, s));
     HIP_OK(hipMemsetAsync(z, 0, 2 * vb * N, s));
     HIP_OK(hipMemsetAsync(sy, 0, sizeof(BtSys) * B->ns, s));
-    const int cand[][2] = {{64, 512}, {64, 256}, {32, 512}, {32, 256}, {16, 256}, {64, 384}};
+    // SpMV blockings from the batch's row-length histogram: for rows per wavefront
+    // 16, 32 and 64, nonzero caps at that many rows of median, 90th-percentile and
+    // mean length (rounded up to 64, at least the longest row, at most SPMV_WCAP),
+    // and SPMV_WCAP itself
+    std::vector<int> lens(B->N);
+    long long tot = 0;
+    for (long long i = 0; i < N; i++) { lens[i] = B->rp_h[i + 1] - B->rp_h[i]; tot += lens[i]; }
+    std::vector<int> sorted_lens = lens;
+    std::sort(sorted_lens.begin(), sorted_lens.end());
+    const double p50 = N ? sorted_lens[N / 2] : 1, p90 = N ? sorted_lens[(N * 9) / 10] : 1;
+    const double mean = N ? (double)tot / N : 1;
+    std::vector<std::array<int, 2>> cand;
+    for (int rc : {16, 32, 64})
+        for (double per : {p50, p90, mean, (double)SPMV_WCAP / rc}) {
+            int cap = (int)std::ceil(rc * per / 64.0) * 64;
+            cap = std::min(SPMV_WCAP, std::max(cap, std::max(B->max_row, 64)));
+            const std::array<int, 2> c = {rc, cap};
+            if (std::find(cand.begin(), cand.end(), c) == cand.end()) cand.push_back(c);
+        }
     const int tiles[] = {512, 1024, 2048, 4096};
     const int reps = 5;
     double best = 1e300;
@@ -1084,7 +1108,7 @@ extern "C" int bkh_batch_tune(bkh_batch *B, int fp32, double *best_ms) {
         hipEventElapsedTime(&ms, t->ev0, t->ev1);
         return ms / reps;
     };
-    for (int q = 0; q < (int)(sizeof cand / sizeof cand[0]); q++) {
+    for (int q = 0; q < (int)cand.size(); q++) {
         if (bt_layout(B, cand[q][0], cand[q][1], B->tile_rows)) return BKH_DEVICE_ERROR;
         const double ms = time_spmv();
         if (ms < best) { best = ms; bi = q; }

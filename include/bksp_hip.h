@@ -241,8 +241,10 @@ long long bkh_batch_nnz(const bkh_batch *B);
 long long bkh_batch_offset(const bkh_batch *B, int s);  /* first row of system s */
 /* The SpMV blocking and vector tile size in use (set by bkh_batch_tune). */
 void bkh_batch_layout(const bkh_batch *B, int *rows_cap, int *nnz_cap, int *tile_rows);
-/* Autotuner: times candidate SpMV blockings (rows and nonzeros per wavefront) and
- * vector tile sizes on this batch (FP32 SpMV if fp32) and keeps the fastest. */
+/* Autotuner: times candidate SpMV blockings (16, 32 or 64 rows per wavefront, each
+ * with nonzero caps of that many median, 90th-percentile and mean-length rows of this
+ * batch, and 512) and vector tile sizes on this batch (FP32 SpMV if fp32) and keeps
+ * the fastest. */
 int bkh_batch_tune(bkh_batch *B, int fp32, double *best_ms);
 /* y = A x over the batch; x, y device vectors of N doubles (floats if fp32). */
 int bkh_batch_spmv(const bkh_batch *B, int fp32, const void *d_x, void *d_y);
