@@ -123,20 +123,25 @@ __global__ __launch_bounds__(256) void k_fsai_rows(int n, const int *rp, const i
     }
 }
 
-// Weights staged in LDS as BF16: W2T[o][i] and W2[i][o] (A operands of the
-// backward's transposes and of dH1 = W2 dZ2), and W2T with its input index permuted
+// Weights staged in LDS as BF16: W2[i][o] (the A operand of the backward's
+// dH1 = W2 dZ2), and W2T with its input index permuted
 // for the forward layer 2 on the K = 32 matrix-core form: lane group g of layer 1's
 // output holds, for row blocks m = 2q and 2q + 1, hidden units 16m + 4g + r (r < 4),
 // which are exactly the eight B-operand values of K-block q when K index 8g + j of
 // that block stands for unit 32q + 16(j / 4) + 4g + j % 4. w2p[o][32q + 8g + j] is W2
 // at that unit, so the activations stay in registers between the layers.
-// The forward-only kernel keeps w2p alone (8 KB of LDS, so occupancy is set by
-// registers); the arrays only the backward reads are one element there.
+// The forward-only kernel keeps w2p alone (9 KB of LDS, so occupancy is set by
+// registers); w2, which only the backward reads, is one element there. Rows are
+// padded (FS_P2 / FS_P shorts) so that the 16 rows one read instruction touches fall
+// in different LDS banks (an unpadded 128-byte row stride puts all 16 in one bank
+// group).
+#define FS_P2 (FS_H + 8)     // w2p row stride (16-byte reads)
+#define FS_P (FS_H + 4)      // w2 row stride (8-byte reads)
+#define FS_TS 20             // row stride of the backward's per-wave transpose tiles
 template <bool BWD>
 struct alignas(16) FsSmem {
-    short w2t[BWD ? FS_H * FS_H : 1];
-    short w2[BWD ? FS_H * FS_H : 1];
-    short w2p[FS_H * FS_H];
+    short w2p[FS_H * FS_P2];           // first: 16-byte aligned for its 16-byte reads
+    short w2[BWD ? FS_H * FS_P : 1];
 };
 
 __device__ __forceinline__ int fs_k32_unit(int kk) {
@@ -178,17 +183,13 @@ __global__ __launch_bounds__(256, BWD ? 1 : 4) void k_fsai_mlp(int n, long long 
                                                   const int *perm, double *gval, float *gval32, double *gtval,
                                                   float *gtval32, const double *gE, float *part) {
     __shared__ FsSmem<BWD> sm;
-    __shared__ __attribute__((aligned(16))) short lds_all[4][BWD ? 3 * FS_H * 16 + 16 * 16 : 1];
+    __shared__ __attribute__((aligned(16))) short lds_all[4][BWD ? 3 * FS_H * FS_TS + 16 * FS_TS : 1];
     const int L = threadIdx.x & 63, w = threadIdx.x >> 6, gq = L >> 4, el = L & 15;
     for (int e = threadIdx.x; e < FS_H * FS_H; e += 256) {
         const int i = e / FS_H, o = e % FS_H;
-        const short b = f2bf(params[FS_W2 + e]);
-        if constexpr (BWD) {
-            sm.w2[e] = b;                     // [i][o]
-            sm.w2t[o * FS_H + i] = b;         // [o][i]
-        }
+        if constexpr (BWD) sm.w2[i * FS_P + o] = f2bf(params[FS_W2 + e]);   // [i][o]
         const int o2 = e / FS_H, kk = e % FS_H;
-        sm.w2p[e] = f2bf(params[FS_W2 + fs_k32_unit(kk) * FS_H + o2]);   // [o][kk]
+        sm.w2p[o2 * FS_P2 + kk] = f2bf(params[FS_W2 + fs_k32_unit(kk) * FS_H + o2]);   // [o][kk]
     }
     __syncthreads();
     // layer-1 A operand: W1^T[16m + L%16][4gq + j] = W1[4gq + j][16m + L%16]
@@ -212,7 +213,7 @@ __global__ __launch_bounds__(256, BWD ? 1 : 4) void k_fsai_mlp(int n, long long 
         }
     const float b3 = params[FS_B3];
     short *lds = lds_all[w];
-    short *Sdz2 = lds, *Sa1 = lds + FS_H * 16, *Sdz1 = lds + 2 * FS_H * 16, *Sx = lds + 3 * FS_H * 16;
+    short *Sdz2 = lds, *Sa1 = lds + FS_H * FS_TS, *Sdz1 = lds + 2 * FS_H * FS_TS, *Sx = lds + 3 * FS_H * FS_TS;
     f4v gW2[4][4], gW1[4];
     float gw3[4][4], gb2[4][4], gb1[4][4], gb3 = 0.0f;
     if (BWD) {
@@ -263,7 +264,7 @@ __global__ __launch_bounds__(256, BWD ? 1 : 4) void k_fsai_mlp(int n, long long 
             f4v acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int q = 0; q < 2; q++) {
-                const s8v A2 = *(const s8v *)&sm.w2p[(16 * mo + el) * FS_H + 32 * q + 8 * gq];
+                const s8v A2 = *(const s8v *)&sm.w2p[(16 * mo + el) * FS_P2 + 32 * q + 8 * gq];
                 acc = mfma_bf16_k32(A2, B2k[q], acc);
             }
 #pragma unroll
@@ -315,7 +316,7 @@ __global__ __launch_bounds__(256, BWD ? 1 : 4) void k_fsai_mlp(int n, long long 
             f4v acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int mo = 0; mo < 4; mo++) {
-                const s4v A3 = *(const s4v *)&sm.w2[(16 * q + el) * FS_H + 16 * mo + 4 * gq];
+                const s4v A3 = *(const s4v *)&sm.w2[(16 * q + el) * FS_P + 16 * mo + 4 * gq];
                 acc = mfma_bf16(A3, Bdz[mo], acc);
             }
 #pragma unroll
@@ -330,28 +331,28 @@ __global__ __launch_bounds__(256, BWD ? 1 : 4) void k_fsai_mlp(int n, long long 
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int row = 16 * m + 4 * gq + r;
-                Sdz2[row * 16 + el] = Bdz[m][r];
-                Sa1[row * 16 + el] = B2[m][r];
-                Sdz1[row * 16 + el] = f2bf(dz1[m][r]);
+                Sdz2[row * FS_TS + el] = Bdz[m][r];
+                Sa1[row * FS_TS + el] = B2[m][r];
+                Sdz1[row * FS_TS + el] = f2bf(dz1[m][r]);
             }
 #pragma unroll
-        for (int q = 0; q < 4; q++) Sx[(4 * gq + q) * 16 + el] = xb[q];
+        for (int q = 0; q < 4; q++) Sx[(4 * gq + q) * FS_TS + el] = xb[q];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
         for (int mo = 0; mo < 4; mo++) {
-            const s4v Ad = *(const s4v *)&Sdz2[(16 * mo + el) * 16 + 4 * gq];
+            const s4v Ad = *(const s4v *)&Sdz2[(16 * mo + el) * FS_TS + 4 * gq];
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const s4v Bh = *(const s4v *)&Sa1[(16 * q + el) * 16 + 4 * gq];
+                const s4v Bh = *(const s4v *)&Sa1[(16 * q + el) * FS_TS + 4 * gq];
                 gW2[mo][q] = mfma_bf16(Ad, Bh, gW2[mo][q]);        // dW2^T[o][i]
             }
         }
-        const s4v Bx = *(const s4v *)&Sx[el * 16 + 4 * gq];
+        const s4v Bx = *(const s4v *)&Sx[el * FS_TS + 4 * gq];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const s4v Ad1 = *(const s4v *)&Sdz1[(16 * q + el) * 16 + 4 * gq];
+            const s4v Ad1 = *(const s4v *)&Sdz1[(16 * q + el) * FS_TS + 4 * gq];
             gW1[q] = mfma_bf16(Ad1, Bx, gW1[q]);                    // dW1^T[i][f]
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
