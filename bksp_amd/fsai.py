@@ -111,6 +111,7 @@ def solve_compare(model, seed, m, tol=1e-8, max_iter=200000, b_seed=gen.SEEDS["c
     run("jacobi", lambda: F.pcg_plain(1, db, dx, tol, max_iter))
     run("learned", lambda: F.pcg(db, dx, tol, max_iter))
     out["learned"]["factor_build_ms"] = build["ms"]
+    out["learned"]["edges"] = F.nnzl
     if plain:
         run("plain_cg", lambda: F.pcg_plain(0, db, dx, tol, max_iter))
     F.destroy()
