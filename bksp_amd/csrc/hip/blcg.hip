@@ -23,7 +23,7 @@
 #include "bkh_solver_common.hpp"
 
 struct BlkMats {
-    double C[SM_N], alpha[SM_N], W[SM_N], R1[SM_N], R1i[SM_N], R2i[SM_N], betaT[SM_N];
+    double C[SM_N], alpha[SM_N], W[SM_N], R1[SM_N], R1i[SM_N], R2i[SM_N], R12i[SM_N], betaT[SM_N];
     double work[2 * SM_N];
     double fnb2;
     int m, s;
@@ -134,6 +134,9 @@ __global__ __launch_bounds__(BS) void k_blk_fin(const double *part, int nblocks,
     sm_copy<W>(S, mt->R1, T1);
     sm_mul<W>(S, R, 0, T1, 0, T2);                  // T2 = beta = R2 R1
     sm_copy<W>(S, Ri, mt->R2i);
+    sm_copy<W>(S, mt->R1i, G);
+    sm_mul<W>(S, G, 0, Ri, 0, T1);                 // R1^-1 R2^-1 for the row-stream vp
+    sm_copy<W>(S, T1, mt->R12i);
     if (OP == FIN_QR2_INIT) {                     // R0 = V C  (blcg.c:58)
         sm_copy<W>(S, T2, mt->C);
         return;
@@ -268,8 +271,7 @@ __global__ __launch_bounds__(256) void k_blk_vr(long long m, T *V, const BlkMats
     gram_block_partial<S>(acc, sh, part);
 }
 
-// V = (V R1^-1) R2^-1 (pass 1's product recomputed, see k_blkr_vr); P = V + P beta^T
-// (or P = pscale V at the start).
+// V = V R2^-1 (k_blk_vr stored V R1^-1); P = V + P beta^T (or P = pscale V at the start).
 template <int S, typename T>
 __global__ __launch_bounds__(256) void k_blk_vp(long long m, T *V, T *P, const BlkMats *mt,
                                                 double pscale, const Ctl *c) {
@@ -458,9 +460,9 @@ __global__ __launch_bounds__(RP_BS) void k_blkr_xv(long long m, T *__restrict__ 
 }
 
 // CholeskyQR2 pass 1 without its store: the Gram of V R1^-1 (each row rounded to T as
-// a stored copy would be). k_blkr_vp recomputes V R1^-1 from the same V with the same
-// roundings before applying R2^-1, so the iterates are bitwise those of storing the
-// intermediate block, for one fewer write of V per iteration.
+// a stored copy would be). k_blkr_vp then applies the product R1^-1 R2^-1 (formed by
+// the pass-2 finalize) to the same V in one step, so the intermediate block is never
+// written and the panel kernel does one s x s product per row instead of two.
 template <int S, typename T>
 __global__ __launch_bounds__(RP_BS) void k_blkr_vr(long long m, const T *__restrict__ V,
                                                    const BlkMats *__restrict__ mt, const Ctl *c,
@@ -483,7 +485,8 @@ __global__ __launch_bounds__(RP_BS) void k_blkr_vr(long long m, const T *__restr
     row_gram_partial<S>(g, part);
 }
 
-// V = V R2^-1; P = V + P beta^T (or P = pscale V at the start).
+// V = V R1^-1 R2^-1 (V as k_blkr_vr read it); P = V + P beta^T (or P = pscale V at the
+// start).
 template <int S, typename T>
 __global__ __launch_bounds__(RP_BS) void k_blkr_vp(long long m, T *__restrict__ V, T *__restrict__ P,
                                                    const BlkMats *__restrict__ mt, double pscale, const Ctl *c) {
@@ -498,9 +501,7 @@ __global__ __launch_bounds__(RP_BS) void k_blkr_vp(long long m, T *__restrict__ 
         if (withp) rp.get(p);
         const long long nr = row + step;
         if (nr < m) { rv.load(V + nr * S); if (withp) rp.load(P + nr * S); }
-        double o1[S];
-        row_mul<S, T, true>(v, mt->R1i, o1);
-        row_mul<S, T, true>(o1, mt->R2i, o);
+        row_mul<S, T, true>(v, mt->R12i, o);
         row_store<S>(V + row * S, o);
         if (withp) {
             double d[S];
@@ -695,15 +696,13 @@ __global__ __launch_bounds__(RP_BS) void k_blkr2_vp(long long m, double *__restr
     RowRaw<S / 2, double> rv, rp;
     if (row < m) { rv.load(V + row * S + off); if (withp) rp.load(P + row * S + off); }
     for (; row < m; row += step) {
-        double v[S / 2], vf[S], o1[S / 2], o1f[S], o[S / 2], p[S / 2];
+        double v[S / 2], vf[S], o[S / 2], p[S / 2];
         rv.get(v);
         if (withp) rp.get(p);
         const long long nr = row + step;
         if (nr < m) { rv.load(V + nr * S + off); if (withp) rp.load(P + nr * S + off); }
         pair_full<S>(v, hi, vf);
-        pair_mul<S>(vf, mt->R1i, hi, false, o1);         // pass 1's V R1^-1, recomputed
-        pair_full<S>(o1, hi, o1f);
-        pair_mul<S>(o1f, mt->R2i, hi, false, o);
+        pair_mul<S>(vf, mt->R12i, hi, false, o);         // V R1^-1 R2^-1
         row_store<S / 2>(V + row * S + off, o);
         if (withp) {
             double pf[S], d[S / 2];
