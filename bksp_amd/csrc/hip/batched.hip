@@ -356,14 +356,14 @@ struct BtGrid {
     int bad_map, pad;
 };
 
-#define BT_SPIN_MAX (1u << 24)
+#define BT_SPIN_MAX (1u << 24)   // ~1 s of 128-cycle sleeps
 
-// Waits until *w >= target; false (and `abort` set) if the bound is exceeded or
-// another workgroup aborted.
-__device__ __forceinline__ bool bt_wait(BtGrid *g, unsigned *w, unsigned target) {
+// Waits until *w >= target; false (and `abort` set) if more than spin_max polls pass
+// or another workgroup aborted.
+__device__ __forceinline__ bool bt_wait(BtGrid *g, unsigned *w, unsigned target, unsigned spin_max) {
     unsigned spins = 0;
     while (__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-        if (__hip_atomic_load(&g->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || ++spins > BT_SPIN_MAX) {
+        if (__hip_atomic_load(&g->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || ++spins > spin_max) {
             __hip_atomic_store(&g->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return false;
         }
@@ -384,7 +384,8 @@ struct BtSeq {
     unsigned gen = 0, arrived = 0, xarrived = 0;
 };
 
-__device__ __forceinline__ bool bt_grid_sync(BtGrid *g, unsigned nwg, BtSeq &q, int *flag, bool hier) {
+__device__ __forceinline__ bool bt_grid_sync(BtGrid *g, unsigned nwg, BtSeq &q, int *flag, bool hier,
+                                             unsigned spin_max) {
     __syncthreads();
     if (threadIdx.x == 0) {
         const unsigned target = q.gen + 1;
@@ -393,7 +394,7 @@ __device__ __forceinline__ bool bt_grid_sync(BtGrid *g, unsigned nwg, BtSeq &q, 
             if (atomicAdd(&g->arrive, 1u) == q.arrived + nwg - 1)
                 __hip_atomic_store(&g->gen, target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             else
-                bt_wait(g, &g->gen, target);
+                bt_wait(g, &g->gen, target, spin_max);
             __threadfence();
         } else {
             const int x = (int)blockIdx.x % BKH_NXCD;
@@ -404,11 +405,11 @@ __device__ __forceinline__ bool bt_grid_sync(BtGrid *g, unsigned nwg, BtSeq &q, 
                 if (atomicAdd(&g->arrive, 1u) == q.arrived + BKH_NXCD - 1)
                     __hip_atomic_store(&g->gen, target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 else
-                    bt_wait(g, &g->gen, target);
+                    bt_wait(g, &g->gen, target, spin_max);
                 __threadfence();                            // invalidate the XCD's L2 (and L1)
                 __hip_atomic_store(&g->xgen[x], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             } else {
-                bt_wait(g, &g->xgen[x], target);
+                bt_wait(g, &g->xgen[x], target, spin_max);
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // acquire only: no L2 write-back
             }
         }
@@ -444,6 +445,7 @@ struct BtPArgs {
     BtGrid *grid;
     long long N;
     int nunit, ntile, ns, max_it;
+    unsigned spin_max;       // poll bound of every barrier wait
 };
 
 template <int L, typename VT, typename T>
@@ -473,7 +475,7 @@ __global__ __launch_bounds__(256) void k_bt_persist(BtPArgs<VT, T> a) {
     }
     if (threadIdx.x == 0 && bt_xcc_id() != (int)blockIdx.x % BKH_NXCD) atomicOr(&a.grid->bad_map, 1);
     bool hier = false;
-#define BT_SYNC() if (!bt_grid_sync(a.grid, nwg, seq, &flag, hier)) return
+#define BT_SYNC() if (!bt_grid_sync(a.grid, nwg, seq, &flag, hier, a.spin_max)) return
     // r0 = b - A x, p0 = rs = r0, then each system's initial scalars
     BT_DEAL(u, a.nunit) bt_spmv_unit<VT, T, 0>(u, a.rp, a.ci, a.val, a.wblk, a.usys, a.sy, a.x, a.R, e0, nullptr,
                                                prod_all, sh);
@@ -804,6 +806,10 @@ static int bt_core_persist(ThreadCtx *t, const bkh_batch *B, const VT *val, cons
     a.val = val; a.b = b; a.rs_in = nullptr; a.x = x; a.R = R; a.P = P; a.rs = rs;
     a.sy = sy; a.part = part; a.grid = grid; a.N = B->N;
     a.nunit = B->nunit; a.ntile = B->ntile; a.ns = B->ns; a.max_it = max_it;
+    // BKH_BT_SPIN lowers the barrier poll bound (0: a waiting workgroup gives up at once),
+    // so the abort path can be exercised from the host; every wave still exits
+    const char *sp = getenv("BKH_BT_SPIN");
+    a.spin_max = (sp && *sp) ? (unsigned)strtoul(sp, nullptr, 10) : BT_SPIN_MAX;
     void *args[] = {&a};
     const hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_bt_persist<L, VT, T>), dim3(G),
                                                     dim3(256), args, 0, s);

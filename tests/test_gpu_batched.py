@@ -87,6 +87,25 @@ def test_batched_persistent_equals_per_phase_graphs(precision):
     assert (out[2][1] >= 0).all() and out[2][2].max() < 1.5e-9
 
 
+def test_persistent_barrier_wait_bound_aborts_cleanly(monkeypatch):
+    """A zero poll bound (set from the host) makes every workgroup that has to wait at a
+    grid barrier give up at once: the launch ends, the call reports a device error, and
+    the next solve on the same device runs normally."""
+    systems = small_batch(6160)
+    B = dv.Batch(systems)
+    b = gen.normal(407, B.N)
+    db = dv.DeviceArray.from_numpy(b)
+    dx = dv.DeviceArray.from_numpy(np.zeros(B.N))
+    monkeypatch.setenv("BKH_BT_SPIN", "0")
+    with pytest.raises(dv.BkspDeviceError, match="grid barrier"):
+        B.bicgstabl(db, 2, dx, 1e-9, 500, path=2)
+    monkeypatch.delenv("BKH_BT_SPIN")
+    dx.upload(np.zeros(B.N))
+    info, ret, tr = B.bicgstabl(db, 2, dx, 1e-9, 500, path=2)
+    assert dv.Batch.last_path() == 2
+    assert (ret >= 0).all() and tr.max() < 1.5e-9
+
+
 def test_batched_fp32_refined_reaches_fp64_tolerance():
     systems = small_batch()
     B = dv.Batch(systems, fp32=True)
