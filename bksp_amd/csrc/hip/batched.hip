@@ -638,12 +638,12 @@ static int bt_layout(bkh_batch *B, int rows_cap, int nnz_cap, int tile_rows) {
     bt_free_layout(B);
     B->nunit = (int)us.size();
     B->ntile = (int)ts.size();
-    bool ok = hipMalloc(&B->wblk, sizeof(int) * wb.size()) == hipSuccess &&
-              hipMalloc(&B->usys, sizeof(int) * std::max<size_t>(us.size(), 1)) == hipSuccess &&
-              hipMalloc(&B->ub, sizeof(int) * ub.size()) == hipSuccess &&
-              hipMalloc(&B->trow, sizeof(int) * tr.size()) == hipSuccess &&
-              hipMalloc(&B->tsys, sizeof(int) * std::max<size_t>(ts.size(), 1)) == hipSuccess &&
-              hipMalloc(&B->tb, sizeof(int) * tb.size()) == hipSuccess;
+    bool ok = bkh_dmalloc(&B->wblk, sizeof(int) * wb.size()) == hipSuccess &&
+              bkh_dmalloc(&B->usys, sizeof(int) * std::max<size_t>(us.size(), 1)) == hipSuccess &&
+              bkh_dmalloc(&B->ub, sizeof(int) * ub.size()) == hipSuccess &&
+              bkh_dmalloc(&B->trow, sizeof(int) * tr.size()) == hipSuccess &&
+              bkh_dmalloc(&B->tsys, sizeof(int) * std::max<size_t>(ts.size(), 1)) == hipSuccess &&
+              bkh_dmalloc(&B->tb, sizeof(int) * tb.size()) == hipSuccess;
     ok = ok && hipMemcpy(B->wblk, wb.data(), sizeof(int) * wb.size(), hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(B->usys, us.data(), sizeof(int) * us.size(), hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(B->ub, ub.data(), sizeof(int) * ub.size(), hipMemcpyHostToDevice) == hipSuccess &&
@@ -695,10 +695,10 @@ extern "C" bkh_batch *bkh_batch_create(int ns, const int *n_each, const int *con
     }
     if (B->max_row > SPMV_WCAP) { bkh_set_error("batched SpMV supports rows of <= %d nonzeros", SPMV_WCAP); delete B; return nullptr; }
     const size_t nz = (size_t)B->nnz + 4;
-    bool ok = hipMalloc(&B->rp, sizeof(int) * (B->N + 1)) == hipSuccess &&
-              hipMalloc(&B->ci, sizeof(int) * nz) == hipSuccess &&
-              hipMalloc(&B->val, sizeof(double) * nz) == hipSuccess;
-    if (ok && want_fp32) ok = hipMalloc(&B->val32, sizeof(float) * nz) == hipSuccess;
+    bool ok = bkh_dmalloc(&B->rp, sizeof(int) * (B->N + 1)) == hipSuccess &&
+              bkh_dmalloc(&B->ci, sizeof(int) * nz) == hipSuccess &&
+              bkh_dmalloc(&B->val, sizeof(double) * nz) == hipSuccess;
+    if (ok && want_fp32) ok = bkh_dmalloc(&B->val32, sizeof(float) * nz) == hipSuccess;
     ok = ok && hipMemcpy(B->rp, B->rp_h.data(), sizeof(int) * (B->N + 1), hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(B->ci, ci.data(), sizeof(int) * nz, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(B->val, v.data(), sizeof(double) * nz, hipMemcpyHostToDevice) == hipSuccess;

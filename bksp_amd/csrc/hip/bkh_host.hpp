@@ -10,6 +10,20 @@
 #include <vector>
 #include <utility>
 #include "bkh_core.hpp"
+#include <atomic>
+
+// Every device allocation of the library goes through bkh_dmalloc, so a test can make
+// the n-th next one fail (bkh_debug_fail_alloc) and check that each caller turns the
+// failure into an error return.
+extern std::atomic<int> g_bkh_fail_alloc;
+template <typename T>
+inline hipError_t bkh_dmalloc(T **p, size_t bytes) {
+    if (g_bkh_fail_alloc.load(std::memory_order_relaxed) > 0 && g_bkh_fail_alloc.fetch_sub(1) == 1) {
+        *p = nullptr;
+        return hipErrorOutOfMemory;
+    }
+    return hipMalloc((void **)p, bytes);
+}
 #include "../../../include/bksp_hip.h"
 
 // Records a message retrievable with bkh_last_error() (thread-local).

@@ -96,7 +96,7 @@ extern "C" int bkh_blbicgstab(bkh_matrix *A, int s, const double *B, double *X, 
     const long long m = A->n, ms = m * (long long)s;
     hipStream_t st = t->stream;
     DevBuf base;
-    HIP_OK(hipMalloc((void **)&base.p, sizeof(double) * (size_t)(6 * ms + 2 * s * s)));
+    HIP_OK(bkh_dmalloc((void **)&base.p, sizeof(double) * (size_t)(6 * ms + 2 * s * s)));
     double *P = base.p, *R = P + ms, *Rs = R + ms, *V = Rs + ms, *T = V + ms, *W = T + ms;
     double *Md = W + ms;                                           // s x s coefficients
     const int G = vec_grid(ms, 256);

@@ -39,6 +39,11 @@ extern "C" int bkh_init(void) {
     return g_init_rc;
 }
 
+std::atomic<int> g_bkh_fail_alloc{0};
+
+extern "C" void bkh_debug_fail_alloc(int n) { g_bkh_fail_alloc.store(n > 0 ? n : 0); }
+extern "C" int bkh_debug_fail_alloc_pending(void) { return std::max(0, g_bkh_fail_alloc.load()); }
+
 ThreadCtx *bkh_tctx() {
     if (g_tctx) return g_tctx;
     if (bkh_init() != 0) return nullptr;
@@ -47,7 +52,7 @@ ThreadCtx *bkh_tctx() {
     if (hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking) != hipSuccess ||
         hipHostMalloc((void **)&t->ctl_host, sizeof(Ctl), hipHostMallocDefault) != hipSuccess ||
         hipEventCreate(&t->ev0) != hipSuccess || hipEventCreate(&t->ev1) != hipSuccess ||
-        hipMalloc((void **)&t->red2, sizeof(double) * RED2_CAP) != hipSuccess) {
+        bkh_dmalloc((void **)&t->red2, sizeof(double) * RED2_CAP) != hipSuccess) {
         bkh_set_error("failed to create the thread's HIP stream/events");
         delete t;
         return nullptr;
@@ -94,7 +99,7 @@ void *bkh_workspace(size_t bytes) {
         t->ws_bytes = 0;
     }
     size_t want = bytes + bytes / 4;     // headroom: a regrowth invalidates cached graphs
-    if (hipMalloc(&t->ws, want) != hipSuccess) {
+    if (bkh_dmalloc(&t->ws, want) != hipSuccess) {
         bkh_set_error("workspace allocation of %zu bytes failed", want);
         t->ws = nullptr;
         return nullptr;
@@ -156,8 +161,8 @@ extern "C" int bkh_sync(void) {
 extern "C" void *bkh_malloc(size_t bytes) {
     if (!bkh_tctx()) return nullptr;
     void *p = nullptr;
-    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) {
-        bkh_set_error("hipMalloc(%zu) failed", bytes);
+    if (bkh_dmalloc(&p, bytes ? bytes : 1) != hipSuccess) {
+        bkh_set_error("bkh_dmalloc(%zu) failed", bytes);
         return nullptr;
     }
     return p;
@@ -416,11 +421,11 @@ static bool upload_sell(bkh_matrix *A, int n, const std::vector<int> &rp, const 
             }
         }
     const size_t ne = (size_t)tot + 4;
-    bool ok = hipMalloc(&A->sso, sizeof(int) * (nsl + 1)) == hipSuccess &&
-              hipMalloc(&A->sval, sizeof(double) * ne) == hipSuccess &&
-              hipMalloc(&A->scol, sizeof(int) * ne) == hipSuccess;
-    if (ok && fits) ok = hipMalloc(&A->scd, sizeof(short) * ne) == hipSuccess;
-    if (ok && want_fp32) ok = hipMalloc(&A->sval32, sizeof(float) * ne) == hipSuccess;
+    bool ok = bkh_dmalloc(&A->sso, sizeof(int) * (nsl + 1)) == hipSuccess &&
+              bkh_dmalloc(&A->sval, sizeof(double) * ne) == hipSuccess &&
+              bkh_dmalloc(&A->scol, sizeof(int) * ne) == hipSuccess;
+    if (ok && fits) ok = bkh_dmalloc(&A->scd, sizeof(short) * ne) == hipSuccess;
+    if (ok && want_fp32) ok = bkh_dmalloc(&A->sval32, sizeof(float) * ne) == hipSuccess;
     if (!ok) return false;
     ok = hipMemcpyAsync(A->sso, so.data(), sizeof(int) * (nsl + 1), hipMemcpyHostToDevice, s) == hipSuccess &&
          hipMemcpyAsync(A->sval, sv.data(), sizeof(double) * ne, hipMemcpyHostToDevice, s) == hipSuccess &&
@@ -479,10 +484,10 @@ static bool upload_dia(bkh_matrix *A, int n, const std::vector<int> &rp, const s
         }
     }
     const size_t ne = (size_t)tot + 4;
-    bool ok = hipMalloc(&A->dso, sizeof(int) * (nsl + 1)) == hipSuccess &&
-              hipMalloc(&A->dval, sizeof(double) * ne) == hipSuccess &&
-              hipMalloc(&A->doff, sizeof(int) * (offs.size() + 8)) == hipSuccess;
-    if (ok && want_fp32) ok = hipMalloc(&A->dval32, sizeof(float) * ne) == hipSuccess;
+    bool ok = bkh_dmalloc(&A->dso, sizeof(int) * (nsl + 1)) == hipSuccess &&
+              bkh_dmalloc(&A->dval, sizeof(double) * ne) == hipSuccess &&
+              bkh_dmalloc(&A->doff, sizeof(int) * (offs.size() + 8)) == hipSuccess;
+    if (ok && want_fp32) ok = bkh_dmalloc(&A->dval32, sizeof(float) * ne) == hipSuccess;
     if (!ok) return false;
     offs.resize(offs.size() + 8, 0);                              // read-ahead padding of the kernels
     ok = hipMemcpyAsync(A->dso, so.data(), sizeof(int) * (nsl + 1), hipMemcpyHostToDevice, s) == hipSuccess &&
@@ -524,13 +529,13 @@ static bkh_matrix *upload(int n, const std::vector<int> &rp, const std::vector<i
     // +4 elements: the SpMV's 16-byte vector loads start at an aligned index and
     // may cover up to 3 entries past the last nonzero (masked, never used).
     size_t nz = (size_t)(A->nnz > 0 ? A->nnz : 1) + 4;
-    bool ok = hipMalloc(&A->rp, sizeof(int) * (n + 1)) == hipSuccess &&
-              hipMalloc(&A->ci, sizeof(int) * nz) == hipSuccess &&
-              hipMalloc(&A->val, sizeof(double) * nz) == hipSuccess &&
-              hipMalloc(&A->diag, sizeof(double) * n) == hipSuccess &&
-              hipMalloc(&A->blk, sizeof(int) * blk.size()) == hipSuccess &&
-              hipMalloc(&A->wblk, sizeof(int) * wblk.size()) == hipSuccess;
-    if (ok && want_fp32) ok = hipMalloc(&A->val32, sizeof(float) * nz) == hipSuccess;
+    bool ok = bkh_dmalloc(&A->rp, sizeof(int) * (n + 1)) == hipSuccess &&
+              bkh_dmalloc(&A->ci, sizeof(int) * nz) == hipSuccess &&
+              bkh_dmalloc(&A->val, sizeof(double) * nz) == hipSuccess &&
+              bkh_dmalloc(&A->diag, sizeof(double) * n) == hipSuccess &&
+              bkh_dmalloc(&A->blk, sizeof(int) * blk.size()) == hipSuccess &&
+              bkh_dmalloc(&A->wblk, sizeof(int) * wblk.size()) == hipSuccess;
+    if (ok && want_fp32) ok = bkh_dmalloc(&A->val32, sizeof(float) * nz) == hipSuccess;
     if (!ok) {
         bkh_set_error("device allocation for a %d-row matrix failed", n);
         free_matrix(A);
@@ -558,7 +563,7 @@ static bkh_matrix *upload(int n, const std::vector<int> &rp, const std::vector<i
                 }
         }
         if (fits && A->max_row <= SPMV_WCAP) {
-            ok = hipMalloc(&A->cd16, sizeof(short) * nz) == hipSuccess &&
+            ok = bkh_dmalloc(&A->cd16, sizeof(short) * nz) == hipSuccess &&
                  hipMemcpyAsync(A->cd16, cd.data(), sizeof(short) * A->nnz, hipMemcpyHostToDevice, s) == hipSuccess &&
                  hipStreamSynchronize(s) == hipSuccess;
         }
@@ -751,7 +756,7 @@ int true_residual(ThreadCtx *t, const bkh_matrix *A, double sigma, const double 
     void *mem = nullptr;
     size_t bytes = carve_size(sizeof(Ctl)) + carve_size(sizeof(double) * G * 2) +
                    carve_size(sizeof(double) * A->n);
-    HIP_OK(hipMalloc(&mem, bytes));
+    HIP_OK(bkh_dmalloc(&mem, bytes));
     Carve cv(mem);
     Ctl *c = cv.take<Ctl>(1);
     double *part = cv.take<double>((size_t)G * 2);
@@ -900,7 +905,7 @@ extern "C" int bkh_sp_norm_f(int n, const int *row_ptr, const int *col, const do
     void *mem = nullptr;
     size_t bytes = carve_size(sizeof(Ctl)) + carve_size(sizeof(double) * G) + carve_size(sizeof(int) * (n + 1)) +
                    carve_size(sizeof(int) * (nnz + 1)) + carve_size(sizeof(double) * (nnz + 1));
-    HIP_OK(hipMalloc(&mem, bytes));
+    HIP_OK(bkh_dmalloc(&mem, bytes));
     Carve cv(mem);
     Ctl *c = cv.take<Ctl>(1);
     double *part = cv.take<double>(G);

@@ -466,11 +466,11 @@ extern "C" bkh_fsai_model *bkh_fsai_model_create(const float *init) {
     ThreadCtx *t = bkh_tctx();
     if (!t) return nullptr;
     bkh_fsai_model *M = new bkh_fsai_model;
-    bool ok = hipMalloc(&M->params, sizeof(float) * FS_NP) == hipSuccess &&
-              hipMalloc(&M->m, sizeof(float) * FS_NP) == hipSuccess &&
-              hipMalloc(&M->v, sizeof(float) * FS_NP) == hipSuccess &&
-              hipMalloc(&M->grad, sizeof(double) * FS_NP) == hipSuccess &&
-              hipMalloc(&M->part, sizeof(float) * FS_NP * FS_WAVES) == hipSuccess;
+    bool ok = bkh_dmalloc(&M->params, sizeof(float) * FS_NP) == hipSuccess &&
+              bkh_dmalloc(&M->m, sizeof(float) * FS_NP) == hipSuccess &&
+              bkh_dmalloc(&M->v, sizeof(float) * FS_NP) == hipSuccess &&
+              bkh_dmalloc(&M->grad, sizeof(double) * FS_NP) == hipSuccess &&
+              bkh_dmalloc(&M->part, sizeof(float) * FS_NP * FS_WAVES) == hipSuccess;
     std::vector<float> h(FS_NP, 0.0f);
     if (init) for (int k = 0; k < FS_NP; k++) h[k] = init[k];
     ok = ok && hipMemcpy(M->params, h.data(), sizeof(float) * FS_NP, hipMemcpyHostToDevice) == hipSuccess &&
@@ -559,17 +559,17 @@ extern "C" bkh_fsai_field *bkh_fsai_field_create(int n, const int *rpl, const in
     for (int i = 0; i < n; i++)                            // G^T rows in the 't' order of the upload
         for (int k = rpl[i]; k < rpl[i + 1]; k++) perm[k] = cnt[cil[k]]++;
     const size_t np = (size_t)std::max(P, 1) * n;
-    bool ok = hipMalloc(&F->rpl, sizeof(int) * (n + 1)) == hipSuccess &&
-              hipMalloc(&F->cil, sizeof(int) * nnzl) == hipSuccess &&
-              hipMalloc(&F->erow, sizeof(int) * nnzl) == hipSuccess &&
-              hipMalloc(&F->perm, sizeof(int) * nnzl) == hipSuccess &&
-              hipMalloc(&F->vl, sizeof(double) * nnzl) == hipSuccess &&
-              hipMalloc(&F->rows, sizeof(double4) * n) == hipSuccess &&
-              hipMalloc(&F->rowx, sizeof(double2) * n) == hipSuccess &&
-              hipMalloc(&F->Z, sizeof(double) * np) == hipSuccess &&
-              hipMalloc(&F->W, sizeof(double) * np) == hipSuccess &&
-              hipMalloc(&F->U, sizeof(double) * np) == hipSuccess &&
-              hipMalloc(&F->gE, sizeof(double) * nnzl) == hipSuccess;
+    bool ok = bkh_dmalloc(&F->rpl, sizeof(int) * (n + 1)) == hipSuccess &&
+              bkh_dmalloc(&F->cil, sizeof(int) * nnzl) == hipSuccess &&
+              bkh_dmalloc(&F->erow, sizeof(int) * nnzl) == hipSuccess &&
+              bkh_dmalloc(&F->perm, sizeof(int) * nnzl) == hipSuccess &&
+              bkh_dmalloc(&F->vl, sizeof(double) * nnzl) == hipSuccess &&
+              bkh_dmalloc(&F->rows, sizeof(double4) * n) == hipSuccess &&
+              bkh_dmalloc(&F->rowx, sizeof(double2) * n) == hipSuccess &&
+              bkh_dmalloc(&F->Z, sizeof(double) * np) == hipSuccess &&
+              bkh_dmalloc(&F->W, sizeof(double) * np) == hipSuccess &&
+              bkh_dmalloc(&F->U, sizeof(double) * np) == hipSuccess &&
+              bkh_dmalloc(&F->gE, sizeof(double) * nnzl) == hipSuccess;
     hipStream_t s = t->stream;
     ok = ok && hipMemcpyAsync(F->rpl, rpl, sizeof(int) * (n + 1), hipMemcpyHostToDevice, s) == hipSuccess &&
          hipMemcpyAsync(F->cil, cil, sizeof(int) * nnzl, hipMemcpyHostToDevice, s) == hipSuccess &&

@@ -121,9 +121,9 @@ static bool zdia_upload(ZSell &s, int n, const std::vector<int> &rp, const std::
         }
     }
     offs.push_back(0);
-    if (hipMalloc(&s.dso, so.size() * sizeof(int)) != hipSuccess ||
-        hipMalloc(&s.doff, offs.size() * sizeof(int)) != hipSuccess ||
-        hipMalloc(&s.dval, dv.size() * sizeof(zd)) != hipSuccess) {
+    if (bkh_dmalloc(&s.dso, so.size() * sizeof(int)) != hipSuccess ||
+        bkh_dmalloc(&s.doff, offs.size() * sizeof(int)) != hipSuccess ||
+        bkh_dmalloc(&s.dval, dv.size() * sizeof(zd)) != hipSuccess) {
         bkh_set_error("hipMalloc failed for a complex matrix");
         return false;
     }
@@ -162,10 +162,10 @@ static bool zsell_upload(ZSell &s, int n, const std::vector<int> &rp, const std:
             }
         }
     s.nsl = nsl;
-    if (hipMalloc(&s.so, so.size() * sizeof(int)) != hipSuccess ||
-        hipMalloc(&s.rl, rl.size() * sizeof(int)) != hipSuccess ||
-        hipMalloc(&s.col, col.size() * sizeof(int)) != hipSuccess ||
-        hipMalloc(&s.val, val.size() * sizeof(zd)) != hipSuccess) {
+    if (bkh_dmalloc(&s.so, so.size() * sizeof(int)) != hipSuccess ||
+        bkh_dmalloc(&s.rl, rl.size() * sizeof(int)) != hipSuccess ||
+        bkh_dmalloc(&s.col, col.size() * sizeof(int)) != hipSuccess ||
+        bkh_dmalloc(&s.val, val.size() * sizeof(zd)) != hipSuccess) {
         bkh_set_error("hipMalloc failed for a complex matrix");
         return false;
     }

@@ -228,6 +228,13 @@ int bkh_zqr(int m, int s, double *d_V, double *R);
 /* Average ms of reps back-to-back complex SpMV launches (op 'n' or 't'), HIP events. */
 int bkh_zspmv_bench(const bkh_zmatrix *A, char op, int reps, double *avg_ms);
 
+/* Test hook (no reference counterpart): the n-th device allocation the library makes
+ * from now on fails as if HBM were exhausted (n <= 0 disarms); every entry point then
+ * returns its device-error code. bkh_debug_fail_alloc_pending() is the number of
+ * allocations still to go before it (0 once it has fired, or when disarmed). */
+void bkh_debug_fail_alloc(int n);
+int bkh_debug_fail_alloc_pending(void);
+
 /* ---- batches of independent systems (no reference counterpart: the reference
  * solves one system per call, so a batch is that many sequential dbicgstabl calls) */
 typedef struct bkh_batch bkh_batch;
