@@ -82,6 +82,7 @@ _SIGS = {
     "bkh_zmatrix_create": (c_vp, [c_int, c_vp, c_vp, c_vp]),
     "bkh_zmatrix_destroy": (None, [c_vp]),
     "bkh_zmatrix_dia_entries": (c_ll, [c_vp]),
+    "bkh_zmatrix_split_rows": (c_ll, [c_vp]),
     "bkh_zspmv": (c_int, [c_vp, ctypes.c_char, c_vp, c_vp]),
     "bkh_zspmm1": (c_int, [c_vp, ctypes.c_char, c_vp, c_int, c_vp, c_vp, c_vp]),
     "bkh_zbicg": (c_int, [c_vp, c_vp, c_vp, c_dbl, c_int, c_vp]),

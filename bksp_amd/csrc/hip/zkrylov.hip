@@ -58,11 +58,19 @@ __device__ __forceinline__ void zstore(double *s, int i, zd v) { s[2 * i] = v.x;
 // ------------------------------------------------------------------ matrix
 
 // Row-sliced copy of a CSR matrix: slices of 64 rows, entry k of the slice's row l
-// at so[slice] + 64 k + l, padded to the slice's longest row; rl[row] = row length.
+// at so[slice] + 64 k + l. A slice is as wide as its longest row, but at most
+// max(ZS_MIN_W, twice its mean row length), so padding stays within 2 nnz + 64 ZS_MIN_W
+// per slice; rl[row] = the row's entries held there (min(length, width)). A row longer
+// than its slice's width keeps the rest in a CSR tail: row trow[t] continues with
+// entries tp[t] .. tp[t+1]-1 of tcol / tval, in stored order.
+#define ZS_MIN_W 16
 struct ZSell {
     int nsl = 0;
     int *so = nullptr, *rl = nullptr, *col = nullptr;
     zd *val = nullptr;
+    int ntail = 0;
+    int *trow = nullptr, *tp = nullptr, *tcol = nullptr;
+    zd *tval = nullptr;
     // slice-diagonal copy (banded, ascending rows): slice offsets, per-slice column
     // offsets (position k of slice q at dso[q] / 64 + k), values (absent entries 0)
     long long dia = 0;
@@ -79,6 +87,7 @@ struct bkh_zmatrix {
 static void zsell_free(ZSell &s) {
     hipFree(s.so); hipFree(s.rl); hipFree(s.col); hipFree(s.val);
     hipFree(s.dso); hipFree(s.doff); hipFree(s.dval);
+    hipFree(s.trow); hipFree(s.tp); hipFree(s.tcol); hipFree(s.tval);
     s = ZSell();
 }
 
@@ -139,12 +148,23 @@ static bool zsell_upload(ZSell &s, int n, const std::vector<int> &rp, const std:
                          const std::vector<zd> &v) {
     const int nsl = (n + 63) / 64;
     std::vector<int> so((size_t)nsl + 1, 0), rl((size_t)std::max(n, 1), 0);
+    std::vector<int> trow, tp(1, 0), tcol;
+    std::vector<zd> tval;
     long long tot = 0;
     for (int q = 0; q < nsl; q++) {
+        const int r0 = q * 64, r1 = std::min(n, q * 64 + 64);
         int w = 0;
-        for (int r = q * 64; r < std::min(n, q * 64 + 64); r++) {
-            rl[r] = rp[r + 1] - rp[r];
-            w = std::max(w, rl[r]);
+        for (int r = r0; r < r1; r++) w = std::max(w, rp[r + 1] - rp[r]);
+        const long long wcap = std::max<long long>(ZS_MIN_W, (2LL * (rp[r1] - rp[r0]) + 63) / 64);
+        w = (int)std::min<long long>(w, wcap);
+        for (int r = r0; r < r1; r++) {
+            const int len = rp[r + 1] - rp[r];
+            rl[r] = std::min(len, w);
+            if (len > w) {
+                trow.push_back(r);
+                for (int k = rp[r] + w; k < rp[r + 1]; k++) { tcol.push_back(ci[k]); tval.push_back(v[k]); }
+                tp.push_back((int)tcol.size());
+            }
         }
         tot += 64LL * w;
         if (tot >= (1LL << 31) - 64) { bkh_set_error("complex matrix too large for 32-bit slice offsets"); return false; }
@@ -174,6 +194,21 @@ static bool zsell_upload(ZSell &s, int n, const std::vector<int> &rp, const std:
           hipMemcpy(s.col, col.data(), col.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
           hipMemcpy(s.val, val.data(), val.size() * sizeof(zd), hipMemcpyHostToDevice) == hipSuccess))
         return false;
+    if (!trow.empty()) {
+        if (bkh_dmalloc(&s.trow, trow.size() * sizeof(int)) != hipSuccess ||
+            bkh_dmalloc(&s.tp, tp.size() * sizeof(int)) != hipSuccess ||
+            bkh_dmalloc(&s.tcol, tcol.size() * sizeof(int)) != hipSuccess ||
+            bkh_dmalloc(&s.tval, tval.size() * sizeof(zd)) != hipSuccess) {
+            bkh_set_error("hipMalloc failed for a complex matrix");
+            return false;
+        }
+        if (!(hipMemcpy(s.trow, trow.data(), trow.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(s.tp, tp.data(), tp.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(s.tcol, tcol.data(), tcol.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(s.tval, tval.data(), tval.size() * sizeof(zd), hipMemcpyHostToDevice) == hipSuccess))
+            return false;
+        s.ntail = (int)trow.size();
+    }
     return zdia_upload(s, n, rp, ci, v);
 }
 
@@ -212,6 +247,8 @@ extern "C" bkh_zmatrix *bkh_zmatrix_create(int n, const int *row_ptr, const int 
 }
 
 extern "C" long long bkh_zmatrix_dia_entries(const bkh_zmatrix *A) { return A ? A->A.dia + A->AT.dia : -1; }
+
+extern "C" long long bkh_zmatrix_split_rows(const bkh_zmatrix *A) { return A ? A->A.ntail + A->AT.ntail : -1; }
 
 extern "C" void bkh_zmatrix_destroy(bkh_zmatrix *A) {
     if (!A) return;
@@ -256,6 +293,45 @@ __global__ __launch_bounds__(256) void k_zspmv(int n, const int *__restrict__ so
     y[row] = acc;
 }
 
+// The CSR tails of rows longer than their slice's SELL width, lane per such row: the
+// sum the SELL kernel left in Y continues over the tail in stored order, so every row is
+// summed entry by entry exactly as without the split. MM: the zcsrgemm1 form, each
+// entry scaled by alpha before the product, for s columns.
+template <bool CONJ, bool MM>
+__global__ __launch_bounds__(256) void k_zsp_tail(int nt, int n, int s, const int *__restrict__ trow,
+                                                  const int *__restrict__ tp, const int *__restrict__ tcol,
+                                                  const zd *__restrict__ tval, zd alpha, const zd *X, zd *Y,
+                                                  const Ctl *c) {
+    if (c && ctl_stopped(c)) return;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= nt) return;
+    const int row = trow[t], k0 = tp[t], k1 = tp[t + 1];
+    for (int j = 0; j < s; j++) {
+        const zd *x = X + (size_t)j * n;
+        zd acc = Y[(size_t)j * n + row];
+        for (int k = k0; k < k1; k++) {
+            zd a = tval[k];
+            if (CONJ) a = zconj(a);
+            if (MM) a = zmul(alpha, a);
+            acc = zadd(acc, zmul(a, x[tcol[k]]));
+        }
+        Y[(size_t)j * n + row] = acc;
+    }
+}
+
+template <bool MM>
+static void launch_zsp_tail(hipStream_t st, const ZSell &M, bool herm, int n, int s, zd alpha, const zd *X, zd *Y,
+                            const Ctl *c) {
+    if (M.ntail == 0) return;
+    const int G = (M.ntail + 255) / 256;
+    if (herm)
+        hipLaunchKernelGGL((k_zsp_tail<true, MM>), dim3(G), dim3(256), 0, st, M.ntail, n, s, M.trow, M.tp, M.tcol,
+                           M.tval, alpha, X, Y, c);
+    else
+        hipLaunchKernelGGL((k_zsp_tail<false, MM>), dim3(G), dim3(256), 0, st, M.ntail, n, s, M.trow, M.tp,
+                           M.tcol, M.tval, alpha, X, Y, c);
+}
+
 // dia: the solvers' own products (finite x) run on the slice-diagonal copy when the
 // matrix has one; the public entry points keep SELL-64.
 static void launch_zspmv(hipStream_t s, const bkh_zmatrix *A, bool herm, const zd *x, zd *y, const Ctl *c,
@@ -275,6 +351,7 @@ static void launch_zspmv(hipStream_t s, const bkh_zmatrix *A, bool herm, const z
         hipLaunchKernelGGL((k_zspmv<true>), dim3(G), dim3(256), 0, s, A->n, M.so, M.rl, M.col, M.val, x, y, c);
     else
         hipLaunchKernelGGL((k_zspmv<false>), dim3(G), dim3(256), 0, s, A->n, M.so, M.rl, M.col, M.val, x, y, c);
+    launch_zsp_tail<false>(s, M, herm, A->n, 1, zmk(1.0, 0.0), x, y, c);
 }
 
 // Y = beta Y + alpha op(A) X for s columns (column-major, ld n), the reference's
@@ -324,6 +401,7 @@ extern "C" int bkh_zspmm1(const bkh_zmatrix *A, char op, const double *alpha, in
     else
         hipLaunchKernelGGL((k_zspmm1<false>), dim3(G), dim3(256), 0, t->stream, A->n, s, M.so, M.rl, M.col, M.val, al,
                            be, (const zd *)d_X, (zd *)d_Y);
+    launch_zsp_tail<true>(t->stream, M, op == 't', A->n, s, al, (const zd *)d_X, (zd *)d_Y, nullptr);
     LAUNCH_OK();
     HIP_OK(hipStreamSynchronize(t->stream));
     return 0;
@@ -1483,6 +1561,7 @@ static void launch_zspmm1(hipStream_t st, const bkh_zmatrix *A, int s, zd alpha,
     const int G = (A->n + 255) / 256;
     hipLaunchKernelGGL((k_zspmm1<false>), dim3(G), dim3(256), 0, st, A->n, s, A->A.so, A->A.rl, A->A.col, A->A.val,
                        alpha, beta, X, Y);
+    launch_zsp_tail<true>(st, A->A, false, A->n, s, alpha, X, Y, nullptr);
 }
 
 // CholeskyQR2 of V (m x s) in place; op2 = ZB_QR2_INIT or ZB_QR2_ITER.

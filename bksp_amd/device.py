@@ -191,6 +191,7 @@ class ZMatrix:
         if not self.handle:
             raise BkspDeviceError("bkh_zmatrix_create failed: " + last_error())
         self.dia_entries = load().bkh_zmatrix_dia_entries(self.handle)
+        self.split_rows = load().bkh_zmatrix_split_rows(self.handle)
 
     def destroy(self):
         if self.handle:

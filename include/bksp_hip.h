@@ -203,6 +203,10 @@ void bkh_zmatrix_destroy(bkh_zmatrix *A);
 /* Stored positions of the slice-diagonal copies of A and A^T the complex solvers'
  * SpMV uses (0: none; BKH_DIA=0 at creation turns them off). */
 long long bkh_zmatrix_dia_entries(const bkh_zmatrix *A);
+/* Rows of A and A^T longer than their 64-row slice's SELL width (at most
+ * max(16, twice the slice's mean row length)); their remaining entries form a CSR tail
+ * summed after the slice, in stored order. */
+long long bkh_zmatrix_split_rows(const bkh_zmatrix *A);
 /* y = A x (op 'n') or A^H x (op 't'), each row summed in the order of src/matvec.c:55-98
  * (replaces zcsrgemm0 through mkl_cspblas_zcsrgemv, include/bksp_internal.h:54-59). */
 int bkh_zspmv(const bkh_zmatrix *A, char op, const double *d_x, double *d_y);

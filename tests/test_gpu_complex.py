@@ -125,6 +125,54 @@ def test_zshbicgstabl_parity(l):
     assert rel(X, Xk) < 1e-8
 
 
+def arrow_zsystem(m, seed):
+    """zsystem(m) plus a dense first row and first column (seeded complex entries of
+    scale 10, the diagonal being about 7e3): one row of A and one of A^T far longer than
+    their slice's other rows."""
+    import scipy.sparse as sp
+    n, rp, ci, v = zsystem(m, seed=seed)
+    rng = np.random.default_rng(seed + 100)
+    D = sp.lil_matrix((n, n), dtype=np.complex128)
+    D[0, 1:] = 10.0 * zrand(rng, n - 1)
+    D[1:, 0] = (10.0 * zrand(rng, n - 1))[:, None]
+    B = (zcsr_dense(n, rp, ci, v) + D.tocsr()).tocsr()
+    B.sort_indices()
+    return n, B.indptr.astype(np.int32), B.indices.astype(np.int32), B.data.astype(np.complex128)
+
+
+def test_dense_row_and_column_split_bitwise():
+    """A row longer than its slice's SELL width keeps its head in the slice and the rest
+    in a CSR tail summed after it in stored order: still bitwise the oracle's products,
+    and the solvers (no slice-diagonal copy for such a matrix) still match the oracle."""
+    n, rp, ci, v = arrow_zsystem(40, seed=6)
+    M = dv.ZMatrix(n, rp, ci, v)
+    assert M.split_rows == 2 and M.dia_entries == 0, (M.split_rows, M.dia_entries)
+    rng = np.random.default_rng(22)
+    x = zrand(rng, n)
+    dx, dy = dv.DeviceArray.from_numpy(x), dv.DeviceArray((n,), np.complex128)
+    for op in "nt":
+        M.spmv(dx, dy, op)
+        assert np.array_equal(dy.numpy(), po.zcsrgemm0(op, n, rp, ci, v, x)[:, 0]), op
+    X, C = zrand(rng, n, 3), zrand(rng, n, 3)
+    dX = dv.DeviceArray.from_numpy(X.ravel(order="F"))
+    for op in "nt":
+        dC = dv.DeviceArray.from_numpy(C.ravel(order="F"))
+        M.spmm1(3, dX, dC, 0.7 - 0.4j, -1.25 + 0.5j, op)
+        want = po.zcsrgemm1(op, 0.7 - 0.4j, n, rp, ci, v, X, -1.25 + 0.5j, C)
+        assert np.array_equal(dC.numpy().reshape(3, n).T, want), op
+    A = zcsr_dense(n, rp, ci, v)
+    b = zrand(np.random.default_rng(23), n)
+    for kind, args, ora in (("bicg", (1e-10, 3000), lambda t, k: po.zbicg(n, rp, ci, v, b, t, k)),
+                            ("bicgstabl", (2, 1e-10, 2000),
+                             lambda t, k: po.zbicgstabl(n, rp, ci, v, b, 2, t, k))):
+        info, x, _ = gpu_solve(kind, n, rp, ci, v, b, *args)
+        cret, cx = ora(1e-10, args[-1])
+        assert info.ret >= 0 and cret >= 0 and close_index(info.ret, cret), (kind, info.ret, cret)
+        res = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+        assert res < 1e-9 and abs(info.true_res - res) <= 1e-6 * res, (kind, res, info.true_res)
+        assert rel(x, oracle_at_index(ora, info.ret)) < 1e-8, kind
+
+
 def test_real_data_matches_real_solver():
     """With zero imaginary parts the complex BiCGSTAB(l) performs the real one's
     arithmetic: same break index as the real device solver and the oracle."""
