@@ -228,6 +228,11 @@ __global__ __launch_bounds__(BS) void k_cgf_a_sell(int n, int nsl, int chunk, co
     if (ctl_stopped(c)) return;
     const int k = c->base + j;
     const bool writer = blockIdx.x == 0 && threadIdx.x == 0;
+    const T *pold = (k & 1) ? P1 : P0;
+    T *pnew = (k & 1) ? P0 : P1;
+    const int g = xcd_block(blockIdx.x, gridDim.x);
+    const int l = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int send = min(nsl, (g + 1) * chunk);
     double rr;
     if (k == c->first) {
         rr = c->s[CGF_RR + (k & 1)];
@@ -238,11 +243,6 @@ __global__ __launch_bounds__(BS) void k_cgf_a_sell(int n, int nsl, int chunk, co
     }
     const bool fresh = k == c->first && c->s[CGF_FRESH] != 0.0;
     const T be = fresh ? (T)0 : (T)(rr / c->s[CGF_RR + ((k + 1) & 1)]);
-    const T *pold = (k & 1) ? P1 : P0;
-    T *pnew = (k & 1) ? P0 : P1;
-    const int g = xcd_block(blockIdx.x, gridDim.x);
-    const int l = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int send = min(nsl, (g + 1) * chunk);
     double dv[1] = {0.0};
     for (int sl = g * chunk + w; sl < send; sl += BS / 64) {
         const int row = sl * 64 + l;
@@ -268,6 +268,17 @@ __global__ __launch_bounds__(512) void k_cgf_b(long long n, T *x, T *r, const T 
                                                double *partB) {
     if (ctl_stopped(c)) return;
     const int k = c->base + j;
+    const T *p = (k & 1) ? P0 : P1;
+    const long long tid = (long long)blockIdx.x * 512 + threadIdx.x, stride = (long long)gridDim.x * 512;
+    constexpr int NV = VEC ? 16 / (int)sizeof(T) : 1;
+    typedef T tv __attribute__((ext_vector_type(NV)));
+    // VEC: each thread's first vectors of x, r, p, q are loaded before kernel A's
+    // partials are summed, so their latency overlaps that prologue
+    tv x0, r0, p0, q0;
+    if (VEC && tid < n / NV) {
+        x0 = ((const tv *)x)[tid]; r0 = ((const tv *)r)[tid];
+        p0 = ((const tv *)p)[tid]; q0 = ((const tv *)q)[tid];
+    }
     const double pq = block_sum_parts<512>(partA, npA);
     if (bad(pq)) {                                    // breakdown (zero or non-finite p.q)
         if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -276,17 +287,18 @@ __global__ __launch_bounds__(512) void k_cgf_b(long long n, T *x, T *r, const T 
         return;
     }
     const T al = (T)(c->s[CGF_RR + (k & 1)] / pq);
-    const T *p = (k & 1) ? P0 : P1;
     double v[1] = {0.0};
-    const long long tid = (long long)blockIdx.x * 512 + threadIdx.x, stride = (long long)gridDim.x * 512;
     if constexpr (VEC) {
         // 16-byte vectors (every array 16-byte aligned), the odd last entry by thread 0
-        constexpr int NV = 16 / (int)sizeof(T);
-        typedef T tv __attribute__((ext_vector_type(NV)));
         const long long nv = n / NV;
         for (long long iv = tid; iv < nv; iv += stride) {
-            tv xv = ((const tv *)x)[iv], rv = ((const tv *)r)[iv];
-            const tv pv = ((const tv *)p)[iv], qv = ((const tv *)q)[iv];
+            tv xv, rv, pv, qv;
+            if (iv == tid) {
+                xv = x0; rv = r0; pv = p0; qv = q0;
+            } else {
+                xv = ((const tv *)x)[iv]; rv = ((const tv *)r)[iv];
+                pv = ((const tv *)p)[iv]; qv = ((const tv *)q)[iv];
+            }
 #pragma unroll
             for (int u = 0; u < NV; u++) {
                 xv[u] = fma(al, pv[u], xv[u]);
@@ -352,10 +364,22 @@ template <int BS>
 __device__ __forceinline__ void block_sum_parts3(const double *part, int np, double (&o)[3]) {
     __shared__ double sh[3 * (BS / 64) + 3];
     double a[3] = {0.0, 0.0, 0.0};
-    for (int i = threadIdx.x; i < np; i += BS) {
-        a[0] += part[3 * i];
-        a[1] += part[3 * i + 1];
-        a[2] += part[3 * i + 2];
+    // rows i and i + BS loaded together (one memory round trip for up to 2 BS rows),
+    // then added in row order
+    for (int i = threadIdx.x; i < np; i += 2 * BS) {
+        const bool h = i + BS < np;
+        double u[3], t[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int v = 0; v < 3; v++) u[v] = part[3 * i + v];
+        if (h) {
+#pragma unroll
+            for (int v = 0; v < 3; v++) t[v] = part[3 * (i + BS) + v];
+        }
+#pragma unroll
+        for (int v = 0; v < 3; v++) {
+            a[v] += u[v];
+            if (h) a[v] += t[v];
+        }
     }
     const int w = threadIdx.x >> 6;
 #pragma unroll
@@ -427,26 +451,69 @@ __global__ __launch_bounds__(BS) void k_cg1_sell(int n, int nsl, int chunk, cons
     if (ctl_stopped(c)) return;
     const int k = c->base + j;
     const bool writer = blockIdx.x == 0 && threadIdx.x == 0;
-    double ad, bd;
-    if (!cg1_scalars<BS>(c, k, partPrev, np, writer, ad, bd)) return;
     const bool start = (k == c->first);            // q = A p explicitly
-    const float al = (float)ad, be = (float)bd;
-    const bool odd = (k & 1) != 0;
-    // alpha_{k-2} of the update kernel k-1 deferred (none at a pass's first kernel)
-    const float ald = (odd && !start) ? (float)c->s[CG1_ALPHAD] : 0.0f;
-    if (!odd && writer) c->s[CG1_ALPHAD] = ad;
     const size_t po = (size_t)((k + 1) & 1) * n, pn = (size_t)(k & 1) * n;
     const float *rp_ = R + po, *qp_ = Q + po, *pp_ = P + po;
     float *rn_ = R + pn, *qn_ = Q + pn, *pn_ = P + pn;
     const int g = xcd_block(blockIdx.x, gridDim.x);
     const int l = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int send = min(nsl, (g + 1) * chunk);
+    const int sl0 = g * chunk + w;
+    // DIA: the wave's first slice has its first eight positions (values, and the
+    // gathered r and q -- or p at a pass's first kernel) loaded before the previous
+    // kernel's partials are summed, so their latency overlaps that prologue.
+    float v0[8], r0[8], g0[8];
+    int w0 = 0;
+    if constexpr (DIA) {
+        const int row = sl0 * 64 + l;
+        if (sl0 < send && row < n) {
+            const int s0 = so[sl0];
+            w0 = (so[sl0 + 1] - s0) >> 6;
+            if (w0 > 0) {
+                const int *o = (const int *)col + (s0 >> 6);
+                const float *gs = start ? pp_ : qp_;
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int cc = min(max(row + o[u], 0), n - 1);
+                    v0[u] = (float)val[s0 + l + 64 * min(u, w0 - 1)];
+                    r0[u] = rp_[cc];
+                    g0[u] = gs[cc];
+                }
+            }
+        }
+    }
+    double ad, bd;
+    if (!cg1_scalars<BS>(c, k, partPrev, np, writer, ad, bd)) return;
+    const float al = (float)ad, be = (float)bd;
+    const bool odd = (k & 1) != 0;
+    // alpha_{k-2} of the update kernel k-1 deferred (none at a pass's first kernel)
+    const float ald = (odd && !start) ? (float)c->s[CG1_ALPHAD] : 0.0f;
+    if (!odd && writer) c->s[CG1_ALPHAD] = ad;
     double dv[3] = {0.0, 0.0, 0.0};
-    for (int sl = g * chunk + w; sl < send; sl += BS / 64) {
+    for (int sl = sl0; sl < send; sl += BS / 64) {
         const int row = sl * 64 + l;
         if (row >= n) continue;
         float acc = 0.0f;
-        if (start)
+        if (DIA && sl == sl0 && w0 > 0) {
+            // the prefetched positions, then the slice's positions from 8 on
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const float gv = start ? fmaf(be, g0[u], r0[u]) : fmaf(-al, g0[u], r0[u]);
+                acc = add_rn(acc, mul_rn(u < w0 ? v0[u] : 0.0f, gv));
+            }
+            const int s0 = so[sl], wd = (so[sl + 1] - s0) >> 6;
+            const int *o = (const int *)col + (s0 >> 6);
+            for (int k0 = 8; k0 < wd; k0 += 8) {
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int kk = min(k0 + u, wd - 1);
+                    const int cc = min(max(row + o[k0 + u], 0), n - 1);
+                    const float v = (float)val[s0 + l + 64 * kk];
+                    const float gv = start ? fmaf(be, pp_[cc], rp_[cc]) : fmaf(-al, qp_[cc], rp_[cc]);
+                    acc = add_rn(acc, mul_rn(k0 + u < wd ? v : 0.0f, gv));
+                }
+            }
+        } else if (start)
             sell_row_entries<IT, DIA, 8>(n, row, l, sl, rp, so, col, [&](int e, int cc, bool live) {
                 const float v = (float)val[e];
                 acc = add_rn(acc, mul_rn(live ? v : 0.0f, fmaf(be, pp_[cc], rp_[cc])));

@@ -369,7 +369,7 @@ struct Cg1ResumeFin {
 
 // Launch shape of the single-kernel FP32 iteration. Every workgroup first re-sums
 // all partials of the previous kernel (3 per workgroup), so large matrices run
-// CG1_NPART workgroups of 512 threads (8 waves, about two slices per wave at 128^3):
+// CG1_NPART workgroups of 512 threads (8 waves, four slices per wave at 128^3):
 // measured 15.0 ms per 128^3 solve against 16.2 ms with 2048 workgroups of 256
 // threads, whose partials cost ~100 MB of L2 reads per kernel; 1024 workgroups of
 // 1024 threads were slower again (17.6 ms).
