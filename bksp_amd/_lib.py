@@ -79,6 +79,7 @@ _SIGS = {
     "bkh_matrix_set_spmv": (c_int, [c_vp, c_int]),
     "bkh_spmv": (c_int, [c_vp, ctypes.c_char, c_dbl, c_vp, c_dbl, c_vp]),
     "bkh_spmm": (c_int, [c_vp, c_int, c_dbl, c_vp, c_dbl, c_vp]),
+    "bkh_spmm_gram": (c_int, [c_vp, c_int, c_vp, c_vp, c_dp, ctypes.POINTER(c_int)]),
     "bkh_zmatrix_create": (c_vp, [c_int, c_vp, c_vp, c_vp]),
     "bkh_zmatrix_destroy": (None, [c_vp]),
     "bkh_zmatrix_dia_entries": (c_ll, [c_vp]),

@@ -680,3 +680,133 @@ __global__ __launch_bounds__(BS, MINW) void k_spmm_gram(int n, int nsl, int chun
         part[(size_t)lb * S * S + t] = v;
     }
 }
+
+// The same product as k_spmm_gram on the slice-diagonal copy, software-pipelined: a
+// slice's record (DiaSlice) arrives with one scalar load, and the values and the KB
+// gathers of a wave's next pass are issued before the current pass is summed, so a
+// wave keeps two passes of loads in flight. Interior slices (every row + offset in
+// [0, n)) address each gather as the uniform X + o S plus the lane's byte offset, with
+// no clamp. P's own row for the Gram is the gathered row of the slice's offset-0
+// position. Needs every slice with 1..KB positions and an offset 0 (A->dmeta). Each
+// row sums its positions in order with the roundings of k_spmm_sell_v, so AP is
+// bitwise the plain SpMM's, and the MFMA order per workgroup is k_spmm_gram's at the
+// same chunking. 128^3 Poisson, s = 8: 117.8 -> 102.5 us FP64, 66.4 -> 53.3 us FP32.
+template <int S, typename T, int KB, int LPR>
+struct DiaStage {
+    static constexpr int CPL = S / LPR, NV = (KB + LPR - 1) / LPR;
+    typedef T tvec __attribute__((ext_vector_type(CPL)));
+    T vl[NV];
+    tvec xv[KB];
+};
+
+template <int S, typename T, int BS, int KB, int MINW>
+__global__ __launch_bounds__(BS, MINW) void k_spmm_gram_dia(int n, int nsl, int chunk,
+                                                            const DiaSlice *__restrict__ meta,
+                                                            const T *__restrict__ val, const T *X, T *Y,
+                                                            const Ctl *ctl, double *__restrict__ part) {
+    constexpr int LPR = spmm_lpr<S, T>();
+    static_assert(LPR > 0 && S <= 16 && KB <= DIA_META_POS, "k_spmm_gram_dia: 16-byte rows, S <= 16");
+    constexpr int CPL = S / LPR, RW = 64 / LPR, NWV = BS / 64, NV = (KB + LPR - 1) / LPR;
+    constexpr int SPB = NWV / LPR;                                  // slices per pass
+    static_assert(SPB >= 1 && NWV % LPR == 0, "a pass must cover whole slices");
+    typedef DiaStage<S, T, KB, LPR> St;
+    typedef typename St::tvec tvec;
+    if (ctl && ctl_stopped(ctl)) return;
+    __shared__ T tp[NWV][RW * S], ta[NWV][RW * S];
+    __shared__ double red[NWV][S * S];
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int sub = lane % LPR, rw = lane / LPR;
+    const int row_in = (w % LPR) * RW + rw;                         // row within the slice
+    const int wsl = __builtin_amdgcn_readfirstlane(w / LPR);         // slice within the pass
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int q0 = lb * chunk, q1 = min(nsl, q0 + chunk);
+    mf4 g = {0.0, 0.0, 0.0, 0.0};
+    // loads of one pass (slice index clamped: a pass past the range loads a real slice
+    // and is never summed)
+    auto load = [&](St &st, const DiaSlice &m, int sl) {
+        const int row = sl * 64 + row_in;
+        const int base = m.s0 + row_in;
+#pragma unroll
+        for (int i = 0; i < NV; i++)
+            st.vl[i] = __builtin_nontemporal_load(val + base + 64 * min(sub + LPR * i, m.len - 1));
+        if (m.inter) {
+            const unsigned lo = ((unsigned)row * S + sub * CPL) * (unsigned)sizeof(T);
+#pragma unroll
+            for (int q = 0; q < KB; q++)
+                st.xv[q] = *(const tvec *)((const char *)(X + (long long)m.o[q] * S) + lo);
+        } else {
+#pragma unroll
+            for (int q = 0; q < KB; q++) {
+                const int c = min(max(row + m.o[q], 0), n - 1);
+                st.xv[q] = *(const tvec *)(X + (size_t)c * S + sub * CPL);
+            }
+        }
+    };
+    auto compute = [&](St &st, const DiaSlice &m, int qb, int sl) {
+        const int row = sl * 64 + row_in;
+        const bool live = qb + wsl < q1 && row < n;
+        T acc[CPL];
+#pragma unroll
+        for (int j = 0; j < CPL; j++) acc[j] = (T)0;
+#pragma unroll
+        for (int q = 0; q < KB; q++) {
+            const T a = grp_bcast<LPR>(st.vl[q / LPR], q % LPR);
+            if (q < m.len) {
+#pragma unroll
+                for (int j = 0; j < CPL; j++) acc[j] = add_rn(acc[j], mul_rn(a, st.xv[q][j]));
+            }
+        }
+        tvec out;
+#pragma unroll
+        for (int j = 0; j < CPL; j++) out[j] = acc[j];
+        if (live) __builtin_nontemporal_store(out, (tvec *)(Y + (size_t)row * S + sub * CPL));
+        tvec pv = st.xv[0];
+#pragma unroll
+        for (int q = 1; q < KB; q++)
+            if (q == m.cq) pv = st.xv[q];
+#pragma unroll
+        for (int j = 0; j < CPL; j++) {
+            tp[w][rw * S + sub * CPL + j] = live ? pv[j] : (T)0;
+            ta[w][rw * S + sub * CPL + j] = live ? out[j] : (T)0;
+        }
+        wave_lds_sync();
+        const int a = lane & 15;
+#pragma unroll
+        for (int q = 0; q < RW / 4; q++) {
+            const int r = 4 * q + (lane >> 4);
+            const double av = a < S ? (double)tp[w][r * S + a] : 0.0;
+            const double bv = a < S ? (double)ta[w][r * S + a] : 0.0;
+            g = mfma16x16x4(av, bv, g);
+        }
+        wave_lds_sync();
+    };
+    auto slice_of = [&](int idx) { return __builtin_amdgcn_readfirstlane(min(idx, nsl - 1)); };
+    St A, B;
+    int qb = q0;
+    int slA = slice_of(qb + wsl);
+    DiaSlice mA = meta[slA];
+    if (qb < q1) load(A, mA, slA);
+    for (; qb < q1; qb += 2 * SPB) {
+        const int slB = slice_of(qb + SPB + wsl);
+        const DiaSlice mB = meta[slB];
+        load(B, mB, slB);
+        compute(A, mA, qb, slA);
+        if (qb + SPB >= q1) break;
+        slA = slice_of(qb + 2 * SPB + wsl);
+        mA = meta[slA];
+        load(A, mA, slA);
+        compute(B, mB, qb + SPB, slB);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int a = (lane >> 4) + 4 * r, b = lane & 15;
+        if (a < S && b < S) red[w][a + b * S] = g[r];
+    }
+    __syncthreads();
+    for (int t = tid; t < S * S; t += BS) {
+        double v = red[0][t];
+#pragma unroll
+        for (int k = 1; k < NWV; k++) v += red[k][t];
+        part[(size_t)lb * S * S + t] = v;
+    }
+}

@@ -83,6 +83,16 @@ enum { SPMV_BLOCK = 0, SPMV_WAVE = 1, SPMV_SELL = 2, SPMV_SELL32 = 3 };
 // ascending offsets keep each row's summation order) and the copy stays below
 // DIA_MAX_PAD of the nonzeros. Used by the CG iteration kernels.
 #define DIA_MAX_PAD 1.1
+// One 64-byte record per slice of the slice-diagonal copy, read with a single scalar
+// load by the pipelined block kernels: the slice's first entry, its number of stored
+// positions, the position whose offset is 0, whether every row + offset of the slice
+// lies in [0, n) (then no gather needs a clamp), and the offsets (past `len`, the last
+// one repeated). Built when every slice has 1..DIA_META_POS positions and an offset 0.
+#define DIA_META_POS 12
+struct DiaSlice {
+    int s0, len, cq, inter;
+    int o[DIA_META_POS];
+};
 
 // Device-resident CSR. Rows are stored in the reference's accumulation order
 // (see bkh_matrix_create), so a sequential per-row sum reproduces src/matvec.c.
@@ -111,6 +121,8 @@ struct bkh_matrix {
     double *dval = nullptr;            // its values (absent entries 0)
     float *dval32 = nullptr;           // FP32 copy
     int *doff = nullptr;               // column offset of stored position k of slice q at dso[q] / 64 + k
+    DiaSlice *dmeta = nullptr;         // per-slice records (nullptr: some slice does not qualify)
+    int dia_maxpos = 0;                // most stored positions of any slice
     int variant = SPMV_WAVE;           // kernel used by every SpMV on this matrix
     int max_row = 0;                   // longest row (nonzeros)
     bkh_matrix *T = nullptr;           // A^T in the same format (BiCG)

@@ -903,6 +903,27 @@ static int launch_spmm_gram(hipStream_t s, const bkh_matrix *A, const VT *val, c
         } else {
             if (val == (const VT *)A->val) { dv = (const VT *)A->dval; sv = (const VT *)A->sval; }
         }
+        // the pipelined slice-diagonal kernel (values in the blocks' precision, every slice
+        // with <= 8 positions and an offset 0, byte offsets of the block within 32 bits);
+        // 256 workgroups of 1024 threads at every precision
+        if constexpr (std::is_same<VT, T>::value) {
+            static const bool pipe_off = [] { const char *e = getenv("BKH_DIA_PIPE"); return e && e[0] == '0'; }();
+            if (!pipe_off && A->dia_entries > 0 && dv && A->dmeta && A->dia_maxpos <= 8 &&
+                (unsigned long long)A->n * S * sizeof(T) < (1ULL << 32)) {
+                constexpr int BSP = 1024, SPBP = BSP / 64 / LPR;
+                const int np = (A->nsl + SPBP - 1) / SPBP;
+                const int g0 = std::max(1, std::min(256, np));
+                const int ch = ((np + g0 - 1) / g0) * SPBP;
+                const int gp = (A->nsl + ch - 1) / ch;
+                if (A->dia_maxpos <= 7)
+                    hipLaunchKernelGGL((k_spmm_gram_dia<S, T, BSP, 7, 4>), dim3(gp), dim3(BSP), 0, s, (int)A->n,
+                                       A->nsl, ch, (const DiaSlice *)A->dmeta, (const T *)dv, P, AP, c, part);
+                else
+                    hipLaunchKernelGGL((k_spmm_gram_dia<S, T, BSP, 8, 4>), dim3(gp), dim3(BSP), 0, s, (int)A->n,
+                                       A->nsl, ch, (const DiaSlice *)A->dmeta, (const T *)dv, P, AP, c, part);
+                return gp;
+            }
+        }
         if (A->dia_entries > 0 && dv)
             hipLaunchKernelGGL((k_spmm_gram<S, VT, T, int, true, BSG, UNR, KB, MINW>), dim3(G), dim3(BSG), 0, s, (int)A->n, A->nsl,
                                chunk, A->rp, A->dso, dv, (const int *)A->doff, P, AP, c, part);
@@ -1417,4 +1438,56 @@ extern "C" int bkh_gram(int m, int s, const double *X, const double *Y, double *
         G[e] = acc;
     }
     return 0;
+}
+
+// AP = A P and G = P^T AP (s x s, column-major, to host) for m x s column-major P, by
+// the path the block CG iteration takes: the fused SpMM + Gram kernel when the matrix
+// allows it, else the SpMM and the Gram pass. Partial rows summed in order on the host.
+// Returns 0, or 1 when the fused kernel ran and 2 when the pipelined slice-diagonal
+// form of it ran (through *path), for tests that compare the paths.
+template <int S>
+static int spmm_gram_host(ThreadCtx *t, const bkh_matrix *A, const double *P, double *AP, double *G, int *path) {
+    hipStream_t st = t->stream;
+    const long long m = A->n;
+    const int Gt = vec_grid(m * S, 256), Gb = block_grid((int)m);
+    const int nrows = std::max(Gb, 1024);
+    void *ws = bkh_workspace(2 * carve_size(sizeof(double) * m * S) + carve_size(sizeof(double) * nrows * SM_N));
+    if (!ws) return BKH_DEVICE_ERROR;
+    Carve cv(ws);
+    double *Pt = cv.take<double>(m * S), *APt = cv.take<double>(m * S);
+    double *part = cv.take<double>((size_t)nrows * SM_N);
+    hipLaunchKernelGGL((k_cm2rm<double, double>), dim3(Gt), dim3(256), 0, st, m, S, P, Pt);
+    int ng = launch_spmm_gram<S>(st, A, A->val, (const double *)Pt, APt, (const Ctl *)nullptr, part);
+    *path = 0;
+    if (ng) {
+        *path = (A->dmeta && A->dia_maxpos <= 8 && A->dia_entries > 0) ? 2 : 1;
+    } else {
+        launch_spmm<S>(st, A, A->val, (const double *)Pt, APt, 1.0, 0.0, (const Ctl *)nullptr, true);
+        ng = panel_gram<S, double>(st, m, Pt, APt, (const Ctl *)nullptr, part);
+    }
+    hipLaunchKernelGGL((k_rm2cm<double, double>), dim3(Gt), dim3(256), 0, st, m, S, (const double *)APt, AP);
+    LAUNCH_OK();
+    std::vector<double> h((size_t)ng * S * S);
+    HIP_OK(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    for (int e = 0; e < S * S; e++) {
+        double acc = 0.0;
+        for (int b = 0; b < ng; b++) acc += h[(size_t)b * S * S + e];
+        G[e] = acc;
+    }
+    return 0;
+}
+
+extern "C" int bkh_spmm_gram(const bkh_matrix *A, int s, const double *P, double *AP, double *G, int *path) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (!A || !P || !AP || !G || !path) return BKH_BAD_ARGUMENT;
+    if (s < 1 || s > SM_MAX) { bkh_set_error("SpMM + Gram supports 1 <= s <= %d", SM_MAX); return BKH_BAD_ARGUMENT; }
+    switch (s) {
+#define CASE(S_) case S_: return spmm_gram_host<S_>(t, A, P, AP, G, path);
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+        CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
+#undef CASE
+    }
+    return BKH_BAD_ARGUMENT;
 }

@@ -379,7 +379,7 @@ static void free_matrix(bkh_matrix *A) {
     hipFree(A->rp); hipFree(A->ci); hipFree(A->val); hipFree(A->val32);
     hipFree(A->diag); hipFree(A->blk); hipFree(A->wblk); hipFree(A->cd16);
     hipFree(A->sso); hipFree(A->sval); hipFree(A->sval32); hipFree(A->scol); hipFree(A->scd);
-    hipFree(A->dso); hipFree(A->dval); hipFree(A->dval32); hipFree(A->doff);
+    hipFree(A->dso); hipFree(A->dval); hipFree(A->dval32); hipFree(A->doff); hipFree(A->dmeta);
     if (A->T) free_matrix(A->T);
     delete A;
 }
@@ -499,9 +499,33 @@ static bool upload_dia(bkh_matrix *A, int n, const std::vector<int> &rp, const s
         for (size_t e = 0; e < ne; e++) d32[e] = (float)dv[e];
         ok = hipMemcpyAsync(A->dval32, d32.data(), sizeof(float) * ne, hipMemcpyHostToDevice, s) == hipSuccess;
     }
+    // per-slice records for the pipelined block kernels
+    std::vector<DiaSlice> meta((size_t)nsl);
+    bool qualifies = true;
+    int maxpos = 0;
+    for (int q = 0; q < nsl && qualifies; q++) {
+        DiaSlice &m = meta[q];
+        m.s0 = so[q];
+        m.len = (so[q + 1] - so[q]) / 64;
+        maxpos = std::max(maxpos, m.len);
+        if (m.len < 1 || m.len > DIA_META_POS) { qualifies = false; break; }
+        const int *o = offs.data() + so[q] / 64;
+        m.cq = -1;
+        for (int k = 0; k < DIA_META_POS; k++) m.o[k] = o[std::min(k, m.len - 1)];
+        for (int k = 0; k < m.len; k++)
+            if (o[k] == 0) m.cq = k;
+        if (m.cq < 0) qualifies = false;
+        const long long lo = 64LL * q + o[0], hi = 64LL * q + 63 + o[m.len - 1];
+        m.inter = (lo >= 0 && hi <= (long long)n - 1) ? 1 : 0;
+    }
+    if (ok && qualifies) {
+        ok = bkh_dmalloc(&A->dmeta, sizeof(DiaSlice) * nsl) == hipSuccess &&
+             hipMemcpyAsync(A->dmeta, meta.data(), sizeof(DiaSlice) * nsl, hipMemcpyHostToDevice, s) == hipSuccess;
+    }
     ok = ok && hipStreamSynchronize(s) == hipSuccess;
     if (!ok) return false;
     A->dia_entries = tot;
+    A->dia_maxpos = maxpos;
     return true;
 }
 

@@ -121,6 +121,18 @@ class Matrix:
         if rc:
             raise BkspDeviceError(f"bkh_spmm -> {rc}: {last_error()}")
 
+    def spmm_gram(self, s, d_P, d_AP):
+        """AP = A P and the Gram P^T A P (s x s numpy array) for s column-major columns,
+        by block CG's path; returns (G, path) with path 0 (separate SpMM and Gram),
+        1 (fused kernel) or 2 (pipelined slice-diagonal fused kernel)."""
+        G = np.zeros(s * s)
+        path = ctypes.c_int(-1)
+        rc = load().bkh_spmm_gram(self.handle, s, d_P.ptr, d_AP.ptr, G.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                  ctypes.byref(path))
+        if rc:
+            raise BkspDeviceError(f"bkh_spmm_gram -> {rc}: {last_error()}")
+        return G.reshape(s, s, order="F"), path.value
+
     def spmv_bytes(self, fp32=False):
         """Bytes one SpMV with the current kernel moves (format bytes)."""
         return load().bkh_matrix_spmv_bytes(self.handle, int(fp32))

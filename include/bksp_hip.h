@@ -129,6 +129,12 @@ int bkh_sp_norm_f(int n, const int *row_ptr, const int *col, const double *val,
 int bkh_dot(long long n, const double *d_x, const double *d_y, double *out);
 /* G = X^T Y (s x s, column-major, to host) for m x s column-major X, Y, on FP64 MFMA. */
 int bkh_gram(int m, int s, const double *d_X, const double *d_Y, double *G);
+/* d_AP = A d_P and G = P^T A P (s x s, column-major, to host) for m x s column-major
+ * d_P, s <= 16, by the path block CG's iteration takes (src/blcg.c:74-79, its
+ * mkl_dcsrmm and the cblas_dgemm after it): one fused SpMM + Gram kernel when the
+ * matrix allows it, else the SpMM and the Gram pass. *path: 0 separate, 1 fused,
+ * 2 fused on the slice-diagonal copy with per-slice records (pipelined). */
+int bkh_spmm_gram(const bkh_matrix *A, int s, const double *d_P, double *d_AP, double *G, int *path);
 
 /* ---- solvers on device vectors (x: initial guess in, solution out) ----
  * The return value is what the matching bksp.h entry point returns. */

@@ -54,3 +54,62 @@ def test_poisson_sell_bitwise_and_columns(s):
         dx.upload(np.ascontiguousarray(X[:, j]))
         M.spmv(dx, dy)
         assert np.array_equal(got[:, j], dy.numpy())
+
+
+def gpu_spmm_gram(M, n, s, P):
+    dP = dv.DeviceArray.from_numpy(np.asfortranarray(P).ravel(order="F"))
+    dAP = dv.DeviceArray((n * s,))
+    G, path = M.spmm_gram(s, dP, dAP)
+    return dAP.numpy().reshape(s, n).T, G, path
+
+
+@pytest.mark.parametrize("s", [1, 2, 4, 8, 16])
+@pytest.mark.parametrize("m", [24, 33])
+def test_spmm_gram_pipelined_dia(m, s):
+    """Block CG's AP = A P with P^T AP (src/blcg.c:74-79) on Poisson grids whose slices
+    carry per-slice records (m = 33: n = 35,937 ends in a partial slice and the first
+    and last planes need clamped gathers): AP is bitwise the oracle's symmetric product;
+    the Gram agrees with P^T (AP) formed on the host in FP64 to 1e-13 relative."""
+    n, rp, ci, v = gen.poisson3d(m, lower=True)
+    M = dv.Matrix(n, rp, ci, v, symmetric=True)
+    P = np.random.default_rng(400 + s + m).standard_normal((n, s))
+    AP, G, path = gpu_spmm_gram(M, n, s, P)
+    want = po.csrmm("S", n, s, 1.0, rp, ci, v, P, 0.0, np.zeros((n, s)))
+    assert np.array_equal(AP, want)
+    if s * 8 in (16, 32, 64, 128):
+        assert path == 2
+    ref = P.T @ want
+    assert np.max(np.abs(G - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+def lower_of(n, rp, ci, v):
+    """The lower triangle (diagonal included) of a CSR matrix, as the stored triangle of
+    a symmetric one."""
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    keep = ci <= rows
+    lr = np.concatenate([[0], np.cumsum(np.bincount(rows[keep], minlength=n))]).astype(np.int32)
+    return lr, ci[keep].astype(np.int32), v[keep]
+
+
+@pytest.mark.parametrize("s", [2, 8])
+@pytest.mark.parametrize("kind", ["bfwb398", "random"])
+def test_spmm_gram_other_matrices(bfwb398, kind, s):
+    """Symmetric matrices (block CG's input, src/blcg.c:18-20) without per-slice
+    records: the reference's bfwb398 fixture and the lower triangle of a random-column
+    matrix (no slice-diagonal copy: the SELL fused kernel or the separate SpMM and Gram
+    pass, which mirrors the upper triangle of the symmetric P^T A P). AP bitwise, the
+    Gram to 1e-13 relative of P^T (AP) on the host."""
+    if kind == "bfwb398":
+        n, rp, ci, v, _ = bfwb398
+    else:
+        n, rp0, ci0, v0 = gen.random_rows(606, 3000)
+        rp, ci, v = lower_of(n, rp0, ci0, v0)
+    M = dv.Matrix(n, rp, ci, v, symmetric=True)
+    P = np.random.default_rng(500 + s).standard_normal((n, s))
+    AP, G, path = gpu_spmm_gram(M, n, s, P)
+    want = po.csrmm("S", n, s, 1.0, rp, ci, v, P, 0.0, np.zeros((n, s)))
+    assert np.array_equal(AP, want)
+    if kind == "random":
+        assert path in (0, 1)
+    ref = P.T @ want
+    assert np.max(np.abs(G - ref)) <= 1e-13 * np.max(np.abs(ref))
