@@ -129,6 +129,7 @@ _SIGS = {
     "bkh_batch_last_path": (c_int, []),
     "bkh_debug_fail_alloc": (None, [c_int]),
     "bkh_debug_fail_alloc_pending": (c_int, []),
+    "bkh_debug_fin_lds_form": (None, [c_int]),
     "bkh_batch_bicgstabl": (c_int, [c_vp, c_vp, c_int, c_vp, c_dbl, c_int, ctypes.POINTER(BkhOpts), c_ip,
                                     c_dp, ctypes.POINTER(BkhInfo)]),
     "bkh_matrix_spmv_bytes": (c_ll, [c_vp, c_int]),

@@ -161,3 +161,110 @@ __device__ __forceinline__ double sm_fro2(int s, const double *A, double *red) {
     sm_sync<W>();
     return *red;
 }
+
+// ---- register forms for s <= 8 on one wave: lane j holds column j of an s x s
+// column-major matrix in `double v[S]` (lanes >= s carry values nobody reads), and an
+// entry of another column arrives by a readlane broadcast instead of an LDS round trip
+// and a barrier. Each routine performs the operations of its LDS counterpart above in
+// the same order with the same roundings, so the results are bitwise the same.
+__device__ __forceinline__ double rg_bcast(double v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+// Column j (this lane) of the upper Cholesky factor of G (sm_chol_upper): in, column j
+// of G; out, column j of R. Returns false (uniformly) if a pivot is not positive.
+template <int S>
+__device__ __forceinline__ bool rg_chol_upper(double (&r)[S], double shift) {
+    const int j = (int)(threadIdx.x & 63);
+#pragma unroll
+    for (int i = 0; i < S; i++) r[i] = (i <= j) ? r[i] + ((i == j) ? shift : 0.0) : 0.0;
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < S; k++) {
+        const double d = rg_bcast(r[k], k);
+        if (!(d > 0.0)) ok = false;
+        const double rk = d > 0.0 ? sqrt(d) : 1.0;
+        if (j == k) r[k] = rk;
+        if (j > k) r[k] = r[k] / rk;
+#pragma unroll
+        for (int i = k + 1; i < S; i++) {
+            const double rki = rg_bcast(r[k], i);
+            if (i <= j) r[i] = r[i] - rki * r[k];
+        }
+    }
+    return ok;
+}
+
+// Column j of R^-1 for upper-triangular R held by columns (sm_triu_inv).
+template <int S>
+__device__ __forceinline__ void rg_triu_inv(const double (&r)[S], double (&ri)[S]) {
+    const int j = (int)(threadIdx.x & 63);
+#pragma unroll
+    for (int i = S - 1; i >= 0; i--) {
+        double acc = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = i + 1; k < S; k++) {
+            const double rik = rg_bcast(r[i], k);
+            if (k <= j) acc -= rik * ri[k];
+        }
+        ri[i] = (i <= j) ? acc / rg_bcast(r[i], i) : 0.0;
+    }
+}
+
+// Column j of C = A B (sm_mul with ta = tb = 0: fma chain over k): a, b column j of A
+// and B on this lane; A's columns are read from lanes a0 .. a0 + S - 1.
+template <int S>
+__device__ __forceinline__ void rg_mul(const double (&a)[S], int a0, const double (&b)[S], double (&c)[S]) {
+#pragma unroll
+    for (int i = 0; i < S; i++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < S; k++) acc = fma(rg_bcast(a[i], a0 + k), b[k], acc);
+        c[i] = acc;
+    }
+}
+
+// Gauss-Jordan with partial pivoting on [A | I] (sm_inverse): lane c < 2S holds
+// column c of the augmented matrix; on return lanes S .. 2S - 1 hold A^-1's columns.
+// Returns false (uniformly) if a pivot was zero.
+template <int S>
+__device__ __forceinline__ bool rg_inverse(double (&m)[S]) {
+    const int c = (int)(threadIdx.x & 63);
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < S; k++) {
+        int p = k;
+        double best = fabs(rg_bcast(m[k], k));
+#pragma unroll
+        for (int i = k + 1; i < S; i++) {
+            const double v = fabs(rg_bcast(m[i], k));
+            if (v > best) { best = v; p = i; }
+        }
+        if (!(best > 0.0)) ok = false;
+        // row swap: p is uniform, so each candidate is a scalar branch over fixed
+        // registers (no dynamically indexed array)
+#pragma unroll
+        for (int i = k + 1; i < S; i++)
+            if (i == p) {
+                const double tmp = m[k];
+                m[k] = m[i];
+                m[i] = tmp;
+            }
+        const double inv = ok ? 1.0 / rg_bcast(m[k], k) : 0.0;
+        m[k] *= inv;
+#pragma unroll
+        for (int i = 0; i < S; i++) {
+            if (i == k) continue;
+            const double mik = rg_bcast(m[i], k);
+            if (c != k) m[i] -= mik * m[k];
+        }
+        if (c == k) {
+#pragma unroll
+            for (int i = 0; i < S; i++)
+                if (i != k) m[i] = 0.0;
+        }
+    }
+    return ok;
+}

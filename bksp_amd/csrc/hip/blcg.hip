@@ -77,9 +77,106 @@ __device__ __forceinline__ void fin_sum_parts(const double *part, int nblocks, d
     __syncthreads();
 }
 
+// The small dense steps for s <= 8 on the first wave, the matrices held by columns in
+// registers (bkh_small.hpp's rg_* forms, bitwise the LDS forms): G is the summed
+// Gram in LDS. 128^3 block CG FP32-IR, s = 8: the three fins 28.5 -> 26.5 us per
+// iteration (each launch costs ~4.4 us before any work; the rest is the partial sums).
+template <int S, int OP>
+__device__ __forceinline__ void blk_fin_regs(const double *G, BlkMats *mt, Ctl *c) {
+    const int j = (int)(threadIdx.x & 63);
+    const int jc = j < S ? j : S - 1;               // lanes past the columns load column S - 1
+    const bool own = j < S;
+    double g[S];
+#pragma unroll
+    for (int i = 0; i < S; i++) g[i] = G[i + jc * S];
+    if constexpr (OP == FIN_ALPHA) {
+        // alpha = (P^T A P)^-1 (dgetrf/dgetri, blcg.c:81-82), W = alpha C (:85-86)
+        double m[S];
+#pragma unroll
+        for (int i = 0; i < S; i++) m[i] = own ? g[i] : ((i == j - S) ? 1.0 : 0.0);
+        if (!rg_inverse<S>(m)) { if (j == 0) ctl_breakdown(c); return; }
+        if (j >= S && j < 2 * S) {
+#pragma unroll
+            for (int i = 0; i < S; i++) mt->alpha[i + (j - S) * S] = m[i];
+        }
+        double cc[S], w[S];
+#pragma unroll
+        for (int i = 0; i < S; i++) cc[i] = mt->C[i + jc * S];
+        rg_mul<S>(m, S, cc, w);
+        if (own) {
+#pragma unroll
+            for (int i = 0; i < S; i++) mt->W[i + j * S] = w[i];
+        }
+        return;
+    } else {
+        // CholQR pass: G = R^T R; pass 1 retries with a diagonal shift when G is
+        // numerically singular (shifted CholeskyQR, Fukaya et al. 2020)
+        double r[S];
+#pragma unroll
+        for (int i = 0; i < S; i++) r[i] = g[i];
+        bool ok = rg_chol_upper<S>(r, 0.0);
+        if (!ok && OP == FIN_QR1) {
+            double tr = 0.0;
+#pragma unroll
+            for (int k = 0; k < S; k++) tr += G[k + k * S];
+            const double shift = 11.0 * ((double)mt->m * S + S * (S + 1)) * 1.1102230246251565e-16 * tr;
+#pragma unroll
+            for (int i = 0; i < S; i++) r[i] = g[i];
+            ok = rg_chol_upper<S>(r, shift);
+        }
+        if (!ok) { if (j == 0) ctl_breakdown(c); return; }
+        double ri[S];
+        rg_triu_inv<S>(r, ri);
+        if constexpr (OP == FIN_QR1) {
+            if (own) {
+#pragma unroll
+                for (int i = 0; i < S; i++) { mt->R1[i + j * S] = r[i]; mt->R1i[i + j * S] = ri[i]; }
+            }
+            return;
+        } else {
+            // pass 2: full R of the QR = R2 R1; R1^-1 R2^-1 for the row-stream vp
+            double r1[S], r1i[S], t2[S], t1[S];
+#pragma unroll
+            for (int i = 0; i < S; i++) { r1[i] = mt->R1[i + jc * S]; r1i[i] = mt->R1i[i + jc * S]; }
+            rg_mul<S>(r, 0, r1, t2);
+            rg_mul<S>(r1i, 0, ri, t1);
+            if (own) {
+#pragma unroll
+                for (int i = 0; i < S; i++) { mt->R2i[i + j * S] = ri[i]; mt->R12i[i + j * S] = t1[i]; }
+            }
+            if constexpr (OP == FIN_QR2_INIT) {                    // R0 = V C  (blcg.c:58)
+                if (own) {
+#pragma unroll
+                    for (int i = 0; i < S; i++) mt->C[i + j * S] = t2[i];
+                }
+                return;
+            } else {
+                // C = beta C; error = ||C||_F / ||B||_F (blcg.c:98-105); next P uses beta^T
+                double cc[S], bc[S];
+#pragma unroll
+                for (int i = 0; i < S; i++) cc[i] = mt->C[i + jc * S];
+                rg_mul<S>(t2, 0, cc, bc);
+                if (own) {
+#pragma unroll
+                    for (int i = 0; i < S; i++) { mt->C[i + j * S] = bc[i]; mt->betaT[j + i * S] = t2[i]; }
+                }
+                double f2 = 0.0;
+#pragma unroll
+                for (int jj = 0; jj < S; jj++)
+#pragma unroll
+                    for (int i = 0; i < S; i++) {
+                        const double v = rg_bcast(bc[i], jj);
+                        f2 += v * v;
+                    }
+                if (j == 0) ctl_check(c, f2);
+            }
+        }
+    }
+}
+
 // One workgroup: sums the s x s Gram partials of `nblocks` workgroups in a fixed
 // order, then runs the block solver's small dense step `op`.
-template <int S, int OP, int BS>
+template <int S, int OP, int BS, bool REG = true>
 __global__ __launch_bounds__(BS) void k_blk_fin(const double *part, int nblocks, BlkMats *mt,
                                                 Ctl *c) {
     if (ctl_stopped(c)) return;
@@ -102,6 +199,10 @@ __global__ __launch_bounds__(BS) void k_blk_fin(const double *part, int nblocks,
     // 192.9 -> 190.2 ms FP32, the same iterates)
     constexpr bool W = S * S <= 64;
     if (W && t >= 64) return;
+    if constexpr (W && REG) {
+        blk_fin_regs<S, OP>(G, mt, c);
+        return;
+    }
     if (OP == FIN_ALPHA) {
         // alpha = (P^T A P)^-1 (dgetrf/dgetri, blcg.c:81-82), W = alpha C (:85-86)
         if (!sm_inverse<W>(S, G, Ri, work, flag)) { if (t == 0) ctl_breakdown(c); return; }
@@ -150,6 +251,14 @@ __global__ __launch_bounds__(BS) void k_blk_fin(const double *part, int nblocks,
     if (t == 0) ctl_check(c, f2);
 }
 
+// Test hook: 1 runs the s <= 8 small dense steps in their LDS form instead of the
+// register form (the two are bitwise the same); cached graphs are retired.
+static std::atomic<int> g_fin_lds_form{0};
+extern "C" void bkh_debug_fin_lds_form(int on) {
+    g_fin_lds_form.store(on ? 1 : 0);
+    bkh_graph_epoch_bump();
+}
+
 template <int S, int OP>
 static void launch_blk_fin(hipStream_t s, const double *part, int nblocks, BlkMats *mt, Ctl *c) {
     const int E = (OP == FIN_FNB) ? 1 : S * S;
@@ -158,7 +267,10 @@ static void launch_blk_fin(hipStream_t s, const double *part, int nblocks, BlkMa
     // keep 256 (their 64 partial loads per thread would not fit 128 registers)
     constexpr int FBS = S <= 8 ? 1024 : 256;
     if (nblocks <= (S <= 8 ? FIN_DIRECT_WIDE : FIN_DIRECT)) {
-        hipLaunchKernelGGL((k_blk_fin<S, OP, FBS>), dim3(1), dim3(FBS), 0, s, part, nblocks, mt, c);
+        if (g_fin_lds_form)
+            hipLaunchKernelGGL((k_blk_fin<S, OP, FBS, false>), dim3(1), dim3(FBS), 0, s, part, nblocks, mt, c);
+        else
+            hipLaunchKernelGGL((k_blk_fin<S, OP, FBS>), dim3(1), dim3(FBS), 0, s, part, nblocks, mt, c);
         return;
     }
     double *r2 = bkh_red2();

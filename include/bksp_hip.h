@@ -244,6 +244,10 @@ int bkh_zspmv_bench(const bkh_zmatrix *A, char op, int reps, double *avg_ms);
  * allocations still to go before it (0 once it has fired, or when disarmed). */
 void bkh_debug_fail_alloc(int n);
 int bkh_debug_fail_alloc_pending(void);
+/* Test hook (no reference counterpart): on = 1 runs the block solvers' s <= 8 small
+ * dense steps in their LDS form, on = 0 (default) in the register form; the two give
+ * bitwise the same results. Retires cached iteration graphs. */
+void bkh_debug_fin_lds_form(int on);
 
 /* ---- batches of independent systems (no reference counterpart: the reference
  * solves one system per call, so a batch is that many sequential dbicgstabl calls) */

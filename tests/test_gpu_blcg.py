@@ -208,3 +208,28 @@ def test_blcg_dia_layout_bitwise_equal_to_sell(m, s):
             out.append((info.ret, info.inner_iters, dX.numpy()))
         assert out[0][:2] == out[1][:2]
         assert np.array_equal(out[0][2], out[1][2])
+
+
+@pytest.mark.parametrize("m,s", [(24, 8), (20, 4), (20, 3), (16, 1)])
+def test_small_dense_steps_register_form_bitwise(m, s):
+    """The s x s steps of the iteration (inverse of P^T A P, the two CholeskyQR
+    factors, their products and ||C||_F; src/blcg.c:78-105) in the register form equal
+    the LDS form bit for bit: same iteration counts and the same X, FP64 and FP32-IR."""
+    n, rp, ci, v = gen.poisson3d(m, lower=True)
+    B = gen.normal(91, n * s).reshape(s, n).T.copy()
+    M = dv.Matrix(n, rp, ci, v, symmetric=True, fp32=True)
+    dB = dv.DeviceArray.from_numpy(np.ascontiguousarray(B.T))
+    lib = dv.load()
+    try:
+        for prec in (0, dv.BKH_FP32_IR):
+            out = []
+            for lds in (0, 1):
+                lib.bkh_debug_fin_lds_form(lds)
+                dX = dv.DeviceArray.from_numpy(np.zeros((s, n)))
+                info = dv.blcg(M, s, dB, dX, 1e-9, 3000, precision=prec)
+                assert info.ret >= 0 and info.true_res < 1e-9
+                out.append((info.ret, info.inner_iters, dX.numpy()))
+            assert out[0][:2] == out[1][:2]
+            assert np.array_equal(out[0][2], out[1][2])
+    finally:
+        lib.bkh_debug_fin_lds_form(0)
