@@ -263,13 +263,45 @@ __device__ __forceinline__ void reg_spmv(int n, const int (&rb)[RPT], const int 
     }
 }
 
-template <int L, int RPT, int REG_BS>
+// The same SpMV with the matrix held in registers (KD > 0): on the slice-diagonal copy a
+// wave's rows of one q lie in one slice, so the slice's positions are uniform (KD of
+// them, absent entries 0, column clamped into [0, n)) and each thread keeps its rows'
+// KD values in registers for the whole solve; an SpMV is then the LDS staging, one
+// barrier and KD LDS reads per row. Every row sums its positions in ascending offset
+// order, i.e. its stored order; a 0 value adds +-0 to a sum that starts at +0 and so
+// never holds -0, so for finite vectors the sums are bitwise those of reg_spmv.
+// xs: the staging buffer with DIA_HALO zeros before entry 0 and zeros from entry n to
+// its end, so a position's column needs no clamp (every |offset| <= DIA_HALO; an
+// absent position reads a zero or a finite entry and adds +-0 either way).
+#define DIA_HALO 128
+template <int RPT, int REG_BS, int KD>
+__device__ __forceinline__ void dia_reg_spmv(int n, const double (&dv)[RPT][KD], const int (&doff)[RPT][KD],
+                                             const double (&in)[RPT], double (&out)[RPT], double *xs) {
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+        const int i = threadIdx.x + q * REG_BS;
+        if (i < n) xs[i] = in[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+        const int i = threadIdx.x + q * REG_BS;
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < KD; k++) s = add_rn(s, mul_rn(dv[q][k], xs[i + doff[q][k]]));
+        out[q] = s;
+    }
+}
+
+template <int L, int RPT, int REG_BS, int KD = 0>
 __global__ __launch_bounds__(REG_BS) void k_bicgstabl_reg(int n, const int *rp, const int *ci, const double *val,
                                                           const int *so, const double *sval, const int *scol,
-                                                          const double *b, double *xg, Ctl *c) {
+                                                          const double *b, double *xg, Ctl *c,
+                                                          const DiaSlice *meta = nullptr, const double *dval = nullptr,
+                                                          int nsl = 0) {
     constexpr int NG = (L + 1) * (L + 2) / 2;
     constexpr int REG_NW = REG_BS / 64;
-    __shared__ double xs[2][RPT * REG_BS];
+    __shared__ double xs_mem[2][RPT * REG_BS + (KD > 0 ? 2 * DIA_HALO : 0)];
     __shared__ double shr[2][REG_NW * (NG > 2 ? NG : 2)];
     __shared__ double coef[3 * MR_D + 4];
     __shared__ int flags[2];
@@ -279,8 +311,19 @@ __global__ __launch_bounds__(REG_BS) void k_bicgstabl_reg(int n, const int *rp, 
     // reads its own slots, so no barrier guards them), the other vectors in registers
     __shared__ double rs_s[RPT * REG_BS], x_s[RPT * REG_BS];
     const int tid = threadIdx.x;
+    // slice-diagonal form: entry 0 of each staging buffer sits DIA_HALO zeros in, and
+    // the tail past n is zero
+    constexpr int XOFF = KD > 0 ? DIA_HALO : 0;
+    double *const xs0 = &xs_mem[0][XOFF], *const xs1 = &xs_mem[1][XOFF];
+    if constexpr (KD > 0) {
+        for (int e = tid; e < RPT * REG_BS + 2 * DIA_HALO; e += REG_BS)
+            if (e < DIA_HALO || e >= DIA_HALO + n) { xs_mem[0][e] = 0.0; xs_mem[1][e] = 0.0; }
+    }
     int rb[RPT], re[RPT];
     double R[L + 1][RPT], P[L + 1][RPT];
+    constexpr int KDA = KD > 0 ? KD : 1;
+    double dv[RPT][KDA];
+    int doff[RPT][KDA];
 #define RS(q) rs_s[(q) * REG_BS + tid]
 #define XX(q) x_s[(q) * REG_BS + tid]
 #pragma unroll
@@ -290,7 +333,21 @@ __global__ __launch_bounds__(REG_BS) void k_bicgstabl_reg(int n, const int *rp, 
         rb[q] = live ? rp[i] : 0;
         re[q] = live ? rp[i + 1] : 0;
         XX(q) = live ? xg[i] : 0.0;
+        if constexpr (KD > 0) {
+            // this wave's rows of q form one slice (REG_BS is a multiple of 64)
+            const int sl = __builtin_amdgcn_readfirstlane(min((tid >> 6) + q * (REG_BS / 64), nsl - 1));
+            const DiaSlice m = meta[sl];
+#pragma unroll
+            for (int k = 0; k < KD; k++) {
+                doff[q][k] = m.o[k];
+                dv[q][k] = (live && k < m.len) ? dval[m.s0 + 64 * k + (i & 63)] : 0.0;
+            }
+        }
     }
+    auto spmv = [&](const double (&in)[RPT], double (&out)[RPT], double *xsb) {
+        if constexpr (KD > 0) dia_reg_spmv<RPT, REG_BS, KDA>(n, dv, doff, in, out, xsb);
+        else reg_spmv<RPT, REG_BS>(n, rb, re, ci, val, so, sval, scol, in, out, xsb);
+    };
     int sx = 0, sr = 0;                                   // alternating LDS buffers
     const int max_it = c->max_it;
     const double tol = c->tol;
@@ -299,7 +356,7 @@ __global__ __launch_bounds__(REG_BS) void k_bicgstabl_reg(int n, const int *rp, 
         double ax[RPT], x0[RPT];
 #pragma unroll
         for (int q = 0; q < RPT; q++) x0[q] = XX(q);
-        reg_spmv<RPT, REG_BS>(n, rb, re, ci, val, so, sval, scol, x0, ax, xs[sx]);
+        spmv(x0, ax, sx ? xs1 : xs0);
         sx ^= 1;
         double v2[2] = {0.0, 0.0};
 #pragma unroll
@@ -328,7 +385,7 @@ __global__ __launch_bounds__(REG_BS) void k_bicgstabl_reg(int n, const int *rp, 
                     for (int q = 0; q < RPT; q++)               // (:64-67)
 #pragma unroll
                         for (int a = 0; a <= j; a++) P[a][q] = add_rn(mul_rn(beta, P[a][q]), R[a][q]);
-                    reg_spmv<RPT, REG_BS>(n, rb, re, ci, val, so, sval, scol, P[j], P[j + 1], xs[sx]);   // (:68-69)
+                    spmv(P[j], P[j + 1], sx ? xs1 : xs0);   // (:68-69)
                     sx ^= 1;
                     double d1[1] = {0.0};
 #pragma unroll
@@ -346,7 +403,7 @@ __global__ __launch_bounds__(REG_BS) void k_bicgstabl_reg(int n, const int *rp, 
                             for (int a = 0; a <= j; a++) R[a][q] = fma(-alpha, P[a + 1][q], R[a][q]);
                             XX(q) = fma(alpha, P[0][q], XX(q));
                         }
-                        reg_spmv<RPT, REG_BS>(n, rb, re, ci, val, so, sval, scol, R[j], R[j + 1], xs[sx]);   // (:76-77)
+                        spmv(R[j], R[j + 1], sx ? xs1 : xs0);   // (:76-77)
                         sx ^= 1;
                         if (j < L - 1) {
                             double d2[1] = {0.0};
@@ -466,6 +523,26 @@ static bool reg_launch(hipStream_t s, int n, const bkh_matrix *A, const double *
 #define REG_GO(RPT_, BS_) \
     { hipLaunchKernelGGL((k_bicgstabl_reg<L, RPT_, BS_>), dim3(1), dim3(BS_), 0, s, n, A->rp, A->ci, A->val, so, sv, sc, \
                          b, x, c); return true; }
+    // matrix values in registers on the slice-diagonal copy (<= 5 or <= 7 positions per
+    // slice): 64x64 convection-diffusion BiCGSTAB(2) 1.94 -> ? ms
+#define DIA_GO(RPT_, BS_, KD_) \
+    { hipLaunchKernelGGL((k_bicgstabl_reg<L, RPT_, BS_, KD_>), dim3(1), dim3(BS_), 0, s, n, A->rp, A->ci, A->val, so, sv, \
+                         sc, b, x, c, (const DiaSlice *)A->dmeta, (const double *)A->dval, A->nsl); return true; }
+    const bool dia = A->dmeta && A->dia_entries > 0 && A->dval && A->dia_maxoff <= DIA_HALO &&
+                     !getenv("BKH_SMALL_NODIA");
+    if constexpr (L <= 2) {
+        if (dia && A->dia_maxpos <= 5) {
+            if (n <= 1024) DIA_GO(1, 1024, 5)
+            if (n <= 2048) DIA_GO(4, 512, 5)
+            if (n <= 4096) DIA_GO(8, 512, 5)
+        }
+        if (dia && A->dia_maxpos <= 7) {
+            if (n <= 1024) DIA_GO(1, 1024, 7)
+            if (n <= 2048) DIA_GO(4, 512, 7)
+            if (n <= 4096) DIA_GO(8, 512, 7)
+        }
+    }
+#undef DIA_GO
     if constexpr (L <= 2) {
         if (n <= 1024) REG_GO(1, 1024)
         if (n <= 2048) REG_GO(2, 1024)

@@ -123,6 +123,7 @@ struct bkh_matrix {
     int *doff = nullptr;               // column offset of stored position k of slice q at dso[q] / 64 + k
     DiaSlice *dmeta = nullptr;         // per-slice records (nullptr: some slice does not qualify)
     int dia_maxpos = 0;                // most stored positions of any slice
+    int dia_maxoff = 0;                // largest |column offset| of the copy
     int variant = SPMV_WAVE;           // kernel used by every SpMV on this matrix
     int max_row = 0;                   // longest row (nonzeros)
     bkh_matrix *T = nullptr;           // A^T in the same format (BiCG)

@@ -526,6 +526,9 @@ static bool upload_dia(bkh_matrix *A, int n, const std::vector<int> &rp, const s
     if (!ok) return false;
     A->dia_entries = tot;
     A->dia_maxpos = maxpos;
+    int maxoff = 0;
+    for (size_t e = 0; e + 8 < offs.size(); e++) maxoff = std::max(maxoff, std::abs(offs[e]));
+    A->dia_maxoff = maxoff;
     return true;
 }
 

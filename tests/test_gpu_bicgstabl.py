@@ -178,3 +178,33 @@ def test_register_kernel_matches_global_memory_kernel(monkeypatch, l):
     assert ir.ret >= 0 and abs(ir.ret - ig.ret) <= 1 and ir.true_res < 2e-10
     print(f"l={l}: register kernel {ir.ret} outer, {ir.time_ms:.3f} ms; global-memory kernel {ig.ret}, "
           f"{ig.time_ms:.3f} ms")
+
+
+@pytest.mark.parametrize("l", [1, 2])
+def test_register_matrix_kernel_bitwise(monkeypatch, l):
+    """On the slice-diagonal copy the single-workgroup kernel also holds the matrix
+    values in registers (BKH_SMALL_NODIA=1 selects the kernel that streams them from
+    L2 each SpMV): the same sums in the same order, so bitwise the same iterates and
+    the same convergence index."""
+    n, rp, ci, v, xt, b = cfg1()
+    M = dv.Matrix(n, rp, ci, v)
+    assert M.dia_entries > 0
+    db = dv.DeviceArray.from_numpy(b)
+    res = {}
+    for mode in ("dia", "stream"):
+        if mode == "stream":
+            monkeypatch.setenv("BKH_SMALL_NODIA", "1")
+        out = []
+        for k in (1, 5):
+            dx = dv.DeviceArray.from_numpy(np.zeros(n))
+            dv.bicgstabl(M, db, l, dx, 1e-300, k, path=2)
+            out.append(dx.numpy())
+        dx = dv.DeviceArray.from_numpy(np.zeros(n))
+        info = dv.bicgstabl(M, db, l, dx, 1e-10, 2000, path=2)
+        out.append(dx.numpy())
+        res[mode] = (out, info)
+    (xd, idia), (xs, ist) = res["dia"], res["stream"]
+    for a, c in zip(xd, xs):
+        assert np.array_equal(a, c)
+    assert idia.ret == ist.ret and idia.ret >= 0 and idia.true_res < 2e-10
+    print(f"l={l}: matrix in registers {idia.ret} outer, {idia.time_ms:.3f} ms; streamed {ist.time_ms:.3f} ms")
