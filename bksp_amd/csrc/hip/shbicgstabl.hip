@@ -39,6 +39,9 @@ struct ShState {
     double cA[SH_MAXS], cB[SH_MAXS], cC[SH_MAXS], cD[SH_MAXS], cE[SH_MAXS], ak[SH_MAXS];
     // MR-step coefficients of k_sh_upd
     double xc[SH_MAXS][MR_LMAX], psi_inv[SH_MAXS], om[MR_D];
+    // every step's coefficients (cA, cB, cC, cD, cE, ak of step j at ck[k][j]) for the
+    // fused per-shift pass, shift-major so a shift's are read with scalar loads
+    double ck[SH_MAXS][MR_LMAX][6];
 };
 
 // r0 = b - A x0, p0 = rs = r0 (:32-42); pi = pi_old = theta = 1 (:60-66)
@@ -60,7 +63,7 @@ struct ShInitFin {
 // and the coefficients of this step's fused update, one shift per thread; then
 // pi_old <- pi, pi <- pi_new and alpha_old <- alpha (:156-161).
 __global__ __launch_bounds__(256) void k_sh_alpha_fin(const double *part, int nblocks, Ctl *c,
-                                                      ShState *st) {
+                                                      ShState *st, int j) {
     if (ctl_stopped(c)) return;
     double acc[1];
     sum_partials<1>(part, nblocks, acc);
@@ -92,6 +95,8 @@ __global__ __launch_bounds__(256) void k_sh_alpha_fin(const double *part, int nb
         st->cD[k] = -1.0 / pn;
         st->cE[k] = 1.0 / (th * ak);
         st->ak[k] = ak;
+        double *ck = st->ck[k][j];
+        ck[0] = st->cA[k]; ck[1] = st->cB[k]; ck[2] = st->cC[k]; ck[3] = st->cD[k]; ck[4] = st->cE[k]; ck[5] = ak;
         st->pi_old[k] = pi;
         st->pi[k] = pn;
         if (!(pn == pn) || pn == 0.0) ctl_breakdown(c);
@@ -227,12 +232,18 @@ __global__ __launch_bounds__(256) void k_sh_p(long long n, int j, double *P, con
 
 // Seed part of BiCG step j: r_q -= alpha p_{q+1} (q <= j), x += alpha p_0
 // (:107-108, :137-139). Runs after the shift kernel, which reads the old r.
+// snap (fused per-shift pass): r_q before this update goes to snap[j(j+1)/2 + q].
 __global__ __launch_bounds__(256) void k_sh_seed_r(long long n, int j, const double *P, double *R,
-                                                   double *x, const Ctl *c) {
+                                                   double *x, const Ctl *c, double *snap) {
     if (ctl_stopped(c)) return;
     const double al = c->s[SB_ALPHA];
+    double *sj = snap ? snap + (size_t)(j * (j + 1) / 2) * n : nullptr;
     ROWLOOP(i, n) {
-        for (int q = 0; q <= j; q++) R[q * n + i] = fma(-al, P[(q + 1) * n + i], R[q * n + i]);
+        for (int q = 0; q <= j; q++) {
+            const double r = R[q * n + i];
+            if (sj) sj[q * n + i] = r;
+            R[q * n + i] = fma(-al, P[(q + 1) * n + i], r);
+        }
         x[i] = fma(al, P[i], x[i]);
     }
 }
@@ -389,13 +400,84 @@ __global__ __launch_bounds__(256) void k_sh_upd(long long n, int G, const double
     }
 }
 
+// Every per-shift vector operation of one outer iteration in one pass: the l BiCG
+// steps of k_sh_step (shbicgstabl.c:111-134, :146-153) and the MR update of k_sh_upd
+// (:202-237). p_shift[k][1..l] live only within an outer iteration (step j writes
+// p_shift[k][j+1] before step j+1 reads it), so a thread keeps them in registers and
+// reads and writes only p_shift[k][0] and x_k, shift-major (shift k at k n, the
+// reference's own x layout). The seed's r_q at the start of step j come from the
+// snapshots k_sh_seed_r wrote (snap[j(j+1)/2 + q]); r_j after step j's update is
+// snap[(j+1)(j+2)/2 + j], or the final r_{l-1}; the MR update uses the final r_0..r_{l-1}.
+// Each element goes through k_sh_step's and k_sh_upd's operations in order with the
+// same roundings to PT, so the iterates are bitwise those of the per-step kernels, while
+// each shift moves 4 n PT per outer iteration instead of (l + 1)(l + 4) n. A workgroup
+// row (blockIdx.y) takes kchunk shifts; a shift's coefficients are uniform loads.
+template <int L, typename PT>
+__global__ __launch_bounds__(256) void k_sh_fused(long long n, int ns, int kchunk, const double *R,
+                                                  const double *snap, PT *XSM, PT *PSM, const ShState *st,
+                                                  const Ctl *c) {
+    if (ctl_stopped(c)) return;
+    constexpr int NS = L * (L + 1) / 2;
+    const int kb = (int)blockIdx.y * kchunk, ke = min(ns, kb + kchunk);
+    double om[L + 1];
+#pragma unroll
+    for (int q = 1; q <= L; q++) om[q] = st->om[q];
+    ROWLOOP(i, n) {
+        double sn[NS], rf[L];
+#pragma unroll
+        for (int e = 0; e < NS; e++) sn[e] = snap[(size_t)e * n + i];
+#pragma unroll
+        for (int q = 0; q < L; q++) rf[q] = R[(size_t)q * n + i];
+        for (int k = kb; k < ke; k++) {
+            const double *ck = st->ck[k][0];
+            const double sg = st->sigma[k];
+            PT p[L + 1];
+            p[0] = PSM[(size_t)k * n + i];
+            PT x = XSM[(size_t)k * n + i];
+#pragma unroll
+            for (int j = 0; j < L; j++) {
+                const double cA = ck[6 * j], cB = ck[6 * j + 1], cC = ck[6 * j + 2], cD = ck[6 * j + 3],
+                             cE = ck[6 * j + 4], ak = ck[6 * j + 5];
+                const int s0 = j * (j + 1) / 2;
+                const double rjn = (j + 1 < L) ? sn[(j + 1) * (j + 2) / 2 + j] : rf[L - 1];
+                PT out[L + 1];
+                double p0 = 0.0, pj = 0.0;
+#pragma unroll
+                for (int q = 0; q <= j; q++) {
+                    const PT v = (PT)fma(cB, sn[s0 + q], mul_rn(cA, (double)p[q]));
+                    out[q] = v;
+                    if (q == 0) p0 = (double)v;
+                    if (q == j) pj = (double)v;
+                }
+                double t = mul_rn(sn[s0 + j], cC);
+                t = fma(cD, rjn, t);
+                t = mul_rn(t, cE);
+                t = fma(-sg, pj, t);
+                out[j + 1] = (PT)t;
+                x = (PT)fma(ak, p0, (double)x);
+#pragma unroll
+                for (int q = 0; q <= j + 1; q++) p[q] = out[q];
+            }
+            double xk = (double)x;
+#pragma unroll
+            for (int q = 0; q < L; q++) xk += st->xc[k][q] * rf[q];
+            double pp = (double)p[0];
+#pragma unroll
+            for (int q = 1; q <= L; q++) pp -= om[q] * (double)p[q];
+            XSM[(size_t)k * n + i] = (PT)xk;
+            PSM[(size_t)k * n + i] = (PT)(pp * st->psi_inv[k]);
+        }
+    }
+}
+
 template <typename PT>
 __global__ __launch_bounds__(256) void k_sh_zero(PT *p, long long n) { ROWLOOP(i, n) p[i] = (PT)0; }
 
-// Tiled transposes between the reference's x layout (shift k at x + (k+1) n) and
-// the interleaved n x SP block (32 x 32 tiles through LDS, both sides coalesced).
-template <typename PT>
-__global__ __launch_bounds__(256) void k_xs_in(long long n, int ns, int SP, const double *x, PT *XS) {
+// Tiled transposes between shift-major blocks (shift k at xm + k n: the reference's x
+// layout from x + n) and the interleaved n x SP block (32 x 32 tiles through LDS, both
+// sides coalesced).
+template <typename ST, typename PT>
+__global__ __launch_bounds__(256) void k_xs_in(long long n, int ns, int SP, const ST *xm, PT *XS) {
     __shared__ double tile[32][33];
     const long long i0 = (long long)blockIdx.x * 32;
     const int k0 = blockIdx.y * 32, tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
@@ -403,7 +485,7 @@ __global__ __launch_bounds__(256) void k_xs_in(long long n, int ns, int SP, cons
     for (int jj = 0; jj < 4; jj++) {
         const int k = k0 + ty + 8 * jj;
         const long long i = i0 + tx;
-        tile[ty + 8 * jj][tx] = (k < ns && i < n) ? x[(size_t)(k + 1) * n + i] : 0.0;
+        tile[ty + 8 * jj][tx] = (k < ns && i < n) ? (double)xm[(size_t)k * n + i] : 0.0;
     }
     __syncthreads();
 #pragma unroll
@@ -710,6 +792,11 @@ __global__ __launch_bounds__(256) void k_to_f64(long long m, const PT *a, double
     ROWLOOP(i, m) b[i] = (double)a[i];
 }
 
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void k_conv(long long m, const TI *a, TO *b) {
+    ROWLOOP(i, m) b[i] = (TO)a[i];
+}
+
 struct MsBufs {
     float *R, *RH, *D, *P, *V, *S, *T;
 };
@@ -828,14 +915,14 @@ static void sh_gram_spmv(hipStream_t s, const bkh_matrix *A, double *R, long lon
 
 // p_{j+1} = A p_j with rs.p_{j+1}, then the per-shift alpha finalize.
 static void sh_alpha_spmv(hipStream_t s, const bkh_matrix *A, const double *x, double *y, const double *rs,
-                          Ctl *c, double *part, ShState *st) {
+                          Ctl *c, double *part, ShState *st, int j) {
     Epi<1, double> e;
     memset(&e, 0, sizeof e);
     e.w[0] = rs;
     e.a[0] = 0;
     e.b[0] = EPI_Y;
     const int G = launch_spmv_kernel<double, double, 1>(s, A, A->val, x, y, 1.0, 0.0, e, c, part, NoFin());
-    hipLaunchKernelGGL(k_sh_alpha_fin, dim3(1), dim3(256), 0, s, (const double *)part, G, c, st);
+    hipLaunchKernelGGL(k_sh_alpha_fin, dim3(1), dim3(256), 0, s, (const double *)part, G, c, st, j);
 }
 
 template <typename PT>
@@ -860,6 +947,20 @@ static void sh_shift_upd(int l, const ShGrid &g, hipStream_t s, long long n, con
     }
 }
 
+// The fused per-shift pass (k_sh_fused): rows x chunks of SH_KCHUNK shifts.
+#define SH_KCHUNK 16
+template <typename PT>
+static void sh_shift_fused(int l, hipStream_t s, long long n, int ns, const double *R, const double *snap, PT *XSM,
+                           PT *PSM, const ShState *st, const Ctl *c) {
+    const dim3 grid((unsigned)vec_grid(n, 256, 1), (unsigned)((ns + SH_KCHUNK - 1) / SH_KCHUNK));
+    switch (l) {
+#define CASE(L_) case L_: hipLaunchKernelGGL((k_sh_fused<L_, PT>), grid, dim3(256), 0, s, n, ns, SH_KCHUNK, R, snap, XSM, PSM, st, c); break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
+        default: hipLaunchKernelGGL((k_sh_fused<8, PT>), grid, dim3(256), 0, s, n, ns, SH_KCHUNK, R, snap, XSM, PSM, st, c); break;
+#undef CASE
+    }
+}
+
 static void sh_gram_dispatch(int l, hipStream_t s, const bkh_matrix *A, double *R, long long n, Ctl *c,
                              double *part, ShState *st) {
     switch (l) {
@@ -874,13 +975,17 @@ static void sh_gram_dispatch(int l, hipStream_t s, const bkh_matrix *A, double *
     }
 }
 
-// Shifted BiCGSTAB(l) with the shifts' p_shift and x_k in interleaved PT blocks:
-// the seed in x[0, n), shift k's solution in XS (n x SP), starting from x's shifted
-// parts (the reference's x + (k+1) n layout).
+// Shifted BiCGSTAB(l). Unfused (snap == nullptr): the shifts' p_shift and x_k in
+// interleaved PT blocks, the seed in x[0, n), shift k's solution in XS (n x SP),
+// starting from x's shifted parts (the reference's x + (k+1) n layout). Fused (snap: the
+// seed's r snapshots, l(l+1)/2 n doubles): p_shift[k][0] in PS and x_k in XS, both
+// shift-major (XS is x + n itself when PT is double), one k_sh_fused pass per outer
+// iteration.
 template <typename PT>
 static int sh_core(ThreadCtx *t, const bkh_matrix *A, int l, const double *b, double *x, double *R,
                    double *P, double *rs, PT *PS, PT *XS, int SP, ShState *st, const double *sigma, int ns,
-                   Ctl *c, double *part, double tol, int max_it, int K) {
+                   Ctl *c, double *part, double tol, int max_it, int K, double *snap) {
+    const bool fused = snap != nullptr;
     hipStream_t s = t->stream;
     const long long n = A->n;
     const int G = vec_grid(n, 256);
@@ -897,11 +1002,17 @@ static int sh_core(ThreadCtx *t, const bkh_matrix *A, int l, const double *b, do
         delete h;
         if (e != hipSuccess || e2 != hipSuccess) { bkh_set_error("ShState upload failed"); return BKH_DEVICE_ERROR; }
     }
-    if (ns > 0) {
+    if (ns > 0 && fused) {
+        const long long nps = (long long)ns * n;                // p_shift[k][0] starts at 0 (:53 calloc)
+        hipLaunchKernelGGL(k_sh_zero<PT>, dim3(vec_grid(nps, 256)), dim3(256), 0, s, PS, nps);
+        if ((const void *)XS != (const void *)(x + n))
+            hipLaunchKernelGGL((k_conv<double, PT>), dim3(vec_grid(nps, 256)), dim3(256), 0, s, nps,
+                               (const double *)(x + n), XS);
+    } else if (ns > 0) {
         const long long nps = (long long)SP * (l + 1) * n;     // p_shift starts at 0 (:53 calloc)
         hipLaunchKernelGGL(k_sh_zero<PT>, dim3(vec_grid(nps, 256)), dim3(256), 0, s, PS, nps);
-        hipLaunchKernelGGL(k_xs_in<PT>, dim3((unsigned)((n + 31) / 32), (unsigned)((SP + 31) / 32)), dim3(256), 0, s,
-                           n, ns, SP, (const double *)x, XS);
+        hipLaunchKernelGGL((k_xs_in<double, PT>), dim3((unsigned)((n + 31) / 32), (unsigned)((SP + 31) / 32)), dim3(256),
+                           0, s, n, ns, SP, (const double *)(x + n), XS);
     }
     Epi<0> e0;
     memset(&e0, 0, sizeof e0);
@@ -912,19 +1023,21 @@ static int sh_core(ThreadCtx *t, const bkh_matrix *A, int l, const double *b, do
     hipGraphExec_t ge = capture_cached(t, {0x5B01, (uintptr_t)A, (uintptr_t)b, (uintptr_t)x, (uintptr_t)R, (uintptr_t)P,
                                            (uintptr_t)rs, (uintptr_t)PS, (uintptr_t)XS, (uintptr_t)SP, (uintptr_t)st,
                                            (uintptr_t)c, (uintptr_t)part, (uintptr_t)ns, (uintptr_t)l,
-                                           (uintptr_t)sizeof(PT)}, s, K, [&]() {
+                                           (uintptr_t)sizeof(PT), (uintptr_t)snap}, s, K, [&]() {
         for (int j = 0; j < l; j++) {
             hipLaunchKernelGGL(k_sh_p, dim3(G), dim3(256), 0, s, n, j, P, (const double *)R, c);
-            sh_alpha_spmv(s, A, P + j * n, P + (j + 1) * n, rs, c, part, st);
-            if (ns > 0) sh_shift_step<PT>(j, g, s, n, (const double *)P, (const double *)R, XS, PS, st, c);
-            hipLaunchKernelGGL(k_sh_seed_r, dim3(G), dim3(256), 0, s, n, j, (const double *)P, R, x, c);
+            sh_alpha_spmv(s, A, P + j * n, P + (j + 1) * n, rs, c, part, st, j);
+            if (ns > 0 && !fused) sh_shift_step<PT>(j, g, s, n, (const double *)P, (const double *)R, XS, PS, st, c);
+            hipLaunchKernelGGL(k_sh_seed_r, dim3(G), dim3(256), 0, s, n, j, (const double *)P, R, x, c,
+                               (ns > 0 && fused) ? snap : (double *)nullptr);
             if (j < l - 1)
                 launch_spmv_dot_w<double, double>(s, A, A->val, R + j * n, R + (j + 1) * n, rs, c, part,
                                                   ShGammaFin());
             else
                 sh_gram_dispatch(l, s, A, R, n, c, part, st);
         }
-        if (ns > 0) sh_shift_upd<PT>(l, g, s, n, (const double *)R, XS, PS, st, c);
+        if (ns > 0 && fused) sh_shift_fused<PT>(l, s, n, ns, (const double *)R, (const double *)snap, XS, PS, st, c);
+        else if (ns > 0) sh_shift_upd<PT>(l, g, s, n, (const double *)R, XS, PS, st, c);
         hipLaunchKernelGGL(k_sh_seed_upd, dim3(G), dim3(256), 0, s, n, l, P, R, x, (const double *)rs, c, part);
         launch_finalize<2>(s, part, G, c, ShUpdFin());
     });
@@ -966,9 +1079,13 @@ extern "C" int bkh_shbicgstabl(bkh_matrix *A, const double *b, int l, const doub
     const size_t pts = p32 ? sizeof(float) : sizeof(double);
     const size_t blk = (size_t)SP * n;                     // one interleaved block
     const bool shifts = ns > 0;
+    // the fused per-shift pass unless BKH_SH_UNFUSED=1 (the per-step kernels, bitwise the same)
+    const char *uf = getenv("BKH_SH_UNFUSED");
+    const bool fused = shifts && !(uf && uf[0] == '1');
     size_t bytes = carve_size(sizeof(Ctl)) * 2 + carve_size(sizeof(ShState)) + carve_size(sizeof(double) * nparts) +
                    (2 * (l + 1) + 1) * carve_size(sizeof(double) * n) + carve_size(sizeof(MsCol) * SP) +
                    carve_size(sizeof(double) * (size_t)MS_NB * 2 * SP);
+    if (fused) bytes += carve_size(sizeof(double) * n * (size_t)(l * (l + 1) / 2));
     if (shifts) {
         bytes += carve_size(pts * blk * (l + 1)) + carve_size(pts * blk);       // p_shift, x_k
         if (p32) bytes += carve_size(sizeof(double) * blk) + 7 * carve_size(sizeof(float) * blk);
@@ -985,6 +1102,9 @@ extern "C" int bkh_shbicgstabl(bkh_matrix *A, const double *b, int l, const doub
     char *PS = shifts ? cv.take<char>(pts * blk * (l + 1)) : nullptr;
     char *XS = shifts ? cv.take<char>(pts * blk) : nullptr;
     double *XS64 = p32 ? nullptr : (double *)XS;
+    double *snap = fused ? cv.take<double>(n * (size_t)(l * (l + 1) / 2)) : nullptr;
+    // fused: x_k shift-major, in x itself (FP64) or in a shift-major FP32 copy
+    char *XW = (fused && !p32) ? (char *)(x + n) : XS;
     MsBufs B;
     memset(&B, 0, sizeof B);
     if (shifts && p32) {
@@ -994,16 +1114,20 @@ extern "C" int bkh_shbicgstabl(bkh_matrix *A, const double *b, int l, const doub
     }
     hipStream_t s = t->stream;
     HIP_OK(hipEventRecord(t->ev0, s));
-    int rc = p32 ? sh_core<float>(t, A, l, b, x, R, P, rs, (float *)PS, (float *)XS, SP, st, sigma, ns, c, part, tol,
-                                  max_iter, K)
-                 : sh_core<double>(t, A, l, b, x, R, P, rs, (double *)PS, (double *)XS, SP, st, sigma, ns, c, part,
-                                   tol, max_iter, K);
+    int rc = p32 ? sh_core<float>(t, A, l, b, x, R, P, rs, (float *)PS, (float *)XW, SP, st, sigma, ns, c, part, tol,
+                                  max_iter, K, snap)
+                 : sh_core<double>(t, A, l, b, x, R, P, rs, (double *)PS, (double *)XW, SP, st, sigma, ns, c, part,
+                                   tol, max_iter, K, snap);
     if (rc) return rc;
     const double fnb = t->ctl_host->fnb;
     if (shifts && p32) {
         // FP64 refinement of every shift (the FP32 p_shift limits them to ~kappa eps32)
-        hipLaunchKernelGGL(k_to_f64<float>, dim3(vec_grid((long long)blk, 256)), dim3(256), 0, s, (long long)blk,
-                           (const float *)XS, XS64);
+        if (fused)
+            hipLaunchKernelGGL((k_xs_in<float, double>), dim3((unsigned)((n + 31) / 32), (unsigned)((SP + 31) / 32)),
+                               dim3(256), 0, s, n, ns, SP, (const float *)XS, XS64);
+        else
+            hipLaunchKernelGGL(k_to_f64<float>, dim3(vec_grid((long long)blk, 256)), dim3(256), 0, s, (long long)blk,
+                               (const float *)XS, XS64);
         LAUNCH_OK();
         if (fnb > 0.0) {
             const int max_inner = o.max_inner > 0 ? o.max_inner : 1000;
@@ -1012,9 +1136,15 @@ extern "C" int bkh_shbicgstabl(bkh_matrix *A, const double *b, int l, const doub
                 return BKH_DEVICE_ERROR;
         }
     }
-    if (shifts) {
+    if (shifts && (p32 || !fused)) {
         hipLaunchKernelGGL(k_xs_out<double>, dim3((unsigned)((n + 31) / 32), (unsigned)((SP + 31) / 32)), dim3(256), 0,
                            s, n, ns, SP, (const double *)XS64, x);
+        LAUNCH_OK();
+    } else if (shifts && shift_true_res) {
+        // fused FP64: the solutions are already in x; the per-shift residual kernel reads
+        // the interleaved block
+        hipLaunchKernelGGL((k_xs_in<double, double>), dim3((unsigned)((n + 31) / 32), (unsigned)((SP + 31) / 32)),
+                           dim3(256), 0, s, n, ns, SP, (const double *)(x + n), XS64);
         LAUNCH_OK();
     }
     if (ctl_fetch(t, c)) return BKH_DEVICE_ERROR;          // the seed's control block for finish_solve

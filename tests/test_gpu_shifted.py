@@ -162,3 +162,30 @@ def test_dropin_shifted_bicg_reference_scenario(pde2961):
     X = x.reshape(5, n)
     for q in range(5):
         assert rel(X[q], Xk[q]) < 1e-8, (q, rel(X[q], Xk[q]))
+
+
+@pytest.mark.parametrize("l,ns,prec", [(4, 64, 0), (4, 64, 1), (1, 3, 0), (2, 17, 0), (3, 5, 1)])
+def test_fused_shift_pass_bitwise(monkeypatch, l, ns, prec):
+    """The per-shift work of an outer iteration in one pass (p_shift[k][1..l] in
+    registers, the seed's r from snapshots) gives bitwise the iterates of the per-step
+    kernels (BKH_SH_UNFUSED=1): the seed and every shifted solution, after 3 outer
+    iterations and at convergence, FP64 and FP32-IR."""
+    n, rp, ci, v, b, sig = cfg3(20)
+    sig = sig[::max(1, 64 // ns)][:ns]
+    M = dv.Matrix(n, rp, ci, v, fp32=True)
+    db = dv.DeviceArray.from_numpy(b)
+    out = {}
+    for mode in ("fused", "unfused"):
+        if mode == "unfused":
+            monkeypatch.setenv("BKH_SH_UNFUSED", "1")
+        res = []
+        for tol, mit in ((1e-300, 3), (1e-8, 1000)):
+            dx = dv.DeviceArray.from_numpy(np.zeros(n * (ns + 1)))
+            info, tr = dv.shbicgstabl(M, db, l, sig, dx, tol, mit, precision=prec)
+            res.append((info.ret, info.inner_iters, dx.numpy(), tr))
+        out[mode] = res
+    for (ra, ia, xa, ta), (rb, ib, xb, tb) in zip(out["fused"], out["unfused"]):
+        assert ra == rb and ia == ib
+        assert np.array_equal(xa, xb)
+        assert np.array_equal(ta, tb)
+    assert out["fused"][1][0] >= 0 and np.all(out["fused"][1][3] < 2e-8)
