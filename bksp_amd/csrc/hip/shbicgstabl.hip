@@ -29,6 +29,7 @@
 #include "bkh_spmv.hpp"
 
 #define SH_MAXS 256
+#define SH_KCHUNK 16        // shifts per workgroup row of the fused per-shift pass
 
 enum { SB_G0 = 0, SB_ALPHA = 1, SB_BETA = 2, SB_OML = 3, SB_AOLD = 4, SB_CX = 8, SB_CR = 20, SB_CP = 32 };
 
@@ -418,54 +419,75 @@ __global__ __launch_bounds__(256) void k_sh_fused(long long n, int ns, int kchun
                                                   const Ctl *c) {
     if (ctl_stopped(c)) return;
     constexpr int NS = L * (L + 1) / 2;
+    constexpr int CW = 6 * L + L + 2;                    // a shift's coefficients: steps, xc, 1/psi, sigma
+    constexpr int U = 4;                                 // shifts in flight per thread
+    __shared__ double cks[SH_KCHUNK][CW];
     const int kb = (int)blockIdx.y * kchunk, ke = min(ns, kb + kchunk);
+    for (int e = threadIdx.x; e < (ke - kb) * CW; e += 256) {
+        const int k = kb + e / CW, w = e % CW;
+        cks[e / CW][w] = w < 6 * L ? st->ck[k][w / 6][w % 6]
+                       : w < 7 * L ? st->xc[k][w - 6 * L] : (w == 7 * L ? st->psi_inv[k] : st->sigma[k]);
+    }
     double om[L + 1];
 #pragma unroll
     for (int q = 1; q <= L; q++) om[q] = st->om[q];
+    __syncthreads();
     ROWLOOP(i, n) {
         double sn[NS], rf[L];
 #pragma unroll
         for (int e = 0; e < NS; e++) sn[e] = snap[(size_t)e * n + i];
 #pragma unroll
         for (int q = 0; q < L; q++) rf[q] = R[(size_t)q * n + i];
-        for (int k = kb; k < ke; k++) {
-            const double *ck = st->ck[k][0];
-            const double sg = st->sigma[k];
-            PT p[L + 1];
-            p[0] = PSM[(size_t)k * n + i];
-            PT x = XSM[(size_t)k * n + i];
+        for (int k0 = kb; k0 < ke; k0 += U) {
+            PT pl[U], xl[U];
 #pragma unroll
-            for (int j = 0; j < L; j++) {
-                const double cA = ck[6 * j], cB = ck[6 * j + 1], cC = ck[6 * j + 2], cD = ck[6 * j + 3],
-                             cE = ck[6 * j + 4], ak = ck[6 * j + 5];
-                const int s0 = j * (j + 1) / 2;
-                const double rjn = (j + 1 < L) ? sn[(j + 1) * (j + 2) / 2 + j] : rf[L - 1];
-                PT out[L + 1];
-                double p0 = 0.0, pj = 0.0;
-#pragma unroll
-                for (int q = 0; q <= j; q++) {
-                    const PT v = (PT)fma(cB, sn[s0 + q], mul_rn(cA, (double)p[q]));
-                    out[q] = v;
-                    if (q == 0) p0 = (double)v;
-                    if (q == j) pj = (double)v;
-                }
-                double t = mul_rn(sn[s0 + j], cC);
-                t = fma(cD, rjn, t);
-                t = mul_rn(t, cE);
-                t = fma(-sg, pj, t);
-                out[j + 1] = (PT)t;
-                x = (PT)fma(ak, p0, (double)x);
-#pragma unroll
-                for (int q = 0; q <= j + 1; q++) p[q] = out[q];
+            for (int u = 0; u < U; u++) {                // loads of U shifts in flight together
+                const int k = min(k0 + u, ke - 1);
+                pl[u] = PSM[(size_t)k * n + i];
+                xl[u] = XSM[(size_t)k * n + i];
             }
-            double xk = (double)x;
 #pragma unroll
-            for (int q = 0; q < L; q++) xk += st->xc[k][q] * rf[q];
-            double pp = (double)p[0];
+            for (int u = 0; u < U; u++) {
+                if (k0 + u >= ke) break;
+                const double *cf = cks[k0 + u - kb];
+                const double sg = cf[7 * L + 1];
+                PT p[L + 1];
+                p[0] = pl[u];
+                PT x = xl[u];
 #pragma unroll
-            for (int q = 1; q <= L; q++) pp -= om[q] * (double)p[q];
-            XSM[(size_t)k * n + i] = (PT)xk;
-            PSM[(size_t)k * n + i] = (PT)(pp * st->psi_inv[k]);
+                for (int j = 0; j < L; j++) {
+                    const double cA = cf[6 * j], cB = cf[6 * j + 1], cC = cf[6 * j + 2], cD = cf[6 * j + 3],
+                                 cE = cf[6 * j + 4], ak = cf[6 * j + 5];
+                    const int s0 = j * (j + 1) / 2;
+                    const double rjn = (j + 1 < L) ? sn[(j + 1) * (j + 2) / 2 + j] : rf[L - 1];
+                    PT out[L + 1];
+                    double p0 = 0.0, pj = 0.0;
+#pragma unroll
+                    for (int q = 0; q <= j; q++) {
+                        const PT v = (PT)fma(cB, sn[s0 + q], mul_rn(cA, (double)p[q]));
+                        out[q] = v;
+                        if (q == 0) p0 = (double)v;
+                        if (q == j) pj = (double)v;
+                    }
+                    double t = mul_rn(sn[s0 + j], cC);
+                    t = fma(cD, rjn, t);
+                    t = mul_rn(t, cE);
+                    t = fma(-sg, pj, t);
+                    out[j + 1] = (PT)t;
+                    x = (PT)fma(ak, p0, (double)x);
+#pragma unroll
+                    for (int q = 0; q <= j + 1; q++) p[q] = out[q];
+                }
+                double xk = (double)x;
+#pragma unroll
+                for (int q = 0; q < L; q++) xk += cf[6 * L + q] * rf[q];
+                double pp = (double)p[0];
+#pragma unroll
+                for (int q = 1; q <= L; q++) pp -= om[q] * (double)p[q];
+                const int k = k0 + u;
+                XSM[(size_t)k * n + i] = (PT)xk;
+                PSM[(size_t)k * n + i] = (PT)(pp * cf[7 * L]);
+            }
         }
     }
 }
@@ -948,7 +970,6 @@ static void sh_shift_upd(int l, const ShGrid &g, hipStream_t s, long long n, con
 }
 
 // The fused per-shift pass (k_sh_fused): rows x chunks of SH_KCHUNK shifts.
-#define SH_KCHUNK 16
 template <typename PT>
 static void sh_shift_fused(int l, hipStream_t s, long long n, int ns, const double *R, const double *snap, PT *XSM,
                            PT *PSM, const ShState *st, const Ctl *c) {
