@@ -850,6 +850,7 @@ struct ZSh {
     zd cA, cB, cC, d1, d2;           // coefficients of the BiCG step's p_shift updates
     zd cx[ZS_LMAX];                  // x_k coefficients of the MR update
     zd cpsi;                         // 1 / psi
+    zd ck[ZS_LMAX][6];               // every step's cA, cB, cC, d1, d2, alpha_s (fused pass)
 };
 
 __global__ void k_zsh_setup(Ctl *c, ZSh *sh, const double *sigma, int ns, int l) {
@@ -910,7 +911,7 @@ __global__ __launch_bounds__(256) void k_zsh_x(long long n, zd *x, const zd *P, 
 }
 
 // Per-shift scalars of step j (:414-424) and the coefficients of both p_shift updates.
-__global__ void k_zsh_sc1(int ns, ZSh *sh, const Ctl *c) {
+__global__ void k_zsh_sc1(int ns, ZSh *sh, const Ctl *c, int j) {
     if (ctl_stopped(c)) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ns) return;
@@ -929,6 +930,8 @@ __global__ void k_zsh_sc1(int ns, ZSh *sh, const Ctl *c) {
     z.cC = zdiv(one, z.pi);
     z.d1 = zdiv(zmk(-1.0, 0.0), z.pi_new);
     z.d2 = zdiv(one, zmul(z.theta, z.alpha_s));
+    z.ck[j][0] = z.cA; z.ck[j][1] = z.cB; z.ck[j][2] = z.cC;
+    z.ck[j][3] = z.d1; z.ck[j][4] = z.d2; z.ck[j][5] = z.alpha_s;
     sh[k] = z;
 }
 
@@ -951,14 +954,21 @@ __global__ __launch_bounds__(256) void k_zsh_v1(long long n, int j, int l2, zd *
 }
 
 // r_i += (-alpha) p_{i+1}, i <= j (:443-446).
-__global__ __launch_bounds__(256) void k_zsh_r(long long n, int j, zd *R, const zd *P, const Ctl *c) {
+// snap (fused per-shift pass): r_i before this update to snap[j(j+1)/2 + i], and on the
+// last step (j = l - 1) r_j after it to snap[l(l+1)/2].
+__global__ __launch_bounds__(256) void k_zsh_r(long long n, int j, zd *R, const zd *P, const Ctl *c, zd *snap,
+                                               int l) {
     if (ctl_stopped(c)) return;
     const zd mal = zneg(zload(c->s, ZS_ALPHA / 2));
+    zd *sj = snap ? snap + (size_t)(j * (j + 1) / 2) * n : nullptr;
     ZLOOP(e, n) {
         for (int i = 0; i <= j; i++) {
             const size_t k = (size_t)i * n + e;
-            R[k] = zadd(R[k], zmul(mal, P[k + n]));
+            const zd r = R[k];
+            if (sj) sj[(size_t)i * n + e] = r;
+            R[k] = zadd(r, zmul(mal, P[k + n]));
         }
+        if (sj && j == l - 1) snap[(size_t)(l * (l + 1) / 2) * n + e] = R[(size_t)j * n + e];
     }
 }
 
@@ -1110,6 +1120,89 @@ __global__ __launch_bounds__(256) void k_zsh_v3(long long n, int l, zd *PS, zd *
     }
 }
 
+// Every per-shift vector operation of one outer iteration in one pass: k_zsh_v1 and
+// k_zsh_v2 of each BiCG step j (:426-437, :452-461) and k_zsh_v3 (:551-557).
+// p_shift[k][1..l] live only within an outer iteration, so a thread keeps them in
+// registers and reads and writes only p_shift[k][0] (at PS + k (l+1) n) and x_k (at
+// x + (k+1) n). The seed's r_i before step j's update come from the snapshots k_zsh_r
+// wrote, r_j after it from the next step's snapshot (the last step's has its own slot,
+// since the MR part's Gram-Schmidt then rewrites r_2..r_l in place), and the MR update
+// reads R as k_zsh_v3 does. The same complex operations in the same order as those
+// kernels, so the iterates are bitwise theirs; each shift moves 4 n complex values per
+// outer iteration. A workgroup row (blockIdx.y) takes ZSH_KCHUNK shifts, their
+// coefficients staged in LDS; four shifts' loads are in flight per thread.
+#define ZSH_KCHUNK 16
+template <int L>
+__global__ __launch_bounds__(256) void k_zsh_fused(long long n, int ns, zd *PS, zd *XS, const zd *R,
+                                                   const zd *snap, const ZSh *sh, const Ctl *c) {
+    if (ctl_stopped(c)) return;
+    constexpr int NS = L * (L + 1) / 2;
+    constexpr int CW = 6 * L + L + 2;                 // steps, cx, cpsi, sigma
+    constexpr int U = 4;
+    __shared__ zd cks[ZSH_KCHUNK][CW];
+    const int kb = (int)blockIdx.y * ZSH_KCHUNK, ke = min(ns, kb + ZSH_KCHUNK);
+    for (int e = threadIdx.x; e < (ke - kb) * CW; e += 256) {
+        const ZSh &z = sh[kb + e / CW];
+        const int w = e % CW;
+        cks[e / CW][w] = w < 6 * L ? z.ck[w / 6][w % 6] : w < 7 * L ? z.cx[w - 6 * L] : (w == 7 * L ? z.cpsi : z.sigma);
+    }
+    const zd *om = (const zd *)&c->s[ZS_OMEGA];
+    zd mom[L];
+#pragma unroll
+    for (int i = 0; i < L; i++) mom[i] = zneg(om[i]);
+    __syncthreads();
+    const size_t l2n = (size_t)(L + 1) * n;
+    ZLOOP(e, n) {
+        zd sn[NS + 1], rf[L];
+#pragma unroll
+        for (int q = 0; q <= NS; q++) sn[q] = snap[(size_t)q * n + e];
+#pragma unroll
+        for (int i = 0; i < L; i++) rf[i] = R[(size_t)i * n + e];
+        for (int k0 = kb; k0 < ke; k0 += U) {
+            zd pl[U], xl[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int k = min(k0 + u, ke - 1);
+                pl[u] = PS[(size_t)k * l2n + e];
+                xl[u] = XS[(size_t)(k + 1) * n + e];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (k0 + u >= ke) break;
+                const zd *cf = cks[k0 + u - kb];
+                const zd ms = zneg(cf[7 * L + 1]);
+                zd p[L + 1];
+                p[0] = pl[u];
+                zd xk = xl[u];
+#pragma unroll
+                for (int j = 0; j < L; j++) {
+                    const zd cA = cf[6 * j], cB = cf[6 * j + 1], cC = cf[6 * j + 2], d1 = cf[6 * j + 3],
+                             d2 = cf[6 * j + 4], as = cf[6 * j + 5];
+                    const int s0 = j * (j + 1) / 2;
+#pragma unroll
+                    for (int i = 0; i <= j; i++) p[i] = zadd(zmul(cA, p[i]), zmul(cB, sn[s0 + i]));
+                    p[j + 1] = zmul(cC, sn[s0 + j]);
+                    xk = zadd(xk, zmul(as, p[0]));
+                    const zd rjn = (j + 1 < L) ? sn[(j + 1) * (j + 2) / 2 + j] : sn[NS];
+                    zd v = zadd(p[j + 1], zmul(d1, rjn));
+                    v = zmul(d2, v);
+                    p[j + 1] = zadd(v, zmul(ms, p[j]));
+                }
+                zd xe = zadd(xk, zmul(cf[6 * L], rf[0]));
+                zd p0 = zadd(p[0], zmul(mom[L - 1], p[L]));
+#pragma unroll
+                for (int i = 1; i < L; i++) {
+                    p0 = zadd(p0, zmul(mom[i - 1], p[i]));
+                    xe = zadd(xe, zmul(cf[6 * L + i], rf[i]));
+                }
+                const int k = k0 + u;
+                XS[(size_t)(k + 1) * n + e] = xe;
+                PS[(size_t)k * l2n + e] = zmul(cf[7 * L], p0);
+            }
+        }
+    }
+}
+
 // Seed residual (:561-564): r_0 += (-omega_d_{i-1}) r_i, i = 1..l; partial |r_0|^2.
 __global__ __launch_bounds__(256) void k_zsh_r0(long long n, int l, zd *R, const Ctl *c, double *part) {
     if (ctl_stopped(c)) return;
@@ -1136,11 +1229,15 @@ extern "C" int bkh_zshbicgstabl(const bkh_zmatrix *A, const double *d_b, int l, 
     const long long n = A->n;
     const int G = vec_grid(n, 256), l2 = l + 1;
     const int Gs = std::max(1, (int)std::min<long long>(G, std::max(1LL, 65536LL / std::max(ns, 1))));
+    // the fused per-shift pass unless BKH_ZSH_UNFUSED=1 (the per-step kernels, bitwise the same)
+    const char *uf = getenv("BKH_ZSH_UNFUSED");
+    const bool fused = ns > 0 && !(uf && uf[0] == '1');
+    const size_t nsnap = fused ? (size_t)(l * (l + 1) / 2 + 1) * n : 1;
     void *ws = bkh_workspace(2 * carve_size(sizeof(Ctl)) + carve_size(sizeof(double) * 8 * G) +
                              carve_size(sizeof(zd) * l * l) + carve_size(sizeof(ZSh) * std::max(ns, 1)) +
                              carve_size(sizeof(double) * 2 * std::max(ns, 1)) +
                              (2 * l2 + 2) * carve_size(sizeof(zd) * n) +
-                             carve_size(sizeof(zd) * (size_t)n * l2 * std::max(ns, 1)));
+                             carve_size(sizeof(zd) * (size_t)n * l2 * std::max(ns, 1)) + carve_size(sizeof(zd) * nsnap));
     if (!ws) return BKH_DEVICE_ERROR;
     Carve cv(ws);
     Ctl *c = cv.take<Ctl>(1), *co = cv.take<Ctl>(1);
@@ -1150,6 +1247,7 @@ extern "C" int bkh_zshbicgstabl(const bkh_zmatrix *A, const double *d_b, int l, 
     double *dsig = cv.take<double>(2 * std::max(ns, 1));
     zd *R = cv.take<zd>((size_t)l2 * n), *P = cv.take<zd>((size_t)l2 * n), *rs = cv.take<zd>(n), *tmp = cv.take<zd>(n);
     zd *PS = cv.take<zd>((size_t)n * l2 * std::max(ns, 1));
+    zd *snap = fused ? cv.take<zd>(nsnap) : nullptr;
     const zd *b = (const zd *)d_b;
     zd *x = (zd *)d_x;
     hipStream_t s = t->stream;
@@ -1168,7 +1266,7 @@ extern "C" int bkh_zshbicgstabl(const bkh_zmatrix *A, const double *d_b, int l, 
     const int SB = (ns + 63) / 64;
     hipGraphExec_t ge = capture_cached(t, {0x2B18, (uintptr_t)A, (uintptr_t)b, (uintptr_t)x, (uintptr_t)c,
                                            (uintptr_t)part, (uintptr_t)R, (uintptr_t)l, (uintptr_t)ns,
-                                           (uintptr_t)PS}, s, 1, [=]() {
+                                           (uintptr_t)PS, (uintptr_t)snap}, s, 1, [=]() {
         for (int j = 0; j < l; j++) {
             launch_zdots<1>(s, n, zdots(rs, R + (size_t)j * n), c, part);
             launch_finalize<2>(s, part, G, c, ZshBetaFin{j, l});
@@ -1178,13 +1276,14 @@ extern "C" int bkh_zshbicgstabl(const bkh_zmatrix *A, const double *d_b, int l, 
             launch_finalize<2>(s, part, G, c, ZshAlphaFin());
             hipLaunchKernelGGL(k_zsh_x, dim3(G), dim3(256), 0, s, n, x, (const zd *)P, (const Ctl *)c);
             if (ns > 0) {
-                hipLaunchKernelGGL(k_zsh_sc1, dim3(SB), dim3(64), 0, s, ns, sh, (const Ctl *)c);
-                hipLaunchKernelGGL(k_zsh_v1, dim3(Gs, ns), dim3(256), 0, s, n, j, l2, PS, x, (const zd *)R,
-                                   (const ZSh *)sh, (const Ctl *)c);
+                hipLaunchKernelGGL(k_zsh_sc1, dim3(SB), dim3(64), 0, s, ns, sh, (const Ctl *)c, j);
+                if (!fused)
+                    hipLaunchKernelGGL(k_zsh_v1, dim3(Gs, ns), dim3(256), 0, s, n, j, l2, PS, x, (const zd *)R,
+                                       (const ZSh *)sh, (const Ctl *)c);
             }
-            hipLaunchKernelGGL(k_zsh_r, dim3(G), dim3(256), 0, s, n, j, R, (const zd *)P, (const Ctl *)c);
+            hipLaunchKernelGGL(k_zsh_r, dim3(G), dim3(256), 0, s, n, j, R, (const zd *)P, (const Ctl *)c, snap, l);
             launch_zspmv(s, A, false, R + (size_t)j * n, R + (size_t)(j + 1) * n, c);
-            if (ns > 0)
+            if (ns > 0 && !fused)
                 hipLaunchKernelGGL(k_zsh_v2, dim3(Gs, ns), dim3(256), 0, s, n, j, l2, PS, (const zd *)R,
                                    (const ZSh *)sh, (const Ctl *)c);
             hipLaunchKernelGGL(k_zsh_rot, dim3(std::max(SB, 1)), dim3(64), 0, s, ns, sh, c);
@@ -1204,8 +1303,18 @@ extern "C" int bkh_zshbicgstabl(const bkh_zmatrix *A, const double *d_b, int l, 
         hipLaunchKernelGGL(k_zsh_seed, dim3(G), dim3(256), 0, s, n, l, R, P, x, (const Ctl *)c);
         if (ns > 0) {
             hipLaunchKernelGGL(k_zsh_sc2, dim3(SB), dim3(64), 0, s, ns, l, sh, (const double *)tau, (const Ctl *)c);
-            hipLaunchKernelGGL(k_zsh_v3, dim3(Gs, ns), dim3(256), 0, s, n, l, PS, x, (const zd *)R,
-                               (const ZSh *)sh, (const Ctl *)c);
+            if (fused) {
+                const dim3 gf((unsigned)vec_grid(n, 256, 1), (unsigned)((ns + ZSH_KCHUNK - 1) / ZSH_KCHUNK));
+                switch (l) {
+#define ZCASE(L_) case L_: hipLaunchKernelGGL((k_zsh_fused<L_>), gf, dim3(256), 0, s, n, ns, PS, x, (const zd *)R, \
+                                              (const zd *)snap, (const ZSh *)sh, (const Ctl *)c); break;
+                    ZCASE(1) ZCASE(2) ZCASE(3) ZCASE(4) ZCASE(5) ZCASE(6) ZCASE(7) ZCASE(8)
+#undef ZCASE
+                }
+            } else {
+                hipLaunchKernelGGL(k_zsh_v3, dim3(Gs, ns), dim3(256), 0, s, n, l, PS, x, (const zd *)R,
+                                   (const ZSh *)sh, (const Ctl *)c);
+            }
         }
         hipLaunchKernelGGL(k_zsh_r0, dim3(G), dim3(256), 0, s, n, l, R, (const Ctl *)c, part);
         launch_finalize<1>(s, part, G, c, ZCheckFin());

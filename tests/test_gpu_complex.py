@@ -345,3 +345,23 @@ def test_zqreco_lapack_convention(m, s):
             assert np.abs(R - Ro).max() <= 1e-12 * np.abs(Ro).max()
         else:
             assert abs(R[s - 1, s - 1]) < 1e-12 * np.abs(R).max()
+
+
+@pytest.mark.parametrize("l", [1, 2, 4])
+def test_zshbicgstabl_fused_shift_pass_bitwise(monkeypatch, l):
+    """All per-shift work of an outer iteration in one pass (p_shift[k][1..l] in
+    registers, the seed's r from snapshots) gives bitwise the iterates of the per-step
+    kernels (BKH_ZSH_UNFUSED=1), after 2 outer iterations and at convergence."""
+    n, rp, ci, v = zsystem(24, seed=8)
+    b = zrand(np.random.default_rng(9), n)
+    sig = np.array([0.02 + 0.1j, 0.1, 0.5 - 0.2j, 1.0 + 1.0j, 2.0, 0.3 + 0.7j, 0.7, 0.05j, 1.5 - 0.5j])
+    out = {}
+    for mode in ("fused", "unfused"):
+        if mode == "unfused":
+            monkeypatch.setenv("BKH_ZSH_UNFUSED", "1")
+        out[mode] = [gpu_solve("shifted", n, rp, ci, v, b, l, sig, tol, mi) for tol, mi in ((1e-300, 2), (1e-10, 1000))]
+    for (ia, xa, ta), (ib, xb, tb) in zip(out["fused"], out["unfused"]):
+        assert ia.ret == ib.ret
+        assert np.array_equal(xa, xb)
+        assert np.array_equal(ta, tb)
+    assert out["fused"][1][0].ret >= 0
