@@ -125,6 +125,8 @@ _SIGS = {
     "bkh_batch_offset": (c_ll, [c_vp, c_int]),
     "bkh_batch_layout": (None, [c_vp, c_ip, c_ip, c_ip]),
     "bkh_batch_tune": (c_int, [c_vp, c_int, c_dp]),
+    "bkh_batch_set_spmv_kernel": (c_int, [c_vp, c_int]),
+    "bkh_batch_spmv_kernel": (c_int, [c_vp]),
     "bkh_batch_spmv": (c_int, [c_vp, c_int, c_vp, c_vp]),
     "bkh_batch_last_path": (c_int, []),
     "bkh_debug_fail_alloc": (None, [c_int]),

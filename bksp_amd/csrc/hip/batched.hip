@@ -54,6 +54,31 @@ struct bkh_batch {
     int *trow = nullptr, *tsys = nullptr, *tb = nullptr;
     int ntile = 0;
     int rows_cap = SPMV_WROWS, nnz_cap = SPMV_WCAP, tile_rows = 1024;
+    // column-chunked copy (built when every row's columns ascend): items of <= LC_ROWS
+    // rows of one system, entries chunk-major, R + 1 entry pointers per chunk
+    int lc_ok = 0, use_lc = 0, nitem = 0;
+    struct LcItem *items = nullptr;
+    double *lval = nullptr;
+    float *lval32 = nullptr;
+    unsigned short *lcol = nullptr;
+    int *lrp = nullptr, *ib = nullptr;
+};
+
+// Column-chunked batched SpMV. An item is LC_ROWS (or fewer) consecutive rows of one
+// system; its entries are stored chunk by chunk (chunk c: the rows' entries whose
+// system-local column lies in [c LC_CH, (c+1) LC_CH), row by row in stored order, the
+// column as a 16-bit offset into the chunk), with R + 1 entry pointers per chunk. For
+// each chunk the workgroup stages that part of the system's x in LDS, then takes the
+// chunk's entries in blocks of BLK: a lane per entry forms v x in LDS, a lane per row
+// continues its running sum over its segment of the block. The random-column gathers
+// become LDS reads; columns ascend within a row, so each row still sums its entries in
+// stored order and y is bitwise the CSR-stream kernel's.
+#define LC_CH 8192
+#define LC_NT 1024
+#define LC_RPT 4
+#define LC_ROWS (LC_NT * LC_RPT)
+struct LcItem {
+    int sys, xoff, r0, R, nc, rpb, ns, pad;
 };
 
 // ------------------------------------------------------------------ kernels
@@ -140,6 +165,73 @@ __global__ __launch_bounds__(256) void k_bt_spmv(const int *__restrict__ rp, con
     __shared__ double sh[NV * 4 > 0 ? NV * 4 : 1];
     bt_spmv_unit<VT, T, NV>(xcd_block(blockIdx.x, gridDim.x), rp, ci, val, wblk, usys, sy, x, y, e, part, prod_all,
                             sh);
+}
+
+template <typename VT, typename T, int NV, int BLK>
+__global__ __launch_bounds__(LC_NT) void k_bt_spmv_lc(const LcItem *__restrict__ items, const VT *__restrict__ ev,
+                                                      const unsigned short *__restrict__ ec,
+                                                      const int *__restrict__ erp, const BtSys *sy, const T *x,
+                                                      T *y, Epi<NV, T> e, double *part) {
+    __shared__ T xs[LC_CH];
+    __shared__ T prod[BLK];
+    __shared__ double sh[NV > 0 ? NV * (LC_NT / 64) : 1];
+    const int it = xcd_block(blockIdx.x, gridDim.x);
+    const LcItem d = items[it];
+    if (sy && sy[d.sys].done) return;
+    const int t = threadIdx.x;
+    T acc[LC_RPT];
+#pragma unroll
+    for (int q = 0; q < LC_RPT; q++) acc[q] = (T)0;
+    for (int c = 0; c < d.nc; c++) {
+        const int base = c * LC_CH, len = min(LC_CH, d.ns - base);
+        __syncthreads();                                   // previous chunk's readers are done
+        for (int i = t; i < len; i += LC_NT) xs[i] = x[d.xoff + base + i];
+        const int *rp = erp + d.rpb + c * (d.R + 1);
+        int a[LC_RPT], b[LC_RPT];
+#pragma unroll
+        for (int q = 0; q < LC_RPT; q++) {
+            const int r = t + LC_NT * q;
+            a[q] = r < d.R ? rp[r] : 0;
+            b[q] = r < d.R ? rp[r + 1] : 0;
+        }
+        const int E0 = rp[0], E1 = rp[d.R];
+        __syncthreads();
+        for (int B0 = E0; B0 < E1; B0 += BLK) {
+#pragma unroll
+            for (int k = t; k < BLK; k += LC_NT) {
+                const int q = B0 + k;
+                if (q < E1) prod[k] = mul_rn((T)ev[q], xs[ec[q]]);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < LC_RPT; q++) {
+                const int lo = max(a[q], B0), hi = min(b[q], B0 + BLK);
+                for (int k = lo; k < hi; k++) acc[q] = add_rn(acc[q], prod[k - B0]);
+            }
+            __syncthreads();
+        }
+    }
+    double dv[NV > 0 ? NV : 1];
+#pragma unroll
+    for (int i = 0; i < NV; i++) dv[i] = 0.0;
+#pragma unroll
+    for (int q = 0; q < LC_RPT; q++) {
+        const int r = t + LC_NT * q;
+        if (r < d.R) {
+            const int row = d.r0 + r;
+            y[row] = acc[q];
+#pragma unroll
+            for (int i = 0; i < NV; i++)
+                dv[i] += epi_operand(e, e.a[i], row, (double)acc[q]) * epi_operand(e, e.b[i], row, (double)acc[q]);
+        }
+    }
+    if constexpr (NV > 0) {
+        block_sum<NV, LC_NT>(dv, sh);
+        if (t == 0) {
+#pragma unroll
+            for (int i = 0; i < NV; i++) part[(size_t)it * NV + i] = dv[i];
+        }
+    }
 }
 
 #define TILE_ROWS(i, t_)                                                              \
@@ -594,6 +686,7 @@ __global__ __launch_bounds__(256) void k_bt_ir_update(const int *trow, const int
 
 static void bt_free(bkh_batch *B) {
     hipFree(B->rp); hipFree(B->ci); hipFree(B->val); hipFree(B->val32);
+    hipFree(B->items); hipFree(B->lval); hipFree(B->lval32); hipFree(B->lcol); hipFree(B->lrp); hipFree(B->ib);
     hipFree(B->wblk); hipFree(B->usys); hipFree(B->ub);
     hipFree(B->trow); hipFree(B->tsys); hipFree(B->tb);
 }
@@ -657,6 +750,70 @@ static int bt_layout(bkh_batch *B, int rows_cap, int nnz_cap, int tile_rows) {
     return 0;
 }
 
+// The column-chunked copy (see k_bt_spmv_lc) when every row's columns ascend; the
+// SpMV then runs the chunked kernel by default. Returns 0 or a device error.
+static int bt_build_lc(bkh_batch *B, const std::vector<int> &ci, const std::vector<double> &v, int want_fp32) {
+    for (long long r = 0; r < B->N; r++)
+        for (int k = B->rp_h[r] + 1; k < B->rp_h[r + 1]; k++)
+            if (ci[k] <= ci[k - 1]) return 0;                   // a row not in ascending order
+    std::vector<LcItem> items;
+    std::vector<int> lrp, ib{0};
+    std::vector<double> lv;
+    std::vector<unsigned short> lc;
+    lv.reserve((size_t)B->nnz + 4);
+    lc.reserve((size_t)B->nnz + 4);
+    for (int s = 0; s < B->ns; s++) {
+        const int xo = (int)B->off[s], n = (int)(B->off[s + 1] - B->off[s]);
+        const int nc = (n + LC_CH - 1) / LC_CH;
+        for (int r0 = 0; r0 < n; r0 += LC_ROWS) {
+            LcItem it;
+            it.sys = s; it.xoff = xo; it.r0 = xo + r0; it.R = std::min(LC_ROWS, n - r0); it.nc = nc;
+            it.rpb = (int)lrp.size(); it.ns = n; it.pad = 0;
+            std::vector<int> cur(it.R);
+            for (int r = 0; r < it.R; r++) cur[r] = B->rp_h[it.r0 + r];
+            for (int c = 0; c < nc; c++) {
+                const int hi = (c + 1) * LC_CH;
+                for (int r = 0; r < it.R; r++) {
+                    lrp.push_back((int)lv.size());
+                    const int end = B->rp_h[it.r0 + r + 1];
+                    while (cur[r] < end && ci[cur[r]] - xo < hi) {
+                        lv.push_back(v[cur[r]]);
+                        lc.push_back((unsigned short)(ci[cur[r]] - xo - c * LC_CH));
+                        cur[r]++;
+                    }
+                }
+                lrp.push_back((int)lv.size());
+            }
+            items.push_back(it);
+        }
+        ib.push_back((int)items.size());
+    }
+    const size_t ne = lv.size() + 4;
+    lv.resize(ne, 0.0);
+    lc.resize(ne, 0);
+    bool ok = bkh_dmalloc(&B->items, sizeof(LcItem) * items.size()) == hipSuccess &&
+              bkh_dmalloc(&B->lval, sizeof(double) * ne) == hipSuccess &&
+              bkh_dmalloc(&B->lcol, sizeof(unsigned short) * ne) == hipSuccess &&
+              bkh_dmalloc(&B->lrp, sizeof(int) * lrp.size()) == hipSuccess &&
+              bkh_dmalloc(&B->ib, sizeof(int) * ib.size()) == hipSuccess;
+    if (ok && want_fp32) ok = bkh_dmalloc(&B->lval32, sizeof(float) * ne) == hipSuccess;
+    ok = ok && hipMemcpy(B->items, items.data(), sizeof(LcItem) * items.size(), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(B->lval, lv.data(), sizeof(double) * ne, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(B->lcol, lc.data(), sizeof(unsigned short) * ne, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(B->lrp, lrp.data(), sizeof(int) * lrp.size(), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(B->ib, ib.data(), sizeof(int) * ib.size(), hipMemcpyHostToDevice) == hipSuccess;
+    if (ok && want_fp32) {
+        std::vector<float> lv32(ne);
+        for (size_t q = 0; q < ne; q++) lv32[q] = (float)lv[q];
+        ok = hipMemcpy(B->lval32, lv32.data(), sizeof(float) * ne, hipMemcpyHostToDevice) == hipSuccess;
+    }
+    if (!ok) { bkh_set_error("batch chunked copy upload failed"); return BKH_DEVICE_ERROR; }
+    B->nitem = (int)items.size();
+    B->lc_ok = 1;
+    B->use_lc = 1;
+    return 0;
+}
+
 extern "C" bkh_batch *bkh_batch_create(int ns, const int *n_each, const int *const *row_ptr,
                                        const int *const *col, const double *const *val, int want_fp32) {
     ThreadCtx *t = bkh_tctx();
@@ -713,6 +870,11 @@ extern "C" bkh_batch *bkh_batch_create(int ns, const int *n_each, const int *con
         delete B;
         return nullptr;
     }
+    if (bt_build_lc(B, ci, v, want_fp32)) {
+        bt_free(B);
+        delete B;
+        return nullptr;
+    }
     return B;
 }
 
@@ -728,6 +890,16 @@ extern "C" long long bkh_batch_nnz(const bkh_batch *B) { return B ? B->nnz : 0; 
 extern "C" long long bkh_batch_offset(const bkh_batch *B, int s) {
     return (B && s >= 0 && s <= B->ns) ? B->off[s] : -1;
 }
+// 1 selects the column-chunked SpMV (BKH_BAD_ARGUMENT when the batch has no chunked
+// copy: some row's columns do not ascend), 0 the CSR-stream units.
+extern "C" int bkh_batch_set_spmv_kernel(bkh_batch *B, int chunked) {
+    if (!B || (chunked && !B->lc_ok)) return BKH_BAD_ARGUMENT;
+    B->use_lc = chunked ? 1 : 0;
+    bkh_graph_epoch_bump();
+    return 0;
+}
+extern "C" int bkh_batch_spmv_kernel(const bkh_batch *B) { return B ? B->use_lc : -1; }
+
 extern "C" void bkh_batch_layout(const bkh_batch *B, int *rows_cap, int *nnz_cap, int *tile_rows) {
     if (!B) return;
     if (rows_cap) *rows_cap = B->rows_cap;
@@ -740,13 +912,22 @@ extern "C" void bkh_batch_layout(const bkh_batch *B, int *rows_cap, int *nnz_cap
 template <typename VT, typename T, int NV>
 static void bt_spmv(hipStream_t s, const bkh_batch *B, const VT *val, const BtSys *sy, const T *x, T *y,
                     const Epi<NV, T> &e, double *part) {
+    if (B->use_lc) {
+        // the chunked copy's values in the precision asked for (FP64 val or its FP32 copy)
+        const VT *lv = (sizeof(VT) == sizeof(double)) ? (const VT *)B->lval : (const VT *)B->lval32;
+        constexpr int BLK = sizeof(T) == sizeof(double) ? 2048 : 4096;
+        hipLaunchKernelGGL((k_bt_spmv_lc<VT, T, NV, BLK>), dim3(B->nitem), dim3(LC_NT), 0, s, (const LcItem *)B->items,
+                           lv, (const unsigned short *)B->lcol, (const int *)B->lrp, sy, x, y, e, part);
+        return;
+    }
     hipLaunchKernelGGL((k_bt_spmv<VT, T, NV>), dim3(B->nunit), dim3(256), 0, s, B->rp, B->ci, val, B->wblk,
                        B->usys, sy, x, y, e, part);
 }
 
 template <int OP, int NV>
 static void bt_fin(hipStream_t s, const bkh_batch *B, const double *part, bool units, BtSys *sy, int l) {
-    hipLaunchKernelGGL((k_bt_fin<OP, NV>), dim3(B->ns), dim3(256), 0, s, part, units ? B->ub : B->tb, sy, l);
+    hipLaunchKernelGGL((k_bt_fin<OP, NV>), dim3(B->ns), dim3(256), 0, s, part,
+                       units ? (B->use_lc ? (const int *)B->ib : B->ub) : B->tb, sy, l);
 }
 
 template <typename VT, typename T>
@@ -885,7 +1066,8 @@ static int bt_solve(ThreadCtx *t, const bkh_batch *B, const double *b, double *x
     const long long N = B->N;
     const int ns = B->ns;
     constexpr int NG = (L + 1) * (L + 2) / 2;
-    const size_t nparts = (size_t)std::max((long long)B->nunit * NG, (long long)B->ntile * 2);
+    const size_t nparts = (size_t)std::max(std::max((long long)B->nunit, (long long)B->nitem) * NG,
+                                           (long long)B->ntile * 2);
     const int K = o.graph_iters > 0 ? o.graph_iters : 4;
     const bool ir = o.precision == BKH_FP32_IR;
     if (ir && !B->val32) { bkh_set_error("FP32_IR needs a batch created with want_fp32"); return BKH_BAD_ARGUMENT; }
@@ -1114,12 +1296,20 @@ extern "C" int bkh_batch_tune(bkh_batch *B, int fp32, double *best_ms) {
         hipEventElapsedTime(&ms, t->ev0, t->ev1);
         return ms / reps;
     };
+    B->use_lc = 0;
     for (int q = 0; q < (int)cand.size(); q++) {
         if (bt_layout(B, cand[q][0], cand[q][1], B->tile_rows)) return BKH_DEVICE_ERROR;
         const double ms = time_spmv();
         if (ms < best) { best = ms; bi = q; }
     }
     if (bt_layout(B, cand[bi][0], cand[bi][1], B->tile_rows)) return BKH_DEVICE_ERROR;
+    if (B->lc_ok) {                                       // the column-chunked kernel against the best blocking
+        B->use_lc = 1;
+        const double ms = time_spmv();
+        if (ms < best) best = ms;
+        else B->use_lc = 0;
+    }
+    bkh_graph_epoch_bump();
     if (best_ms) *best_ms = best;
     // vector tiles: time the BiCG update kernel (r -= alpha p, x += alpha p) per size
     double bt = 1e300;

@@ -384,6 +384,16 @@ class Batch:
         load().bkh_batch_layout(self.handle, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
         return a.value, b.value, c.value
 
+    @property
+    def spmv_kernel(self):
+        """1: the column-chunked SpMV, 0: the CSR-stream units."""
+        return load().bkh_batch_spmv_kernel(self.handle)
+
+    @spmv_kernel.setter
+    def spmv_kernel(self, chunked):
+        if load().bkh_batch_set_spmv_kernel(self.handle, int(chunked)):
+            raise ValueError("this batch has no column-chunked copy (some row's columns do not ascend)")
+
     def tune(self, fp32=False):
         ms = ctypes.c_double()
         rc = load().bkh_batch_tune(self.handle, int(fp32), ctypes.byref(ms))

@@ -262,10 +262,16 @@ long long bkh_batch_nnz(const bkh_batch *B);
 long long bkh_batch_offset(const bkh_batch *B, int s);  /* first row of system s */
 /* The SpMV blocking and vector tile size in use (set by bkh_batch_tune). */
 void bkh_batch_layout(const bkh_batch *B, int *rows_cap, int *nnz_cap, int *tile_rows);
+/* SpMV kernel: 1 = column-chunked (each workgroup stages 8192-column chunks of its
+ * system's x in LDS; built at create time when every row's columns ascend, and then
+ * the default), 0 = CSR-stream units. Both give bitwise the same y. set returns
+ * BKH_BAD_ARGUMENT for 1 on a batch without the chunked copy. */
+int bkh_batch_set_spmv_kernel(bkh_batch *B, int chunked);
+int bkh_batch_spmv_kernel(const bkh_batch *B);
 /* Autotuner: times candidate SpMV blockings (16, 32 or 64 rows per wavefront, each
  * with nonzero caps of that many median, 90th-percentile and mean-length rows of this
- * batch, and 512) and vector tile sizes on this batch (FP32 SpMV if fp32) and keeps
- * the fastest. */
+ * batch, and 512) and vector tile sizes on this batch (FP32 SpMV if fp32), and the
+ * column-chunked kernel against the best blocking, and keeps the fastest. */
 int bkh_batch_tune(bkh_batch *B, int fp32, double *best_ms);
 /* y = A x over the batch; x, y device vectors of N doubles (floats if fp32). */
 int bkh_batch_spmv(const bkh_batch *B, int fp32, const void *d_x, void *d_y);

@@ -23,25 +23,82 @@ def true_res(sy, b, x):
     return np.linalg.norm(b - y) / np.linalg.norm(b)
 
 
-def test_batched_spmv_bitwise_and_tuner():
-    systems = small_batch()
-    B = dv.Batch(systems, fp32=True)
-    x = gen.normal(7, B.N)
+def spmv_both_kernels(B, systems, x):
+    """y from the column-chunked and the CSR-stream kernel, each bitwise the oracle's
+    row-ordered product."""
     dx, dy = dv.DeviceArray.from_numpy(x), dv.DeviceArray.from_numpy(np.zeros(B.N))
-    B.spmv(dx, dy)
-    y = dy.numpy()
+    ys = {}
+    for k in (1, 0):
+        B.spmv_kernel = k
+        dy.upload(np.zeros(B.N))
+        B.spmv(dx, dy)
+        ys[k] = dy.numpy()
     for q, (n, rp, ci, v) in enumerate(systems):
         o = B.offsets[q]
         ref = po.csrgemv("n", n, rp, ci, v, x[o:o + n])
-        assert np.array_equal(y[o:o + n], ref), q
+        assert np.array_equal(ys[1][o:o + n], ref), q
+        assert np.array_equal(ys[0][o:o + n], ref), q
+    return ys[1], dx, dy
+
+
+def test_batched_spmv_bitwise_and_tuner():
+    systems = small_batch()
+    B = dv.Batch(systems, fp32=True)
+    assert B.spmv_kernel == 1                      # rows ascend: the chunked copy is the default
+    x = gen.normal(7, B.N)
+    y, dx, dy = spmv_both_kernels(B, systems, x)
     ms = B.tune(fp32=False)
     assert ms > 0
     layout = B.layout()
     dy.upload(np.zeros(B.N))
     B.spmv(dx, dy)
-    assert np.array_equal(dy.numpy(), y), layout   # per-row order does not depend on blocking
+    assert np.array_equal(dy.numpy(), y), (layout, B.spmv_kernel)   # row order does not depend on blocking
     B.tune(fp32=True)
-    print("tuned layout", B.layout())
+    print("tuned layout", B.layout(), "chunked" if B.spmv_kernel else "csr-stream")
+
+
+def test_batched_spmv_chunked_many_chunks_and_items():
+    """Systems of several 8192-column chunks and several 4096-row items, beside tiny
+    ones; FP64 and FP32 products bitwise against the oracle / the CSR-stream kernel."""
+    sizes = [20000, 9000, 8192, 4097, 33, 16385, 100]
+    systems = [gen.random_rows(7070 + q, n) for q, n in enumerate(sizes)]
+    B = dv.Batch(systems, fp32=True)
+    assert B.spmv_kernel == 1
+    x = gen.normal(8, B.N)
+    spmv_both_kernels(B, systems, x)
+    x32 = x.astype(np.float32)
+    dx, dy = dv.DeviceArray.from_numpy(x32), dv.DeviceArray.from_numpy(np.zeros(B.N, np.float32))
+    ys = {}
+    for k in (1, 0):
+        B.spmv_kernel = k
+        dy.upload(np.zeros(B.N, np.float32))
+        B.spmv(dx, dy, fp32=True)
+        ys[k] = dy.numpy()
+    assert np.array_equal(ys[1], ys[0])
+    assert np.all(np.isfinite(ys[1]))
+
+
+def test_batched_spmv_unsorted_row_falls_back():
+    """A row whose columns descend leaves the batch without the chunked copy: the
+    CSR-stream kernel runs, and asking for the chunked one is refused."""
+    systems = small_batch(8080)
+    n, rp, ci, v = systems[1]
+    ci, v = ci.copy(), v.copy()
+    a, e = rp[5], rp[6]
+    assert e - a >= 2
+    ci[a:e], v[a:e] = ci[a:e][::-1].copy(), v[a:e][::-1].copy()
+    systems[1] = (n, rp, ci, v)
+    B = dv.Batch(systems)
+    assert B.spmv_kernel == 0
+    with pytest.raises(ValueError):
+        B.spmv_kernel = 1
+    x = gen.normal(9, B.N)
+    dx, dy = dv.DeviceArray.from_numpy(x), dv.DeviceArray.from_numpy(np.zeros(B.N))
+    B.spmv(dx, dy)
+    y = dy.numpy()
+    for q, (n, rp, ci, v) in enumerate(systems):
+        o = B.offsets[q]
+        assert np.array_equal(y[o:o + n], po.csrgemv("n", n, rp, ci, v, x[o:o + n])), q
 
 
 def test_batched_fp64_matches_oracle_per_system():
@@ -70,11 +127,18 @@ def test_batched_fp64_matches_oracle_per_system():
 @pytest.mark.parametrize("precision", [dv.BKH_FP64, dv.BKH_FP32_IR])
 def test_batched_persistent_equals_per_phase_graphs(precision):
     """The persistent launch runs the per-phase kernels' arithmetic in the same order
-    (same units, tiles and partial rows), so x is bitwise the graph path's."""
+    (same units, tiles and partial rows), so x is bitwise the graph path's with the
+    CSR-stream SpMV; with the chunked SpMV the dot-product partials are grouped by
+    item instead of unit, and x agrees to rounding."""
     systems = small_batch(5150)
     B = dv.Batch(systems, fp32=True)
     b = gen.normal(406, B.N)
     db = dv.DeviceArray.from_numpy(b)
+    dx = dv.DeviceArray.from_numpy(np.zeros(B.N))
+    info, ret_lc, tr_lc = B.bicgstabl(db, 2, dx, 1e-9, 500, precision=precision, path=1)
+    x_lc = dx.numpy()
+    assert (ret_lc >= 0).all() and tr_lc.max() < 1.5e-9
+    B.spmv_kernel = 0
     out = {}
     for path in (1, 2, 0):
         dx = dv.DeviceArray.from_numpy(np.zeros(B.N))
@@ -85,6 +149,7 @@ def test_batched_persistent_equals_per_phase_graphs(precision):
         assert np.array_equal(out[path][0], out[1][0]), path
         assert np.array_equal(out[path][1], out[1][1]), path
     assert (out[2][1] >= 0).all() and out[2][2].max() < 1.5e-9
+    assert np.linalg.norm(x_lc - out[1][0]) / np.linalg.norm(out[1][0]) < 1e-6
 
 
 def test_persistent_barrier_wait_bound_aborts_cleanly(monkeypatch):
@@ -140,12 +205,20 @@ def test_cfg4_full_batch():
           f"max inner={info.inner_iters} max true res={tr.max():.2e} layout={B.layout()}")
     assert (ret >= 0).all()
     assert tr.max() < 1e-8
+    kernel = B.spmv_kernel
+    print(f"cfg4 tuned SpMV: {'chunked' if kernel else 'csr-stream'}")
+    # the persistent launch against the graph path with the same (CSR-stream) SpMV
+    B.spmv_kernel = 0
+    dx.upload(np.zeros(B.N))
+    info0, ret, _ = B.bicgstabl(db, 2, dx, 1e-8, 1000, precision=dv.BKH_FP32_IR)
+    print(f"cfg4 fp32ir csr-stream SpMV {info0.time_ms:.1f} ms")
     x_graph = dx.numpy()
     dx.upload(np.zeros(B.N))
     infop, retp, _ = B.bicgstabl(db, 2, dx, 1e-8, 1000, precision=dv.BKH_FP32_IR, path=2)
     assert dv.Batch.last_path() == 2
     print(f"cfg4 fp32ir persistent launch {infop.time_ms:.1f} ms")
     assert np.array_equal(dx.numpy(), x_graph) and np.array_equal(retp, ret)
+    B.spmv_kernel = kernel
     dx.upload(np.zeros(B.N))
     info64, ret64, tr64 = B.bicgstabl(db, 2, dx, 1e-8, 1000)
     print(f"cfg4 fp64 batched {info64.time_ms:.1f} ms max outer={info64.iters} max true res={tr64.max():.2e}")
