@@ -69,8 +69,8 @@ struct bkh_batch {
 // system-local column lies in [c LC_CH, (c+1) LC_CH), row by row in stored order, the
 // column as a 16-bit offset into the chunk), with R + 1 entry pointers per chunk. For
 // each chunk the workgroup stages that part of the system's x in LDS, then takes the
-// chunk's entries in blocks of BLK: a lane per entry forms v x in LDS, a lane per row
-// continues its running sum over its segment of the block. The random-column gathers
+// chunk's entries in blocks of BLK: each lane forms BLK / LC_NT consecutive products
+// v x into LDS, a lane per row continues its running sum over its segment of the block. The random-column gathers
 // become LDS reads; columns ascend within a row, so each row still sums its entries in
 // stored order and y is bitwise the CSR-stream kernel's.
 #define LC_CH 8192
@@ -167,14 +167,25 @@ __global__ __launch_bounds__(256) void k_bt_spmv(const int *__restrict__ rp, con
                             sh);
 }
 
+template <typename E, int K> struct LcVec { typedef E type __attribute__((ext_vector_type(K))); };
+
+// A lane takes BLK / LC_NT consecutive entries of the block (every chunk's entries start
+// at a multiple of 8) with one vector load of values and one of columns, and stores its
+// products with one LDS store; the chunk of x is loaded with all of a lane's loads in
+// flight before the LDS stores.
 template <typename VT, typename T, int NV, int BLK>
 __global__ __launch_bounds__(LC_NT) void k_bt_spmv_lc(const LcItem *__restrict__ items, const VT *__restrict__ ev,
                                                       const unsigned short *__restrict__ ec,
                                                       const int *__restrict__ erp, const BtSys *sy, const T *x,
                                                       T *y, Epi<NV, T> e, double *part) {
+    constexpr int EPL = BLK / LC_NT, XPT = LC_CH / LC_NT;
+    static_assert(EPL >= 2 && EPL <= 8 && BLK % LC_NT == 0, "entries per lane");
+    typedef typename LcVec<VT, EPL>::type vvec;
+    typedef typename LcVec<unsigned short, EPL>::type cvec;
+    typedef typename LcVec<T, EPL>::type pvec;
     __shared__ T xs[LC_CH];
-    __shared__ T prod[BLK];
-    __shared__ double sh[NV > 0 ? NV * (LC_NT / 64) : 1];
+    __shared__ __attribute__((aligned(32))) T prod[BLK];
+    static_assert(NV * (LC_NT / 64) * sizeof(double) <= BLK * sizeof(T), "partials fit in prod");
     const int it = xcd_block(blockIdx.x, gridDim.x);
     const LcItem d = items[it];
     if (sy && sy[d.sys].done) return;
@@ -184,8 +195,13 @@ __global__ __launch_bounds__(LC_NT) void k_bt_spmv_lc(const LcItem *__restrict__
     for (int q = 0; q < LC_RPT; q++) acc[q] = (T)0;
     for (int c = 0; c < d.nc; c++) {
         const int base = c * LC_CH, len = min(LC_CH, d.ns - base);
-        __syncthreads();                                   // previous chunk's readers are done
-        for (int i = t; i < len; i += LC_NT) xs[i] = x[d.xoff + base + i];
+        const T *xc = x + d.xoff + base;
+        T xr[XPT];
+#pragma unroll
+        for (int j = 0; j < XPT; j++) {
+            const int i = t + LC_NT * j;
+            xr[j] = i < len ? xc[i] : (T)0;
+        }
         const int *rp = erp + d.rpb + c * (d.R + 1);
         int a[LC_RPT], b[LC_RPT];
 #pragma unroll
@@ -195,18 +211,25 @@ __global__ __launch_bounds__(LC_NT) void k_bt_spmv_lc(const LcItem *__restrict__
             b[q] = r < d.R ? rp[r + 1] : 0;
         }
         const int E0 = rp[0], E1 = rp[d.R];
+        __syncthreads();                                   // the previous chunk's readers are done
+#pragma unroll
+        for (int j = 0; j < XPT; j++) xs[t + LC_NT * j] = xr[j];
         __syncthreads();
         for (int B0 = E0; B0 < E1; B0 += BLK) {
+            const int k = B0 + EPL * t;
+            if (k < E1) {
+                const vvec v = *(const vvec *)(ev + k);
+                const cvec cc = *(const cvec *)(ec + k);
+                pvec pr;
 #pragma unroll
-            for (int k = t; k < BLK; k += LC_NT) {
-                const int q = B0 + k;
-                if (q < E1) prod[k] = mul_rn((T)ev[q], xs[ec[q]]);
+                for (int i = 0; i < EPL; i++) pr[i] = mul_rn((T)v[i], xs[cc[i]]);
+                *(pvec *)(prod + EPL * t) = pr;
             }
             __syncthreads();
 #pragma unroll
             for (int q = 0; q < LC_RPT; q++) {
                 const int lo = max(a[q], B0), hi = min(b[q], B0 + BLK);
-                for (int k = lo; k < hi; k++) acc[q] = add_rn(acc[q], prod[k - B0]);
+                for (int j = lo; j < hi; j++) acc[q] = add_rn(acc[q], prod[j - B0]);
             }
             __syncthreads();
         }
@@ -226,7 +249,8 @@ __global__ __launch_bounds__(LC_NT) void k_bt_spmv_lc(const LcItem *__restrict__
         }
     }
     if constexpr (NV > 0) {
-        block_sum<NV, LC_NT>(dv, sh);
+        // prod is free: every lane has passed the barrier after its last read of it
+        block_sum<NV, LC_NT>(dv, (double *)prod);
         if (t == 0) {
 #pragma unroll
             for (int i = 0; i < NV; i++) part[(size_t)it * NV + i] = dv[i];
@@ -783,12 +807,13 @@ static int bt_build_lc(bkh_batch *B, const std::vector<int> &ci, const std::vect
                     }
                 }
                 lrp.push_back((int)lv.size());
+                while (lv.size() % 8) { lv.push_back(0.0); lc.push_back(0); }   // next chunk 8-aligned
             }
             items.push_back(it);
         }
         ib.push_back((int)items.size());
     }
-    const size_t ne = lv.size() + 4;
+    const size_t ne = lv.size() + 8;
     lv.resize(ne, 0.0);
     lc.resize(ne, 0);
     bool ok = bkh_dmalloc(&B->items, sizeof(LcItem) * items.size()) == hipSuccess &&
@@ -915,7 +940,7 @@ static void bt_spmv(hipStream_t s, const bkh_batch *B, const VT *val, const BtSy
     if (B->use_lc) {
         // the chunked copy's values in the precision asked for (FP64 val or its FP32 copy)
         const VT *lv = (sizeof(VT) == sizeof(double)) ? (const VT *)B->lval : (const VT *)B->lval32;
-        constexpr int BLK = sizeof(T) == sizeof(double) ? 2048 : 4096;
+        constexpr int BLK = sizeof(T) == sizeof(double) ? 2048 : 8192;
         hipLaunchKernelGGL((k_bt_spmv_lc<VT, T, NV, BLK>), dim3(B->nitem), dim3(LC_NT), 0, s, (const LcItem *)B->items,
                            lv, (const unsigned short *)B->lcol, (const int *)B->lrp, sy, x, y, e, part);
         return;
