@@ -23,6 +23,7 @@
 // itself and keeps the fastest. With opts path 2 the whole solve is one persistent
 // cooperative launch instead (k_bt_persist), with the same arithmetic.
 #include <math.h>
+#include <climits>
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -61,13 +62,16 @@ struct bkh_batch {
     double *lval = nullptr;
     float *lval32 = nullptr;
     unsigned short *lcol = nullptr;
-    int *lrp = nullptr, *ib = nullptr;
+    unsigned short *lrp = nullptr;
+    int *lcb = nullptr, *ib = nullptr;
 };
 
 // Column-chunked batched SpMV. An item is LC_ROWS (or fewer) consecutive rows of one
 // system; its entries are stored chunk by chunk (chunk c: the rows' entries whose
 // system-local column lies in [c LC_CH, (c+1) LC_CH), row by row in stored order, the
-// column as a 16-bit offset into the chunk), with R + 1 entry pointers per chunk. For
+// column as a 16-bit offset into the chunk), with the chunk's first entry (a multiple
+// of 8) and R + 1 16-bit entry offsets from it per chunk (an item has as many rows, at
+// most LC_ROWS, as keep every chunk's entries below 65536). For
 // each chunk the workgroup stages that part of the system's x in LDS, then takes the
 // chunk's entries in blocks of BLK: each lane forms BLK / LC_NT consecutive products
 // v x into LDS, a lane per row continues its running sum over its segment of the block. The random-column gathers
@@ -78,7 +82,7 @@ struct bkh_batch {
 #define LC_RPT 4
 #define LC_ROWS (LC_NT * LC_RPT)
 struct LcItem {
-    int sys, xoff, r0, R, nc, rpb, ns, pad;
+    int sys, xoff, r0, R, nc, rpb, ns, cbi;   // cbi: index of chunk 0's entry base
 };
 
 // ------------------------------------------------------------------ kernels
@@ -176,7 +180,8 @@ template <typename E, int K> struct LcVec { typedef E type __attribute__((ext_ve
 template <typename VT, typename T, int NV, int BLK>
 __global__ __launch_bounds__(LC_NT) void k_bt_spmv_lc(const LcItem *__restrict__ items, const VT *__restrict__ ev,
                                                       const unsigned short *__restrict__ ec,
-                                                      const int *__restrict__ erp, const BtSys *sy, const T *x,
+                                                      const unsigned short *__restrict__ erp,
+                                                      const int *__restrict__ ecb, const BtSys *sy, const T *x,
                                                       T *y, Epi<NV, T> e, double *part) {
     constexpr int EPL = BLK / LC_NT, XPT = LC_CH / LC_NT;
     static_assert(EPL >= 2 && EPL <= 8 && BLK % LC_NT == 0, "entries per lane");
@@ -202,15 +207,24 @@ __global__ __launch_bounds__(LC_NT) void k_bt_spmv_lc(const LcItem *__restrict__
             const int i = t + LC_NT * j;
             xr[j] = i < len ? xc[i] : (T)0;
         }
-        const int *rp = erp + d.rpb + c * (d.R + 1);
+        const unsigned short *rp = erp + d.rpb + c * (d.R + 1);
+        const int E0 = ecb[d.cbi + c], E1 = E0 + rp[d.R];
         int a[LC_RPT], b[LC_RPT];
+        // loads from clamped rows, all in flight before the selects (R >= 1)
+        unsigned short ra[LC_RPT], rb[LC_RPT];
 #pragma unroll
         for (int q = 0; q < LC_RPT; q++) {
-            const int r = t + LC_NT * q;
-            a[q] = r < d.R ? rp[r] : 0;
-            b[q] = r < d.R ? rp[r + 1] : 0;
+            const int r = min(t + LC_NT * q, d.R - 1);
+            ra[q] = rp[r];
+            rb[q] = rp[r + 1];
         }
-        const int E0 = rp[0], E1 = rp[d.R];
+#pragma unroll
+        for (int q = 0; q < LC_RPT; q++) {
+            const bool in = t + LC_NT * q < d.R;
+            a[q] = in ? E0 + ra[q] : 0;
+            b[q] = in ? E0 + rb[q] : 0;
+            asm volatile("" ::"v"(a[q]), "v"(b[q]));   // keeps these loads ahead of the barrier
+        }
         __syncthreads();                                   // the previous chunk's readers are done
 #pragma unroll
         for (int j = 0; j < XPT; j++) xs[t + LC_NT * j] = xr[j];
@@ -710,7 +724,8 @@ __global__ __launch_bounds__(256) void k_bt_ir_update(const int *trow, const int
 
 static void bt_free(bkh_batch *B) {
     hipFree(B->rp); hipFree(B->ci); hipFree(B->val); hipFree(B->val32);
-    hipFree(B->items); hipFree(B->lval); hipFree(B->lval32); hipFree(B->lcol); hipFree(B->lrp); hipFree(B->ib);
+    hipFree(B->items); hipFree(B->lval); hipFree(B->lval32); hipFree(B->lcol); hipFree(B->lrp); hipFree(B->lcb);
+    hipFree(B->ib);
     hipFree(B->wblk); hipFree(B->usys); hipFree(B->ub);
     hipFree(B->trow); hipFree(B->tsys); hipFree(B->tb);
 }
@@ -781,7 +796,8 @@ static int bt_build_lc(bkh_batch *B, const std::vector<int> &ci, const std::vect
         for (int k = B->rp_h[r] + 1; k < B->rp_h[r + 1]; k++)
             if (ci[k] <= ci[k - 1]) return 0;                   // a row not in ascending order
     std::vector<LcItem> items;
-    std::vector<int> lrp, ib{0};
+    std::vector<int> cbase, ib{0};
+    std::vector<unsigned short> lrp;
     std::vector<double> lv;
     std::vector<unsigned short> lc;
     lv.reserve((size_t)B->nnz + 4);
@@ -789,16 +805,27 @@ static int bt_build_lc(bkh_batch *B, const std::vector<int> &ci, const std::vect
     for (int s = 0; s < B->ns; s++) {
         const int xo = (int)B->off[s], n = (int)(B->off[s + 1] - B->off[s]);
         const int nc = (n + LC_CH - 1) / LC_CH;
-        for (int r0 = 0; r0 < n; r0 += LC_ROWS) {
+        std::vector<int> cnt(nc);
+        for (int r0 = 0; r0 < n;) {
+            // the most rows (<= LC_ROWS) whose entries in every chunk fit 16-bit offsets
+            int R = std::min(LC_ROWS, n - r0);
+            for (;;) {
+                std::fill(cnt.begin(), cnt.end(), 0);
+                for (int k = B->rp_h[xo + r0]; k < B->rp_h[xo + r0 + R]; k++) cnt[(ci[k] - xo) / LC_CH]++;
+                if (*std::max_element(cnt.begin(), cnt.end()) <= 65535 || R == 1) break;
+                R = (R + 1) / 2;
+            }
             LcItem it;
-            it.sys = s; it.xoff = xo; it.r0 = xo + r0; it.R = std::min(LC_ROWS, n - r0); it.nc = nc;
-            it.rpb = (int)lrp.size(); it.ns = n; it.pad = 0;
-            std::vector<int> cur(it.R);
-            for (int r = 0; r < it.R; r++) cur[r] = B->rp_h[it.r0 + r];
+            it.sys = s; it.xoff = xo; it.r0 = xo + r0; it.R = R; it.nc = nc;
+            it.rpb = (int)lrp.size(); it.ns = n; it.cbi = (int)cbase.size();
+            std::vector<int> cur(R);
+            for (int r = 0; r < R; r++) cur[r] = B->rp_h[it.r0 + r];
             for (int c = 0; c < nc; c++) {
                 const int hi = (c + 1) * LC_CH;
-                for (int r = 0; r < it.R; r++) {
-                    lrp.push_back((int)lv.size());
+                const int base = (int)lv.size();                   // a multiple of 8
+                cbase.push_back(base);
+                for (int r = 0; r < R; r++) {
+                    lrp.push_back((unsigned short)(lv.size() - base));
                     const int end = B->rp_h[it.r0 + r + 1];
                     while (cur[r] < end && ci[cur[r]] - xo < hi) {
                         lv.push_back(v[cur[r]]);
@@ -806,26 +833,30 @@ static int bt_build_lc(bkh_batch *B, const std::vector<int> &ci, const std::vect
                         cur[r]++;
                     }
                 }
-                lrp.push_back((int)lv.size());
+                lrp.push_back((unsigned short)(lv.size() - base));
                 while (lv.size() % 8) { lv.push_back(0.0); lc.push_back(0); }   // next chunk 8-aligned
             }
             items.push_back(it);
+            r0 += R;
         }
         ib.push_back((int)items.size());
     }
+    if (lv.size() >= (size_t)INT_MAX - 16) return 0;          // entry indices stay int32
     const size_t ne = lv.size() + 8;
     lv.resize(ne, 0.0);
     lc.resize(ne, 0);
     bool ok = bkh_dmalloc(&B->items, sizeof(LcItem) * items.size()) == hipSuccess &&
               bkh_dmalloc(&B->lval, sizeof(double) * ne) == hipSuccess &&
               bkh_dmalloc(&B->lcol, sizeof(unsigned short) * ne) == hipSuccess &&
-              bkh_dmalloc(&B->lrp, sizeof(int) * lrp.size()) == hipSuccess &&
+              bkh_dmalloc(&B->lrp, sizeof(unsigned short) * lrp.size()) == hipSuccess &&
+              bkh_dmalloc(&B->lcb, sizeof(int) * cbase.size()) == hipSuccess &&
               bkh_dmalloc(&B->ib, sizeof(int) * ib.size()) == hipSuccess;
     if (ok && want_fp32) ok = bkh_dmalloc(&B->lval32, sizeof(float) * ne) == hipSuccess;
     ok = ok && hipMemcpy(B->items, items.data(), sizeof(LcItem) * items.size(), hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(B->lval, lv.data(), sizeof(double) * ne, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(B->lcol, lc.data(), sizeof(unsigned short) * ne, hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemcpy(B->lrp, lrp.data(), sizeof(int) * lrp.size(), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(B->lrp, lrp.data(), sizeof(unsigned short) * lrp.size(), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(B->lcb, cbase.data(), sizeof(int) * cbase.size(), hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(B->ib, ib.data(), sizeof(int) * ib.size(), hipMemcpyHostToDevice) == hipSuccess;
     if (ok && want_fp32) {
         std::vector<float> lv32(ne);
@@ -942,7 +973,8 @@ static void bt_spmv(hipStream_t s, const bkh_batch *B, const VT *val, const BtSy
         const VT *lv = (sizeof(VT) == sizeof(double)) ? (const VT *)B->lval : (const VT *)B->lval32;
         constexpr int BLK = sizeof(T) == sizeof(double) ? 2048 : 8192;
         hipLaunchKernelGGL((k_bt_spmv_lc<VT, T, NV, BLK>), dim3(B->nitem), dim3(LC_NT), 0, s, (const LcItem *)B->items,
-                           lv, (const unsigned short *)B->lcol, (const int *)B->lrp, sy, x, y, e, part);
+                           lv, (const unsigned short *)B->lcol, (const unsigned short *)B->lrp,
+                           (const int *)B->lcb, sy, x, y, e, part);
         return;
     }
     hipLaunchKernelGGL((k_bt_spmv<VT, T, NV>), dim3(B->nunit), dim3(256), 0, s, B->rp, B->ci, val, B->wblk,
