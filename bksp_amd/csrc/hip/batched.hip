@@ -19,9 +19,10 @@
 // With BKH_FP32_IR the iteration runs in FP32 on a correction, inside per-system
 // FP64 iterative refinement (true residual b - A x in FP64 for every system at once).
 // bkh_batch_tune() times candidate SpMV blockings (rows per wavefront, nonzeros per
-// wavefront from the batch's row-length histogram) and vector tile sizes on the batch
-// itself and keeps the fastest. With opts path 2 the whole solve is one persistent
-// cooperative launch instead (k_bt_persist), with the same arithmetic.
+// wavefront from the batch's row-length histogram), the column-chunked SpMV and vector
+// tile sizes on the batch itself and keeps the fastest. With opts path 2 the whole solve
+// is one persistent cooperative launch instead (k_bt_persist_lc with the chunked SpMV,
+// k_bt_persist with the CSR-stream one), with the same arithmetic.
 #include <math.h>
 #include <climits>
 #include <algorithm>
@@ -90,10 +91,49 @@ struct LcItem {
 // Every phase is a device function of the unit / tile / system it serves, so the
 // same code runs as one launch per phase (graph path) and inside the persistent
 // kernel below. `sh` is workgroup scratch for the partial sums (>= 4 NV doubles).
-template <int NV>
-__device__ __forceinline__ void unit_partial(double (&v)[NV], double *part, int u, double *sh) {
-    block_sum<NV, 256>(v, sh);
-    if (threadIdx.x == 0) {
+// threadIdx.x behind an empty asm: expressions of it stay inside the phase that forms
+// them (the persistent kernels inline every phase into one loop; hoisted to its entry,
+// each phase's per-lane addresses would all stay live and spill).
+__device__ __forceinline__ int phase_tid() {
+    int t = (int)threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
+// Sums NV values over each 256-thread group of a workgroup of NGRP such groups (all
+// groups call it together); the totals are valid in each group's thread 0 and are
+// formed in block_sum<NV, 256>'s order (wave sums, then waves 0..3 in turn). `sh`
+// holds NGRP * NV * 4 doubles.
+template <int NV, int NGRP>
+__device__ __forceinline__ void group_sum(double (&v)[NV], double *sh) {
+    if constexpr (NGRP == 1) {
+        block_sum<NV, 256>(v, sh);
+    } else {
+        const int tid = phase_tid(), lane = tid & 63, w = tid >> 6;
+        double *gs = sh + (w >> 2) * NV * 4;
+#pragma unroll
+        for (int i = 0; i < NV; i++) v[i] = wave_sum(v[i]);
+        if (lane == 0) {
+#pragma unroll
+            for (int i = 0; i < NV; i++) gs[i * 4 + (w & 3)] = v[i];
+        }
+        __syncthreads();
+        if ((tid & 255) == 0) {
+#pragma unroll
+            for (int i = 0; i < NV; i++) {
+                double t = gs[i * 4];
+                for (int k = 1; k < 4; k++) t += gs[i * 4 + k];
+                v[i] = t;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int NV, int NGRP = 1>
+__device__ __forceinline__ void unit_partial(double (&v)[NV], double *part, int u, double *sh, bool valid = true) {
+    group_sum<NV, NGRP>(v, sh);
+    if ((phase_tid() & 255) == 0 && valid) {
 #pragma unroll
         for (int i = 0; i < NV; i++) part[(size_t)u * NV + i] = v[i];
     }
@@ -177,24 +217,36 @@ template <typename E, int K> struct LcVec { typedef E type __attribute__((ext_ve
 // at a multiple of 8) with one vector load of values and one of columns, and stores its
 // products with one LDS store; the chunk of x is loaded with all of a lane's loads in
 // flight before the LDS stores.
+template <typename VT>
+struct LcArgs {
+    const LcItem *items;
+    const VT *ev;
+    const unsigned short *ec, *erp;
+    const int *ecb;
+};
+
+// Products per block of the chunked SpMV: the FP64 workgroup's LDS (x chunk + products)
+// stays at 80 KB, so two workgroups share a CU.
+template <typename T> constexpr int lc_blk() { return sizeof(T) == sizeof(double) ? 2048 : 8192; }
+
+// Item `it` of the chunked SpMV by the whole (LC_NT-thread) workgroup; xs (LC_CH) and
+// prod (BLK, 32-byte aligned) are its LDS.
 template <typename VT, typename T, int NV, int BLK>
-__global__ __launch_bounds__(LC_NT) void k_bt_spmv_lc(const LcItem *__restrict__ items, const VT *__restrict__ ev,
-                                                      const unsigned short *__restrict__ ec,
-                                                      const unsigned short *__restrict__ erp,
-                                                      const int *__restrict__ ecb, const BtSys *sy, const T *x,
-                                                      T *y, Epi<NV, T> e, double *part) {
+__device__ __forceinline__ void bt_spmv_item(int it, const LcArgs<VT> &la, const BtSys *sy, const T *x, T *y,
+                                             const Epi<NV, T> &e, double *part, T *xs, T *prod) {
     constexpr int EPL = BLK / LC_NT, XPT = LC_CH / LC_NT;
     static_assert(EPL >= 2 && EPL <= 8 && BLK % LC_NT == 0, "entries per lane");
     typedef typename LcVec<VT, EPL>::type vvec;
     typedef typename LcVec<unsigned short, EPL>::type cvec;
     typedef typename LcVec<T, EPL>::type pvec;
-    __shared__ T xs[LC_CH];
-    __shared__ __attribute__((aligned(32))) T prod[BLK];
     static_assert(NV * (LC_NT / 64) * sizeof(double) <= BLK * sizeof(T), "partials fit in prod");
-    const int it = xcd_block(blockIdx.x, gridDim.x);
-    const LcItem d = items[it];
+    const VT *__restrict__ ev = la.ev;
+    const unsigned short *__restrict__ ec = la.ec;
+    const unsigned short *__restrict__ erp = la.erp;
+    const int *__restrict__ ecb = la.ecb;
+    const LcItem d = la.items[it];
     if (sy && sy[d.sys].done) return;
-    const int t = threadIdx.x;
+    const int t = phase_tid();
     T acc[LC_RPT];
 #pragma unroll
     for (int q = 0; q < LC_RPT; q++) acc[q] = (T)0;
@@ -272,16 +324,26 @@ __global__ __launch_bounds__(LC_NT) void k_bt_spmv_lc(const LcItem *__restrict__
     }
 }
 
+template <typename VT, typename T, int NV, int BLK>
+__global__ __launch_bounds__(LC_NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_bt_spmv_lc(LcArgs<VT> la, const BtSys *sy, const T *x, T *y,
+                                                      Epi<NV, T> e, double *part) {
+    __shared__ T xs[LC_CH];
+    __shared__ __attribute__((aligned(32))) T prod[BLK];
+    bt_spmv_item<VT, T, NV, BLK>(xcd_block(blockIdx.x, gridDim.x), la, sy, x, y, e, part, xs, prod);
+}
+
+// A tile is taken by 256 threads: the whole workgroup of the per-phase kernels, or one
+// of the four 256-thread groups of the persistent chunked kernel's workgroups.
 #define TILE_ROWS(i, t_)                                                              \
     const int sys = tsys[t_];                                                         \
-    for (long long i = (long long)trow[t_] + threadIdx.x; i < trow[t_ + 1]; i += 256)
+    for (long long i = (long long)trow[t_] + (phase_tid() & 255); i < trow[t_ + 1]; i += 256)
 
 // r0 = b - A x (A x in r0 on entry); p0 = rs = r0; partials {r0.r0, b.b}.
-template <typename T>
+template <typename T, int NGRP = 1>
 __device__ __forceinline__ void bt_init_tile(int t, const int *trow, const int *tsys, const T *b, T *R, T *P,
-                                             T *rs, const BtSys *sy, double *part, double *sh) {
+                                             T *rs, const BtSys *sy, double *part, double *sh, bool valid = true) {
     double v[2] = {0.0, 0.0};
-    if (!sy[tsys[t]].done) {
+    if (valid && !sy[tsys[t]].done) {
         TILE_ROWS(i, t) {
             const T ri = b[i] - R[i];
             R[i] = ri; P[i] = ri; rs[i] = ri;
@@ -290,7 +352,7 @@ __device__ __forceinline__ void bt_init_tile(int t, const int *trow, const int *
         }
         (void)sys;
     }
-    unit_partial<2>(v, part, t, sh);
+    unit_partial<2, NGRP>(v, part, t, sh, valid);
 }
 template <typename T>
 __global__ __launch_bounds__(256) void k_bt_init(const int *trow, const int *tsys, long long N, const T *b,
@@ -333,12 +395,13 @@ __global__ __launch_bounds__(256) void k_bt_r(const int *trow, const int *tsys, 
 }
 
 // MR update of x, r_0, p_0 with partials {r0.r0, rs.r0} (:113-125)
-template <typename T>
+template <typename T, int NGRP = 1>
 __device__ __forceinline__ void bt_upd_tile(int t, const int *trow, const int *tsys, long long N, int l, T *P,
-                                            T *R, T *x, const T *rs, const BtSys *sy, double *part, double *sh) {
+                                            T *R, T *x, const T *rs, const BtSys *sy, double *part, double *sh,
+                                            bool valid = true) {
     double v[2] = {0.0, 0.0};
-    const int sy_i = tsys[t];
-    if (!sy[sy_i].done) {
+    const int sy_i = valid ? tsys[t] : 0;
+    if (valid && !sy[sy_i].done) {
         double cx[MR_D], cr[MR_D], cp[MR_D];
         for (int k = 0; k <= l; k++) { cx[k] = sy[sy_i].cx[k]; cr[k] = sy[sy_i].cr[k]; cp[k] = sy[sy_i].cp[k]; }
         TILE_ROWS(i, t) {
@@ -357,7 +420,7 @@ __device__ __forceinline__ void bt_upd_tile(int t, const int *trow, const int *t
         }
         (void)sys;
     }
-    unit_partial<2>(v, part, t, sh);
+    unit_partial<2, NGRP>(v, part, t, sh, valid);
 }
 template <typename T>
 __global__ __launch_bounds__(256) void k_bt_upd(const int *trow, const int *tsys, long long N, int l, T *P, T *R,
@@ -375,18 +438,21 @@ __device__ __forceinline__ void bt_stop(BtSys &s, int status) {
 // System q's finalize: its partial rows [rng[q], rng[q+1]) summed in a fixed order by
 // the workgroup, then its scalar step in thread 0 (the same recurrences as the
 // single-system path).
-template <int OP, int NV>
-__device__ __forceinline__ void bt_fin_sys(int q, const double *part, const int *rng, BtSys *sy, int l, double *sh) {
-    BtSys &S = sy[q];
-    if (OP != BT_RESID && S.done) return;
+template <int OP, int NV, int NGRP = 1>
+__device__ __forceinline__ void bt_fin_sys(int q, const double *part, const int *rng, BtSys *sy, int l, double *sh,
+                                           bool valid = true) {
+    const bool live = valid && (OP == BT_RESID || !sy[q].done);
+    const int lt = phase_tid() & 255;
     double acc[NV];
 #pragma unroll
     for (int i = 0; i < NV; i++) acc[i] = 0.0;
-    for (int r = rng[q] + threadIdx.x; r < rng[q + 1]; r += 256)
+    if (live)
+        for (int r = rng[q] + lt; r < rng[q + 1]; r += 256)
 #pragma unroll
-        for (int i = 0; i < NV; i++) acc[i] += part[(size_t)r * NV + i];
-    block_sum<NV, 256>(acc, sh);
-    if (threadIdx.x != 0) return;
+            for (int i = 0; i < NV; i++) acc[i] += part[(size_t)r * NV + i];
+    group_sum<NV, NGRP>(acc, sh);
+    if (lt != 0 || !live) return;
+    BtSys &S = sy[q];
     if constexpr (OP == BT_RESID) {                         // FP64 true residual: ||r||, ||b||, ratio
         S.err = sqrt(acc[0]);
         S.fnb = sqrt(acc[1]);
@@ -484,6 +550,7 @@ struct BtGrid {
     int abort, iters;
     unsigned xarrive[BKH_NXCD], xgen[BKH_NXCD];
     int bad_map, pad;
+    unsigned qctr[3][BKH_NXCD];   // per-XCD item queues of the chunked persistent kernel
 };
 
 #define BT_SPIN_MAX (1u << 24)   // ~1 s of 128-cycle sleeps
@@ -563,6 +630,23 @@ __device__ __forceinline__ bool bt_grid_sync(BtGrid *g, unsigned nwg, BtSeq &q, 
              k = x_ * per_ + (int)blockIdx.x / BKH_NXCD,                                     \
              end_ = min((count), (x_ + 1) * per_);                                            \
          k < end_; k += (int)gridDim.x / BKH_NXCD)
+
+// Next SpMV item for this workgroup, or -1: the XCD's contiguous share of [0, count),
+// handed out one item at a time from the XCD's queue counter of the current phase
+// (items differ in cost by an order of magnitude; a static deal leaves workgroups idle
+// at the barrier). `slot` broadcasts thread 0's draw.
+__device__ __forceinline__ int bt_grab(BtGrid *g, unsigned gen, int count, int *slot) {
+    const int x = (int)blockIdx.x % BKH_NXCD, per = (count + BKH_NXCD - 1) / BKH_NXCD;
+    const int end = min(count, (x + 1) * per);
+    __syncthreads();                                   // the previous draw has been read
+    if (threadIdx.x == 0) {
+        const unsigned k = __hip_atomic_fetch_add(&g->qctr[gen % 3][x], 1u, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+        *slot = x * per + (int)k < end ? x * per + (int)k : -1;
+    }
+    __syncthreads();
+    return *slot;
+}
 
 template <typename VT, typename T>
 struct BtPArgs {
@@ -654,6 +738,119 @@ __global__ __launch_bounds__(256) void k_bt_persist(BtPArgs<VT, T> a) {
         if (active == 0) { it++; break; }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) a.grid->iters = it;
+#undef BT_SYNC
+}
+
+// The persistent solve with the chunked SpMV: LC_NT-thread workgroups (two per CU), each
+// SpMV item taken by a whole workgroup (bt_spmv_item, as k_bt_spmv_lc), tiles and
+// systems by its four 256-thread groups (four per workgroup step, the same per-tile and
+// per-system arithmetic as the 256-thread per-phase kernels); items are drawn from
+// per-XCD queues (bt_grab), tiles and systems dealt as in k_bt_persist. x is bitwise the graph
+// path's with the chunked SpMV. LDS: the x chunk and the products only; the groups'
+// partial sums use the product buffer, the barrier's flags the x chunk's first words
+// (each is free whenever the other use runs).
+
+template <typename T>
+__device__ __forceinline__ Epi<1, T> epi_dot(const T *w) {
+    Epi<1, T> e;
+    e.w[0] = w;
+    e.a[0] = 0;
+    e.b[0] = EPI_Y;
+    return e;
+}
+// The MR step's Gram entries (r_p, r_q), p <= q <= L, r_L = y.
+template <int L, typename T>
+__device__ __forceinline__ Epi<(L + 1) * (L + 2) / 2, T> epi_gram(T *R, long long N) {
+    Epi<(L + 1) * (L + 2) / 2, T> e;
+    for (int k = 0; k < L; k++) e.w[k] = R + k * N;
+    int idx = 0;
+    for (int p = 0; p <= L; p++)
+        for (int q = p; q <= L; q++) {
+            e.a[idx] = (unsigned char)(p == L ? EPI_Y : p);
+            e.b[idx] = (unsigned char)(q == L ? EPI_Y : q);
+            idx++;
+        }
+    return e;
+}
+
+template <int L, typename VT, typename T>
+__global__ __launch_bounds__(LC_NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_bt_persist_lc(
+    BtPArgs<VT, T> A, LcArgs<VT> la, const int *ib, int nitem) {
+    constexpr int NG = (L + 1) * (L + 2) / 2, BLK = lc_blk<T>(), NGRP = LC_NT / 256;
+    static_assert(NGRP * NG * 4 * sizeof(double) <= BLK * sizeof(T), "group sums fit in prod");
+    __shared__ T xs[LC_CH];
+    __shared__ __attribute__((aligned(32))) T prod[BLK];
+    double *sh = (double *)prod;
+    int *flag = (int *)xs, *active = (int *)xs + 1, *slot = (int *)xs + 2;
+    const unsigned nwg = gridDim.x;
+    BtSeq seq;
+    if (threadIdx.x == 0 && bt_xcc_id() != (int)blockIdx.x % BKH_NXCD) atomicOr(&A.grid->bad_map, 1);
+    bool hier = false;
+    // after barrier g, the queue of phase g + 1 (last drawn from in phase g - 2) is reset
+#define BT_SYNC()                                                                                     \
+    if (!bt_grid_sync(A.grid, nwg, seq, flag, hier, A.spin_max)) return;                              \
+    if (threadIdx.x == 0 && blockIdx.x < BKH_NXCD)                                                    \
+    __hip_atomic_store(&A.grid->qctr[(seq.gen + 1) % 3][blockIdx.x], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+#define BT_ITEMS(it) for (int it = bt_grab(A.grid, seq.gen, nitem, slot); it >= 0; it = bt_grab(A.grid, seq.gen, nitem, slot))
+#define BT_TILES(t) BT_DEAL(k_, (A.ntile + NGRP - 1) / NGRP) for (int t = k_ * NGRP + (phase_tid() >> 8), once_ = 1; once_; once_ = 0)
+#define BT_SYSTEMS(q) BT_DEAL(k_, (A.ns + NGRP - 1) / NGRP) for (int q = k_ * NGRP + (phase_tid() >> 8), once_ = 1; once_; once_ = 0)
+    {
+        Epi<0, T> e0;
+        BT_ITEMS(it) bt_spmv_item<VT, T, 0, BLK>(it, la, A.sy, A.x, A.R, e0, nullptr, xs, prod);
+    }
+    BT_SYNC();
+    hier = __hip_atomic_load(&A.grid->bad_map, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+    BT_TILES(t) bt_init_tile<T, NGRP>(t, A.trow, A.tsys, A.b, A.R, A.P, A.rs, A.sy, A.part, sh, t < A.ntile);
+    BT_SYNC();
+    BT_SYSTEMS(q) bt_fin_sys<BT_INIT, 2, NGRP>(q, A.part, A.tb, A.sy, L, sh, q < A.ns);
+    BT_SYNC();
+    int it = 0;
+    for (; it < A.max_it; it++) {
+        for (int j = 0; j < L; j++) {
+            BT_TILES(t) if (t < A.ntile) bt_p_tile<T>(t, A.trow, A.tsys, A.N, j, A.P, A.R, A.sy);
+            BT_SYNC();
+            BT_ITEMS(u) bt_spmv_item<VT, T, 1, BLK>(u, la, A.sy, A.P + j * A.N, A.P + (j + 1) * A.N,
+                                                              epi_dot<T>(A.rs), A.part, xs, prod);
+            BT_SYNC();
+            BT_SYSTEMS(q) bt_fin_sys<BT_ALPHA, 1, NGRP>(q, A.part, ib, A.sy, L, sh, q < A.ns);
+            BT_SYNC();
+            BT_TILES(t) if (t < A.ntile) bt_r_tile<T>(t, A.trow, A.tsys, A.N, j, A.P, A.R, A.x, A.sy);
+            BT_SYNC();
+            if (j < L - 1) {
+                BT_ITEMS(u) bt_spmv_item<VT, T, 1, BLK>(u, la, A.sy, A.R + j * A.N,
+                                                                  A.R + (j + 1) * A.N, epi_dot<T>(A.rs), A.part,
+                                                                  xs, prod);
+                BT_SYNC();
+                BT_SYSTEMS(q) bt_fin_sys<BT_GAMMA, 1, NGRP>(q, A.part, ib, A.sy, L, sh, q < A.ns);
+            } else {
+                BT_ITEMS(u) bt_spmv_item<VT, T, NG, BLK>(u, la, A.sy, A.R + (L - 1) * A.N,
+                                                                   A.R + L * A.N, epi_gram<L, T>(A.R, A.N),
+                                                                   A.part, xs, prod);
+                BT_SYNC();
+                BT_SYSTEMS(q) bt_fin_sys<BT_MR, NG, NGRP>(q, A.part, ib, A.sy, L, sh, q < A.ns);
+            }
+            BT_SYNC();
+        }
+        BT_TILES(t) bt_upd_tile<T, NGRP>(t, A.trow, A.tsys, A.N, L, A.P, A.R, A.x, A.rs, A.sy, A.part, sh,
+                                         t < A.ntile);
+        BT_SYNC();
+        BT_SYSTEMS(q) bt_fin_sys<BT_UPD, 2, NGRP>(q, A.part, A.tb, A.sy, L, sh, q < A.ns);
+        BT_SYNC();
+        // every workgroup reads the same settled flags, so all leave together
+        if (threadIdx.x == 0) *active = 0;
+        __syncthreads();
+        int mine = 0;
+        for (int q = threadIdx.x; q < A.ns; q += LC_NT) mine += A.sy[q].done ? 0 : 1;
+        if (mine) atomicAdd(active, mine);
+        __syncthreads();
+        const bool stop = *active == 0;
+        __syncthreads();                           // *active read before the next phase's LDS stores
+        if (stop) { it++; break; }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) A.grid->iters = it;
+#undef BT_ITEMS
+#undef BT_TILES
+#undef BT_SYSTEMS
 #undef BT_SYNC
 }
 
@@ -965,16 +1162,23 @@ extern "C" void bkh_batch_layout(const bkh_batch *B, int *rows_cap, int *nnz_cap
 
 // ----------------------------------------------------------------- launches
 
+// The chunked copy's arrays, values in the precision asked for (FP64 or the FP32 copy).
+template <typename VT>
+static LcArgs<VT> lc_args(const bkh_batch *B) {
+    LcArgs<VT> la;
+    la.items = B->items;
+    la.ev = (sizeof(VT) == sizeof(double)) ? (const VT *)B->lval : (const VT *)B->lval32;
+    la.ec = B->lcol;
+    la.erp = B->lrp;
+    la.ecb = B->lcb;
+    return la;
+}
 template <typename VT, typename T, int NV>
 static void bt_spmv(hipStream_t s, const bkh_batch *B, const VT *val, const BtSys *sy, const T *x, T *y,
                     const Epi<NV, T> &e, double *part) {
     if (B->use_lc) {
-        // the chunked copy's values in the precision asked for (FP64 val or its FP32 copy)
-        const VT *lv = (sizeof(VT) == sizeof(double)) ? (const VT *)B->lval : (const VT *)B->lval32;
-        constexpr int BLK = sizeof(T) == sizeof(double) ? 2048 : 8192;
-        hipLaunchKernelGGL((k_bt_spmv_lc<VT, T, NV, BLK>), dim3(B->nitem), dim3(LC_NT), 0, s, (const LcItem *)B->items,
-                           lv, (const unsigned short *)B->lcol, (const unsigned short *)B->lrp,
-                           (const int *)B->lcb, sy, x, y, e, part);
+        hipLaunchKernelGGL((k_bt_spmv_lc<VT, T, NV, lc_blk<T>()>), dim3(B->nitem), dim3(LC_NT), 0, s,
+                           lc_args<VT>(B), sy, x, y, e, part);
         return;
     }
     hipLaunchKernelGGL((k_bt_spmv<VT, T, NV>), dim3(B->nunit), dim3(256), 0, s, B->rp, B->ci, val, B->wblk,
@@ -1025,6 +1229,7 @@ template <int L, typename VT, typename T>
 static int bt_core_persist(ThreadCtx *t, const bkh_batch *B, const VT *val, const T *b, T *x, T *R, T *P, T *rs,
                            BtSys *sy, double *part, BtGrid *grid, int max_it) {
     hipStream_t s = t->stream;
+    const bool lc = B->use_lc != 0;
     static int G_res = [] {
         int dev = 0, ncu = 0, per = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
@@ -1033,8 +1238,17 @@ static int bt_core_persist(ThreadCtx *t, const bkh_batch *B, const VT *val, cons
             return 0;
         return ncu * per;
     }();
-    const int work = std::max(std::max(B->nunit, B->ntile), B->ns);
-    int G = std::min(G_res, (work + BKH_NXCD - 1) / BKH_NXCD * BKH_NXCD);
+    static int G_lc = [] {
+        int dev = 0, ncu = 0, per = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_bt_persist_lc<L, VT, T>, LC_NT, 0) != hipSuccess)
+            return 0;
+        return ncu * per;
+    }();
+    const int work = lc ? std::max(std::max(B->nitem, (B->ntile + 3) / 4), (B->ns + 3) / 4)
+                        : std::max(std::max(B->nunit, B->ntile), B->ns);
+    int G = std::min(lc ? G_lc : G_res, (work + BKH_NXCD - 1) / BKH_NXCD * BKH_NXCD);
     G -= G % BKH_NXCD;
     if (G < BKH_NXCD) { bkh_set_error("persistent batched solve: no resident grid (occupancy %d)", G_res); return BKH_DEVICE_ERROR; }
     HIP_OK(hipMemsetAsync(grid, 0, sizeof(BtGrid), s));
@@ -1048,9 +1262,16 @@ static int bt_core_persist(ThreadCtx *t, const bkh_batch *B, const VT *val, cons
     // so the abort path can be exercised from the host; every wave still exits
     const char *sp = getenv("BKH_BT_SPIN");
     a.spin_max = (sp && *sp) ? (unsigned)strtoul(sp, nullptr, 10) : BT_SPIN_MAX;
+    LcArgs<VT> la = lc_args<VT>(B);
+    const int *ib = B->ib;
+    int nitem = B->nitem;
     void *args[] = {&a};
-    const hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_bt_persist<L, VT, T>), dim3(G),
-                                                    dim3(256), args, 0, s);
+    void *args_lc[] = {&a, &la, &ib, &nitem};
+    const hipError_t e =
+        lc ? hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_bt_persist_lc<L, VT, T>), dim3(G),
+                                        dim3(LC_NT), args_lc, 0, s)
+           : hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_bt_persist<L, VT, T>), dim3(G), dim3(256),
+                                        args, 0, s);
     if (e != hipSuccess) { bkh_set_error("cooperative launch of the batched solve: %s", hipGetErrorString(e)); return BKH_DEVICE_ERROR; }
     BtGrid hg;
     HIP_OK(hipMemcpyAsync(&hg, grid, sizeof hg, hipMemcpyDeviceToHost, s));
@@ -1130,10 +1351,10 @@ static int bt_solve(ThreadCtx *t, const bkh_batch *B, const double *b, double *x
     if (ir && !B->val32) { bkh_set_error("FP32_IR needs a batch created with want_fp32"); return BKH_BAD_ARGUMENT; }
     const size_t vbytes = ir ? sizeof(float) : sizeof(double);
     // path 0 / 1: one launch per phase in hipGraphs; 2: one persistent launch per solve.
-    // cfg4's 512 systems (14.8M rows, 251M nonzeros), BiCGSTAB(2) to 1e-8: graphs
-    // 173.9 ms FP64 and 146.7 ms FP32-IR, persistent 225.3 and 211.1 ms with the
-    // two-level barrier (467.0 and 307.0 ms with agent-scope fences in every
-    // workgroup); x is bitwise the same on both paths.
+    // 512 uneven systems (14.8M rows, 251M nonzeros), BiCGSTAB(2) to 1e-8, chunked SpMV:
+    // graphs 96.5 ms FP64 and 67.7 ms FP32-IR, persistent 118.0 and 78.4 ms (CSR-stream
+    // SpMV: graphs 173.9 / 150.4, persistent 225.3 / 214.9); x is bitwise the same on
+    // both paths with the same SpMV.
     const bool persist = o.path == 2;
     size_t bytes = carve_size(sizeof(Ctl)) + carve_size(sizeof(BtGrid)) + carve_size(sizeof(BtSys) * ns) + carve_size(sizeof(double) * nparts) +
                    carve_size(sizeof(double) * ns) + carve_size(sizeof(int) * ns) + carve_size(sizeof(double) * ns) +

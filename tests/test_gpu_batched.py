@@ -125,20 +125,24 @@ def test_batched_fp64_matches_oracle_per_system():
 
 
 @pytest.mark.parametrize("precision", [dv.BKH_FP64, dv.BKH_FP32_IR])
-def test_batched_persistent_equals_per_phase_graphs(precision):
+@pytest.mark.parametrize("chunked", [1, 0])
+def test_batched_persistent_equals_per_phase_graphs(precision, chunked):
     """The persistent launch runs the per-phase kernels' arithmetic in the same order
-    (same units, tiles and partial rows), so x is bitwise the graph path's with the
-    CSR-stream SpMV; with the chunked SpMV the dot-product partials are grouped by
-    item instead of unit, and x agrees to rounding."""
+    (the same SpMV items or units, tiles and partial rows; with the chunked SpMV its
+    workgroups take four 256-thread tiles or systems at a time), so x is bitwise the
+    graph path's with the same SpMV kernel. The two SpMV kernels give the same products
+    but group the dot-product partials by item or by unit, so across kernels x agrees
+    to rounding."""
     systems = small_batch(5150)
     B = dv.Batch(systems, fp32=True)
     b = gen.normal(406, B.N)
     db = dv.DeviceArray.from_numpy(b)
+    B.spmv_kernel = 1 - chunked
     dx = dv.DeviceArray.from_numpy(np.zeros(B.N))
-    info, ret_lc, tr_lc = B.bicgstabl(db, 2, dx, 1e-9, 500, precision=precision, path=1)
-    x_lc = dx.numpy()
-    assert (ret_lc >= 0).all() and tr_lc.max() < 1.5e-9
-    B.spmv_kernel = 0
+    info, ret_o, tr_o = B.bicgstabl(db, 2, dx, 1e-9, 500, precision=precision, path=1)
+    x_other = dx.numpy()
+    assert (ret_o >= 0).all() and tr_o.max() < 1.5e-9
+    B.spmv_kernel = chunked
     out = {}
     for path in (1, 2, 0):
         dx = dv.DeviceArray.from_numpy(np.zeros(B.N))
@@ -149,7 +153,7 @@ def test_batched_persistent_equals_per_phase_graphs(precision):
         assert np.array_equal(out[path][0], out[1][0]), path
         assert np.array_equal(out[path][1], out[1][1]), path
     assert (out[2][1] >= 0).all() and out[2][2].max() < 1.5e-9
-    assert np.linalg.norm(x_lc - out[1][0]) / np.linalg.norm(out[1][0]) < 1e-6
+    assert np.linalg.norm(x_other - out[1][0]) / np.linalg.norm(out[1][0]) < 1e-6
 
 
 def test_persistent_barrier_wait_bound_aborts_cleanly(monkeypatch):
@@ -207,7 +211,13 @@ def test_cfg4_full_batch():
     assert tr.max() < 1e-8
     kernel = B.spmv_kernel
     print(f"cfg4 tuned SpMV: {'chunked' if kernel else 'csr-stream'}")
-    # the persistent launch against the graph path with the same (CSR-stream) SpMV
+    x_tuned = dx.numpy()
+    dx.upload(np.zeros(B.N))
+    infop, retp, _ = B.bicgstabl(db, 2, dx, 1e-8, 1000, precision=dv.BKH_FP32_IR, path=2)
+    assert dv.Batch.last_path() == 2
+    print(f"cfg4 fp32ir persistent launch, tuned SpMV: {infop.time_ms:.1f} ms")
+    assert np.array_equal(dx.numpy(), x_tuned) and np.array_equal(retp, ret)
+    # the persistent launch against the graph path with the CSR-stream SpMV
     B.spmv_kernel = 0
     dx.upload(np.zeros(B.N))
     info0, ret, _ = B.bicgstabl(db, 2, dx, 1e-8, 1000, precision=dv.BKH_FP32_IR)
@@ -216,10 +226,15 @@ def test_cfg4_full_batch():
     dx.upload(np.zeros(B.N))
     infop, retp, _ = B.bicgstabl(db, 2, dx, 1e-8, 1000, precision=dv.BKH_FP32_IR, path=2)
     assert dv.Batch.last_path() == 2
-    print(f"cfg4 fp32ir persistent launch {infop.time_ms:.1f} ms")
+    print(f"cfg4 fp32ir persistent launch, csr-stream SpMV: {infop.time_ms:.1f} ms")
     assert np.array_equal(dx.numpy(), x_graph) and np.array_equal(retp, ret)
     B.spmv_kernel = kernel
     dx.upload(np.zeros(B.N))
     info64, ret64, tr64 = B.bicgstabl(db, 2, dx, 1e-8, 1000)
     print(f"cfg4 fp64 batched {info64.time_ms:.1f} ms max outer={info64.iters} max true res={tr64.max():.2e}")
     assert (ret64 >= 0).all() and tr64.max() < 1.5e-8
+    x64 = dx.numpy()
+    dx.upload(np.zeros(B.N))
+    info64p, ret64p, _ = B.bicgstabl(db, 2, dx, 1e-8, 1000, path=2)
+    print(f"cfg4 fp64 persistent launch {info64p.time_ms:.1f} ms")
+    assert np.array_equal(dx.numpy(), x64) and np.array_equal(ret64p, ret64)
