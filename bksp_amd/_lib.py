@@ -48,6 +48,12 @@ class BkhOpts(ctypes.Structure):
                 ("max_inner", c_int), ("inner_tol", c_dbl), ("path", c_int)]
 
 
+class BkhMatrixTuning(ctypes.Structure):
+    _fields_ = [("spmv_variant", c_int), ("rows_per_wave", c_int), ("nnz_per_wave", c_int),
+                ("candidates", c_int), ("spmv_ms", c_dbl), ("spmv_ms_default", c_dbl),
+                ("spmm_width", c_int * 17), ("spmm_ms", c_dbl * 17), ("spmm_ms_full", c_dbl * 17)]
+
+
 class BkhInfo(ctypes.Structure):
     _fields_ = [("ret", c_int), ("status", c_int), ("iters", c_int), ("inner_iters", c_int),
                 ("passes", c_int), ("rec_res", c_dbl), ("true_res", c_dbl), ("time_ms", c_dbl)]
@@ -77,6 +83,7 @@ _SIGS = {
     "bkh_matrix_rows": (c_int, [c_vp]),
     "bkh_matrix_blocks": (c_int, [c_vp]),
     "bkh_matrix_set_spmv": (c_int, [c_vp, c_int]),
+    "bkh_matrix_tune": (c_int, [c_vp, c_int, c_vp]),
     "bkh_spmv": (c_int, [c_vp, ctypes.c_char, c_dbl, c_vp, c_dbl, c_vp]),
     "bkh_spmm": (c_int, [c_vp, c_int, c_dbl, c_vp, c_dbl, c_vp]),
     "bkh_spmm_gram": (c_int, [c_vp, c_int, c_vp, c_vp, c_dp, ctypes.POINTER(c_int)]),

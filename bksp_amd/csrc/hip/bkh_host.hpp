@@ -125,6 +125,8 @@ struct bkh_matrix {
     int dia_maxpos = 0;                // most stored positions of any slice
     int dia_maxoff = 0;                // largest |column offset| of the copy
     int variant = SPMV_WAVE;           // kernel used by every SpMV on this matrix
+    int wrows = SPMV_WROWS, wcap = SPMV_WCAP;   // the wave variant's rows / nonzeros per wavefront
+    int spmm_w[17] = {};               // bkh_spmm: columns per pass for s = 1..16 (0: s)
     int max_row = 0;                   // longest row (nonzeros)
     bkh_matrix *T = nullptr;           // A^T in the same format (BiCG)
     int stored_triangle = 0;
@@ -193,6 +195,10 @@ hipGraphExec_t graph_cache_find(ThreadCtx *t, const GraphKey &key);
 void graph_cache_put(ThreadCtx *t, const GraphKey &key, hipGraphExec_t ge);
 uintptr_t bkh_graph_epoch();
 void bkh_graph_epoch_bump();
+// bkh_spmm's pass over s column-major columns in chunks of w (blcg.hip); the autotuner
+// times it per width
+int spmm_chunked(ThreadCtx *t, const bkh_matrix *A, int s, int w, double alpha, const double *X, double beta,
+                 double *Y);
 
 // Eager mode (environment variable BKH_EAGER=1): an iteration body is kept as a
 // function and run_graph_until_done issues its launches directly on the stream

@@ -1479,38 +1479,50 @@ extern "C" int bkh_qr(int m, int s, double *V, double *R) {
 // Y = alpha A X + beta Y for s column-major columns (the mkl_dcsrmm dispatch of
 // include/bksp_internal.h:63-79): s <= 16 through the row-major SpMM kernel, wider
 // blocks column by column through the SpMV.
+static_assert(SM_MAX == 16, "bkh_matrix::spmm_w holds widths for s = 1..16");
+
+// Y = alpha A X + beta Y over s column-major columns, w columns per pass: each chunk is
+// turned row-major, multiplied by the S = w SpMM kernel (one sweep over A per chunk) and
+// turned back. A column's products do not depend on w (every row sums in stored order).
+int spmm_chunked(ThreadCtx *t, const bkh_matrix *A, int s, int w, double alpha, const double *X, double beta,
+                 double *Y) {
+    hipStream_t st = t->stream;
+    const long long m = A->n;
+    w = std::max(1, std::min(w, std::min(s, SM_MAX)));
+    void *ws = bkh_workspace(2 * carve_size(sizeof(double) * m * w));
+    if (!ws) return BKH_DEVICE_ERROR;
+    Carve cv(ws);
+    double *Xt = cv.take<double>(m * w), *Yt = cv.take<double>(m * w);
+    for (int j = 0; j < s; j += w) {
+        const int wj = std::min(w, s - j);
+        const double *Xj = X + (size_t)j * m;
+        double *Yj = Y + (size_t)j * m;
+        const int Gt = vec_grid(m * wj, 256);
+        hipLaunchKernelGGL((k_cm2rm<double, double>), dim3(Gt), dim3(256), 0, st, m, wj, Xj, Xt);
+        if (beta != 0.0)
+            hipLaunchKernelGGL((k_cm2rm<double, double>), dim3(Gt), dim3(256), 0, st, m, wj, (const double *)Yj, Yt);
+        switch (wj) {
+#define CASE(S_) case S_: launch_spmm<S_>(st, A, A->val, (const double *)Xt, Yt, alpha, beta, (const Ctl *)nullptr); break;
+            CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+            CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
+#undef CASE
+        }
+        hipLaunchKernelGGL((k_rm2cm<double, double>), dim3(Gt), dim3(256), 0, st, m, wj, (const double *)Yt, Yj);
+    }
+    LAUNCH_OK();
+    return 0;
+}
+
 extern "C" int bkh_spmm(const bkh_matrix *A, int s, double alpha, const double *X, double beta, double *Y) {
     ThreadCtx *t = bkh_tctx();
     if (!t) return BKH_DEVICE_ERROR;
     if (!A || !X || !Y || s < 1) return BKH_BAD_ARGUMENT;
-    hipStream_t st = t->stream;
-    const long long m = A->n;
-    if (s > SM_MAX) {
-        Epi<0> e;
-        memset(&e, 0, sizeof e);
-        for (int j = 0; j < s; j++)
-            launch_spmv<double, double, 0>(st, A, A->val, X + (size_t)j * m, Y + (size_t)j * m, alpha, beta, e,
-                                           nullptr, nullptr, NoFin(), false);
-        LAUNCH_OK();
-        HIP_OK(hipStreamSynchronize(st));
-        return 0;
-    }
-    void *ws = bkh_workspace(2 * carve_size(sizeof(double) * m * s));
-    if (!ws) return BKH_DEVICE_ERROR;
-    Carve cv(ws);
-    double *Xt = cv.take<double>(m * s), *Yt = cv.take<double>(m * s);
-    const int Gt = vec_grid(m * s, 256);
-    hipLaunchKernelGGL((k_cm2rm<double, double>), dim3(Gt), dim3(256), 0, st, m, s, X, Xt);
-    if (beta != 0.0) hipLaunchKernelGGL((k_cm2rm<double, double>), dim3(Gt), dim3(256), 0, st, m, s, (const double *)Y, Yt);
-    switch (s) {
-#define CASE(S_) case S_: launch_spmm<S_>(st, A, A->val, (const double *)Xt, Yt, alpha, beta, (const Ctl *)nullptr); break;
-        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
-        CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
-#undef CASE
-    }
-    hipLaunchKernelGGL((k_rm2cm<double, double>), dim3(Gt), dim3(256), 0, st, m, s, (const double *)Yt, Y);
-    LAUNCH_OK();
-    HIP_OK(hipStreamSynchronize(st));
+    // columns per pass: the autotuner's choice for this s (for s > 16, for 16), else all
+    // of them up to 16
+    const int tuned = A->spmm_w[std::min(s, SM_MAX)];
+    const int rc = spmm_chunked(t, A, s, tuned > 0 ? tuned : SM_MAX, alpha, X, beta, Y);
+    if (rc) return rc;
+    HIP_OK(hipStreamSynchronize(t->stream));
     return 0;
 }
 

@@ -1,5 +1,7 @@
 // core.hip -- runtime, memory, matrix upload and single-kernel entry points.
+#include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <mutex>
 #include <new>
 #include <stdlib.h>
@@ -666,6 +668,157 @@ extern "C" int bkh_matrix_set_spmv(bkh_matrix *A, int variant) {
     int rc = set_variant(A, variant);
     if (rc) { bkh_set_error("SpMV variant %d is not available for this matrix", variant); return rc; }
     if (A->T) set_variant(A->T, variant);
+    return 0;
+}
+
+// ------------------------------------------------------------- autotuner
+
+// The wave variant's blocks for `rows` rows / `cap` nonzeros per wavefront, with the
+// 16-bit column deltas (relative to a block's first row) rebuilt for them when the
+// matrix keeps them (dropped if a delta no longer fits). 0 or a device error.
+static int set_wave_blocking(bkh_matrix *A, const std::vector<int> &rp, const std::vector<int> &ci, int rows,
+                             int cap) {
+    if (rows == A->wrows && cap == A->wcap) return 0;
+    std::vector<int> wb;
+    build_blocks(A->n, rp, rows, cap, wb);
+    int *d = nullptr;
+    if (bkh_dmalloc(&d, sizeof(int) * wb.size()) != hipSuccess ||
+        hipMemcpy(d, wb.data(), sizeof(int) * wb.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        hipFree(d);
+        bkh_set_error("autotuner: wave blocks upload failed");
+        return BKH_DEVICE_ERROR;
+    }
+    bkh_graph_epoch_bump();                       // cached graphs hold the old blocks
+    hipFree(A->wblk);
+    A->wblk = d;
+    A->nwblk = (int)wb.size() - 1;
+    A->wrows = rows;
+    A->wcap = cap;
+    if (A->cd16) {
+        std::vector<short> cd((size_t)A->nnz);
+        bool fits = true;
+        for (size_t b = 0; b + 1 < wb.size() && fits; b++)
+            for (int r = wb[b]; r < wb[b + 1] && fits; r++)
+                for (int k = rp[r]; k < rp[r + 1]; k++) {
+                    const int dd = ci[k] - wb[b];
+                    if (dd < -32768 || dd > 32767) { fits = false; break; }
+                    cd[k] = (short)dd;
+                }
+        if (!fits) {
+            hipFree(A->cd16);
+            A->cd16 = nullptr;
+        } else if (hipMemcpy(A->cd16, cd.data(), sizeof(short) * A->nnz, hipMemcpyHostToDevice) != hipSuccess) {
+            bkh_set_error("autotuner: column deltas upload failed");
+            return BKH_DEVICE_ERROR;
+        }
+    }
+    return 0;
+}
+
+extern "C" int bkh_matrix_tune(bkh_matrix *A, int max_s, bkh_matrix_tuning *out) {
+    ThreadCtx *t = bkh_tctx();
+    if (!t) return BKH_DEVICE_ERROR;
+    if (!A || !out || max_s < 0 || max_s > 16) return BKH_BAD_ARGUMENT;
+    memset(out, 0, sizeof *out);
+    hipStream_t s = t->stream;
+    HIP_OK(hipStreamSynchronize(s));
+    const int n = A->n;
+    std::vector<int> rp((size_t)n + 1), ci;
+    HIP_OK(hipMemcpy(rp.data(), A->rp, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+    if (A->cd16) {
+        ci.resize((size_t)A->nnz);
+        HIP_OK(hipMemcpy(ci.data(), A->ci, sizeof(int) * A->nnz, hipMemcpyDeviceToHost));
+    }
+    const int ms = std::max(1, max_s);
+    void *ws = bkh_workspace(4 * carve_size(sizeof(double) * (size_t)n * ms));
+    if (!ws) return BKH_DEVICE_ERROR;
+    Carve cv(ws);
+    double *x = cv.take<double>((size_t)n * ms), *y = cv.take<double>((size_t)n * ms);
+    HIP_OK(hipMemsetAsync(x, 0, sizeof(double) * n * ms, s));
+    HIP_OK(hipMemsetAsync(y, 0, sizeof(double) * n * ms, s));
+    const int reps = 5;
+    auto timed = [&](auto &&f) -> double {
+        for (int r = 0; r < reps + 1; r++) {
+            if (r == 1) hipEventRecord(t->ev0, s);
+            f();
+        }
+        hipEventRecord(t->ev1, s);
+        hipEventSynchronize(t->ev1);
+        float msv = 0.f;
+        hipEventElapsedTime(&msv, t->ev0, t->ev1);
+        return msv / reps;
+    };
+    Epi<0> e0;
+    memset(&e0, 0, sizeof e0);
+    auto spmv_once = [&]() {
+        launch_spmv<double, double, 0>(s, A, A->val, x, y, 1.0, 0.0, e0, nullptr, nullptr, NoFin(), true);
+    };
+    // SpMV candidates {variant, rows, cap}
+    struct Cand { int v, rows, cap; };
+    std::vector<Cand> cand;
+    if (A->nsl > 0) {
+        cand.push_back({SPMV_SELL, A->wrows, A->wcap});
+        if (A->scd) cand.push_back({SPMV_SELL32, A->wrows, A->wcap});
+    }
+    if (A->max_row <= SPMV_WCAP && n > 0) {
+        std::vector<int> len(n);
+        long long tot = 0;
+        for (int i = 0; i < n; i++) { len[i] = rp[i + 1] - rp[i]; tot += len[i]; }
+        std::sort(len.begin(), len.end());
+        const double p50 = len[n / 2], p90 = len[(size_t)n * 9 / 10], mean = (double)tot / n;
+        for (int rows : {16, 32, 64})
+            for (double per : {p50, p90, mean, (double)SPMV_WCAP / rows}) {
+                int cap = (int)std::ceil(rows * per / 64.0) * 64;
+                cap = std::min(SPMV_WCAP, std::max(cap, std::max(A->max_row, 64)));
+                bool dup = false;
+                for (const Cand &c : cand) dup |= (c.v == SPMV_WAVE && c.rows == rows && c.cap == cap);
+                if (!dup) cand.push_back({SPMV_WAVE, rows, cap});
+            }
+    }
+    cand.push_back({SPMV_BLOCK, A->wrows, A->wcap});
+    const int v0 = A->variant, r0 = A->wrows, c0 = A->wcap;
+    out->spmv_ms_default = timed(spmv_once);
+    double best = 1e300;
+    Cand bc = {v0, r0, c0};
+    for (const Cand &c : cand) {
+        if (c.v == SPMV_WAVE) {
+            const int rc = set_wave_blocking(A, rp, ci.empty() ? rp : ci, c.rows, c.cap);
+            if (rc) return rc;
+        }
+        A->variant = c.v;
+        const double msv = timed(spmv_once);
+        if (msv < best) { best = msv; bc = c; }
+    }
+    A->variant = bc.v;
+    {
+        const int rc = set_wave_blocking(A, rp, ci.empty() ? rp : ci, bc.v == SPMV_WAVE ? bc.rows : r0,
+                                         bc.v == SPMV_WAVE ? bc.cap : c0);
+        if (rc) return rc;
+    }
+    bkh_graph_epoch_bump();
+    out->spmv_variant = A->variant;
+    out->rows_per_wave = A->wrows;
+    out->nnz_per_wave = A->wcap;
+    out->candidates = (int)cand.size();
+    out->spmv_ms = best;
+    // bkh_spmm's columns per pass for each s
+    for (int sc = 1; sc <= max_s; sc++) {
+        double bw = 1e300;
+        int wbest = sc;
+        for (int w : {1, 2, 4, 8, 16, sc}) {
+            if (w > sc) continue;
+            int rc = 0;
+            const double msv = timed([&]() { rc |= spmm_chunked(t, A, sc, w, 1.0, x, 0.0, y); });
+            if (rc) return rc;
+            if (w == sc) out->spmm_ms_full[sc] = msv;
+            if (msv < bw) { bw = msv; wbest = w; }
+        }
+        A->spmm_w[sc] = wbest;
+        out->spmm_width[sc] = wbest;
+        out->spmm_ms[sc] = bw;
+    }
+    LAUNCH_OK();
+    HIP_OK(hipStreamSynchronize(s));
     return 0;
 }
 

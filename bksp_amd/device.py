@@ -110,6 +110,22 @@ class Matrix:
             raise BkspDeviceError("bkh_matrix_set_spmv failed")
         self.nblocks = load().bkh_matrix_blocks(self.handle)
 
+    def tune(self, max_s=16):
+        """bkh_matrix_tune: keeps the fastest SpMV variant / wave blocking and bkh_spmm's
+        columns per pass for s = 1..max_s; returns what it chose and measured."""
+        from ._lib import BkhMatrixTuning
+        r = BkhMatrixTuning()
+        rc = load().bkh_matrix_tune(self.handle, int(max_s), ctypes.byref(r))
+        if rc:
+            raise BkspDeviceError(f"bkh_matrix_tune -> {rc}: {last_error()}")
+        self.nblocks = load().bkh_matrix_blocks(self.handle)
+        return {"spmv_variant": r.spmv_variant, "rows_per_wave": r.rows_per_wave,
+                "nnz_per_wave": r.nnz_per_wave, "candidates": r.candidates,
+                "spmv_ms": r.spmv_ms, "spmv_ms_default": r.spmv_ms_default,
+                "spmm_width": {s: r.spmm_width[s] for s in range(1, max_s + 1)},
+                "spmm_ms": {s: r.spmm_ms[s] for s in range(1, max_s + 1)},
+                "spmm_ms_full": {s: r.spmm_ms_full[s] for s in range(1, max_s + 1)}}
+
     def spmv(self, d_x, d_y, op="n", alpha=1.0, beta=0.0):
         rc = load().bkh_spmv(self.handle, op.encode(), alpha, d_x.ptr, beta, d_y.ptr)
         if rc:

@@ -94,6 +94,29 @@ int         bkh_matrix_blocks(const bkh_matrix *A);
  * workgroup), 1 = wave CSR-stream (64 rows / 512 nonzeros per wave). Chosen at
  * creation (1 unless a row exceeds 512 nonzeros); the autotuner may override. */
 int         bkh_matrix_set_spmv(bkh_matrix *A, int variant);
+/* Autotuner (no reference counterpart: the reference's SpMV/SpMM are fixed loops over
+ * CSR, src/matvec.c). Times, on this matrix: the SpMV variants it holds (SELL-64, on its
+ * slice-diagonal copy when it has one, with 16- or 32-bit columns; the wave CSR-stream
+ * variant at rows per wavefront 16, 32 and 64 with nonzero caps -- the row-bin
+ * threshold: a longer row gets a block of its own -- from the matrix's row-length
+ * histogram (that many rows of median, 90th-percentile and mean length, rounded up to
+ * 64, at least the longest row, at most 512); the workgroup CSR-stream variant) and
+ * keeps the fastest; then, for s = 1..max_s (<= 16), bkh_spmm's columns per pass
+ * (1, 2, 4, 8, 16 and s, at most s) and keeps the fastest per s (for s > 16,
+ * bkh_spmm passes in chunks of the width chosen for 16). Every choice gives bitwise
+ * the same products. */
+typedef struct {
+    int spmv_variant;       /* 0 workgroup CSR-stream, 1 wave CSR-stream, 2 SELL-64, 3 SELL-64 int32 */
+    int rows_per_wave;      /* wave variant's blocking in use */
+    int nnz_per_wave;
+    int candidates;         /* SpMV candidates timed */
+    double spmv_ms;         /* the chosen SpMV's time */
+    double spmv_ms_default; /* the SpMV variant in use before tuning */
+    int spmm_width[17];     /* [s]: columns per bkh_spmm pass, s = 1..max_s */
+    double spmm_ms[17];     /* [s]: its time */
+    double spmm_ms_full[17];/* [s]: time with all s columns in one pass */
+} bkh_matrix_tuning;
+int         bkh_matrix_tune(bkh_matrix *A, int max_s, bkh_matrix_tuning *out);
 
 /* ---- kernels on device vectors ----
  * y = alpha*op(A)*x + beta*y, op = 'n' (A) or 't' (A^T). beta == 0 overwrites y.
