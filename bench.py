@@ -273,6 +273,11 @@ def bench_cfg4(dv, gen):
         return worst
 
     worst_seq, ms_seq = _timed(seq, 1)
+    # the whole solve as one persistent cooperative launch (bitwise the graph path's x)
+    (ip32, rp32, tp32), msp32 = _timed(lambda: run(precision=dv.BKH_FP32_IR, path=2), 1)
+    same32 = bool(np.array_equal(dx.numpy(), x32))
+    (ip64, rp64, tp64), msp64 = _timed(lambda: run(path=2), 1)
+    same64 = bool(np.array_equal(dx.numpy(), x64))
     layout = Bt.layout()
     return {"workload": f"512 uneven systems n in [2^10, 2^17], N={Bt.N}, nnz={Bt.nnz}, BiCGSTAB(2), tol 1e-8",
             "sequential_fp64_ms": round(ms_seq, 1), "sequential_max_true_res": worst_seq,
@@ -282,8 +287,37 @@ def bench_cfg4(dv, gen):
             "fp32_ir_passes": i32.passes, "converged": int((r32 >= 0).sum()),
             "speedup_vs_sequential_fp64": round(ms_seq / ms32, 2),
             "speedup_vs_batched_fp64": round(ms64 / ms32, 3),
-            "autotuned_layout": {"rows_per_wave": layout[0], "nnz_per_wave": layout[1],
+            "persistent_launch": {"fp32_ir_ms": round(msp32, 1), "fp64_ms": round(msp64, 1),
+                                  "x_equals_graph_path": same32 and same64,
+                                  "max_true_res": max(float(tp32.max()), float(tp64.max()))},
+            "autotuned_layout": {"spmv_kernel": "column-chunked" if Bt.spmv_kernel else "csr-stream",
+                                 "rows_per_wave": layout[0], "nnz_per_wave": layout[1],
                                  "tile_rows": layout[2], "fp32_spmv_ms": round(tune_ms, 3)}}
+
+
+def bench_tuner(dv, gen):
+    """bkh_matrix_tune on the 128^3 Poisson matrix and on a 2M-row random-column matrix:
+    the SpMV variant / wave blocking and bkh_spmm columns per pass it keeps, against the
+    creation-time choice; and bkh_spmm with s = 32 columns (chunks of the tuned width)."""
+    out = {}
+    for name, (n, rp, ci, v), sym in (("poisson128", gen.poisson3d(128, lower=True), True),
+                                      ("random_rows_2M", gen.random_rows(6060, 1 << 21), False)):
+        M = dv.Matrix(n, rp, ci, v, symmetric=sym)
+        r = M.tune(16)
+        names = {0: "workgroup CSR-stream", 1: "wave CSR-stream", 2: "SELL-64", 3: "SELL-64 int32 columns"}
+        X = dv.DeviceArray.from_numpy(gen.normal(9, n * 32))
+        Y = dv.DeviceArray((n * 32,))
+        M.spmm(32, X, Y)
+        _, ms32 = _timed(lambda: M.spmm(32, X, Y), 3)
+        out[name] = {"n": n, "nnz": M.nnz, "spmv": names[r["spmv_variant"]], "rows_per_wave": r["rows_per_wave"],
+                     "nnz_per_wave": r["nnz_per_wave"], "candidates": r["candidates"],
+                     "spmv_ms": round(r["spmv_ms"], 4), "spmv_ms_creation_choice": round(r["spmv_ms_default"], 4),
+                     "spmm_width": {s_: r["spmm_width"][s_] for s_ in (2, 4, 8, 12, 16)},
+                     "spmm_ms": {s_: round(r["spmm_ms"][s_], 4) for s_ in (2, 4, 8, 12, 16)},
+                     "spmm_ms_one_pass": {s_: round(r["spmm_ms_full"][s_], 4) for s_ in (2, 4, 8, 12, 16)},
+                     "spmm_s32_ms_incl_host_sync": round(ms32, 3)}
+        del M, X, Y
+    return out
 
 
 def bench_cfg5(dv, gen, n_train=200, m_train=64, n_test=50, m_test=128, n_plain=2, epochs=4):
@@ -364,7 +398,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--m", type=int, default=128)
-    ap.add_argument("--configs", default="cfg1,cfg2b,cfg3,cfg4,cfg5,complex",
+    ap.add_argument("--configs", default="cfg1,cfg2b,cfg3,cfg4,cfg5,complex,tuner",
                     help="extra workloads reported beside the main line ('' for none)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -543,7 +577,8 @@ def main():
         del M, db, dx, dy, db32, dy32
         for name, fn in (("cfg1", lambda: bench_cfg1(dv, gen, po)), ("cfg2_block", lambda: bench_cfg2_block(dv, gen)),
                          ("cfg3", lambda: bench_cfg3(dv, gen)), ("cfg4", lambda: bench_cfg4(dv, gen)),
-                         ("cfg5", lambda: bench_cfg5(dv, gen)), ("complex", lambda: bench_complex(dv, gen, po))):
+                         ("cfg5", lambda: bench_cfg5(dv, gen)), ("complex", lambda: bench_complex(dv, gen, po)),
+                         ("tuner", lambda: bench_tuner(dv, gen))):
             key = {"cfg2_block": "cfg2b"}.get(name, name)
             if key in want:
                 try:
