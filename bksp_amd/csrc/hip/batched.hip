@@ -361,14 +361,45 @@ __global__ __launch_bounds__(256) void k_bt_init(const int *trow, const int *tsy
     bt_init_tile<T>(blockIdx.x, trow, tsys, b, R, P, rs, sy, part, sh);
 }
 
+// The vector tiles step a lane through its rows four at a time (rows i, i + 256,
+// i + 512, i + 768): every load of the four is issued before the arithmetic, which then
+// runs row by row in the order of the plain loop, so results and partial sums are those
+// of the one-row-at-a-time loop.
+// U rows per step (template parameter of the tile functions: 4 in the per-phase kernels,
+// 1 in the persistent kernels, where the registers of four rows would spill).
+#define TILE_ROWS4(t_)                                                                            \
+    const long long e_ = trow[(t_) + 1];                                                          \
+    for (long long i0_ = (long long)trow[t_] + (phase_tid() & 255); i0_ < e_; i0_ += 256 * U)
+// the rows of one step and whether each is in the tile
+template <int U>
+__device__ __forceinline__ void tile_step(long long i0, long long e, long long (&ii)[U], bool (&in)[U]) {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        ii[u] = i0 + 256 * u;
+        in[u] = ii[u] < e;
+    }
+}
+
 // p_q = beta p_q + r_q, q <= j (bicgstabl.c:64-67)
-template <typename T>
+template <typename T, int U = 4>
 __device__ __forceinline__ void bt_p_tile(int t, const int *trow, const int *tsys, long long N, int j, T *P,
                                           const T *R, const BtSys *sy) {
-    if (sy[tsys[t]].done) return;
-    TILE_ROWS(i, t) {
-        const T be = (T)sy[sys].beta;
-        for (int q = 0; q <= j; q++) P[q * N + i] = add_rn(mul_rn(be, P[q * N + i]), R[q * N + i]);
+    const int sys = tsys[t];
+    if (sy[sys].done) return;
+    const T be = (T)sy[sys].beta;
+    TILE_ROWS4(t) {
+        long long ii[U];
+        bool in[U];
+        tile_step<U>(i0_, e_, ii, in);
+        for (int q = 0; q <= j; q++) {
+            T pv[U] = {}, rv[U] = {};
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (in[u]) { pv[u] = P[q * N + ii[u]]; rv[u] = R[q * N + ii[u]]; }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (in[u]) P[q * N + ii[u]] = add_rn(mul_rn(be, pv[u]), rv[u]);
+        }
     }
 }
 template <typename T>
@@ -378,14 +409,32 @@ __global__ __launch_bounds__(256) void k_bt_p(const int *trow, const int *tsys, 
 }
 
 // r_q -= alpha p_{q+1}, q <= j; x += alpha p_0 (:73-75, :79)
-template <typename T>
+template <typename T, int U = 4>
 __device__ __forceinline__ void bt_r_tile(int t, const int *trow, const int *tsys, long long N, int j, const T *P,
                                           T *R, T *x, const BtSys *sy) {
-    if (sy[tsys[t]].done) return;
-    TILE_ROWS(i, t) {
-        const T al = (T)sy[sys].alpha;
-        for (int q = 0; q <= j; q++) R[q * N + i] = fma(-al, P[(q + 1) * N + i], R[q * N + i]);
-        x[i] = fma(al, P[i], x[i]);
+    const int sys = tsys[t];
+    if (sy[sys].done) return;
+    const T al = (T)sy[sys].alpha;
+    TILE_ROWS4(t) {
+        long long ii[U];
+        bool in[U];
+        tile_step<U>(i0_, e_, ii, in);
+        for (int q = 0; q <= j; q++) {
+            T pv[U] = {}, rv[U] = {};
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (in[u]) { pv[u] = P[(q + 1) * N + ii[u]]; rv[u] = R[q * N + ii[u]]; }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (in[u]) R[q * N + ii[u]] = fma(-al, pv[u], rv[u]);
+        }
+        T pv[U] = {}, xv[U] = {};
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (in[u]) { pv[u] = P[ii[u]]; xv[u] = x[ii[u]]; }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (in[u]) x[ii[u]] = fma(al, pv[u], xv[u]);
     }
 }
 template <typename T>
@@ -395,7 +444,7 @@ __global__ __launch_bounds__(256) void k_bt_r(const int *trow, const int *tsys, 
 }
 
 // MR update of x, r_0, p_0 with partials {r0.r0, rs.r0} (:113-125)
-template <typename T, int NGRP = 1>
+template <typename T, int NGRP = 1, int U = 4>
 __device__ __forceinline__ void bt_upd_tile(int t, const int *trow, const int *tsys, long long N, int l, T *P,
                                             T *R, T *x, const T *rs, const BtSys *sy, double *part, double *sh,
                                             bool valid = true) {
@@ -404,21 +453,40 @@ __device__ __forceinline__ void bt_upd_tile(int t, const int *trow, const int *t
     if (valid && !sy[sy_i].done) {
         double cx[MR_D], cr[MR_D], cp[MR_D];
         for (int k = 0; k <= l; k++) { cx[k] = sy[sy_i].cx[k]; cr[k] = sy[sy_i].cr[k]; cp[k] = sy[sy_i].cp[k]; }
-        TILE_ROWS(i, t) {
-            double xi = (double)x[i], r0 = (double)R[i], p0 = (double)P[i];
-            xi += cx[0] * r0;
+        TILE_ROWS4(t) {
+            long long ii[U];
+            bool in[U];
+            tile_step<U>(i0_, e_, ii, in);
+            double xi[U] = {}, r0[U] = {}, p0[U] = {}, rsv[U] = {};
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (in[u]) {
+                    xi[u] = (double)x[ii[u]]; r0[u] = (double)R[ii[u]]; p0[u] = (double)P[ii[u]];
+                    rsv[u] = (double)rs[ii[u]];
+                }
+#pragma unroll
+            for (int u = 0; u < U; u++) xi[u] += cx[0] * r0[u];
             for (int k = 1; k <= l; k++) {
-                const double rk = (double)R[k * N + i];
-                if (k < l) xi += cx[k] * rk;
-                r0 -= cr[k] * rk;
-                p0 -= cp[k] * (double)P[k * N + i];
+                double rk[U] = {}, pk[U] = {};
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    if (in[u]) { rk[u] = (double)R[k * N + ii[u]]; pk[u] = (double)P[k * N + ii[u]]; }
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    if (k < l) xi[u] += cx[k] * rk[u];
+                    r0[u] -= cr[k] * rk[u];
+                    p0[u] -= cp[k] * pk[u];
+                }
             }
-            x[i] = (T)xi; R[i] = (T)r0; P[i] = (T)p0;
-            const double r0t = (double)(T)r0;
-            v[0] += r0t * r0t;
-            v[1] += (double)rs[i] * r0t;
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (in[u]) {
+                    x[ii[u]] = (T)xi[u]; R[ii[u]] = (T)r0[u]; P[ii[u]] = (T)p0[u];
+                    const double r0t = (double)(T)r0[u];
+                    v[0] += r0t * r0t;
+                    v[1] += rsv[u] * r0t;
+                }
         }
-        (void)sys;
     }
     unit_partial<2, NGRP>(v, part, t, sh, valid);
 }
@@ -702,14 +770,14 @@ __global__ __launch_bounds__(256) void k_bt_persist(BtPArgs<VT, T> a) {
     int it = 0;
     for (; it < a.max_it; it++) {
         for (int j = 0; j < L; j++) {
-            BT_DEAL(t, a.ntile) bt_p_tile<T>(t, a.trow, a.tsys, N, j, a.P, a.R, a.sy);
+            BT_DEAL(t, a.ntile) bt_p_tile<T, 1>(t, a.trow, a.tsys, N, j, a.P, a.R, a.sy);
             BT_SYNC();
             BT_DEAL(u, a.nunit) bt_spmv_unit<VT, T, 1>(u, a.rp, a.ci, a.val, a.wblk, a.usys, a.sy, a.P + j * N,
                                                        a.P + (j + 1) * N, e1, a.part, prod_all, sh);
             BT_SYNC();
             BT_DEAL(q, a.ns) bt_fin_sys<BT_ALPHA, 1>(q, a.part, a.ub, a.sy, L, sh);
             BT_SYNC();
-            BT_DEAL(t, a.ntile) bt_r_tile<T>(t, a.trow, a.tsys, N, j, a.P, a.R, a.x, a.sy);
+            BT_DEAL(t, a.ntile) bt_r_tile<T, 1>(t, a.trow, a.tsys, N, j, a.P, a.R, a.x, a.sy);
             BT_SYNC();
             if (j < L - 1) {
                 BT_DEAL(u, a.nunit) bt_spmv_unit<VT, T, 1>(u, a.rp, a.ci, a.val, a.wblk, a.usys, a.sy, a.R + j * N,
@@ -724,7 +792,7 @@ __global__ __launch_bounds__(256) void k_bt_persist(BtPArgs<VT, T> a) {
             }
             BT_SYNC();
         }
-        BT_DEAL(t, a.ntile) bt_upd_tile<T>(t, a.trow, a.tsys, N, L, a.P, a.R, a.x, a.rs, a.sy, a.part, sh);
+        BT_DEAL(t, a.ntile) bt_upd_tile<T, 1, 1>(t, a.trow, a.tsys, N, L, a.P, a.R, a.x, a.rs, a.sy, a.part, sh);
         BT_SYNC();
         BT_DEAL(q, a.ns) bt_fin_sys<BT_UPD, 2>(q, a.part, a.tb, a.sy, L, sh);
         BT_SYNC();
@@ -807,14 +875,14 @@ __global__ __launch_bounds__(LC_NT) __attribute__((amdgpu_waves_per_eu(8, 8))) v
     int it = 0;
     for (; it < A.max_it; it++) {
         for (int j = 0; j < L; j++) {
-            BT_TILES(t) if (t < A.ntile) bt_p_tile<T>(t, A.trow, A.tsys, A.N, j, A.P, A.R, A.sy);
+            BT_TILES(t) if (t < A.ntile) bt_p_tile<T, 1>(t, A.trow, A.tsys, A.N, j, A.P, A.R, A.sy);
             BT_SYNC();
             BT_ITEMS(u) bt_spmv_item<VT, T, 1, BLK>(u, la, A.sy, A.P + j * A.N, A.P + (j + 1) * A.N,
                                                               epi_dot<T>(A.rs), A.part, xs, prod);
             BT_SYNC();
             BT_SYSTEMS(q) bt_fin_sys<BT_ALPHA, 1, NGRP>(q, A.part, ib, A.sy, L, sh, q < A.ns);
             BT_SYNC();
-            BT_TILES(t) if (t < A.ntile) bt_r_tile<T>(t, A.trow, A.tsys, A.N, j, A.P, A.R, A.x, A.sy);
+            BT_TILES(t) if (t < A.ntile) bt_r_tile<T, 1>(t, A.trow, A.tsys, A.N, j, A.P, A.R, A.x, A.sy);
             BT_SYNC();
             if (j < L - 1) {
                 BT_ITEMS(u) bt_spmv_item<VT, T, 1, BLK>(u, la, A.sy, A.R + j * A.N,
@@ -831,7 +899,7 @@ __global__ __launch_bounds__(LC_NT) __attribute__((amdgpu_waves_per_eu(8, 8))) v
             }
             BT_SYNC();
         }
-        BT_TILES(t) bt_upd_tile<T, NGRP>(t, A.trow, A.tsys, A.N, L, A.P, A.R, A.x, A.rs, A.sy, A.part, sh,
+        BT_TILES(t) bt_upd_tile<T, NGRP, 1>(t, A.trow, A.tsys, A.N, L, A.P, A.R, A.x, A.rs, A.sy, A.part, sh,
                                          t < A.ntile);
         BT_SYNC();
         BT_SYSTEMS(q) bt_fin_sys<BT_UPD, 2, NGRP>(q, A.part, A.tb, A.sy, L, sh, q < A.ns);
