@@ -377,6 +377,52 @@ __global__ __launch_bounds__(256) void k_zspmm1(int n, int s, const int *__restr
     }
 }
 
+// k_zspmm1 for s = S <= 8 known at compile time: each stored entry is loaded once and
+// applied to all S columns (the runtime-s form re-reads A per column); per column every
+// row still starts from beta Y and adds (alpha a) x in stored order, so Y is bitwise the
+// same.
+template <int S, bool CONJ>
+__global__ __launch_bounds__(256) void k_zspmm1_t(int n, const int *__restrict__ so, const int *__restrict__ rl,
+                                                  const int *__restrict__ col, const zd *__restrict__ val,
+                                                  zd alpha, zd beta, const zd *X, zd *Y) {
+    const int row = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (row >= n) return;
+    const int len = rl[row];
+    const int base = so[row >> 6] + (row & 63);
+    zd acc[S];
+#pragma unroll
+    for (int j = 0; j < S; j++) acc[j] = zmul(beta, Y[(size_t)j * n + row]);
+    for (int k = 0; k < len; k++) {
+        const int e = base + 64 * k;
+        zd a = val[e];
+        if (CONJ) a = zconj(a);
+        const zd aa = zmul(alpha, a);
+        const int c = col[e];
+#pragma unroll
+        for (int j = 0; j < S; j++) acc[j] = zadd(acc[j], zmul(aa, X[(size_t)j * n + c]));
+    }
+#pragma unroll
+    for (int j = 0; j < S; j++) Y[(size_t)j * n + row] = acc[j];
+}
+
+// Y = beta Y + alpha op(A) X over the SELL copy M (s columns); the long rows' CSR tail
+// is added by launch_zsp_tail.
+template <bool CONJ>
+static void zspmm1_launch(hipStream_t st, const ZSell &M, int n, int s, zd al, zd be, const zd *X, zd *Y) {
+    const int G = (n + 255) / 256;
+    switch (s) {
+#define ZSPMM_CASE(S_)                                                                                          \
+    case S_:                                                                                                    \
+        hipLaunchKernelGGL((k_zspmm1_t<S_, CONJ>), dim3(G), dim3(256), 0, st, n, M.so, M.rl, M.col, M.val, al, be, X, Y); \
+        break;
+        ZSPMM_CASE(1) ZSPMM_CASE(2) ZSPMM_CASE(3) ZSPMM_CASE(4)
+        ZSPMM_CASE(5) ZSPMM_CASE(6) ZSPMM_CASE(7) ZSPMM_CASE(8)
+#undef ZSPMM_CASE
+    default:
+        hipLaunchKernelGGL((k_zspmm1<CONJ>), dim3(G), dim3(256), 0, st, n, s, M.so, M.rl, M.col, M.val, al, be, X, Y);
+    }
+}
+
 extern "C" int bkh_zspmv(const bkh_zmatrix *A, char op, const double *d_x, double *d_y) {
     ThreadCtx *t = bkh_tctx();
     if (!t) return BKH_DEVICE_ERROR;
@@ -395,12 +441,9 @@ extern "C" int bkh_zspmm1(const bkh_zmatrix *A, char op, const double *alpha, in
     const ZSell &M = op == 't' ? A->AT : A->A;
     const int G = (A->n + 255) / 256;
     const zd al = zmk(alpha[0], alpha[1]), be = zmk(beta[0], beta[1]);
-    if (op == 't')
-        hipLaunchKernelGGL((k_zspmm1<true>), dim3(G), dim3(256), 0, t->stream, A->n, s, M.so, M.rl, M.col, M.val, al,
-                           be, (const zd *)d_X, (zd *)d_Y);
-    else
-        hipLaunchKernelGGL((k_zspmm1<false>), dim3(G), dim3(256), 0, t->stream, A->n, s, M.so, M.rl, M.col, M.val, al,
-                           be, (const zd *)d_X, (zd *)d_Y);
+    (void)G;
+    if (op == 't') zspmm1_launch<true>(t->stream, M, A->n, s, al, be, (const zd *)d_X, (zd *)d_Y);
+    else zspmm1_launch<false>(t->stream, M, A->n, s, al, be, (const zd *)d_X, (zd *)d_Y);
     launch_zsp_tail<true>(t->stream, M, op == 't', A->n, s, al, (const zd *)d_X, (zd *)d_Y, nullptr);
     LAUNCH_OK();
     HIP_OK(hipStreamSynchronize(t->stream));
@@ -1512,6 +1555,64 @@ __global__ __launch_bounds__(256) void k_zgram(long long m, int s, const zd *X, 
 
 static inline int zgram_grid(long long m) { return (int)std::max(1LL, std::min((m + 63) / 64, 256LL)); }
 
+// The same partial Gram for s = S <= 4: every thread accumulates all S x S entries over
+// its rows (grid-stride) in registers, then the workgroup sums them (wave sums, then the
+// four waves in order) into the block's partial row.
+template <int S, bool CONJ>
+__global__ __launch_bounds__(256) void k_zgram_r(long long m, const zd *X, const zd *Y, const Ctl *c, double *part) {
+    if (c && ctl_stopped(c)) return;
+    constexpr int SS = S * S;
+    double re[SS], im[SS];
+#pragma unroll
+    for (int e = 0; e < SS; e++) { re[e] = 0.0; im[e] = 0.0; }
+    ZLOOP(r, m) {
+        zd x[S], y[S];
+#pragma unroll
+        for (int i = 0; i < S; i++) {
+            x[i] = X[(size_t)i * m + r];
+            if (CONJ) x[i] = zconj(x[i]);
+            y[i] = Y[(size_t)i * m + r];
+        }
+#pragma unroll
+        for (int j = 0; j < S; j++)
+#pragma unroll
+            for (int i = 0; i < S; i++) {
+                const zd pr = zmul(x[i], y[j]);
+                re[i + j * S] += pr.x;
+                im[i + j * S] += pr.y;
+            }
+    }
+    __shared__ double sh[2 * SS * 4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int e = 0; e < SS; e++) {
+        re[e] = wave_sum(re[e]);
+        im[e] = wave_sum(im[e]);
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int e = 0; e < SS; e++) { sh[(2 * e) * 4 + w] = re[e]; sh[(2 * e + 1) * 4 + w] = im[e]; }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < 2 * SS) {
+        const double *v = sh + threadIdx.x * 4;
+        part[(size_t)blockIdx.x * 2 * SS + threadIdx.x] = ((v[0] + v[1]) + v[2]) + v[3];
+    }
+}
+
+// Partial Gram rows (zgram_grid(m) of them) of f(X)^T Y, f = conj when CONJ.
+template <bool CONJ>
+static void zgram_launch(hipStream_t st, long long m, int s, const zd *X, const zd *Y, const Ctl *c, double *part) {
+    const int Gg = zgram_grid(m);
+    switch (s) {
+    case 1: hipLaunchKernelGGL((k_zgram_r<1, CONJ>), dim3(Gg), dim3(256), 0, st, m, X, Y, c, part); break;
+    case 2: hipLaunchKernelGGL((k_zgram_r<2, CONJ>), dim3(Gg), dim3(256), 0, st, m, X, Y, c, part); break;
+    case 3: hipLaunchKernelGGL((k_zgram_r<3, CONJ>), dim3(Gg), dim3(256), 0, st, m, X, Y, c, part); break;
+    case 4: hipLaunchKernelGGL((k_zgram_r<4, CONJ>), dim3(Gg), dim3(256), 0, st, m, X, Y, c, part); break;
+    default: hipLaunchKernelGGL((k_zgram<CONJ>), dim3(Gg), dim3(256), 0, st, m, s, X, Y, c, part);
+    }
+}
+
 enum { ZB_QR1 = 0, ZB_QR2_INIT = 1, ZB_QR2_ITER = 2, ZB_BETA = 3, ZB_ALPHA = 4, ZB_XI = 5 };
 
 // Sums the Gram partials (fixed order) and runs the s x s step `op`:
@@ -1536,13 +1637,24 @@ __global__ __launch_bounds__(256) void k_zblk_fin(const double *part, int nblock
         zsm_triu_inv(s, T1, st->t);
         return;
     }
-    if (t < ss) {
+    {
+        // the partial rows in 256 / ss interleaved groups, then the groups in order
+        const int ng = 256 / ss, e = t % ss, g = t / ss;
         double re = 0.0, im = 0.0;
-        for (int b = 0; b < nblocks; b++) {
-            re += part[((size_t)b * ss + t) * 2];
-            im += part[((size_t)b * ss + t) * 2 + 1];
+        if (g < ng)
+            for (int b = g; b < nblocks; b += ng) {
+                re += part[((size_t)b * ss + e) * 2];
+                im += part[((size_t)b * ss + e) * 2 + 1];
+            }
+        double *W2 = (double *)W;
+        __syncthreads();
+        if (g < ng) { W2[2 * (g * ss + e)] = re; W2[2 * (g * ss + e) + 1] = im; }
+        __syncthreads();
+        if (t < ss) {
+            double sr = 0.0, si = 0.0;
+            for (int k = 0; k < ng; k++) { sr += W2[2 * (k * ss + t)]; si += W2[2 * (k * ss + t) + 1]; }
+            G[t] = zmk(sr, si);
         }
-        G[t] = zmk(re, im);
     }
     __syncthreads();
     if (op == ZB_BETA) {
@@ -1568,6 +1680,86 @@ __global__ __launch_bounds__(256) void k_zblk_fin(const double *part, int nblock
     zsm_triu_inv(s, T1, st->R2i);
     zsm_mul(s, T1, st->R1, T2);                    // R = R2 R1
     zsm_copy(s, T2, op == ZB_QR2_INIT ? st->xi : st->t);
+}
+
+// k_zpan / k_zpan_vt for s = S known at compile time: the row's entries stay in
+// registers and the s x s matrices in LDS (the runtime-s forms index private arrays and
+// spill them); the same operations in the same order.
+template <int S>
+__global__ __launch_bounds__(256) void k_zpan_t(long long m, const zd *X, const zd *M, zd a, const zd *Y, zd b,
+                                                zd *O, const Ctl *c) {
+    if (c && ctl_stopped(c)) return;
+    __shared__ zd Ms[S * S];
+    for (int e = threadIdx.x; e < S * S; e += 256) Ms[e] = M[e];
+    __syncthreads();
+    ZLOOP(r, m) {
+        zd xr[S];
+#pragma unroll
+        for (int l = 0; l < S; l++) xr[l] = X[(size_t)l * m + r];
+#pragma unroll
+        for (int q = 0; q < S; q++) {
+            zd acc = zmk(0.0, 0.0);
+#pragma unroll
+            for (int l = 0; l < S; l++) acc = zadd(acc, zmul(xr[l], Ms[l + q * S]));
+            zd o = zmul(a, acc);
+            if (Y) o = zadd(o, zmul(b, Y[(size_t)q * m + r]));
+            O[(size_t)q * m + r] = o;
+        }
+    }
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void k_zpan_vt_t(long long m, zd *V, const zd *P1, const ZBState *st,
+                                                   const Ctl *c) {
+    if (ctl_stopped(c)) return;
+    __shared__ zd al[S * S], tt[S * S];
+    for (int e = threadIdx.x; e < S * S; e += 256) {
+        const int l = e % S, q = e / S;
+        al[e] = st->alpha[l + q * S];
+        tt[e] = st->t[l + q * S];
+    }
+    __syncthreads();
+    ZLOOP(r, m) {
+        zd v[S], p[S], w[S];
+#pragma unroll
+        for (int l = 0; l < S; l++) { p[l] = P1[(size_t)l * m + r]; v[l] = V[(size_t)l * m + r]; }
+#pragma unroll
+        for (int q = 0; q < S; q++) {
+            zd acc = zmk(0.0, 0.0);
+#pragma unroll
+            for (int l = 0; l < S; l++) acc = zadd(acc, zmul(p[l], al[l + q * S]));
+            w[q] = zadd(zmul(zmk(-1.0, 0.0), acc), zmul(zmk(1.0, 0.0), v[q]));
+        }
+#pragma unroll
+        for (int q = 0; q < S; q++) {
+            zd acc = zmk(0.0, 0.0);
+#pragma unroll
+            for (int l = 0; l < S; l++) acc = zadd(acc, zmul(w[l], tt[l + q * S]));
+            V[(size_t)q * m + r] = zmul(zmk(1.0, 0.0), acc);
+        }
+    }
+}
+
+#define ZB_SWITCH(s_, CALL)                                                                   \
+    switch (s_) {                                                                             \
+    case 1: CALL(1); break; case 2: CALL(2); break; case 3: CALL(3); break; case 4: CALL(4); break; \
+    case 5: CALL(5); break; case 6: CALL(6); break; case 7: CALL(7); break; case 8: CALL(8); break; \
+    case 9: CALL(9); break; case 10: CALL(10); break; case 11: CALL(11); break; case 12: CALL(12); break; \
+    case 13: CALL(13); break; case 14: CALL(14); break; case 15: CALL(15); break; default: CALL(16); break; \
+    }
+
+static void zpan_launch(hipStream_t st, int G, long long m, int s, const zd *X, const zd *M, zd a, const zd *Y, zd b,
+                        zd *O, const Ctl *c) {
+#define ZPAN_CALL(S_) hipLaunchKernelGGL((k_zpan_t<S_>), dim3(G), dim3(256), 0, st, m, X, M, a, Y, b, O, c)
+    ZB_SWITCH(s, ZPAN_CALL)
+#undef ZPAN_CALL
+}
+
+static void zpan_vt_launch(hipStream_t st, int G, long long m, int s, zd *V, const zd *P1, const ZBState *bs,
+                           const Ctl *c) {
+#define ZPANVT_CALL(S_) hipLaunchKernelGGL((k_zpan_vt_t<S_>), dim3(G), dim3(256), 0, st, m, V, P1, bs, c)
+    ZB_SWITCH(s, ZPANVT_CALL)
+#undef ZPANVT_CALL
 }
 
 // One row per thread, all s columns: O[r,:] = a (X[r,:] M) (+ b Y[r,:]); M from the
@@ -1667,24 +1859,22 @@ struct ZFnbFin {
 };
 
 static void launch_zspmm1(hipStream_t st, const bkh_zmatrix *A, int s, zd alpha, const zd *X, zd beta, zd *Y) {
-    const int G = (A->n + 255) / 256;
-    hipLaunchKernelGGL((k_zspmm1<false>), dim3(G), dim3(256), 0, st, A->n, s, A->A.so, A->A.rl, A->A.col, A->A.val,
-                       alpha, beta, X, Y);
+    zspmm1_launch<false>(st, A->A, A->n, s, alpha, beta, X, Y);
     launch_zsp_tail<true>(st, A->A, false, A->n, s, alpha, X, Y, nullptr);
 }
 
 // CholeskyQR2 of V (m x s) in place; op2 = ZB_QR2_INIT or ZB_QR2_ITER.
 static void zqr2(hipStream_t st, long long m, int s, zd *V, ZBState *bs, Ctl *c, double *part, int op2) {
     const int Gg = zgram_grid(m), Gp = vec_grid(m, 256);
-    hipLaunchKernelGGL((k_zgram<true>), dim3(Gg), dim3(256), 0, st, m, s, (const zd *)V, (const zd *)V,
+    zgram_launch<true>(st, m, s, (const zd *)V, (const zd *)V,
                        (const Ctl *)c, part);
     hipLaunchKernelGGL(k_zblk_fin, dim3(1), dim3(256), 0, st, (const double *)part, Gg, s, (int)ZB_QR1, 0, 0, bs, c);
-    hipLaunchKernelGGL(k_zpan, dim3(Gp), dim3(256), 0, st, m, s, (const zd *)V, (const zd *)bs->R1i, zmk(1.0, 0.0),
+    zpan_launch(st, Gp, m, s, (const zd *)V, (const zd *)bs->R1i, zmk(1.0, 0.0),
                        (const zd *)nullptr, zmk(0.0, 0.0), V, (const Ctl *)c);
-    hipLaunchKernelGGL((k_zgram<true>), dim3(Gg), dim3(256), 0, st, m, s, (const zd *)V, (const zd *)V,
+    zgram_launch<true>(st, m, s, (const zd *)V, (const zd *)V,
                        (const Ctl *)c, part);
     hipLaunchKernelGGL(k_zblk_fin, dim3(1), dim3(256), 0, st, (const double *)part, Gg, s, op2, 0, 0, bs, c);
-    hipLaunchKernelGGL(k_zpan, dim3(Gp), dim3(256), 0, st, m, s, (const zd *)V, (const zd *)bs->R2i, zmk(1.0, 0.0),
+    zpan_launch(st, Gp, m, s, (const zd *)V, (const zd *)bs->R2i, zmk(1.0, 0.0),
                        (const zd *)nullptr, zmk(0.0, 0.0), V, (const Ctl *)c);
 }
 
@@ -1738,35 +1928,35 @@ extern "C" int bkh_zblbicgstabl(const bkh_zmatrix *A, int s, const double *d_B, 
     hipGraphExec_t ge = capture_cached(t, {0x2B19, (uintptr_t)A, (uintptr_t)B, (uintptr_t)X, (uintptr_t)c,
                                            (uintptr_t)part, (uintptr_t)Pb, (uintptr_t)l, (uintptr_t)s}, st, 1, [=]() {
         for (int j = 0; j < l; j++) {
-            hipLaunchKernelGGL((k_zgram<false>), dim3(Gg), dim3(256), 0, st, m, s, (const zd *)Rs, (const zd *)Vi(j),
+            zgram_launch<false>(st, m, s, (const zd *)Rs, (const zd *)Vi(j),
                                (const Ctl *)c, part);
             hipLaunchKernelGGL(k_zblk_fin, dim3(1), dim3(256), 0, st, (const double *)part, Gg, s, (int)ZB_BETA, j, l,
                                bs, c);
             for (int i = 0; i <= j; i++)                         // P_i = (-1) P_i beta + 1 V_i (:353-358)
-                hipLaunchKernelGGL(k_zpan, dim3(Gp), dim3(256), 0, st, m, s, (const zd *)Pi(i), (const zd *)bs->beta,
+                zpan_launch(st, Gp, m, s, (const zd *)Pi(i), (const zd *)bs->beta,
                                    mone, (const zd *)Vi(i), one, Pi(i), (const Ctl *)c);
             launch_zspmm1(st, A, s, one, Pi(j), zero, Pi(j + 1));
-            hipLaunchKernelGGL((k_zgram<false>), dim3(Gg), dim3(256), 0, st, m, s, (const zd *)Rs,
+            zgram_launch<false>(st, m, s, (const zd *)Rs,
                                (const zd *)Pi(j + 1), (const Ctl *)c, part);
             hipLaunchKernelGGL(k_zblk_fin, dim3(1), dim3(256), 0, st, (const double *)part, Gg, s, (int)ZB_ALPHA, j,
                                l, bs, c);
             // i = 0 (:371-381): tmp = P0 alpha; V0 = V0 - A tmp; qr(V0) -> t; X += tmp xi;
             // xi = t xi; t = t^-1
-            hipLaunchKernelGGL(k_zpan, dim3(Gp), dim3(256), 0, st, m, s, (const zd *)Pi(0), (const zd *)bs->alpha, one,
+            zpan_launch(st, Gp, m, s, (const zd *)Pi(0), (const zd *)bs->alpha, one,
                                (const zd *)nullptr, zero, tmp, (const Ctl *)c);
             launch_zspmm1(st, A, s, mone, tmp, one, Vi(0));
             zqr2(st, m, s, Vi(0), bs, c, part, ZB_QR2_ITER);
-            hipLaunchKernelGGL(k_zpan, dim3(Gp), dim3(256), 0, st, m, s, (const zd *)tmp, (const zd *)bs->xi, one,
+            zpan_launch(st, Gp, m, s, (const zd *)tmp, (const zd *)bs->xi, one,
                                (const zd *)X, one, X, (const Ctl *)c);
             hipLaunchKernelGGL(k_zblk_fin, dim3(1), dim3(256), 0, st, (const double *)part, 0, s, (int)ZB_XI, j,
                                l, bs, c);
             for (int i = 1; i <= j; i++)                         // :392-396
-                hipLaunchKernelGGL(k_zpan_vt, dim3(Gp), dim3(256), 0, st, m, s, Vi(i), (const zd *)Pi(i + 1),
+                zpan_vt_launch(st, Gp, m, s, Vi(i), (const zd *)Pi(i + 1),
                                    (const ZBState *)bs, (const Ctl *)c);
             launch_zspmm1(st, A, s, one, Vi(j), zero, Vi(j + 1));
         }
         for (int i = 0; i <= l; i++)                             // R_i = V_i xi (:406-409)
-            hipLaunchKernelGGL(k_zpan, dim3(Gp), dim3(256), 0, st, m, s, (const zd *)Vi(i), (const zd *)bs->xi, one,
+            zpan_launch(st, Gp, m, s, (const zd *)Vi(i), (const zd *)bs->xi, one,
                                (const zd *)nullptr, zero, Ri(i), (const Ctl *)c);
         for (int j = 0; j < l; j++) {                            // MR (:413-432)
             zd *rj = Ri(j + 1);
@@ -1782,7 +1972,7 @@ extern "C" int bkh_zblbicgstabl(const bkh_zmatrix *A, int s, const double *d_B, 
         }
         hipLaunchKernelGGL(k_zblk_upd, dim3(Gv), dim3(256), 0, st, ms, l, (const zd *)Vb, Pb, tmp, (const Ctl *)c);
         launch_zspmm1(st, A, s, mone, tmp, one, Vi(0));
-        hipLaunchKernelGGL(k_zpan, dim3(Gp), dim3(256), 0, st, m, s, (const zd *)tmp, (const zd *)bs->xi, one,
+        zpan_launch(st, Gp, m, s, (const zd *)tmp, (const zd *)bs->xi, one,
                            (const zd *)X, one, X, (const Ctl *)c);
         hipLaunchKernelGGL(k_zblk_check, dim3(1), dim3(64), 0, st, s, (const ZBState *)bs, c);
     });

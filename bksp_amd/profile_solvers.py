@@ -27,7 +27,7 @@ from . import gen
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--solver", choices=["cg", "blcg", "shifted", "pcg", "bicgstabl", "zbicgstabl", "zshbicgstabl",
-                                         "batched", "batch_spmv"],
+                                         "zblbicgstabl", "batched", "batch_spmv"],
                     default="cg")
     ap.add_argument("--precision", choices=["fp64", "fp32ir"], default="fp64")
     ap.add_argument("--precond", choices=["jacobi", "fsai"], default="jacobi")
@@ -102,6 +102,18 @@ def main():
             def run():
                 dx.zero()
                 return Bt.bicgstabl(db, 2, dx, 1e-8, 1000, precision=prec)[0]
+    elif args.solver == "zblbicgstabl":
+        m = args.m or 96
+        n, rp, ci, v = gen.zconvdiff3d(m)
+        M = dv.ZMatrix(n, rp, ci, v)
+        sb = 4
+        B = (gen.normal(305, n * sb) + 1j * gen.normal(306, n * sb)).reshape(sb, n)
+        dB = dv.DeviceArray.from_numpy(B.ravel())
+        dx = dv.DeviceArray((n * sb,), np.complex128)
+
+        def run():
+            dx.zero()
+            return dv.zblbicgstabl(M, sb, dB, 4, dx, 1e-8, 1000)
     elif args.solver in ("zbicgstabl", "zshbicgstabl"):
         m = args.m or 96
         n, rp, ci, v = gen.zconvdiff3d(m)
