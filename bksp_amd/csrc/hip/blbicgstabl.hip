@@ -319,11 +319,12 @@ __global__ __launch_bounds__(256) void k_bb_copy2(long long ms, const double *V0
     }
 }
 
+// On the SELL-64 / slice-diagonal copies when the matrix has them (launch_spmm): the
+// same stored-order row sums as the CSR-stream form.
 template <int S>
 static void bb_spmm(hipStream_t s, const bkh_matrix *A, const double *X, double *Y, double alpha, double beta,
                     const Ctl *c) {
-    hipLaunchKernelGGL((k_spmm<S, double, double>), dim3(A->nblk), dim3(SPMV_BS), 0, s, A->rp, A->ci, A->val,
-                       A->blk, X, Y, alpha, beta, c);
+    launch_spmm<S>(s, A, A->val, X, Y, alpha, beta, c, true);
 }
 
 template <int L>

@@ -936,55 +936,6 @@ static void panel_vp(hipStream_t s, long long m, T *V, T *P, const BlkMats *mt, 
 // Y = alpha A X + beta Y: the SELL-64 kernel when the matrix has that copy in the
 // values' precision, else CSR-stream. dia: the solvers' own products (finite X) may
 // run on the slice-diagonal copy.
-template <int S, typename VT, typename T>
-static void launch_spmm(hipStream_t s, const bkh_matrix *A, const VT *val, const T *X, T *Y,
-                        double alpha, double beta, const Ctl *c, bool dia = false) {
-    constexpr int LPRD = spmm_lpr<S, T>();
-    if constexpr (LPRD > 0) {
-        const VT *dv = nullptr;
-        if (dia && A->dia_entries > 0 && A->nsl > 0 && A->variant == SPMV_SELL) {
-            if constexpr (sizeof(VT) == sizeof(float)) dv = (val == (const VT *)A->val32) ? (const VT *)A->dval32 : nullptr;
-            else dv = (val == (const VT *)A->val) ? (const VT *)A->dval : nullptr;
-        }
-        if (dv && (((uintptr_t)X | (uintptr_t)Y) & 15) == 0) {
-            const int G = (int)((A->n + 256 / LPRD - 1) / (256 / LPRD));
-            hipLaunchKernelGGL((k_spmm_sell_v<S, VT, T, int, true>), dim3(G), dim3(256), 0, s, A->n, A->rp, A->dso,
-                               dv, (const int *)A->doff, X, Y, alpha, beta, c);
-            return;
-        }
-    }
-    const VT *sv = nullptr;
-    if (A->nsl > 0) {
-        if constexpr (sizeof(VT) == sizeof(float)) sv = (val == (const VT *)A->val32) ? (const VT *)A->sval32 : nullptr;
-        else sv = (val == (const VT *)A->val) ? (const VT *)A->sval : nullptr;
-    }
-    if (sv && (A->scd || A->scol)) {
-        constexpr int LPR = spmm_lpr<S, T>();
-        if constexpr (LPR > 0) {
-            if ((((uintptr_t)X | (uintptr_t)Y) & 15) == 0) {
-                const int G = (int)((A->n + 256 / LPR - 1) / (256 / LPR));
-                if (A->scd)
-                    hipLaunchKernelGGL((k_spmm_sell_v<S, VT, T, short>), dim3(G), dim3(256), 0, s, A->n, A->rp,
-                                       A->sso, sv, (const short *)A->scd, X, Y, alpha, beta, c);
-                else
-                    hipLaunchKernelGGL((k_spmm_sell_v<S, VT, T, int>), dim3(G), dim3(256), 0, s, A->n, A->rp,
-                                       A->sso, sv, (const int *)A->scol, X, Y, alpha, beta, c);
-                return;
-            }
-        }
-        const int G = (A->n + 255) / 256;
-        if (A->scd)
-            hipLaunchKernelGGL((k_spmm_sell<S, VT, T, short>), dim3(G), dim3(256), 0, s, A->n, A->rp, A->sso, sv,
-                               (const short *)A->scd, X, Y, alpha, beta, c);
-        else
-            hipLaunchKernelGGL((k_spmm_sell<S, VT, T, int>), dim3(G), dim3(256), 0, s, A->n, A->rp, A->sso, sv,
-                               (const int *)A->scol, X, Y, alpha, beta, c);
-        return;
-    }
-    hipLaunchKernelGGL((k_spmm<S, VT, T>), dim3(A->nblk), dim3(SPMV_BS), 0, s, A->rp, A->ci, val, A->blk,
-                       X, Y, alpha, beta, c);
-}
-
 // AP = A P with the Gram P^T AP fused on MFMA (k_spmm_gram) when the matrix has a
 // SELL-64 copy in the values' precision and the blocks allow 16-byte rows: returns the
 // partial rows written (at most FIN_DIRECT), or 0 when the caller must run the SpMM

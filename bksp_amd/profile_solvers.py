@@ -27,7 +27,7 @@ from . import gen
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--solver", choices=["cg", "blcg", "shifted", "pcg", "bicgstabl", "zbicgstabl", "zshbicgstabl",
-                                         "zblbicgstabl", "batched", "batch_spmv"],
+                                         "zblbicgstabl", "blbicgstabl", "batched", "batch_spmv"],
                     default="cg")
     ap.add_argument("--precision", choices=["fp64", "fp32ir"], default="fp64")
     ap.add_argument("--precond", choices=["jacobi", "fsai"], default="jacobi")
@@ -102,6 +102,18 @@ def main():
             def run():
                 dx.zero()
                 return Bt.bicgstabl(db, 2, dx, 1e-8, 1000, precision=prec)[0]
+    elif args.solver == "blbicgstabl":
+        m = args.m or 96
+        px, py, pz = gen.cfg3_peclet()
+        n, rp, ci, v = gen.convdiff3d(m, px, py, pz)
+        M = dv.Matrix(n, rp, ci, v)
+        sb = 4
+        dB = dv.DeviceArray.from_numpy(gen.normal(307, n * sb))
+        dx = dv.DeviceArray((n * sb,))
+
+        def run():
+            dx.zero()
+            return dv.blbicgstabl(M, sb, dB, 4, dx, 1e-8, 1000)
     elif args.solver == "zblbicgstabl":
         m = args.m or 96
         n, rp, ci, v = gen.zconvdiff3d(m)
