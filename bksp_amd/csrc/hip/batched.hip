@@ -546,6 +546,11 @@ __device__ __forceinline__ void bt_fin_sys(int q, const double *part, const int 
         return;
     }
     if constexpr (OP == BT_MR) {
+        // l from the Gram's size, NV = (l + 1)(l + 2) / 2, as a constant: the MR step's
+        // arrays stay in registers
+        constexpr int l = NV == 3 ? 1 : NV == 6 ? 2 : NV == 10 ? 3 : NV == 15 ? 4 : NV == 21 ? 5 : NV == 28 ? 6
+                        : NV == 36 ? 7 : 8;
+        static_assert((l + 1) * (l + 2) / 2 == NV, "MR Gram size");
         double G[MR_D][MR_D];
         gram_unpack(l, acc, G);
         if (G[0][0] == 0.0) { S.done = 1; S.status = ST_CONVERGED; S.result = S.it; return; }

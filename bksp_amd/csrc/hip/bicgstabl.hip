@@ -49,8 +49,10 @@ struct BlGammaFin {   // gamma1 = rs . r_{j+1}; beta = -alpha gamma1 / gamma0   
     }
 };
 
+// l is a compile-time constant, so the MR step's small arrays stay in registers.
+template <int L>
 struct BlMrFin {      // Gram of r_0..r_l -> MR coefficients on the unmodified r_k
-    int l;
+    static constexpr int l = L;
     __device__ void operator()(Ctl *c, const double *tot) const {
         double G[MR_D][MR_D];
         gram_unpack(l, tot, G);
@@ -166,7 +168,7 @@ static void launch_gram_spmv(hipStream_t s, const bkh_matrix *A, const VT *val, 
             e.b[idx] = (unsigned char)(b == L ? EPI_Y : b);
             idx++;
         }
-    BlMrFin fin{L};
+    BlMrFin<L> fin;
     launch_spmv<VT, T, NV>(s, A, val, R + (L - 1) * n, R + L * n, 1.0, 0.0, e, c, part, fin);
 }
 
