@@ -958,7 +958,7 @@ __global__ void k_zsh_sc1(int ns, ZSh *sh, const Ctl *c, int j) {
     if (ctl_stopped(c)) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ns) return;
-    ZSh z = sh[k];
+    ZSh &z = sh[k];   // in place: a copy of the whole state spills to scratch
     const zd al = zload(c->s, ZS_ALPHA / 2), be = zload(c->s, ZS_BETA / 2), ao = zload(c->s, ZS_ALOLD / 2);
     const zd one = zmk(1.0, 0.0);
     const zd as = zmul(al, z.sigma);
@@ -975,7 +975,6 @@ __global__ void k_zsh_sc1(int ns, ZSh *sh, const Ctl *c, int j) {
     z.d2 = zdiv(one, zmul(z.theta, z.alpha_s));
     z.ck[j][0] = z.cA; z.ck[j][1] = z.cB; z.ck[j][2] = z.cC;
     z.ck[j][3] = z.d1; z.ck[j][4] = z.d2; z.ck[j][5] = z.alpha_s;
-    sh[k] = z;
 }
 
 // p_shift updates of step j before the seed's r update (:426-437): p_s,i = cA p_s,i +
@@ -1102,7 +1101,7 @@ __global__ void k_zsh_sc2(int ns, int l, ZSh *sh, const double *tau, const Ctl *
     if (ctl_stopped(c)) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ns) return;
-    ZSh z = sh[k];
+    ZSh &z = sh[k];   // in place: a copy of the whole state spills to scratch
     const zd *om = (const zd *)&c->s[ZS_OMEGA];
     const zd sg = z.sigma, msg = zneg(sg);
     zd out[ZS_LMAX + 1];
@@ -1138,7 +1137,6 @@ __global__ void k_zsh_sc2(int ns, int l, ZSh *sh, const double *tau, const Ctl *
         z.cx[j + 1] = zdiv(zadd(osd[j + 1], acc), e);
     }
     z.cpsi = zdiv(zmk(1.0, 0.0), psi);
-    sh[k] = z;
 }
 
 // Shifted MR update (:551-557): x_k += cx_0 r_0; p_s,0 += (-omega_{l-1}) p_s,l;
