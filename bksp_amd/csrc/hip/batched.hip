@@ -605,14 +605,16 @@ __global__ __launch_bounds__(256) void k_bt_fin(const double *part, const int *r
 // one L2 as in the per-phase launches. The loop ends when no system is active or
 // after max_it iterations, the same test k_bt_alldone makes.
 //
-// Grid barrier, two levels. The workgroups of one XCD (workgroup g runs on XCD g % 8,
-// checked against the XCC_ID hardware register at entry) arrive on that XCD's
-// monotone counter once their stores have completed (they sit in the XCD's shared
-// L2). The last arriver of the XCD is its leader for this barrier: an agent-scope
+// Grid barrier, by XCD. The workgroups of one XCD (workgroup g runs on XCD g % 8,
+// checked against the XCC_ID hardware register at entry) arrive, once their stores
+// have completed (they sit in the XCD's shared L2), on a sub-group counter (8
+// workgroups each), whose last arriver arrives on the XCD's monotone counter. The
+// XCD's last arriver is its leader for this barrier: an agent-scope
 // fence writes the XCD's L2 back, it arrives on the global counter (barrier k
 // completes at 8 k leader arrivals; the last publishes generation k), waits for it,
 // invalidates the XCD's L2 with a second agent-scope fence and publishes the XCD's
-// generation; the others wait for that and make an acquire-only fence. So each
+// generation; the sub-group leaders wait for that and publish their sub-group's, the
+// others wait for their sub-group's; all make an acquire-only fence. So each
 // barrier writes each XCD's L2 back once, as a kernel boundary does, instead of once
 // per workgroup. If any workgroup finds itself on another XCD than g % 8, every barrier
 // falls back to agent-scope fences in every workgroup. All waits are bounded: a wait
@@ -622,6 +624,7 @@ struct BtGrid {
     unsigned arrive, gen;
     int abort, iters;
     unsigned xarrive[BKH_NXCD], xgen[BKH_NXCD];
+    unsigned sarrive[BKH_NXCD][8], sgen[BKH_NXCD][8];   // sub-groups of an XCD's workgroups
     int bad_map, pad;
     unsigned qctr[3][BKH_NXCD];   // per-XCD item queues of the chunked persistent kernel
 };
@@ -651,7 +654,7 @@ __device__ __forceinline__ int bt_xcc_id() {
 // arrival counts at the end of the last one (every workgroup passes the same
 // sequence of barriers in the same modes, so all agree on them).
 struct BtSeq {
-    unsigned gen = 0, arrived = 0, xarrived = 0;
+    unsigned gen = 0, arrived = 0, xarrived = 0, sarrived = 0;
 };
 
 __device__ __forceinline__ bool bt_grid_sync(BtGrid *g, unsigned nwg, BtSeq &q, int *flag, bool hier,
@@ -667,29 +670,39 @@ __device__ __forceinline__ bool bt_grid_sync(BtGrid *g, unsigned nwg, BtSeq &q, 
                 bt_wait(g, &g->gen, target, spin_max);
             __threadfence();
         } else {
-            const int x = (int)blockIdx.x % BKH_NXCD;
-            const unsigned nx = nwg / BKH_NXCD;
+            // arrival in three levels: a sub-group of up to 8 of the XCD's workgroups,
+            // the XCD's sub-group leaders, the 8 XCD leaders (64 arrivals on one counter
+            // serialise: 24-26 us per barrier against 11-13 us this way, measured alone)
+            const int x = (int)blockIdx.x % BKH_NXCD, w = (int)blockIdx.x / BKH_NXCD, sg = w % 8;
+            const unsigned nx = nwg / BKH_NXCD, nsg = nx < 8 ? nx : 8, nin = (nx - (unsigned)sg + 7) / 8;
             __builtin_amdgcn_s_waitcnt(0);                 // this workgroup's stores are in L2
-            if (atomicAdd(&g->xarrive[x], 1u) == q.xarrived + nx - 1) {
-                __threadfence();                            // write the XCD's L2 back
-                if (atomicAdd(&g->arrive, 1u) == q.arrived + BKH_NXCD - 1)
-                    __hip_atomic_store(&g->gen, target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                else
-                    bt_wait(g, &g->gen, target, spin_max);
-                __threadfence();                            // invalidate the XCD's L2 (and L1)
-                __hip_atomic_store(&g->xgen[x], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            if (atomicAdd(&g->sarrive[x][sg], 1u) == q.sarrived + nin - 1) {
+                if (atomicAdd(&g->xarrive[x], 1u) == q.xarrived + nsg - 1) {
+                    __threadfence();                        // write the XCD's L2 back
+                    if (atomicAdd(&g->arrive, 1u) == q.arrived + BKH_NXCD - 1)
+                        __hip_atomic_store(&g->gen, target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                    else
+                        bt_wait(g, &g->gen, target, spin_max);
+                    __threadfence();                        // invalidate the XCD's L2 (and L1)
+                    __hip_atomic_store(&g->xgen[x], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                } else {
+                    bt_wait(g, &g->xgen[x], target, spin_max);
+                }
+                __hip_atomic_store(&g->sgen[x][sg], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             } else {
-                bt_wait(g, &g->xgen[x], target, spin_max);
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // acquire only: no L2 write-back
+                bt_wait(g, &g->sgen[x][sg], target, spin_max);
             }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");       // acquire only: no L2 write-back
         }
         *flag = __hip_atomic_load(&g->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
     q.gen++;
     if (hier) {
+        const unsigned nx = nwg / BKH_NXCD, sg = (blockIdx.x / BKH_NXCD) % 8;
         q.arrived += BKH_NXCD;
-        q.xarrived += nwg / BKH_NXCD;
+        q.xarrived += nx < 8 ? nx : 8;
+        q.sarrived += (nx - sg + 7) / 8;
     } else {
         q.arrived += nwg;
     }
@@ -1425,8 +1438,8 @@ static int bt_solve(ThreadCtx *t, const bkh_batch *B, const double *b, double *x
     const size_t vbytes = ir ? sizeof(float) : sizeof(double);
     // path 0 / 1: one launch per phase in hipGraphs; 2: one persistent launch per solve.
     // 512 uneven systems (14.8M rows, 251M nonzeros), BiCGSTAB(2) to 1e-8, chunked SpMV:
-    // graphs 96.5 ms FP64 and 67.7 ms FP32-IR, persistent 118.0 and 78.4 ms (CSR-stream
-    // SpMV: graphs 173.9 / 150.4, persistent 225.3 / 214.9); x is bitwise the same on
+    // graphs 95.2 ms FP64 and 64.6 ms FP32-IR, persistent 112.4 and 72.0 ms (CSR-stream
+    // SpMV: graphs 173.9 / 148.1, persistent FP32-IR 192.0); x is bitwise the same on
     // both paths with the same SpMV.
     const bool persist = o.path == 2;
     size_t bytes = carve_size(sizeof(Ctl)) + carve_size(sizeof(BtGrid)) + carve_size(sizeof(BtSys) * ns) + carve_size(sizeof(double) * nparts) +
