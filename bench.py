@@ -273,7 +273,10 @@ def bench_cfg4(dv, gen):
         return worst
 
     worst_seq, ms_seq = _timed(seq, 1)
-    # the whole solve as one persistent cooperative launch (bitwise the graph path's x)
+    # the whole solve as one persistent cooperative launch (bitwise the graph path's x);
+    # one untimed call first, as for the graph path (first launch of the kernel)
+    run(precision=dv.BKH_FP32_IR, path=2)
+    run(path=2)
     (ip32, rp32, tp32), msp32 = _timed(lambda: run(precision=dv.BKH_FP32_IR, path=2), 1)
     same32 = bool(np.array_equal(dx.numpy(), x32))
     (ip64, rp64, tp64), msp64 = _timed(lambda: run(path=2), 1)
