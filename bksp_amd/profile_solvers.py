@@ -32,6 +32,7 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32ir"], default="fp64")
     ap.add_argument("--precond", choices=["jacobi", "fsai"], default="jacobi")
     ap.add_argument("--m", type=int, default=0)
+    ap.add_argument("--path", type=int, default=0, help="batched: 2 = the whole solve as one persistent launch")
     args = ap.parse_args()
     prec = dv.BKH_FP32_IR if args.precision == "fp32ir" else dv.BKH_FP64
     if args.solver == "cg":
@@ -101,7 +102,7 @@ def main():
         else:
             def run():
                 dx.zero()
-                return Bt.bicgstabl(db, 2, dx, 1e-8, 1000, precision=prec)[0]
+                return Bt.bicgstabl(db, 2, dx, 1e-8, 1000, precision=prec, path=args.path)[0]
     elif args.solver == "blbicgstabl":
         m = args.m or 96
         px, py, pz = gen.cfg3_peclet()
